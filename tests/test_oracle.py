@@ -1,0 +1,425 @@
+"""Pins the CPU oracle (oracle/oracle.c) against what the reference's own tests
+assert, against closed forms, and against independent library routines
+(numpy.linalg, scipy's SuperLU).  Nothing here needs a GPU."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from tests import meshes
+
+
+def welded(orc, pos, faces, tol=1e-12):
+    return orc.weld(pos, faces, tol)
+
+
+# ------------------------------------------------------------------ meshgen / weld
+def test_cube_counts_match_the_reference(orc):
+    p, f = orc.cube(6)  # meshgen.rs:331-335
+    assert p.shape == (294, 3) and f.shape == (432, 3)
+
+
+def test_grid_counts_and_distinct_positions(orc):
+    p, f = orc.grid(10, 8)  # meshgen.rs:358-368
+    assert len(p) == 11 * 9 and len(f) == 2 * 10 * 8
+    assert len({tuple(x) for x in p}) == len(p)
+
+
+def test_weld_cube_to_218_432(orc):
+    pw, fw = welded(orc, *orc.cube(6))  # parity.rs:31-33
+    assert len(pw) == 218 and len(fw) == 432
+
+
+def test_weld_two_quads_share_the_seam(orc):
+    # weld.rs:69-98
+    p, f = [], []
+    for quad in range(2):
+        base = len(p)
+        for j in range(2):
+            for i in range(2):
+                p.append((quad + i, j, 0.0))
+        a, b, c, d = base, base + 1, base + 2, base + 3
+        f += [(a, b, d), (a, d, c)]
+    pw, fw = orc.weld(np.array(p, float), np.array(f, np.uint32), 1e-6)
+    assert len(pw) == 6 and len(fw) == 4
+
+
+def test_weld_drops_degenerate_and_zero_tol_is_identity(orc):
+    p = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0.5, 0.5, 0)], float)
+    _, fw = orc.weld(p, np.array([(0, 1, 2), (0, 0, 3)], np.uint32), 1e-12)  # weld.rs:100-112
+    assert len(fw) == 1
+    pw, fw = orc.weld(p[:3], np.array([(0, 1, 2)], np.uint32), 0.0)  # weld.rs:114-125
+    assert len(pw) == 3 and len(fw) == 1
+
+
+def test_weld_first_occurrence_order(orc):
+    p = np.array([(0, 0, 0), (1, 0, 0), (1, 0, 1e-15), (0, 0, 0), (2, 0, 0)], float)
+    f = np.array([(0, 1, 4), (3, 2, 4)], np.uint32)
+    pw, fw = orc.weld(p, f, 1e-12)
+    assert len(pw) == 3
+    np.testing.assert_array_equal(fw, [[0, 1, 2], [0, 1, 2]])
+
+
+# ------------------------------------------------------------------ segment
+def test_cube_segments_into_six_49_vertex_charts(orc):
+    pw, fw = welded(orc, *orc.cube(6))
+    ch = orc.Charts(pw, fw, 30.0)
+    assert len(ch) == 6  # parity.rs:35-40
+    for i in range(6):
+        c = ch.chart(i)
+        assert len(c["vertex_ids"]) == 49
+        assert abs(c["area3d"] - 1.0) < 1e-9
+        assert len(c["border"]) == 24
+        assert len(c["border_loops"]) == 1 and len(c["border_loops"][0]) == 25
+        assert c["border_loops"][0][0] == c["border_loops"][0][-1]
+        assert np.all(np.diff(c["vertex_ids"].astype(np.int64)) > 0)
+
+
+def test_flat_grid_one_chart_one_loop(orc):
+    # segment.rs:435-469
+    p = np.array([(i, j, 0.0) for j in range(3) for i in range(3)], float)
+    f = []
+    for j in range(2):
+        for i in range(2):
+            a = j * 3 + i
+            b, c = a + 1, a + 3
+            f += [(a, b, c + 1), (a, c + 1, c)]
+    ch = orc.Charts(p, np.array(f, np.uint32), 30.0)
+    assert len(ch.cut()) == 0 and len(ch) == 1
+    c = ch.chart(0)
+    assert len(c["face_ids"]) == 8 and len(c["vertex_ids"]) == 9 and len(c["border"]) == 8
+    assert len(c["border_loops"]) == 1 and len(c["border_loops"][0]) == 9
+
+
+def test_open_triangle_and_closed_tetrahedron(orc):
+    ch = orc.Charts(np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0)], float), np.array([(0, 1, 2)], np.uint32), 30.0)
+    loop = ch.chart(0)["border_loops"][0]  # segment.rs:471-487
+    assert len(loop) == 4 and loop[0] == loop[3]
+    p = np.array([(1, 0, 0), (0, 1, 0), (0, 0, 1), (0, 0, 0)], float)
+    f = np.array([(0, 1, 2), (0, 3, 1), (0, 2, 3), (1, 3, 2)], np.uint32)
+    ch = orc.Charts(p, f, 130.0)  # segment.rs:489-510
+    assert len(ch.cut()) == 0 and len(ch) == 1
+    c = ch.chart(0)
+    assert c["border_loops"] == [] and len(c["border"]) == 0
+
+
+def test_cylinder_welds_to_two_border_loops(orc):
+    pw, fw = welded(orc, *meshes.cylinder(1.0, 2.0, 32, 16))  # meshgen.rs:370-380
+    assert len(pw) == 32 * 17 and len(fw) == 2 * 32 * 16
+    ch = orc.Charts(pw, fw, 30.0)
+    assert len(ch) == 1 and len(ch.chart(0)["border_loops"]) == 2
+
+
+def test_annulus_and_sphere_counts(orc):
+    pw, fw = welded(orc, *meshes.torus_annulus(2.0, 0.7, 64, 40))  # parity.rs:115-156
+    assert len(pw) == 2560 and len(fw) == 4992
+    ch = orc.Charts(pw, fw, 30.0)
+    assert len(ch) == 1
+    c = ch.chart(0)
+    assert len(c["border_loops"]) == 2 and len(c["border"]) == 128
+    pw, fw = welded(orc, *meshes.uv_sphere(48, 24))  # parity.rs:76-110
+    assert len(pw) == 1036 and len(fw) == 2068
+
+
+# ------------------------------------------------------------------ eig3 / frame / pins
+def test_eig3_known_spectra(orc):
+    w, _ = orc.eig3(np.diag([2.0, 5.0, 1.0]))  # lscm_eig.rs:72-79
+    np.testing.assert_allclose(w, [2, 5, 1], atol=1e-12)
+    m = np.array([[2.0, 1, 0], [1, 2, 0], [0, 0, 3]])  # lscm_eig.rs:81-112
+    w, v = orc.eig3(m)
+    np.testing.assert_allclose(sorted(w), [1, 3, 3], atol=1e-9)
+    np.testing.assert_allclose(v.T @ v, np.eye(3), atol=1e-9)
+    np.testing.assert_allclose(m @ v, v * w, atol=1e-9)
+
+
+def test_eig3_eigenvalues_match_numpy_eigh(orc):
+    """The eigenvalues are exact; the returned V is orthonormal.  The reference
+    accumulates V' = J*V (lscm_eig.rs:57-62), i.e. V = J_n...J_1 rather than the
+    eigenvector matrix J_1...J_n, so for a matrix that needs more than one
+    rotation the columns of V are not eigenvectors.  The restatement keeps that
+    (tangent_frame and the pin winding depend on it); a single-plane matrix
+    needs one rotation, where both products agree."""
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        b = rng.normal(size=(3, 3))
+        m = b @ b.T
+        w, v = orc.eig3(m)
+        np.testing.assert_allclose(np.sort(w), np.linalg.eigvalsh(m), rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(v.T @ v, np.eye(3), atol=1e-12)
+    for _ in range(10):
+        b = rng.normal(size=(2, 2))
+        m = np.zeros((3, 3))
+        m[:2, :2] = b @ b.T
+        m[2, 2] = 7.0
+        w, v = orc.eig3(m)
+        np.testing.assert_allclose(m @ v, v * w, atol=1e-9 * np.abs(w).max())
+
+
+def test_tangent_frame_is_an_isometry_of_planar_points(orc):
+    """A cloud in a plane z = const needs one Jacobi rotation, where the
+    reference frame is the PCA frame: the projection is an isometry and the
+    first axis carries the largest variance (lscm.rs:152-183).  For a general
+    orientation the frame is orthonormal but not principal (see eig3 above),
+    so the projection only shortens distances."""
+    rng = np.random.default_rng(2)
+    q = rng.normal(size=(50, 2)) * [3.0, 1.0]
+    th = 0.7
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    p3 = np.c_[q @ R.T, np.full(50, 0.25)] + [1.0, -2.0, 0.0]
+    p2 = orc.tangent_frame(p3)
+    d3 = np.linalg.norm(p3[:, None] - p3[None], axis=-1)
+    d2 = np.linalg.norm(p2[:, None] - p2[None], axis=-1)
+    np.testing.assert_allclose(d2, d3, atol=1e-10)
+    assert p2[:, 0].var() >= p2[:, 1].var()
+    R3, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    p3g = np.c_[q, 0.1 * rng.normal(size=50)] @ R3.T
+    p2g = orc.tangent_frame(p3g)
+    d3 = np.linalg.norm(p3g[:, None] - p3g[None], axis=-1)
+    d2 = np.linalg.norm(p2g[:, None] - p2g[None], axis=-1)
+    assert (d2 <= d3 + 1e-12).all()
+    np.testing.assert_allclose(p2g.mean(axis=0), 0.0, atol=1e-12)
+
+
+def test_rect_point_walks_the_perimeter_clockwise(orc):
+    # lscm.rs:185-204: bottom-right corner, then along the bottom edge towards -x
+    np.testing.assert_allclose(orc.rect_point(0.0, 2.0, 1.0), [1.0, -0.5])
+    np.testing.assert_allclose(orc.rect_point(1.0 / 6.0, 2.0, 1.0), [0.0, -0.5])
+    np.testing.assert_allclose(orc.rect_point(2.0 / 6.0, 2.0, 1.0), [-1.0, -0.5])
+    np.testing.assert_allclose(orc.rect_point(0.5, 2.0, 1.0), [-1.0, 0.5])
+    np.testing.assert_allclose(orc.rect_point(5.0 / 6.0, 2.0, 1.0), [1.0, 0.5])
+    np.testing.assert_allclose(orc.rect_point(1.25, 2.0, 1.0), orc.rect_point(0.25, 2.0, 1.0))
+
+
+def test_anchor_ids_farthest_points(orc):
+    p = np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0.5, 0.5, 0), (0.5, 0.5, 0.1)], float)
+    np.testing.assert_array_equal(orc.anchor_ids(p, 4), [0, 1, 2, 3])
+    np.testing.assert_array_equal(orc.anchor_ids(p[:3], 4), [0, 1, 2])
+
+
+# ------------------------------------------------------------------ cube closed form (assembly + pins)
+def cube_system(orc):
+    pw, fw = welded(orc, *orc.cube(6))
+    ch = orc.Charts(pw, fw, 30.0)
+    return pw, ch
+
+
+def test_cube_assembly_is_72_times_the_dirichlet_laplacian(orc):
+    pw, ch = cube_system(orc)
+    for i in range(6):
+        s = ch.system(i)
+        assert s["nfree"] == 25 and len(s["pin_ids"]) == 24 and len(s["vals"]) == 210
+        n = 2 * s["nfree"]
+        A = sp.csr_matrix((s["vals"], s["cols"], s["row_ptr"]), shape=(n, n)).toarray()
+        # A = L (x) I2 with L = 72 * (5-point Laplacian on the inner 5x5 of the face grid)
+        Lu, Lv = A[0::2, 0::2], A[1::2, 1::2]
+        np.testing.assert_array_equal(A[0::2, 1::2], 0)
+        np.testing.assert_array_equal(Lu, Lv)
+        c = ch.chart(i)
+        P = pw[c["vertex_ids"]][s["free_ids"]]
+        # recover the 2-D grid coordinates of the free vertices on the face
+        axes = [k for k in range(3) if np.ptp(pw[c["vertex_ids"]][:, k]) > 0]
+        g = np.rint(P[:, axes] * 6).astype(int)
+        T = np.zeros((25, 25))
+        idx = {tuple(x): t for t, x in enumerate(g)}
+        for t, (a, b) in enumerate(g):
+            T[t, t] = 4 * 72.0
+            for da, db in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                if (a + da, b + db) in idx:
+                    T[t, idx[(a + da, b + db)]] = -72.0
+        np.testing.assert_allclose(Lu, T, rtol=1e-12)
+
+
+def test_cube_pins_lie_on_the_unit_perimeter_and_solution_is_affine(orc):
+    pw, ch = cube_system(orc)
+    for i in range(6):
+        s = ch.system(i)
+        uv, st = ch.unfold(i)
+        assert st["iters"] == 2
+        pin = s["pin_uv"]
+        on_edge = np.isclose(np.abs(pin), 0.5, atol=1e-7).any(axis=1)
+        assert on_edge.all() and np.abs(pin).max() < 0.5 + 1e-7
+        c = ch.chart(i)
+        P = pw[c["vertex_ids"]]
+        axes = [k for k in range(3) if np.ptp(P[:, k]) > 0]
+        X = np.c_[P[:, axes], np.ones(len(P))]
+        coef, *_ = np.linalg.lstsq(X, uv, rcond=None)
+        assert np.abs(X @ coef - uv).max() < 1e-7  # isometric (affine) image of the flat face
+        J = coef[:2]
+        np.testing.assert_allclose(np.abs(np.linalg.det(J)), 1.0, atol=1e-6)
+
+
+# ------------------------------------------------------------------ Sparse LU known answers (sparse.rs:272-423)
+def test_lu_small_symmetric(orc):
+    r = [0, 0, 1, 1, 1, 2, 2]
+    c = [0, 1, 0, 1, 2, 1, 2]
+    v = [4, 1, 1, 3, 1, 1, 2.0]
+    x = orc.sparse_solve(3, r, c, v, [1.0, 2.0, 3.0])
+    A = np.zeros((3, 3))
+    A[r, c] = v
+    np.testing.assert_allclose(A @ x, [1, 2, 3], atol=1e-9)
+
+
+def test_lu_pinned_chain(orc):
+    r = [0, 1, 1, 1, 2, 2, 2, 3, 3]
+    c = [0, 1, 0, 2, 2, 1, 3, 3, 2]
+    v = [1, -2, 1, 1, -2, 1, 1, -2, 1.0]
+    b = [5.0, 0, 0, 0]
+    x = orc.sparse_solve(4, r, c, v, b)
+    A = np.zeros((4, 4))
+    A[r, c] = v
+    np.testing.assert_allclose(A @ x, b, atol=1e-9)
+
+
+def test_lu_duplicates_summed_and_diagonal(orc):
+    x = orc.sparse_solve(2, [0, 1], [0, 1], [4.0, 2.0], [8.0, 6.0])
+    np.testing.assert_allclose(x, [2, 3], atol=1e-12)
+    x = orc.sparse_solve(2, [0, 1, 0], [0, 1, 1], [4.0, 2.0, 1.0], [8.0, 6.0])
+    assert abs(x[0] - 1.25) < 1e-12
+    x = orc.sparse_solve(1, [0, 0], [0, 0], [3.0, 2.0], [10.0])
+    assert abs(x[0] - 2.0) < 1e-12
+
+
+def test_lu_banded_grid(orc):
+    n = 25
+    r, c, v = [], [], []
+    for i in range(n):
+        rr, cc = divmod(i, 5)
+        r.append(i); c.append(i); v.append(4.0)
+        for dr, dc in ((0, 1), (0, -1), (1, 0), (-1, 0)):
+            a, b = rr + dr, cc + dc
+            if 0 <= a < 5 and 0 <= b < 5:
+                r.append(i); c.append(a * 5 + b); v.append(-1.0)
+    b = np.arange(n) * 0.1 + 1.0
+    x = orc.sparse_solve(n, r, c, v, b)
+    A = np.zeros((n, n))
+    np.add.at(A, (r, c), v)
+    np.testing.assert_allclose(A @ x, b, atol=1e-8)
+
+
+def test_lu_pivot_swaps(orc):
+    x = orc.sparse_solve(3, [0, 1, 2], [2, 1, 0], [1.0, 1.0, 1.0], [1.0, 2.0, 3.0])
+    np.testing.assert_allclose(x, [3, 2, 1], atol=1e-12)
+    r = [0, 1, 1, 1, 2, 2]
+    c = [1, 0, 1, 2, 1, 2]
+    v = [2.0, 2, 5, 1, 1, 4]
+    A = np.zeros((3, 3))
+    A[r, c] = v
+    xt = np.array([1.0, -2.0, 3.0])
+    x = orc.sparse_solve(3, r, c, v, A @ xt)
+    np.testing.assert_allclose(x, xt, atol=1e-9)
+
+
+def test_lu_singular_returns_none(orc):
+    assert orc.sparse_solve(2, [0, 0, 1, 1], [0, 1, 0, 1], [1.0, 1.0, 1.0, 1.0], [1.0, 2.0]) is None
+
+
+def test_lu_matches_dense_solve_on_random_systems(orc):
+    rng = np.random.default_rng(3)
+    for n in (5, 17, 40):
+        for _ in range(5):
+            A = rng.normal(size=(n, n)) * (rng.random((n, n)) < 0.3) + np.eye(n) * 0.5
+            b = rng.normal(size=n)
+            r, c = np.nonzero(A)
+            x = orc.sparse_solve(n, r, c, A[r, c], b)
+            np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8, atol=1e-8)
+
+
+# ------------------------------------------------------------------ unfold vs SuperLU
+def test_unfold_grid32_matches_superlu(orc):
+    pw, fw = welded(orc, *orc.grid(32, 32))
+    ch = orc.Charts(pw, fw, 30.0)
+    s = ch.system(0)
+    n = 2 * s["nfree"]
+    assert s["nfree"] == 961 and n == 1922  # lscm.rs:740-747: the reference's own GPU-vs-CPU fixture
+    A = sp.csr_matrix((s["vals"], s["cols"], s["row_ptr"]), shape=(n, n))
+    assert abs(A - A.T).max() == 0
+    x = spl.splu(A.tocsc()).solve(s["rhs"][:n])
+    uv, st = ch.unfold(0)
+    ref = s["p2"].copy()
+    ref[s["free_ids"]] = x.reshape(-1, 2)
+    ref[s["pin_ids"]] = s["pin_uv"]
+    assert st["iters"] == 2
+    assert np.abs(uv - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def test_unfold_closed_chart_uses_four_anchors(orc):
+    pw, fw = welded(orc, *meshes.uv_sphere(16, 8))
+    ch = orc.Charts(pw, fw, 180.0)
+    assert len(ch) == 1 and ch.chart(0)["border_loops"] == []
+    s = ch.system(0)
+    assert len(s["pin_ids"]) == 4
+    uv, st = ch.unfold(0)
+    assert np.isfinite(uv).all()
+    np.testing.assert_allclose(uv[s["pin_ids"]], s["p2"][s["pin_ids"]], atol=1e-12)
+
+
+def test_worker_thread_path_is_bitwise_stable(orc):
+    pw, fw = welded(orc, *orc.cube(6))
+    ch = orc.Charts(pw, fw, 30.0)
+    a = ch.par_unfold(1)
+    for w in (2, 3, 8):
+        b = ch.par_unfold(w)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+# ------------------------------------------------------------------ CG / IC(0) / colouring
+def test_cpu_cg_1d_laplacian(orc):
+    v, c, p, b = meshes.laplacian_1d(200)
+    x = orc.cg_solve(v, c, p, b, 2000)
+    np.testing.assert_allclose(x, 1.0, atol=1e-8)
+
+
+def test_ic0_reproduces_matrix_on_its_pattern(orc):
+    v, c, p = meshes.diag_dominant_banded(200, 3)  # precond.rs:708-737
+    f = orc.ic0_factor(v, c, p)
+    n = len(p) - 1
+    L = meshes.to_dense(f["l_vals"], f["l_cols"], f["l_ptr"])
+    A = meshes.to_dense(v, c, p)
+    LL = L @ L.T
+    mask = A != 0
+    np.testing.assert_allclose(LL[mask], A[mask], rtol=1e-9, atol=1e-9)
+    LT = meshes.to_dense(f["lt_vals"], f["lt_cols"], f["lt_ptr"])
+    np.testing.assert_array_equal(LT, L.T)
+    for ptr, cols in ((f["l_ptr"], f["l_cols"]), (f["lt_ptr"], f["lt_cols"])):
+        assert all(cols[ptr[i + 1] - 1] == i for i in range(n))  # diagonal last
+
+
+def test_ic0_levels_partition_and_respect_dependencies(orc):
+    v, c, p = meshes.diag_dominant_banded(150, 2)  # precond.rs:739-755
+    f = orc.ic0_factor(v, c, p)
+    n = len(p) - 1
+    for lev, ptr, cols in ((f["fwd_level"], f["l_ptr"], f["l_cols"]), (f["bwd_level"], f["lt_ptr"], f["lt_cols"])):
+        assert lev.min() == 0 and len(lev) == n
+        for i in range(n):
+            for k in range(ptr[i], ptr[i + 1] - 1):
+                assert lev[cols[k]] < lev[i]
+
+
+def test_ic0_apply_equals_dense_triangular_solves(orc):
+    v, c, p = meshes.diag_dominant_banded(400, 3)  # precond.rs:887-927 host side
+    f = orc.ic0_factor(v, c, p)
+    n = len(p) - 1
+    r = np.array([((i * 7919) % 101) / 101.0 - 0.5 for i in range(n)])
+    z = orc.ic0_apply(f, r)
+    L = meshes.to_dense(f["l_vals"], f["l_cols"], f["l_ptr"])
+    np.testing.assert_allclose(z, np.linalg.solve(L.T, np.linalg.solve(L, r)), rtol=1e-10, atol=1e-12)
+
+
+def test_ic0_on_a_1d_laplacian_is_exact_cholesky(orc):
+    v, c, p, b = meshes.laplacian_1d(50)
+    f = orc.ic0_factor(v, c, p)
+    # tridiagonal matrices have no fill, so IC(0) is the exact Cholesky factor
+    L = meshes.to_dense(f["l_vals"], f["l_cols"], f["l_ptr"])
+    np.testing.assert_allclose(L, np.linalg.cholesky(meshes.to_dense(v, c, p)), atol=1e-12)
+    assert f["n_fwd"] == 50  # natural order chains one level per row
+
+
+def test_coloring_is_valid_and_bounded(orc):
+    v, c, p, _ = meshes.laplacian_1d(2000)
+    col = orc.coloring(c, p)
+    assert col.max() + 1 <= 8
+    for i in range(2000):
+        for k in range(p[i], p[i + 1]):
+            if c[k] != i:
+                assert col[c[k]] != col[i]
