@@ -1,0 +1,121 @@
+/*
+ * ruv_lscm.h -- C ABI of the MI355X LSCM solve engine (libruv_lscm.so).
+ *
+ * The library replaces the researchuv-gpu CUDA backend behind ResearchUV's
+ * solver switch (UnfoldOptions.solver = SolverBackend::Gpu,
+ * crates/researchuv-unwrap/src/lscm.rs:57-66,677-681).  Every entry point takes
+ * plain host pointers and sizes; the *_dev entry points take device pointers
+ * (memory from hipMalloc or any allocator on the same device).  No pointer is
+ * retained after a call returns.
+ *
+ * Threading: every entry point is thread-safe.  Each calling thread gets its
+ * own HIP stream (default priority), so concurrent callers do not serialise on
+ * a shared stream and never wait on each other's work.
+ *
+ * Status codes: 0 on success, a negative RUV_ERR_* otherwise; the reference
+ * returned Option::None in every failure case and the caller fell back to the
+ * CPU LU (lscm.rs:678).  ruv_last_error() gives the calling thread's message
+ * for its last failure.
+ */
+#ifndef RUV_LSCM_H
+#define RUV_LSCM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ruv_status {
+    RUV_OK = 0,
+    RUV_ERR_UNAVAILABLE = -1,   /* no HIP device (GpuSolver::global() == None, solver.rs:104-126)        */
+    RUV_ERR_BAD_DIMS = -2,      /* n == 0 or length mismatch (solver.rs:296-299)                           */
+    RUV_ERR_BREAKDOWN = -3,     /* |p.Ap| <= 1e-300 or non-finite (solver.rs:505-508)                      */
+    RUV_ERR_NOT_CONVERGED = -4, /* max_iter reached (solver.rs:551-553)                                    */
+    RUV_ERR_NONFINITE = -5,     /* non-finite solution (solver.rs:561-563)                                 */
+    RUV_ERR_DEVICE = -6,        /* a HIP runtime call failed                                               */
+    RUV_ERR_PRECOND = -7,       /* preconditioner setup failed: zero diagonal (precond.rs:596-608,
+                                   amg.rs:140-142), IC(0) pivot (precond.rs:179-188), AMG depth
+                                   (amg.rs:159-161)                                                         */
+    RUV_ERR_INVALID = -8        /* null pointer or unknown enum value                                     */
+};
+
+/* Precond (crates/researchuv-gpu/src/precond.rs:20-47); same names as RESEARCHUV_GPU_PRECOND
+ * (lscm.rs:492-499): none, jacobi, ic0, ic0color, amg, amgsa. */
+enum ruv_precond {
+    RUV_PC_NONE = 0,
+    RUV_PC_JACOBI = 1,
+    RUV_PC_IC0 = 2,
+    RUV_PC_IC0COLOR = 3,
+    RUV_PC_AMG = 4,
+    RUV_PC_AMGSA = 5
+};
+
+/* Solve statistics (no reference counterpart: the reference discarded its iteration count,
+ * lscm.rs:500-502). */
+typedef struct ruv_solve_info {
+    uint64_t iters;          /* PCG iterations (inner FP32 iterations summed for the mixed form)      */
+    uint32_t refinements;    /* outer FP64 refinement steps (0 for the plain FP64 form)               */
+    int32_t precond;         /* preconditioner actually applied                                        */
+    double true_rel_resid;   /* ||b - A x||_2 / ||b||_2 computed in FP64 after the solve              */
+    double seconds;          /* device time of the solve (HIP events), excluding setup and copies     */
+} ruv_solve_info;
+
+/* GpuSolver::global().is_some() (solver.rs:104-113): 1 when a HIP device is usable. */
+int ruv_available(void);
+
+/* Message for the calling thread's last non-zero status ("" if none). */
+const char* ruv_last_error(void);
+
+/* GpuSolver::cg_solve_precond (solver.rs:284-565): PCG in FP64, x0 = 0, stop when the recursive residual
+ * satisfies ||r|| <= tol*||b|| (tol absolute when ||b|| <= 1e-300), at most max_iter iterations; flexible CG
+ * for the AMG preconditioners (solver.rs:474-481).  A is SPD CSR (vals/cols of length nnz, row_ptr of length
+ * n+1), b of length b_len (must equal n).  x_out (n) and iters_out are written on success.  An interleaved
+ * A = L (x) I2 (the LSCM free block of lscm.rs:605-640 as produced by Sparse::to_csr) is detected and
+ * streamed as L against two right-hand sides; the iterates are those of CG on A. */
+int ruv_cg_solve_precond(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                         const double* b, int64_t b_len, uint64_t max_iter, double tol, int32_t precond,
+                         double* x_out, uint64_t* iters_out);
+
+/* GpuSolver::cg_solve (solver.rs:274-277): ruv_cg_solve_precond with RUV_PC_NONE. */
+int ruv_cg_solve(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                 const double* b, int64_t b_len, uint64_t max_iter, double tol, double* x_out);
+
+/* GpuSolver::apply_ic0_for_test (solver.rs:205-270): z = (L L^T)^-1 r with the IC(0) factor of A
+ * (precond.rs:145-219) applied by the device triangular sweeps. */
+int ruv_apply_ic0(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                  const double* r, double* z_out);
+
+/* Mixed-precision form (no reference counterpart; the reference GPU path is FP64 throughout,
+ * solver.rs:284-565): the SpMV, preconditioner and Krylov vectors run in FP32 inside an FP64
+ * iterative-refinement loop (r = b - A x, the x update and the CG alpha/beta reductions in FP64) until the
+ * true FP64 residual satisfies ||b - A x|| <= tol*||b||.  max_iter bounds the summed inner iterations. */
+int ruv_cg_solve_mixed(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                       const double* b, int64_t b_len, uint64_t max_iter, double tol, int32_t precond,
+                       double* x_out, ruv_solve_info* info);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Device-resident plans: upload and set up once, solve many times on device buffers.  A plan holds the
+ * system L (x) I_R for R = 1 (generic CSR) or R = 2 (the LSCM free block: L is the cotan Laplacian on the
+ * free vertices, vectors interleaved (u0,v0,u1,v1,...) as in lscm.rs:616-629).  mode 0 = plain FP64 PCG,
+ * mode 1 = FP32 PCG inside FP64 refinement.  No reference counterpart (the reference re-uploaded and re-set
+ * up every solve, solver.rs:301-314).
+ * ------------------------------------------------------------------------------------------------------- */
+typedef struct ruv_plan ruv_plan;
+
+enum ruv_mode { RUV_MODE_FP64 = 0, RUV_MODE_MIXED = 1 };
+
+int ruv_plan_create(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t nrows, int64_t nnz,
+                    int32_t rhs_per_row, int32_t precond, int32_t mode, ruv_plan** out);
+/* d_b, d_x: device arrays of nrows*rhs_per_row doubles (interleaved).  tol as in ruv_cg_solve_precond for
+ * mode 0 (recursive residual) and ruv_cg_solve_mixed for mode 1 (true residual). */
+int ruv_plan_solve_dev(ruv_plan* plan, const double* d_b, double* d_x, uint64_t max_iter, double tol,
+                       ruv_solve_info* info);
+/* Host-buffer convenience around ruv_plan_solve_dev (copies b in, x out). */
+int ruv_plan_solve(ruv_plan* plan, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
+void ruv_plan_destroy(ruv_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RUV_LSCM_H */

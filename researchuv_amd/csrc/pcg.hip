@@ -1,0 +1,488 @@
+// pcg.hip -- device PCG for the LSCM free block on MI355X (gfx950, wave64).
+//
+// Replaces GpuSolver::cg_solve_precond (crates/researchuv-gpu/src/solver.rs:284-565) and its kernels
+// (kernels.cu:7-102).  Design differences from the reference, all MI355X-first:
+//  * every scalar of the recurrence (alpha, beta, r.r, r.z, p.Ap) lives on the device: a kernel that
+//    produces a dot product writes one partial per workgroup, and every workgroup of the kernel that
+//    consumes it sums those partials itself in a fixed order (deterministic, no host round trip and no
+//    extra finalize launch); the reference synchronised and copied partials to the host on every dot
+//    (solver.rs:569-615);
+//  * the operator is streamed as L (x) I_R: for the LSCM system (A = L (x) I2, lscm.rs:605-640) each row of
+//    L is read once and applied to the interleaved (u, v) pair, halving matrix traffic;
+//  * the x/r/z updates of an iteration are one fused kernel (the reference launched axpy, axpy, mul and
+//    copy separately);
+//  * the host only polls a small status block every few iterations (chunks of iterations in flight).
+// The plain form keeps the reference's recurrence and stopping rule exactly: x0 = 0,
+// ||r_k|| <= tol*||b|| on the recursive residual, breakdown when |p.Ap| <= 1e-300 or non-finite.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ruv_internal.hpp"
+#include "pcg_kernels.hpp"
+
+namespace ruv {
+
+#define HIPCHK(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(RUV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+// Device buffer owner (freed on scope exit; the reference's DeviceMem, solver.rs:671-729).
+struct DevArena {
+    std::vector<void*> ptrs;
+    ~DevArena() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t alloc(T** out, size_t count) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) ptrs.push_back(p);
+        *out = (T*)p;
+        return e;
+    }
+    template <typename T>
+    hipError_t upload(T** out, const T* host, size_t count, hipStream_t s) {
+        hipError_t e = alloc(out, count);
+        if (e != hipSuccess) return e;
+        if (count == 0) return hipSuccess;
+        return hipMemcpyAsync(*out, host, count * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+};
+
+int grid_for(int64_t n) {
+    int64_t g = (n + BLK - 1) / BLK;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, MAXG));
+}
+
+// A triangular factor on the device (DeviceIc0, precond.rs:502-518).
+template <typename T>
+struct DevTri {
+    int n = 0;
+    int *l_ptr = nullptr, *l_cols = nullptr, *lt_ptr = nullptr, *lt_cols = nullptr;
+    T *l_vals = nullptr, *lt_vals = nullptr;
+    int *fwd_rows = nullptr, *fwd_off = nullptr, *bwd_rows = nullptr, *bwd_off = nullptr;
+    int n_fwd = 0, n_bwd = 0;
+};
+
+template <typename T>
+int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
+    d->n = f.l.n;
+    std::vector<T> lv(f.l.vals.begin(), f.l.vals.end()), ltv(f.lt.vals.begin(), f.lt.vals.end());
+    HIPCHK(m.upload(&d->l_ptr, f.l.ptr.data(), f.l.ptr.size(), s));
+    HIPCHK(m.upload(&d->l_cols, f.l.cols.data(), f.l.cols.size(), s));
+    HIPCHK(m.upload(&d->l_vals, lv.data(), lv.size(), s));
+    HIPCHK(m.upload(&d->lt_ptr, f.lt.ptr.data(), f.lt.ptr.size(), s));
+    HIPCHK(m.upload(&d->lt_cols, f.lt.cols.data(), f.lt.cols.size(), s));
+    HIPCHK(m.upload(&d->lt_vals, ltv.data(), ltv.size(), s));
+    HIPCHK(m.upload(&d->fwd_rows, f.fwd_rows.data(), f.fwd_rows.size(), s));
+    HIPCHK(m.upload(&d->fwd_off, f.fwd_off.data(), f.fwd_off.size(), s));
+    HIPCHK(m.upload(&d->bwd_rows, f.bwd_rows.data(), f.bwd_rows.size(), s));
+    HIPCHK(m.upload(&d->bwd_off, f.bwd_off.data(), f.bwd_off.size(), s));
+    d->n_fwd = (int)f.fwd_off.size() - 1;
+    d->n_bwd = (int)f.bwd_off.size() - 1;
+    // the host vectors lv/ltv must outlive the async copies
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------
+// Plan: a device-resident system L (x) I_R with its preconditioner and work vectors.
+// ------------------------------------------------------------------------------------------------------
+struct PlanImpl {
+    int R = 1, n = 0, mode = 0, precond = 0, G = 1;
+    int64_t nnz = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevArena mem;
+    int *ptr = nullptr, *cols = nullptr;
+    double* vals64 = nullptr;
+    float* vals32 = nullptr;
+    double* dinv64 = nullptr;
+    float* dinv32 = nullptr;
+    DevTri<double> tri64;
+    DevTri<float> tri32;
+    bool has_tri = false;
+    // permutation round trip for the colour-ordered IC(0) (precond.rs:613-630)
+    int *to_orig = nullptr, *to_perm = nullptr;
+    bool permuted = false;
+    // work vectors
+    double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr, *y64 = nullptr,
+           *t64 = nullptr;
+    float *x32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *y32 = nullptr,
+          *t32 = nullptr;
+    double *part_a = nullptr, *part_b = nullptr, *part_c = nullptr;
+    SolveState* st = nullptr;
+    SolveState* h_st = nullptr;
+    double* h_part = nullptr;
+    double inner_tol = 1e-4;
+    int max_refine = 40;
+
+    ~PlanImpl() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (h_st) (void)hipHostFree(h_st);
+        if (h_part) (void)hipHostFree(h_part);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
+    P.R = R;
+    P.n = a.n;
+    P.nnz = a.nnz();
+    P.mode = mode;
+    P.precond = precond;
+    P.G = grid_for(a.n);
+    HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&P.ev0));
+    HIPCHK(hipEventCreate(&P.ev1));
+    hipStream_t s = P.stream;
+    HIPCHK(P.mem.upload(&P.ptr, a.ptr.data(), a.ptr.size(), s));
+    HIPCHK(P.mem.upload(&P.cols, a.cols.data(), a.cols.size(), s));
+    HIPCHK(P.mem.upload(&P.vals64, a.vals.data(), a.vals.size(), s));
+    std::vector<float> v32;
+    if (mode == RUV_MODE_MIXED) {
+        v32.assign(a.vals.begin(), a.vals.end());
+        HIPCHK(P.mem.upload(&P.vals32, v32.data(), v32.size(), s));
+    }
+    const size_t N = (size_t)a.n * R;
+    // Jacobi inverse diagonal (precond.rs:595-611): a vanishing diagonal is a setup failure.
+    std::vector<double> dinv(a.n, 1.0);
+    std::vector<float> dinv32;
+    if (precond == RUV_PC_JACOBI) {
+        for (int32_t i = 0; i < a.n; ++i) {
+            double d = a.get(i, i);
+            if (std::fabs(d) < 1e-300) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(i));
+            dinv[i] = 1.0 / d;
+        }
+        HIPCHK(P.mem.upload(&P.dinv64, dinv.data(), dinv.size(), s));
+        if (mode == RUV_MODE_MIXED) {
+            dinv32.assign(dinv.begin(), dinv.end());
+            HIPCHK(P.mem.upload(&P.dinv32, dinv32.data(), dinv32.size(), s));
+        }
+    } else if (precond == RUV_PC_IC0 || precond == RUV_PC_IC0COLOR) {
+        TriFactor f;
+        if (precond == RUV_PC_IC0) {
+            if (!factor_ic0(a, &f)) return fail(RUV_ERR_PRECOND, "ic0: vanishing pivot or diagonal");
+        } else {
+            std::vector<uint32_t> color = graph_coloring(a);
+            std::vector<int32_t> to_perm, to_orig;
+            color_permutation(color, &to_perm, &to_orig);
+            Csr pa = permute_csr(a, to_perm, to_orig);
+            if (!factor_ic0(pa, &f)) return fail(RUV_ERR_PRECOND, "ic0color: vanishing pivot or diagonal");
+            HIPCHK(P.mem.upload(&P.to_orig, to_orig.data(), to_orig.size(), s));
+            HIPCHK(P.mem.upload(&P.to_perm, to_perm.data(), to_perm.size(), s));
+            P.permuted = true;
+        }
+        int rc = (mode == RUV_MODE_MIXED) ? upload_tri(P.mem, f, s, &P.tri32) : upload_tri(P.mem, f, s, &P.tri64);
+        if (rc) return rc;
+        P.has_tri = true;
+    } else if (precond != RUV_PC_NONE) {
+        return fail(RUV_ERR_INVALID, "precond " + std::to_string(precond) + " is not available in this build");
+    }
+    HIPCHK(P.mem.alloc(&P.x64, N));
+    HIPCHK(P.mem.alloc(&P.r64, N));
+    if (mode == RUV_MODE_FP64) {
+        HIPCHK(P.mem.alloc(&P.z64, N));
+        HIPCHK(P.mem.alloc(&P.p64, N));
+        HIPCHK(P.mem.alloc(&P.q64, N));
+        HIPCHK(P.mem.alloc(&P.y64, N));
+        HIPCHK(P.mem.alloc(&P.t64, N));
+    } else {
+        HIPCHK(P.mem.alloc(&P.x32, N));
+        HIPCHK(P.mem.alloc(&P.r32, N));
+        HIPCHK(P.mem.alloc(&P.z32, N));
+        HIPCHK(P.mem.alloc(&P.p32, N));
+        HIPCHK(P.mem.alloc(&P.q32, N));
+        HIPCHK(P.mem.alloc(&P.y32, N));
+        HIPCHK(P.mem.alloc(&P.t32, N));
+    }
+    HIPCHK(P.mem.alloc(&P.part_a, MAXG));
+    HIPCHK(P.mem.alloc(&P.part_b, MAXG));
+    HIPCHK(P.mem.alloc(&P.part_c, MAXG));
+    HIPCHK(P.mem.alloc(&P.st, 1));
+    HIPCHK(hipHostMalloc((void**)&P.h_st, sizeof(SolveState), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&P.h_part, sizeof(double) * MAXG, hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(s));  // the staging vectors above must outlive their copies
+    return RUV_OK;
+}
+
+// Sum of the G partials of part (host side, block order: deterministic).
+int host_sum(PlanImpl& P, const double* part, double* out) {
+    HIPCHK(hipMemcpyAsync(P.h_part, part, sizeof(double) * P.G, hipMemcpyDeviceToHost, P.stream));
+    HIPCHK(hipStreamSynchronize(P.stream));
+    double s = 0.0;
+    for (int i = 0; i < P.G; ++i) s += P.h_part[i];
+    *out = s;
+    return RUV_OK;
+}
+
+// z = M^-1 r for the triangular preconditioners (forward over L, backward over L^T).
+template <typename T, int R>
+void apply_tri(PlanImpl& P, const DevTri<T>& f, const T* r, T* z, T* y, T* t, const SolveState* st) {
+    const T* in = r;
+    T* out = z;
+    if (P.permuted) {
+        hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_orig, r, t, st);
+        in = t;
+    }
+    hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, f.n_fwd, f.fwd_off, f.fwd_rows,
+                       f.l_ptr, f.l_cols, f.l_vals, in, y, st);
+    hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, f.n_bwd, f.bwd_off, f.bwd_rows,
+                       f.lt_ptr, f.lt_cols, f.lt_vals, (const T*)y, P.permuted ? t : out, st);
+    if (P.permuted) hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_perm, (const T*)t, out, st);
+}
+
+// PCG on the device.  On entry r holds the right-hand side (x0 = 0), x is overwritten.
+// tol_abs_from: tol relative to ||r0|| (tol absolute when ||r0|| <= 1e-300), as solver.rs:465-468.
+template <typename T, int R>
+int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* x, T* r, T* z, T* p, T* q,
+            T* y, T* t, uint64_t max_iter, double tol, uint64_t* iters_out, int* done_out) {
+    const int G = P.G, n = P.n;
+    hipStream_t s = P.stream;
+    const int pc = P.precond;
+    // x = 0; bb = r.r
+    hipLaunchKernelGGL((k_fill0<T>), dim3(G), dim3(BLK), 0, s, (int64_t)n * R, x);
+    hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, (const T*)r, P.part_a, (const SolveState*)nullptr);
+    hipLaunchKernelGGL(k_init_state, dim3(1), dim3(BLK), 0, s, P.part_a, G, tol, max_iter, P.st);
+    // z0 = M^-1 r0; p = z0; part_c = r.z
+    if (pc == RUV_PC_JACOBI) {
+        hipLaunchKernelGGL((k_jacobi<T, R>), dim3(G), dim3(BLK), 0, s, n, dinv, (const T*)r, z);
+    } else if (pc == RUV_PC_IC0 || pc == RUV_PC_IC0COLOR) {
+        apply_tri<T, R>(P, tri, r, z, y, t, nullptr);
+    }
+    const T* zz = (pc == RUV_PC_NONE) ? (const T*)r : (const T*)z;
+    hipLaunchKernelGGL((k_copy<T>), dim3(G), dim3(BLK), 0, s, (int64_t)n * R, zz, p);
+    hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, zz, P.part_c, (const SolveState*)nullptr);
+    HIPCHK(hipGetLastError());
+
+    uint64_t launched = 0;
+    int chunk = 4;
+    for (;;) {
+        for (int it = 0; it < chunk; ++it) {
+            hipLaunchKernelGGL((k_spmv_pq<T, R>), dim3(G), dim3(BLK), 0, s, n, P.ptr, P.cols, vals, (const T*)p, q,
+                               P.part_a, (const double*)P.part_c, G, P.st);
+            if (pc == RUV_PC_JACOBI) {
+                hipLaunchKernelGGL((k_update<T, R, 1>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
+                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                hipLaunchKernelGGL((k_pupdate<T, R, 1>), dim3(G), dim3(BLK), 0, s, n, (const T*)z, p,
+                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+            } else if (pc == RUV_PC_NONE) {
+                hipLaunchKernelGGL((k_update<T, R, 0>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
+                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                hipLaunchKernelGGL((k_pupdate<T, R, 0>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, p,
+                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+            } else {
+                hipLaunchKernelGGL((k_update<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
+                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                apply_tri<T, R>(P, tri, r, z, y, t, P.st);
+                hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, (const T*)z, P.part_c,
+                                   (const SolveState*)P.st);
+                hipLaunchKernelGGL((k_pupdate<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)z, p,
+                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+            }
+        }
+        launched += chunk;
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(P.h_st, P.st, sizeof(SolveState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (P.h_st->done != RUNNING) break;
+        if (launched > max_iter + 1) {  // cannot happen: the device stops at max_iter
+            return fail(RUV_ERR_NOT_CONVERGED, "pcg: iteration accounting overrun");
+        }
+        chunk = std::min(chunk * 2, 64);
+    }
+    *iters_out = P.h_st->iters;
+    *done_out = P.h_st->done;
+    return RUV_OK;
+}
+
+int status_of(int done, uint64_t iters) {
+    if (done == CONVERGED) return RUV_OK;
+    if (done == BREAKDOWN) return fail(RUV_ERR_BREAKDOWN, "pcg: breakdown (|p.Ap| <= 1e-300 or non-finite) at iteration " + std::to_string(iters));
+    return fail(RUV_ERR_NOT_CONVERGED, "pcg: not converged after " + std::to_string(iters) + " iterations");
+}
+
+template <int R>
+int true_residual(PlanImpl& P, const double* b, const double* x, double* rel) {
+    hipLaunchKernelGGL((k_residual<R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.ptr, P.cols, P.vals64, x, b, P.r64,
+                       P.part_a);
+    hipLaunchKernelGGL((k_dot<double, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, b, b, P.part_b, (const SolveState*)nullptr);
+    HIPCHK(hipGetLastError());
+    double rr = 0, bb = 0;
+    int rc = host_sum(P, P.part_a, &rr);
+    if (rc) return rc;
+    rc = host_sum(P, P.part_b, &bb);
+    if (rc) return rc;
+    *rel = bb > 0 ? std::sqrt(rr) / std::sqrt(bb) : std::sqrt(rr);
+    return RUV_OK;
+}
+
+template <int R>
+int solve_fp64(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info) {
+    hipStream_t s = P.stream;
+    HIPCHK(hipMemcpyAsync(P.r64, d_b, sizeof(double) * (size_t)P.n * R, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipEventRecord(P.ev0, s));
+    uint64_t iters = 0;
+    int done = 0;
+    int rc = run_pcg<double, R>(P, P.vals64, P.dinv64, P.tri64, d_x, P.r64, P.z64, P.p64, P.q64, P.y64, P.t64,
+                                max_iter, tol, &iters, &done);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(P.ev1, s));
+    HIPCHK(hipEventSynchronize(P.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
+    info->iters = iters;
+    info->refinements = 0;
+    info->precond = P.precond;
+    info->seconds = ms * 1e-3;
+    rc = status_of(done, iters);
+    if (rc) return rc;
+    return true_residual<R>(P, d_b, d_x, &info->true_rel_resid);
+}
+
+template <int R>
+int solve_mixed(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info) {
+    hipStream_t s = P.stream;
+    const int G = P.G, n = P.n;
+    const int64_t N = (int64_t)n * R;
+    HIPCHK(hipEventRecord(P.ev0, s));
+    hipLaunchKernelGGL((k_fill0<double>), dim3(G), dim3(BLK), 0, s, N, d_x);
+    HIPCHK(hipMemcpyAsync(P.r64, d_b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL((k_dot<double, R>), dim3(G), dim3(BLK), 0, s, n, d_b, d_b, P.part_a, (const SolveState*)nullptr);
+    double bb = 0;
+    int rc = host_sum(P, P.part_a, &bb);
+    if (rc) return rc;
+    const double bnorm = std::sqrt(bb);
+    const double target = (bnorm <= 1e-300) ? tol : tol * bnorm;
+    double rr = bb;
+    uint64_t total = 0;
+    uint32_t refs = 0;
+    for (;;) {
+        if (std::sqrt(rr) <= target) break;
+        if (refs >= (uint32_t)P.max_refine) {
+            return fail(RUV_ERR_NOT_CONVERGED, "mixed: refinement stalled at rel. residual " +
+                                                   std::to_string(std::sqrt(rr) / std::max(bnorm, 1e-300)));
+        }
+        if (total >= max_iter) return fail(RUV_ERR_NOT_CONVERGED, "mixed: inner iteration budget exhausted");
+        // r32 = fp32(r64); inner FP32 PCG to inner_tol relative to ||r32||
+        hipLaunchKernelGGL((k_to_f32), dim3(G), dim3(BLK), 0, s, N, (const double*)P.r64, P.r32);
+        uint64_t it = 0;
+        int done = 0;
+        double itol = std::max(P.inner_tol, 0.25 * target / std::max(std::sqrt(rr), 1e-300));
+        rc = run_pcg<float, R>(P, P.vals32, P.dinv32, P.tri32, P.x32, P.r32, P.z32, P.p32, P.q32, P.y32, P.t32,
+                               max_iter - total, itol, &it, &done);
+        if (rc) return rc;
+        total += it;
+        refs++;
+        if (done == BREAKDOWN && it <= 1) return status_of(done, it);
+        // x += d; r = b - A x (FP64) ; rr
+        hipLaunchKernelGGL((k_axpy_f32), dim3(G), dim3(BLK), 0, s, N, (const float*)P.x32, d_x);
+        hipLaunchKernelGGL((k_residual<R>), dim3(G), dim3(BLK), 0, s, n, P.ptr, P.cols, P.vals64, (const double*)d_x,
+                           d_b, P.r64, P.part_a);
+        HIPCHK(hipGetLastError());
+        rc = host_sum(P, P.part_a, &rr);
+        if (rc) return rc;
+        if (!std::isfinite(rr)) return fail(RUV_ERR_NONFINITE, "mixed: non-finite residual");
+    }
+    HIPCHK(hipEventRecord(P.ev1, s));
+    HIPCHK(hipEventSynchronize(P.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P.ev0, P.ev1));
+    info->iters = total;
+    info->refinements = refs;
+    info->precond = P.precond;
+    info->seconds = ms * 1e-3;
+    info->true_rel_resid = bnorm > 0 ? std::sqrt(rr) / bnorm : std::sqrt(rr);
+    return RUV_OK;
+}
+
+}  // namespace
+
+int plan_create(const Csr& a, int R, int precond, int mode, PlanImpl** out) {
+    if (R != 1 && R != 2) return fail(RUV_ERR_INVALID, "rhs_per_row must be 1 or 2");
+    if (mode != RUV_MODE_FP64 && mode != RUV_MODE_MIXED) return fail(RUV_ERR_INVALID, "unknown mode");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RUV_ERR_UNAVAILABLE, "no HIP device");
+    std::unique_ptr<PlanImpl> P(new PlanImpl());
+    int rc = plan_setup(*P, a, R, precond, mode);
+    if (rc) return rc;
+    *out = P.release();
+    return RUV_OK;
+}
+
+void plan_destroy(PlanImpl* p) { delete p; }
+
+int plan_solve_dev(PlanImpl* P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info) {
+    ruv_solve_info tmp{};
+    if (!info) info = &tmp;
+    if (P->mode == RUV_MODE_FP64) return P->R == 2 ? solve_fp64<2>(*P, d_b, d_x, max_iter, tol, info)
+                                                   : solve_fp64<1>(*P, d_b, d_x, max_iter, tol, info);
+    return P->R == 2 ? solve_mixed<2>(*P, d_b, d_x, max_iter, tol, info) : solve_mixed<1>(*P, d_b, d_x, max_iter, tol, info);
+}
+
+int plan_solve_host(PlanImpl* P, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info) {
+    const size_t N = (size_t)P->n * P->R;
+    double *db = nullptr, *dx = nullptr;
+    HIPCHK(hipMallocAsync((void**)&db, N * sizeof(double), P->stream));
+    HIPCHK(hipMallocAsync((void**)&dx, N * sizeof(double), P->stream));
+    HIPCHK(hipMemcpyAsync(db, b, N * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    int rc = plan_solve_dev(P, db, dx, max_iter, tol, info);
+    if (rc == RUV_OK) {
+        hipError_t e = hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, P->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        if (e != hipSuccess) rc = fail(RUV_ERR_DEVICE, std::string("copy x: ") + hipGetErrorString(e));
+    }
+    (void)hipFreeAsync(db, P->stream);
+    (void)hipFreeAsync(dx, P->stream);
+    (void)hipStreamSynchronize(P->stream);
+    if (rc == RUV_OK) {
+        for (size_t i = 0; i < N; ++i)
+            if (!std::isfinite(x[i])) return fail(RUV_ERR_NONFINITE, "non-finite solution");
+    }
+    return rc;
+}
+
+int device_solve(const Csr& a, int R, const double* b, uint64_t max_iter, double tol, int precond, int mode,
+                 double* x, ruv_solve_info* info) {
+    PlanImpl* P = nullptr;
+    int rc = plan_create(a, R, precond, mode, &P);
+    if (rc) return rc;
+    rc = plan_solve_host(P, b, x, max_iter, tol, info);
+    plan_destroy(P);
+    return rc;
+}
+
+int device_apply_ic0(const Csr& a, int R, const double* r, double* z) {
+    PlanImpl* P = nullptr;
+    int rc = plan_create(a, R, RUV_PC_IC0, RUV_MODE_FP64, &P);
+    if (rc) return rc;
+    std::unique_ptr<PlanImpl> hold(P);
+    const size_t N = (size_t)P->n * R;
+    HIPCHK(hipMemcpyAsync(P->r64, r, N * sizeof(double), hipMemcpyHostToDevice, P->stream));
+    if (R == 2) apply_tri<double, 2>(*P, P->tri64, P->r64, P->z64, P->y64, P->t64, nullptr);
+    else apply_tri<double, 1>(*P, P->tri64, P->r64, P->z64, P->y64, P->t64, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(z, P->z64, N * sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return RUV_OK;
+}
+
+}  // namespace ruv

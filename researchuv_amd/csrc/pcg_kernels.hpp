@@ -1,0 +1,303 @@
+// pcg_kernels.hpp -- HIP kernels of the device PCG (included by pcg.hip only).
+//
+// Vectors hold R interleaved right-hand sides per operator row (R = 2 for the LSCM (u, v) pair).  Every
+// reduction is deterministic: a wave64 xor-butterfly, then the waves of the workgroup in index order, then
+// the per-workgroup partials in index order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ruv {
+
+constexpr int BLK = 256;          // threads per workgroup of the vector / SpMV kernels
+constexpr int WAVES = BLK / 64;   // wave64
+constexpr int MAXG = 1024;        // cap on workgroups, hence on partials per reduction
+constexpr int TRI_BLK = 1024;     // single-workgroup triangular sweep
+
+enum Done : int32_t { RUNNING = 0, CONVERGED = 1, BREAKDOWN = 2, MAXITER = 3 };
+
+// Device-resident scalar state of one PCG run.
+struct SolveState {
+    double rz;        // r.z of the current residual
+    double tol_abs;   // stop when ||r|| <= tol_abs
+    double rr;        // last r.r
+    double pap;       // last p.Ap
+    uint64_t iters;   // iterations started
+    uint64_t max_iter;
+    int32_t done;
+    int32_t pad;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup; every thread returns the same value.  sh holds WAVES doubles.
+__device__ __forceinline__ double block_sum_all(double v, double* sh) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) s += sh[i];
+    return s;
+}
+
+// Sum of G per-workgroup partials, in a fixed order; identical in every workgroup.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int G, double* sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < G; i += BLK) v += part[i];
+    return block_sum_all(v, sh);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_fill0(int64_t n, T* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_copy(int64_t n, const T* __restrict__ src, T* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(BLK) void k_to_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) dst[i] = (float)src[i];
+}
+
+// x64 += d32 (the FP64 solution update of the refinement loop)
+__global__ __launch_bounds__(BLK) void k_axpy_f32(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] += (double)d[i];
+}
+
+// per-workgroup partials of a.b (FP64 accumulation)
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void k_dot(int n, const T* __restrict__ a, const T* __restrict__ b,
+                                             double* __restrict__ part, const SolveState* st) {
+    __shared__ double sh[WAVES];
+    if (st && st->done) return;
+    double acc = 0.0;
+    const int64_t N = (int64_t)n * R;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < N; i += (int64_t)gridDim.x * BLK)
+        acc += (double)a[i] * (double)b[i];
+    const double s = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// State at x0 = 0, r0 = b (solver.rs:461-468): tol_abs = tol*||b|| (tol when ||b|| <= 1e-300).
+__global__ __launch_bounds__(BLK) void k_init_state(const double* __restrict__ part_bb, int G, double tol,
+                                                    uint64_t max_iter, SolveState* st) {
+    __shared__ double sh[WAVES];
+    const double bb = sum_partials(part_bb, G, sh);
+    if (threadIdx.x == 0) {
+        const double bnorm = sqrt(bb);
+        st->tol_abs = (bnorm <= 1e-300) ? tol : tol * bnorm;
+        st->rr = bb;
+        st->rz = 0.0;
+        st->pap = 0.0;
+        st->iters = 0;
+        st->max_iter = max_iter;
+        st->done = (sqrt(bb) <= st->tol_abs) ? CONVERGED : RUNNING;
+    }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void k_jacobi(int n, const T* __restrict__ dinv, const T* __restrict__ r,
+                                                T* __restrict__ z) {
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+        const T d = dinv[i];
+#pragma unroll
+        for (int c = 0; c < R; ++c) z[(int64_t)i * R + c] = d * r[(int64_t)i * R + c];
+    }
+}
+
+// q = A p and partials of p.q.  Workgroup 0 also opens the iteration: rz <- sum(part_rz), iteration
+// count, and the max_iter stop of `while !converged && iters < max_iter` (solver.rs:489-490).
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void k_spmv_pq(int n, const int* __restrict__ ptr, const int* __restrict__ cols,
+                                                 const T* __restrict__ vals, const T* __restrict__ p,
+                                                 T* __restrict__ q, double* __restrict__ part_pq,
+                                                 const double* __restrict__ part_rz, int G, SolveState* st) {
+    __shared__ double sh[WAVES];
+    if (st->done) return;
+    if (blockIdx.x == 0) {
+        const double rz = sum_partials(part_rz, G, sh);
+        if (threadIdx.x == 0) {
+            if (st->iters >= st->max_iter) {
+                st->done = MAXITER;
+            } else {
+                st->iters += 1;
+                st->rz = rz;
+            }
+        }
+    }
+    double acc = 0.0;
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+        T s[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c] = T(0);
+        const int a = ptr[i], b = ptr[i + 1];
+        for (int k = a; k < b; ++k) {
+            const T v = vals[k];
+            const int64_t col = (int64_t)cols[k] * R;
+#pragma unroll
+            for (int c = 0; c < R; ++c) s[c] += v * p[col + c];
+        }
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            q[(int64_t)i * R + c] = s[c];
+            acc += (double)p[(int64_t)i * R + c] * (double)s[c];
+        }
+    }
+    const double bs = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) part_pq[blockIdx.x] = bs;
+}
+
+// alpha = rz / pAp (breakdown check, solver.rs:505-509); x += alpha p; r -= alpha q; z = M^-1 r for
+// PC = 1 (Jacobi); partials of r.r and (PC 0/1) r.z.
+template <typename T, int R, int PC>
+__global__ __launch_bounds__(BLK) void k_update(int n, const T* __restrict__ p, const T* __restrict__ q,
+                                                T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                                const T* __restrict__ dinv, const double* __restrict__ part_pq,
+                                                double* __restrict__ part_rr, double* __restrict__ part_rz, int G,
+                                                SolveState* st) {
+    __shared__ double sh[WAVES];
+    if (st->done) return;
+    const double pap = sum_partials(part_pq, G, sh);
+    if (!(fabs(pap) > 1e-300) || !isfinite(pap)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = BREAKDOWN;
+            st->pap = pap;
+        }
+        return;
+    }
+    const double alpha = st->rz / pap;
+    const T a = (T)alpha;
+    double arr = 0.0, arz = 0.0;
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+        const T d = (PC == 1) ? dinv[i] : T(1);
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            const int64_t k = (int64_t)i * R + c;
+            const T xi = x[k] + a * p[k];
+            const T ri = r[k] - a * q[k];
+            x[k] = xi;
+            r[k] = ri;
+            arr += (double)ri * (double)ri;
+            if (PC == 1) {
+                const T zi = d * ri;
+                z[k] = zi;
+                arz += (double)ri * (double)zi;
+            }
+        }
+    }
+    const double srr = block_sum_all(arr, sh);
+    const double srz = (PC == 1) ? block_sum_all(arz, sh) : srr;
+    if (threadIdx.x == 0) {
+        part_rr[blockIdx.x] = srr;
+        if (PC != 2) part_rz[blockIdx.x] = srz;
+        if (blockIdx.x == 0) st->pap = pap;
+    }
+}
+
+// Convergence on the recursive residual (solver.rs:518-522), then beta = rz_new / rz and p = z + beta p.
+template <typename T, int R, int PC>
+__global__ __launch_bounds__(BLK) void k_pupdate(int n, const T* __restrict__ z, T* __restrict__ p,
+                                                 const double* __restrict__ part_rr,
+                                                 const double* __restrict__ part_rz, int G, SolveState* st) {
+    __shared__ double sh[WAVES];
+    if (st->done) return;
+    const double rr = sum_partials(part_rr, G, sh);
+    if (sqrt(rr) <= st->tol_abs) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = CONVERGED;
+            st->rr = rr;
+        }
+        return;
+    }
+    const double rznew = (PC == 0) ? rr : sum_partials(part_rz, G, sh);
+    const double beta = rznew / st->rz;
+    const T bt = (T)beta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rr = rr;
+    const int64_t N = (int64_t)n * R;
+    for (int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x; k < N; k += (int64_t)gridDim.x * BLK)
+        p[k] = z[k] + bt * p[k];
+}
+
+// One triangular solve over a diagonal-last CSR factor, all levels in one single-workgroup launch: the
+// rows of a level are independent; the workgroup barrier separates levels (the reference launched one
+// kernel per level, solver.rs:359-403, kernels.cu:81-95).
+template <typename T, int R>
+__global__ __launch_bounds__(TRI_BLK) void k_tri_sweep(int nlev, const int* __restrict__ off,
+                                                       const int* __restrict__ rows, const int* __restrict__ ptr,
+                                                       const int* __restrict__ cols, const T* __restrict__ vals,
+                                                       const T* __restrict__ in, T* out, const SolveState* st) {
+    if (st && st->done) return;
+    for (int l = 0; l < nlev; ++l) {
+        const int a = off[l], b = off[l + 1];
+        for (int t = a + (int)threadIdx.x; t < b; t += TRI_BLK) {
+            const int i = rows[t];
+            const int e = ptr[i + 1] - 1;  // diagonal last
+            T s[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) s[c] = in[(int64_t)i * R + c];
+            for (int k = ptr[i]; k < e; ++k) {
+                const T v = vals[k];
+                const int64_t col = (int64_t)cols[k] * R;
+#pragma unroll
+                for (int c = 0; c < R; ++c) s[c] -= v * out[col + c];
+            }
+            const T d = vals[e];
+#pragma unroll
+            for (int c = 0; c < R; ++c) out[(int64_t)i * R + c] = s[c] / d;
+        }
+        __syncthreads();
+    }
+}
+
+// dst[i] = src[idx[i]] per row (permutation round trips, kernels.cu:98-102)
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void k_gather(int n, const int* __restrict__ idx, const T* __restrict__ src,
+                                                T* __restrict__ dst, const SolveState* st) {
+    if (st && st->done) return;
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+        const int64_t s = (int64_t)idx[i] * R;
+#pragma unroll
+        for (int c = 0; c < R; ++c) dst[(int64_t)i * R + c] = src[s + c];
+    }
+}
+
+// r = b - A x in FP64 and partials of r.r (the refinement residual and the final true residual)
+template <int R>
+__global__ __launch_bounds__(BLK) void k_residual(int n, const int* __restrict__ ptr, const int* __restrict__ cols,
+                                                  const double* __restrict__ vals, const double* __restrict__ x,
+                                                  const double* __restrict__ b, double* __restrict__ r,
+                                                  double* __restrict__ part_rr) {
+    __shared__ double sh[WAVES];
+    double acc = 0.0;
+    for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+        double s[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c] = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const double v = vals[k];
+            const int64_t col = (int64_t)cols[k] * R;
+#pragma unroll
+            for (int c = 0; c < R; ++c) s[c] += v * x[col + c];
+        }
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            const double rc = b[(int64_t)i * R + c] - s[c];
+            r[(int64_t)i * R + c] = rc;
+            acc += rc * rc;
+        }
+    }
+    const double bs = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = bs;
+}
+
+}  // namespace ruv
