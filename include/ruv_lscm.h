@@ -114,6 +114,59 @@ int ruv_plan_solve_dev(ruv_plan* plan, const double* d_b, double* d_x, uint64_t 
 /* Host-buffer convenience around ruv_plan_solve_dev (copies b in, x out). */
 int ruv_plan_solve(ruv_plan* plan, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
 void ruv_plan_destroy(ruv_plan* plan);
+/* Measurement hook (no reference counterpart): run the plan's inner-precision SpMV kernel (the one its PCG
+ * iterations launch) reps times back to back on the plan's stream and return the mean launch duration from
+ * HIP events, plus the kernel's algorithmic bytes per launch: nnz*(s_v + 4) + (nrows + 1)*4 +
+ * 2*nrows*rhs_per_row*s_x (every matrix entry, index, x element and y element counted once). */
+int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
+
+/* ---------------------------------------------------------------------------------------------------------
+ * Charting and the per-chart LSCM driver (host side of the unfold path; the solves run on the device).
+ * ------------------------------------------------------------------------------------------------------- */
+
+/* weld::weld (crates/researchuv-unwrap/src/weld.rs:19-57).  pos_out (nv*3) and faces_out (nf*3) are caller
+ * buffers of the input sizes; *nv_out / *nf_out receive the welded counts. */
+int ruv_weld(const double* pos, int64_t nv, const uint32_t* faces, int64_t nf, double tol, double* pos_out,
+             uint32_t* faces_out, int64_t* nv_out, int64_t* nf_out);
+
+/* segment::segment (segment.rs:130-140) with build_charts / chart_border_loops (:143-331): the charts of a
+ * welded mesh.  The handle keeps the charts; the mesh arrays are not retained. */
+typedef struct ruv_charts ruv_charts;
+int ruv_segment(const double* pos, int64_t nv, const uint32_t* faces, int64_t nf, double angle_min_deg,
+                ruv_charts** out);
+int64_t ruv_charts_count(const ruv_charts* charts);
+/* sizes (6): nfaces, nverts, nborder, nloops, total loop length, cut edges of the whole segmentation */
+int ruv_chart_sizes(const ruv_charts* charts, int64_t i, int64_t* sizes, double* area3d);
+/* segment::Chart fields (segment.rs:32-46); loops flattened with loop_off (nloops + 1).  Any pointer may
+ * be null to skip that field. */
+int ruv_chart_arrays(const ruv_charts* charts, int64_t i, uint32_t* face_ids, uint32_t* vertex_ids,
+                     uint32_t* tris, uint32_t* border, uint32_t* loops, int64_t* loop_off);
+void ruv_charts_destroy(ruv_charts* charts);
+
+/* The free-vertex system of chart i (lscm.rs:512-666): tangent frame p2 (nv*2), pins (local ids ascending
+ * by source id, pin_uv 2 per pin), free_ids (nfree), and A = L (x) I2 as L in CSR (nfree rows; columns
+ * ascending) with the interleaved right-hand side rhs (2*nfree).  sizes (5): nv, nfree, npins, nnz(L),
+ * weighted edges.  Call with null arrays to get the sizes. */
+int ruv_lscm_system(const double* mesh_pos, const ruv_charts* charts, int64_t i, double angle_mix, int64_t* sizes,
+                    double* p2, int32_t* pin_ids, double* pin_uv, int32_t* free_ids, double* l_vals,
+                    int32_t* l_cols, int32_t* l_ptr, double* rhs);
+
+typedef struct ruv_unfold_opts {
+    int32_t max_iter;   /* UnfoldOptions.max_iter (0 -> 100, lscm.rs:46,574)                            */
+    double angle_mix;   /* UnfoldOptions.angle_mix (1.0, lscm.rs:50)                                    */
+    double mix_w;       /* UnfoldOptions.mix_w (1.0, lscm.rs:54); <= 0 is rejected like lscm.rs:479     */
+    int32_t precond;    /* ruv_precond; RESEARCHUV_GPU_PRECOND default is jacobi (lscm.rs:492-499)       */
+    int32_t mode;       /* ruv_mode                                                                     */
+    int32_t min_vars;   /* GPU size gate on 2*nfree: lscm.rs:462 uses 1024; 0 disables it              */
+    double tol;         /* 1e-12 reproduces lscm.rs:501 (mode 0); the mixed form targets the true residual */
+} ruv_unfold_opts;
+
+/* lscm::unfold_chart (lscm.rs:512-727) with SolverBackend::Gpu.  uv_out (nv*2) in chart-local order; stats
+ * (optional, 4): iters, err_accum, target, area3d (UnfoldResult, lscm.rs:104-116); info optional.  A chart
+ * below the size gate returns RUV_ERR_BAD_DIMS and a failed solve its status: the caller then runs its CPU
+ * LU, as lscm.rs:677-681 does. */
+int ruv_unfold_chart(const double* mesh_pos, const ruv_charts* charts, int64_t i, const ruv_unfold_opts* opts,
+                     double* uv_out, double* stats, ruv_solve_info* info);
 
 #ifdef __cplusplus
 }

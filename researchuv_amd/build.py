@@ -26,7 +26,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature"
 HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["pcg.hip"]
-HOST_SOURCES = ["host_precond.cpp", "api.cpp"]
+HOST_SOURCES = ["host_precond.cpp", "host_mesh.cpp", "host_lscm.cpp", "api.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -12,6 +12,7 @@ int plan_create(const Csr& a, int R, int precond, int mode, PlanImpl** out);
 void plan_destroy(PlanImpl* p);
 int plan_solve_dev(PlanImpl* P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info);
 int plan_solve_host(PlanImpl* P, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
+int plan_spmv_time(PlanImpl* P, int reps, double* sec, double* bytes);
 }  // namespace ruv
 
 struct ruv_plan {
@@ -124,10 +125,125 @@ int ruv_plan_solve(ruv_plan* plan, const double* b, double* x, uint64_t max_iter
     return ruv::plan_solve_host(plan->impl, b, x, max_iter, tol, info);
 }
 
+int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch, double* bytes_per_launch) {
+    if (!plan || !seconds_per_launch || !bytes_per_launch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::plan_spmv_time(plan->impl, reps, seconds_per_launch, bytes_per_launch);
+}
+
 void ruv_plan_destroy(ruv_plan* plan) {
     if (!plan) return;
     ruv::plan_destroy(plan->impl);
     delete plan;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------
+// charting and the per-chart driver
+// ------------------------------------------------------------------------------------------------------
+#include "host_lscm.hpp"
+
+struct ruv_charts {
+    ruv::Charts c;
+};
+
+extern "C" {
+
+int ruv_weld(const double* pos, int64_t nv, const uint32_t* faces, int64_t nf, double tol, double* pos_out,
+             uint32_t* faces_out, int64_t* nv_out, int64_t* nf_out) {
+    if ((nv && !pos) || (nf && !faces) || !pos_out || !faces_out || !nv_out || !nf_out)
+        return ruv::fail(RUV_ERR_INVALID, "null argument");
+    for (int64_t k = 0; k < 3 * nf; ++k)
+        if ((int64_t)faces[k] >= nv) return ruv::fail(RUV_ERR_BAD_DIMS, "face index out of range");
+    std::vector<double> p;
+    std::vector<uint32_t> f;
+    ruv::weld(pos, nv, faces, nf, tol, &p, &f);
+    std::copy(p.begin(), p.end(), pos_out);
+    std::copy(f.begin(), f.end(), faces_out);
+    *nv_out = (int64_t)p.size() / 3;
+    *nf_out = (int64_t)f.size() / 3;
+    return RUV_OK;
+}
+
+int ruv_segment(const double* pos, int64_t nv, const uint32_t* faces, int64_t nf, double angle_min_deg,
+                ruv_charts** out) {
+    if (!out || (nv && !pos) || (nf && !faces)) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    for (int64_t k = 0; k < 3 * nf; ++k)
+        if ((int64_t)faces[k] >= nv) return ruv::fail(RUV_ERR_BAD_DIMS, "face index out of range");
+    auto* h = new ruv_charts();
+    ruv::segment(pos, nv, faces, nf, angle_min_deg, &h->c);
+    *out = h;
+    return RUV_OK;
+}
+
+int64_t ruv_charts_count(const ruv_charts* charts) { return charts ? (int64_t)charts->c.charts.size() : 0; }
+
+int ruv_chart_sizes(const ruv_charts* charts, int64_t i, int64_t* sizes, double* area3d) {
+    if (!charts || !sizes || i < 0 || i >= (int64_t)charts->c.charts.size())
+        return ruv::fail(RUV_ERR_INVALID, "bad chart handle or index");
+    const ruv::Chart& ch = charts->c.charts[i];
+    int64_t tot = 0;
+    for (const auto& l : ch.loops) tot += (int64_t)l.size();
+    sizes[0] = (int64_t)ch.face_ids.size();
+    sizes[1] = (int64_t)ch.vertex_ids.size();
+    sizes[2] = (int64_t)ch.border.size();
+    sizes[3] = (int64_t)ch.loops.size();
+    sizes[4] = tot;
+    sizes[5] = (int64_t)charts->c.cut.size();
+    if (area3d) *area3d = ch.area3d;
+    return RUV_OK;
+}
+
+int ruv_chart_arrays(const ruv_charts* charts, int64_t i, uint32_t* face_ids, uint32_t* vertex_ids, uint32_t* tris,
+                     uint32_t* border, uint32_t* loops, int64_t* loop_off) {
+    if (!charts || i < 0 || i >= (int64_t)charts->c.charts.size())
+        return ruv::fail(RUV_ERR_INVALID, "bad chart handle or index");
+    const ruv::Chart& ch = charts->c.charts[i];
+    if (face_ids) std::copy(ch.face_ids.begin(), ch.face_ids.end(), face_ids);
+    if (vertex_ids) std::copy(ch.vertex_ids.begin(), ch.vertex_ids.end(), vertex_ids);
+    if (tris) std::copy(ch.tris.begin(), ch.tris.end(), tris);
+    if (border) std::copy(ch.border.begin(), ch.border.end(), border);
+    int64_t o = 0;
+    if (loop_off) loop_off[0] = 0;
+    for (size_t l = 0; l < ch.loops.size(); ++l) {
+        if (loops) std::copy(ch.loops[l].begin(), ch.loops[l].end(), loops + o);
+        o += (int64_t)ch.loops[l].size();
+        if (loop_off) loop_off[l + 1] = o;
+    }
+    return RUV_OK;
+}
+
+void ruv_charts_destroy(ruv_charts* charts) { delete charts; }
+
+int ruv_lscm_system(const double* mesh_pos, const ruv_charts* charts, int64_t i, double angle_mix, int64_t* sizes,
+                    double* p2, int32_t* pin_ids, double* pin_uv, int32_t* free_ids, double* l_vals, int32_t* l_cols,
+                    int32_t* l_ptr, double* rhs) {
+    if (!mesh_pos || !charts || !sizes || i < 0 || i >= (int64_t)charts->c.charts.size())
+        return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    ruv::ChartSystem S;
+    ruv::build_system(mesh_pos, charts->c.charts[i], angle_mix, &S);
+    sizes[0] = S.nv;
+    sizes[1] = S.nfree;
+    sizes[2] = (int64_t)S.pin_ids.size();
+    sizes[3] = S.l.nnz();
+    sizes[4] = (int64_t)S.edges;
+    if (p2) std::copy(S.p2.begin(), S.p2.end(), p2);
+    if (pin_ids) std::copy(S.pin_ids.begin(), S.pin_ids.end(), pin_ids);
+    if (pin_uv) std::copy(S.pin_uv.begin(), S.pin_uv.end(), pin_uv);
+    if (free_ids) std::copy(S.free_ids.begin(), S.free_ids.end(), free_ids);
+    if (l_vals) std::copy(S.l.vals.begin(), S.l.vals.end(), l_vals);
+    if (l_cols) std::copy(S.l.cols.begin(), S.l.cols.end(), l_cols);
+    if (l_ptr) std::copy(S.l.ptr.begin(), S.l.ptr.end(), l_ptr);
+    if (rhs) std::copy(S.rhs.begin(), S.rhs.end(), rhs);
+    return RUV_OK;
+}
+
+int ruv_unfold_chart(const double* mesh_pos, const ruv_charts* charts, int64_t i, const ruv_unfold_opts* opts,
+                     double* uv_out, double* stats, ruv_solve_info* info) {
+    if (!mesh_pos || !charts || !opts || !uv_out || i < 0 || i >= (int64_t)charts->c.charts.size())
+        return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    if (opts->precond < RUV_PC_NONE || opts->precond > RUV_PC_AMGSA) return ruv::fail(RUV_ERR_INVALID, "unknown precond");
+    return ruv::unfold_chart(mesh_pos, charts->c.charts[i], *opts, uv_out, stats, info);
 }
 
 }  // extern "C"

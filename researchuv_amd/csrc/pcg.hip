@@ -460,6 +460,42 @@ int plan_solve_host(PlanImpl* P, const double* b, double* x, uint64_t max_iter, 
     return rc;
 }
 
+int plan_spmv_time(PlanImpl* P, int reps, double* sec, double* bytes) {
+    if (reps <= 0) return fail(RUV_ERR_INVALID, "reps must be positive");
+    hipStream_t s = P->stream;
+    const int64_t N = (int64_t)P->n * P->R;
+    const bool f32 = P->mode == RUV_MODE_MIXED;
+    // a converged-free state so the kernel does its full work; p = 1
+    hipLaunchKernelGGL((k_fill0<double>), dim3(P->G), dim3(BLK), 0, s, (int64_t)MAXG, P->part_c);
+    SolveState init{};
+    init.max_iter = ~0ull;
+    HIPCHK(hipMemcpyAsync(P->st, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    if (f32) hipLaunchKernelGGL((k_fill<float>), dim3(P->G), dim3(BLK), 0, s, N, 1.0f, P->p32);
+    else hipLaunchKernelGGL((k_fill<double>), dim3(P->G), dim3(BLK), 0, s, N, 1.0, P->p64);
+    HIPCHK(hipStreamSynchronize(s));
+    auto launch = [&]() {
+        if (f32) {
+            if (P->R == 2) hipLaunchKernelGGL((k_spmv_pq<float, 2>), dim3(P->G), dim3(BLK), 0, s, P->n, P->ptr, P->cols, (const float*)P->vals32, (const float*)P->p32, P->q32, P->part_a, (const double*)P->part_c, P->G, P->st);
+            else hipLaunchKernelGGL((k_spmv_pq<float, 1>), dim3(P->G), dim3(BLK), 0, s, P->n, P->ptr, P->cols, (const float*)P->vals32, (const float*)P->p32, P->q32, P->part_a, (const double*)P->part_c, P->G, P->st);
+        } else {
+            if (P->R == 2) hipLaunchKernelGGL((k_spmv_pq<double, 2>), dim3(P->G), dim3(BLK), 0, s, P->n, P->ptr, P->cols, (const double*)P->vals64, (const double*)P->p64, P->q64, P->part_a, (const double*)P->part_c, P->G, P->st);
+            else hipLaunchKernelGGL((k_spmv_pq<double, 1>), dim3(P->G), dim3(BLK), 0, s, P->n, P->ptr, P->cols, (const double*)P->vals64, (const double*)P->p64, P->q64, P->part_a, (const double*)P->part_c, P->G, P->st);
+        }
+    };
+    launch();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P->ev0, s));
+    for (int i = 0; i < reps; ++i) launch();
+    HIPCHK(hipEventRecord(P->ev1, s));
+    HIPCHK(hipEventSynchronize(P->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, P->ev0, P->ev1));
+    *sec = ms * 1e-3 / reps;
+    const double sv = f32 ? 4.0 : 8.0;
+    *bytes = (double)P->nnz * (sv + 4.0) + (double)(P->n + 1) * 4.0 + 2.0 * (double)N * sv;
+    return RUV_OK;
+}
+
 int device_solve(const Csr& a, int R, const double* b, uint64_t max_iter, double tol, int precond, int mode,
                  double* x, ruv_solve_info* info) {
     PlanImpl* P = nullptr;

@@ -61,6 +61,11 @@ __global__ __launch_bounds__(BLK) void k_fill0(int64_t n, T* __restrict__ x) {
 }
 
 template <typename T>
+__global__ __launch_bounds__(BLK) void k_fill(int64_t n, T v, T* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] = v;
+}
+
+template <typename T>
 __global__ __launch_bounds__(BLK) void k_copy(int64_t n, const T* __restrict__ src, T* __restrict__ dst) {
     for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) dst[i] = src[i];
 }
