@@ -121,6 +121,47 @@ void ruv_plan_destroy(ruv_plan* plan);
 int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
 
 /* ---------------------------------------------------------------------------------------------------------
+ * Batched solves (no reference counterpart: the reference solved one chart per worker-thread call,
+ * pipeline.rs:171-194 -> lscm.rs:471-506).  A batch holds nchart independent systems L_c (x) I_R as one
+ * block-diagonal CSR: rows of chart c are [chart_row_off[c], chart_row_off[c+1]) and its columns stay inside
+ * that range (row_ptr is 64-bit; columns are global row indices).  Every chart runs its own PCG (own
+ * alpha/beta/stopping test) inside shared launches; precond none or jacobi.  mode 0: FP64 PCG per chart with
+ * the recursive-residual test tol*||b_c||; mode 1: FP32 PCG inside per-chart FP64 refinement until
+ * ||b_c - A_c x_c|| <= tol*||b_c||.  info->iters is the largest per-chart iteration count and
+ * info->true_rel_resid the worst chart; chart_iters / chart_true_resid (nchart, optional) give every chart.
+ * ------------------------------------------------------------------------------------------------------- */
+typedef struct ruv_batch ruv_batch;
+int ruv_batch_create(int64_t nrows, const int64_t* row_ptr, const int32_t* cols, const double* vals, int32_t nchart,
+                     const int64_t* chart_row_off, int32_t rhs_per_row, int32_t precond, int32_t mode,
+                     ruv_batch** out);
+int ruv_batch_solve_dev(ruv_batch* batch, const double* d_b, double* d_x, uint64_t max_iter, double tol,
+                        ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
+int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_iter, double tol,
+                    ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
+/* Measurement hook: mean duration and algorithmic bytes of the batch's inner-precision SpMV launch over
+ * every tile (row_ptr counted at 8 bytes per row). */
+int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
+void ruv_batch_destroy(ruv_batch* batch);
+
+/* Build the LSCM systems of nmesh meshes on host worker threads (the worker-thread unfold stage,
+ * pipeline.rs:171-194 over exec::par_map, core/src/exec.rs:14-60; threads 0 = all cores): each mesh m
+ * (positions pos[pos_off[m]..pos_off[m+1]) as xyz triples, faces[face_off[m]..face_off[m+1]) as index
+ * triples local to the mesh) is welded, segmented and every chart assembled; charts are concatenated in
+ * (mesh, chart) order as the diagonal blocks of one batch (rhs_per_row = 2). */
+typedef struct ruv_lscm_batch ruv_lscm_batch;
+int ruv_lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                         const int64_t* face_off, double weld_tol, double angle_min_deg, double angle_mix,
+                         int32_t threads, ruv_lscm_batch** out);
+/* sizes (4): nchart, nrows (free vertices), nnz(L), total chart vertices */
+int ruv_lscm_batch_sizes(const ruv_lscm_batch* lb, int64_t* sizes);
+/* any pointer may be null: row_ptr (nrows+1), cols / vals (nnz), rhs (2*nrows), chart_row_off (nchart+1),
+ * chart_mesh / chart_nv / chart_npins (nchart) */
+int ruv_lscm_batch_arrays(const ruv_lscm_batch* lb, int64_t* row_ptr, int32_t* cols, double* vals, double* rhs,
+                          int64_t* chart_row_off, int32_t* chart_mesh, int64_t* chart_nv, int64_t* chart_npins);
+int ruv_batch_create_lscm(const ruv_lscm_batch* lb, int32_t precond, int32_t mode, ruv_batch** out);
+void ruv_lscm_batch_destroy(ruv_lscm_batch* lb);
+
+/* ---------------------------------------------------------------------------------------------------------
  * Charting and the per-chart LSCM driver (host side of the unfold path; the solves run on the device).
  * ------------------------------------------------------------------------------------------------------- */
 

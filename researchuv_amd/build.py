@@ -23,10 +23,10 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", f"-I{ROOT}/include", f"-I{CSRC}"]
 HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-munsafe-fp-atomics"]
-HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__"]
+HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["pcg.hip"]
-HOST_SOURCES = ["host_precond.cpp", "host_mesh.cpp", "host_lscm.cpp", "api.cpp"]
+HIP_SOURCES = ["pcg.hip", "batch.hip"]
+HOST_SOURCES = ["host_precond.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp", "api.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -35,7 +35,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, *COMMON, *HIP_FLAGS, "-c", path, "-o", obj]
     else:
-        cmd = [HIPCC, *COMMON, *HOST_FLAGS, "-c", path, "-o", obj]
+        cmd = [HIPCC, *COMMON, *HOST_FLAGS, "-x", "c++", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

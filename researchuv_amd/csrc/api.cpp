@@ -4,6 +4,7 @@
 #include <cmath>
 #include <string>
 
+#include "host_lscm.hpp"
 #include "ruv_internal.hpp"
 
 namespace ruv {
@@ -13,7 +14,23 @@ void plan_destroy(PlanImpl* p);
 int plan_solve_dev(PlanImpl* P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info);
 int plan_solve_host(PlanImpl* P, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
 int plan_spmv_time(PlanImpl* P, int reps, double* sec, double* bytes);
+struct BatchImpl;
+int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const double* vals, int nchart,
+                 const int64_t* crow, int R, int precond, int mode, BatchImpl** out);
+void batch_destroy(BatchImpl* q);
+int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_iter, double tol,
+                    ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid);
+int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
+int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                     const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
+                     LscmBatch** out);
+void lscm_batch_destroy(LscmBatch* b);
 }  // namespace ruv
+
+struct ruv_batch {
+    ruv::BatchImpl* impl;
+    int64_t n;  // rows * rhs_per_row
+};
 
 struct ruv_plan {
     ruv::PlanImpl* impl;
@@ -244,6 +261,143 @@ int ruv_unfold_chart(const double* mesh_pos, const ruv_charts* charts, int64_t i
         return ruv::fail(RUV_ERR_INVALID, "bad argument");
     if (opts->precond < RUV_PC_NONE || opts->precond > RUV_PC_AMGSA) return ruv::fail(RUV_ERR_INVALID, "unknown precond");
     return ruv::unfold_chart(mesh_pos, charts->c.charts[i], *opts, uv_out, stats, info);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------
+// batched solves
+// ------------------------------------------------------------------------------------------------------
+extern "C" {
+
+int ruv_batch_create(int64_t nrows, const int64_t* row_ptr, const int32_t* cols, const double* vals, int32_t nchart,
+                     const int64_t* chart_row_off, int32_t rhs_per_row, int32_t precond, int32_t mode,
+                     ruv_batch** out) {
+    if (!out) return ruv::fail(RUV_ERR_INVALID, "null batch pointer");
+    *out = nullptr;
+    if (nrows <= 0 || nchart <= 0) return ruv::fail(RUV_ERR_BAD_DIMS, "empty batch");
+    if (!row_ptr || !cols || !vals || !chart_row_off) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    if (nrows >= INT32_MAX) return ruv::fail(RUV_ERR_BAD_DIMS, "too many rows for 32-bit column indices");
+    if (row_ptr[0] != 0 || chart_row_off[0] != 0 || chart_row_off[nchart] != nrows)
+        return ruv::fail(RUV_ERR_BAD_DIMS, "row_ptr / chart_row_off do not start at 0 or end at nrows");
+    for (int32_t c = 0; c < nchart; ++c) {
+        if (chart_row_off[c + 1] < chart_row_off[c]) return ruv::fail(RUV_ERR_BAD_DIMS, "chart_row_off not monotone");
+        for (int64_t i = chart_row_off[c]; i < chart_row_off[c + 1]; ++i) {
+            if (row_ptr[i + 1] < row_ptr[i]) return ruv::fail(RUV_ERR_BAD_DIMS, "row_ptr not monotone");
+            for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+                if (cols[k] < chart_row_off[c] || cols[k] >= chart_row_off[c + 1])
+                    return ruv::fail(RUV_ERR_BAD_DIMS, "column outside its chart's block (chart " + std::to_string(c) + ")");
+        }
+    }
+    ruv::BatchImpl* impl = nullptr;
+    int rc = ruv::batch_create(nrows, row_ptr, cols, vals, nchart, chart_row_off, rhs_per_row, precond, mode, &impl);
+    if (rc) return rc;
+    *out = new ruv_batch{impl, nrows * rhs_per_row};
+    return RUV_OK;
+}
+
+int ruv_batch_solve_dev(ruv_batch* batch, const double* d_b, double* d_x, uint64_t max_iter, double tol,
+                        ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid) {
+    if (!batch || !d_b || !d_x) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_solve_dev(batch->impl, d_b, d_x, max_iter, tol, info, chart_iters, chart_true_resid);
+}
+
+int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_iter, double tol,
+                    ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid) {
+    if (!batch || !b || !x) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    const size_t N = (size_t)batch->n;
+    double *db = nullptr, *dx = nullptr;
+    hipError_t e = hipMalloc((void**)&db, N * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void**)&dx, N * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(db, b, N * sizeof(double), hipMemcpyHostToDevice);
+    int rc = (e == hipSuccess) ? ruv_batch_solve_dev(batch, db, dx, max_iter, tol, info, chart_iters, chart_true_resid)
+                               : ruv::fail(RUV_ERR_DEVICE, hipGetErrorString(e));
+    if (rc == RUV_OK) {
+        e = hipMemcpy(x, dx, N * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = ruv::fail(RUV_ERR_DEVICE, hipGetErrorString(e));
+    }
+    if (db) (void)hipFree(db);
+    if (dx) (void)hipFree(dx);
+    if (rc == RUV_OK)
+        for (size_t i = 0; i < N; ++i)
+            if (!std::isfinite(x[i])) return ruv::fail(RUV_ERR_NONFINITE, "non-finite solution");
+    return rc;
+}
+
+int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch) {
+    if (!batch || !seconds_per_launch || !bytes_per_launch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_spmv_time(batch->impl, reps, seconds_per_launch, bytes_per_launch);
+}
+
+void ruv_batch_destroy(ruv_batch* batch) {
+    if (!batch) return;
+    ruv::batch_destroy(batch->impl);
+    delete batch;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------
+// host batch building
+// ------------------------------------------------------------------------------------------------------
+struct ruv_lscm_batch {
+    ruv::LscmBatch* b;
+};
+
+extern "C" {
+
+int ruv_lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                         const int64_t* face_off, double weld_tol, double angle_min_deg, double angle_mix,
+                         int32_t threads, ruv_lscm_batch** out) {
+    if (!out || nmesh <= 0 || !pos || !pos_off || !faces || !face_off) return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    for (int32_t m = 0; m < nmesh; ++m) {
+        if (pos_off[m + 1] < pos_off[m] || face_off[m + 1] < face_off[m]) return ruv::fail(RUV_ERR_BAD_DIMS, "offsets not monotone");
+        const int64_t nv = pos_off[m + 1] - pos_off[m];
+        for (int64_t k = 3 * face_off[m]; k < 3 * face_off[m + 1]; ++k)
+            if ((int64_t)faces[k] >= nv) return ruv::fail(RUV_ERR_BAD_DIMS, "face index outside its mesh");
+    }
+    ruv::LscmBatch* b = nullptr;
+    int rc = ruv::lscm_batch_build(nmesh, pos, pos_off, faces, face_off, weld_tol, angle_min_deg, angle_mix, threads, &b);
+    if (rc) return rc;
+    *out = new ruv_lscm_batch{b};
+    return RUV_OK;
+}
+
+int ruv_lscm_batch_sizes(const ruv_lscm_batch* lb, int64_t* sizes) {
+    if (!lb || !sizes) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    sizes[0] = lb->b->nchart;
+    sizes[1] = lb->b->nrows;
+    sizes[2] = (int64_t)lb->b->vals.size();
+    sizes[3] = lb->b->nv_total;
+    return RUV_OK;
+}
+
+int ruv_lscm_batch_arrays(const ruv_lscm_batch* lb, int64_t* row_ptr, int32_t* cols, double* vals, double* rhs,
+                          int64_t* chart_row_off, int32_t* chart_mesh, int64_t* chart_nv, int64_t* chart_npins) {
+    if (!lb) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    const ruv::LscmBatch& b = *lb->b;
+    if (row_ptr) std::copy(b.row_ptr.begin(), b.row_ptr.end(), row_ptr);
+    if (cols) std::copy(b.cols.begin(), b.cols.end(), cols);
+    if (vals) std::copy(b.vals.begin(), b.vals.end(), vals);
+    if (rhs) std::copy(b.rhs.begin(), b.rhs.end(), rhs);
+    if (chart_row_off) std::copy(b.chart_row_off.begin(), b.chart_row_off.end(), chart_row_off);
+    if (chart_mesh) std::copy(b.chart_mesh.begin(), b.chart_mesh.end(), chart_mesh);
+    if (chart_nv) std::copy(b.chart_nv.begin(), b.chart_nv.end(), chart_nv);
+    if (chart_npins) std::copy(b.chart_npins.begin(), b.chart_npins.end(), chart_npins);
+    return RUV_OK;
+}
+
+int ruv_batch_create_lscm(const ruv_lscm_batch* lb, int32_t precond, int32_t mode, ruv_batch** out) {
+    if (!lb) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    const ruv::LscmBatch& b = *lb->b;
+    return ruv_batch_create(b.nrows, b.row_ptr.data(), b.cols.data(), b.vals.data(), (int32_t)b.nchart,
+                            b.chart_row_off.data(), 2, precond, mode, out);
+}
+
+void ruv_lscm_batch_destroy(ruv_lscm_batch* lb) {
+    if (!lb) return;
+    ruv::lscm_batch_destroy(lb->b);
+    delete lb;
 }
 
 }  // extern "C"

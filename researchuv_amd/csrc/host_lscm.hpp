@@ -22,6 +22,19 @@ struct ChartSystem {
     std::vector<double> rhs;        // 2 * nfree, interleaved (u, v)
 };
 
+// Many charts' systems laid out as one block-diagonal batch (see host_batch.cpp).
+struct LscmBatch {
+    int64_t nchart = 0, nrows = 0, nv_total = 0;
+    std::vector<int64_t> row_ptr{0};
+    std::vector<int32_t> cols;
+    std::vector<double> vals;
+    std::vector<double> rhs;
+    std::vector<int64_t> chart_row_off{0};
+    std::vector<int32_t> chart_mesh;   // source mesh of each chart
+    std::vector<int64_t> chart_nv;     // chart vertex count
+    std::vector<int64_t> chart_npins;
+};
+
 void eig3(const double m[9], double w[3], double v[9]);
 void tangent_frame(const double* p3, int64_t n, double* p2);
 void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S);

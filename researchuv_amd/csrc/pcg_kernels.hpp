@@ -9,11 +9,10 @@
 
 #include <cstdint>
 
+#include "dev_common.hpp"
+
 namespace ruv {
 
-constexpr int BLK = 256;          // threads per workgroup of the vector / SpMV kernels
-constexpr int WAVES = BLK / 64;   // wave64
-constexpr int MAXG = 1024;        // cap on workgroups, hence on partials per reduction
 constexpr int TRI_BLK = 1024;     // single-workgroup triangular sweep
 
 enum Done : int32_t { RUNNING = 0, CONVERGED = 1, BREAKDOWN = 2, MAXITER = 3 };
@@ -29,31 +28,6 @@ struct SolveState {
     int32_t done;
     int32_t pad;
 };
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Sum over the workgroup; every thread returns the same value.  sh holds WAVES doubles.
-__device__ __forceinline__ double block_sum_all(double v, double* sh) {
-    v = wave_sum(v);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < WAVES; ++i) s += sh[i];
-    return s;
-}
-
-// Sum of G per-workgroup partials, in a fixed order; identical in every workgroup.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int G, double* sh) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < G; i += BLK) v += part[i];
-    return block_sum_all(v, sh);
-}
 
 template <typename T>
 __global__ __launch_bounds__(BLK) void k_fill0(int64_t n, T* __restrict__ x) {

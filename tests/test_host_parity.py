@@ -85,3 +85,32 @@ def test_heightfield_chart_is_bit_exact(orc, ra_lscm):
     n = 2 * sp_["nfree"]
     assert np.array_equal(so["rhs"][:n], sp_["rhs"])
     assert len(so["vals"]) == 2 * len(sp_["l_vals"])
+
+
+def test_batch_builder_matches_per_chart_systems_and_thread_counts(ra_lscm):
+    """The worker-thread batch builder (exec::par_map semantics: results in input order, identical for any
+    worker count) lays out exactly the per-chart systems."""
+    from researchuv_amd import batch as B
+    from researchuv_amd import meshgen
+
+    sizes = meshgen.patch_sizes(12, vmin=64, vmax=900, seed=4)
+    meshes = [meshgen.heightfield(int(a), int(b), seed=40 + k) for k, (a, b) in enumerate(sizes)]
+    a1 = B.LscmBatchBuilder(meshes, threads=1).arrays(csr=True)
+    a4 = B.LscmBatchBuilder(meshes, threads=4).arrays(csr=True)
+    for k in a1:
+        assert np.array_equal(a1[k], a4[k]), k
+    off = a1["chart_row_off"]
+    c = 0
+    for pos, faces in meshes:
+        pw, fw = ra_lscm.weld(pos, faces, 1e-12)
+        seg = ra_lscm.segment(pw, fw, 180.0)
+        for i in range(len(seg)):
+            s = seg.system(i)
+            r0, r1 = off[c], off[c + 1]
+            assert r1 - r0 == s["nfree"]
+            assert np.array_equal(a1["rhs"][2 * r0:2 * r1], s["rhs"])
+            lo, hi = a1["row_ptr"][r0], a1["row_ptr"][r1]
+            assert np.array_equal(a1["vals"][lo:hi], s["l_vals"])
+            assert np.array_equal(a1["cols"][lo:hi] - r0, s["l_cols"])
+            c += 1
+    assert c == len(off) - 1
