@@ -200,7 +200,14 @@ typedef struct ruv_unfold_opts {
     int32_t mode;       /* ruv_mode                                                                     */
     int32_t min_vars;   /* GPU size gate on 2*nfree: lscm.rs:462 uses 1024; 0 disables it              */
     double tol;         /* 1e-12 reproduces lscm.rs:501 (mode 0); the mixed form targets the true residual */
+    int32_t device_assembly; /* 1: cotan weights, L and rhs computed on the device (bitwise the host ones) */
 } ruv_unfold_opts;
+
+/* The same system as ruv_lscm_system with the cotan weights (lscm.rs:252-274) and the free-block assembly
+ * (lscm.rs:594-666) computed on the device: one thread per edge, one per free row.  sizes (5) as in
+ * ruv_lscm_system; call with null arrays for the sizes.  The result is bitwise the host assembly. */
+int ruv_lscm_system_device(const double* mesh_pos, const ruv_charts* charts, int64_t i, double angle_mix,
+                           int64_t* sizes, double* l_vals, int32_t* l_cols, int32_t* l_ptr, double* rhs);
 
 /* lscm::unfold_chart (lscm.rs:512-727) with SolverBackend::Gpu.  uv_out (nv*2) in chart-local order; stats
  * (optional, 4): iters, err_accum, target, area3d (UnfoldResult, lscm.rs:104-116); info optional.  A chart

@@ -60,6 +60,13 @@ def lib():
     """The loaded library; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own copy of the HIP runtime (same SONAME).  Loading torch first makes this library
+        # bind to that copy, so a process that also uses torch.cuda has exactly one HIP runtime; loading this
+        # library first would leave torch a second runtime that finds no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run `python researchuv_amd/build.py` (no CPU fallback exists)")
         L = C.CDLL(LIB_PATH)

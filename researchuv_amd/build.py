@@ -25,8 +25,8 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature"
              "-munsafe-fp-atomics"]
 HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["pcg.hip", "batch.hip"]
-HOST_SOURCES = ["host_precond.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp", "api.cpp"]
+HIP_SOURCES = ["pcg.hip", "batch.hip", "assemble.hip"]
+HOST_SOURCES = ["host_precond.cpp", "host_amg.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp", "api.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -401,3 +401,22 @@ void ruv_lscm_batch_destroy(ruv_lscm_batch* lb) {
 }
 
 }  // extern "C"
+
+extern "C" int ruv_lscm_system_device(const double* mesh_pos, const ruv_charts* charts, int64_t i, double angle_mix,
+                                      int64_t* sizes, double* l_vals, int32_t* l_cols, int32_t* l_ptr, double* rhs) {
+    if (!mesh_pos || !charts || !sizes || i < 0 || i >= (int64_t)charts->c.charts.size())
+        return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    ruv::ChartSystem S;
+    int rc = ruv::device_assemble(mesh_pos, charts->c.charts[i], angle_mix, &S);
+    if (rc) return rc;
+    sizes[0] = S.nv;
+    sizes[1] = S.nfree;
+    sizes[2] = (int64_t)S.pin_ids.size();
+    sizes[3] = S.l.nnz();
+    sizes[4] = (int64_t)S.edges;
+    if (l_vals) std::copy(S.l.vals.begin(), S.l.vals.end(), l_vals);
+    if (l_cols) std::copy(S.l.cols.begin(), S.l.cols.end(), l_cols);
+    if (l_ptr) std::copy(S.l.ptr.begin(), S.l.ptr.end(), l_ptr);
+    if (rhs) std::copy(S.rhs.begin(), S.rhs.end(), rhs);
+    return RUV_OK;
+}

@@ -295,8 +295,8 @@ void tangent_frame(const double* p3, int64_t n, double* p2) {
     }
 }
 
-// Pins, cotan weights and the free block of one chart (lscm.rs:512-666; default path B = 1 - angle_mix).
-void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S) {
+// Tangent frame, pins and the free map of one chart (lscm.rs:513-542, 582-592).
+void build_system_pins(const double* pos, const Chart& ch, ChartSystem* S) {
     const int64_t nv = (int64_t)ch.vertex_ids.size();
     S->nv = nv;
     S->p3.resize(3 * nv);
@@ -312,7 +312,6 @@ void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSys
     S->pin_ids.clear();
     S->pin_uv.clear();
     std::vector<char> is_free(nv, 1);
-    std::vector<double> pin_at(2 * nv, 0.0);
     for (const auto& [v, t] : pins) {
         const int64_t li = ch.local_of(v);
         if (li < 0 || !is_free[li]) continue;
@@ -320,8 +319,6 @@ void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSys
         S->pin_uv.push_back(t.first);
         S->pin_uv.push_back(t.second);
         is_free[li] = 0;
-        pin_at[2 * li] = t.first;
-        pin_at[2 * li + 1] = t.second;
     }
     S->free_pos.assign(nv, -1);
     S->free_ids.clear();
@@ -330,8 +327,21 @@ void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSys
             S->free_pos[i] = (int32_t)S->free_ids.size();
             S->free_ids.push_back((int32_t)i);
         }
-    const int32_t nfree = (int32_t)S->free_ids.size();
-    S->nfree = nfree;
+    S->nfree = (int32_t)S->free_ids.size();
+}
+
+// Pins, cotan weights and the free block of one chart (lscm.rs:512-666; default path B = 1 - angle_mix).
+void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S) {
+    build_system_pins(pos, ch, S);
+    const int64_t nv = S->nv;
+    const int32_t nfree = S->nfree;
+    std::vector<char> is_free(nv, 0);
+    std::vector<double> pin_at(2 * nv, 0.0);
+    for (int32_t q = 0; q < nfree; ++q) is_free[S->free_ids[q]] = 1;
+    for (size_t q = 0; q < S->pin_ids.size(); ++q) {
+        pin_at[2 * S->pin_ids[q]] = S->pin_uv[2 * q];
+        pin_at[2 * S->pin_ids[q] + 1] = S->pin_uv[2 * q + 1];
+    }
 
     // edge_weights (lscm.rs:252-274): contributions per undirected key, summed in triangle order
     const size_t ntri = ch.tris.size() / 3;
@@ -374,6 +384,7 @@ void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSys
         const uint32_t i = (uint32_t)(key >> 32), j = (uint32_t)(key & 0xffffffffu);
         const V3 d3 = sub(at(S->p3.data(), j), at(S->p3.data(), i));
         const double c = w / std::max(dot(d3, d3), EPS_DEGENERATE);
+        if (c == 0.0) continue;  // an underflowed weight adds exact zeros, which Sparse::add skips
         edges.emplace_back(key, c);
         deg[i + 1]++;
         deg[j + 1]++;
@@ -451,7 +462,12 @@ int64_t Chart::local_of(uint32_t v) const {
 int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, double* uv_out, double* stats,
                  ruv_solve_info* info_out) {
     ChartSystem S;
-    build_system(pos, ch, o.angle_mix, &S);
+    if (o.device_assembly) {
+        int rc = device_assemble(pos, ch, o.angle_mix, &S);
+        if (rc) return rc;
+    } else {
+        build_system(pos, ch, o.angle_mix, &S);
+    }
     const int64_t nv = S.nv;
     std::vector<double> uv(S.p2);
     double err_accum = 1.0, step = 1.0, prev_err = INFINITY;

@@ -37,7 +37,10 @@ struct LscmBatch {
 
 void eig3(const double m[9], double w[3], double v[9]);
 void tangent_frame(const double* p3, int64_t n, double* p2);
+void build_system_pins(const double* pos, const Chart& ch, ChartSystem* S);
 void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S);
+// the same system with L and the rhs assembled on the device (assemble.hip)
+int device_assemble(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S);
 int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, double* uv_out, double* stats,
                  ruv_solve_info* info);
 

@@ -25,6 +25,7 @@
 #include <string>
 #include <vector>
 
+#include "amg_kernels.hpp"
 #include "ruv_internal.hpp"
 #include "pcg_kernels.hpp"
 
@@ -97,6 +98,64 @@ int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
     return RUV_OK;
 }
 
+// One AMG level on the device (amg.rs:74-102, V-cycle buffers).
+template <typename T>
+struct DevAmgLevel {
+    int n = 0, nc = 0;
+    int *a_ptr = nullptr, *a_cols = nullptr, *p_ptr = nullptr, *p_cols = nullptr, *r_ptr = nullptr, *r_cols = nullptr;
+    T *a_vals = nullptr, *dinv = nullptr, *p_vals = nullptr, *r_vals = nullptr;
+    T *rin = nullptr, *t = nullptr, *e1 = nullptr, *e = nullptr;  // level input residual, temps, output
+    double* dense_inv = nullptr;                                  // coarsest: FP64 inverse
+    bool coarsest = false, diag_only = false;
+};
+
+template <typename T>
+struct DevAmg {
+    std::vector<DevAmgLevel<T>> lv;
+    T omega = T(2.0 / 3.0);
+};
+
+template <typename T>
+int upload_amg(DevArena& m, const std::vector<AmgHostLevel>& H, int R, hipStream_t s, DevAmg<T>* d) {
+    std::vector<std::vector<T>> keep;  // staging that must outlive the async copies
+    auto up = [&](T** dst, const std::vector<double>& src) {
+        keep.emplace_back(src.begin(), src.end());
+        return m.upload(dst, keep.back().data(), keep.back().size(), s);
+    };
+    d->lv.resize(H.size());
+    for (size_t l = 0; l < H.size(); ++l) {
+        const AmgHostLevel& h = H[l];
+        DevAmgLevel<T>& L = d->lv[l];
+        L.n = h.op.n;
+        L.nc = h.n_coarse;
+        L.coarsest = (l + 1 == H.size());
+        L.diag_only = h.diag_only;
+        std::vector<double> dinv(h.op.n);
+        for (int32_t i = 0; i < h.op.n; ++i) dinv[i] = 1.0 / h.op.get(i, i);
+        HIPCHK(m.upload(&L.a_ptr, h.op.ptr.data(), h.op.ptr.size(), s));
+        HIPCHK(m.upload(&L.a_cols, h.op.cols.data(), h.op.cols.size(), s));
+        HIPCHK(up(&L.a_vals, h.op.vals));
+        HIPCHK(up(&L.dinv, dinv));
+        const size_t N = (size_t)h.op.n * R;
+        if (l > 0) HIPCHK(m.alloc(&L.rin, N));
+        HIPCHK(m.alloc(&L.t, N));
+        HIPCHK(m.alloc(&L.e1, N));
+        HIPCHK(m.alloc(&L.e, N));
+        if (!L.coarsest) {
+            HIPCHK(m.upload(&L.p_ptr, h.p.ptr.data(), h.p.ptr.size(), s));
+            HIPCHK(m.upload(&L.p_cols, h.p.cols.data(), h.p.cols.size(), s));
+            HIPCHK(up(&L.p_vals, h.p.vals));
+            HIPCHK(m.upload(&L.r_ptr, h.r.ptr.data(), h.r.ptr.size(), s));
+            HIPCHK(m.upload(&L.r_cols, h.r.cols.data(), h.r.cols.size(), s));
+            HIPCHK(up(&L.r_vals, h.r.vals));
+        } else if (h.has_factor) {
+            HIPCHK(m.upload(&L.dense_inv, h.dense_chol.data(), h.dense_chol.size(), s));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------------
@@ -116,6 +175,9 @@ struct PlanImpl {
     DevTri<double> tri64;
     DevTri<float> tri32;
     bool has_tri = false;
+    DevAmg<double> amg64;
+    DevAmg<float> amg32;
+    int amg_levels = 0;
     // permutation round trip for the colour-ordered IC(0) (precond.rs:613-630)
     int *to_orig = nullptr, *to_perm = nullptr;
     bool permuted = false;
@@ -194,6 +256,13 @@ int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
         int rc = (mode == RUV_MODE_MIXED) ? upload_tri(P.mem, f, s, &P.tri32) : upload_tri(P.mem, f, s, &P.tri64);
         if (rc) return rc;
         P.has_tri = true;
+    } else if (precond == RUV_PC_AMG || precond == RUV_PC_AMGSA) {
+        std::vector<AmgHostLevel> H;
+        std::string why;
+        if (!amg_hierarchy(a, precond == RUV_PC_AMGSA, &H, &why)) return fail(RUV_ERR_PRECOND, "amg: " + why);
+        int rc = (mode == RUV_MODE_MIXED) ? upload_amg(P.mem, H, R, s, &P.amg32) : upload_amg(P.mem, H, R, s, &P.amg64);
+        if (rc) return rc;
+        P.amg_levels = (int)H.size();
     } else if (precond != RUV_PC_NONE) {
         return fail(RUV_ERR_INVALID, "precond " + std::to_string(precond) + " is not available in this build");
     }
@@ -250,6 +319,51 @@ void apply_tri(PlanImpl& P, const DevTri<T>& f, const T* r, T* z, T* y, T* t, co
     if (P.permuted) hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_perm, (const T*)t, out, st);
 }
 
+template <typename T> DevTri<T>& tri_of(PlanImpl& P);
+template <> DevTri<double>& tri_of<double>(PlanImpl& P) { return P.tri64; }
+template <> DevTri<float>& tri_of<float>(PlanImpl& P) { return P.tri32; }
+template <typename T> DevAmg<T>& amg_of(PlanImpl& P);
+template <> DevAmg<double>& amg_of<double>(PlanImpl& P) { return P.amg64; }
+template <> DevAmg<float>& amg_of<float>(PlanImpl& P) { return P.amg32; }
+
+int grid_rows(int n) { return grid_for(n); }
+
+// One V(1,1) cycle from level l: e = M_l^-1 r (see amg_kernels.hpp).
+template <typename T, int R>
+void amg_cycle(PlanImpl& P, DevAmg<T>& A, size_t l, const T* r, T* e, const int32_t* done) {
+    DevAmgLevel<T>& L = A.lv[l];
+    hipStream_t s = P.stream;
+    const int g = grid_rows(L.n);
+    if (L.coarsest) {
+        if (L.dense_inv) {
+            hipLaunchKernelGGL((k_amg_dense<T, R>), dim3((L.n + WAVES - 1) / WAVES), dim3(BLK), 0, s, L.n,
+                               (const double*)L.dense_inv, r, e, done);
+        } else {
+            hipLaunchKernelGGL((k_amg_diag<T, R>), dim3(g), dim3(BLK), 0, s, L.n, (const T*)L.dinv, r, e, done);
+        }
+        return;
+    }
+    DevAmgLevel<T>& C = A.lv[l + 1];
+    hipLaunchKernelGGL((k_amg_pre<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals,
+                       (const T*)L.dinv, A.omega, r, L.t, done);
+    hipLaunchKernelGGL((k_amg_spmv<T, R>), dim3(grid_rows(L.nc)), dim3(BLK), 0, s, L.nc, L.r_ptr, L.r_cols,
+                       (const T*)L.r_vals, (const T*)L.t, C.rin, done);
+    amg_cycle<T, R>(P, A, l + 1, C.rin, C.e, done);
+    hipLaunchKernelGGL((k_amg_prolong<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.p_ptr, L.p_cols, (const T*)L.p_vals,
+                       (const T*)C.e, (const T*)L.dinv, A.omega, r, L.e1, done);
+    hipLaunchKernelGGL((k_amg_post<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals,
+                       (const T*)L.dinv, A.omega, r, (const T*)L.e1, e, done);
+}
+
+// z = M^-1 r for the preconditioners that are not element-wise (IC(0) variants, AMG)
+template <typename T, int R>
+void apply_external(PlanImpl& P, const T* r, T* z, T* y, T* t, const SolveState* st) {
+    if (P.precond == RUV_PC_AMG || P.precond == RUV_PC_AMGSA)
+        amg_cycle<T, R>(P, amg_of<T>(P), 0, r, z, st ? &st->done : nullptr);
+    else
+        apply_tri<T, R>(P, tri_of<T>(P), r, z, y, t, st);
+}
+
 // PCG on the device.  On entry r holds the right-hand side (x0 = 0), x is overwritten.
 // tol_abs_from: tol relative to ||r0|| (tol absolute when ||r0|| <= 1e-300), as solver.rs:465-468.
 template <typename T, int R>
@@ -265,8 +379,8 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* 
     // z0 = M^-1 r0; p = z0; part_c = r.z
     if (pc == RUV_PC_JACOBI) {
         hipLaunchKernelGGL((k_jacobi<T, R>), dim3(G), dim3(BLK), 0, s, n, dinv, (const T*)r, z);
-    } else if (pc == RUV_PC_IC0 || pc == RUV_PC_IC0COLOR) {
-        apply_tri<T, R>(P, tri, r, z, y, t, nullptr);
+    } else if (pc != RUV_PC_NONE) {
+        apply_external<T, R>(P, r, z, y, t, nullptr);
     }
     const T* zz = (pc == RUV_PC_NONE) ? (const T*)r : (const T*)z;
     hipLaunchKernelGGL((k_copy<T>), dim3(G), dim3(BLK), 0, s, (int64_t)n * R, zz, p);
@@ -292,7 +406,7 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* 
             } else {
                 hipLaunchKernelGGL((k_update<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
                                    dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
-                apply_tri<T, R>(P, tri, r, z, y, t, P.st);
+                apply_external<T, R>(P, r, z, y, t, P.st);
                 hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, (const T*)z, P.part_c,
                                    (const SolveState*)P.st);
                 hipLaunchKernelGGL((k_pupdate<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)z, p,

@@ -238,3 +238,51 @@ def test_plan_device_buffers_roundtrip(gpu, orc):
         assert info.true_rel_resid <= 1e-8
         assert uv_rel_err(uv_from(s, x.cpu().numpy()), uv_cpu) < 1e-6
     plan.close()
+
+
+# ------------------------------------------------------------------ AMG (amg.rs, solver.rs:937-1102)
+def test_amg_beats_jacobi_on_the_poisson_system(gpu):
+    import researchuv_amd as ra
+
+    say("amg vs jacobi poisson 400^2")
+    v, c, p, b = meshes.poisson_2d(400)
+    a = csr(v, c, p)
+    n = a.rows()
+    xj, itj = gpu.cg_solve_precond(a, b, 100_000, 1e-10, ra.Precond.JACOBI)
+    xa, ita = gpu.cg_solve_precond(a, b, 100_000, 1e-10, ra.Precond.AMG)
+    xs, its = gpu.cg_solve_precond(a, b, 100_000, 1e-10, ra.Precond.AMG_SA)
+    for x in (xa, xs):
+        for i in (0, n // 2, n - 1):
+            assert abs(x[i] - xj[i]) < 1e-4
+    say(f"  jacobi={itj} amg={ita} amgsa={its}")
+    assert ita < itj and its < itj
+
+
+def test_smoothed_amg_on_the_elongated_system(gpu):
+    import researchuv_amd as ra
+
+    say("amgsa on 1-D laplacian n=200000")
+    v, c, p, b = meshes.laplacian_1d(200_000)
+    a = csr(v, c, p)
+    xs, its = gpu.cg_solve_precond(a, b, 200_000, 1e-10, ra.Precond.AMG_SA)
+    assert np.abs(xs - 1.0).max() < 1e-5
+    say(f"  amgsa iterations {its}")
+    assert its < 500
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_lscm_grid64_amgsa_matches_cpu_lu(gpu, orc, mixed):
+    import researchuv_amd as ra
+
+    say(f"lscm grid(64,64) amgsa mixed={mixed}")
+    ch, s, uv_cpu = lscm_case(orc, orc.grid(64, 64))
+    a = csr(s["vals"], s["cols"], s["row_ptr"])
+    n = a.rows()
+    if mixed:
+        r = gpu.cg_solve_mixed(a, s["rhs"][:n], 100_000, 1e-9, ra.Precond.AMG_SA)
+    else:
+        r = gpu.cg_solve_precond(a, s["rhs"][:n], 1024 * 16, 1e-12, ra.Precond.AMG_SA)
+    assert r is not None, gpu.last_error
+    err = uv_rel_err(uv_from(s, r[0]), uv_cpu)
+    say(f"  uv_err={err:.3e}")
+    assert err < 1e-6
