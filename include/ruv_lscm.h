@@ -203,6 +203,20 @@ typedef struct ruv_unfold_opts {
     int32_t device_assembly; /* 1: cotan weights, L and rhs computed on the device (bitwise the host ones) */
 } ruv_unfold_opts;
 
+/* ---------------------------------------------------------------------------------------------------------
+ * Differentiable form (no reference counterpart).  One chart, chart-local: p3 (nv*3) positions, tris
+ * (ntri*3), frozen pins (pin_ids, pin_uv 2 per pin); default path (angle_mix = 1).  Forward: uv (nv*2) and the
+ * solver's quadratic energy E = sum over assembled edges of c_e |uv_i - uv_j|^2 (c_e the cotan weight of
+ * lscm.rs:252-274,610).  VJP: grad_p3 (nv*3) = (d uv / d p)^T g_uv + energy_weight * (dE/dp at fixed uv),
+ * computed with one extra PCG on the same SPD matrix (L^T = L) and a device gather through the assembly.
+ * ------------------------------------------------------------------------------------------------------- */
+int ruv_lscm_forward(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                     const double* pin_uv, int64_t npins, int32_t precond, int32_t mode, double tol, double* uv_out,
+                     double* energy_out, ruv_solve_info* info);
+int ruv_lscm_vjp(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                 const double* pin_uv, int64_t npins, const double* uv, const double* g_uv, double energy_weight,
+                 int32_t precond, int32_t mode, double tol, double* grad_p3, ruv_solve_info* info);
+
 /* The same system as ruv_lscm_system with the cotan weights (lscm.rs:252-274) and the free-block assembly
  * (lscm.rs:594-666) computed on the device: one thread per edge, one per free row.  sizes (5) as in
  * ruv_lscm_system; call with null arrays for the sizes.  The result is bitwise the host assembly. */

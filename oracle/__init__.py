@@ -287,3 +287,23 @@ def coloring(cols, row_ptr):
     lib().orc_coloring(n, _p(np.ascontiguousarray(cols, np.int32)), _p(np.ascontiguousarray(row_ptr, np.int32)),
                        _p(out))
     return out
+
+
+def lscm_loss_frozen(p3, tris, pin_ids, pin_uv, target):
+    """E + |uv - target|^2 with frozen pins, solved by the sparse LU -> (loss, uv) (None when the LU fails)."""
+    L = lib()
+    if not getattr(L, "_loss_ready", False):
+        P = C.c_void_p
+        L.orc_lscm_loss_frozen.restype = C.c_double
+        L.orc_lscm_loss_frozen.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int, P, P, P]
+        L._loss_ready = True
+    p3 = np.ascontiguousarray(p3, np.float64)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    pin_ids = np.ascontiguousarray(pin_ids, np.int32)
+    pin_uv = np.ascontiguousarray(pin_uv, np.float64)
+    target = np.ascontiguousarray(target, np.float64)
+    uv = np.zeros((len(p3), 2))
+    ok = C.c_int(0)
+    loss = L.orc_lscm_loss_frozen(_p(p3), len(p3), _p(tris), len(tris), _p(pin_ids), _p(pin_uv), len(pin_ids),
+                                  _p(target), _p(uv), C.byref(ok))
+    return (loss, uv) if ok.value else None

@@ -1116,6 +1116,77 @@ static void chartsys_build(const double* pos, const orc_chart* ch, double angle_
     free(ew); free(isfree); free(pin_lookup);
 }
 
+/* Differentiable-form reference: the free block assembled from positions p3 with FROZEN pins (given ids and
+ * targets, angle_mix = 1 so the rhs holds pin terms only), solved by the sparse LU (sparse.rs:219-265).
+ * Writes uv (nv*2) and returns the loss E + sum_v |uv_v - target_v|^2 with E = sum over assembled edges of
+ * c_e |uv_i - uv_j|^2 (c_e of lscm.rs:252-274,610).  *ok = 0 when the LU fails. */
+double orc_lscm_loss_frozen(const double* p3, int nv, const uint32_t* tris, int ntri, const int32_t* pin_ids,
+                            const double* pin_uv, int npins, const double* target, double* uv, int* ok) {
+    uint8_t* isfree = (uint8_t*)malloc((size_t)(nv ? nv : 1));
+    memset(isfree, 1, (size_t)(nv ? nv : 1));
+    double* pin_lookup = (double*)calloc(2 * (size_t)(nv ? nv : 1), sizeof(double));
+    for (int q = 0; q < npins; ++q) {
+        isfree[pin_ids[q]] = 0;
+        pin_lookup[2 * pin_ids[q]] = pin_uv[2 * q];
+        pin_lookup[2 * pin_ids[q] + 1] = pin_uv[2 * q + 1];
+    }
+    int32_t* free_pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nv ? nv : 1));
+    int nfree = 0;
+    for (int i = 0; i < nv; ++i) free_pos[i] = isfree[i] ? nfree++ : -1;
+    ewrec* ew;
+    size_t ne = edge_weights(p3, tris, ntri, &ew);
+    int32_t nvar = 2 * nfree + 1, aux = 2 * nfree;
+    double* rhs = (double*)calloc((size_t)nvar, sizeof(double));
+    double* cval = (double*)calloc(ne ? ne : 1, sizeof(double));
+    sparse m;
+    sparse_init(&m, nvar);
+    for (size_t e = 0; e < ne; ++e) {
+        uint32_t i = ew[e].i, j = ew[e].j;
+        if (ew[e].w <= EPS_DEGENERATE) continue;
+        v3 d3 = v3sub(P3(p3, j), P3(p3, i));
+        double dd = v3dot(d3, d3);
+        double c = ew[e].w / (dd > EPS_DEGENERATE ? dd : EPS_DEGENERATE);
+        cval[e] = c;
+        for (int side = 0; side < 2; ++side) {
+            uint32_t aa = side ? j : i, bb = side ? i : j;
+            if (!isfree[aa]) continue;
+            int32_t ra = 2 * free_pos[aa];
+            srow_add(&m.rows[ra], ra, c);
+            srow_add(&m.rows[ra + 1], ra + 1, c);
+            if (isfree[bb]) {
+                srow_add(&m.rows[ra], 2 * free_pos[bb], -c);
+                srow_add(&m.rows[ra + 1], 2 * free_pos[bb] + 1, -c);
+            } else {
+                rhs[ra] += c * pin_lookup[2 * bb];
+                rhs[ra + 1] += c * pin_lookup[2 * bb + 1];
+            }
+        }
+    }
+    srow_add(&m.rows[aux], aux, 1.0);
+    rhs[aux] = 1.0;
+    double* sol = (double*)malloc(sizeof(double) * (size_t)nvar);
+    *ok = sparse_solve(&m, rhs, sol) == 0;
+    double loss = 0.0;
+    if (*ok) {
+        for (int v = 0; v < nv; ++v) {
+            uv[2 * v] = isfree[v] ? sol[2 * free_pos[v]] : pin_lookup[2 * v];
+            uv[2 * v + 1] = isfree[v] ? sol[2 * free_pos[v] + 1] : pin_lookup[2 * v + 1];
+        }
+        for (size_t e = 0; e < ne; ++e) {
+            if (cval[e] == 0.0) continue;
+            double du = uv[2 * ew[e].i] - uv[2 * ew[e].j], dv = uv[2 * ew[e].i + 1] - uv[2 * ew[e].j + 1];
+            loss += cval[e] * (du * du + dv * dv);
+        }
+        for (int v = 0; v < nv; ++v) {
+            double du = uv[2 * v] - target[2 * v], dv = uv[2 * v + 1] - target[2 * v + 1];
+            loss += du * du + dv * dv;
+        }
+    }
+    sparse_free(&m);
+    free(sol); free(rhs); free(cval); free(ew); free(free_pos); free(isfree); free(pin_lookup);
+    return loss;
+}
+
 /* Export the chart system: sizes[0]=nv, [1]=nfree, [2]=npins, [3]=nnz of the leading 2*nfree block.
  * Call with NULL arrays first to get sizes. CSR follows Sparse::to_csr(2*nfree) (sparse.rs:69-84). */
 int orc_chart_system(const orc_charts* cc, int ci, const double* pos, double angle_mix, double mix_w,

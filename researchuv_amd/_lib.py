@@ -50,6 +50,7 @@ class UnfoldOpts(C.Structure):
         ("mode", C.c_int32),
         ("min_vars", C.c_int32),
         ("tol", C.c_double),
+        ("device_assembly", C.c_int32),
     ]
 
 

@@ -420,3 +420,32 @@ extern "C" int ruv_lscm_system_device(const double* mesh_pos, const ruv_charts* 
     if (rhs) std::copy(S.rhs.begin(), S.rhs.end(), rhs);
     return RUV_OK;
 }
+
+namespace ruv {
+int lscm_forward(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                 const double* pin_uv, int64_t npins, int precond, int mode, double tol, double* uv_out,
+                 double* energy_out, ruv_solve_info* info);
+int lscm_vjp(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+             const double* pin_uv, int64_t npins, const double* uv, const double* g_uv, double a_energy, int precond,
+             int mode, double tol, double* grad_p3, ruv_solve_info* info);
+}  // namespace ruv
+
+extern "C" int ruv_lscm_forward(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri,
+                                const int32_t* pin_ids, const double* pin_uv, int64_t npins, int32_t precond,
+                                int32_t mode, double tol, double* uv_out, double* energy_out, ruv_solve_info* info) {
+    if (!p3 || nv <= 0 || !tris || ntri < 0 || (npins && (!pin_ids || !pin_uv)) || !uv_out)
+        return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    if (precond < RUV_PC_NONE || precond > RUV_PC_AMGSA) return ruv::fail(RUV_ERR_INVALID, "unknown precond");
+    return ruv::lscm_forward(p3, nv, tris, ntri, pin_ids, pin_uv, npins, precond, mode, tol, uv_out, energy_out, info);
+}
+
+extern "C" int ruv_lscm_vjp(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                            const double* pin_uv, int64_t npins, const double* uv, const double* g_uv,
+                            double energy_weight, int32_t precond, int32_t mode, double tol, double* grad_p3,
+                            ruv_solve_info* info) {
+    if (!p3 || nv <= 0 || !tris || ntri < 0 || (npins && (!pin_ids || !pin_uv)) || !uv || !g_uv || !grad_p3)
+        return ruv::fail(RUV_ERR_INVALID, "bad argument");
+    if (precond < RUV_PC_NONE || precond > RUV_PC_AMGSA) return ruv::fail(RUV_ERR_INVALID, "unknown precond");
+    return ruv::lscm_vjp(p3, nv, tris, ntri, pin_ids, pin_uv, npins, uv, g_uv, energy_weight, precond, mode, tol,
+                         grad_p3, info);
+}

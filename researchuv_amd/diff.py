@@ -1,0 +1,66 @@
+"""Differentiable LSCM solve of one chart (positions -> uv) and its vector-Jacobian product.
+
+Pins are frozen constants of the positions; the default path (angle_mix = 1) is assumed.  The reverse pass
+is one extra PCG solve with the same SPD matrix followed by a device gather through the cotan assembly
+(include/ruv_lscm.h: ruv_lscm_forward / ruv_lscm_vjp).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .solver import Precond
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _setup(L):
+    if getattr(L, "_ruv_diff_ready", False):
+        return L
+    P = C.c_void_p
+    L.ruv_lscm_forward.restype = C.c_int
+    L.ruv_lscm_forward.argtypes = [P, C.c_int64, P, C.c_int64, P, P, C.c_int64, C.c_int32, C.c_int32, C.c_double, P,
+                                   P, P]
+    L.ruv_lscm_vjp.restype = C.c_int
+    L.ruv_lscm_vjp.argtypes = [P, C.c_int64, P, C.c_int64, P, P, C.c_int64, P, P, C.c_double, C.c_int32, C.c_int32,
+                               C.c_double, P, P]
+    L._ruv_diff_ready = True
+    return L
+
+
+class ChartSolve:
+    """One chart's topology and frozen pins; forward / vjp take positions (nv, 3)."""
+
+    def __init__(self, tris, pin_ids, pin_uv, precond=Precond.JACOBI, mixed=False, tol=1e-12):
+        self.tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+        self.pin_ids = np.ascontiguousarray(pin_ids, np.int32)
+        self.pin_uv = np.ascontiguousarray(pin_uv, np.float64).reshape(-1, 2)
+        self.precond = int(precond)
+        self.mode = 1 if mixed else 0
+        self.tol = tol
+        self.L = _setup(_lib.lib())
+
+    def forward(self, p3):
+        p3 = np.ascontiguousarray(p3, np.float64).reshape(-1, 3)
+        uv = np.zeros((len(p3), 2))
+        e = C.c_double(0)
+        info = _lib.SolveInfo()
+        _lib.check(self.L.ruv_lscm_forward(_p(p3), len(p3), _p(self.tris), len(self.tris), _p(self.pin_ids),
+                                           _p(self.pin_uv), len(self.pin_ids), self.precond, self.mode, self.tol,
+                                           _p(uv), C.byref(e), C.byref(info)))
+        return uv, e.value
+
+    def vjp(self, p3, uv, g_uv, energy_weight=0.0):
+        p3 = np.ascontiguousarray(p3, np.float64).reshape(-1, 3)
+        uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+        g = np.ascontiguousarray(g_uv, np.float64).reshape(-1, 2)
+        grad = np.zeros_like(p3)
+        info = _lib.SolveInfo()
+        _lib.check(self.L.ruv_lscm_vjp(_p(p3), len(p3), _p(self.tris), len(self.tris), _p(self.pin_ids),
+                                       _p(self.pin_uv), len(self.pin_ids), _p(uv), _p(g), energy_weight,
+                                       self.precond, self.mode, self.tol, _p(grad), C.byref(info)))
+        return grad

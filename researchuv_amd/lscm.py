@@ -42,6 +42,8 @@ def _setup(L):
     L.ruv_charts_destroy.argtypes = [P]
     L.ruv_lscm_system.restype = C.c_int
     L.ruv_lscm_system.argtypes = [P, P, C.c_int64, C.c_double, P, P, P, P, P, P, P, P, P]
+    L.ruv_lscm_system_device.restype = C.c_int
+    L.ruv_lscm_system_device.argtypes = [P, P, C.c_int64, C.c_double, P, P, P, P, P]
     L.ruv_unfold_chart.restype = C.c_int
     L.ruv_unfold_chart.argtypes = [P, P, C.c_int64, P, P, P, P]
     L._ruv_lscm_ready = True
@@ -73,10 +75,11 @@ class UnfoldOptions:
     mixed: bool = False
     min_vars: int = 1024  # GPU_MIN_VARS (lscm.rs:462)
     tol: float = 1e-12    # lscm.rs:501
+    device_assembly: bool = True
 
     def c_opts(self):
         return _lib.UnfoldOpts(self.max_iter, self.angle_mix, self.mix_w, int(self.precond), 1 if self.mixed else 0,
-                               self.min_vars, self.tol)
+                               self.min_vars, self.tol, 1 if self.device_assembly else 0)
 
 
 def weld(positions, faces, tol=1e-12):
@@ -166,6 +169,18 @@ class Segmentation:
         _lib.check(L.ruv_lscm_system(_p(self.positions), self.h, i, angle_mix, _p(sz), _p(out["p2"]),
                                      _p(out["pin_ids"]), _p(out["pin_uv"]), _p(out["free_ids"]), _p(out["l_vals"]),
                                      _p(out["l_cols"]), _p(out["l_ptr"]), _p(out["rhs"])))
+        return out
+
+    def system_device(self, i, angle_mix=1.0):
+        """L and rhs of chart i assembled on the device (bitwise the host assembly)."""
+        L = lib()
+        sz = np.zeros(5, np.int64)
+        _lib.check(L.ruv_lscm_system_device(_p(self.positions), self.h, i, angle_mix, _p(sz), None, None, None, None))
+        nv, nfree, npins, nnz, _ = (int(x) for x in sz)
+        out = dict(nv=nv, nfree=nfree, edges=int(sz[4]), l_vals=np.zeros(nnz), l_cols=np.zeros(nnz, np.int32),
+                   l_ptr=np.zeros(nfree + 1, np.int32), rhs=np.zeros(2 * nfree))
+        _lib.check(L.ruv_lscm_system_device(_p(self.positions), self.h, i, angle_mix, _p(sz), _p(out["l_vals"]),
+                                            _p(out["l_cols"]), _p(out["l_ptr"]), _p(out["rhs"])))
         return out
 
     def unfold_chart(self, i, opts: Optional[UnfoldOptions] = None):
