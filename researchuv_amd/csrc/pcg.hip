@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -192,9 +193,12 @@ struct PlanImpl {
     double* h_part = nullptr;
     double inner_tol = 1e-4;
     int max_refine = 40;
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
 
     ~PlanImpl() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (h_st) (void)hipHostFree(h_st);
         if (h_part) (void)hipHostFree(h_part);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -206,6 +210,8 @@ struct PlanImpl {
 namespace {
 
 int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
+    const char* ng = std::getenv("RUV_NO_GRAPH");  // direct launches instead of graph replay (A/B checks)
+    P.use_graph = !(ng && ng[0] == '1');
     P.R = R;
     P.n = a.n;
     P.nnz = a.nnz();
@@ -387,10 +393,8 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* 
     hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, zz, P.part_c, (const SolveState*)nullptr);
     HIPCHK(hipGetLastError());
 
-    uint64_t launched = 0;
-    int chunk = 4;
-    for (;;) {
-        for (int it = 0; it < chunk; ++it) {
+    auto body = [&](int iters) {
+        for (int it = 0; it < iters; ++it) {
             hipLaunchKernelGGL((k_spmv_pq<T, R>), dim3(G), dim3(BLK), 0, s, n, P.ptr, P.cols, vals, (const T*)p, q,
                                P.part_a, (const double*)P.part_c, G, P.st);
             if (pc == RUV_PC_JACOBI) {
@@ -413,7 +417,39 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* 
                                    (const double*)P.part_b, (const double*)P.part_c, G, P.st);
             }
         }
-        launched += chunk;
+    };
+    // A block of GRAPH_ITERS iterations is captured once per plan into a HIP graph and replayed: the
+    // kernel arguments (device pointers, the state block) never change between solves of a plan.  The graph
+    // holds kernel nodes only (no memset / memcpy nodes).
+    hipGraphExec_t gexec = nullptr;
+    if (P.use_graph) {
+        if (!P.gexec) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            body(GRAPH_ITERS);
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (e != hipSuccess) return fail(RUV_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&P.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(RUV_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        gexec = P.gexec;
+    }
+    // Iterations in flight between polls: start small, then aim at the predicted remainder from the decay of
+    // the recursive residual so that the post-convergence overshoot (early-exiting kernels) stays small.
+    uint64_t launched = 0;
+    uint64_t next = gexec ? GRAPH_ITERS : 4;
+    double prev_rr = -1.0;
+    uint64_t prev_it = 0;
+    for (;;) {
+        if (gexec) {
+            const uint64_t reps = (next + GRAPH_ITERS - 1) / GRAPH_ITERS;
+            for (uint64_t k = 0; k < reps; ++k) HIPCHK(hipGraphLaunch(gexec, s));
+            launched += reps * GRAPH_ITERS;
+        } else {
+            body((int)next);
+            launched += next;
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(P.h_st, P.st, sizeof(SolveState), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -421,7 +457,17 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* 
         if (launched > max_iter + 1) {  // cannot happen: the device stops at max_iter
             return fail(RUV_ERR_NOT_CONVERGED, "pcg: iteration accounting overrun");
         }
-        chunk = std::min(chunk * 2, 64);
+        const double rr = P.h_st->rr, tol2 = P.h_st->tol_abs * P.h_st->tol_abs;
+        const uint64_t it = P.h_st->iters;
+        uint64_t want = std::min<uint64_t>(2 * next, 256);
+        if (prev_rr > 0.0 && rr > 0.0 && rr < prev_rr && it > prev_it && tol2 > 0.0) {
+            const double rate = std::log(rr / prev_rr) / (double)(it - prev_it);  // < 0
+            const double need = std::log(tol2 / rr) / rate;
+            if (std::isfinite(need) && need > 0.0) want = (uint64_t)(need * 1.05) + 2;
+        }
+        prev_rr = rr;
+        prev_it = it;
+        next = std::max<uint64_t>(4, std::min<uint64_t>(want, std::max<uint64_t>(8, 2 * it)));
     }
     *iters_out = P.h_st->iters;
     *done_out = P.h_st->done;

@@ -14,6 +14,7 @@
 namespace ruv {
 
 constexpr int TRI_BLK = 1024;     // single-workgroup triangular sweep
+constexpr int GRAPH_ITERS = 8;    // PCG iterations per captured HIP graph
 
 enum Done : int32_t { RUNNING = 0, CONVERGED = 1, BREAKDOWN = 2, MAXITER = 3 };
 

@@ -286,3 +286,34 @@ def test_lscm_grid64_amgsa_matches_cpu_lu(gpu, orc, mixed):
     err = uv_rel_err(uv_from(s, r[0]), uv_cpu)
     say(f"  uv_err={err:.3e}")
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("pc", ["JACOBI", "AMG_SA", "IC0"])
+def test_graph_replay_matches_direct_launches(gpu, orc, pc, monkeypatch):
+    """The PCG iterations replay a captured HIP graph; every replay (several solves on one plan, with eager
+    work and synchronisations in between) must give bitwise the result of direct launches."""
+    import researchuv_amd as ra
+
+    say(f"graph vs direct {pc}")
+    _, s, _ = lscm_case(orc, orc.grid(48, 48))
+    n = len(s["row_ptr"]) - 1
+    lv, lc, lp = [], [], [0]
+    for i in range(0, n, 2):
+        for k in range(s["row_ptr"][i], s["row_ptr"][i + 1]):
+            lc.append(s["cols"][k] // 2)
+            lv.append(s["vals"][k])
+        lp.append(len(lv))
+    Lm = csr(lv, lc, lp)
+    b = s["rhs"][:n]
+    monkeypatch.setenv("RUV_NO_GRAPH", "1")
+    direct = ra.Plan(Lm, 2, getattr(ra.Precond, pc), mixed=True)
+    monkeypatch.delenv("RUV_NO_GRAPH")
+    graph = ra.Plan(Lm, 2, getattr(ra.Precond, pc), mixed=True)
+    x0, i0 = direct.solve(b, 100_000, 1e-10)
+    for rep in range(3):
+        x1, i1 = graph.solve(b * (1.0 + rep), 100_000, 1e-10)
+        xd, idd = direct.solve(b * (1.0 + rep), 100_000, 1e-10)
+        assert np.array_equal(x1, xd), rep
+        assert i1.iters == idd.iters
+    direct.close()
+    graph.close()
