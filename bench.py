@@ -1,16 +1,21 @@
 #!/usr/bin/env python3
 """LSCM solve throughput on one MI355X.
 
-Workload (default, `chart512`): the single-large-chart configuration -- a 512 x 512 vertex height field over
-the unit square (z = sum of 8 sinusoids, amplitude 0.1, seeded frequencies 1-16, xy jitter +-0.2 spacing),
-welded and segmented at 180 degrees into one chart, pinned on its border by the reference's border pass.
-One step = one full solve of the chart's LSCM system (both u and v) to a true FP64 relative residual
-<= 1e-8, with L, the right-hand side and the solution resident in HBM.
+Default workload (`batch`): 4096 open height-field patch charts, vertex counts log-uniform in [64, 65536],
+aspect ratio uniform in [1, 8], z = sum of 8 seeded sinusoids (amplitude 0.1, frequencies 1-16), xy jitter
++-0.2 spacing; each patch is welded, segmented at 180 degrees into one chart and pinned on its border by the
+reference's border pass, and all charts form one batch.  One step = solving every chart's LSCM system (u and v)
+to a true FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 per chart, with the matrix, right-hand sides and
+solution resident in HBM.
 
-value = LSCM unknowns (2 * free vertices) solved per second by the mixed-precision solve (FP32 Jacobi-PCG
-inside FP64 iterative refinement).  The plain FP64 Jacobi-PCG on the same system is timed beside it.
-roofline = the FP32 SpMV kernel the PCG iterations launch, timed live with HIP events on the solver's
-stream; cpu_baseline = the CPU oracle's sparse-LU unfold (the reference's CPU path) on a bounded sample.
+value = LSCM unknowns (2 x free vertices, summed over the charts) solved per second by the mixed-precision
+batched solve (FP32 Jacobi-PCG inside per-chart FP64 iterative refinement).  Beside it: the same batch in plain
+FP64, the plain FP64 drop-in called once per chart as the reference's driver does (on an evenly spaced sample
+of the charts), the UV and LSCM-energy error of the mixed solution against a CPU FP64 sparse LU (SuperLU) on a
+sample of charts, the step kernel's roofline (timed with HIP events on the solver's stream) and the CPU
+oracle's worker-thread sparse-LU path on a bounded sample of the same patch family.
+
+`--workload chart512`: the single 512 x 512 vertex height-field chart (one solve per step).
 """
 from __future__ import annotations
 
@@ -24,6 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "LSCM unknowns solved/s @ rel. residual 1e-8; SpMV HBM GB/s % peak vs FP64 PCG"
+TOL = 1e-8
 
 
 def log(*a):
@@ -41,18 +48,56 @@ def build_chart512(seed=1, n=512):
 
 
 def build_batch(count=4096, seed=0, threads=16):
-    """Config: 4096 open grid-patch charts, vertex counts log-uniform in [64, 65536], aspect 1-8, seeded
-    sinusoidal z and +-0.2 jitter; each patch its own mesh and chart (segmented at 180 degrees)."""
+    """4096 open grid-patch charts, vertex counts log-uniform in [64, 65536], aspect 1-8, seeded sinusoidal z
+    and +-0.2 jitter; each patch its own mesh and chart (segmented at 180 degrees)."""
     from researchuv_amd import batch as B
     from researchuv_amd import meshgen
 
     meshes = list(meshgen.patch_batch(count, seed=seed))
     lb = B.LscmBatchBuilder(meshes, angle_min_deg=180.0, threads=threads)
-    del meshes
-    return lb
+    return lb, meshes
 
 
-def cpu_baseline(seconds_target=12.0):
+def host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline_batch(meshes, budget_verts=250_000, vmax=4096):
+    """The CPU oracle's worker-thread path (pipeline.rs:171-194 over exec.rs:28-60, each chart one sparse-LU
+    factor+solve, sparse.rs:91-265) on the first patches of the batch with at most `vmax` vertices, up to
+    `budget_verts` vertices in total (the LU of the largest patches would take minutes each)."""
+    import numpy as np
+
+    import oracle
+
+    pos_all, faces_all, nv, picked = [], [], 0, 0
+    for k, (pos, faces) in enumerate(meshes):
+        if len(pos) > vmax:
+            continue
+        faces_all.append(faces.astype(np.int64) + sum(len(p) for p in pos_all))
+        pos_all.append(pos + np.array([3.0 * k, 0.0, 0.0]))  # disjoint copies: no welding across patches
+        nv += len(pos)
+        picked += 1
+        if nv >= budget_verts:
+            break
+    P = np.concatenate(pos_all)
+    F = np.concatenate(faces_all).astype(np.uint32)
+    pw, fw = oracle.weld(P, F, 1e-12)
+    ch = oracle.Charts(pw, fw, 180.0)
+    unk = 0
+    for i in range(len(ch)):
+        unk += 2 * ch.system(i)["nfree"]
+    w = host_threads()
+    t0 = time.perf_counter()
+    ch.par_unfold(w, max_iter=1)
+    took = time.perf_counter() - t0
+    return dict(value=unk / took, unit="unknowns/s", cores=w, kind="port",
+                sample=f"oracle worker-thread sparse-LU path (one factor+solve per chart) over the first {picked} "
+                       f"batch patches with <= {vmax} vertices ({len(ch)} charts, {unk} unknowns), {took:.2f} s "
+                       f"on {w} threads; smaller charts than the batch average, so this favours the CPU")
+
+
+def cpu_baseline_chart(seconds_target=12.0):
     """The CPU oracle's sparse-LU unfold (lscm.rs:512-727 with SolverBackend::Cpu, sparse.rs:91-265) on a
     height-field chart of the same family, sized to take about `seconds_target` on one core."""
     import oracle
@@ -65,15 +110,164 @@ def cpu_baseline(seconds_target=12.0):
         ch = oracle.Charts(pw, fw, 180.0)
         s = ch.system(0)
         t0 = time.perf_counter()
-        ch.unfold(0)
+        ch.unfold(0, max_iter=1)
         took = time.perf_counter() - t0
         unk = 2 * s["nfree"]
         if took > seconds_target / 4 or side >= 160:
             break
         side = int(side * 1.4)
     return dict(value=unk / took, unit="unknowns/s", cores=1, kind="port",
-                sample=f"oracle sparse-LU unfold of one {side}x{side} height-field chart ({unk} unknowns), "
-                       f"2 factor+solve passes as the reference driver does, {took:.2f} s on 1 core")
+                sample=f"oracle sparse-LU unfold (one factor+solve) of one {side}x{side} height-field chart "
+                       f"({unk} unknowns), {took:.2f} s on 1 core")
+
+
+def chart_blocks(a, lb):
+    """Per chart: (L as scipy CSR, rhs (n, 2)) from the batch builder's arrays."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    off = a["chart_row_off"]
+    ptr, cols, vals, rhs = a["row_ptr"], a["cols"], a["vals"], a["rhs"]
+
+    def block(c):
+        r0, r1 = int(off[c]), int(off[c + 1])
+        p = ptr[r0:r1 + 1] - ptr[r0]
+        L = sp.csr_matrix((vals[ptr[r0]:ptr[r1]], cols[ptr[r0]:ptr[r1]] - r0, p), shape=(r1 - r0, r1 - r0))
+        return L, rhs[2 * r0:2 * r1].reshape(-1, 2)
+
+    return block
+
+
+def run_batch(args):
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import torch
+
+    from researchuv_amd import solver
+
+    t0 = time.perf_counter()
+    lb, meshes = build_batch(args.charts)
+    a = lb.arrays(csr=True)
+    log(f"batch: {lb.nchart} charts, {lb.nv_total} verts, {2 * lb.nrows} unknowns, nnz(L)={lb.nnz}, "
+        f"host build {time.perf_counter() - t0:.1f}s")
+    unknowns = 2 * lb.nrows
+    off = a["chart_row_off"]
+    dev = torch.device("cuda:0")
+    b = torch.tensor(a["rhs"], dtype=torch.float64, device=dev)
+    x = torch.zeros_like(b)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    bt = lb.device_batch(mixed=True)
+    setup_s = time.perf_counter() - t0
+    sched = bt.schedule()
+    log(f"mixed batch setup {setup_s:.2f}s (sliced ELL + autotune): {sched}")
+    for _ in range(args.warmup):
+        bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        info, it, res = bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL, per_chart=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = unknowns * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"mixed batched: {ms_per_step:.1f} ms/batch, max inner iters {info.iters}, worst true resid "
+        f"{info.true_rel_resid:.2e}")
+    sec, byt = bt.spmv_time(20)
+    achieved = byt / sec
+    log(f"step kernel (direction + SpMV): {sec * 1e3:.3f} ms/launch, {byt / 1e9:.3f} GB -> {achieved / 1e9:.0f} GB/s")
+    bt.close()
+
+    # the same batch in plain FP64 (recursive-residual test at tol)
+    bt64 = lb.device_batch(mixed=False)
+    x64 = torch.zeros_like(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info64, it64, res64 = bt64.solve_dev(b.data_ptr(), x64.data_ptr(), 100000, TOL, per_chart=True)
+    torch.cuda.synchronize()
+    el64 = time.perf_counter() - t0
+    sec64, byt64 = bt64.spmv_time(10)
+    bt64.close()
+    xa = x.cpu().numpy().reshape(-1, 2)
+    xb = x64.cpu().numpy().reshape(-1, 2)
+    worst = 0.0
+    for c in range(lb.nchart):
+        da, db = xa[off[c]:off[c + 1]], xb[off[c]:off[c + 1]]
+        if len(db):
+            worst = max(worst, float(np.linalg.norm(da - db, axis=1).max() /
+                                     max(np.linalg.norm(db, axis=1).max(), 1e-300)))
+    log(f"fp64 batched: {el64 * 1e3:.1f} ms, worst true resid {info64.true_rel_resid:.2e}; "
+        f"mixed vs fp64 uv diff {worst:.2e}")
+
+    # UV and energy error of the mixed solution against a CPU FP64 sparse LU (SuperLU), evenly spaced charts
+    block = chart_blocks(a, lb)
+    sizes = np.diff(off)
+    pick = [c for c in range(0, lb.nchart, max(1, lb.nchart // 64)) if sizes[c] <= 20000][:32]
+    uv_err, en_err = 0.0, 0.0
+    for c in pick:
+        L, r = block(c)
+        lu = spla.splu(L.tocsc())
+        xr = np.stack([lu.solve(r[:, 0]), lu.solve(r[:, 1])], axis=1)
+        xg = xa[off[c]:off[c + 1]]
+        uv_err = max(uv_err, float(np.linalg.norm(xg - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max()))
+
+        def energy(u):  # the quadratic the solve minimises, per coordinate: 1/2 u^T L u - r^T u
+            return float(sum(0.5 * u[:, k] @ (L @ u[:, k]) - r[:, k] @ u[:, k] for k in range(2)))
+
+        e_ref = energy(xr)
+        en_err = max(en_err, abs(energy(xg) - e_ref) / max(abs(e_ref), 1e-300))
+    log(f"vs SuperLU on {len(pick)} charts: uv err {uv_err:.2e}, energy rel err {en_err:.2e}")
+
+    # plain FP64 drop-in, one call per chart as the reference's driver makes (host CSR of A = L (x) I2 in,
+    # host solution out; lscm.rs:471-506 -> solver.rs:284-565), on an evenly spaced sample of charts
+    gs = solver.GpuSolver.new()
+    sample = list(range(0, lb.nchart, max(1, lb.nchart // args.dropin_charts)))
+    I2 = sp.identity(2, format="csr")
+    prepared = []
+    for c in sample:
+        L, r = block(c)
+        A = sp.kron(L, I2, format="csr")
+        A.sort_indices()
+        prepared.append((solver.CsrMatrix(A.data, A.indices.astype(np.int32), A.indptr.astype(np.int32)),
+                         r.reshape(-1).copy()))
+    gs.cg_solve_precond(prepared[0][0], prepared[0][1], 100000, TOL, solver.Precond.JACOBI)  # warm
+    t0 = time.perf_counter()
+    unk_d = 0
+    for A, r in prepared:
+        out = gs.cg_solve_precond(A, r, 100000, TOL, solver.Precond.JACOBI)
+        assert out is not None, gs.last_error
+        unk_d += len(r)
+    el_d = time.perf_counter() - t0
+    value_d = unk_d / el_d
+    log(f"fp64 drop-in per chart: {len(prepared)} charts, {unk_d} unknowns in {el_d:.2f}s -> {value_d:.3e} /s")
+
+    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
+                traffic=None, kernel="kb_dir_spmv<float,2,%d>" % (0 if sched["col_bytes"] == 2 else 1),
+                bytes_per_launch=byt, us_per_launch=sec * 1e6)
+    tf = os.path.join(ROOT, "profiles", "traffic_batch.json")
+    if os.path.exists(tf):
+        roof["traffic"] = json.load(open(tf)).get("hbm_bytes_per_launch")
+    return dict(
+        value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
+        config={"workload": f"batch: {lb.nchart} uneven height-field patch charts (64-65536 vertices, log-uniform), "
+                            f"{lb.nv_total} vertices, {unknowns} unknowns", "charts": lb.nchart,
+                "unknowns": unknowns, "nnz_L": lb.nnz, "precond": "jacobi", "tol_true_rel_resid": TOL,
+                "schedule": sched},
+        extra={"setup_s": setup_s, "max_inner_iters": int(info.iters), "mean_inner_iters": float(np.mean(it)),
+               "worst_true_rel_resid": float(info.true_rel_resid),
+               "uv_rel_err_vs_cpu_lu": uv_err, "energy_rel_err_vs_cpu_lu": en_err,
+               "cpu_lu_check": f"SuperLU on {len(pick)} evenly spaced charts of <= 20000 rows",
+               "fp64_batched": {"value": unknowns / el64, "ms": 1e3 * el64, "max_iters": int(info64.iters),
+                                "worst_true_rel_resid": float(info64.true_rel_resid),
+                                "step_kernel_GBps": byt64 / sec64 / 1e9},
+               "fp64_dropin_per_chart": {"value": value_d, "charts": len(prepared), "unknowns": unk_d,
+                                         "seconds": el_d},
+               "speedup_vs_fp64_batched": value / (unknowns / el64),
+               "speedup_vs_fp64_dropin_per_chart": value / value_d,
+               "uv_rel_diff_mixed_vs_fp64": worst},
+        roofline=roof, meshes=meshes)
 
 
 def run_chart512(args):
@@ -84,54 +278,44 @@ def run_chart512(args):
 
     t0 = time.perf_counter()
     s = build_chart512()
-    setup_host = time.perf_counter() - t0
     nfree = s["nfree"]
     unknowns = 2 * nfree
     L = solver.CsrMatrix(s["l_vals"], s["l_cols"], s["l_ptr"])
-    log(f"nfree={nfree} unknowns={unknowns} nnz(L)={len(s['l_vals'])} host setup {setup_host:.2f}s")
-
+    log(f"nfree={nfree} unknowns={unknowns} nnz(L)={len(s['l_vals'])} host setup {time.perf_counter() - t0:.2f}s")
     dev = torch.device("cuda:0")
     b = torch.tensor(s["rhs"], dtype=torch.float64, device=dev)
     x = torch.zeros_like(b)
     x64 = torch.zeros_like(b)
     torch.cuda.synchronize()
-    tol = 1e-8
     max_iter = 1_000_000
-
-    plan = solver.Plan(L, 2, solver.Precond.JACOBI, mixed=True)
+    pc = getattr(solver.Precond, args.precond)
+    t0 = time.perf_counter()
+    plan = solver.Plan(L, 2, pc, mixed=True)
+    setup_s = time.perf_counter() - t0
     for _ in range(args.warmup):
-        plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, tol)
+        plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, TOL)
     torch.cuda.synchronize()
-    infos = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        infos.append(plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, tol))
+        info = plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, TOL)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = unknowns * args.steps / elapsed
-    info = infos[-1]
     log(f"mixed: {ms_per_step:.2f} ms/solve, inner iters {info.iters}, refinements {info.refinements}, "
         f"true rel residual {info.true_rel_resid:.2e}")
-
-    # plain FP64 PCG (the drop-in) on the same system, recursive tol 1e-8
-    plan64 = solver.Plan(L, 2, solver.Precond.JACOBI, mixed=False)
-    plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, tol)
+    plan64 = solver.Plan(L, 2, pc, mixed=False)
+    plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, TOL)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.fp64_steps):
-        info64 = plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, tol)
+        info64 = plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, TOL)
     torch.cuda.synchronize()
     el64 = (time.perf_counter() - t0) / args.fp64_steps
-    value64 = unknowns / el64
     log(f"fp64: {1e3 * el64:.2f} ms/solve, iters {info64.iters}, true rel residual {info64.true_rel_resid:.2e}")
-
-    # agreement of the two solutions in UV terms (lscm.rs:749-758 metric)
     xa = x.cpu().numpy().reshape(-1, 2)
     xb = x64.cpu().numpy().reshape(-1, 2)
     uv_diff = float(np.linalg.norm(xa - xb, axis=1).max() / np.linalg.norm(xb, axis=1).max())
-
-    # roofline of the dominant kernel (FP32 SpMV over L, two right-hand sides)
     import ctypes as C
 
     from researchuv_amd import _lib
@@ -140,105 +324,31 @@ def run_chart512(args):
     byt = C.c_double(0)
     _lib.check(_lib.lib().ruv_plan_spmv_time(plan.h, 200, C.byref(sec), C.byref(byt)))
     achieved = byt.value / sec.value
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_chart512.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("spmv_hbm_bytes_per_launch")
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=traffic, kernel="k_spmv_pq<float,2>", bytes_per_launch=byt.value,
-                us_per_launch=sec.value * 1e6)
+                traffic=None, kernel="k_spmv_pq<float,2>", bytes_per_launch=byt.value, us_per_launch=sec.value * 1e6)
     log(f"spmv: {sec.value * 1e6:.2f} us/launch, {byt.value / 1e6:.2f} MB algorithmic -> {achieved / 1e9:.0f} GB/s")
-
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
-        config={"workload": "chart512: one 512x512-vertex height-field chart, 520200 unknowns",
-                "unknowns": unknowns, "nnz_L": int(len(s["l_vals"])), "precond": "jacobi",
-                "tol_true_rel_resid": tol},
-        extra={"inner_iters": int(info.iters), "refinements": int(info.refinements),
+        config={"workload": f"chart512: one 512x512-vertex height-field chart, {unknowns} unknowns",
+                "unknowns": unknowns, "nnz_L": int(len(s["l_vals"])), "precond": args.precond.lower(),
+                "tol_true_rel_resid": TOL},
+        extra={"setup_s": setup_s, "inner_iters": int(info.iters), "refinements": int(info.refinements),
                "true_rel_resid": float(info.true_rel_resid),
-               "fp64_pcg": {"value": value64, "ms_per_solve": 1e3 * el64, "iters": int(info64.iters),
+               "fp64_pcg": {"value": unknowns / el64, "ms_per_solve": 1e3 * el64, "iters": int(info64.iters),
                             "true_rel_resid": float(info64.true_rel_resid)},
-               "speedup_vs_fp64": value / value64, "uv_rel_diff_mixed_vs_fp64": uv_diff},
-        roofline=roof)
-
-
-def run_batch(args):
-    import numpy as np
-    import torch
-
-    t0 = time.perf_counter()
-    lb = build_batch(args.charts)
-    a = lb.arrays()
-    log(f"batch: {lb.nchart} charts, {lb.nv_total} verts, {2 * lb.nrows} unknowns, nnz(L)={lb.nnz}, "
-        f"host build {time.perf_counter() - t0:.1f}s")
-    unknowns = 2 * lb.nrows
-    tol = 1e-8
-    dev = torch.device("cuda:0")
-    b = torch.tensor(a["rhs"], dtype=torch.float64, device=dev)
-    x = torch.zeros_like(b)
-    torch.cuda.synchronize()
-    bt = lb.device_batch(mixed=True)
-    for _ in range(args.warmup):
-        bt.solve_dev(b.data_ptr(), x.data_ptr(), 16384, tol)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        info, it, res = bt.solve_dev(b.data_ptr(), x.data_ptr(), 16384, tol, per_chart=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    value = unknowns * args.steps / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    log(f"mixed batched: {ms_per_step:.1f} ms/batch, max inner iters {info.iters}, worst true resid "
-        f"{info.true_rel_resid:.2e}")
-    sec, byt = bt.spmv_time(20)
-    achieved = byt / sec
-    log(f"batched spmv: {sec * 1e3:.3f} ms/launch, {byt / 1e9:.2f} GB -> {achieved / 1e9:.0f} GB/s")
-    bt.close()
-    # FP64 comparison 1: the same batch in plain FP64 (recursive-residual test at tol)
-    bt64 = lb.device_batch(mixed=False)
-    x64 = torch.zeros_like(b)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    info64, it64, res64 = bt64.solve_dev(b.data_ptr(), x64.data_ptr(), 16384, tol, per_chart=True)
-    torch.cuda.synchronize()
-    el64 = time.perf_counter() - t0
-    bt64.close()
-    xa = x.cpu().numpy().reshape(-1, 2)
-    xb = x64.cpu().numpy().reshape(-1, 2)
-    off = a["chart_row_off"]
-    worst = 0.0
-    for c in range(lb.nchart):
-        da = xa[off[c]:off[c + 1]]
-        db = xb[off[c]:off[c + 1]]
-        if len(db):
-            worst = max(worst, float(np.linalg.norm(da - db, axis=1).max() / max(np.linalg.norm(db, axis=1).max(), 1e-300)))
-    log(f"fp64 batched: {el64 * 1e3:.1f} ms, worst true resid {info64.true_rel_resid:.2e}; mixed vs fp64 uv diff {worst:.2e}")
-    roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel="kb_spmv<float,2>", bytes_per_launch=byt, us_per_launch=sec * 1e6)
-    tf = os.path.join(ROOT, "profiles", "traffic_batch.json")
-    if os.path.exists(tf):
-        roof["traffic"] = json.load(open(tf)).get("spmv_hbm_bytes_per_launch")
-    return dict(
-        value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
-        config={"workload": f"batch: {lb.nchart} uneven height-field patch charts, {lb.nv_total} vertices, "
-                            f"{unknowns} unknowns", "charts": lb.nchart, "unknowns": unknowns, "nnz_L": lb.nnz,
-                "precond": "jacobi", "tol_true_rel_resid": tol},
-        extra={"max_inner_iters": int(info.iters), "mean_inner_iters": float(np.mean(it)),
-               "worst_true_rel_resid": float(info.true_rel_resid),
-               "fp64_batched": {"value": unknowns / el64, "ms": 1e3 * el64, "max_iters": int(info64.iters),
-                                "worst_true_rel_resid": float(info64.true_rel_resid)},
-               "speedup_vs_fp64_batched": (unknowns / elapsed * args.steps) / (unknowns / el64),
-               "uv_rel_diff_mixed_vs_fp64": worst},
-        roofline=roof)
+               "speedup_vs_fp64": value / (unknowns / el64), "uv_rel_diff_mixed_vs_fp64": uv_diff},
+        roofline=roof, meshes=None)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="chart512", choices=["chart512", "batch"])
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch", choices=["batch", "chart512"])
     ap.add_argument("--charts", type=int, default=4096, help="batch workload: number of charts")
+    ap.add_argument("--dropin-charts", type=int, default=64, help="batch: charts in the per-chart drop-in sample")
+    ap.add_argument("--precond", default="JACOBI", help="chart512: preconditioner name")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-steps", type=int, default=2)
     args = ap.parse_args()
@@ -247,24 +357,20 @@ def main():
               f"has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from researchuv_amd import solver
-
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
     log("building workload", args.workload)
-    if args.workload == "batch":
-        out = run_batch(args)
-    else:
-        out = run_chart512(args)
+    out = run_batch(args) if args.workload == "batch" else run_chart512(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle sparse LU)")
-        cpu = cpu_baseline()
+        cpu = cpu_baseline_batch(out["meshes"]) if out["meshes"] is not None else cpu_baseline_chart()
         log(f"cpu: {cpu['value']:.0f} unknowns/s ({cpu['sample']})")
-
     result = {
-        "metric": "LSCM unknowns solved/s @ rel. residual 1e-8 (mixed FP32 PCG + FP64 refinement)",
+        "metric": METRIC,
         "value": out["value"],
         "unit": "unknowns/s",
         "n_gpus": 1,

@@ -138,9 +138,13 @@ int ruv_batch_solve_dev(ruv_batch* batch, const double* d_b, double* d_x, uint64
                         ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
 int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
-/* Measurement hook: mean duration and algorithmic bytes of the batch's inner-precision SpMV launch over
- * every tile (row_ptr counted at 8 bytes per row). */
+/* Measurement hook: mean duration and algorithmic bytes of the batch's inner-precision step kernel (new
+ * search direction + SpMV) over every tile: the sliced-ELL values and column offsets as stored, the slice
+ * table, and four vectors (z, old p read; new p, q written) streamed once. */
 int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
+/* The schedule the batch's autotuner chose: rows per workgroup tile, bytes per stored column index (2: 16-bit
+ * offsets from the row, 4: 32-bit rows) and the number of stored sliced-ELL entries (padding included). */
+int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries);
 void ruv_batch_destroy(ruv_batch* batch);
 
 /* Build the LSCM systems of nmesh meshes on host worker threads (the worker-thread unfold stage,

@@ -59,6 +59,8 @@ def lib():
         L.orc_unfold_chart.argtypes = [P, C.c_int, P, C.c_int, C.c_double, C.c_double, P, P]
         L.orc_par_unfold.restype = C.c_int
         L.orc_par_unfold.argtypes = [P, P, C.c_int, C.c_double, C.c_double, P, P]
+        L.orc_par_unfold_iters.restype = C.c_int
+        L.orc_par_unfold_iters.argtypes = [P, P, C.c_int, C.c_int, C.c_double, C.c_double, P, P]
         L.orc_cg_solve.restype = C.c_int
         L.orc_cg_solve.argtypes = [C.c_int32, P, P, P, P, C.c_int64, P]
         L.orc_ic0_factor.restype = C.c_int64
@@ -177,8 +179,9 @@ class Charts:
         lib().orc_unfold_chart(self.h, i, _p(self.pos), max_iter, angle_mix, mix_w, _p(uv), _p(st))
         return uv, dict(iters=int(st[0]), err_accum=st[1], target=st[2], area3d=st[3])
 
-    def par_unfold(self, workers, angle_mix=1.0, mix_w=1.0):
-        """Worker-thread path over every chart (pipeline.rs:171-194, exec.rs:28-60)."""
+    def par_unfold(self, workers, angle_mix=1.0, mix_w=1.0, max_iter=0):
+        """Worker-thread path over every chart (pipeline.rs:171-194, exec.rs:28-60).  max_iter 0 runs the
+        driver's own iteration count; 1 one factor+solve per chart."""
         n = len(self)
         sizes = []
         for i in range(n):
@@ -188,7 +191,7 @@ class Charts:
         off = np.zeros(n + 1, np.int64)
         off[1:] = np.cumsum(sizes)
         uv = np.zeros((int(off[-1]), 2))
-        lib().orc_par_unfold(self.h, _p(self.pos), workers, angle_mix, mix_w, _p(off), _p(uv))
+        lib().orc_par_unfold_iters(self.h, _p(self.pos), workers, max_iter, angle_mix, mix_w, _p(off), _p(uv))
         return [uv[off[i]:off[i + 1]] for i in range(n)]
 
 

@@ -1276,18 +1276,22 @@ int orc_unfold_chart(const orc_charts* cc, int ci, const double* pos, int max_it
  * Unfolds charts [0, nch) with `workers` threads; uv written at uv_off[ci]*2. */
 typedef struct {
     const orc_charts* cc; const double* pos; int lo, hi; double angle_mix, mix_w; double* uv; const int64_t* off;
+    int max_iter;
 } parjob;
 static void* par_worker(void* arg) {
     parjob* j = (parjob*)arg;
     double st[4];
-    for (int ci = j->lo; ci < j->hi; ++ci) orc_unfold_chart(j->cc, ci, j->pos, 0, j->angle_mix, j->mix_w, j->uv + 2 * j->off[ci], st);
+    for (int ci = j->lo; ci < j->hi; ++ci)
+        orc_unfold_chart(j->cc, ci, j->pos, j->max_iter, j->angle_mix, j->mix_w, j->uv + 2 * j->off[ci], st);
     return NULL;
 }
-int orc_par_unfold(const orc_charts* cc, const double* pos, int workers, double angle_mix, double mix_w,
-                   const int64_t* uv_off, double* uv) {
+/* max_iter 0: the driver's own iteration count (two factor+solve passes with the default options); 1: one
+ * factor+solve per chart, the unit of "unknowns solved". */
+int orc_par_unfold_iters(const orc_charts* cc, const double* pos, int workers, int max_iter, double angle_mix,
+                         double mix_w, const int64_t* uv_off, double* uv) {
     int n = cc->n;
     if (workers <= 1 || n <= 1) {
-        parjob j = {cc, pos, 0, n, angle_mix, mix_w, uv, uv_off};
+        parjob j = {cc, pos, 0, n, angle_mix, mix_w, uv, uv_off, max_iter};
         par_worker(&j);
         return 0;
     }
@@ -1298,13 +1302,17 @@ int orc_par_unfold(const orc_charts* cc, const double* pos, int workers, double 
     int nt = 0;
     for (int lo = 0; lo < n && nt < 256; lo += chunk) {
         int hi = lo + chunk < n ? lo + chunk : n;
-        parjob j = {cc, pos, lo, hi, angle_mix, mix_w, uv, uv_off};
+        parjob j = {cc, pos, lo, hi, angle_mix, mix_w, uv, uv_off, max_iter};
         jobs[nt] = j;
         pthread_create(&th[nt], NULL, par_worker, &jobs[nt]);
         nt++;
     }
     for (int t = 0; t < nt; ++t) pthread_join(th[t], NULL);
     return 0;
+}
+int orc_par_unfold(const orc_charts* cc, const double* pos, int workers, double angle_mix, double mix_w,
+                   const int64_t* uv_off, double* uv) {
+    return orc_par_unfold_iters(cc, pos, workers, 0, angle_mix, mix_w, uv_off, uv);
 }
 
 /* ------------------------------------------------------------------ CG (lscm.rs:417-459) */

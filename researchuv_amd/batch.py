@@ -30,6 +30,8 @@ def _setup(L):
     L.ruv_batch_solve.argtypes = [P, P, P, C.c_uint64, C.c_double, P, P, P]
     L.ruv_batch_spmv_time.restype = C.c_int
     L.ruv_batch_spmv_time.argtypes = [P, C.c_int32, P, P]
+    L.ruv_batch_schedule.restype = C.c_int
+    L.ruv_batch_schedule.argtypes = [P, P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
     L.ruv_lscm_batch_build.restype = C.c_int
@@ -109,6 +111,14 @@ class Batch:
         byt = C.c_double(0)
         _lib.check(lib().ruv_batch_spmv_time(self.h, reps, C.byref(sec), C.byref(byt)))
         return sec.value, byt.value
+
+    def schedule(self) -> dict:
+        """The autotuned schedule: rows per workgroup tile, bytes per column index, stored ELL entries."""
+        rows = C.c_int32(0)
+        cb = C.c_int32(0)
+        ne = C.c_int64(0)
+        _lib.check(lib().ruv_batch_schedule(self.h, C.byref(rows), C.byref(cb), C.byref(ne)))
+        return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value)
 
     def close(self):
         if self.h:

@@ -21,6 +21,7 @@ void batch_destroy(BatchImpl* q);
 int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid);
 int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
+int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
                      LscmBatch** out);
@@ -327,6 +328,11 @@ int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_i
 int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch) {
     if (!batch || !seconds_per_launch || !bytes_per_launch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_spmv_time(batch->impl, reps, seconds_per_launch, bytes_per_launch);
+}
+
+int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries);
 }
 
 void ruv_batch_destroy(ruv_batch* batch) {

@@ -1,26 +1,29 @@
-// batch.hip -- batched LSCM solves: thousands of independent charts, one set of launches per iteration.
+// batch.hip -- batched LSCM solves: thousands of independent charts, two launches per PCG step.
 //
 // The reference solved charts one at a time from worker threads (crates/researchuv-unwrap/src/pipeline.rs:
 // 171-194 -> lscm.rs:471-506 -> solver.rs:284-565), each solve paying its own uploads, host-synchronised dot
-// products and launches.  Here every chart of a batch is one diagonal block of a single block-diagonal CSR
-// (rows of L, vectors interleaved (u, v)); the rows are cut into tiles of at most TILE rows that never span
-// two charts, and each workgroup owns one tile.  Each chart keeps its OWN CG scalars (alpha, beta, r.z,
-// convergence): a tile's partial dot products are summed per chart by every workgroup of that chart, in a
-// fixed order, so the recurrence of every chart is exactly an independent PCG.  Converged charts drop out:
-// the host compacts the list of live tiles between chunks of iterations, so late iterations launch only the
-// tiles of the charts still running, spread over the whole GPU.
+// products and launches.  Here every chart of a batch is one diagonal block of a single block-diagonal matrix
+// in sliced-ELL form (batch_kernels.hpp); each chart keeps its own CG scalars, so each chart runs exactly an
+// independent PCG while all of them share the launches.  Converged charts drop out: between chunks of steps the
+// host compacts the list of live tiles, so late steps launch only the tiles of the charts still running.
 //
 // Mixed precision: the inner PCG runs in FP32 (matrix, Jacobi diagonal, Krylov vectors); the outer loop keeps
 // x and r = b - A x in FP64 per chart and stops a chart when its true residual ||b - A x|| <= tol ||b||.
+//
+// Schedule: the number of slices per tile (rows per workgroup) is picked per batch by timing the step kernel
+// over the whole batch for each candidate (RUV_BATCH_SLICES_PER_TILE=1|2|4|8 forces one).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <string>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
-#include "dev_common.hpp"
+#include "batch_kernels.hpp"
 #include "ruv_internal.hpp"
 
 namespace ruv {
@@ -31,305 +34,39 @@ namespace ruv {
         if (e_ != hipSuccess) return fail(RUV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-namespace {
-
-constexpr int TILE = 512;  // rows per workgroup tile (two per thread)
-enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4 };
-
-struct BDev {
-    const int64_t* ptr;
-    const int* cols;
-    const int* tile_chart;
-    const int64_t* tile_r0;
-    const int64_t* tile_r1;
-    const int* chart_t0;  // C + 1
-    double* c_rz;
-    double* c_tol;
-    double* c_bnorm;
-    double* c_true;
-    int* c_done;
-    int* c_final;
-    unsigned* c_iters;
-    double* part_a;
-    double* part_b;
-    double* part_c;
-    unsigned max_iter;
-    int nchart;
-};
-
-template <typename T, int R>
-__device__ __forceinline__ double row_spmv(const BDev& B, const T* __restrict__ vals, const T* __restrict__ p,
-                                           T* __restrict__ q, int64_t i) {
-    T s[R];
-#pragma unroll
-    for (int c = 0; c < R; ++c) s[c] = T(0);
-    const int64_t a = B.ptr[i], b = B.ptr[i + 1];
-    for (int64_t k = a; k < b; ++k) {
-        const T v = vals[k];
-        const int64_t col = (int64_t)B.cols[k] * R;
-#pragma unroll
-        for (int c = 0; c < R; ++c) s[c] += v * p[col + c];
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int c = 0; c < R; ++c) {
-        q[i * R + c] = s[c];
-        acc += (double)p[i * R + c] * (double)s[c];
-    }
-    return acc;
-}
-
-// q = A p over a tile, partial p.q; the chart's first tile opens the iteration (rz, count, max_iter stop).
-template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ act, const T* __restrict__ vals,
-                                               const T* __restrict__ p, T* __restrict__ q) {
-    __shared__ double sh[WAVES];
-    const int t = act[blockIdx.x];
-    const int c = B.tile_chart[t];
-    if (B.c_done[c] != B_RUNNING) return;
-    const int t0 = B.chart_t0[c];
-    if (t == t0) {
-        const double rz = sum_range(B.part_c, t0, B.chart_t0[c + 1], sh);
-        if (threadIdx.x == 0) {
-            if (B.c_iters[c] >= B.max_iter) {
-                B.c_done[c] = B_MAXITER;
-            } else {
-                B.c_iters[c] += 1;
-                B.c_rz[c] = rz;
-            }
-        }
-    }
-    double acc = 0.0;
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) acc += row_spmv<T, R>(B, vals, p, q, i);
-    const double s = block_sum_all(acc, sh);
-    if (threadIdx.x == 0) B.part_a[t] = s;
-}
-
-// alpha = rz / pAp per chart; x += alpha p; r -= alpha q; z = D^-1 r (PC 1) ; partials r.r, r.z
-template <typename T, int R, int PC>
-__global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__ act, const T* __restrict__ dinv,
-                                                 const T* __restrict__ p, const T* __restrict__ q,
-                                                 T* __restrict__ x, T* __restrict__ r, T* __restrict__ z) {
-    __shared__ double sh[WAVES];
-    const int t = act[blockIdx.x];
-    const int c = B.tile_chart[t];
-    if (B.c_done[c] != B_RUNNING) return;
-    const int t0 = B.chart_t0[c];
-    const double pap = sum_range(B.part_a, t0, B.chart_t0[c + 1], sh);
-    if (!(fabs(pap) > 1e-300) || !isfinite(pap)) {
-        if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_BREAKDOWN;
-        return;
-    }
-    const T a = (T)(B.c_rz[c] / pap);
-    double arr = 0.0, arz = 0.0;
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
-        const T d = (PC == 1) ? dinv[i] : T(1);
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) {
-            const int64_t k = i * R + cc;
-            x[k] = x[k] + a * p[k];
-            const T ri = r[k] - a * q[k];
-            r[k] = ri;
-            arr += (double)ri * (double)ri;
-            if (PC == 1) {
-                const T zi = d * ri;
-                z[k] = zi;
-                arz += (double)ri * (double)zi;
-            }
-        }
-    }
-    const double srr = block_sum_all(arr, sh);
-    const double srz = (PC == 1) ? block_sum_all(arz, sh) : srr;
-    if (threadIdx.x == 0) {
-        B.part_b[t] = srr;
-        B.part_c[t] = srz;
-    }
-}
-
-// per-chart convergence on the recursive residual, then p = z + beta p
-template <typename T, int R, int PC>
-__global__ __launch_bounds__(BLK) void kb_pupdate(BDev B, const int* __restrict__ act, const T* __restrict__ z,
-                                                  T* __restrict__ p) {
-    __shared__ double sh[WAVES];
-    const int t = act[blockIdx.x];
-    const int c = B.tile_chart[t];
-    if (B.c_done[c] != B_RUNNING) return;
-    const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
-    const double rr = sum_range(B.part_b, t0, t1, sh);
-    if (sqrt(rr) <= B.c_tol[c]) {
-        if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_CONVERGED;
-        return;
-    }
-    const double rznew = (PC == 0) ? rr : sum_range(B.part_c, t0, t1, sh);
-    const T bt = (T)(rznew / B.c_rz[c]);
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1 * R; k += BLK) p[k] = z[k] + bt * p[k];
-}
-
-// start of a solve: x64 = 0, r64 = b, partials of b.b
-template <int R>
-__global__ __launch_bounds__(BLK) void kb_start(BDev B, const double* __restrict__ b, double* __restrict__ x64,
-                                                double* __restrict__ r64) {
-    __shared__ double sh[WAVES];
-    const int t = blockIdx.x;
-    double acc = 0.0;
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1 * R; k += BLK) {
-        const double v = b[k];
-        x64[k] = 0.0;
-        r64[k] = v;
-        acc += v * v;
-    }
-    const double s = block_sum_all(acc, sh);
-    if (threadIdx.x == 0) B.part_b[t] = s;
-}
-
-// per chart: ||b||, not final
-__global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart) return;
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
-    B.c_bnorm[c] = sqrt(s);
-    B.c_final[c] = 0;
-    B.c_true[c] = 1.0;
-}
-
-// inner start for the charts not yet final: x = 0, r = T(r64), z = D^-1 r, p = z; partials r.r and r.z
-template <typename T, int R, int PC>
-__global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ act, const double* __restrict__ r64,
-                                               const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
-                                               T* __restrict__ z, T* __restrict__ p) {
-    __shared__ double sh[WAVES];
-    const int t = act[blockIdx.x];
-    const int c = B.tile_chart[t];
-    double arr = 0.0, arz = 0.0;
-    if (!B.c_final[c]) {
-        const int64_t r1 = B.tile_r1[t];
-        for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
-            const T d = (PC == 1) ? dinv[i] : T(1);
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) {
-                const int64_t k = i * R + cc;
-                const T ri = (T)r64[k];
-                const T zi = d * ri;
-                x[k] = T(0);
-                r[k] = ri;
-                if (PC == 1) z[k] = zi;
-                p[k] = zi;
-                arr += (double)ri * (double)ri;
-                arz += (double)ri * (double)zi;
-            }
-        }
-    }
-    const double srr = block_sum_all(arr, sh);
-    const double srz = block_sum_all(arz, sh);
-    if (threadIdx.x == 0) {
-        B.part_b[t] = srr;
-        B.part_c[t] = srz;
-    }
-}
-
-// per chart inner tolerance and state.  plain: tol * ||b|| on the recursive residual (solver.rs:465-468);
-// mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
-__global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart) return;
-    if (B.c_final[c]) {
-        B.c_done[c] = B_SKIP;
-        return;
-    }
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
-    const double rn = sqrt(s), bn = B.c_bnorm[c];
-    double ta;
-    if (mixed) ta = fmax(inner_tol * rn, 0.25 * ((bn <= 1e-300) ? tol : tol * bn));
-    else ta = (bn <= 1e-300) ? tol : tol * bn;
-    B.c_tol[c] = ta;
-    B.c_iters[c] = 0;
-    B.c_done[c] = (rn <= ta) ? B_CONVERGED : B_RUNNING;
-}
-
-// x64 += x (the inner correction) on the live charts
-template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kb_axpy(BDev B, const int* __restrict__ act, const T* __restrict__ xin,
-                                               double* __restrict__ x64) {
-    const int t = act[blockIdx.x];
-    if (B.c_final[B.tile_chart[t]]) return;
-    const int64_t r1 = B.tile_r1[t] * R;
-    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1; k += BLK) x64[k] += (double)xin[k];
-}
-
-// r64 = b - A64 x64 in FP64 on the live charts, partials of r64.r64 (runs after kb_axpy: rows of a tile read
-// x64 of rows other tiles update)
-template <int R>
-__global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict__ act,
-                                                   const double* __restrict__ vals64, const double* __restrict__ b,
-                                                   const double* __restrict__ x64, double* __restrict__ r64) {
-    __shared__ double sh[WAVES];
-    const int t = act[blockIdx.x];
-    const int c = B.tile_chart[t];
-    double acc = 0.0;
-    if (!B.c_final[c]) {
-        const int64_t r1 = B.tile_r1[t];
-        for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
-            double s[R];
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) s[cc] = 0.0;
-            for (int64_t k = B.ptr[i]; k < B.ptr[i + 1]; ++k) {
-                const double v = vals64[k];
-                const int64_t col = (int64_t)B.cols[k] * R;
-#pragma unroll
-                for (int cc = 0; cc < R; ++cc) s[cc] += v * x64[col + cc];
-            }
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) {
-                const double rc = b[i * R + cc] - s[cc];
-                r64[i * R + cc] = rc;
-                acc += rc * rc;
-            }
-        }
-    }
-    const double ss = block_sum_all(acc, sh);
-    if (threadIdx.x == 0) B.part_b[t] = ss;
-}
-
-// per chart: true residual; final when ||b - A x|| <= tol ||b||
-__global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart || B.c_final[c]) return;
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
-    const double rn = sqrt(s), bn = B.c_bnorm[c];
-    B.c_true[c] = (bn > 0.0) ? rn / bn : rn;
-    if (rn <= ((bn <= 1e-300) ? tol : tol * bn)) B.c_final[c] = 1;
-}
-
-}  // namespace
-
 struct BatchImpl {
-    int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0;
-    int64_t nrows = 0, nnz = 0;
+    int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
+    int ci = 0;          // 0: 16-bit column offsets, 1: 32-bit columns
+    int spt = 2;         // slices per tile (chosen by the autotuner)
+    int64_t nrows = 0, nnz = 0, nell = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> mem;
     BDev B{};
-    double* vals64 = nullptr;
-    float* vals32 = nullptr;
-    double* dinv64 = nullptr;
-    float* dinv32 = nullptr;
-    double *x64 = nullptr, *r64 = nullptr;
-    double *xd = nullptr, *rd = nullptr, *zd = nullptr, *pd = nullptr, *qd = nullptr;
-    float *xf = nullptr, *rf = nullptr, *zf = nullptr, *pf = nullptr, *qf = nullptr;
+    // host copies of the slice table and chart row offsets (tile building)
+    std::vector<int64_t> h_sl_row0;
+    std::vector<int32_t> h_sl_n;
+    std::vector<int32_t> chart_s0;  // first slice of each chart, nchart + 1
+    std::vector<int64_t> crow;      // first row of each chart, nchart + 1
+    std::vector<int> chart_t0;
+    // tiles (rebuilt by the autotuner)
+    int32_t *d_tile_s0 = nullptr, *d_tile_chart = nullptr, *d_chart_t0 = nullptr;
+    int64_t *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;
+    double *d_part_a = nullptr, *d_part_b = nullptr, *d_part_c = nullptr, *d_part_o = nullptr;
     int* act = nullptr;
     int* h_act = nullptr;
+    // matrix
+    float* ev32 = nullptr;
+    double* ev64 = nullptr;
+    float* dinv32 = nullptr;
+    double* dinv64 = nullptr;
+    // vectors
+    double *x64 = nullptr, *r64 = nullptr;
+    void *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv0 = nullptr, *pv1 = nullptr, *qv = nullptr;
     int* h_done = nullptr;
     int* h_final = nullptr;
     unsigned* h_iters = nullptr;
     double* h_true = nullptr;
-    std::vector<int> chart_t0;
     std::vector<uint64_t> iters_total;
     double inner_tol = 1e-4;
     int max_rounds = 40;
@@ -348,10 +85,22 @@ struct BatchImpl {
         if (e == hipSuccess && n) e = hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, stream);
         return e;
     }
+    void free_tiles() {
+        for (void* p : {(void*)d_tile_s0, (void*)d_tile_chart, (void*)d_chart_t0, (void*)d_tile_r0, (void*)d_tile_r1,
+                        (void*)d_part_a, (void*)d_part_b, (void*)d_part_c, (void*)d_part_o, (void*)act})
+            if (p) (void)hipFree(p);
+        d_tile_s0 = d_tile_chart = d_chart_t0 = nullptr;
+        d_tile_r0 = d_tile_r1 = nullptr;
+        d_part_a = d_part_b = d_part_c = d_part_o = nullptr;
+        act = nullptr;
+        if (h_act) (void)hipHostFree(h_act);
+        h_act = nullptr;
+    }
     ~BatchImpl() {
         if (stream) (void)hipStreamSynchronize(stream);
+        free_tiles();
         for (void* p : mem) (void)hipFree(p);
-        for (void* p : {(void*)h_act, (void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
+        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
             if (p) (void)hipHostFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -361,6 +110,81 @@ struct BatchImpl {
 
 namespace {
 
+// run f(lo, hi) over [0, n) on up to 16 host threads
+template <typename F>
+void parallel_for(int64_t n, F f) {
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(16, n / 4096));
+    if (nt == 1) {
+        f((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; ++k) th.emplace_back(f, n * k / nt, n * (k + 1) / nt);
+    for (auto& t : th) t.join();
+}
+
+// (Re)build the tiles for `spt` slices per tile and their per-tile buffers.
+int build_tiles(BatchImpl& Q, int spt) {
+    std::vector<int32_t> tile_s0, tile_chart;
+    std::vector<int64_t> tr0, tr1;
+    Q.chart_t0.assign(Q.nchart + 1, 0);
+    for (int c = 0; c < Q.nchart; ++c) {
+        Q.chart_t0[c] = (int)tile_chart.size();
+        const int s0 = Q.chart_s0[c], s1 = Q.chart_s0[c + 1];
+        if (s0 == s1) {  // an empty chart still owns one empty tile (slices [s0, s0)) so its scalars exist
+            tile_s0.push_back(s0);
+            tile_chart.push_back(c);
+            tr0.push_back(Q.crow[c]);
+            tr1.push_back(Q.crow[c]);
+            continue;
+        }
+        for (int s = s0; s < s1; s += spt) {
+            const int e = std::min(s + spt, s1);
+            tile_s0.push_back(s);
+            tile_chart.push_back(c);
+            tr0.push_back(Q.h_sl_row0[s]);
+            tr1.push_back(Q.h_sl_row0[e - 1] + Q.h_sl_n[e - 1]);
+        }
+    }
+    Q.chart_t0[Q.nchart] = (int)tile_chart.size();
+    const int T = (int)tile_chart.size();
+    std::vector<int32_t> ts0(tile_s0);
+    ts0.push_back(Q.nslice);  // tile t spans slices [ts0[t], ts0[t + 1])
+    Q.free_tiles();
+    Q.ntile = T;
+    Q.spt = spt;
+    auto up = [&](auto** dst, const auto* src, size_t n) -> hipError_t {
+        using E = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
+        hipError_t e = hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(E));
+        if (e == hipSuccess && n) e = hipMemcpyAsync(*dst, src, n * sizeof(E), hipMemcpyHostToDevice, Q.stream);
+        return e;
+    };
+    HIPCHK(up(&Q.d_tile_s0, ts0.data(), ts0.size()));
+    HIPCHK(up(&Q.d_tile_chart, tile_chart.data(), tile_chart.size()));
+    HIPCHK(up(&Q.d_chart_t0, Q.chart_t0.data(), Q.chart_t0.size()));
+    HIPCHK(up(&Q.d_tile_r0, tr0.data(), tr0.size()));
+    HIPCHK(up(&Q.d_tile_r1, tr1.data(), tr1.size()));
+    HIPCHK(hipMalloc((void**)&Q.d_part_a, sizeof(double) * (size_t)T));
+    HIPCHK(hipMalloc((void**)&Q.d_part_b, sizeof(double) * 2 * (size_t)T));
+    HIPCHK(hipMalloc((void**)&Q.d_part_c, sizeof(double) * 2 * (size_t)T));
+    HIPCHK(hipMalloc((void**)&Q.d_part_o, sizeof(double) * (size_t)T));
+    HIPCHK(hipMalloc((void**)&Q.act, sizeof(int) * (size_t)T));
+    HIPCHK(hipHostMalloc((void**)&Q.h_act, sizeof(int) * (size_t)T, hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(Q.stream));  // host staging vectors die at return
+    BDev& B = Q.B;
+    B.tile_s0 = Q.d_tile_s0;
+    B.tile_chart = Q.d_tile_chart;
+    B.chart_t0 = Q.d_chart_t0;
+    B.tile_r0 = Q.d_tile_r0;
+    B.tile_r1 = Q.d_tile_r1;
+    B.part_a = Q.d_part_a;
+    B.part_b = Q.d_part_b;
+    B.part_c = Q.d_part_c;
+    B.part_o = Q.d_part_o;
+    B.ntile = T;
+    return RUV_OK;
+}
+
 int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* cols, const double* vals,
                 int nchart, const int64_t* crow, int R, int precond, int mode) {
     Q.R = R;
@@ -369,101 +193,135 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     Q.nchart = nchart;
     Q.nrows = nrows;
     Q.nnz = ptr[nrows];
+    Q.crow.assign(crow, crow + nchart + 1);
     HIPCHK(hipStreamCreateWithFlags(&Q.stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&Q.ev0));
     HIPCHK(hipEventCreate(&Q.ev1));
-    // tiles
-    std::vector<int> tile_chart;
-    std::vector<int64_t> tr0, tr1;
-    Q.chart_t0.assign(nchart + 1, 0);
+    // slices: each chart's rows in runs of at most SLICE
+    std::vector<int64_t> sl_off;
+    std::vector<int32_t> sl_w;
+    Q.chart_s0.assign(nchart + 1, 0);
+    int64_t nell = 0;
     for (int c = 0; c < nchart; ++c) {
-        Q.chart_t0[c] = (int)tile_chart.size();
-        for (int64_t r = crow[c]; r < crow[c + 1]; r += TILE) {
-            tile_chart.push_back(c);
-            tr0.push_back(r);
-            tr1.push_back(std::min<int64_t>(r + TILE, crow[c + 1]));
-        }
-        if (crow[c + 1] == crow[c]) {  // an empty chart still owns one (empty) tile so its scalars exist
-            tile_chart.push_back(c);
-            tr0.push_back(crow[c]);
-            tr1.push_back(crow[c]);
+        Q.chart_s0[c] = (int)Q.h_sl_row0.size();
+        for (int64_t r = crow[c]; r < crow[c + 1]; r += SLICE) {
+            const int n = (int)std::min<int64_t>(SLICE, crow[c + 1] - r);
+            int w = 0;
+            for (int64_t i = r; i < r + n; ++i) w = std::max<int>(w, (int)(ptr[i + 1] - ptr[i]));
+            Q.h_sl_row0.push_back(r);
+            Q.h_sl_n.push_back(n);
+            sl_w.push_back(w);
+            sl_off.push_back(nell);
+            nell += (int64_t)w * n;
         }
     }
-    Q.chart_t0[nchart] = (int)tile_chart.size();
-    Q.ntile = (int)tile_chart.size();
-    const int T = Q.ntile;
-    const size_t N = (size_t)nrows * R;
-    int64_t *d_ptr = nullptr, *d_r0 = nullptr, *d_r1 = nullptr;
-    int *d_cols = nullptr, *d_tc = nullptr, *d_ct0 = nullptr;
-    HIPCHK(Q.upload(&d_ptr, ptr, (size_t)nrows + 1));
-    HIPCHK(Q.upload(&d_cols, cols, (size_t)Q.nnz));
-    HIPCHK(Q.upload(&d_tc, tile_chart.data(), tile_chart.size()));
-    HIPCHK(Q.upload(&d_r0, tr0.data(), tr0.size()));
-    HIPCHK(Q.upload(&d_r1, tr1.data(), tr1.size()));
-    HIPCHK(Q.upload(&d_ct0, Q.chart_t0.data(), Q.chart_t0.size()));
-    HIPCHK(Q.upload(&Q.vals64, vals, (size_t)Q.nnz));
-    std::vector<float> v32;
+    Q.chart_s0[nchart] = (int)Q.h_sl_row0.size();
+    Q.nslice = (int)Q.h_sl_row0.size();
+    Q.nell = nell;
+    // 16-bit column offsets when every column is within +-32767 rows of its row
+    bool narrow = true;
+    for (int64_t i = 0; i < nrows && narrow; ++i)
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
+            if (cols[k] - i > 32767 || cols[k] - i < -32767) {
+                narrow = false;
+                break;
+            }
+    Q.ci = narrow ? 0 : 1;
+    // fill the sliced ELL (and the Jacobi diagonal) on the host threads
+    std::vector<double> e64((size_t)nell, 0.0);
+    std::vector<int16_t> c16(narrow ? (size_t)nell : 0, 0);
+    std::vector<int32_t> c32(narrow ? 0 : (size_t)nell, 0);
+    std::vector<double> dinv(precond == RUV_PC_JACOBI ? (size_t)nrows : 0, 0.0);
+    std::vector<int64_t> bad(1, -1);
+    const int nsl = Q.nslice;
+    parallel_for(nsl, [&](int64_t s0, int64_t s1) {
+        for (int64_t s = s0; s < s1; ++s) {
+            const int64_t r0 = Q.h_sl_row0[s];
+            const int n = Q.h_sl_n[s];
+            for (int l = 0; l < n; ++l) {
+                const int64_t i = r0 + l;
+                int k = 0;
+                double d = 0.0;
+                for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a, ++k) {
+                    const int64_t e = sl_off[s] + (int64_t)k * n + l;
+                    e64[e] = vals[a];
+                    if (narrow) c16[e] = (int16_t)(cols[a] - i);
+                    else c32[e] = cols[a];
+                    if (cols[a] == i) d = vals[a];
+                }
+                for (; k < sl_w[s]; ++k) {  // padding: (0, the row itself)
+                    const int64_t e = sl_off[s] + (int64_t)k * n + l;
+                    if (!narrow) c32[e] = (int32_t)i;
+                }
+                if (!dinv.empty()) {
+                    if (std::fabs(d) < 1e-300) bad[0] = i;
+                    else dinv[i] = 1.0 / d;
+                }
+            }
+        }
+    });
+    if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(bad[0]));
+    int64_t *d_row0 = nullptr, *d_off = nullptr;
+    int32_t *d_w = nullptr, *d_n = nullptr;
+    void* d_col = nullptr;
+    HIPCHK(Q.upload(&d_row0, Q.h_sl_row0.data(), Q.h_sl_row0.size()));
+    HIPCHK(Q.upload(&d_off, sl_off.data(), sl_off.size()));
+    HIPCHK(Q.upload(&d_w, sl_w.data(), sl_w.size()));
+    HIPCHK(Q.upload(&d_n, Q.h_sl_n.data(), Q.h_sl_n.size()));
+    if (narrow) {
+        int16_t* p = nullptr;
+        HIPCHK(Q.upload(&p, c16.data(), c16.size()));
+        d_col = p;
+    } else {
+        int32_t* p = nullptr;
+        HIPCHK(Q.upload(&p, c32.data(), c32.size()));
+        d_col = p;
+    }
+    HIPCHK(Q.upload(&Q.ev64, e64.data(), e64.size()));
+    std::vector<float> f32;
     if (mode == RUV_MODE_MIXED) {
-        v32.assign(vals, vals + Q.nnz);
-        HIPCHK(Q.upload(&Q.vals32, v32.data(), v32.size()));
+        f32.resize(e64.size());
+        parallel_for((int64_t)e64.size(), [&](int64_t a, int64_t b) {
+            for (int64_t k = a; k < b; ++k) f32[k] = (float)e64[k];
+        });
+        HIPCHK(Q.upload(&Q.ev32, f32.data(), f32.size()));
     }
-    std::vector<double> dinv;
     std::vector<float> dinv32;
-    if (precond == RUV_PC_JACOBI) {
-        dinv.assign(nrows, 0.0);
-        for (int64_t i = 0; i < nrows; ++i) {
-            double d = 0.0;
-            for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
-                if (cols[k] == i) d = vals[k];
-            if (std::fabs(d) < 1e-300) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(i));
-            dinv[i] = 1.0 / d;
-        }
+    if (!dinv.empty()) {
         HIPCHK(Q.upload(&Q.dinv64, dinv.data(), dinv.size()));
         if (mode == RUV_MODE_MIXED) {
             dinv32.assign(dinv.begin(), dinv.end());
             HIPCHK(Q.upload(&Q.dinv32, dinv32.data(), dinv32.size()));
         }
     }
+    const size_t N = (size_t)nrows * R;
+    const size_t sv = mode == RUV_MODE_MIXED ? sizeof(float) : sizeof(double);
     HIPCHK(Q.alloc(&Q.r64, N));
-    if (mode == RUV_MODE_MIXED) {
-        HIPCHK(Q.alloc(&Q.xf, N));
-        HIPCHK(Q.alloc(&Q.rf, N));
-        HIPCHK(Q.alloc(&Q.zf, N));
-        HIPCHK(Q.alloc(&Q.pf, N));
-        HIPCHK(Q.alloc(&Q.qf, N));
-    } else {
-        HIPCHK(Q.alloc(&Q.xd, N));
-        HIPCHK(Q.alloc(&Q.rd, N));
-        HIPCHK(Q.alloc(&Q.zd, N));
-        HIPCHK(Q.alloc(&Q.pd, N));
-        HIPCHK(Q.alloc(&Q.qd, N));
+    for (void** v : {&Q.xv, &Q.rv, &Q.zv, &Q.pv0, &Q.pv1, &Q.qv}) {
+        char* p = nullptr;
+        HIPCHK(Q.alloc(&p, N * sv));
+        *v = p;
     }
     BDev& B = Q.B;
-    B.ptr = d_ptr;
-    B.cols = d_cols;
-    B.tile_chart = d_tc;
-    B.tile_r0 = d_r0;
-    B.tile_r1 = d_r1;
-    B.chart_t0 = d_ct0;
+    B.sl_row0 = d_row0;
+    B.sl_off = d_off;
+    B.sl_w = d_w;
+    B.sl_n = d_n;
+    B.ecol = d_col;
     B.nchart = nchart;
-    HIPCHK(Q.alloc(&B.c_rz, nchart));
+    HIPCHK(Q.alloc(&B.c_rz, 2 * (size_t)nchart));
     HIPCHK(Q.alloc(&B.c_tol, nchart));
     HIPCHK(Q.alloc(&B.c_bnorm, nchart));
     HIPCHK(Q.alloc(&B.c_true, nchart));
     HIPCHK(Q.alloc(&B.c_done, nchart));
     HIPCHK(Q.alloc(&B.c_final, nchart));
     HIPCHK(Q.alloc(&B.c_iters, nchart));
-    HIPCHK(Q.alloc(&B.part_a, T));
-    HIPCHK(Q.alloc(&B.part_b, T));
-    HIPCHK(Q.alloc(&B.part_c, T));
-    HIPCHK(Q.alloc(&Q.act, T));
-    HIPCHK(hipHostMalloc((void**)&Q.h_act, sizeof(int) * (size_t)T, hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_done, sizeof(int) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_final, sizeof(int) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_iters, sizeof(unsigned) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_true, sizeof(double) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(Q.stream));  // staging vectors above must outlive their copies
-    return RUV_OK;
+    return build_tiles(Q, 2);
 }
 
 // upload the tiles of the charts whose flag[c] satisfies pred; returns the count
@@ -478,33 +336,51 @@ int set_active(BatchImpl& Q, Pred pred, int* count) {
     return RUV_OK;
 }
 
-template <typename T, int R, int PC>
-int inner_pcg(BatchImpl& Q, const T* vals, const T* dinv, T* x, T* r, T* z, T* p, T* q, double tol, uint64_t max_iter) {
+inline dim3 grid_of(int na) { return dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)); }
+
+template <typename T, int R, int CI>
+void launch_dir_spmv(BatchImpl& Q, int na, int par, uint32_t iter, const T* z, const T* pold, T* pnew, T* q) {
+    const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
+    hipLaunchKernelGGL((kb_dir_spmv<T, R, CI>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B, (const int*)Q.act, na, par,
+                       iter, vals, z, pold, pnew, q);
+}
+
+template <typename T, int R, int PC, int CI>
+int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
     hipStream_t s = Q.stream;
     BDev& B = Q.B;
     B.max_iter = (unsigned)std::min<uint64_t>(max_iter, 0xffffffffu);
+    const T* dinv = (sizeof(T) == 4) ? (const T*)(const void*)Q.dinv32 : (const T*)(const void*)Q.dinv64;
+    T* x = (T*)Q.xv;
+    T* r = (T*)Q.rv;
+    T* z = (PC == 1) ? (T*)Q.zv : r;
+    T* pb[2] = {(T*)Q.pv0, (T*)Q.pv1};
+    T* q = (T*)Q.qv;
     int na = 0;
     int rc = set_active(Q, [&](int c) { return Q.h_final[c] == 0; }, &na);
     if (rc) return rc;
     if (na == 0) return RUV_OK;
-    hipLaunchKernelGGL((kb_prep<T, R, PC>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, (const double*)Q.r64,
-                       dinv, x, r, z, p);
+    // step 0 reads its old direction from pb[1]
+    hipLaunchKernelGGL((kb_prep<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
+                       (const double*)Q.r64, dinv, x, r, z, pb[1]);
     hipLaunchKernelGGL(kb_init, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol, Q.inner_tol,
                        Q.mode == RUV_MODE_MIXED ? 1 : 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     int chunk = 8;
-    const T* zz = (PC == 0) ? (const T*)r : (const T*)z;
+    uint32_t iter = 0;
+    int par = 0;
     for (;;) {
         rc = set_active(Q, [&](int c) { return Q.h_done[c] == B_RUNNING; }, &na);
         if (rc) return rc;
         if (na == 0) break;
         for (int it = 0; it < chunk; ++it) {
-            hipLaunchKernelGGL((kb_spmv<T, R>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, vals, (const T*)p, q);
-            hipLaunchKernelGGL((kb_update<T, R, PC>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, dinv,
-                               (const T*)p, (const T*)q, x, r, z);
-            hipLaunchKernelGGL((kb_pupdate<T, R, PC>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, zz, p);
+            launch_dir_spmv<T, R, CI>(Q, na, par, iter, z, pb[par ^ 1], pb[par], q);
+            hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
+                               dinv, (const T*)pb[par], (const T*)q, x, r, z);
+            par ^= 1;
+            ++iter;
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
@@ -524,18 +400,11 @@ int inner_pcg(BatchImpl& Q, const T* vals, const T* dinv, T* x, T* r, T* z, T* p
     return RUV_OK;
 }
 
-template <typename T, int R, int PC>
+template <typename T, int R, int PC, int CI>
 int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, double tol) {
     hipStream_t s = Q.stream;
     BDev& B = Q.B;
     const bool mixed = Q.mode == RUV_MODE_MIXED;
-    const T* vals = mixed ? (const T*)(const void*)Q.vals32 : (const T*)(const void*)Q.vals64;
-    const T* dinv = mixed ? (const T*)(const void*)Q.dinv32 : (const T*)(const void*)Q.dinv64;
-    T* x = mixed ? (T*)(void*)Q.xf : (T*)(void*)Q.xd;
-    T* r = mixed ? (T*)(void*)Q.rf : (T*)(void*)Q.rd;
-    T* z = mixed ? (T*)(void*)Q.zf : (T*)(void*)Q.zd;
-    T* p = mixed ? (T*)(void*)Q.pf : (T*)(void*)Q.pd;
-    T* q = mixed ? (T*)(void*)Q.qf : (T*)(void*)Q.qd;
     Q.iters_total.assign(Q.nchart, 0);
     hipLaunchKernelGGL((kb_start<R>), dim3(Q.ntile), dim3(BLK), 0, s, B, d_b, d_x, Q.r64);
     hipLaunchKernelGGL(kb_bnorm, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B);
@@ -544,15 +413,16 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
     int na = 0;
     for (int round = 0;; ++round) {
         if (round >= Q.max_rounds) return fail(RUV_ERR_NOT_CONVERGED, "batch: refinement did not converge");
-        int rc = inner_pcg<T, R, PC>(Q, vals, dinv, x, r, z, p, q, tol, max_iter);
+        int rc = inner_pcg<T, R, PC, CI>(Q, tol, max_iter);
         if (rc) return rc;
         rc = set_active(Q, [&](int c) { return Q.h_final[c] == 0; }, &na);
         if (rc) return rc;
         if (na == 0) break;
         // x64 += x_inner over the live charts, then the FP64 residual and the per-chart outer test
-        hipLaunchKernelGGL((kb_axpy<T, R>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, (const T*)x, d_x);
-        hipLaunchKernelGGL((kb_residual<R>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q.act, (const double*)Q.vals64,
-                           d_b, (const double*)d_x, Q.r64);
+        hipLaunchKernelGGL((kb_axpy<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
+                           (const T*)Q.xv, d_x);
+        hipLaunchKernelGGL((kb_residual<R, CI>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
+                           (const double*)Q.ev64, d_b, (const double*)d_x, Q.r64);
         hipLaunchKernelGGL(kb_outer, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_final, B.c_final, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
@@ -565,6 +435,83 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
     HIPCHK(hipMemcpyAsync(Q.h_true, B.c_true, sizeof(double) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return RUV_OK;
+}
+
+template <typename T, int R, int CI>
+int run_pc(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, double tol) {
+    return Q.precond == RUV_PC_JACOBI ? batch_run<T, R, 1, CI>(Q, d_b, d_x, max_iter, tol)
+                                      : batch_run<T, R, 0, CI>(Q, d_b, d_x, max_iter, tol);
+}
+template <typename T, int R>
+int run_ci(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, double tol) {
+    return Q.ci == 0 ? run_pc<T, R, 0>(Q, d_b, d_x, max_iter, tol) : run_pc<T, R, 1>(Q, d_b, d_x, max_iter, tol);
+}
+
+// The step kernel over every tile with never-converging state: mean seconds per launch.
+template <typename T, int R, int CI>
+int time_step_kernel(BatchImpl& Q, int reps, double* sec) {
+    hipStream_t s = Q.stream;
+    BDev& B = Q.B;
+    B.max_iter = 0xffffffffu;
+    const int C = Q.nchart;
+    std::vector<int> zeros(C, 0);
+    std::vector<double> ones(2 * (size_t)C, 1.0), never(C, -1.0);
+    HIPCHK(hipMemcpyAsync(B.c_done, zeros.data(), sizeof(int) * C, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B.c_rz, ones.data(), sizeof(double) * 2 * C, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B.c_tol, never.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    std::vector<double> pones(2 * (size_t)Q.ntile, 1.0);
+    HIPCHK(hipMemcpyAsync(B.part_b, pones.data(), sizeof(double) * pones.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(B.part_c, pones.data(), sizeof(double) * pones.size(), hipMemcpyHostToDevice, s));
+    const size_t N = (size_t)Q.nrows * Q.R * sizeof(T);
+    for (void* v : {Q.zv, Q.pv0, Q.pv1, Q.qv}) HIPCHK(hipMemsetAsync(v, 0, N, s));
+    int na = 0;
+    int rc = set_active(Q, [](int) { return true; }, &na);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(s));
+    launch_dir_spmv<T, R, CI>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(Q.ev0, s));
+    for (int i = 0; i < reps; ++i)
+        launch_dir_spmv<T, R, CI>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+    HIPCHK(hipEventRecord(Q.ev1, s));
+    HIPCHK(hipEventSynchronize(Q.ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, Q.ev0, Q.ev1));
+    *sec = ms * 1e-3 / reps;
+    return RUV_OK;
+}
+
+int time_step(BatchImpl& Q, int reps, double* sec) {
+    const bool f = Q.mode == RUV_MODE_MIXED;
+    if (Q.R == 2) {
+        if (f) return Q.ci == 0 ? time_step_kernel<float, 2, 0>(Q, reps, sec) : time_step_kernel<float, 2, 1>(Q, reps, sec);
+        return Q.ci == 0 ? time_step_kernel<double, 2, 0>(Q, reps, sec) : time_step_kernel<double, 2, 1>(Q, reps, sec);
+    }
+    if (f) return Q.ci == 0 ? time_step_kernel<float, 1, 0>(Q, reps, sec) : time_step_kernel<float, 1, 1>(Q, reps, sec);
+    return Q.ci == 0 ? time_step_kernel<double, 1, 0>(Q, reps, sec) : time_step_kernel<double, 1, 1>(Q, reps, sec);
+}
+
+// Pick slices-per-tile by timing the step kernel over the whole batch.
+int autotune(BatchImpl& Q) {
+    const char* env = std::getenv("RUV_BATCH_SLICES_PER_TILE");
+    if (env) {
+        const int v = std::atoi(env);
+        if (v == 1 || v == 2 || v == 4 || v == 8) return build_tiles(Q, v);
+    }
+    int best = 2;
+    double best_t = 1e300;
+    for (int spt : {1, 2, 4, 8}) {
+        int rc = build_tiles(Q, spt);
+        if (rc) return rc;
+        double t = 0.0;
+        rc = time_step(Q, 3, &t);
+        if (rc) return rc;
+        if (t < best_t) {
+            best_t = t;
+            best = spt;
+        }
+    }
+    return Q.spt == best ? RUV_OK : build_tiles(Q, best);
 }
 
 }  // namespace
@@ -580,6 +527,8 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     std::unique_ptr<BatchImpl> Q(new BatchImpl());
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
+    rc = autotune(*Q);
+    if (rc) return rc;
     *out = Q.release();
     return RUV_OK;
 }
@@ -588,26 +537,22 @@ void batch_destroy(BatchImpl* q) { delete q; }
 
 int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid) {
+    const uintptr_t align = sizeof(double) * (uintptr_t)Q->R;  // rows are loaded as R-wide vectors
+    if (((uintptr_t)d_b % align) != 0 || ((uintptr_t)d_x % align) != 0)
+        return fail(RUV_ERR_INVALID, "device vectors must be aligned to rhs_per_row * 8 bytes");
     HIPCHK(hipEventRecord(Q->ev0, Q->stream));
     int rc;
-    const bool j = Q->precond == RUV_PC_JACOBI;
-    if (Q->mode == RUV_MODE_MIXED) {
-        if (Q->R == 2) rc = j ? batch_run<float, 2, 1>(*Q, d_b, d_x, max_iter, tol) : batch_run<float, 2, 0>(*Q, d_b, d_x, max_iter, tol);
-        else rc = j ? batch_run<float, 1, 1>(*Q, d_b, d_x, max_iter, tol) : batch_run<float, 1, 0>(*Q, d_b, d_x, max_iter, tol);
-    } else {
-        if (Q->R == 2) rc = j ? batch_run<double, 2, 1>(*Q, d_b, d_x, max_iter, tol) : batch_run<double, 2, 0>(*Q, d_b, d_x, max_iter, tol);
-        else rc = j ? batch_run<double, 1, 1>(*Q, d_b, d_x, max_iter, tol) : batch_run<double, 1, 0>(*Q, d_b, d_x, max_iter, tol);
-    }
+    if (Q->mode == RUV_MODE_MIXED) rc = Q->R == 2 ? run_ci<float, 2>(*Q, d_b, d_x, max_iter, tol) : run_ci<float, 1>(*Q, d_b, d_x, max_iter, tol);
+    else rc = Q->R == 2 ? run_ci<double, 2>(*Q, d_b, d_x, max_iter, tol) : run_ci<double, 1>(*Q, d_b, d_x, max_iter, tol);
     if (rc) return rc;
     HIPCHK(hipEventRecord(Q->ev1, Q->stream));
     HIPCHK(hipEventSynchronize(Q->ev1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, Q->ev0, Q->ev1));
     if (info) {
-        uint64_t tot = 0, mx = 0;
+        uint64_t mx = 0;
         double worst = 0.0;
         for (int c = 0; c < Q->nchart; ++c) {
-            tot += Q->iters_total[c];
             mx = std::max<uint64_t>(mx, Q->iters_total[c]);
             worst = std::max(worst, Q->h_true[c]);
         }
@@ -616,7 +561,6 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
         info->precond = Q->precond;
         info->true_rel_resid = worst;
         info->seconds = ms * 1e-3;
-        (void)tot;
     }
     if (chart_iters)
         for (int c = 0; c < Q->nchart; ++c) chart_iters[c] = Q->iters_total[c];
@@ -625,50 +569,24 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
     return RUV_OK;
 }
 
+// Mean duration and algorithmic bytes of the step kernel (direction + SpMV) over the whole batch.  Bytes:
+// the sliced-ELL values and column offsets as stored (padding included), the slice table, and four vectors
+// streamed once (z, old p read; new p, q written); neighbour gathers are served by the caches.
 int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
     if (reps <= 0) return fail(RUV_ERR_INVALID, "reps must be positive");
-    hipStream_t s = Q->stream;
-    BDev B = Q->B;
-    B.max_iter = 0xffffffffu;
-    std::vector<int> zeros(Q->nchart, 0);
-    HIPCHK(hipMemcpyAsync(B.c_done, zeros.data(), sizeof(int) * Q->nchart, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(B.c_iters, zeros.data(), sizeof(int) * Q->nchart, hipMemcpyHostToDevice, s));
-    int na = 0;
-    int rc = set_active(*Q, [](int) { return true; }, &na);
+    int rc = time_step(*Q, reps, sec);
     if (rc) return rc;
-    const bool f32 = Q->mode == RUV_MODE_MIXED;
-    const int64_t N = Q->nrows * Q->R;
-    std::vector<double> ones_d;
-    std::vector<float> ones_f;
-    if (f32) {
-        ones_f.assign(N, 1.0f);
-        HIPCHK(hipMemcpyAsync(Q->pf, ones_f.data(), sizeof(float) * N, hipMemcpyHostToDevice, s));
-    } else {
-        ones_d.assign(N, 1.0);
-        HIPCHK(hipMemcpyAsync(Q->pd, ones_d.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
-    }
-    HIPCHK(hipMemsetAsync(B.part_c, 0, sizeof(double) * Q->ntile, s));
-    HIPCHK(hipStreamSynchronize(s));
-    auto launch = [&]() {
-        if (f32) {
-            if (Q->R == 2) hipLaunchKernelGGL((kb_spmv<float, 2>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q->act, (const float*)Q->vals32, (const float*)Q->pf, Q->qf);
-            else hipLaunchKernelGGL((kb_spmv<float, 1>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q->act, (const float*)Q->vals32, (const float*)Q->pf, Q->qf);
-        } else {
-            if (Q->R == 2) hipLaunchKernelGGL((kb_spmv<double, 2>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q->act, (const double*)Q->vals64, (const double*)Q->pd, Q->qd);
-            else hipLaunchKernelGGL((kb_spmv<double, 1>), dim3(na), dim3(BLK), 0, s, B, (const int*)Q->act, (const double*)Q->vals64, (const double*)Q->pd, Q->qd);
-        }
-    };
-    launch();
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(Q->ev0, s));
-    for (int i = 0; i < reps; ++i) launch();
-    HIPCHK(hipEventRecord(Q->ev1, s));
-    HIPCHK(hipEventSynchronize(Q->ev1));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, Q->ev0, Q->ev1));
-    *sec = ms * 1e-3 / reps;
-    const double sv = f32 ? 4.0 : 8.0;
-    *bytes = (double)Q->nnz * (sv + 4.0) + (double)(Q->nrows + 1) * 8.0 + 2.0 * (double)N * sv;
+    const double sv = Q->mode == RUV_MODE_MIXED ? 4.0 : 8.0;
+    const double sc = Q->ci == 0 ? 2.0 : 4.0;
+    *bytes = (double)Q->nell * (sv + sc) + (double)Q->nslice * 24.0 + (double)Q->ntile * 28.0 +
+             4.0 * (double)Q->nrows * Q->R * sv;
+    return RUV_OK;
+}
+
+int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries) {
+    if (rows_per_tile) *rows_per_tile = Q->spt * SLICE;
+    if (col_bytes) *col_bytes = Q->ci == 0 ? 2 : 4;
+    if (ell_entries) *ell_entries = Q->nell;
     return RUV_OK;
 }
 

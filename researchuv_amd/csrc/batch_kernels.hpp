@@ -1,0 +1,415 @@
+// batch_kernels.hpp -- device side of the batched solver: sliced-ELL storage and a two-kernel PCG iteration.
+//
+// Storage.  Every chart is one diagonal block; its rows are cut into slices of at most SLICE (= BLK) rows, one
+// row per thread.  Slice s holds rows [sl_row0, sl_row0 + n_s) padded to the slice's longest row w_s (diagonal
+// included); entry k of row lane sits at sl_off + k * n_s + lane, so every load of a slice is one coalesced
+// sweep.  Columns are stored as 16-bit offsets from the row (mesh charts keep neighbours close) or, when any
+// offset does not fit, as 32-bit rows.  Padding entries are (value 0, offset 0).  A tile is a run of slices of
+// one chart; one workgroup owns one tile.
+//
+// Iteration.  With z = M r (Jacobi) the reference recurrence (solver.rs:405-552)
+//     alpha = rz / pAp;  x += alpha p;  r -= alpha q;  z = M r;  beta = rz' / rz;  p = z + beta p
+// needs two chart-wide reductions per step, so it runs as two kernels:
+//   kb_dir_spmv  opens the step (convergence on ||r||, beta from the r.z partials), forms the new direction
+//                p' = z + beta p for its own rows AND for every neighbour it gathers (from z and the old p, so no
+//                third kernel and no extra pass), q = A p', partials p'.q;
+//   kb_update    alpha = rz / p'.q; x, r, z updated; partials r.r and r.z.
+// The old and new directions live in two buffers swapped every step, and the per-step scalars (r.r and r.z
+// partials, chart r.z) are double-buffered by step parity, so no workgroup reads a value another workgroup of
+// the same launch writes.  All workgroups of a chart sum the same partials in the same order, so they take the
+// same decisions and the recurrence of each chart is exactly an independent PCG.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dev_common.hpp"
+
+namespace ruv {
+
+constexpr int SLICE = BLK;  // rows per slice (one per thread)
+constexpr int XCDS = 8;     // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
+
+enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4 };
+
+struct BDev {
+    // sliced ELL (per slice)
+    const int64_t* sl_row0;
+    const int64_t* sl_off;
+    const int32_t* sl_w;
+    const int32_t* sl_n;
+    const void* ecol;  // int16 offsets (CI = 0) or int32 rows (CI = 1)
+    // tiles: tile t = slices [tile_s0[t], tile_s0[t + 1]) = rows [tile_r0[t], tile_r1[t]) of chart tile_chart[t]
+    const int32_t* tile_s0;
+    const int64_t* tile_r0;
+    const int64_t* tile_r1;
+    const int32_t* tile_chart;
+    const int32_t* chart_t0;  // nchart + 1
+    // per chart
+    double* c_rz;  // [2][nchart] by step parity
+    double* c_tol;
+    double* c_bnorm;
+    double* c_true;
+    int32_t* c_done;
+    int32_t* c_final;
+    uint32_t* c_iters;
+    // per tile partials
+    double* part_a;  // p.q
+    double* part_b;  // [2][ntile] r.r by parity
+    double* part_c;  // [2][ntile] r.z by parity
+    double* part_o;  // outer FP64 residual / ||b||
+    uint32_t max_iter;
+    int32_t nchart;
+    int32_t ntile;
+};
+
+template <typename T, int R>
+struct alignas(sizeof(T) * R) Vr {
+    T c[R];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ Vr<T, R> ld(const T* __restrict__ p, int64_t row) {
+    return *reinterpret_cast<const Vr<T, R>*>(p + row * R);
+}
+template <typename T, int R>
+__device__ __forceinline__ void st(T* __restrict__ p, int64_t row, const Vr<T, R>& v) {
+    *reinterpret_cast<Vr<T, R>*>(p + row * R) = v;
+}
+
+template <int CI>
+__device__ __forceinline__ int64_t col_of(const void* __restrict__ ec, int64_t e, int64_t row) {
+    if constexpr (CI == 0) return row + (int64_t)static_cast<const int16_t*>(ec)[e];
+    else return (int64_t) static_cast<const int32_t*>(ec)[e];
+}
+
+// Workgroup -> slot of the active-tile list such that each XCD receives one contiguous run of slots (tiles of a
+// chart are contiguous, so a chart's rows, neighbours and partials stay in one L2).  Returns -1 for the few
+// padding workgroups of the last run.
+__device__ __forceinline__ int xcd_slot(int na) {
+    const int per = (na + XCDS - 1) / XCDS;
+    const int b = (int)blockIdx.x;
+    const int slot = (b % XCDS) * per + b / XCDS;
+    return slot < na ? slot : -1;
+}
+
+// sums of two partial arrays over [a, b), fixed order, identical in every workgroup
+__device__ __forceinline__ void sum_range2(const double* __restrict__ pa, const double* __restrict__ pb, int64_t a,
+                                           int64_t b, double* sh, double* sa, double* sb) {
+    double va = 0.0, vb = 0.0;
+    for (int64_t i = a + threadIdx.x; i < b; i += BLK) {
+        va += pa[i];
+        vb += pb[i];
+    }
+    va = wave_sum(va);
+    vb = wave_sum(vb);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = va;
+        sh[WAVES + (threadIdx.x >> 6)] = vb;
+    }
+    __syncthreads();
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) {
+        s0 += sh[i];
+        s1 += sh[WAVES + i];
+    }
+    *sa = s0;
+    *sb = s1;
+}
+
+// the new search direction of one row; used for own rows and for gathered neighbours alike, so both see
+// bit-identical values
+template <typename T>
+__device__ __forceinline__ T dir(T z, T p, T beta) {
+    return fma(beta, p, z);
+}
+
+// Opens step `iter` (parity par) of every live chart, then p' = z + beta p and q = A p' over one tile.
+template <typename T, int R, int CI>
+__global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict__ act, int na, int par,
+                                                   uint32_t iter, const T* __restrict__ vals,
+                                                   const T* __restrict__ z, const T* __restrict__ pold,
+                                                   T* __restrict__ pnew, T* __restrict__ q) {
+    __shared__ double sh[2 * WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    const int c = B.tile_chart[t];
+    if (B.c_done[c] != B_RUNNING) return;
+    const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
+    double rr, rz;
+    sum_range2(B.part_b + (int64_t)par * B.ntile, B.part_c + (int64_t)par * B.ntile, t0, t1, sh, &rr, &rz);
+    const bool lead = (t == t0) && threadIdx.x == 0;
+    if (sqrt(rr) <= B.c_tol[c]) {
+        if (lead) B.c_done[c] = B_CONVERGED;
+        return;
+    }
+    if (iter >= B.max_iter) {
+        if (lead) B.c_done[c] = B_MAXITER;
+        return;
+    }
+    const double rzold = B.c_rz[(int64_t)(par ^ 1) * B.nchart + c];
+    if (!isfinite(rz) || !(fabs(rzold) > 0.0)) {
+        if (lead) B.c_done[c] = B_BREAKDOWN;
+        return;
+    }
+    if (lead) {
+        B.c_rz[(int64_t)par * B.nchart + c] = rz;
+        B.c_iters[c] = iter + 1;
+    }
+    const T beta = (T)(rz / rzold);
+    double acc = 0.0;
+    const int s1 = B.tile_s0[t + 1];
+    for (int s = B.tile_s0[t]; s < s1; ++s) {
+        const int n = B.sl_n[s];
+        if ((int)threadIdx.x >= n) continue;
+        const int64_t i = B.sl_row0[s] + threadIdx.x;
+        const int64_t e0 = B.sl_off[s] + threadIdx.x;
+        const int w = B.sl_w[s];
+        T acc_q[R];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc_q[cc] = T(0);
+#pragma unroll 4
+        for (int k = 0; k < w; ++k) {
+            const int64_t e = e0 + (int64_t)k * n;
+            const T v = vals[e];
+            const int64_t j = col_of<CI>(B.ecol, e, i);
+            const Vr<T, R> zj = ld<T, R>(z, j);
+            const Vr<T, R> pj = ld<T, R>(pold, j);
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
+        }
+        const Vr<T, R> zi = ld<T, R>(z, i);
+        const Vr<T, R> pi = ld<T, R>(pold, i);
+        Vr<T, R> pn, qv;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            pn.c[cc] = dir(zi.c[cc], pi.c[cc], beta);
+            qv.c[cc] = acc_q[cc];
+            acc += (double)pn.c[cc] * (double)acc_q[cc];
+        }
+        st<T, R>(pnew, i, pn);
+        st<T, R>(q, i, qv);
+    }
+    const double s = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) B.part_a[t] = s;
+}
+
+// alpha = rz / p.q;  x += alpha p;  r -= alpha q;  z = D^-1 r (PC 1; z aliases r for PC 0);
+// partials r.r and r.z for step parity par ^ 1.
+template <typename T, int R, int PC>
+__global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__ act, int na, int par,
+                                                 const T* __restrict__ dinv, const T* __restrict__ p,
+                                                 const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
+                                                 T* __restrict__ z) {
+    __shared__ double sh[2 * WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    const int c = B.tile_chart[t];
+    if (B.c_done[c] != B_RUNNING) return;
+    const int t0 = B.chart_t0[c];
+    const double pq = sum_range(B.part_a, t0, B.chart_t0[c + 1], sh);
+    if (!(fabs(pq) > 1e-300) || !isfinite(pq)) {
+        if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_BREAKDOWN;
+        return;
+    }
+    const T a = (T)(B.c_rz[(int64_t)par * B.nchart + c] / pq);
+    double arr = 0.0, arz = 0.0;
+    const int64_t r1 = B.tile_r1[t];
+    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
+        const T d = (PC == 1) ? dinv[i] : T(1);
+        const Vr<T, R> pi = ld<T, R>(p, i), qi = ld<T, R>(q, i);
+        Vr<T, R> xi = ld<T, R>(x, i), ri = ld<T, R>(r, i), zi;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            xi.c[cc] = fma(a, pi.c[cc], xi.c[cc]);
+            ri.c[cc] = fma(-a, qi.c[cc], ri.c[cc]);
+            zi.c[cc] = d * ri.c[cc];
+            arr += (double)ri.c[cc] * (double)ri.c[cc];
+            arz += (double)ri.c[cc] * (double)zi.c[cc];
+        }
+        st<T, R>(x, i, xi);
+        st<T, R>(r, i, ri);
+        if (PC == 1) st<T, R>(z, i, zi);
+    }
+    double srr, srz;
+    {
+        double va = wave_sum(arr), vb = wave_sum(arz);
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            sh[threadIdx.x >> 6] = va;
+            sh[WAVES + (threadIdx.x >> 6)] = vb;
+        }
+        __syncthreads();
+        srr = 0.0;
+        srz = 0.0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            srr += sh[w];
+            srz += sh[WAVES + w];
+        }
+    }
+    if (threadIdx.x == 0) {
+        B.part_b[(int64_t)(par ^ 1) * B.ntile + t] = srr;
+        B.part_c[(int64_t)(par ^ 1) * B.ntile + t] = (PC == 1) ? srz : srr;
+    }
+}
+
+// inner start on the live charts: x = 0, r = T(r64), z = D^-1 r, p = 0 (the first step's old direction);
+// partials r.r / r.z for parity 0
+template <typename T, int R, int PC>
+__global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ act, int na,
+                                               const double* __restrict__ r64, const T* __restrict__ dinv,
+                                               T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                               T* __restrict__ p) {
+    __shared__ double sh[WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    double arr = 0.0, arz = 0.0;
+    const int64_t r1 = B.tile_r1[t];
+    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
+        const T d = (PC == 1) ? dinv[i] : T(1);
+        Vr<T, R> xi, ri, zi;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            ri.c[cc] = (T)r64[i * R + cc];
+            zi.c[cc] = d * ri.c[cc];
+            xi.c[cc] = T(0);
+            arr += (double)ri.c[cc] * (double)ri.c[cc];
+            arz += (double)ri.c[cc] * (double)zi.c[cc];
+        }
+        st<T, R>(x, i, xi);
+        st<T, R>(r, i, ri);
+        if (PC == 1) st<T, R>(z, i, zi);
+        st<T, R>(p, i, xi);
+    }
+    const double srr = block_sum_all(arr, sh);
+    const double srz = block_sum_all(arz, sh);
+    if (threadIdx.x == 0) {
+        B.part_b[t] = srr;
+        B.part_c[t] = srz;
+    }
+}
+
+// per chart (not final): inner tolerance, step state.  plain: tol * ||b|| on the recursive residual
+// (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
+__global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= B.nchart) return;
+    if (B.c_final[c]) {
+        B.c_done[c] = B_SKIP;
+        return;
+    }
+    double s = 0.0;
+    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
+    const double rn = sqrt(s), bn = B.c_bnorm[c];
+    double ta;
+    if (mixed) ta = fmax(inner_tol * rn, 0.25 * ((bn <= 1e-300) ? tol : tol * bn));
+    else ta = (bn <= 1e-300) ? tol : tol * bn;
+    B.c_tol[c] = ta;
+    B.c_iters[c] = 0;
+    B.c_rz[B.nchart + c] = 1.0;  // "old" r.z of step 0; its direction is zero, so only finiteness matters
+    B.c_done[c] = (rn <= ta) ? B_CONVERGED : B_RUNNING;
+}
+
+// start of a solve: x64 = 0, r64 = b, partials of b.b (every tile)
+template <int R>
+__global__ __launch_bounds__(BLK) void kb_start(BDev B, const double* __restrict__ b, double* __restrict__ x64,
+                                                double* __restrict__ r64) {
+    __shared__ double sh[WAVES];
+    const int t = blockIdx.x;
+    double acc = 0.0;
+    const int64_t r1 = B.tile_r1[t] * R;
+    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1; k += BLK) {
+        const double v = b[k];
+        x64[k] = 0.0;
+        r64[k] = v;
+        acc += v * v;
+    }
+    const double s = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) B.part_o[t] = s;
+}
+
+// per chart: ||b||, not final
+__global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= B.nchart) return;
+    double s = 0.0;
+    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_o[t];
+    B.c_bnorm[c] = sqrt(s);
+    B.c_final[c] = 0;
+    B.c_true[c] = 1.0;
+}
+
+// x64 += x (the inner correction) on the live charts
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kb_axpy(BDev B, const int* __restrict__ act, int na,
+                                               const T* __restrict__ xin, double* __restrict__ x64) {
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    if (B.c_final[B.tile_chart[t]]) return;
+    const int64_t r1 = B.tile_r1[t] * R;
+    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1; k += BLK) x64[k] += (double)xin[k];
+}
+
+// r64 = b - A64 x64 in FP64 on the live charts, partials of r64.r64 (after kb_axpy: a tile's rows read x64 of
+// rows other tiles update)
+template <int R, int CI>
+__global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict__ act, int na,
+                                                   const double* __restrict__ vals64, const double* __restrict__ b,
+                                                   const double* __restrict__ x64, double* __restrict__ r64) {
+    __shared__ double sh[WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    const int c = B.tile_chart[t];
+    if (B.c_final[c]) return;
+    double acc = 0.0;
+    const int s1 = B.tile_s0[t + 1];
+    for (int s = B.tile_s0[t]; s < s1; ++s) {
+        const int n = B.sl_n[s];
+        if ((int)threadIdx.x >= n) continue;
+        const int64_t i = B.sl_row0[s] + threadIdx.x;
+        const int64_t e0 = B.sl_off[s] + threadIdx.x;
+        const int w = B.sl_w[s];
+        double sum[R];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) sum[cc] = 0.0;
+        for (int k = 0; k < w; ++k) {
+            const int64_t e = e0 + (int64_t)k * n;
+            const double v = vals64[e];
+            const Vr<double, R> xj = ld<double, R>(x64, col_of<CI>(B.ecol, e, i));
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) sum[cc] = fma(v, xj.c[cc], sum[cc]);
+        }
+        const Vr<double, R> bi = ld<double, R>(b, i);
+        Vr<double, R> ri;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            ri.c[cc] = bi.c[cc] - sum[cc];
+            acc += ri.c[cc] * ri.c[cc];
+        }
+        st<double, R>(r64, i, ri);
+    }
+    const double ss = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) B.part_o[t] = ss;
+}
+
+// per chart: true residual; final when ||b - A x|| <= tol ||b||
+__global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= B.nchart || B.c_final[c]) return;
+    double s = 0.0;
+    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_o[t];
+    const double rn = sqrt(s), bn = B.c_bnorm[c];
+    B.c_true[c] = (bn > 0.0) ? rn / bn : rn;
+    if (rn <= ((bn <= 1e-300) ? tol : tol * bn)) B.c_final[c] = 1;
+}
+
+}  // namespace ruv

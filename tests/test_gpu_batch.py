@@ -56,9 +56,12 @@ def test_batch_matches_per_chart_lu(orc, mixed):
     assert (it > 0).sum() > 0
 
 
-def test_batch_is_independent_of_composition(orc):
-    """Each chart runs its own PCG: solving a chart alone or inside a batch gives the same iterates."""
+def test_batch_is_independent_of_composition(orc, monkeypatch):
+    """Each chart runs its own PCG: solving a chart alone or inside a batch gives the same iterates (with the
+    tile schedule held fixed, since the tiles decide the order of the partial sums)."""
     from researchuv_amd import batch as B
+
+    monkeypatch.setenv("RUV_BATCH_SLICES_PER_TILE", "2")
 
     meshes = _meshes(6, seed=5)
     lb = B.LscmBatchBuilder(meshes, threads=2)
@@ -71,3 +74,95 @@ def test_batch_is_independent_of_composition(orc):
         xc, _, itc, _ = lbc.device_batch(mixed=False).solve(ac["rhs"], tol=1e-10)
         assert itc[0] == it_all[c]
         np.testing.assert_array_equal(xc, x_all[2 * off[c]:2 * off[c + 1]])
+
+
+def test_batch_schedules_agree(orc, monkeypatch):
+    """Every tile schedule the autotuner can pick solves the same batch to the same tolerance."""
+    from researchuv_amd import batch as B
+
+    meshes = _meshes(10, seed=8)
+    lb = B.LscmBatchBuilder(meshes, threads=2)
+    a = lb.arrays()
+    xs = []
+    for spt in (1, 2, 4, 8):
+        monkeypatch.setenv("RUV_BATCH_SLICES_PER_TILE", str(spt))
+        say(f"schedule {spt} slices per tile")
+        bt = lb.device_batch(mixed=True)
+        assert bt.schedule()["rows_per_tile"] == 256 * spt
+        assert bt.schedule()["col_bytes"] == 2
+        x, info, it, res = bt.solve(a["rhs"], max_iter=20000, tol=1e-10)
+        assert (res <= 1e-10).all()
+        xs.append(x)
+        bt.close()
+    for x in xs[1:]:
+        assert np.abs(x - xs[0]).max() <= 1e-7 * np.abs(xs[0]).max()
+
+
+def test_batch_wide_columns_and_empty_chart():
+    """A chart whose rows are randomly permuted has column offsets beyond 16 bits (32-bit column path); an empty
+    chart between two charts keeps its own (trivial) scalars.  Checked against SuperLU per chart."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from researchuv_amd import batch as B
+    from researchuv_amd import lscm, meshgen
+
+    say("wide-column batch (permuted 190x190 chart) + empty chart")
+    pos, faces = meshgen.heightfield(190, 190, seed=4)
+    pw, fw = lscm.weld(pos, faces, 1e-12)
+    s = lscm.segment(pw, fw, 180.0).system(0)
+    n = s["nfree"]
+    L = sp.csr_matrix((s["l_vals"], s["l_cols"], s["l_ptr"]), shape=(n, n))
+    perm = np.random.default_rng(1).permutation(n)
+    Lp = L[perm][:, perm].tocsr()
+    Lp.sort_indices()
+    rhs = s["rhs"].reshape(-1, 2)[perm]
+    small = meshgen.heightfield(20, 20, seed=2)
+    s2 = lscm.segment(*lscm.weld(*small, 1e-12), 180.0).system(0)
+    L2 = sp.csr_matrix((s2["l_vals"], s2["l_cols"], s2["l_ptr"]), shape=(s2["nfree"],) * 2)
+    A = sp.block_diag([Lp, L2], format="csr")
+    A.sort_indices()
+    off = [0, n, n, n + s2["nfree"]]  # chart 1 is empty
+    b = np.concatenate([rhs.reshape(-1), s2["rhs"]])
+    bt = B.Batch.from_csr(A.indptr.astype(np.int64), A.indices, A.data, off, mixed=True)
+    assert bt.schedule()["col_bytes"] == 4
+    x, info, it, res = bt.solve(b, max_iter=40000, tol=1e-10)
+    assert it[1] == 0
+    lu = spla.splu(Lp.tocsc())
+    xr = np.stack([lu.solve(rhs[:, 0]), lu.solve(rhs[:, 1])], axis=1)
+    xa = x[:2 * n].reshape(-1, 2)
+    err = np.linalg.norm(xa - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max()
+    lu2 = spla.splu(L2.tocsc())
+    r2 = s2["rhs"].reshape(-1, 2)
+    xr2 = np.stack([lu2.solve(r2[:, 0]), lu2.solve(r2[:, 1])], axis=1)
+    err2 = np.linalg.norm(x[2 * n:].reshape(-1, 2) - xr2, axis=1).max() / np.linalg.norm(xr2, axis=1).max()
+    say(f"  uv err permuted chart {err:.2e}, small chart {err2:.2e}")
+    assert err < 1e-6 and err2 < 1e-6
+
+
+def test_batch_of_small_spd_systems_matches_golden():
+    """One right-hand side per row (rhs_per_row = 1): the golden small SPD systems as one batch."""
+    import os
+
+    import scipy.sparse as sp
+
+    from researchuv_amd import batch as B
+
+    say("batch of the golden small SPD systems, R = 1")
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "spd_small.npz"))
+    mats, bs, xs, off = [], [], [], [0]
+    for k in range(int(z["count"])):
+        v, c, p = z[f"m{k}_vals"], z[f"m{k}_cols"], z[f"m{k}_row_ptr"]
+        m = len(p) - 1
+        mats.append(sp.csr_matrix((v, c, p), shape=(m, m)))
+        bs.append(z[f"m{k}_b"])
+        xs.append(z[f"m{k}_x"])
+        off.append(off[-1] + m)
+    A = sp.block_diag(mats, format="csr")
+    A.sort_indices()
+    for mixed in (False, True):
+        bt = B.Batch.from_csr(A.indptr.astype(np.int64), A.indices, A.data, off, mixed=mixed, R=1)
+        x, info, it, res = bt.solve(np.concatenate(bs), max_iter=1000, tol=1e-12)
+        for k in range(len(xs)):
+            xk = x[off[k]:off[k + 1]]
+            assert np.abs(xk - xs[k]).max() <= 1e-9 * max(1.0, np.abs(xs[k]).max()), (mixed, k)

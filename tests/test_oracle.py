@@ -361,6 +361,14 @@ def test_worker_thread_path_is_bitwise_stable(orc):
         b = ch.par_unfold(w)
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+    # one factor+solve per chart: the same as the driver stopped after its first pass, and (the system being
+    # fixed) the same uv as the full driver, whose second pass reproduces the first
+    one = ch.par_unfold(4, max_iter=1)
+    for i in range(len(ch)):
+        uv1, st = ch.unfold(i, max_iter=1)
+        assert st["iters"] == 1
+        assert np.array_equal(one[i], uv1)
+        np.testing.assert_allclose(uv1, a[i], atol=1e-12)
 
 
 # ------------------------------------------------------------------ CG / IC(0) / colouring
