@@ -158,6 +158,12 @@ def run_batch(args):
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
 
+    if args.calibrate_copy:  # a device copy of known size, the scale for HBM counter runs
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.perf_counter()
     bt = lb.device_batch(mixed=True)
     setup_s = time.perf_counter() - t0
@@ -166,18 +172,23 @@ def run_batch(args):
     for _ in range(args.warmup):
         bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL)
     torch.cuda.synchronize()
+    bt.kernel_timing(True)  # HIP events around every step-kernel launch of the timed solves
     t0 = time.perf_counter()
     for _ in range(args.steps):
         info, it, res = bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL, per_chart=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    ks = bt.kernel_stats()
+    bt.kernel_timing(False)
     value = unknowns * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"mixed batched: {ms_per_step:.1f} ms/batch, max inner iters {info.iters}, worst true resid "
         f"{info.true_rel_resid:.2e}")
+    achieved = ks["bytes"] / ks["seconds"]
+    log(f"step kernel over the timed solves: {ks['launches']} launches, {1e6 * ks['seconds'] / ks['launches']:.1f} "
+        f"us/launch, {ks['bytes'] / ks['launches'] / 1e9:.3f} GB/launch -> {achieved / 1e9:.0f} GB/s")
     sec, byt = bt.spmv_time(20)
-    achieved = byt / sec
-    log(f"step kernel (direction + SpMV): {sec * 1e3:.3f} ms/launch, {byt / 1e9:.3f} GB -> {achieved / 1e9:.0f} GB/s")
+    log(f"step kernel over the whole batch: {sec * 1e3:.3f} ms/launch, {byt / 1e9:.3f} GB -> {byt / sec / 1e9:.0f} GB/s")
     bt.close()
 
     # the same batch in plain FP64 (recursive-residual test at tol)
@@ -245,7 +256,9 @@ def run_batch(args):
 
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
                 traffic=None, kernel="kb_dir_spmv<float,2,%d>" % (0 if sched["col_bytes"] == 2 else 1),
-                bytes_per_launch=byt, us_per_launch=sec * 1e6)
+                launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
+                us_per_launch=1e6 * ks["seconds"] / ks["launches"],
+                whole_batch_launch={"us": sec * 1e6, "bytes": byt, "GBps": byt / sec / 1e9})
     tf = os.path.join(ROOT, "profiles", "traffic_batch.json")
     if os.path.exists(tf):
         roof["traffic"] = json.load(open(tf)).get("hbm_bytes_per_launch")
@@ -351,6 +364,8 @@ def main():
     ap.add_argument("--precond", default="JACOBI", help="chart512: preconditioner name")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="batch: copy 1 GiB device to device first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one GPU and this build "

@@ -139,12 +139,21 @@ int ruv_batch_solve_dev(ruv_batch* batch, const double* d_b, double* d_x, uint64
 int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
 /* Measurement hook: mean duration and algorithmic bytes of the batch's inner-precision step kernel (new
- * search direction + SpMV) over every tile: the sliced-ELL values and column offsets as stored, the slice
- * table, and four vectors (z, old p read; new p, q written) streamed once. */
+ * search direction + SpMV) over every tile: the stored values and column indices, the slice tables, and four
+ * vectors (z, old p read; new p, q written) streamed once. */
 int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
-/* The schedule the batch's autotuner chose: rows per workgroup tile, bytes per stored column index (2: 16-bit
- * offsets from the row, 4: 32-bit rows) and the number of stored sliced-ELL entries (padding included). */
-int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries);
+/* The batch's storage and the schedule its autotuner chose: rows per workgroup tile; bytes per stored column
+ * index of the general slices (2: 16-bit offsets from the row, 4: 32-bit rows); stored values (padding
+ * included) and stored column indices; slices in stencil form (column offsets per slice, none per entry) and
+ * all slices.  Any pointer may be null. */
+int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
+                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
+/* Step-kernel accounting over the solves that follow (measurement hook): with enable != 0 every launch of the
+ * step kernel is bracketed by HIP events on the batch's stream.  ruv_batch_kernel_stats returns, and resets,
+ * the launch count, the summed device time of those launches and their algorithmic bytes (per chart: its
+ * sliced-ELL entries as stored, slice table and four vectors, times the steps the chart ran). */
+int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable);
+int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes);
 void ruv_batch_destroy(ruv_batch* batch);
 
 /* Build the LSCM systems of nmesh meshes on host worker threads (the worker-thread unfold stage,

@@ -31,7 +31,11 @@ def _setup(L):
     L.ruv_batch_spmv_time.restype = C.c_int
     L.ruv_batch_spmv_time.argtypes = [P, C.c_int32, P, P]
     L.ruv_batch_schedule.restype = C.c_int
-    L.ruv_batch_schedule.argtypes = [P, P, P, P]
+    L.ruv_batch_kernel_timing.restype = C.c_int
+    L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
+    L.ruv_batch_kernel_stats.restype = C.c_int
+    L.ruv_batch_kernel_stats.argtypes = [P, P, P, P]
+    L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
     L.ruv_lscm_batch_build.restype = C.c_int
@@ -117,8 +121,25 @@ class Batch:
         rows = C.c_int32(0)
         cb = C.c_int32(0)
         ne = C.c_int64(0)
-        _lib.check(lib().ruv_batch_schedule(self.h, C.byref(rows), C.byref(cb), C.byref(ne)))
-        return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value)
+        nc = C.c_int64(0)
+        st = C.c_int32(0)
+        ns = C.c_int32(0)
+        _lib.check(lib().ruv_batch_schedule(self.h, C.byref(rows), C.byref(cb), C.byref(ne), C.byref(nc),
+                                            C.byref(st), C.byref(ns)))
+        return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value, col_entries=nc.value,
+                    stencil_slices=st.value, slices=ns.value)
+
+    def kernel_timing(self, enable=True):
+        """Bracket every step-kernel launch of the following solves with HIP events."""
+        _lib.check(lib().ruv_batch_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_stats(self) -> dict:
+        """Launches, summed device seconds and algorithmic bytes of the step kernel since the last call."""
+        n = C.c_int64(0)
+        sec = C.c_double(0)
+        byt = C.c_double(0)
+        _lib.check(lib().ruv_batch_kernel_stats(self.h, C.byref(n), C.byref(sec), C.byref(byt)))
+        return dict(launches=n.value, seconds=sec.value, bytes=byt.value)
 
     def close(self):
         if self.h:

@@ -21,7 +21,10 @@ void batch_destroy(BatchImpl* q);
 int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid);
 int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
-int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries);
+int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
+                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
+int batch_kernel_timing(BatchImpl* Q, int enable);
+int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
                      LscmBatch** out);
@@ -330,9 +333,21 @@ int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_laun
     return ruv::batch_spmv_time(batch->impl, reps, seconds_per_launch, bytes_per_launch);
 }
 
-int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries) {
+int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
+                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices) {
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
-    return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries);
+    return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
+                               slices);
+}
+
+int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_kernel_timing(batch->impl, enable);
+}
+
+int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_kernel_stats(batch->impl, launches, seconds, bytes);
 }
 
 void ruv_batch_destroy(ruv_batch* batch) {

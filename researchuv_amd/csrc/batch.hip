@@ -11,7 +11,8 @@
 // x and r = b - A x in FP64 per chart and stops a chart when its true residual ||b - A x|| <= tol ||b||.
 //
 // Schedule: the number of slices per tile (rows per workgroup) is picked per batch by timing the step kernel
-// over the whole batch for each candidate (RUV_BATCH_SLICES_PER_TILE=1|2|4|8 forces one).
+// over the whole batch for each candidate (RUV_BATCH_SLICES_PER_TILE=1|2|4|8 forces one).  The storage form of
+// each slice (stencil or general, see batch_kernels.hpp) is decided from its column offsets at setup.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +37,10 @@ namespace ruv {
 
 struct BatchImpl {
     int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
-    int ci = 0;          // 0: 16-bit column offsets, 1: 32-bit columns
+    int ci = 0;          // general slices: 0 = 16-bit column offsets, 1 = 32-bit columns
     int spt = 2;         // slices per tile (chosen by the autotuner)
-    int64_t nrows = 0, nnz = 0, nell = 0;
+    int nstencil = 0;    // slices stored in stencil form
+    int64_t nrows = 0, nnz = 0, nell = 0, ncol = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void*> mem;
@@ -70,6 +72,13 @@ struct BatchImpl {
     std::vector<uint64_t> iters_total;
     double inner_tol = 1e-4;
     int max_rounds = 40;
+    // step-kernel accounting (optional): an event pair around every launch, summed after each solve
+    bool ktime = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    int64_t k_launches = 0;
+    double k_seconds = 0.0, k_bytes = 0.0;
+    std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
 
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
@@ -102,6 +111,7 @@ struct BatchImpl {
         for (void* p : mem) (void)hipFree(p);
         for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
             if (p) (void)hipHostFree(p);
+        for (hipEvent_t e : kev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -198,61 +208,110 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     HIPCHK(hipEventCreate(&Q.ev0));
     HIPCHK(hipEventCreate(&Q.ev1));
     // slices: each chart's rows in runs of at most SLICE
-    std::vector<int64_t> sl_off;
-    std::vector<int32_t> sl_w;
     Q.chart_s0.assign(nchart + 1, 0);
-    int64_t nell = 0;
     for (int c = 0; c < nchart; ++c) {
         Q.chart_s0[c] = (int)Q.h_sl_row0.size();
         for (int64_t r = crow[c]; r < crow[c + 1]; r += SLICE) {
-            const int n = (int)std::min<int64_t>(SLICE, crow[c + 1] - r);
-            int w = 0;
-            for (int64_t i = r; i < r + n; ++i) w = std::max<int>(w, (int)(ptr[i + 1] - ptr[i]));
             Q.h_sl_row0.push_back(r);
-            Q.h_sl_n.push_back(n);
-            sl_w.push_back(w);
-            sl_off.push_back(nell);
-            nell += (int64_t)w * n;
+            Q.h_sl_n.push_back((int)std::min<int64_t>(SLICE, crow[c + 1] - r));
         }
     }
     Q.chart_s0[nchart] = (int)Q.h_sl_row0.size();
-    Q.nslice = (int)Q.h_sl_row0.size();
-    Q.nell = nell;
-    // 16-bit column offsets when every column is within +-32767 rows of its row
-    bool narrow = true;
-    for (int64_t i = 0; i < nrows && narrow; ++i)
-        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k)
-            if (cols[k] - i > 32767 || cols[k] - i < -32767) {
-                narrow = false;
-                break;
+    const int nsl = Q.nslice = (int)Q.h_sl_row0.size();
+    // classify: a stencil slice has at most SLOTS distinct column offsets (col - row); its slot k holds the
+    // entry at offset dtab[k] of every row (or 0), so it stores no column indices and its gathers coalesce.
+    // Any other slice is padded to its longest row and stores a column (offset) per entry.
+    std::vector<int32_t> sl_w(nsl, 0), dtab((size_t)nsl * SLOTS, 0);
+    std::vector<char> stencil(nsl, 0);
+    std::vector<int64_t> wide(1, 0);  // general entries whose offset does not fit 16 bits
+    parallel_for(nsl, [&](int64_t s0, int64_t s1) {
+        std::vector<int64_t> ds;
+        int64_t wide_local = 0;
+        for (int64_t s = s0; s < s1; ++s) {
+            const int64_t r0 = Q.h_sl_row0[s];
+            const int n = Q.h_sl_n[s];
+            ds.clear();
+            int wmax = 0;
+            for (int64_t i = r0; i < r0 + n; ++i) {
+                wmax = std::max<int>(wmax, (int)(ptr[i + 1] - ptr[i]));
+                for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) ds.push_back((int64_t)cols[a] - i);
             }
+            std::sort(ds.begin(), ds.end());
+            ds.erase(std::unique(ds.begin(), ds.end()), ds.end());
+            if ((int)ds.size() <= SLOTS && (ds.empty() || (ds.front() >= INT32_MIN && ds.back() <= INT32_MAX))) {
+                stencil[s] = 1;
+                sl_w[s] = (int)ds.size();
+                for (size_t k = 0; k < ds.size(); ++k) dtab[(size_t)s * SLOTS + k] = (int32_t)ds[k];
+            } else {
+                sl_w[s] = wmax;
+                for (int64_t i = r0; i < r0 + n; ++i)
+                    for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a)
+                        if (cols[a] - i > 32767 || cols[a] - i < -32767) ++wide_local;
+            }
+        }
+        if (wide_local) __atomic_fetch_add(&wide[0], wide_local, __ATOMIC_RELAXED);
+    });
+    const bool narrow = wide[0] == 0;
     Q.ci = narrow ? 0 : 1;
-    // fill the sliced ELL (and the Jacobi diagonal) on the host threads
+    std::vector<int64_t> sl_off(nsl), sl_coff(nsl);
+    int64_t nell = 0, ncol = 0;
+    Q.nstencil = 0;
+    for (int s = 0; s < nsl; ++s) {
+        sl_off[s] = nell;
+        nell += (int64_t)sl_w[s] * Q.h_sl_n[s];
+        if (stencil[s]) {
+            sl_coff[s] = -1;
+            ++Q.nstencil;
+        } else {
+            sl_coff[s] = ncol;
+            ncol += (int64_t)sl_w[s] * Q.h_sl_n[s];
+        }
+    }
+    Q.nell = nell;
+    Q.ncol = ncol;
+    {
+        const double sv = mode == RUV_MODE_MIXED ? 4.0 : 8.0, sc = narrow ? 2.0 : 4.0;
+        Q.chart_step_bytes.assign(nchart, 0.0);
+        for (int c = 0; c < nchart; ++c) {
+            double b = 4.0 * (double)(crow[c + 1] - crow[c]) * R * sv;
+            for (int s = Q.chart_s0[c]; s < Q.chart_s0[c + 1]; ++s)
+                b += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + 32.0 +
+                     (stencil[s] ? 4.0 * SLOTS : 0.0);
+            Q.chart_step_bytes[c] = b;
+        }
+    }
+    // fill the values (and the general slices' columns, and the Jacobi diagonal) on the host threads
     std::vector<double> e64((size_t)nell, 0.0);
-    std::vector<int16_t> c16(narrow ? (size_t)nell : 0, 0);
-    std::vector<int32_t> c32(narrow ? 0 : (size_t)nell, 0);
+    std::vector<int16_t> c16(narrow ? (size_t)ncol : 0, 0);
+    std::vector<int32_t> c32(narrow ? 0 : (size_t)ncol, 0);
     std::vector<double> dinv(precond == RUV_PC_JACOBI ? (size_t)nrows : 0, 0.0);
     std::vector<int64_t> bad(1, -1);
-    const int nsl = Q.nslice;
     parallel_for(nsl, [&](int64_t s0, int64_t s1) {
         for (int64_t s = s0; s < s1; ++s) {
             const int64_t r0 = Q.h_sl_row0[s];
             const int n = Q.h_sl_n[s];
+            const int32_t* dt = dtab.data() + (size_t)s * SLOTS;
             for (int l = 0; l < n; ++l) {
                 const int64_t i = r0 + l;
                 int k = 0;
                 double d = 0.0;
                 for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a, ++k) {
-                    const int64_t e = sl_off[s] + (int64_t)k * n + l;
-                    e64[e] = vals[a];
-                    if (narrow) c16[e] = (int16_t)(cols[a] - i);
-                    else c32[e] = cols[a];
                     if (cols[a] == i) d = vals[a];
+                    if (stencil[s]) {
+                        const int slot = (int)(std::lower_bound(dt, dt + sl_w[s], (int32_t)(cols[a] - i)) - dt);
+                        e64[sl_off[s] + (int64_t)slot * n + l] += vals[a];
+                    } else {
+                        e64[sl_off[s] + (int64_t)k * n + l] = vals[a];
+                        const int64_t e = sl_coff[s] + (int64_t)k * n + l;
+                        if (narrow) c16[e] = (int16_t)(cols[a] - i);
+                        else c32[e] = cols[a];
+                    }
                 }
-                for (; k < sl_w[s]; ++k) {  // padding: (0, the row itself)
-                    const int64_t e = sl_off[s] + (int64_t)k * n + l;
-                    if (!narrow) c32[e] = (int32_t)i;
-                }
+                if (!stencil[s])
+                    for (; k < sl_w[s]; ++k) {  // padding: (0, the row itself)
+                        const int64_t e = sl_coff[s] + (int64_t)k * n + l;
+                        if (!narrow) c32[e] = (int32_t)i;
+                    }
                 if (!dinv.empty()) {
                     if (std::fabs(d) < 1e-300) bad[0] = i;
                     else dinv[i] = 1.0 / d;
@@ -261,13 +320,16 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
         }
     });
     if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(bad[0]));
-    int64_t *d_row0 = nullptr, *d_off = nullptr;
-    int32_t *d_w = nullptr, *d_n = nullptr;
+    int64_t *d_row0 = nullptr, *d_off = nullptr, *d_coff = nullptr, *d_crow = nullptr;
+    int32_t *d_w = nullptr, *d_n = nullptr, *d_dtab = nullptr;
     void* d_col = nullptr;
     HIPCHK(Q.upload(&d_row0, Q.h_sl_row0.data(), Q.h_sl_row0.size()));
     HIPCHK(Q.upload(&d_off, sl_off.data(), sl_off.size()));
+    HIPCHK(Q.upload(&d_coff, sl_coff.data(), sl_coff.size()));
     HIPCHK(Q.upload(&d_w, sl_w.data(), sl_w.size()));
     HIPCHK(Q.upload(&d_n, Q.h_sl_n.data(), Q.h_sl_n.size()));
+    HIPCHK(Q.upload(&d_dtab, dtab.data(), dtab.size()));
+    HIPCHK(Q.upload(&d_crow, crow, (size_t)nchart + 1));
     if (narrow) {
         int16_t* p = nullptr;
         HIPCHK(Q.upload(&p, c16.data(), c16.size()));
@@ -305,8 +367,11 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     BDev& B = Q.B;
     B.sl_row0 = d_row0;
     B.sl_off = d_off;
+    B.sl_coff = d_coff;
     B.sl_w = d_w;
     B.sl_n = d_n;
+    B.sl_dtab = d_dtab;
+    B.crow = d_crow;
     B.ecol = d_col;
     B.nchart = nchart;
     HIPCHK(Q.alloc(&B.c_rz, 2 * (size_t)nchart));
@@ -338,11 +403,11 @@ int set_active(BatchImpl& Q, Pred pred, int* count) {
 
 inline dim3 grid_of(int na) { return dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)); }
 
-template <typename T, int R, int CI>
+template <typename T, int R, int CI, int PROBE = 0>
 void launch_dir_spmv(BatchImpl& Q, int na, int par, uint32_t iter, const T* z, const T* pold, T* pnew, T* q) {
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
-    hipLaunchKernelGGL((kb_dir_spmv<T, R, CI>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B, (const int*)Q.act, na, par,
-                       iter, vals, z, pold, pnew, q);
+    hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B, (const int*)Q.act,
+                       na, par, iter, vals, z, pold, pnew, q);
 }
 
 template <typename T, int R, int PC, int CI>
@@ -376,7 +441,22 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
         if (rc) return rc;
         if (na == 0) break;
         for (int it = 0; it < chunk; ++it) {
+            if (Q.ktime) {
+                if (Q.kev_used + 2 > Q.kev.size()) {
+                    hipEvent_t e0, e1;
+                    HIPCHK(hipEventCreate(&e0));
+                    HIPCHK(hipEventCreate(&e1));
+                    Q.kev.push_back(e0);
+                    Q.kev.push_back(e1);
+                }
+                HIPCHK(hipEventRecord(Q.kev[Q.kev_used], s));
+            }
             launch_dir_spmv<T, R, CI>(Q, na, par, iter, z, pb[par ^ 1], pb[par], q);
+            if (Q.ktime) {
+                HIPCHK(hipEventRecord(Q.kev[Q.kev_used + 1], s));
+                Q.kev_used += 2;
+                Q.k_launches += 1;
+            }
             hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
                                dinv, (const T*)pb[par], (const T*)q, x, r, z);
             par ^= 1;
@@ -468,11 +548,11 @@ int time_step_kernel(BatchImpl& Q, int reps, double* sec) {
     int rc = set_active(Q, [](int) { return true; }, &na);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s));
-    launch_dir_spmv<T, R, CI>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+    launch_dir_spmv<T, R, CI, 1>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(Q.ev0, s));
     for (int i = 0; i < reps; ++i)
-        launch_dir_spmv<T, R, CI>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+        launch_dir_spmv<T, R, CI, 1>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
     HIPCHK(hipEventRecord(Q.ev1, s));
     HIPCHK(hipEventSynchronize(Q.ev1));
     float ms = 0;
@@ -504,7 +584,7 @@ int autotune(BatchImpl& Q) {
         int rc = build_tiles(Q, spt);
         if (rc) return rc;
         double t = 0.0;
-        rc = time_step(Q, 3, &t);
+        rc = time_step(Q, 8, &t);
         if (rc) return rc;
         if (t < best_t) {
             best_t = t;
@@ -549,6 +629,15 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
     HIPCHK(hipEventSynchronize(Q->ev1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, Q->ev0, Q->ev1));
+    if (Q->ktime) {
+        for (size_t k = 0; k + 1 < Q->kev_used; k += 2) {
+            float kms = 0;
+            HIPCHK(hipEventElapsedTime(&kms, Q->kev[k], Q->kev[k + 1]));
+            Q->k_seconds += kms * 1e-3;
+        }
+        Q->kev_used = 0;
+        for (int c = 0; c < Q->nchart; ++c) Q->k_bytes += Q->chart_step_bytes[c] * (double)Q->iters_total[c];
+    }
     if (info) {
         uint64_t mx = 0;
         double worst = 0.0;
@@ -576,17 +665,39 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
     if (reps <= 0) return fail(RUV_ERR_INVALID, "reps must be positive");
     int rc = time_step(*Q, reps, sec);
     if (rc) return rc;
-    const double sv = Q->mode == RUV_MODE_MIXED ? 4.0 : 8.0;
-    const double sc = Q->ci == 0 ? 2.0 : 4.0;
-    *bytes = (double)Q->nell * (sv + sc) + (double)Q->nslice * 24.0 + (double)Q->ntile * 28.0 +
-             4.0 * (double)Q->nrows * Q->R * sv;
+    double b = 0.0;
+    for (int c = 0; c < Q->nchart; ++c) b += Q->chart_step_bytes[c];
+    *bytes = b;
     return RUV_OK;
 }
 
-int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries) {
+int batch_kernel_timing(BatchImpl* Q, int enable) {
+    Q->ktime = enable != 0;
+    Q->kev_used = 0;
+    Q->k_launches = 0;
+    Q->k_seconds = 0.0;
+    Q->k_bytes = 0.0;
+    return RUV_OK;
+}
+
+int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes) {
+    if (launches) *launches = Q->k_launches;
+    if (seconds) *seconds = Q->k_seconds;
+    if (bytes) *bytes = Q->k_bytes;
+    Q->k_launches = 0;
+    Q->k_seconds = 0.0;
+    Q->k_bytes = 0.0;
+    return RUV_OK;
+}
+
+int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
+                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices) {
     if (rows_per_tile) *rows_per_tile = Q->spt * SLICE;
     if (col_bytes) *col_bytes = Q->ci == 0 ? 2 : 4;
     if (ell_entries) *ell_entries = Q->nell;
+    if (col_entries) *col_entries = Q->ncol;
+    if (stencil_slices) *stencil_slices = Q->nstencil;
+    if (slices) *slices = Q->nslice;
     return RUV_OK;
 }
 

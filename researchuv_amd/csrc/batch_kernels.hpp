@@ -1,11 +1,18 @@
 // batch_kernels.hpp -- device side of the batched solver: sliced-ELL storage and a two-kernel PCG iteration.
 //
 // Storage.  Every chart is one diagonal block; its rows are cut into slices of at most SLICE (= BLK) rows, one
-// row per thread.  Slice s holds rows [sl_row0, sl_row0 + n_s) padded to the slice's longest row w_s (diagonal
-// included); entry k of row lane sits at sl_off + k * n_s + lane, so every load of a slice is one coalesced
-// sweep.  Columns are stored as 16-bit offsets from the row (mesh charts keep neighbours close) or, when any
-// offset does not fit, as 32-bit rows.  Padding entries are (value 0, offset 0).  A tile is a run of slices of
-// one chart; one workgroup owns one tile.
+// row per thread.  Slice s holds rows [sl_row0, sl_row0 + n_s) in w_s value columns; value k of row lane sits at
+// sl_off + k * n_s + lane, so every load of a slice is one coalesced sweep.  A slice is stored in one of two
+// forms:
+//   stencil  the slice's rows use at most SLOTS distinct column offsets (col - row), listed in dtab; value
+//            column k holds every row's entry at offset dtab[k] (0 where a row has none).  No column indices are
+//            stored, and the gathers of one value column read consecutive rows.  Offsets that land outside the
+//            chart (only ever with value 0) are clamped into it.  Mesh charts in a locality-preserving vertex
+//            order are stencil slices almost everywhere.
+//   general  rows padded to the slice's longest row; a column per entry at sl_coff + k * n_s + lane, as a
+//            16-bit offset from the row or, when some offset in the batch does not fit, a 32-bit row.  Padding is
+//            (value 0, the row itself).
+// A tile is a run of slices of one chart; one workgroup owns one tile.
 //
 // Iteration.  With z = M r (Jacobi) the reference recurrence (solver.rs:405-552)
 //     alpha = rz / pAp;  x += alpha p;  r -= alpha q;  z = M r;  beta = rz' / rz;  p = z + beta p
@@ -29,6 +36,7 @@
 namespace ruv {
 
 constexpr int SLICE = BLK;  // rows per slice (one per thread)
+constexpr int SLOTS = 8;    // column offsets of a stencil slice
 constexpr int XCDS = 8;     // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
 enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4 };
@@ -36,10 +44,13 @@ enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3,
 struct BDev {
     // sliced ELL (per slice)
     const int64_t* sl_row0;
-    const int64_t* sl_off;
+    const int64_t* sl_off;   // first value
+    const int64_t* sl_coff;  // first column of a general slice, -1 for a stencil slice
     const int32_t* sl_w;
     const int32_t* sl_n;
-    const void* ecol;  // int16 offsets (CI = 0) or int32 rows (CI = 1)
+    const int32_t* sl_dtab;  // [nslice][SLOTS] offsets of a stencil slice
+    const int64_t* crow;     // nchart + 1 chart row offsets
+    const void* ecol;        // general slices: int16 offsets (CI = 0) or int32 rows (CI = 1)
     // tiles: tile t = slices [tile_s0[t], tile_s0[t + 1]) = rows [tile_r0[t], tile_r1[t]) of chart tile_chart[t]
     const int32_t* tile_s0;
     const int64_t* tile_r0;
@@ -128,7 +139,9 @@ __device__ __forceinline__ T dir(T z, T p, T beta) {
 }
 
 // Opens step `iter` (parity par) of every live chart, then p' = z + beta p and q = A p' over one tile.
-template <typename T, int R, int CI>
+// PROBE = 1 is the same code under its own symbol for the schedule autotuner and the measurement hook, so a
+// kernel trace of the solves is not mixed with whole-batch timing launches.
+template <typename T, int R, int CI, int PROBE = 0>
 __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict__ act, int na, int par,
                                                    uint32_t iter, const T* __restrict__ vals,
                                                    const T* __restrict__ z, const T* __restrict__ pold,
@@ -161,6 +174,7 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
         B.c_iters[c] = iter + 1;
     }
     const T beta = (T)(rz / rzold);
+    const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     double acc = 0.0;
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
@@ -168,19 +182,33 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
         const int64_t e0 = B.sl_off[s] + threadIdx.x;
+        const int64_t c0 = B.sl_coff[s];
         const int w = B.sl_w[s];
         T acc_q[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc_q[cc] = T(0);
+        if (c0 < 0) {
+            const int32_t* dt = B.sl_dtab + (int64_t)s * SLOTS;
 #pragma unroll 4
-        for (int k = 0; k < w; ++k) {
-            const int64_t e = e0 + (int64_t)k * n;
-            const T v = vals[e];
-            const int64_t j = col_of<CI>(B.ecol, e, i);
-            const Vr<T, R> zj = ld<T, R>(z, j);
-            const Vr<T, R> pj = ld<T, R>(pold, j);
+            for (int k = 0; k < w; ++k) {
+                const T v = vals[e0 + (int64_t)k * n];
+                const int64_t j = min(max(i + dt[k], lo), hi);
+                const Vr<T, R> zj = ld<T, R>(z, j);
+                const Vr<T, R> pj = ld<T, R>(pold, j);
 #pragma unroll
-            for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
+                for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
+            }
+        } else {
+            const int64_t f0 = c0 + threadIdx.x;
+#pragma unroll 4
+            for (int k = 0; k < w; ++k) {
+                const T v = vals[e0 + (int64_t)k * n];
+                const int64_t j = col_of<CI>(B.ecol, f0 + (int64_t)k * n, i);
+                const Vr<T, R> zj = ld<T, R>(z, j);
+                const Vr<T, R> pj = ld<T, R>(pold, j);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
+            }
         }
         const Vr<T, R> zi = ld<T, R>(z, i);
         const Vr<T, R> pi = ld<T, R>(pold, i);
@@ -371,20 +399,23 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
     const int c = B.tile_chart[t];
     if (B.c_final[c]) return;
     double acc = 0.0;
+    const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
         const int64_t e0 = B.sl_off[s] + threadIdx.x;
+        const int64_t c0 = B.sl_coff[s];
         const int w = B.sl_w[s];
         double sum[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) sum[cc] = 0.0;
         for (int k = 0; k < w; ++k) {
-            const int64_t e = e0 + (int64_t)k * n;
-            const double v = vals64[e];
-            const Vr<double, R> xj = ld<double, R>(x64, col_of<CI>(B.ecol, e, i));
+            const double v = vals64[e0 + (int64_t)k * n];
+            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
+                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
+            const Vr<double, R> xj = ld<double, R>(x64, j);
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) sum[cc] = fma(v, xj.c[cc], sum[cc]);
         }
