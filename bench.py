@@ -158,10 +158,10 @@ def run_batch(args):
     x = torch.zeros_like(b)
     torch.cuda.synchronize()
 
-    if args.calibrate_copy:  # a device copy of known size, the scale for HBM counter runs
+    if args.calibrate_copy:  # a streaming kernel of known traffic (1 GiB read, 1 GiB written) for counter runs
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     t0 = time.perf_counter()
@@ -365,7 +365,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="batch: copy 1 GiB device to device first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
+                    help="batch: stream 1 GiB through one kernel first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one GPU and this build "

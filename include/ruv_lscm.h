@@ -125,7 +125,8 @@ int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch,
  * pipeline.rs:171-194 -> lscm.rs:471-506).  A batch holds nchart independent systems L_c (x) I_R as one
  * block-diagonal CSR: rows of chart c are [chart_row_off[c], chart_row_off[c+1]) and its columns stay inside
  * that range (row_ptr is 64-bit; columns are global row indices).  Every chart runs its own PCG (own
- * alpha/beta/stopping test) inside shared launches; precond none or jacobi.  mode 0: FP64 PCG per chart with
+ * alpha/beta/stopping test) inside shared launches; precond none, jacobi or amg (an aggregation hierarchy
+ * built on the device at create time, block-diagonal by chart, applied as a V(1,1) cycle).  mode 0: FP64 PCG per chart with
  * the recursive-residual test tol*||b_c||; mode 1: FP32 PCG inside per-chart FP64 refinement until
  * ||b_c - A_c x_c|| <= tol*||b_c||.  info->iters is the largest per-chart iteration count and
  * info->true_rel_resid the worst chart; chart_iters / chart_true_resid (nchart, optional) give every chart.
@@ -153,6 +154,10 @@ int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* 
  * the launch count, the summed device time of those launches and their algorithmic bytes (per chart: its
  * sliced-ELL entries as stored, slice table and four vectors, times the steps the chart ran). */
 int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable);
+/* precond amg: levels of the device-built aggregation hierarchy (finest included), the device time its setup
+ * took inside ruv_batch_create, and the rows of all coarse levels together; zeros for the other
+ * preconditioners. */
+int ruv_batch_precond_info(ruv_batch* batch, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
 int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes);
 void ruv_batch_destroy(ruv_batch* batch);
 

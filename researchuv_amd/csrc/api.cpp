@@ -24,6 +24,7 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
 int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
                    int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
 int batch_kernel_timing(BatchImpl* Q, int enable);
+int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
 int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
@@ -338,6 +339,11 @@ int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* 
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
                                slices);
+}
+
+int ruv_batch_precond_info(ruv_batch* batch, int32_t* levels, double* setup_seconds, int64_t* coarse_rows) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_precond_info(batch->impl, levels, setup_seconds, coarse_rows);
 }
 
 int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable) {

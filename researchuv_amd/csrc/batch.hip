@@ -24,99 +24,16 @@
 #include <type_traits>
 #include <vector>
 
+#include "batch_impl.hpp"
 #include "batch_kernels.hpp"
 #include "ruv_internal.hpp"
 
 namespace ruv {
 
-#define HIPCHK(call)                                                                                         \
-    do {                                                                                                     \
-        hipError_t e_ = (call);                                                                              \
-        if (e_ != hipSuccess) return fail(RUV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct BatchImpl {
-    int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
-    int ci = 0;          // general slices: 0 = 16-bit column offsets, 1 = 32-bit columns
-    int spt = 2;         // slices per tile (chosen by the autotuner)
-    int nstencil = 0;    // slices stored in stencil form
-    int64_t nrows = 0, nnz = 0, nell = 0, ncol = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<void*> mem;
-    BDev B{};
-    // host copies of the slice table and chart row offsets (tile building)
-    std::vector<int64_t> h_sl_row0;
-    std::vector<int32_t> h_sl_n;
-    std::vector<int32_t> chart_s0;  // first slice of each chart, nchart + 1
-    std::vector<int64_t> crow;      // first row of each chart, nchart + 1
-    std::vector<int> chart_t0;
-    // tiles (rebuilt by the autotuner)
-    int32_t *d_tile_s0 = nullptr, *d_tile_chart = nullptr, *d_chart_t0 = nullptr;
-    int64_t *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;
-    double *d_part_a = nullptr, *d_part_b = nullptr, *d_part_c = nullptr, *d_part_o = nullptr;
-    int* act = nullptr;
-    int* h_act = nullptr;
-    // matrix
-    float* ev32 = nullptr;
-    double* ev64 = nullptr;
-    float* dinv32 = nullptr;
-    double* dinv64 = nullptr;
-    // vectors
-    double *x64 = nullptr, *r64 = nullptr;
-    void *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv0 = nullptr, *pv1 = nullptr, *qv = nullptr;
-    int* h_done = nullptr;
-    int* h_final = nullptr;
-    unsigned* h_iters = nullptr;
-    double* h_true = nullptr;
-    std::vector<uint64_t> iters_total;
-    double inner_tol = 1e-4;
-    int max_rounds = 40;
-    // step-kernel accounting (optional): an event pair around every launch, summed after each solve
-    bool ktime = false;
-    std::vector<hipEvent_t> kev;
-    size_t kev_used = 0;
-    int64_t k_launches = 0;
-    double k_seconds = 0.0, k_bytes = 0.0;
-    std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
-
-    template <typename T>
-    hipError_t alloc(T** p, size_t n) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) mem.push_back(q);
-        *p = (T*)q;
-        return e;
-    }
-    template <typename T>
-    hipError_t upload(T** p, const T* h, size_t n) {
-        hipError_t e = alloc(p, n);
-        if (e == hipSuccess && n) e = hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, stream);
-        return e;
-    }
-    void free_tiles() {
-        for (void* p : {(void*)d_tile_s0, (void*)d_tile_chart, (void*)d_chart_t0, (void*)d_tile_r0, (void*)d_tile_r1,
-                        (void*)d_part_a, (void*)d_part_b, (void*)d_part_c, (void*)d_part_o, (void*)act})
-            if (p) (void)hipFree(p);
-        d_tile_s0 = d_tile_chart = d_chart_t0 = nullptr;
-        d_tile_r0 = d_tile_r1 = nullptr;
-        d_part_a = d_part_b = d_part_c = d_part_o = nullptr;
-        act = nullptr;
-        if (h_act) (void)hipHostFree(h_act);
-        h_act = nullptr;
-    }
-    ~BatchImpl() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        free_tiles();
-        for (void* p : mem) (void)hipFree(p);
-        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
-            if (p) (void)hipHostFree(p);
-        for (hipEvent_t e : kev) (void)hipEventDestroy(e);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals);
+template <typename T, int R, int CI>
+int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z);
+int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows);
 
 namespace {
 
@@ -163,6 +80,8 @@ int build_tiles(BatchImpl& Q, int spt) {
     Q.free_tiles();
     Q.ntile = T;
     Q.spt = spt;
+    Q.h_tile_r0 = tr0;
+    Q.h_tile_r1 = tr1;
     auto up = [&](auto** dst, const auto* src, size_t n) -> hipError_t {
         using E = std::remove_pointer_t<std::remove_reference_t<decltype(*dst)>>;
         hipError_t e = hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(E));
@@ -284,7 +203,7 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     std::vector<double> e64((size_t)nell, 0.0);
     std::vector<int16_t> c16(narrow ? (size_t)ncol : 0, 0);
     std::vector<int32_t> c32(narrow ? 0 : (size_t)ncol, 0);
-    std::vector<double> dinv(precond == RUV_PC_JACOBI ? (size_t)nrows : 0, 0.0);
+    std::vector<double> dinv(precond != RUV_PC_NONE ? (size_t)nrows : 0, 0.0);
     std::vector<int64_t> bad(1, -1);
     parallel_for(nsl, [&](int64_t s0, int64_t s1) {
         for (int64_t s = s0; s < s1; ++s) {
@@ -319,7 +238,7 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
             }
         }
     });
-    if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(bad[0]));
+    if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "zero diagonal at row " + std::to_string(bad[0]));
     int64_t *d_row0 = nullptr, *d_off = nullptr, *d_coff = nullptr, *d_crow = nullptr;
     int32_t *d_w = nullptr, *d_n = nullptr, *d_dtab = nullptr;
     void* d_col = nullptr;
@@ -418,7 +337,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
     const T* dinv = (sizeof(T) == 4) ? (const T*)(const void*)Q.dinv32 : (const T*)(const void*)Q.dinv64;
     T* x = (T*)Q.xv;
     T* r = (T*)Q.rv;
-    T* z = (PC == 1) ? (T*)Q.zv : r;
+    T* z = (PC >= 1) ? (T*)Q.zv : r;
     T* pb[2] = {(T*)Q.pv0, (T*)Q.pv1};
     T* q = (T*)Q.qv;
     int na = 0;
@@ -431,6 +350,10 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
     hipLaunchKernelGGL(kb_init, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol, Q.inner_tol,
                        Q.mode == RUV_MODE_MIXED ? 1 : 0);
     HIPCHK(hipGetLastError());
+    if (PC == 2) {  // z0 = M^-1 r0 and its r.z partials (parity 0); kb_init has marked the running charts
+        rc = bamg_apply<T, R, CI>(Q, na, 0, (const T*)r, z);
+        if (rc) return rc;
+    }
     HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     int chunk = 8;
@@ -459,6 +382,10 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
             }
             hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
                                dinv, (const T*)pb[par], (const T*)q, x, r, z);
+            if (PC == 2) {
+                rc = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z);
+                if (rc) return rc;
+            }
             par ^= 1;
             ++iter;
         }
@@ -519,6 +446,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
 
 template <typename T, int R, int CI>
 int run_pc(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, double tol) {
+    if (Q.precond == RUV_PC_AMG) return batch_run<T, R, 2, CI>(Q, d_b, d_x, max_iter, tol);
     return Q.precond == RUV_PC_JACOBI ? batch_run<T, R, 1, CI>(Q, d_b, d_x, max_iter, tol)
                                       : batch_run<T, R, 0, CI>(Q, d_b, d_x, max_iter, tol);
 }
@@ -599,8 +527,8 @@ int autotune(BatchImpl& Q) {
 int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const double* vals, int nchart,
                  const int64_t* crow, int R, int precond, int mode, BatchImpl** out) {
     if (R != 1 && R != 2) return fail(RUV_ERR_INVALID, "rhs_per_row must be 1 or 2");
-    if (precond != RUV_PC_NONE && precond != RUV_PC_JACOBI)
-        return fail(RUV_ERR_INVALID, "the batched solver supports the none and jacobi preconditioners");
+    if (precond != RUV_PC_NONE && precond != RUV_PC_JACOBI && precond != RUV_PC_AMG)
+        return fail(RUV_ERR_INVALID, "the batched solver supports the none, jacobi and amg preconditioners");
     if (mode != RUV_MODE_FP64 && mode != RUV_MODE_MIXED) return fail(RUV_ERR_INVALID, "unknown mode");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RUV_ERR_UNAVAILABLE, "no HIP device");
@@ -609,6 +537,10 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (rc) return rc;
     rc = autotune(*Q);
     if (rc) return rc;
+    if (precond == RUV_PC_AMG) {  // the finest aggregates follow the tiles, so after the autotuner
+        rc = bamg_build(*Q, ptr, cols, vals);
+        if (rc) return rc;
+    }
     *out = Q.release();
     return RUV_OK;
 }
@@ -669,6 +601,13 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
     for (int c = 0; c < Q->nchart; ++c) b += Q->chart_step_bytes[c];
     *bytes = b;
     return RUV_OK;
+}
+
+int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows) {
+    int lv = 0;
+    int rc = bamg_info(*Q, &lv, setup_seconds, coarse_rows);
+    if (levels) *levels = lv;
+    return rc;
 }
 
 int batch_kernel_timing(BatchImpl* Q, int enable) {

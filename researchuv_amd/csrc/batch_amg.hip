@@ -1,0 +1,384 @@
+// batch_amg.hip -- builds the batched aggregation AMG on the device and applies its V(1,1) cycle
+// (kernels and the method in bamg_kernels.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "bamg_kernels.hpp"
+#include "batch_impl.hpp"
+
+namespace ruv {
+
+struct BamgLevel {
+    int64_t n = 0, nnz = 0;
+    int64_t* ptr = nullptr;  // levels >= 1
+    int32_t* cols = nullptr;
+    void* vals = nullptr;    // T
+    void* dinv = nullptr;    // T (level 0: the batch's Jacobi diagonal)
+    int32_t* agg = nullptr;      // row -> next-level aggregate
+    int64_t* members = nullptr;  // next level's aggregates -> AMG_MEMBERS rows here
+    void* b = nullptr;
+    void* z = nullptr;
+};
+
+struct Bamg {
+    std::vector<BamgLevel> lv;  // lv.back() is the coarsest
+    int64_t* tile_c0 = nullptr;  // finest tiles -> their level-1 aggregate range (ntile + 1)
+    int max_tile_rows = 0;
+    double omega = 2.0 / 3.0;
+    double setup_seconds = 0.0;
+    std::vector<void*> mem;
+    ~Bamg() {
+        for (void* p : mem) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) mem.push_back(q);
+        *p = (T*)q;
+        return e;
+    }
+};
+
+void bamg_destroy(Bamg* a) { delete a; }
+
+namespace {
+
+inline dim3 rows_grid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLK - 1) / BLK)); }
+
+// temporary device buffers of one setup step, freed together
+struct Scratch {
+    std::vector<void*> mem;
+    ~Scratch() {
+        for (void* p : mem) (void)hipFree(p);
+    }
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) mem.push_back(q);
+        *p = (T*)q;
+        return e;
+    }
+};
+
+// out[0..n) = exclusive scan of in, out[n] = total
+int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) {
+    const int64_t nb = std::max<int64_t>(1, (n + SCAN_BLK - 1) / SCAN_BLK);
+    int64_t* tot = nullptr;
+    HIPCHK(S.alloc(&tot, (size_t)nb + 1));
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(BLK), 0, s, n, in, out, tot);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(BLK), 0, s, nb, tot);
+    hipLaunchKernelGGL(k_scan_add, rows_grid(n), dim3(BLK), 0, s, n, out, (const int64_t*)tot);
+    HIPCHK(hipMemcpyAsync(out + n, tot + nb, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipGetLastError());
+    return RUV_OK;
+}
+
+// One matching of A within groups and its Galerkin operator.
+struct Matched {
+    int64_t nc = 0;
+    int64_t* mate = nullptr;
+    int64_t* cid = nullptr;  // exclusive scan of leader flags, n + 1
+    int32_t* agg = nullptr;
+    int64_t* lead = nullptr;
+    int32_t* group = nullptr;  // of the aggregates
+    SetupCsr C{};
+    double* cdiag = nullptr;
+};
+
+int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, double theta, int rounds,
+                      Scratch& S, Matched* M) {
+    const int64_t n = A.n;
+    double* smax = nullptr;
+    int64_t *prop = nullptr, *flag = nullptr;
+    HIPCHK(S.alloc(&smax, n));
+    HIPCHK(S.alloc(&prop, n));
+    HIPCHK(S.alloc(&flag, n));
+    HIPCHK(S.alloc(&M->mate, n));
+    HIPCHK(S.alloc(&M->cid, n + 1));
+    HIPCHK(S.alloc(&M->agg, n));
+    HIPCHK(hipMemsetAsync(M->mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
+    hipLaunchKernelGGL(k_smax, rows_grid(n), dim3(BLK), 0, s, A, smax);
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, A, (const double*)smax, group,
+                           (const int64_t*)M->mate, prop, theta);
+        hipLaunchKernelGGL(k_accept, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)prop, M->mate);
+    }
+    hipLaunchKernelGGL(k_leaders, rows_grid(n), dim3(BLK), 0, s, n, M->mate, flag);
+    HIPCHK(hipGetLastError());
+    int rc = scan(s, n, flag, M->cid, S);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(&M->nc, M->cid + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int64_t nc = M->nc;
+    HIPCHK(S.alloc(&M->lead, nc));
+    HIPCHK(S.alloc(&M->group, nc));
+    hipLaunchKernelGGL(k_aggregate, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)M->mate,
+                       (const int64_t*)M->cid, group, M->agg, M->lead, M->group);
+    int64_t *cnt = nullptr, *cptr = nullptr;
+    HIPCHK(S.alloc(&cnt, nc));
+    HIPCHK(S.alloc(&cptr, nc + 1));
+    hipLaunchKernelGGL((k_galerkin<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->lead,
+                       (const int64_t*)M->mate, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
+                       (int32_t*)nullptr, (double*)nullptr, (double*)nullptr);
+    HIPCHK(hipGetLastError());
+    rc = scan(s, nc, cnt, cptr, S);
+    if (rc) return rc;
+    int64_t cnnz = 0;
+    HIPCHK(hipMemcpyAsync(&cnnz, cptr + nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (cnnz >= INT32_MAX) return fail(RUV_ERR_PRECOND, "batch amg: coarse operator too large");
+    int32_t* ccols = nullptr;
+    double *cvals = nullptr;
+    HIPCHK(S.alloc(&ccols, cnnz));
+    HIPCHK(S.alloc(&cvals, cnnz));
+    HIPCHK(S.alloc(&M->cdiag, nc));
+    hipLaunchKernelGGL((k_galerkin<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->lead,
+                       (const int64_t*)M->mate, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
+                       ccols, cvals, M->cdiag);
+    HIPCHK(hipGetLastError());
+    M->C = SetupCsr{nc, cptr, ccols, cvals};
+    return RUV_OK;
+}
+
+__global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, const int64_t* __restrict__ cid1,
+                                 const int64_t* __restrict__ cid2, int64_t* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (k < m) out[k] = cid2[cid1[at[k]]];
+}
+
+}  // namespace
+
+// Build the hierarchy from the host CSR of the batch (the finest operator; tiles already fixed).
+int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals) {
+    hipStream_t s = Q.stream;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    std::unique_ptr<Bamg> A(new Bamg());
+    const bool f32 = Q.mode == RUV_MODE_MIXED;
+    const size_t sv = f32 ? sizeof(float) : sizeof(double);
+    const int R = Q.R;
+    const double theta = 0.25;
+    const int rounds = 3;
+    // finest operator in FP64 CSR for the setup, and the tile groups
+    Scratch S0;
+    int64_t* d_ptr = nullptr;
+    int32_t *d_cols = nullptr, *group = nullptr;
+    double* d_vals = nullptr;
+    const int64_t n0 = Q.nrows;
+    HIPCHK(S0.alloc(&d_ptr, n0 + 1));
+    HIPCHK(S0.alloc(&d_cols, Q.nnz));
+    HIPCHK(S0.alloc(&d_vals, Q.nnz));
+    HIPCHK(S0.alloc(&group, n0));
+    HIPCHK(hipMemcpyAsync(d_ptr, ptr, sizeof(int64_t) * (n0 + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_cols, cols, sizeof(int32_t) * Q.nnz, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_vals, vals, sizeof(double) * Q.nnz, hipMemcpyHostToDevice, s));
+    {
+        std::vector<int32_t> g(n0);
+        for (int t = 0; t < Q.ntile; ++t)
+            for (int64_t i = Q.h_tile_r0[t]; i < Q.h_tile_r1[t]; ++i) g[i] = t;
+        HIPCHK(hipMemcpyAsync(group, g.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    SetupCsr cur{n0, d_ptr, d_cols, d_vals};
+    const double* cur_diag = nullptr;
+    std::vector<int64_t> crow = Q.crow;
+    std::unique_ptr<Scratch> keep(new Scratch());  // owns the current level's setup CSR (levels >= 1)
+    const int max_levels = 24;
+    for (int l = 0;; ++l) {
+        BamgLevel L;
+        L.n = cur.n;
+        // cycle data of this level
+        if (l == 0) {
+            L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
+        } else {
+            int64_t nnz = 0;
+            HIPCHK(hipMemcpyAsync(&nnz, cur.ptr + cur.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            L.nnz = nnz;
+            HIPCHK(A->alloc(&L.ptr, cur.n + 1));
+            HIPCHK(A->alloc(&L.cols, nnz));
+            HIPCHK(hipMemcpyAsync(L.ptr, cur.ptr, sizeof(int64_t) * (cur.n + 1), hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipMemcpyAsync(L.cols, cur.cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s));
+            char *v = nullptr, *d = nullptr;
+            HIPCHK(A->alloc(&v, nnz * sv));
+            HIPCHK(A->alloc(&d, cur.n * sv));
+            if (f32) {
+                hipLaunchKernelGGL((k_cast<float>), rows_grid(nnz), dim3(BLK), 0, s, nnz, cur.vals, (float*)v);
+                hipLaunchKernelGGL((k_cast_inv<float>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (float*)d);
+            } else {
+                hipLaunchKernelGGL((k_cast<double>), rows_grid(nnz), dim3(BLK), 0, s, nnz, cur.vals, (double*)v);
+                hipLaunchKernelGGL((k_cast_inv<double>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (double*)d);
+            }
+            HIPCHK(hipGetLastError());
+            L.vals = v;
+            L.dinv = d;
+            char *b = nullptr, *z = nullptr;
+            HIPCHK(A->alloc(&b, cur.n * R * sv));
+            HIPCHK(A->alloc(&z, cur.n * R * sv));
+            L.b = b;
+            L.z = z;
+        }
+        const bool last = (l + 1 >= max_levels) || (l > 0 && cur.n <= (int64_t)Q.nchart);
+        if (last) {
+            A->lv.push_back(L);
+            break;
+        }
+        // two matchings make the next level
+        std::unique_ptr<Scratch> S1(new Scratch()), S2(new Scratch());
+        Matched M1, M2;
+        int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
+        if (rc) return rc;
+        rc = match_and_coarsen(s, M1.C, M1.group, theta, rounds, *S2, &M2);
+        if (rc) return rc;
+        if (l > 0 && M2.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
+            L.ptr = nullptr;
+            A->lv.push_back(L);
+            break;
+        }
+        HIPCHK(A->alloc(&L.agg, cur.n));
+        HIPCHK(A->alloc(&L.members, (size_t)M2.nc * AMG_MEMBERS));
+        hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
+                           (const int32_t*)M2.agg, L.agg);
+        hipLaunchKernelGGL(k_compose, rows_grid(M2.nc), dim3(BLK), 0, s, M2.nc, (const int64_t*)M2.lead,
+                           (const int64_t*)M2.mate, (const int64_t*)M1.lead, (const int64_t*)M1.mate, L.members);
+        HIPCHK(hipGetLastError());
+        // chart offsets on the next level (and, on the finest, each tile's aggregate range)
+        {
+            std::vector<int64_t> at(crow.begin(), crow.end());
+            if (l == 0) at.insert(at.end(), Q.h_tile_r0.begin(), Q.h_tile_r0.end());
+            int64_t *d_at = nullptr, *d_out = nullptr;
+            HIPCHK(S1->alloc(&d_at, at.size()));
+            HIPCHK(S1->alloc(&d_out, at.size()));
+            HIPCHK(hipMemcpyAsync(d_at, at.data(), sizeof(int64_t) * at.size(), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_gather_offsets, rows_grid((int64_t)at.size()), dim3(BLK), 0, s, (int64_t)at.size(),
+                               (const int64_t*)d_at, (const int64_t*)M1.cid, (const int64_t*)M2.cid, d_out);
+            std::vector<int64_t> out(at.size());
+            HIPCHK(hipMemcpyAsync(out.data(), d_out, sizeof(int64_t) * at.size(), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::vector<int64_t> next(out.begin(), out.begin() + crow.size());
+            if (l == 0) {
+                std::vector<int64_t> tc(out.begin() + crow.size(), out.end());
+                tc.push_back(M2.nc);
+                HIPCHK(A->alloc(&A->tile_c0, tc.size()));
+                HIPCHK(hipMemcpyAsync(A->tile_c0, tc.data(), sizeof(int64_t) * tc.size(), hipMemcpyHostToDevice, s));
+                int mx = 0;
+                for (int t = 0; t < Q.ntile; ++t) mx = std::max<int>(mx, (int)(Q.h_tile_r1[t] - Q.h_tile_r0[t]));
+                A->max_tile_rows = std::max(mx, 1);
+                HIPCHK(hipStreamSynchronize(s));
+            }
+            crow.swap(next);
+        }
+        A->lv.push_back(L);
+        // the next level's operator (kept), its diagonal, and chart groups
+        std::unique_ptr<Scratch> nk(new Scratch());
+        int64_t* nptr = nullptr;
+        int32_t* ncols = nullptr;
+        double *nvals = nullptr, *ndiag = nullptr;
+        int64_t nnz2 = 0;
+        HIPCHK(hipMemcpyAsync(&nnz2, M2.C.ptr + M2.nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(nk->alloc(&nptr, M2.nc + 1));
+        HIPCHK(nk->alloc(&ncols, nnz2));
+        HIPCHK(nk->alloc(&nvals, nnz2));
+        HIPCHK(nk->alloc(&ndiag, M2.nc));
+        int32_t* ngroup = nullptr;
+        HIPCHK(nk->alloc(&ngroup, M2.nc));
+        HIPCHK(hipMemcpyAsync(nptr, M2.C.ptr, sizeof(int64_t) * (M2.nc + 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ncols, M2.C.cols, sizeof(int32_t) * nnz2, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nvals, M2.C.vals, sizeof(double) * nnz2, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ndiag, M2.cdiag, sizeof(double) * M2.nc, hipMemcpyDeviceToDevice, s));
+        int64_t* d_crow = nullptr;
+        HIPCHK(nk->alloc(&d_crow, crow.size()));
+        HIPCHK(hipMemcpyAsync(d_crow, crow.data(), sizeof(int64_t) * crow.size(), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_chart_groups, rows_grid(M2.nc), dim3(BLK), 0, s, M2.nc, (const int64_t*)d_crow,
+                           Q.nchart, ngroup);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        cur = SetupCsr{M2.nc, nptr, ncols, nvals};
+        cur_diag = ndiag;
+        group = ngroup;
+        keep = std::move(nk);  // frees the previous level's setup copy
+        S1.reset();
+        S2.reset();
+    }
+    // finest level's aggregate map and members live on; level 1 must exist
+    if (A->lv.size() < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    A->setup_seconds = ms * 1e-3;
+    if (Q.amg) bamg_destroy(Q.amg);
+    Q.amg = A.release();
+    return RUV_OK;
+}
+
+// z = M^-1 r (V(1,1) cycle) over the live tiles; partial r.z of every live tile into part_c[par].
+template <typename T, int R, int CI>
+int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
+    Bamg& A = *Q.amg;
+    hipStream_t s = Q.stream;
+    const T w = (T)A.omega;
+    const size_t nl = A.lv.size();
+    BamgLevel& L0 = A.lv[0];
+    BamgLevel& L1 = A.lv[1];
+    const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
+    const size_t lds = (size_t)A.max_tile_rows * R * sizeof(T);
+    hipLaunchKernelGGL((kc_down0<T, R, CI>), dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)), dim3(BLK), lds, s, Q.B,
+                       (const int*)Q.act, na, vals, (const T*)L0.dinv, w, r, (const int64_t*)A.tile_c0,
+                       (const int64_t*)L0.members, (T*)L1.b, A.max_tile_rows);
+    auto cyc = [&](size_t l) {
+        BamgLevel& L = A.lv[l];
+        CycLevel<T> c{L.n, L.ptr, L.cols, (const T*)L.vals, (const T*)L.dinv, L.agg, L.members, (T*)L.b, (T*)L.z};
+        return c;
+    };
+    for (size_t l = 1; l + 1 < nl; ++l)
+        hipLaunchKernelGGL((kc_down<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n, w,
+                           (T*)A.lv[l + 1].b);
+    hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
+    for (size_t l = nl - 1; l-- > 1;)
+        hipLaunchKernelGGL((kc_up<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w, (const T*)A.lv[l + 1].z);
+    hipLaunchKernelGGL((kc_up0<T, R, CI>), dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)), dim3(BLK), 0, s, Q.B,
+                       (const int*)Q.act, na, par, vals, (const T*)L0.dinv, w, r, (const int32_t*)L0.agg,
+                       (const T*)L1.z, z);
+    HIPCHK(hipGetLastError());
+    return RUV_OK;
+}
+
+template int bamg_apply<float, 1, 0>(BatchImpl&, int, int, const float*, float*);
+template int bamg_apply<float, 1, 1>(BatchImpl&, int, int, const float*, float*);
+template int bamg_apply<float, 2, 0>(BatchImpl&, int, int, const float*, float*);
+template int bamg_apply<float, 2, 1>(BatchImpl&, int, int, const float*, float*);
+template int bamg_apply<double, 1, 0>(BatchImpl&, int, int, const double*, double*);
+template int bamg_apply<double, 1, 1>(BatchImpl&, int, int, const double*, double*);
+template int bamg_apply<double, 2, 0>(BatchImpl&, int, int, const double*, double*);
+template int bamg_apply<double, 2, 1>(BatchImpl&, int, int, const double*, double*);
+
+int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows) {
+    if (!Q.amg) {
+        if (levels) *levels = 0;
+        if (setup_seconds) *setup_seconds = 0.0;
+        if (coarse_rows) *coarse_rows = 0;
+        return RUV_OK;
+    }
+    if (levels) *levels = (int)Q.amg->lv.size();
+    if (setup_seconds) *setup_seconds = Q.amg->setup_seconds;
+    if (coarse_rows) {
+        int64_t t = 0;
+        for (size_t l = 1; l < Q.amg->lv.size(); ++l) t += Q.amg->lv[l].n;
+        *coarse_rows = t;
+    }
+    return RUV_OK;
+}
+
+}  // namespace ruv

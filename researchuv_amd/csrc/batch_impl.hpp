@@ -1,0 +1,112 @@
+// batch_impl.hpp -- host state of one batch (shared by batch.hip and batch_amg.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "batch_kernels.hpp"
+#include "ruv_internal.hpp"
+
+namespace ruv {
+
+struct Bamg;  // device-built aggregation hierarchy (batch_amg.hip)
+void bamg_destroy(Bamg* a);
+
+#define HIPCHK(call)                                                                                         \
+    do {                                                                                                     \
+        hipError_t e_ = (call);                                                                              \
+        if (e_ != hipSuccess) return fail(RUV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct BatchImpl {
+    int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
+    int ci = 0;          // general slices: 0 = 16-bit column offsets, 1 = 32-bit columns
+    int spt = 2;         // slices per tile (chosen by the autotuner)
+    int nstencil = 0;    // slices stored in stencil form
+    int64_t nrows = 0, nnz = 0, nell = 0, ncol = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void*> mem;
+    BDev B{};
+    // host copies of the slice table and chart row offsets (tile building)
+    std::vector<int64_t> h_sl_row0;
+    std::vector<int32_t> h_sl_n;
+    std::vector<int32_t> chart_s0;  // first slice of each chart, nchart + 1
+    std::vector<int64_t> crow;      // first row of each chart, nchart + 1
+    std::vector<int> chart_t0;
+    std::vector<int64_t> h_tile_r0, h_tile_r1;  // rows of each tile (host copy)
+    // tiles (rebuilt by the autotuner)
+    int32_t *d_tile_s0 = nullptr, *d_tile_chart = nullptr, *d_chart_t0 = nullptr;
+    int64_t *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;
+    double *d_part_a = nullptr, *d_part_b = nullptr, *d_part_c = nullptr, *d_part_o = nullptr;
+    int* act = nullptr;
+    int* h_act = nullptr;
+    // matrix
+    float* ev32 = nullptr;
+    double* ev64 = nullptr;
+    float* dinv32 = nullptr;
+    double* dinv64 = nullptr;
+    // vectors
+    double *x64 = nullptr, *r64 = nullptr;
+    void *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv0 = nullptr, *pv1 = nullptr, *qv = nullptr;
+    int* h_done = nullptr;
+    int* h_final = nullptr;
+    unsigned* h_iters = nullptr;
+    double* h_true = nullptr;
+    std::vector<uint64_t> iters_total;
+    double inner_tol = 1e-4;
+    int max_rounds = 40;
+    // step-kernel accounting (optional): an event pair around every launch, summed after each solve
+    bool ktime = false;
+    std::vector<hipEvent_t> kev;
+    size_t kev_used = 0;
+    int64_t k_launches = 0;
+    double k_seconds = 0.0, k_bytes = 0.0;
+    std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
+    Bamg* amg = nullptr;                   // precond AMG: the hierarchy
+
+    template <typename T>
+    hipError_t alloc(T** p, size_t n) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) mem.push_back(q);
+        *p = (T*)q;
+        return e;
+    }
+    template <typename T>
+    hipError_t upload(T** p, const T* h, size_t n) {
+        hipError_t e = alloc(p, n);
+        if (e == hipSuccess && n) e = hipMemcpyAsync(*p, h, n * sizeof(T), hipMemcpyHostToDevice, stream);
+        return e;
+    }
+    void free_tiles() {
+        for (void* p : {(void*)d_tile_s0, (void*)d_tile_chart, (void*)d_chart_t0, (void*)d_tile_r0, (void*)d_tile_r1,
+                        (void*)d_part_a, (void*)d_part_b, (void*)d_part_c, (void*)d_part_o, (void*)act})
+            if (p) (void)hipFree(p);
+        d_tile_s0 = d_tile_chart = d_chart_t0 = nullptr;
+        d_tile_r0 = d_tile_r1 = nullptr;
+        d_part_a = d_part_b = d_part_c = d_part_o = nullptr;
+        act = nullptr;
+        if (h_act) (void)hipHostFree(h_act);
+        h_act = nullptr;
+    }
+    ~BatchImpl() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (amg) bamg_destroy(amg);
+        free_tiles();
+        for (void* p : mem) (void)hipFree(p);
+        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
+            if (p) (void)hipHostFree(p);
+        for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace ruv

@@ -226,8 +226,8 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
     if (threadIdx.x == 0) B.part_a[t] = s;
 }
 
-// alpha = rz / p.q;  x += alpha p;  r -= alpha q;  z = D^-1 r (PC 1; z aliases r for PC 0);
-// partials r.r and r.z for step parity par ^ 1.
+// alpha = rz / p.q;  x += alpha p;  r -= alpha q;  z = D^-1 r (PC 1; z aliases r for PC 0; PC 2 leaves z and
+// the r.z partials to the AMG cycle); partials r.r and r.z for step parity par ^ 1.
 template <typename T, int R, int PC>
 __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__ act, int na, int par,
                                                  const T* __restrict__ dinv, const T* __restrict__ p,
@@ -264,6 +264,7 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
         st<T, R>(r, i, ri);
         if (PC == 1) st<T, R>(z, i, zi);
     }
+    // PC 2 (AMG): the cycle that follows writes z and the r.z partials
     double srr, srz;
     {
         double va = wave_sum(arr), vb = wave_sum(arz);
@@ -283,12 +284,12 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
     }
     if (threadIdx.x == 0) {
         B.part_b[(int64_t)(par ^ 1) * B.ntile + t] = srr;
-        B.part_c[(int64_t)(par ^ 1) * B.ntile + t] = (PC == 1) ? srz : srr;
+        if (PC != 2) B.part_c[(int64_t)(par ^ 1) * B.ntile + t] = (PC == 1) ? srz : srr;
     }
 }
 
 // inner start on the live charts: x = 0, r = T(r64), z = D^-1 r, p = 0 (the first step's old direction);
-// partials r.r / r.z for parity 0
+// partials r.r / r.z for parity 0 (PC 2: z and r.z come from the AMG cycle)
 template <typename T, int R, int PC>
 __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ act, int na,
                                                const double* __restrict__ r64, const T* __restrict__ dinv,
@@ -314,19 +315,19 @@ __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ a
         st<T, R>(x, i, xi);
         st<T, R>(r, i, ri);
         if (PC == 1) st<T, R>(z, i, zi);
-        st<T, R>(p, i, xi);
+        st<T, R>(p, i, xi);  // xi is zero
     }
     const double srr = block_sum_all(arr, sh);
     const double srz = block_sum_all(arz, sh);
     if (threadIdx.x == 0) {
         B.part_b[t] = srr;
-        B.part_c[t] = srz;
+        if (PC != 2) B.part_c[t] = srz;
     }
 }
 
 // per chart (not final): inner tolerance, step state.  plain: tol * ||b|| on the recursive residual
 // (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
-__global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed) {
+static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= B.nchart) return;
     if (B.c_final[c]) {
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(BLK) void kb_start(BDev B, const double* __restrict
 }
 
 // per chart: ||b||, not final
-__global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {
+static __global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= B.nchart) return;
     double s = 0.0;
@@ -433,7 +434,7 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
 }
 
 // per chart: true residual; final when ||b - A x|| <= tol ||b||
-__global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
+static __global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= B.nchart || B.c_final[c]) return;
     double s = 0.0;
