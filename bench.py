@@ -164,26 +164,33 @@ def run_batch(args):
         torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    pc = getattr(solver.Precond, args.batch_precond)
     t0 = time.perf_counter()
-    bt = lb.device_batch(mixed=True)
+    bt = lb.device_batch(precond=pc, mixed=True)
     setup_s = time.perf_counter() - t0
     sched = bt.schedule()
-    log(f"mixed batch setup {setup_s:.2f}s (sliced ELL + autotune): {sched}")
+    log(f"mixed batch setup {setup_s:.2f}s (sliced ELL + autotune + preconditioner): {sched} {bt.precond_info()}")
+
+    def step():  # one batch: the preconditioner's device setup (AMG hierarchy) and the solve
+        bt.setup_precond()
+        return bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL, per_chart=True)
+
     for _ in range(args.warmup):
-        bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL)
+        step()
     torch.cuda.synchronize()
     bt.kernel_timing(True)  # HIP events around every step-kernel launch of the timed solves
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        info, it, res = bt.solve_dev(b.data_ptr(), x.data_ptr(), 100000, TOL, per_chart=True)
+        info, it, res = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ks = bt.kernel_stats()
     bt.kernel_timing(False)
     value = unknowns * args.steps / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    log(f"mixed batched: {ms_per_step:.1f} ms/batch, max inner iters {info.iters}, worst true resid "
-        f"{info.true_rel_resid:.2e}")
+    pinfo = bt.precond_info()
+    log(f"mixed batched ({args.batch_precond.lower()}): {ms_per_step:.1f} ms/batch, max inner iters {info.iters}, "
+        f"worst true resid {info.true_rel_resid:.2e}, preconditioner setup {pinfo}")
     achieved = ks["bytes"] / ks["seconds"]
     log(f"step kernel over the timed solves: {ks['launches']} launches, {1e6 * ks['seconds'] / ks['launches']:.1f} "
         f"us/launch, {ks['bytes'] / ks['launches'] / 1e9:.3f} GB/launch -> {achieved / 1e9:.0f} GB/s")
@@ -209,7 +216,7 @@ def run_batch(args):
         if len(db):
             worst = max(worst, float(np.linalg.norm(da - db, axis=1).max() /
                                      max(np.linalg.norm(db, axis=1).max(), 1e-300)))
-    log(f"fp64 batched: {el64 * 1e3:.1f} ms, worst true resid {info64.true_rel_resid:.2e}; "
+    log(f"fp64 batched (jacobi): {el64 * 1e3:.1f} ms, worst true resid {info64.true_rel_resid:.2e}; "
         f"mixed vs fp64 uv diff {worst:.2e}")
 
     # UV and energy error of the mixed solution against a CPU FP64 sparse LU (SuperLU), evenly spaced charts
@@ -255,20 +262,21 @@ def run_batch(args):
     log(f"fp64 drop-in per chart: {len(prepared)} charts, {unk_d} unknowns in {el_d:.2f}s -> {value_d:.3e} /s")
 
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel="kb_dir_spmv<float,2,%d>" % (0 if sched["col_bytes"] == 2 else 1),
+                traffic=None, kernel="kb_dir_spmv<float, 2, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1),
                 launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
                 us_per_launch=1e6 * ks["seconds"] / ks["launches"],
                 whole_batch_launch={"us": sec * 1e6, "bytes": byt, "GBps": byt / sec / 1e9})
-    tf = os.path.join(ROOT, "profiles", "traffic_batch.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_batch_{args.batch_precond.lower()}.json")
     if os.path.exists(tf):
         roof["traffic"] = json.load(open(tf)).get("hbm_bytes_per_launch")
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
         config={"workload": f"batch: {lb.nchart} uneven height-field patch charts (64-65536 vertices, log-uniform), "
                             f"{lb.nv_total} vertices, {unknowns} unknowns", "charts": lb.nchart,
-                "unknowns": unknowns, "nnz_L": lb.nnz, "precond": "jacobi", "tol_true_rel_resid": TOL,
-                "schedule": sched},
-        extra={"setup_s": setup_s, "max_inner_iters": int(info.iters), "mean_inner_iters": float(np.mean(it)),
+                "unknowns": unknowns, "nnz_L": lb.nnz, "precond": args.batch_precond.lower(),
+                "tol_true_rel_resid": TOL, "schedule": sched},
+        extra={"setup_s": setup_s, "precond_setup_in_step": pinfo, "max_inner_iters": int(info.iters),
+               "mean_inner_iters": float(np.mean(it)),
                "worst_true_rel_resid": float(info.true_rel_resid),
                "uv_rel_err_vs_cpu_lu": uv_err, "energy_rel_err_vs_cpu_lu": en_err,
                "cpu_lu_check": f"SuperLU on {len(pick)} evenly spaced charts of <= 20000 rows",
@@ -362,6 +370,8 @@ def main():
     ap.add_argument("--charts", type=int, default=4096, help="batch workload: number of charts")
     ap.add_argument("--dropin-charts", type=int, default=64, help="batch: charts in the per-chart drop-in sample")
     ap.add_argument("--precond", default="JACOBI", help="chart512: preconditioner name")
+    ap.add_argument("--batch-precond", default="AMG", choices=["JACOBI", "AMG"],
+                    help="batch: preconditioner of the mixed batched solve (AMG is built on the device each step)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -158,6 +158,12 @@ int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable);
  * took inside ruv_batch_create, and the rows of all coarse levels together; zeros for the other
  * preconditioners. */
 int ruv_batch_precond_info(ruv_batch* batch, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
+/* precond amg: rebuild the hierarchy on the device from the batch's operator (device work only; create builds
+ * it once); a no-op for the other preconditioners. */
+int ruv_batch_setup_precond(ruv_batch* batch);
+/* precond amg: rows of each level, finest first, into rows[0..cap); returns the number of levels (0 for the
+ * other preconditioners). */
+int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap);
 int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes);
 void ruv_batch_destroy(ruv_batch* batch);
 

@@ -33,6 +33,10 @@ def _setup(L):
     L.ruv_batch_schedule.restype = C.c_int
     L.ruv_batch_kernel_timing.restype = C.c_int
     L.ruv_batch_precond_info.restype = C.c_int
+    L.ruv_batch_setup_precond.restype = C.c_int
+    L.ruv_batch_precond_levels.restype = C.c_int
+    L.ruv_batch_precond_levels.argtypes = [P, P, C.c_int32]
+    L.ruv_batch_setup_precond.argtypes = [P]
     L.ruv_batch_precond_info.argtypes = [P, P, P, P]
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
     L.ruv_batch_kernel_stats.restype = C.c_int
@@ -131,13 +135,20 @@ class Batch:
         return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value, col_entries=nc.value,
                     stencil_slices=st.value, slices=ns.value)
 
+    def setup_precond(self):
+        """AMG: rebuild the hierarchy on the device (no-op for the element-wise preconditioners)."""
+        _lib.check(lib().ruv_batch_setup_precond(self.h))
+
     def precond_info(self) -> dict:
         """AMG: levels (finest included), device setup seconds, rows of all coarse levels."""
         lv = C.c_int32(0)
         sec = C.c_double(0)
         rows = C.c_int64(0)
         _lib.check(lib().ruv_batch_precond_info(self.h, C.byref(lv), C.byref(sec), C.byref(rows)))
-        return dict(levels=lv.value, setup_seconds=sec.value, coarse_rows=rows.value)
+        per = np.zeros(64, np.int64)
+        nl = lib().ruv_batch_precond_levels(self.h, _p(per), 64)
+        return dict(levels=lv.value, setup_seconds=sec.value, coarse_rows=rows.value,
+                    level_rows=[int(v) for v in per[:max(nl, 0)]])
 
     def kernel_timing(self, enable=True):
         """Bracket every step-kernel launch of the following solves with HIP events."""

@@ -25,6 +25,8 @@ int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int
                    int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
 int batch_kernel_timing(BatchImpl* Q, int enable);
 int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
+int batch_setup_precond(BatchImpl* Q);
+int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap);
 int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
@@ -339,6 +341,16 @@ int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* 
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
                                slices);
+}
+
+int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap) {
+    if (!batch || (cap > 0 && !rows)) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_precond_levels(batch->impl, rows, cap);
+}
+
+int ruv_batch_setup_precond(ruv_batch* batch) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_setup_precond(batch->impl);
 }
 
 int ruv_batch_precond_info(ruv_batch* batch, int32_t* levels, double* setup_seconds, int64_t* coarse_rows) {

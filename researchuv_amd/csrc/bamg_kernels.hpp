@@ -2,14 +2,19 @@
 // products) and the V(1,1) cycle.
 //
 // The hierarchy follows the reference's aggregation AMG (crates/researchuv-gpu/src/amg.rs:127-455): rows are
-// matched with their strongest unmatched neighbour, two matchings make one level (aggregates of up to four
-// rows, amg.rs SA_AGG = 4), the prolongator is the piecewise-constant aggregate map, the coarse operator is
-// the Galerkin product P^T A P, and the cycle smooths with damped Jacobi (omega = 2/3).  What differs: the
-// matching runs on the device by handshakes (a row proposes its strongest unmatched neighbour, mutual
-// proposals pair up, three rounds), every aggregate stays inside one chart -- and on the finest level inside
-// one workgroup tile -- so the hierarchy of a batch is block-diagonal by chart, and the coarsest level is one
-// Jacobi step (exact once a chart has coarsened to a single row).  No inner products: the cycle is a fixed
-// linear SPD operator, so each chart's PCG stays an independent PCG.
+// matched with their strongest unmatched neighbour, two matchings make one level (amg.rs SA_AGG = 4), the
+// prolongator is the piecewise-constant aggregate map, the coarse operator is the Galerkin product P^T A P,
+// and the cycle smooths with damped Jacobi (omega = 2/3).  What differs: the matching runs on the device by
+// handshakes (a row proposes its strongest unmatched neighbour whose coupling is at least 0.1 of the row's
+// strongest, mutual proposals pair up, four rounds), a row
+// left unmatched joins the pair it is most strongly coupled to, every aggregate stays inside one chart -- and
+// on the finest level inside one workgroup tile -- so the hierarchy of a batch is block-diagonal by chart, and
+// the coarsest level is one Jacobi step (exact once a chart has coarsened to a single row).  No inner
+// products: the cycle is a fixed linear SPD operator, so each chart's PCG stays an independent PCG.
+//
+// Aggregates are kept as member lists (CSR over the finer level's rows, ascending).  Coarse operators are
+// stored as sliced ELL (slices of 256 consecutive rows padded to their longest row, value k of row lane at
+// off + k * n_s + lane, 32-bit columns), so every coarse pass streams coalesced.
 //
 // Setup works on FP64 CSR levels (64-bit row pointers, 32-bit level-global columns).  Ties in the matching are
 // broken by the lower row index, and every Galerkin row is summed in a fixed order, so the hierarchy is
@@ -25,7 +30,6 @@
 
 namespace ruv {
 
-constexpr int AMG_MEMBERS = 4;  // fine rows per aggregate after two matchings
 constexpr int SCAN_BLK = 1024;  // elements per workgroup of the scan
 
 // ---------------------------------------------------------------- exclusive scan of int64 counts
@@ -142,50 +146,106 @@ __global__ __launch_bounds__(BLK) void k_accept(int64_t n, const int64_t* __rest
     if (j >= 0 && prop[j] == i) mate[i] = j;
 }
 
-// unmatched rows become singletons; leader flag = the lower row of its pair
-__global__ __launch_bounds__(BLK) void k_leaders(int64_t n, int64_t* __restrict__ mate, int64_t* __restrict__ flag) {
+// a row still unmatched joins the pair it is most strongly coupled to (same group, coupling >= theta * its
+// strongest); join = that pair's lower row, or -1 (the row stays alone)
+__global__ __launch_bounds__(BLK) void k_join(SetupCsr A, const double* __restrict__ smax,
+                                              const int32_t* __restrict__ group, const int64_t* __restrict__ mate,
+                                              int64_t* __restrict__ join, double theta) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= n) return;
-    if (mate[i] < 0) mate[i] = i;
-    flag[i] = (mate[i] >= i) ? 1 : 0;
-}
-
-// aggregate index of every row; leader list of every aggregate; group of every aggregate
-__global__ __launch_bounds__(BLK) void k_aggregate(int64_t n, const int64_t* __restrict__ mate,
-                                                   const int64_t* __restrict__ cid, const int32_t* __restrict__ group,
-                                                   int32_t* __restrict__ agg, int64_t* __restrict__ lead_of,
-                                                   int32_t* __restrict__ group_c) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= n) return;
-    const int64_t l = min(i, mate[i]);
-    agg[i] = (int32_t)cid[l];
-    if (i == l) {
-        lead_of[cid[i]] = i;
-        group_c[cid[i]] = group[i];
+    if (i >= A.n) return;
+    int64_t best = -1;
+    if (mate[i] < 0) {
+        const double thr = theta * smax[i];
+        const int32_t g = group[i];
+        double bs = 0.0;
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+            const int64_t j = A.cols[k];
+            if (j == i || !(A.vals[k] < 0.0) || mate[j] < 0 || group[j] != g) continue;
+            const double s = -A.vals[k];
+            if (s < thr) continue;
+            if (best < 0 || s > bs || (s == bs && j < best)) {
+                bs = s;
+                best = j;
+            }
+        }
+        if (best >= 0) best = min(best, mate[best]);
     }
+    join[i] = best;
 }
 
-// Galerkin row I = rows {l, mate[l]} with columns mapped through agg, duplicates summed.  The distinct coarse
-// columns are emitted in ascending order by repeated selection, each summed over the entries in row order.
+// roots: the lower row of a pair, and rows left alone
+__global__ __launch_bounds__(BLK) void k_roots(int64_t n, const int64_t* __restrict__ mate,
+                                               const int64_t* __restrict__ join, int64_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (mate[i] >= 0) ? (i < mate[i] ? 1 : 0) : (join[i] < 0 ? 1 : 0);
+}
+
+__global__ __launch_bounds__(BLK) void k_root_agg(int64_t n, const int64_t* __restrict__ mate,
+                                                  const int64_t* __restrict__ join, const int64_t* __restrict__ cid,
+                                                  int32_t* __restrict__ agg) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int64_t root = (mate[i] >= 0) ? min(i, mate[i]) : (join[i] >= 0 ? join[i] : i);
+    agg[i] = (int32_t)cid[root];
+}
+
+// member lists of aggregates from a row -> aggregate map: counts, (scan), slots, then each list sorted
+__global__ __launch_bounds__(BLK) void k_member_count(int64_t n, const int32_t* __restrict__ agg,
+                                                      unsigned long long* __restrict__ cnt) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[agg[i]], 1ull);
+}
+
+__global__ __launch_bounds__(BLK) void k_member_fill(int64_t n, const int32_t* __restrict__ agg,
+                                                     const int64_t* __restrict__ mptr,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     int64_t* __restrict__ mrows) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int64_t a = agg[i];
+    const unsigned long long k = atomicAdd(&cursor[a], 1ull);
+    mrows[mptr[a] + (int64_t)k] = i;
+}
+
+__global__ __launch_bounds__(BLK) void k_member_sort(int64_t nc, const int64_t* __restrict__ mptr,
+                                                     int64_t* __restrict__ mrows, const int32_t* __restrict__ group,
+                                                     int32_t* __restrict__ group_c) {
+    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (I >= nc) return;
+    const int64_t a = mptr[I], b = mptr[I + 1];
+    for (int64_t p = a + 1; p < b; ++p) {
+        const int64_t v = mrows[p];
+        int64_t q = p - 1;
+        while (q >= a && mrows[q] > v) {
+            mrows[q + 1] = mrows[q];
+            --q;
+        }
+        mrows[q + 1] = v;
+    }
+    if (group_c && b > a) group_c[I] = group[mrows[a]];
+}
+
+// Galerkin row I = the rows of aggregate I with columns mapped through agg, duplicates summed.  The distinct
+// coarse columns are emitted in ascending order by repeated selection, each summed over the entries in row
+// order.
 template <bool FILL>
-__global__ __launch_bounds__(BLK) void k_galerkin(int64_t nc, SetupCsr A, const int64_t* __restrict__ lead_of,
-                                                  const int64_t* __restrict__ mate, const int32_t* __restrict__ agg,
+__global__ __launch_bounds__(BLK) void k_galerkin(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
+                                                  const int64_t* __restrict__ mrows, const int32_t* __restrict__ agg,
                                                   int64_t* __restrict__ cnt, const int64_t* __restrict__ cptr,
                                                   int32_t* __restrict__ ccols, double* __restrict__ cvals,
                                                   double* __restrict__ cdiag) {
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (I >= nc) return;
-    const int64_t r0 = lead_of[I], r1 = mate[r0];
-    const int nr = (r1 != r0) ? 2 : 1;
+    const int64_t m0 = mptr[I], m1 = mptr[I + 1];
     int64_t last = -1;
     int64_t out = FILL ? cptr[I] : 0;
     int64_t count = 0;
     double diag = 0.0;
     for (;;) {
-        // smallest coarse column > last
-        int64_t key = INT64_MAX;
-        for (int q = 0; q < nr; ++q) {
-            const int64_t r = q ? r1 : r0;
+        int64_t key = INT64_MAX;  // smallest coarse column > last
+        for (int64_t q = m0; q < m1; ++q) {
+            const int64_t r = mrows[q];
             for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
                 const int64_t c = agg[A.cols[k]];
                 if (c > last && c < key) key = c;
@@ -194,8 +254,8 @@ __global__ __launch_bounds__(BLK) void k_galerkin(int64_t nc, SetupCsr A, const 
         if (key == INT64_MAX) break;
         if (FILL) {
             double v = 0.0;
-            for (int q = 0; q < nr; ++q) {
-                const int64_t r = q ? r1 : r0;
+            for (int64_t q = m0; q < m1; ++q) {
+                const int64_t r = mrows[q];
                 for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k)
                     if (agg[A.cols[k]] == key) v += A.vals[k];
             }
@@ -211,26 +271,40 @@ __global__ __launch_bounds__(BLK) void k_galerkin(int64_t nc, SetupCsr A, const 
     else cdiag[I] = diag;
 }
 
-// composed map of two matchings: fine row -> level aggregate, and each aggregate's (up to 4) fine members
-__global__ __launch_bounds__(BLK) void k_compose(int64_t nc2, const int64_t* __restrict__ lead2,
-                                                 const int64_t* __restrict__ mate2, const int64_t* __restrict__ lead1,
-                                                 const int64_t* __restrict__ mate1, int64_t* __restrict__ members) {
-    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (I >= nc2) return;
-    int64_t m[AMG_MEMBERS] = {-1, -1, -1, -1};
-    int k = 0;
-    const int64_t j0 = lead2[I], j1 = mate2[j0];
-    for (int q = 0; q < 2; ++q) {
-        const int64_t j = q ? j1 : j0;
-        if (q && j1 == j0) break;
-        const int64_t i0 = lead1[j], i1 = mate1[i0];
-        m[k++] = i0;
-        if (i1 != i0) m[k++] = i1;
-    }
-#pragma unroll
-    for (int q = 0; q < AMG_MEMBERS; ++q) members[I * AMG_MEMBERS + q] = m[q];
+// sliced ELL of a coarse operator: slice widths, then the fill (padding: value 0, the row itself)
+__global__ __launch_bounds__(BLK) void k_slice_sizes(int64_t n, const int64_t* __restrict__ ptr,
+                                                     int32_t* __restrict__ w, int64_t* __restrict__ sz) {
+    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    const int64_t nsl = (n + SLICE - 1) / SLICE;
+    if (s >= nsl) return;
+    const int64_t r0 = s * SLICE, r1 = min(n, r0 + SLICE);
+    int64_t m = 0;
+    for (int64_t i = r0; i < r1; ++i) m = max(m, ptr[i + 1] - ptr[i]);
+    w[s] = (int32_t)m;
+    sz[s] = m * (r1 - r0);
 }
 
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ w, T* __restrict__ ev,
+                                                  int32_t* __restrict__ ec) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int64_t s = i / SLICE, lane = i - s * SLICE;
+    const int64_t ns = min((int64_t)SLICE, n - s * SLICE);
+    const int64_t e0 = off[s] + lane;
+    int k = 0;
+    for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
+        ev[e0 + k * ns] = (T)A.vals[a];
+        ec[e0 + k * ns] = A.cols[a];
+    }
+    for (; k < w[s]; ++k) {
+        ev[e0 + k * ns] = T(0);
+        ec[e0 + k * ns] = (int32_t)i;
+    }
+}
+
+// composed map of two matchings: fine row -> level aggregate
 __global__ __launch_bounds__(BLK) void k_compose_agg(int64_t n, const int32_t* __restrict__ agg1,
                                                      const int32_t* __restrict__ agg2, int32_t* __restrict__ agg) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -267,40 +341,65 @@ __global__ __launch_bounds__(BLK) void k_cast(int64_t n, const double* __restric
 template <typename T>
 struct CycLevel {
     int64_t n;
-    const int64_t* ptr;      // level operator (levels >= 1)
-    const int32_t* cols;
-    const T* vals;
+    const int64_t* eoff;     // sliced ELL of the level operator (levels >= 1)
+    const int32_t* ew;
+    const int32_t* ecols;
+    const T* evals;
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
-    const int64_t* members;  // next level's aggregates -> AMG_MEMBERS rows of this level (-1 padded)
+    const int64_t* mptr;     // next level's aggregates -> member rows here
+    const int64_t* mrows;
     T* b;                    // right-hand side on this level
     T* z;                    // correction on this level
+    T* t;                    // work vector on this level
 };
 
-// level >= 1, per aggregate I of the next level: b_next[I] = sum over members m of (b - A (w D^-1 b))_m
+// slice, lane, slice height of row i of a level's sliced ELL
+struct EllRow {
+    int64_t e0, ns;
+    int w;
+};
+template <typename T>
+__device__ __forceinline__ EllRow ell_row(const CycLevel<T>& L, int64_t i) {
+    const int64_t s = i / SLICE;
+    return EllRow{L.eoff[s] + (i - s * SLICE), min((int64_t)SLICE, L.n - s * SLICE), L.ew[s]};
+}
+
+// level >= 1: t = b - A (w D^-1 b)
 template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, int64_t nc, T w, T* __restrict__ bnext) {
+__global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= L.n) return;
+    const EllRow e = ell_row(L, i);
+    const Vr<T, R> bi = ld<T, R>(L.b, i);
+    T s[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) s[c] = bi.c[c];
+    for (int k = 0; k < e.w; ++k) {
+        const int64_t j = L.ecols[e.e0 + k * e.ns];
+        const T v = L.evals[e.e0 + k * e.ns] * (w * L.dinv[j]);
+        const Vr<T, R> bj = ld<T, R>(L.b, j);
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c] = fma(-v, bj.c[c], s[c]);
+    }
+    Vr<T, R> o;
+#pragma unroll
+    for (int c = 0; c < R; ++c) o.c[c] = s[c];
+    st<T, R>(L.t, i, o);
+}
+
+// level >= 1, per aggregate I of the next level: b_next[I] = sum over its member rows of t
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, T* __restrict__ bnext) {
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (I >= nc) return;
     T acc[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) acc[c] = T(0);
-    for (int q = 0; q < AMG_MEMBERS; ++q) {
-        const int64_t m = L.members[I * AMG_MEMBERS + q];
-        if (m < 0) break;
-        const Vr<T, R> bm = ld<T, R>(L.b, m);
-        T s[R];
+    for (int64_t q = L.mptr[I]; q < L.mptr[I + 1]; ++q) {
+        const Vr<T, R> tm = ld<T, R>(L.t, L.mrows[q]);
 #pragma unroll
-        for (int c = 0; c < R; ++c) s[c] = bm.c[c];
-        for (int64_t k = L.ptr[m]; k < L.ptr[m + 1]; ++k) {
-            const int64_t j = L.cols[k];
-            const T v = L.vals[k] * (w * L.dinv[j]);
-            const Vr<T, R> bj = ld<T, R>(L.b, j);
-#pragma unroll
-            for (int c = 0; c < R; ++c) s[c] = fma(-v, bj.c[c], s[c]);
-        }
-#pragma unroll
-        for (int c = 0; c < R; ++c) acc[c] += s[c];
+        for (int c = 0; c < R; ++c) acc[c] += tm.c[c];
     }
     Vr<T, R> o;
 #pragma unroll
@@ -308,32 +407,42 @@ __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, int64_t nc, T w, T
     st<T, R>(bnext, I, o);
 }
 
-// level >= 1, per row: e1 = w D^-1 b + P z_next, then z = e1 + w D^-1 (b - A e1)
+// level >= 1: t = e1 = w D^-1 b + P z_next
 template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kc_up(CycLevel<T> L, T w, const T* __restrict__ znext) {
+__global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* __restrict__ znext) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= L.n) return;
-    T s[R];
-#pragma unroll
-    for (int c = 0; c < R; ++c) s[c] = T(0);
-    for (int64_t k = L.ptr[i]; k < L.ptr[i + 1]; ++k) {
-        const int64_t j = L.cols[k];
-        const T dj = w * L.dinv[j];
-        const Vr<T, R> bj = ld<T, R>(L.b, j);
-        const Vr<T, R> zj = ld<T, R>(znext, L.agg[j]);
-        const T v = L.vals[k];
-#pragma unroll
-        for (int c = 0; c < R; ++c) s[c] = fma(v, fma(dj, bj.c[c], zj.c[c]), s[c]);
-    }
     const T di = w * L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     const Vr<T, R> zc = ld<T, R>(znext, L.agg[i]);
     Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) {
-        const T e1 = fma(di, bi.c[c], zc.c[c]);
-        o.c[c] = fma(di, bi.c[c] - s[c], e1);
+    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c], zc.c[c]);
+    st<T, R>(L.t, i, o);
+}
+
+// level >= 1: z = e1 + w D^-1 (b - A e1), e1 in t
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= L.n) return;
+    const EllRow e = ell_row(L, i);
+    T s[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) s[c] = T(0);
+    for (int k = 0; k < e.w; ++k) {
+        const int64_t j = L.ecols[e.e0 + k * e.ns];
+        const T v = L.evals[e.e0 + k * e.ns];
+        const Vr<T, R> ej = ld<T, R>(L.t, j);
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c] = fma(v, ej.c[c], s[c]);
     }
+    const T di = w * L.dinv[i];
+    const Vr<T, R> bi = ld<T, R>(L.b, i);
+    const Vr<T, R> ei = ld<T, R>(L.t, i);
+    Vr<T, R> o;
+#pragma unroll
+    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c] - s[c], ei.c[c]);
     st<T, R>(L.z, i, o);
 }
 
@@ -356,8 +465,9 @@ __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_down0(BDev B, const int* __restrict__ act, int na, const T* __restrict__ vals,
                                                 const T* __restrict__ dinv, T w, const T* __restrict__ r,
-                                                const int64_t* __restrict__ tile_c0, const int64_t* __restrict__ members,
-                                                T* __restrict__ b1, int max_tile_rows) {
+                                                const int64_t* __restrict__ tile_c0, const int64_t* __restrict__ mptr,
+                                                const int64_t* __restrict__ mrows, T* __restrict__ b1,
+                                                int max_tile_rows) {
     extern __shared__ char smem[];
     T* tl = reinterpret_cast<T*>(smem);  // max_tile_rows * R
     const int slot = xcd_slot(na);
@@ -397,9 +507,8 @@ __global__ __launch_bounds__(BLK) void kc_down0(BDev B, const int* __restrict__ 
         T acc[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        for (int q = 0; q < AMG_MEMBERS; ++q) {
-            const int64_t m = members[I * AMG_MEMBERS + q];
-            if (m < 0) break;
+        for (int64_t q = mptr[I]; q < mptr[I + 1]; ++q) {
+            const int64_t m = mrows[q];
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) acc[cc] += tl[(m - r0) * R + cc];
         }

@@ -30,10 +30,12 @@
 
 namespace ruv {
 
-int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals);
+int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals);
+int bamg_build(BatchImpl& Q);
 template <typename T, int R, int CI>
 int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z);
 int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows);
+int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap);
 
 namespace {
 
@@ -538,7 +540,8 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     rc = autotune(*Q);
     if (rc) return rc;
     if (precond == RUV_PC_AMG) {  // the finest aggregates follow the tiles, so after the autotuner
-        rc = bamg_build(*Q, ptr, cols, vals);
+        rc = bamg_upload_fine(*Q, ptr, cols, vals);
+        if (!rc) rc = bamg_build(*Q);
         if (rc) return rc;
     }
     *out = Q.release();
@@ -602,6 +605,10 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
     *bytes = b;
     return RUV_OK;
 }
+
+int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap) { return bamg_level_rows(*Q, rows, cap); }
+
+int batch_setup_precond(BatchImpl* Q) { return Q->precond == RUV_PC_AMG ? bamg_build(*Q) : RUV_OK; }
 
 int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows) {
     int lv = 0;

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -13,15 +14,18 @@
 namespace ruv {
 
 struct BamgLevel {
-    int64_t n = 0, nnz = 0;
-    int64_t* ptr = nullptr;  // levels >= 1
-    int32_t* cols = nullptr;
-    void* vals = nullptr;    // T
-    void* dinv = nullptr;    // T (level 0: the batch's Jacobi diagonal)
-    int32_t* agg = nullptr;      // row -> next-level aggregate
-    int64_t* members = nullptr;  // next level's aggregates -> AMG_MEMBERS rows here
+    int64_t n = 0, nell = 0;
+    int64_t* eoff = nullptr;  // sliced ELL of the operator (levels >= 1)
+    int32_t* ew = nullptr;
+    int32_t* ecols = nullptr;
+    void* evals = nullptr;    // T
+    void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal)
+    int32_t* agg = nullptr;   // row -> next-level aggregate
+    int64_t* mptr = nullptr;  // next level's aggregates -> member rows here
+    int64_t* mrows = nullptr;
     void* b = nullptr;
     void* z = nullptr;
+    void* t = nullptr;
 };
 
 struct Bamg {
@@ -79,13 +83,38 @@ int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) 
     return RUV_OK;
 }
 
-// One matching of A within groups and its Galerkin operator.
+// Member lists (ascending, CSR over rows) of the nc aggregates of a row -> aggregate map; with group given,
+// also each aggregate's group.
+template <typename Alloc>
+int build_members(hipStream_t s, int64_t n, const int32_t* agg, int64_t nc, const int32_t* group, Alloc& S,
+                  int64_t** mptr, int64_t** mrows, int32_t* group_c) {
+    unsigned long long *cnt = nullptr, *cursor = nullptr;
+    Scratch T;
+    HIPCHK(T.alloc(&cnt, nc));
+    HIPCHK(T.alloc(&cursor, nc));
+    HIPCHK(S.alloc(mptr, nc + 1));
+    HIPCHK(S.alloc(mrows, n));
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * nc, s));
+    HIPCHK(hipMemsetAsync(cursor, 0, sizeof(unsigned long long) * nc, s));
+    hipLaunchKernelGGL(k_member_count, rows_grid(n), dim3(BLK), 0, s, n, agg, cnt);
+    HIPCHK(hipGetLastError());
+    int rc = scan(s, nc, (const int64_t*)cnt, *mptr, T);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_member_fill, rows_grid(n), dim3(BLK), 0, s, n, agg, (const int64_t*)*mptr, cursor, *mrows);
+    hipLaunchKernelGGL(k_member_sort, rows_grid(nc), dim3(BLK), 0, s, nc, (const int64_t*)*mptr, *mrows, group,
+                       group_c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));  // the scratch counters die here
+    return RUV_OK;
+}
+
+// One matching of A within groups (pairs, then lone rows joining pairs) and its Galerkin operator.
 struct Matched {
     int64_t nc = 0;
-    int64_t* mate = nullptr;
-    int64_t* cid = nullptr;  // exclusive scan of leader flags, n + 1
+    int64_t* cid = nullptr;  // exclusive scan of root flags, n + 1
     int32_t* agg = nullptr;
-    int64_t* lead = nullptr;
+    int64_t* mptr = nullptr;
+    int64_t* mrows = nullptr;
     int32_t* group = nullptr;  // of the aggregates
     SetupCsr C{};
     double* cdiag = nullptr;
@@ -95,36 +124,41 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
                       Scratch& S, Matched* M) {
     const int64_t n = A.n;
     double* smax = nullptr;
-    int64_t *prop = nullptr, *flag = nullptr;
+    int64_t *prop = nullptr, *flag = nullptr, *mate = nullptr, *join = nullptr;
     HIPCHK(S.alloc(&smax, n));
     HIPCHK(S.alloc(&prop, n));
     HIPCHK(S.alloc(&flag, n));
-    HIPCHK(S.alloc(&M->mate, n));
+    HIPCHK(S.alloc(&mate, n));
+    HIPCHK(S.alloc(&join, n));
     HIPCHK(S.alloc(&M->cid, n + 1));
     HIPCHK(S.alloc(&M->agg, n));
-    HIPCHK(hipMemsetAsync(M->mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
+    HIPCHK(hipMemsetAsync(mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
     hipLaunchKernelGGL(k_smax, rows_grid(n), dim3(BLK), 0, s, A, smax);
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, A, (const double*)smax, group,
-                           (const int64_t*)M->mate, prop, theta);
-        hipLaunchKernelGGL(k_accept, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)prop, M->mate);
+                           (const int64_t*)mate, prop, theta);
+        hipLaunchKernelGGL(k_accept, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)prop, mate);
     }
-    hipLaunchKernelGGL(k_leaders, rows_grid(n), dim3(BLK), 0, s, n, M->mate, flag);
+    hipLaunchKernelGGL(k_join, rows_grid(n), dim3(BLK), 0, s, A, (const double*)smax, group, (const int64_t*)mate,
+                       join, theta);
+    hipLaunchKernelGGL(k_roots, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join, flag);
     HIPCHK(hipGetLastError());
     int rc = scan(s, n, flag, M->cid, S);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(&M->nc, M->cid + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const int64_t nc = M->nc;
-    HIPCHK(S.alloc(&M->lead, nc));
+    hipLaunchKernelGGL(k_root_agg, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join,
+                       (const int64_t*)M->cid, M->agg);
+    HIPCHK(hipGetLastError());
     HIPCHK(S.alloc(&M->group, nc));
-    hipLaunchKernelGGL(k_aggregate, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)M->mate,
-                       (const int64_t*)M->cid, group, M->agg, M->lead, M->group);
+    rc = build_members(s, n, M->agg, nc, group, S, &M->mptr, &M->mrows, M->group);
+    if (rc) return rc;
     int64_t *cnt = nullptr, *cptr = nullptr;
     HIPCHK(S.alloc(&cnt, nc));
     HIPCHK(S.alloc(&cptr, nc + 1));
-    hipLaunchKernelGGL((k_galerkin<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->lead,
-                       (const int64_t*)M->mate, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
+    hipLaunchKernelGGL((k_galerkin<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+                       (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
                        (int32_t*)nullptr, (double*)nullptr, (double*)nullptr);
     HIPCHK(hipGetLastError());
     rc = scan(s, nc, cnt, cptr, S);
@@ -134,12 +168,12 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(hipStreamSynchronize(s));
     if (cnnz >= INT32_MAX) return fail(RUV_ERR_PRECOND, "batch amg: coarse operator too large");
     int32_t* ccols = nullptr;
-    double *cvals = nullptr;
+    double* cvals = nullptr;
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
     HIPCHK(S.alloc(&M->cdiag, nc));
-    hipLaunchKernelGGL((k_galerkin<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->lead,
-                       (const int64_t*)M->mate, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
+    hipLaunchKernelGGL((k_galerkin<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+                       (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
                        ccols, cvals, M->cdiag);
     HIPCHK(hipGetLastError());
     M->C = SetupCsr{nc, cptr, ccols, cvals};
@@ -154,9 +188,25 @@ __global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, cons
 
 }  // namespace
 
-// Build the hierarchy from the host CSR of the batch (the finest operator; tiles already fixed).
-int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals) {
+// Keep the finest operator (FP64 CSR) and the tile groups on the device for (re)builds of the hierarchy.
+int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals) {
     hipStream_t s = Q.stream;
+    const int64_t n0 = Q.nrows;
+    HIPCHK(Q.upload(&Q.amg_ptr, ptr, (size_t)n0 + 1));
+    HIPCHK(Q.upload(&Q.amg_cols, cols, (size_t)Q.nnz));
+    HIPCHK(Q.upload(&Q.amg_vals, vals, (size_t)Q.nnz));
+    std::vector<int32_t> g(n0);
+    for (int t = 0; t < Q.ntile; ++t)
+        for (int64_t i = Q.h_tile_r0[t]; i < Q.h_tile_r1[t]; ++i) g[i] = t;
+    HIPCHK(Q.upload(&Q.amg_group, g.data(), g.size()));
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
+// Build the hierarchy on the device from the finest operator kept by bamg_upload_fine (tiles fixed).
+int bamg_build(BatchImpl& Q) {
+    hipStream_t s = Q.stream;
+    if (!Q.amg_ptr) return fail(RUV_ERR_INVALID, "batch amg: no finest operator on the device");
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -165,29 +215,14 @@ int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const doub
     const bool f32 = Q.mode == RUV_MODE_MIXED;
     const size_t sv = f32 ? sizeof(float) : sizeof(double);
     const int R = Q.R;
-    const double theta = 0.25;
-    const int rounds = 3;
-    // finest operator in FP64 CSR for the setup, and the tile groups
-    Scratch S0;
-    int64_t* d_ptr = nullptr;
-    int32_t *d_cols = nullptr, *group = nullptr;
-    double* d_vals = nullptr;
+    // matching: coupling threshold relative to a row's strongest, handshake rounds (overridable for tuning)
+    double theta = 0.1;
+    int rounds = 4;
+    if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
+    if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));
     const int64_t n0 = Q.nrows;
-    HIPCHK(S0.alloc(&d_ptr, n0 + 1));
-    HIPCHK(S0.alloc(&d_cols, Q.nnz));
-    HIPCHK(S0.alloc(&d_vals, Q.nnz));
-    HIPCHK(S0.alloc(&group, n0));
-    HIPCHK(hipMemcpyAsync(d_ptr, ptr, sizeof(int64_t) * (n0 + 1), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_cols, cols, sizeof(int32_t) * Q.nnz, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_vals, vals, sizeof(double) * Q.nnz, hipMemcpyHostToDevice, s));
-    {
-        std::vector<int32_t> g(n0);
-        for (int t = 0; t < Q.ntile; ++t)
-            for (int64_t i = Q.h_tile_r0[t]; i < Q.h_tile_r1[t]; ++i) g[i] = t;
-        HIPCHK(hipMemcpyAsync(group, g.data(), sizeof(int32_t) * n0, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    SetupCsr cur{n0, d_ptr, d_cols, d_vals};
+    const int32_t* group = Q.amg_group;
+    SetupCsr cur{n0, Q.amg_ptr, Q.amg_cols, Q.amg_vals};
     const double* cur_diag = nullptr;
     std::vector<int64_t> crow = Q.crow;
     std::unique_ptr<Scratch> keep(new Scratch());  // owns the current level's setup CSR (levels >= 1)
@@ -199,32 +234,43 @@ int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const doub
         if (l == 0) {
             L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
         } else {
-            int64_t nnz = 0;
-            HIPCHK(hipMemcpyAsync(&nnz, cur.ptr + cur.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            // sliced ELL of the level operator
+            const int64_t nsl = (cur.n + SLICE - 1) / SLICE;
+            int64_t* sz = nullptr;
+            Scratch T;
+            HIPCHK(T.alloc(&sz, nsl));
+            HIPCHK(A->alloc(&L.ew, nsl));
+            HIPCHK(A->alloc(&L.eoff, nsl + 1));
+            hipLaunchKernelGGL(k_slice_sizes, rows_grid(nsl), dim3(BLK), 0, s, cur.n, cur.ptr, L.ew, sz);
+            HIPCHK(hipGetLastError());
+            int rc = scan(s, nsl, sz, L.eoff, T);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            L.nnz = nnz;
-            HIPCHK(A->alloc(&L.ptr, cur.n + 1));
-            HIPCHK(A->alloc(&L.cols, nnz));
-            HIPCHK(hipMemcpyAsync(L.ptr, cur.ptr, sizeof(int64_t) * (cur.n + 1), hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipMemcpyAsync(L.cols, cur.cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s));
             char *v = nullptr, *d = nullptr;
-            HIPCHK(A->alloc(&v, nnz * sv));
+            HIPCHK(A->alloc(&L.ecols, L.nell));
+            HIPCHK(A->alloc(&v, L.nell * sv));
             HIPCHK(A->alloc(&d, cur.n * sv));
             if (f32) {
-                hipLaunchKernelGGL((k_cast<float>), rows_grid(nnz), dim3(BLK), 0, s, nnz, cur.vals, (float*)v);
+                hipLaunchKernelGGL((k_ell_fill<float>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
+                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (float*)v, L.ecols);
                 hipLaunchKernelGGL((k_cast_inv<float>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (float*)d);
             } else {
-                hipLaunchKernelGGL((k_cast<double>), rows_grid(nnz), dim3(BLK), 0, s, nnz, cur.vals, (double*)v);
+                hipLaunchKernelGGL((k_ell_fill<double>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
+                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (double*)v, L.ecols);
                 hipLaunchKernelGGL((k_cast_inv<double>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (double*)d);
             }
             HIPCHK(hipGetLastError());
-            L.vals = v;
+            HIPCHK(hipStreamSynchronize(s));
+            L.evals = v;
             L.dinv = d;
-            char *b = nullptr, *z = nullptr;
+            char *b = nullptr, *z = nullptr, *t = nullptr;
             HIPCHK(A->alloc(&b, cur.n * R * sv));
             HIPCHK(A->alloc(&z, cur.n * R * sv));
+            HIPCHK(A->alloc(&t, cur.n * R * sv));
             L.b = b;
             L.z = z;
+            L.t = t;
         }
         const bool last = (l + 1 >= max_levels) || (l > 0 && cur.n <= (int64_t)Q.nchart);
         if (last) {
@@ -239,17 +285,15 @@ int bamg_build(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const doub
         rc = match_and_coarsen(s, M1.C, M1.group, theta, rounds, *S2, &M2);
         if (rc) return rc;
         if (l > 0 && M2.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
-            L.ptr = nullptr;
             A->lv.push_back(L);
             break;
         }
         HIPCHK(A->alloc(&L.agg, cur.n));
-        HIPCHK(A->alloc(&L.members, (size_t)M2.nc * AMG_MEMBERS));
         hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
                            (const int32_t*)M2.agg, L.agg);
-        hipLaunchKernelGGL(k_compose, rows_grid(M2.nc), dim3(BLK), 0, s, M2.nc, (const int64_t*)M2.lead,
-                           (const int64_t*)M2.mate, (const int64_t*)M1.lead, (const int64_t*)M1.mate, L.members);
         HIPCHK(hipGetLastError());
+        rc = build_members(s, cur.n, L.agg, M2.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows, (int32_t*)nullptr);
+        if (rc) return rc;
         // chart offsets on the next level (and, on the finest, each tile's aggregate range)
         {
             std::vector<int64_t> at(crow.begin(), crow.end());
@@ -334,23 +378,28 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     BamgLevel& L1 = A.lv[1];
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
     const size_t lds = (size_t)A.max_tile_rows * R * sizeof(T);
-    hipLaunchKernelGGL((kc_down0<T, R, CI>), dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)), dim3(BLK), lds, s, Q.B,
-                       (const int*)Q.act, na, vals, (const T*)L0.dinv, w, r, (const int64_t*)A.tile_c0,
-                       (const int64_t*)L0.members, (T*)L1.b, A.max_tile_rows);
+    const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
+    hipLaunchKernelGGL((kc_down0<T, R, CI>), gt, dim3(BLK), lds, s, Q.B, (const int*)Q.act, na, vals,
+                       (const T*)L0.dinv, w, r, (const int64_t*)A.tile_c0, (const int64_t*)L0.mptr,
+                       (const int64_t*)L0.mrows, (T*)L1.b, A.max_tile_rows);
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
-        CycLevel<T> c{L.n, L.ptr, L.cols, (const T*)L.vals, (const T*)L.dinv, L.agg, L.members, (T*)L.b, (T*)L.z};
-        return c;
+        return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
+                           (T*)L.b, (T*)L.z, (T*)L.t};
     };
-    for (size_t l = 1; l + 1 < nl; ++l)
-        hipLaunchKernelGGL((kc_down<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n, w,
+    for (size_t l = 1; l + 1 < nl; ++l) {
+        hipLaunchKernelGGL((kc_resid<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
                            (T*)A.lv[l + 1].b);
+    }
     hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
-    for (size_t l = nl - 1; l-- > 1;)
-        hipLaunchKernelGGL((kc_up<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w, (const T*)A.lv[l + 1].z);
-    hipLaunchKernelGGL((kc_up0<T, R, CI>), dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)), dim3(BLK), 0, s, Q.B,
-                       (const int*)Q.act, na, par, vals, (const T*)L0.dinv, w, r, (const int32_t*)L0.agg,
-                       (const T*)L1.z, z);
+    for (size_t l = nl - 1; l-- > 1;) {
+        hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w,
+                           (const T*)A.lv[l + 1].z);
+        hipLaunchKernelGGL((kc_post<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+    }
+    hipLaunchKernelGGL((kc_up0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
+                       (const T*)L0.dinv, w, r, (const int32_t*)L0.agg, (const T*)L1.z, z);
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }
@@ -363,6 +412,13 @@ template int bamg_apply<double, 1, 0>(BatchImpl&, int, int, const double*, doubl
 template int bamg_apply<double, 1, 1>(BatchImpl&, int, int, const double*, double*);
 template int bamg_apply<double, 2, 0>(BatchImpl&, int, int, const double*, double*);
 template int bamg_apply<double, 2, 1>(BatchImpl&, int, int, const double*, double*);
+
+int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap) {
+    if (!Q.amg) return 0;
+    const int nl = (int)Q.amg->lv.size();
+    for (int l = 0; l < nl && l < cap; ++l) rows[l] = Q.amg->lv[l].n;
+    return nl;
+}
 
 int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows) {
     if (!Q.amg) {

@@ -69,6 +69,10 @@ struct BatchImpl {
     double k_seconds = 0.0, k_bytes = 0.0;
     std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
     Bamg* amg = nullptr;                   // precond AMG: the hierarchy
+    int64_t* amg_ptr = nullptr;            // precond AMG: the finest operator in FP64 CSR for (re)builds,
+    int32_t* amg_cols = nullptr;           // and each row's finest-level matching group (its tile)
+    double* amg_vals = nullptr;
+    int32_t* amg_group = nullptr;
 
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
