@@ -15,7 +15,9 @@ of the charts), the UV and LSCM-energy error of the mixed solution against a CPU
 sample of charts, the step kernel's roofline (timed with HIP events on the solver's stream) and the CPU
 oracle's worker-thread sparse-LU path on a bounded sample of the same patch family.
 
-`--workload chart512`: the single 512 x 512 vertex height-field chart (one solve per step).
+`--workload chart512`: the single 512 x 512 vertex height-field chart; `--workload stretched`: the
+ill-conditioned 2048 x 2048 planar grid (column spacing 1 -> 100, 5% flipped diagonals).  Both solve one chart
+through the batched engine (a batch of one) with the device-built AMG, the plan drop-in timed beside it.
 """
 from __future__ import annotations
 
@@ -35,16 +37,6 @@ TOL = 1e-8
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
-
-
-def build_chart512(seed=1, n=512):
-    from researchuv_amd import lscm, meshgen
-
-    pos, faces = meshgen.heightfield(n, n, seed=seed)
-    pw, fw = lscm.weld(pos, faces, 1e-12)
-    seg = lscm.segment(pw, fw, 180.0)
-    assert len(seg) == 1, f"expected one chart, got {len(seg)}"
-    return seg.system(0)
 
 
 def build_batch(count=4096, seed=0, threads=16):
@@ -291,74 +283,118 @@ def run_batch(args):
         roofline=roof, meshes=meshes)
 
 
-def run_chart512(args):
+def build_single(kind, side):
+    """One chart: `chart512` = a side x side height field (seed 1); `stretched` = the planar side x side grid
+    with column spacing growing geometrically 1 -> 100 and 5% of the quad diagonals flipped (seed 1)."""
+    from researchuv_amd import lscm, meshgen
+
+    if kind == "stretched":
+        pos, faces = meshgen.stretched_grid(side, stretch=100.0, flip_frac=0.05, seed=1)
+    else:
+        pos, faces = meshgen.heightfield(side, side, seed=1)
+    pw, fw = lscm.weld(pos, faces, 1e-12)
+    seg = lscm.segment(pw, fw, 180.0)
+    assert len(seg) == 1, f"expected one chart, got {len(seg)}"
+    return seg.system(0)
+
+
+def run_single(args):
+    """One large chart through the batched engine (a batch of one) with the device-built AMG, mixed precision;
+    beside it the plan drop-in (FP64, the reference's preconditioners) with an iteration cap."""
     import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
     import torch
 
+    from researchuv_amd import batch as B
     from researchuv_amd import solver
 
+    kind = args.workload
+    side = args.side or (512 if kind == "chart512" else 2048)
     t0 = time.perf_counter()
-    s = build_chart512()
+    s = build_single(kind, side)
     nfree = s["nfree"]
     unknowns = 2 * nfree
-    L = solver.CsrMatrix(s["l_vals"], s["l_cols"], s["l_ptr"])
-    log(f"nfree={nfree} unknowns={unknowns} nnz(L)={len(s['l_vals'])} host setup {time.perf_counter() - t0:.2f}s")
+    log(f"{kind} {side}x{side}: nfree={nfree} unknowns={unknowns} nnz(L)={len(s['l_vals'])} "
+        f"host build {time.perf_counter() - t0:.2f}s")
     dev = torch.device("cuda:0")
     b = torch.tensor(s["rhs"], dtype=torch.float64, device=dev)
     x = torch.zeros_like(b)
-    x64 = torch.zeros_like(b)
     torch.cuda.synchronize()
-    max_iter = 1_000_000
-    pc = getattr(solver.Precond, args.precond)
+    pc = getattr(solver.Precond, args.batch_precond)
     t0 = time.perf_counter()
-    plan = solver.Plan(L, 2, pc, mixed=True)
+    bt = B.Batch.from_csr(s["l_ptr"].astype(np.int64), s["l_cols"], s["l_vals"], [0, nfree], precond=pc, mixed=True)
     setup_s = time.perf_counter() - t0
+
+    def step():
+        bt.setup_precond()
+        return bt.solve_dev(b.data_ptr(), x.data_ptr(), 1_000_000, TOL, per_chart=True)
+
     for _ in range(args.warmup):
-        plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, TOL)
+        step()
     torch.cuda.synchronize()
+    bt.kernel_timing(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        info = plan.solve_dev(b.data_ptr(), x.data_ptr(), max_iter, TOL)
+        info, it, res = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
+    ks = bt.kernel_stats()
+    bt.kernel_timing(False)
     value = unknowns * args.steps / elapsed
-    log(f"mixed: {ms_per_step:.2f} ms/solve, inner iters {info.iters}, refinements {info.refinements}, "
-        f"true rel residual {info.true_rel_resid:.2e}")
-    plan64 = solver.Plan(L, 2, pc, mixed=False)
-    plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, TOL)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fp64_steps):
-        info64 = plan64.solve_dev(b.data_ptr(), x64.data_ptr(), max_iter, TOL)
-    torch.cuda.synchronize()
-    el64 = (time.perf_counter() - t0) / args.fp64_steps
-    log(f"fp64: {1e3 * el64:.2f} ms/solve, iters {info64.iters}, true rel residual {info64.true_rel_resid:.2e}")
-    xa = x.cpu().numpy().reshape(-1, 2)
-    xb = x64.cpu().numpy().reshape(-1, 2)
-    uv_diff = float(np.linalg.norm(xa - xb, axis=1).max() / np.linalg.norm(xb, axis=1).max())
-    import ctypes as C
-
-    from researchuv_amd import _lib
-
-    sec = C.c_double(0)
-    byt = C.c_double(0)
-    _lib.check(_lib.lib().ruv_plan_spmv_time(plan.h, 200, C.byref(sec), C.byref(byt)))
-    achieved = byt.value / sec.value
+    ms_per_step = 1e3 * elapsed / args.steps
+    pinfo = bt.precond_info()
+    log(f"mixed batch-of-one ({args.batch_precond.lower()}): {ms_per_step:.1f} ms/solve, iters {info.iters}, true "
+        f"{info.true_rel_resid:.2e}, {pinfo}")
+    achieved = ks["bytes"] / ks["seconds"]
+    sched = bt.schedule()
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel="k_spmv_pq<float,2>", bytes_per_launch=byt.value, us_per_launch=sec.value * 1e6)
-    log(f"spmv: {sec.value * 1e6:.2f} us/launch, {byt.value / 1e6:.2f} MB algorithmic -> {achieved / 1e9:.0f} GB/s")
+                traffic=None, kernel="kb_dir_spmv<float, 2, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1),
+                launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
+                us_per_launch=1e6 * ks["seconds"] / ks["launches"])
+    bt.close()
+    xa = x.cpu().numpy().reshape(-1, 2)
+    extra = {"setup_s": setup_s, "precond_setup_in_step": pinfo, "iters": int(info.iters),
+             "true_rel_resid": float(info.true_rel_resid), "schedule": sched}
+    # UV / energy error against SuperLU where its factorisation is affordable
+    if nfree <= 400_000:
+        L = sp.csr_matrix((s["l_vals"], s["l_cols"], s["l_ptr"]), shape=(nfree, nfree))
+        r = s["rhs"].reshape(-1, 2)
+        lu = spla.splu(L.tocsc())
+        xr = np.stack([lu.solve(r[:, 0]), lu.solve(r[:, 1])], axis=1)
+        extra["uv_rel_err_vs_cpu_lu"] = float(np.linalg.norm(xa - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max())
+
+        def energy(u):
+            return float(sum(0.5 * u[:, k] @ (L @ u[:, k]) - r[:, k] @ u[:, k] for k in range(2)))
+
+        extra["energy_rel_err_vs_cpu_lu"] = abs(energy(xa) - energy(xr)) / abs(energy(xr))
+        log(f"vs SuperLU: uv err {extra['uv_rel_err_vs_cpu_lu']:.2e}, energy {extra['energy_rel_err_vs_cpu_lu']:.2e}")
+    # the plan drop-in beside it (plain FP64 PCG, the reference's preconditioners, capped)
+    Lp = solver.CsrMatrix(s["l_vals"], s["l_cols"], s["l_ptr"])
+    x64 = torch.zeros_like(b)
+    for name in args.compare:
+        try:
+            t0 = time.perf_counter()
+            plan = solver.Plan(Lp, 2, getattr(solver.Precond, name), mixed=False)
+            st = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            i64 = plan.solve_dev(b.data_ptr(), x64.data_ptr(), args.compare_iters, TOL)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            extra[f"fp64_plan_{name.lower()}"] = {"value": unknowns / (el + st), "solve_s": el, "host_setup_s": st,
+                                                  "iters": int(i64.iters), "true_rel_resid": float(i64.true_rel_resid)}
+            log(f"fp64 plan {name.lower()}: setup {st:.2f}s solve {el:.2f}s iters {i64.iters}")
+            plan.close()
+        except Exception as e:  # not converged within the cap, or a failed setup: reported, not fatal
+            extra[f"fp64_plan_{name.lower()}"] = {"error": str(e)[:200], "iter_cap": args.compare_iters}
+            log(f"fp64 plan {name.lower()}: {e}")
+    what = ("one 512x512-vertex height-field chart" if kind == "chart512" else
+            f"one {side}x{side}-vertex planar grid, column spacing 1 -> 100 geometric, 5% flipped diagonals")
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
-        config={"workload": f"chart512: one 512x512-vertex height-field chart, {unknowns} unknowns",
-                "unknowns": unknowns, "nnz_L": int(len(s["l_vals"])), "precond": args.precond.lower(),
-                "tol_true_rel_resid": TOL},
-        extra={"setup_s": setup_s, "inner_iters": int(info.iters), "refinements": int(info.refinements),
-               "true_rel_resid": float(info.true_rel_resid),
-               "fp64_pcg": {"value": unknowns / el64, "ms_per_solve": 1e3 * el64, "iters": int(info64.iters),
-                            "true_rel_resid": float(info64.true_rel_resid)},
-               "speedup_vs_fp64": value / (unknowns / el64), "uv_rel_diff_mixed_vs_fp64": uv_diff},
-        roofline=roof, meshes=None)
+        config={"workload": f"{kind}: {what}, {unknowns} unknowns", "unknowns": unknowns,
+                "nnz_L": int(len(s["l_vals"])), "precond": args.batch_precond.lower(), "tol_true_rel_resid": TOL},
+        extra=extra, roofline=roof, meshes=None)
 
 
 def main():
@@ -366,10 +402,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "chart512"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched"])
+    ap.add_argument("--side", type=int, default=0, help="chart512 / stretched: vertices per side (512 / 2048)")
+    ap.add_argument("--compare", nargs="*", default=["JACOBI"], help="single chart: plan preconditioners timed beside")
+    ap.add_argument("--compare-iters", type=int, default=20000, help="single chart: iteration cap of the comparisons")
     ap.add_argument("--charts", type=int, default=4096, help="batch workload: number of charts")
     ap.add_argument("--dropin-charts", type=int, default=64, help="batch: charts in the per-chart drop-in sample")
-    ap.add_argument("--precond", default="JACOBI", help="chart512: preconditioner name")
     ap.add_argument("--batch-precond", default="AMG", choices=["JACOBI", "AMG"],
                     help="batch: preconditioner of the mixed batched solve (AMG is built on the device each step)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -388,7 +426,7 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     log("building workload", args.workload)
-    out = run_batch(args) if args.workload == "batch" else run_chart512(args)
+    out = run_batch(args) if args.workload == "batch" else run_single(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle sparse LU)")
