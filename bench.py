@@ -397,12 +397,80 @@ def run_single(args):
         extra=extra, roofline=roof, meshes=None)
 
 
+def run_diffbatch(args):
+    """The differentiable batch: 256 charts of 128 x 128 height-field vertices (positions rounded to FP32),
+    loss = the solver's LSCM energy + |uv - uv*|^2 to seeded targets.  One step = forward solve + adjoint solve
+    and gather (dJ/dpositions) of the whole batch as one block-diagonal system; value counts both solves'
+    unknowns.  The gradient is checked against FP64 central differences (h = 1e-6) of the forward map on a few
+    coordinates."""
+    import numpy as np
+
+    from researchuv_amd import diff, lscm, meshgen
+    from researchuv_amd.solver import Precond
+
+    count, side = args.charts if args.charts != 4096 else 256, args.side or 128
+    t0 = time.perf_counter()
+    charts, p3s = [], []
+    for k in range(count):
+        pos, faces = meshgen.heightfield(side, side, seed=7000 + k)
+        pos = pos.astype(np.float32).astype(np.float64)
+        pw, fw = lscm.weld(pos, faces, 1e-12)
+        seg = lscm.segment(pw, fw, 180.0)
+        ch = seg.chart(0)
+        sy = seg.system(0)
+        charts.append((len(ch.vertex_ids), ch.tris, sy["pin_ids"], sy["pin_uv"]))
+        p3s.append(pw[ch.vertex_ids])
+    P = np.concatenate(p3s)
+    nfree = sum(int(c[0]) - len(c[2]) for c in charts)
+    unknowns = 2 * nfree
+    target = np.random.default_rng(11).normal(size=(len(P), 2)) * 0.1
+    bs = diff.BatchSolve(charts, precond=Precond.JACOBI, mixed=False, tol=1e-12)
+    log(f"diffbatch: {count} charts of {side}x{side}, {len(P)} vertices, {unknowns} unknowns, host build "
+        f"{time.perf_counter() - t0:.1f}s")
+
+    def step():
+        uv, e = bs.forward(P)
+        g = bs.vjp(P, uv, 2.0 * (uv - target), energy_weight=1.0)
+        return uv, e, g
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        uv, e, g = step()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    loss = e + float(((uv - target) ** 2).sum())
+    # directional check of the adjoint against central differences of the forward map (loss J)
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    worst = 0.0
+    for k in rng.choice(P.size, size=3, replace=False):
+        pp = P.copy().reshape(-1)
+        pm = P.copy().reshape(-1)
+        pp[k] += h
+        pm[k] -= h
+        up, ep = bs.forward(pp.reshape(-1, 3))
+        um, em = bs.forward(pm.reshape(-1, 3))
+        jp = ep + float(((up - target) ** 2).sum())
+        jm = em + float(((um - target) ** 2).sum())
+        worst = max(worst, abs((jp - jm) / (2 * h) - g.reshape(-1)[k]) / np.abs(g).max())
+    log(f"diffbatch: {ms_per_step:.1f} ms per forward+adjoint, loss {loss:.6e}, worst fd mismatch {worst:.2e}")
+    return dict(
+        value=2 * unknowns / (elapsed / args.steps), ms_per_step=ms_per_step, dtype="f64",
+        config={"workload": f"diffbatch: {count} height-field charts of {side}x{side} vertices, forward + adjoint, "
+                            f"{unknowns} unknowns per solve", "unknowns": unknowns, "precond": "jacobi",
+                "tol_rel_resid": 1e-12},
+        extra={"loss": loss, "fd_rel_mismatch_3_coords": worst},
+        roofline=None, meshes=None)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched", "diffbatch"])
     ap.add_argument("--side", type=int, default=0, help="chart512 / stretched: vertices per side (512 / 2048)")
     ap.add_argument("--compare", nargs="*", default=["JACOBI"], help="single chart: plan preconditioners timed beside")
     ap.add_argument("--compare-iters", type=int, default=20000, help="single chart: iteration cap of the comparisons")
@@ -426,7 +494,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(2)
     log("building workload", args.workload)
-    out = run_batch(args) if args.workload == "batch" else run_single(args)
+    if args.workload == "batch":
+        out = run_batch(args)
+    elif args.workload == "diffbatch":
+        out = run_diffbatch(args)
+    else:
+        out = run_single(args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle sparse LU)")

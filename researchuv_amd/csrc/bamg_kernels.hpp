@@ -5,10 +5,10 @@
 // matched with their strongest unmatched neighbour, two matchings make one level (amg.rs SA_AGG = 4), the
 // prolongator is the piecewise-constant aggregate map, the coarse operator is the Galerkin product P^T A P,
 // and the cycle smooths with damped Jacobi (omega = 2/3).  What differs: the matching runs on the device by
-// handshakes (a row proposes its strongest unmatched neighbour whose coupling is at least 0.1 of the row's
-// strongest, mutual proposals pair up, four rounds), a row
-// left unmatched joins the pair it is most strongly coupled to, every aggregate stays inside one chart -- and
-// on the finest level inside one workgroup tile -- so the hierarchy of a batch is block-diagonal by chart, and
+// handshakes (a row proposes its preferred unmatched neighbour among those coupled at least 0.1 as strongly as
+// its strongest -- strength in quarter-octave bins, then a pair hash -- and mutual proposals pair up, four
+// rounds), a row left unmatched joins the pair it is most strongly coupled to, every aggregate stays inside
+// one chart, so the hierarchy of a batch is block-diagonal by chart, and
 // the coarsest level is one Jacobi step (exact once a chart has coarsened to a single row).  No inner
 // products: the cycle is a fixed linear SPD operator, so each chart's PCG stays an independent PCG.
 //
@@ -112,8 +112,24 @@ __global__ __launch_bounds__(BLK) void k_smax(SetupCsr A, double* __restrict__ s
     smax[i] = m;
 }
 
-// an unmatched row proposes its strongest unmatched neighbour of the same group (coupling >= theta * its
-// strongest coupling); ties go to the lower index
+// Preference of the coupling (i, j): its strength in bins of a quarter octave, then a hash of the unordered
+// pair.  Couplings that vary smoothly along a line (uniform or geometric spacing) fall into one bin, where the
+// hash -- the same seen from either end -- acts as a random priority; ordering by exact strength or by index
+// would make every row propose to the same side and leave chains instead of pairs.
+__device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
+    const int64_t a = min(i, j), b = max(i, j);
+    uint32_t h = (uint32_t)a * 0x9E3779B1u ^ ((uint32_t)b + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    const int bin = (int)floor(4.0 * log2(s)) + 4096;  // s > 0
+    return ((uint64_t)(uint32_t)max(bin, 0) << 32) | h;
+}
+
+// an unmatched row proposes its preferred unmatched neighbour of the same group among the couplings
+// >= theta * its strongest
 __global__ __launch_bounds__(BLK) void k_propose(SetupCsr A, const double* __restrict__ smax,
                                                  const int32_t* __restrict__ group, const int64_t* __restrict__ mate,
                                                  int64_t* __restrict__ prop, double theta) {
@@ -123,14 +139,15 @@ __global__ __launch_bounds__(BLK) void k_propose(SetupCsr A, const double* __res
     if (mate[i] < 0) {
         const double thr = theta * smax[i];
         const int32_t g = group[i];
-        double bs = 0.0;
+        uint64_t bp = 0;
         for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
             const int64_t j = A.cols[k];
             if (j == i || !(A.vals[k] < 0.0) || mate[j] >= 0 || group[j] != g) continue;
             const double s = -A.vals[k];
             if (s < thr) continue;
-            if (best < 0 || s > bs || (s == bs && j < best)) {
-                bs = s;
+            const uint64_t p = pair_pref(s, i, j);
+            if (best < 0 || p > bp || (p == bp && j < best)) {
+                bp = p;
                 best = j;
             }
         }
@@ -147,7 +164,7 @@ __global__ __launch_bounds__(BLK) void k_accept(int64_t n, const int64_t* __rest
 }
 
 // a row still unmatched joins the pair it is most strongly coupled to (same group, coupling >= theta * its
-// strongest); join = that pair's lower row, or -1 (the row stays alone)
+// strongest, ties to the lower row); join = that pair's lower row, or -1 (the row stays alone)
 __global__ __launch_bounds__(BLK) void k_join(SetupCsr A, const double* __restrict__ smax,
                                               const int32_t* __restrict__ group, const int64_t* __restrict__ mate,
                                               int64_t* __restrict__ join, double theta) {
@@ -226,46 +243,90 @@ __global__ __launch_bounds__(BLK) void k_member_sort(int64_t nc, const int64_t* 
     if (group_c && b > a) group_c[I] = group[mrows[a]];
 }
 
-// Galerkin row I = the rows of aggregate I with columns mapped through agg, duplicates summed.  The distinct
-// coarse columns are emitted in ascending order by repeated selection, each summed over the entries in row
-// order.
+// Galerkin row I = the rows of aggregate I with columns mapped through agg, duplicates summed, columns
+// ascending, each column summed over its entries in row order.  A row of up to GAL_CAP entries is gathered
+// into LDS (slot-major, one column of slots per thread) and stable-sorted there; a longer row falls back to
+// repeated selection of the next column.  Both give the same sums in the same order.
+constexpr int GAL_BLK = 64;
+constexpr int GAL_CAP = 64;
+
 template <bool FILL>
-__global__ __launch_bounds__(BLK) void k_galerkin(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
-                                                  const int64_t* __restrict__ mrows, const int32_t* __restrict__ agg,
-                                                  int64_t* __restrict__ cnt, const int64_t* __restrict__ cptr,
-                                                  int32_t* __restrict__ ccols, double* __restrict__ cvals,
-                                                  double* __restrict__ cdiag) {
-    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+__global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
+                                                      const int64_t* __restrict__ mrows,
+                                                      const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
+                                                      const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
+                                                      double* __restrict__ cvals, double* __restrict__ cdiag) {
+    __shared__ int32_t skey[GAL_CAP * GAL_BLK];
+    __shared__ double sval[GAL_CAP * GAL_BLK];
+    const int64_t I = (int64_t)blockIdx.x * GAL_BLK + threadIdx.x;
     if (I >= nc) return;
+    const int tid = threadIdx.x;
     const int64_t m0 = mptr[I], m1 = mptr[I + 1];
-    int64_t last = -1;
+    int m = 0;
+    for (int64_t q = m0; q < m1; ++q) {
+        const int64_t r = mrows[q];
+        m += (int)(A.ptr[r + 1] - A.ptr[r]);
+    }
     int64_t out = FILL ? cptr[I] : 0;
     int64_t count = 0;
     double diag = 0.0;
-    for (;;) {
-        int64_t key = INT64_MAX;  // smallest coarse column > last
+    if (m <= GAL_CAP) {
+        int k = 0;
         for (int64_t q = m0; q < m1; ++q) {
             const int64_t r = mrows[q];
-            for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
-                const int64_t c = agg[A.cols[k]];
-                if (c > last && c < key) key = c;
+            for (int64_t e = A.ptr[r]; e < A.ptr[r + 1]; ++e, ++k) {
+                // stable insertion: shift strictly greater keys up
+                const int32_t key = agg[A.cols[e]];
+                const double v = A.vals[e];
+                int p = k - 1;
+                while (p >= 0 && skey[p * GAL_BLK + tid] > key) {
+                    skey[(p + 1) * GAL_BLK + tid] = skey[p * GAL_BLK + tid];
+                    sval[(p + 1) * GAL_BLK + tid] = sval[p * GAL_BLK + tid];
+                    --p;
+                }
+                skey[(p + 1) * GAL_BLK + tid] = key;
+                sval[(p + 1) * GAL_BLK + tid] = v;
             }
         }
-        if (key == INT64_MAX) break;
-        if (FILL) {
+        for (int p = 0; p < m;) {
+            const int32_t key = skey[p * GAL_BLK + tid];
             double v = 0.0;
+            while (p < m && skey[p * GAL_BLK + tid] == key) v += sval[(p++) * GAL_BLK + tid];
+            if (FILL) {
+                ccols[out] = key;
+                cvals[out] = v;
+                ++out;
+                if (key == I) diag = v;
+            }
+            ++count;
+        }
+    } else {
+        int64_t last = -1;
+        for (;;) {
+            int64_t key = INT64_MAX;  // smallest coarse column > last
             for (int64_t q = m0; q < m1; ++q) {
                 const int64_t r = mrows[q];
-                for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k)
-                    if (agg[A.cols[k]] == key) v += A.vals[k];
+                for (int64_t e = A.ptr[r]; e < A.ptr[r + 1]; ++e) {
+                    const int64_t c = agg[A.cols[e]];
+                    if (c > last && c < key) key = c;
+                }
             }
-            ccols[out] = (int32_t)key;
-            cvals[out] = v;
-            ++out;
-            if (key == I) diag = v;
+            if (key == INT64_MAX) break;
+            if (FILL) {
+                double v = 0.0;
+                for (int64_t q = m0; q < m1; ++q) {
+                    const int64_t r = mrows[q];
+                    for (int64_t e = A.ptr[r]; e < A.ptr[r + 1]; ++e)
+                        if (agg[A.cols[e]] == key) v += A.vals[e];
+                }
+                ccols[out] = (int32_t)key;
+                cvals[out] = v;
+                ++out;
+                if (key == I) diag = v;
+            }
+            ++count;
+            last = key;
         }
-        ++count;
-        last = key;
     }
     if (!FILL) cnt[I] = count;
     else cdiag[I] = diag;
@@ -460,23 +521,17 @@ __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
 }
 
 // Finest level (the batch's sliced-ELL operator), per live tile.  b is the PCG residual r.
-//   down: t = r - A (w D^-1 r) for the tile's rows into LDS, then every aggregate of the tile (aggregates of
-//         the finest level never leave their tile) sums its members: b1[I] = sum t_m.
+//   down: t = r - A (w D^-1 r) for the tile's rows; kc_restrict then sums t over each aggregate.
 template <typename T, int R, int CI>
-__global__ __launch_bounds__(BLK) void kc_down0(BDev B, const int* __restrict__ act, int na, const T* __restrict__ vals,
-                                                const T* __restrict__ dinv, T w, const T* __restrict__ r,
-                                                const int64_t* __restrict__ tile_c0, const int64_t* __restrict__ mptr,
-                                                const int64_t* __restrict__ mrows, T* __restrict__ b1,
-                                                int max_tile_rows) {
-    extern __shared__ char smem[];
-    T* tl = reinterpret_cast<T*>(smem);  // max_tile_rows * R
+__global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__ act, int na,
+                                                 const T* __restrict__ vals, const T* __restrict__ dinv, T w,
+                                                 const T* __restrict__ r, T* __restrict__ t_out) {
     const int slot = xcd_slot(na);
     if (slot < 0) return;
     const int t = act[slot];
     const int c = B.tile_chart[t];
     if (B.c_done[c] != B_RUNNING) return;
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
-    const int64_t r0 = B.tile_r0[t];
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
         const int n = B.sl_n[s];
@@ -498,24 +553,10 @@ __global__ __launch_bounds__(BLK) void kc_down0(BDev B, const int* __restrict__ 
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) acc[cc] = fma(-v * dj, rj.c[cc], acc[cc]);
         }
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) tl[(i - r0) * R + cc] = acc[cc];
-    }
-    __syncthreads();
-    const int64_t a0 = tile_c0[t], a1 = tile_c0[t + 1];
-    for (int64_t I = a0 + threadIdx.x; I < a1; I += BLK) {
-        T acc[R];
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        for (int64_t q = mptr[I]; q < mptr[I + 1]; ++q) {
-            const int64_t m = mrows[q];
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) acc[cc] += tl[(m - r0) * R + cc];
-        }
         Vr<T, R> o;
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) o.c[cc] = acc[cc];
-        st<T, R>(b1, I, o);
+        st<T, R>(t_out, i, o);
     }
 }
 

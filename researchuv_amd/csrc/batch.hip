@@ -535,6 +535,11 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RUV_ERR_UNAVAILABLE, "no HIP device");
     std::unique_ptr<BatchImpl> Q(new BatchImpl());
+    // relative reduction each FP32 inner solve aims for before the FP64 residual is recomputed
+    if (const char* e = std::getenv("RUV_BATCH_INNER_TOL")) {
+        const double v = std::atof(e);
+        if (v > 0.0 && v < 1.0) Q->inner_tol = v;
+    }
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

@@ -30,8 +30,6 @@ struct BamgLevel {
 
 struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
-    int64_t* tile_c0 = nullptr;  // finest tiles -> their level-1 aggregate range (ntile + 1)
-    int max_tile_rows = 0;
     double omega = 2.0 / 3.0;
     double setup_seconds = 0.0;
     std::vector<void*> mem;
@@ -157,7 +155,8 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     int64_t *cnt = nullptr, *cptr = nullptr;
     HIPCHK(S.alloc(&cnt, nc));
     HIPCHK(S.alloc(&cptr, nc + 1));
-    hipLaunchKernelGGL((k_galerkin<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+    const dim3 gg((unsigned)std::max<int64_t>(1, (nc + GAL_BLK - 1) / GAL_BLK));
+    hipLaunchKernelGGL((k_galerkin<false>), gg, dim3(GAL_BLK), 0, s, nc, A, (const int64_t*)M->mptr,
                        (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
                        (int32_t*)nullptr, (double*)nullptr, (double*)nullptr);
     HIPCHK(hipGetLastError());
@@ -172,7 +171,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
     HIPCHK(S.alloc(&M->cdiag, nc));
-    hipLaunchKernelGGL((k_galerkin<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+    hipLaunchKernelGGL((k_galerkin<true>), gg, dim3(GAL_BLK), 0, s, nc, A, (const int64_t*)M->mptr,
                        (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
                        ccols, cvals, M->cdiag);
     HIPCHK(hipGetLastError());
@@ -188,7 +187,7 @@ __global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, cons
 
 }  // namespace
 
-// Keep the finest operator (FP64 CSR) and the tile groups on the device for (re)builds of the hierarchy.
+// Keep the finest operator (FP64 CSR) and each row's chart on the device for (re)builds of the hierarchy.
 int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals) {
     hipStream_t s = Q.stream;
     const int64_t n0 = Q.nrows;
@@ -196,8 +195,8 @@ int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, cons
     HIPCHK(Q.upload(&Q.amg_cols, cols, (size_t)Q.nnz));
     HIPCHK(Q.upload(&Q.amg_vals, vals, (size_t)Q.nnz));
     std::vector<int32_t> g(n0);
-    for (int t = 0; t < Q.ntile; ++t)
-        for (int64_t i = Q.h_tile_r0[t]; i < Q.h_tile_r1[t]; ++i) g[i] = t;
+    for (int c = 0; c < Q.nchart; ++c)
+        for (int64_t i = Q.crow[c]; i < Q.crow[c + 1]; ++i) g[i] = c;
     HIPCHK(Q.upload(&Q.amg_group, g.data(), g.size()));
     HIPCHK(hipStreamSynchronize(s));
     return RUV_OK;
@@ -233,6 +232,10 @@ int bamg_build(BatchImpl& Q) {
         // cycle data of this level
         if (l == 0) {
             L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
+            char* t = nullptr;
+            HIPCHK(A->alloc(&t, cur.n * R * sv));
+            HIPCHK(hipMemsetAsync(t, 0, cur.n * R * sv, s));  // rows of converged charts keep finite values
+            L.t = t;
         } else {
             // sliced ELL of the level operator
             const int64_t nsl = (cur.n + SLICE - 1) / SLICE;
@@ -268,6 +271,9 @@ int bamg_build(BatchImpl& Q) {
             HIPCHK(A->alloc(&b, cur.n * R * sv));
             HIPCHK(A->alloc(&z, cur.n * R * sv));
             HIPCHK(A->alloc(&t, cur.n * R * sv));
+            HIPCHK(hipMemsetAsync(b, 0, cur.n * R * sv, s));
+            HIPCHK(hipMemsetAsync(z, 0, cur.n * R * sv, s));
+            HIPCHK(hipMemsetAsync(t, 0, cur.n * R * sv, s));
             L.b = b;
             L.z = z;
             L.t = t;
@@ -294,10 +300,9 @@ int bamg_build(BatchImpl& Q) {
         HIPCHK(hipGetLastError());
         rc = build_members(s, cur.n, L.agg, M2.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows, (int32_t*)nullptr);
         if (rc) return rc;
-        // chart offsets on the next level (and, on the finest, each tile's aggregate range)
+        // chart offsets on the next level
         {
             std::vector<int64_t> at(crow.begin(), crow.end());
-            if (l == 0) at.insert(at.end(), Q.h_tile_r0.begin(), Q.h_tile_r0.end());
             int64_t *d_at = nullptr, *d_out = nullptr;
             HIPCHK(S1->alloc(&d_at, at.size()));
             HIPCHK(S1->alloc(&d_out, at.size()));
@@ -307,18 +312,7 @@ int bamg_build(BatchImpl& Q) {
             std::vector<int64_t> out(at.size());
             HIPCHK(hipMemcpyAsync(out.data(), d_out, sizeof(int64_t) * at.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            std::vector<int64_t> next(out.begin(), out.begin() + crow.size());
-            if (l == 0) {
-                std::vector<int64_t> tc(out.begin() + crow.size(), out.end());
-                tc.push_back(M2.nc);
-                HIPCHK(A->alloc(&A->tile_c0, tc.size()));
-                HIPCHK(hipMemcpyAsync(A->tile_c0, tc.data(), sizeof(int64_t) * tc.size(), hipMemcpyHostToDevice, s));
-                int mx = 0;
-                for (int t = 0; t < Q.ntile; ++t) mx = std::max<int>(mx, (int)(Q.h_tile_r1[t] - Q.h_tile_r0[t]));
-                A->max_tile_rows = std::max(mx, 1);
-                HIPCHK(hipStreamSynchronize(s));
-            }
-            crow.swap(next);
+            crow.swap(out);
         }
         A->lv.push_back(L);
         // the next level's operator (kept), its diagonal, and chart groups
@@ -377,16 +371,15 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     BamgLevel& L0 = A.lv[0];
     BamgLevel& L1 = A.lv[1];
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
-    const size_t lds = (size_t)A.max_tile_rows * R * sizeof(T);
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
-    hipLaunchKernelGGL((kc_down0<T, R, CI>), gt, dim3(BLK), lds, s, Q.B, (const int*)Q.act, na, vals,
-                       (const T*)L0.dinv, w, r, (const int64_t*)A.tile_c0, (const int64_t*)L0.mptr,
-                       (const int64_t*)L0.mrows, (T*)L1.b, A.max_tile_rows);
+    hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals,
+                       (const T*)L0.dinv, w, r, (T*)L0.t);
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
                            (T*)L.b, (T*)L.z, (T*)L.t};
     };
+    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
         hipLaunchKernelGGL((kc_resid<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,

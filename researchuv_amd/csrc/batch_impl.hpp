@@ -59,7 +59,7 @@ struct BatchImpl {
     unsigned* h_iters = nullptr;
     double* h_true = nullptr;
     std::vector<uint64_t> iters_total;
-    double inner_tol = 1e-4;
+    double inner_tol = 1e-5;
     int max_rounds = 40;
     // step-kernel accounting (optional): an event pair around every launch, summed after each solve
     bool ktime = false;

@@ -64,3 +64,32 @@ class ChartSolve:
                                        _p(self.pin_uv), len(self.pin_ids), _p(uv), _p(g), energy_weight,
                                        self.precond, self.mode, self.tol, _p(grad), C.byref(info)))
         return grad
+
+
+class BatchSolve:
+    """Differentiable solve of many charts at once.  The disjoint union of the charts is one chart for the
+    device assembly, the solve and the adjoint gather (its L is block-diagonal by chart), so one forward and
+    one adjoint solve serve the whole batch.  charts: iterable of (nv, tris, pin_ids, pin_uv), chart-local;
+    positions and gradients are the charts' (nv, 3) blocks stacked in order."""
+
+    def __init__(self, charts, precond=Precond.JACOBI, mixed=False, tol=1e-12):
+        tris, pins, puv, off = [], [], [], [0]
+        for nv, t, pid, pu in charts:
+            base = off[-1]
+            tris.append(np.asarray(t, np.int64).reshape(-1, 3) + base)
+            pins.append(np.asarray(pid, np.int64) + base)
+            puv.append(np.asarray(pu, np.float64).reshape(-1, 2))
+            off.append(base + int(nv))
+        self.offsets = np.asarray(off, np.int64)
+        self.cs = ChartSolve(np.concatenate(tris).astype(np.uint32), np.concatenate(pins).astype(np.int32),
+                             np.concatenate(puv), precond=precond, mixed=mixed, tol=tol)
+
+    def forward(self, p3):
+        return self.cs.forward(p3)
+
+    def vjp(self, p3, uv, g_uv, energy_weight=0.0):
+        return self.cs.vjp(p3, uv, g_uv, energy_weight)
+
+    def split(self, a):
+        """Per-chart blocks of a stacked (nv_total, k) array."""
+        return [a[self.offsets[c]:self.offsets[c + 1]] for c in range(len(self.offsets) - 1)]

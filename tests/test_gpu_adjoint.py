@@ -105,3 +105,56 @@ def test_vjp_of_the_solve_map_matches_directional_derivative(orc):
     rhs = float((solver.vjp(p3, uv, g, energy_weight=0.0) * v).sum())
     say(f"  <g, Jv> = {lhs:.8e}  <J^T g, v> = {rhs:.8e}")
     assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), 1e-12)
+
+
+def test_batch_union_matches_per_chart_solves_and_gradients(orc):
+    """The differentiable batch (disjoint union of charts, one forward and one adjoint solve): uv and dJ/dp
+    equal the per-chart solves, and the gradient matches FP64 central differences of the summed CPU-LU loss."""
+    import torch
+
+    from researchuv_amd import diff
+
+    say("differentiable batch of 5 charts")
+    charts, p3s = [], []
+    for k in range(5):
+        p3, tris, pin_ids, pin_uv = _chart(orc, n=8 + k, seed=20 + k)
+        charts.append((len(p3), tris, pin_ids, pin_uv))
+        p3s.append(p3)
+    P = np.concatenate(p3s)
+    rng = np.random.default_rng(3)
+    target = rng.normal(size=(len(P), 2)) * 0.1
+    bs = diff.BatchSolve(charts, tol=1e-13)
+    Fn = _autograd_fn()
+    pt = torch.tensor(P, dtype=torch.float64, requires_grad=True)
+    loss = Fn.apply(pt, torch.tensor(target), bs)
+    loss.backward()
+    g = pt.grad.numpy()
+    uv, _ = bs.forward(P)
+    for c, (nv, tris, pin_ids, pin_uv) in enumerate(charts):
+        cs = diff.ChartSolve(tris, pin_ids, pin_uv, tol=1e-13)
+        pc = bs.split(P)[c]
+        uc, _ = cs.forward(pc)
+        assert np.abs(uc - bs.split(uv)[c]).max() <= 1e-9 * np.abs(uc).max()
+        tc = bs.split(target)[c]
+        gc = cs.vjp(pc, uc, 2.0 * (uc - tc), energy_weight=1.0)
+        assert np.abs(gc - bs.split(g)[c]).max() <= 1e-7 * np.abs(gc).max()
+
+    def cpu_loss(Pm):
+        tot = 0.0
+        for c, (nv, tris, pin_ids, pin_uv) in enumerate(charts):
+            r = orc.lscm_loss_frozen(bs.split(Pm)[c], tris, pin_ids, pin_uv, bs.split(target)[c])
+            tot += r[0]
+        return tot
+
+    assert abs(cpu_loss(P) - loss.item()) <= 1e-9 * abs(loss.item())
+    h = 1e-6
+    worst = 0.0
+    for k in rng.choice(P.size, size=16, replace=False):
+        pp = P.copy().reshape(-1)
+        pm = P.copy().reshape(-1)
+        pp[k] += h
+        pm[k] -= h
+        fd = (cpu_loss(pp.reshape(-1, 3)) - cpu_loss(pm.reshape(-1, 3))) / (2 * h)
+        worst = max(worst, abs(fd - g.reshape(-1)[k]) / np.abs(g).max())
+    say(f"  worst |fd - adjoint| / max|grad| = {worst:.2e}")
+    assert worst < 1e-5
