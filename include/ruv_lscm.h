@@ -254,6 +254,27 @@ int ruv_lscm_system_device(const double* mesh_pos, const ruv_charts* charts, int
 int ruv_unfold_chart(const double* mesh_pos, const ruv_charts* charts, int64_t i, const ruv_unfold_opts* opts,
                      double* uv_out, double* stats, ruv_solve_info* info);
 
+/* ---------------------------------------------------------------------------------------------------------
+ * The differentiable batch (no reference counterpart; the forward is lscm.rs:594-666 + the GPU solve, with pins
+ * frozen as in ruv_lscm_forward).  nchart charts as one disjoint union: chart c owns vertices
+ * [chart_vertex_off[c], chart_vertex_off[c+1]) of p3 (xyz triples); tris index the union and never span two
+ * charts; pin_ids index the union.  The topology, the operator's structure (every edge with a free end, zero
+ * weights held as explicit zeros) and the batched solver (precond none / jacobi / amg, mode as ruv_batch_create,
+ * tol its tolerance) are built once.  forward: positions -> uv of every vertex (free: solved; pins: targets)
+ * and the solver's energy sum_e c_e |uv_i - uv_j|^2.  vjp: with the state of the last forward, the gradient
+ * w.r.t. the positions of energy_weight * E + <g_uv, uv> (one adjoint solve with the same SPD operator and a
+ * gather through the assembly).
+ * ------------------------------------------------------------------------------------------------------- */
+typedef struct ruv_lscm_diff ruv_lscm_diff;
+int ruv_lscm_diff_create(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                         const double* pin_uv, int64_t npins, const int64_t* chart_vertex_off, int32_t nchart,
+                         int32_t precond, int32_t mode, double tol, ruv_lscm_diff** out);
+int ruv_lscm_diff_forward(ruv_lscm_diff* h, const double* p3, double* uv_out, double* energy_out,
+                          ruv_solve_info* info);
+int ruv_lscm_diff_vjp(ruv_lscm_diff* h, const double* g_uv, double energy_weight, double* grad_p3,
+                      ruv_solve_info* info);
+void ruv_lscm_diff_destroy(ruv_lscm_diff* h);
+
 #ifdef __cplusplus
 }
 #endif

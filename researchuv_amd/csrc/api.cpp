@@ -488,3 +488,60 @@ extern "C" int ruv_lscm_vjp(const double* p3, int64_t nv, const uint32_t* tris, 
     return ruv::lscm_vjp(p3, nv, tris, ntri, pin_ids, pin_uv, npins, uv, g_uv, energy_weight, precond, mode, tol,
                          grad_p3, info);
 }
+
+// ------------------------------------------------------------------------------------------------------
+// the differentiable batch on the device
+// ------------------------------------------------------------------------------------------------------
+namespace ruv {
+struct DiffBatchImpl;
+int diff_create(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                const double* pin_uv, int64_t npins, const int64_t* voff, int32_t nchart, int precond, int mode,
+                double tol, DiffBatchImpl** out);
+void diff_destroy(DiffBatchImpl* d);
+int diff_forward(DiffBatchImpl* D, const double* p3, double* uv_out, double* energy_out, ruv_solve_info* info);
+int diff_vjp(DiffBatchImpl* D, const double* g_uv, double a_energy, double* grad_out, ruv_solve_info* info);
+}  // namespace ruv
+
+struct ruv_lscm_diff {
+    ruv::DiffBatchImpl* impl;
+};
+
+extern "C" {
+
+int ruv_lscm_diff_create(const double* p3, int64_t nv, const uint32_t* tris, int64_t ntri, const int32_t* pin_ids,
+                         const double* pin_uv, int64_t npins, const int64_t* chart_vertex_off, int32_t nchart,
+                         int32_t precond, int32_t mode, double tol, ruv_lscm_diff** out) {
+    if (!out) return ruv::fail(RUV_ERR_INVALID, "null handle pointer");
+    *out = nullptr;
+    if (!p3 || !tris || (npins > 0 && (!pin_ids || !pin_uv)) || !chart_vertex_off)
+        return ruv::fail(RUV_ERR_INVALID, "null argument");
+    if (nv <= 0 || ntri <= 0 || npins < 0) return ruv::fail(RUV_ERR_BAD_DIMS, "empty mesh");
+    if (precond != RUV_PC_NONE && precond != RUV_PC_JACOBI && precond != RUV_PC_AMG)
+        return ruv::fail(RUV_ERR_INVALID, "the differentiable batch supports none, jacobi and amg");
+    ruv::DiffBatchImpl* d = nullptr;
+    int rc = ruv::diff_create(p3, nv, tris, ntri, pin_ids, pin_uv, npins, chart_vertex_off, nchart, precond, mode,
+                              tol, &d);
+    if (rc) return rc;
+    *out = new ruv_lscm_diff{d};
+    return RUV_OK;
+}
+
+int ruv_lscm_diff_forward(ruv_lscm_diff* h, const double* p3, double* uv_out, double* energy_out,
+                          ruv_solve_info* info) {
+    if (!h || !p3 || !uv_out) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::diff_forward(h->impl, p3, uv_out, energy_out, info);
+}
+
+int ruv_lscm_diff_vjp(ruv_lscm_diff* h, const double* g_uv, double energy_weight, double* grad_p3,
+                      ruv_solve_info* info) {
+    if (!h || !g_uv || !grad_p3) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::diff_vjp(h->impl, g_uv, energy_weight, grad_p3, info);
+}
+
+void ruv_lscm_diff_destroy(ruv_lscm_diff* h) {
+    if (!h) return;
+    ruv::diff_destroy(h->impl);
+    delete h;
+}
+
+}  // extern "C"

@@ -241,6 +241,10 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
         }
     });
     if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "zero diagonal at row " + std::to_string(bad[0]));
+    Q.h_sl_off = sl_off;
+    Q.h_sl_w = sl_w;
+    Q.h_dtab = dtab;
+    Q.h_stencil = stencil;
     int64_t *d_row0 = nullptr, *d_off = nullptr, *d_coff = nullptr, *d_crow = nullptr;
     int32_t *d_w = nullptr, *d_n = nullptr, *d_dtab = nullptr;
     void* d_col = nullptr;
@@ -620,6 +624,68 @@ int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int
     int rc = bamg_info(*Q, &lv, setup_seconds, coarse_rows);
     if (levels) *levels = lv;
     return rc;
+}
+
+// Position of every CSR entry of the batch's operator in its value arrays (the storage form of batch_setup).
+int batch_value_map(const BatchImpl& Q, const int64_t* ptr, const int32_t* cols, std::vector<int64_t>* map) {
+    map->assign((size_t)Q.nnz, -1);
+    for (int s = 0; s < Q.nslice; ++s) {
+        const int64_t r0 = Q.h_sl_row0[s];
+        const int n = Q.h_sl_n[s];
+        const int32_t* dt = Q.h_dtab.data() + (size_t)s * SLOTS;
+        for (int l = 0; l < n; ++l) {
+            const int64_t i = r0 + l;
+            int k = 0;
+            for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a, ++k) {
+                if (Q.h_stencil[s]) {
+                    const int slot = (int)(std::lower_bound(dt, dt + Q.h_sl_w[s], (int32_t)(cols[a] - i)) - dt);
+                    (*map)[a] = Q.h_sl_off[s] + (int64_t)slot * n + l;
+                } else {
+                    (*map)[a] = Q.h_sl_off[s] + (int64_t)k * n + l;
+                }
+            }
+        }
+    }
+    return RUV_OK;
+}
+
+namespace {
+// new values (CSR order) into the value arrays (entries of one slot sum, as at setup: each slot is written by
+// one CSR entry except with duplicate columns, which the assembly never produces) and the inverse diagonal
+__global__ __launch_bounds__(BLK) void kb_scatter_values(int64_t nnz, const double* __restrict__ v,
+                                                         const int64_t* __restrict__ map, double* __restrict__ e64,
+                                                         float* __restrict__ e32) {
+    const int64_t a = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (a >= nnz) return;
+    const int64_t e = map[a];
+    e64[e] = v[a];
+    if (e32) e32[e] = (float)v[a];
+}
+__global__ __launch_bounds__(BLK) void kb_inv_diag(int64_t n, const double* __restrict__ v,
+                                                   const int64_t* __restrict__ diagpos, double* __restrict__ d64,
+                                                   float* __restrict__ d32) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const double d = 1.0 / v[diagpos[i]];
+    d64[i] = d;
+    if (d32) d32[i] = (float)d;
+}
+}  // namespace
+
+// Replace the operator's values (same structure, CSR order, device) and rebuild what depends on them.
+int batch_update_values(BatchImpl* Q, const double* d_vals, const int64_t* d_map, const int64_t* d_diagpos) {
+    hipStream_t s = Q->stream;
+    const dim3 ge((unsigned)std::max<int64_t>(1, (Q->nnz + BLK - 1) / BLK));
+    const dim3 gr((unsigned)std::max<int64_t>(1, (Q->nrows + BLK - 1) / BLK));
+    hipLaunchKernelGGL(kb_scatter_values, ge, dim3(BLK), 0, s, Q->nnz, d_vals, d_map, Q->ev64, Q->ev32);
+    if (Q->dinv64)
+        hipLaunchKernelGGL(kb_inv_diag, gr, dim3(BLK), 0, s, Q->nrows, d_vals, d_diagpos, Q->dinv64, Q->dinv32);
+    HIPCHK(hipGetLastError());
+    if (Q->precond == RUV_PC_AMG) {
+        HIPCHK(hipMemcpyAsync(Q->amg_vals, d_vals, sizeof(double) * Q->nnz, hipMemcpyDeviceToDevice, s));
+        return bamg_build(*Q);
+    }
+    return RUV_OK;
 }
 
 int batch_kernel_timing(BatchImpl* Q, int enable) {

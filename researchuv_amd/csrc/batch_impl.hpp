@@ -40,6 +40,9 @@ struct BatchImpl {
     std::vector<int64_t> crow;      // first row of each chart, nchart + 1
     std::vector<int> chart_t0;
     std::vector<int64_t> h_tile_r0, h_tile_r1;  // rows of each tile (host copy)
+    std::vector<int64_t> h_sl_off;               // host copies of the storage form (value updates)
+    std::vector<int32_t> h_sl_w, h_dtab;
+    std::vector<char> h_stencil;
     // tiles (rebuilt by the autotuner)
     int32_t *d_tile_s0 = nullptr, *d_tile_chart = nullptr, *d_chart_t0 = nullptr;
     int64_t *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;

@@ -93,3 +93,73 @@ class BatchSolve:
     def split(self, a):
         """Per-chart blocks of a stacked (nv_total, k) array."""
         return [a[self.offsets[c]:self.offsets[c + 1]] for c in range(len(self.offsets) - 1)]
+
+
+class DiffBatch:
+    """The differentiable batch kept on the device (include/ruv_lscm.h: ruv_lscm_diff_*): the charts' disjoint
+    union is assembled and solved by the batched engine (each chart its own PCG), topology and structure built
+    once; forward(p3) -> (uv, energy); vjp(g_uv, energy_weight) on the state of the last forward."""
+
+    def __init__(self, p3, charts, precond=Precond.AMG, mixed=True, tol=1e-10):
+        L = _lib.lib()
+        if not getattr(L, "_ruv_diffb_ready", False):
+            P = C.c_void_p
+            L.ruv_lscm_diff_create.restype = C.c_int
+            L.ruv_lscm_diff_create.argtypes = [P, C.c_int64, P, C.c_int64, P, P, C.c_int64, P, C.c_int32, C.c_int32,
+                                               C.c_int32, C.c_double, P]
+            L.ruv_lscm_diff_forward.restype = C.c_int
+            L.ruv_lscm_diff_forward.argtypes = [P, P, P, P, P]
+            L.ruv_lscm_diff_vjp.restype = C.c_int
+            L.ruv_lscm_diff_vjp.argtypes = [P, P, C.c_double, P, P]
+            L.ruv_lscm_diff_destroy.restype = None
+            L.ruv_lscm_diff_destroy.argtypes = [P]
+            L._ruv_diffb_ready = True
+        self.L = L
+        tris, pins, puv, off = [], [], [], [0]
+        for nv, t, pid, pu in charts:
+            base = off[-1]
+            tris.append(np.asarray(t, np.int64).reshape(-1, 3) + base)
+            pins.append(np.asarray(pid, np.int64) + base)
+            puv.append(np.asarray(pu, np.float64).reshape(-1, 2))
+            off.append(base + int(nv))
+        self.offsets = np.asarray(off, np.int64)
+        self.nv = int(off[-1])
+        T = np.ascontiguousarray(np.concatenate(tris), np.uint32)
+        pid = np.ascontiguousarray(np.concatenate(pins), np.int32)
+        pu = np.ascontiguousarray(np.concatenate(puv), np.float64)
+        p3 = np.ascontiguousarray(p3, np.float64).reshape(-1, 3)
+        self.h = C.c_void_p()
+        _lib.check(L.ruv_lscm_diff_create(_p(p3), self.nv, _p(T), len(T), _p(pid), _p(pu), len(pid), _p(self.offsets),
+                                          len(self.offsets) - 1, int(precond), 1 if mixed else 0, tol,
+                                          C.byref(self.h)))
+
+    def forward(self, p3):
+        p3 = np.ascontiguousarray(p3, np.float64).reshape(-1, 3)
+        uv = np.zeros((self.nv, 2))
+        e = C.c_double(0)
+        info = _lib.SolveInfo()
+        _lib.check(self.L.ruv_lscm_diff_forward(self.h, _p(p3), _p(uv), C.byref(e), C.byref(info)))
+        self.last_info = info
+        return uv, e.value
+
+    def vjp(self, p3, uv, g_uv, energy_weight=0.0):
+        """p3 and uv must be those of the last forward (kept on the device)."""
+        g = np.ascontiguousarray(g_uv, np.float64).reshape(-1, 2)
+        grad = np.zeros((self.nv, 3))
+        info = _lib.SolveInfo()
+        _lib.check(self.L.ruv_lscm_diff_vjp(self.h, _p(g), energy_weight, _p(grad), C.byref(info)))
+        return grad
+
+    def split(self, a):
+        return [a[self.offsets[c]:self.offsets[c + 1]] for c in range(len(self.offsets) - 1)]
+
+    def close(self):
+        if self.h:
+            self.L.ruv_lscm_diff_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -158,3 +158,41 @@ def test_batch_union_matches_per_chart_solves_and_gradients(orc):
         worst = max(worst, abs(fd - g.reshape(-1)[k]) / np.abs(g).max())
     say(f"  worst |fd - adjoint| / max|grad| = {worst:.2e}")
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("pc,mixed", [("JACOBI", False), ("AMG", True)])
+def test_device_diff_batch_matches_union_solve(orc, pc, mixed):
+    """The device-resident differentiable batch (structure built once, batched solves) gives the uv, energy and
+    gradient of the union solve, through the same torch.autograd.Function, and a second forward at moved
+    positions (same structure) matches a fresh solve there."""
+    import torch
+
+    from researchuv_amd import diff
+    from researchuv_amd.solver import Precond
+
+    say(f"device differentiable batch precond={pc} mixed={mixed}")
+    charts, p3s = [], []
+    for k in range(4):
+        p3, tris, pin_ids, pin_uv = _chart(orc, n=9 + k, seed=40 + k)
+        charts.append((len(p3), tris, pin_ids, pin_uv))
+        p3s.append(p3)
+    P = np.concatenate(p3s)
+    rng = np.random.default_rng(8)
+    target = rng.normal(size=(len(P), 2)) * 0.1
+    ref = diff.BatchSolve(charts, tol=1e-13)
+    db = diff.DiffBatch(P, charts, precond=getattr(Precond, pc), mixed=mixed, tol=1e-12)
+    Fn = _autograd_fn()
+    for step in range(2):
+        Pk = P + step * 1e-3 * rng.normal(size=P.shape)
+        pt = torch.tensor(Pk, dtype=torch.float64, requires_grad=True)
+        loss = Fn.apply(pt, torch.tensor(target), db)
+        loss.backward()
+        u_ref, e_ref = ref.forward(Pk)
+        g_ref = ref.vjp(Pk, u_ref, 2.0 * (u_ref - target), energy_weight=1.0)
+        u_dev, e_dev = db.forward(Pk)
+        say(f"  step {step}: uv diff {np.abs(u_dev - u_ref).max():.2e}, grad diff "
+            f"{np.abs(pt.grad.numpy() - g_ref).max() / np.abs(g_ref).max():.2e}, iters {db.last_info.iters}")
+        assert np.abs(u_dev - u_ref).max() <= 1e-8 * np.abs(u_ref).max()
+        assert abs(e_dev - e_ref) <= 1e-9 * abs(e_ref)
+        assert np.abs(pt.grad.numpy() - g_ref).max() <= 1e-6 * np.abs(g_ref).max()
+    db.close()
