@@ -146,9 +146,9 @@ int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_laun
 /* The batch's storage and the schedule its autotuner chose: rows per workgroup tile; bytes per stored column
  * index of the general slices (2: 16-bit offsets from the row, 4: 32-bit rows); stored values (padding
  * included) and stored column indices; slices in stencil form (column offsets per slice, none per entry) and
- * all slices.  Any pointer may be null. */
+ * all slices; 1 when the step kernel loads the values with the non-temporal hint.  Any pointer may be null. */
 int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
-                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
+                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal);
 /* Step-kernel accounting over the solves that follow (measurement hook): with enable != 0 every launch of the
  * step kernel is bracketed by HIP events on the batch's stream.  ruv_batch_kernel_stats returns, and resets,
  * the launch count, the summed device time of those launches and their algorithmic bytes (per chart: its

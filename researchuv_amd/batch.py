@@ -41,7 +41,7 @@ def _setup(L):
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
     L.ruv_batch_kernel_stats.restype = C.c_int
     L.ruv_batch_kernel_stats.argtypes = [P, P, P, P]
-    L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P]
+    L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
     L.ruv_lscm_batch_build.restype = C.c_int
@@ -130,10 +130,11 @@ class Batch:
         nc = C.c_int64(0)
         st = C.c_int32(0)
         ns = C.c_int32(0)
+        nt = C.c_int32(0)
         _lib.check(lib().ruv_batch_schedule(self.h, C.byref(rows), C.byref(cb), C.byref(ne), C.byref(nc),
-                                            C.byref(st), C.byref(ns)))
+                                            C.byref(st), C.byref(ns), C.byref(nt)))
         return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value, col_entries=nc.value,
-                    stencil_slices=st.value, slices=ns.value)
+                    stencil_slices=st.value, slices=ns.value, nontemporal=nt.value)
 
     def setup_precond(self):
         """AMG: rebuild the hierarchy on the device (no-op for the element-wise preconditioners)."""

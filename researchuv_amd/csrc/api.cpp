@@ -22,7 +22,7 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid);
 int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
 int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
-                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices);
+                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal);
 int batch_kernel_timing(BatchImpl* Q, int enable);
 int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
 int batch_setup_precond(BatchImpl* Q);
@@ -337,10 +337,10 @@ int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_laun
 }
 
 int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
-                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices) {
+                       int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal) {
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
-                               slices);
+                               slices, nontemporal);
 }
 
 int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap) {

@@ -331,8 +331,12 @@ inline dim3 grid_of(int na) { return dim3((unsigned)(((na + XCDS - 1) / XCDS) * 
 template <typename T, int R, int CI, int PROBE = 0>
 void launch_dir_spmv(BatchImpl& Q, int na, int par, uint32_t iter, const T* z, const T* pold, T* pnew, T* q) {
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
-    hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B, (const int*)Q.act,
-                       na, par, iter, vals, z, pold, pnew, q);
+    if (Q.nt)
+        hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, 1, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
+                           (const int*)Q.act, na, par, iter, vals, z, pold, pnew, q);
+    else
+        hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, 0, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
+                           (const int*)Q.act, na, par, iter, vals, z, pold, pnew, q);
 }
 
 template <typename T, int R, int PC, int CI>
@@ -505,26 +509,34 @@ int time_step(BatchImpl& Q, int reps, double* sec) {
     return Q.ci == 0 ? time_step_kernel<double, 1, 0>(Q, reps, sec) : time_step_kernel<double, 1, 1>(Q, reps, sec);
 }
 
-// Pick slices-per-tile by timing the step kernel over the whole batch.
+// Pick slices-per-tile and the value-load hint by timing the step kernel over the whole batch.
 int autotune(BatchImpl& Q) {
     const char* env = std::getenv("RUV_BATCH_SLICES_PER_TILE");
     if (env) {
         const int v = std::atoi(env);
-        if (v == 1 || v == 2 || v == 4 || v == 8) return build_tiles(Q, v);
+        if (v == 1 || v == 2 || v == 4 || v == 8) {
+            Q.nt = 0;
+            return build_tiles(Q, v);
+        }
     }
-    int best = 2;
+    int best = 2, best_nt = 0;
     double best_t = 1e300;
     for (int spt : {1, 2, 4, 8}) {
         int rc = build_tiles(Q, spt);
         if (rc) return rc;
-        double t = 0.0;
-        rc = time_step(Q, 8, &t);
-        if (rc) return rc;
-        if (t < best_t) {
-            best_t = t;
-            best = spt;
+        for (int nt : {0, 1}) {
+            Q.nt = nt;
+            double t = 0.0;
+            rc = time_step(Q, 8, &t);
+            if (rc) return rc;
+            if (t < best_t) {
+                best_t = t;
+                best = spt;
+                best_nt = nt;
+            }
         }
     }
+    Q.nt = best_nt;
     return Q.spt == best ? RUV_OK : build_tiles(Q, best);
 }
 
@@ -708,8 +720,9 @@ int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double*
 }
 
 int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
-                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices) {
+                   int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal) {
     if (rows_per_tile) *rows_per_tile = Q->spt * SLICE;
+    if (nontemporal) *nontemporal = Q->nt;
     if (col_bytes) *col_bytes = Q->ci == 0 ? 2 : 4;
     if (ell_entries) *ell_entries = Q->nell;
     if (col_entries) *col_entries = Q->ncol;

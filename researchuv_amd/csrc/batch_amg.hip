@@ -32,14 +32,16 @@ struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
     double omega = 2.0 / 3.0;
     double setup_seconds = 0.0;
+    DevPool* pool = nullptr;
     std::vector<void*> mem;
+    explicit Bamg(DevPool* p) : pool(p) {}
     ~Bamg() {
-        for (void* p : mem) (void)hipFree(p);
+        for (void* p : mem) pool->put(p);
     }
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
         void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        hipError_t e = pool->get(&q, std::max<size_t>(n, 1) * sizeof(T));
         if (e == hipSuccess) mem.push_back(q);
         *p = (T*)q;
         return e;
@@ -52,16 +54,18 @@ namespace {
 
 inline dim3 rows_grid(int64_t n) { return dim3((unsigned)std::max<int64_t>(1, (n + BLK - 1) / BLK)); }
 
-// temporary device buffers of one setup step, freed together
+// temporary device buffers of one setup step, returned to the pool together
 struct Scratch {
+    DevPool* pool;
     std::vector<void*> mem;
+    explicit Scratch(DevPool* p) : pool(p) {}
     ~Scratch() {
-        for (void* p : mem) (void)hipFree(p);
+        for (void* p : mem) pool->put(p);
     }
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
         void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        hipError_t e = pool->get(&q, std::max<size_t>(n, 1) * sizeof(T));
         if (e == hipSuccess) mem.push_back(q);
         *p = (T*)q;
         return e;
@@ -87,7 +91,7 @@ template <typename Alloc>
 int build_members(hipStream_t s, int64_t n, const int32_t* agg, int64_t nc, const int32_t* group, Alloc& S,
                   int64_t** mptr, int64_t** mrows, int32_t* group_c) {
     unsigned long long *cnt = nullptr, *cursor = nullptr;
-    Scratch T;
+    Scratch T(S.pool);
     HIPCHK(T.alloc(&cnt, nc));
     HIPCHK(T.alloc(&cursor, nc));
     HIPCHK(S.alloc(mptr, nc + 1));
@@ -210,7 +214,7 @@ int bamg_build(BatchImpl& Q) {
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s));
-    std::unique_ptr<Bamg> A(new Bamg());
+    std::unique_ptr<Bamg> A(new Bamg(&Q.pool));
     const bool f32 = Q.mode == RUV_MODE_MIXED;
     const size_t sv = f32 ? sizeof(float) : sizeof(double);
     const int R = Q.R;
@@ -224,7 +228,7 @@ int bamg_build(BatchImpl& Q) {
     SetupCsr cur{n0, Q.amg_ptr, Q.amg_cols, Q.amg_vals};
     const double* cur_diag = nullptr;
     std::vector<int64_t> crow = Q.crow;
-    std::unique_ptr<Scratch> keep(new Scratch());  // owns the current level's setup CSR (levels >= 1)
+    std::unique_ptr<Scratch> keep(new Scratch(&Q.pool));  // owns the current level's setup CSR (levels >= 1)
     const int max_levels = 24;
     for (int l = 0;; ++l) {
         BamgLevel L;
@@ -240,7 +244,7 @@ int bamg_build(BatchImpl& Q) {
             // sliced ELL of the level operator
             const int64_t nsl = (cur.n + SLICE - 1) / SLICE;
             int64_t* sz = nullptr;
-            Scratch T;
+            Scratch T(&Q.pool);
             HIPCHK(T.alloc(&sz, nsl));
             HIPCHK(A->alloc(&L.ew, nsl));
             HIPCHK(A->alloc(&L.eoff, nsl + 1));
@@ -284,7 +288,7 @@ int bamg_build(BatchImpl& Q) {
             break;
         }
         // two matchings make the next level
-        std::unique_ptr<Scratch> S1(new Scratch()), S2(new Scratch());
+        std::unique_ptr<Scratch> S1(new Scratch(&Q.pool)), S2(new Scratch(&Q.pool));
         Matched M1, M2;
         int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
         if (rc) return rc;
@@ -316,7 +320,7 @@ int bamg_build(BatchImpl& Q) {
         }
         A->lv.push_back(L);
         // the next level's operator (kept), its diagonal, and chart groups
-        std::unique_ptr<Scratch> nk(new Scratch());
+        std::unique_ptr<Scratch> nk(new Scratch(&Q.pool));
         int64_t* nptr = nullptr;
         int32_t* ncols = nullptr;
         double *nvals = nullptr, *ndiag = nullptr;

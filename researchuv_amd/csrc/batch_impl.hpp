@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -17,6 +18,36 @@ namespace ruv {
 struct Bamg;  // device-built aggregation hierarchy (batch_amg.hip)
 void bamg_destroy(Bamg* a);
 
+// Device blocks kept for reuse: a released block serves a later request of at least its size and at most twice
+// it.  Rebuilds of a hierarchy ask for the same sizes again, so after the first build they allocate nothing.
+// Every user of a pool works on one stream, so a block reused after its release is ordered after its last use.
+struct DevPool {
+    std::multimap<size_t, void*> free_blocks;
+    std::map<void*, size_t> owned;
+    hipError_t get(void** p, size_t bytes) {
+        bytes = std::max<size_t>(256, (bytes + 255) & ~(size_t)255);
+        auto it = free_blocks.lower_bound(bytes);
+        if (it != free_blocks.end() && it->first <= 2 * bytes) {
+            *p = it->second;
+            free_blocks.erase(it);
+            return hipSuccess;
+        }
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return e;
+        owned[q] = bytes;
+        *p = q;
+        return hipSuccess;
+    }
+    void put(void* p) {
+        auto it = owned.find(p);
+        if (it != owned.end()) free_blocks.insert({it->second, p});
+    }
+    ~DevPool() {
+        for (auto& kv : owned) (void)hipFree(kv.first);
+    }
+};
+
 #define HIPCHK(call)                                                                                         \
     do {                                                                                                     \
         hipError_t e_ = (call);                                                                              \
@@ -27,6 +58,7 @@ struct BatchImpl {
     int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
     int ci = 0;          // general slices: 0 = 16-bit column offsets, 1 = 32-bit columns
     int spt = 2;         // slices per tile (chosen by the autotuner)
+    int nt = 0;          // step kernel loads the matrix values non-temporally (chosen by the autotuner)
     int nstencil = 0;    // slices stored in stencil form
     int64_t nrows = 0, nnz = 0, nell = 0, ncol = 0;
     hipStream_t stream = nullptr;
@@ -71,6 +103,7 @@ struct BatchImpl {
     int64_t k_launches = 0;
     double k_seconds = 0.0, k_bytes = 0.0;
     std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
+    DevPool pool;                          // setup and hierarchy blocks (precond AMG)
     Bamg* amg = nullptr;                   // precond AMG: the hierarchy
     int64_t* amg_ptr = nullptr;            // precond AMG: the finest operator in FP64 CSR for (re)builds,
     int32_t* amg_cols = nullptr;           // and each row's finest-level matching group (its tile)

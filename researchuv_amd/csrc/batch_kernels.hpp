@@ -139,9 +139,16 @@ __device__ __forceinline__ T dir(T z, T p, T beta) {
 }
 
 // Opens step `iter` (parity par) of every live chart, then p' = z + beta p and q = A p' over one tile.
-// PROBE = 1 is the same code under its own symbol for the schedule autotuner and the measurement hook, so a
-// kernel trace of the solves is not mixed with whole-batch timing launches.
-template <typename T, int R, int CI, int PROBE = 0>
+// NT = 1 loads the matrix values with the non-temporal hint (they are read once per step; the gathered vectors
+// are the data worth keeping in L2); the autotuner picks it or not.  PROBE = 1 is the same code under its own
+// symbol for the schedule autotuner and the measurement hook, so a kernel trace of the solves is not mixed with
+// whole-batch timing launches.
+template <typename T>
+__device__ __forceinline__ T ld_val(const T* __restrict__ p, int nt) {
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
+template <typename T, int R, int CI, int NT, int PROBE = 0>
 __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict__ act, int na, int par,
                                                    uint32_t iter, const T* __restrict__ vals,
                                                    const T* __restrict__ z, const T* __restrict__ pold,
@@ -191,7 +198,7 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
             const int32_t* dt = B.sl_dtab + (int64_t)s * SLOTS;
 #pragma unroll 4
             for (int k = 0; k < w; ++k) {
-                const T v = vals[e0 + (int64_t)k * n];
+                const T v = ld_val(vals + e0 + (int64_t)k * n, NT);
                 const int64_t j = min(max(i + dt[k], lo), hi);
                 const Vr<T, R> zj = ld<T, R>(z, j);
                 const Vr<T, R> pj = ld<T, R>(pold, j);
@@ -202,7 +209,7 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
             const int64_t f0 = c0 + threadIdx.x;
 #pragma unroll 4
             for (int k = 0; k < w; ++k) {
-                const T v = vals[e0 + (int64_t)k * n];
+                const T v = ld_val(vals + e0 + (int64_t)k * n, NT);
                 const int64_t j = col_of<CI>(B.ecol, f0 + (int64_t)k * n, i);
                 const Vr<T, R> zj = ld<T, R>(z, j);
                 const Vr<T, R> pj = ld<T, R>(pold, j);
