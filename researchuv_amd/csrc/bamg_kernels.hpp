@@ -14,7 +14,8 @@
 //
 // Aggregates are kept as member lists (CSR over the finer level's rows, ascending).  Coarse operators are
 // stored as sliced ELL (slices of 256 consecutive rows padded to their longest row, value k of row lane at
-// off + k * n_s + lane, 32-bit columns), so every coarse pass streams coalesced.
+// off + k * n_s + lane; 16-bit offsets from the row, or 32-bit columns when some offset does not fit), so
+// every coarse pass streams coalesced.
 //
 // Setup works on FP64 CSR levels (64-bit row pointers, 32-bit level-global columns).  Ties in the matching are
 // broken by the lower row index, and every Galerkin row is summed in a fixed order, so the hierarchy is
@@ -345,10 +346,25 @@ __global__ __launch_bounds__(BLK) void k_slice_sizes(int64_t n, const int64_t* _
     sz[s] = m * (r1 - r0);
 }
 
-template <typename T>
+// 1 into *wide when some column of the operator is more than 32767 rows from its row
+__global__ __launch_bounds__(BLK) void k_offsets_wide(int64_t n, const int64_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ cols, int32_t* __restrict__ wide) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+        const int64_t d = (int64_t)cols[a] - i;
+        if (d > 32767 || d < -32767) {
+            atomicOr(wide, 1);
+            return;
+        }
+    }
+}
+
+// CI 0: 16-bit offsets from the row; CI 1: 32-bit columns
+template <typename T, int CI>
 __global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const int64_t* __restrict__ off,
                                                   const int32_t* __restrict__ w, T* __restrict__ ev,
-                                                  int32_t* __restrict__ ec) {
+                                                  void* __restrict__ ec) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     const int64_t s = i / SLICE, lane = i - s * SLICE;
@@ -357,11 +373,13 @@ __global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const i
     int k = 0;
     for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
         ev[e0 + k * ns] = (T)A.vals[a];
-        ec[e0 + k * ns] = A.cols[a];
+        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)((int64_t)A.cols[a] - i);
+        else static_cast<int32_t*>(ec)[e0 + k * ns] = A.cols[a];
     }
     for (; k < w[s]; ++k) {
         ev[e0 + k * ns] = T(0);
-        ec[e0 + k * ns] = (int32_t)i;
+        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = 0;
+        else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)i;
     }
 }
 
@@ -404,7 +422,7 @@ struct CycLevel {
     int64_t n;
     const int64_t* eoff;     // sliced ELL of the level operator (levels >= 1)
     const int32_t* ew;
-    const int32_t* ecols;
+    const void* ecols;       // int16 offsets (CI 0) or int32 columns (CI 1)
     const T* evals;
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
@@ -427,7 +445,7 @@ __device__ __forceinline__ EllRow ell_row(const CycLevel<T>& L, int64_t i) {
 }
 
 // level >= 1: t = b - A (w D^-1 b)
-template <typename T, int R>
+template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= L.n) return;
@@ -437,7 +455,7 @@ __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
 #pragma unroll
     for (int c = 0; c < R; ++c) s[c] = bi.c[c];
     for (int k = 0; k < e.w; ++k) {
-        const int64_t j = L.ecols[e.e0 + k * e.ns];
+        const int64_t j = col_of<CI>(L.ecols, e.e0 + k * e.ns, i);
         const T v = L.evals[e.e0 + k * e.ns] * (w * L.dinv[j]);
         const Vr<T, R> bj = ld<T, R>(L.b, j);
 #pragma unroll
@@ -483,7 +501,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* _
 }
 
 // level >= 1: z = e1 + w D^-1 (b - A e1), e1 in t
-template <typename T, int R>
+template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= L.n) return;
@@ -492,7 +510,7 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
 #pragma unroll
     for (int c = 0; c < R; ++c) s[c] = T(0);
     for (int k = 0; k < e.w; ++k) {
-        const int64_t j = L.ecols[e.e0 + k * e.ns];
+        const int64_t j = col_of<CI>(L.ecols, e.e0 + k * e.ns, i);
         const T v = L.evals[e.e0 + k * e.ns];
         const Vr<T, R> ej = ld<T, R>(L.t, j);
 #pragma unroll

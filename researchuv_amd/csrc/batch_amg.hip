@@ -17,7 +17,8 @@ struct BamgLevel {
     int64_t n = 0, nell = 0;
     int64_t* eoff = nullptr;  // sliced ELL of the operator (levels >= 1)
     int32_t* ew = nullptr;
-    int32_t* ecols = nullptr;
+    void* ecols = nullptr;
+    int ci = 1;               // 0: 16-bit column offsets
     void* evals = nullptr;    // T
     void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal)
     int32_t* agg = nullptr;   // row -> next-level aggregate
@@ -254,19 +255,31 @@ int bamg_build(BatchImpl& Q) {
             if (rc) return rc;
             HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
-            char *v = nullptr, *d = nullptr;
-            HIPCHK(A->alloc(&L.ecols, L.nell));
+            int32_t* wide = nullptr;
+            HIPCHK(T.alloc(&wide, 1));
+            HIPCHK(hipMemsetAsync(wide, 0, sizeof(int32_t), s));
+            hipLaunchKernelGGL(k_offsets_wide, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur.ptr, cur.cols, wide);
+            int32_t hw = 0;
+            HIPCHK(hipMemcpyAsync(&hw, wide, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            L.ci = hw ? 1 : 0;
+            char *v = nullptr, *d = nullptr, *ec = nullptr;
+            HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
+            L.ecols = ec;
             HIPCHK(A->alloc(&v, L.nell * sv));
             HIPCHK(A->alloc(&d, cur.n * sv));
-            if (f32) {
-                hipLaunchKernelGGL((k_ell_fill<float>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (float*)v, L.ecols);
-                hipLaunchKernelGGL((k_cast_inv<float>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (float*)d);
-            } else {
-                hipLaunchKernelGGL((k_ell_fill<double>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (double*)v, L.ecols);
-                hipLaunchKernelGGL((k_cast_inv<double>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (double*)d);
-            }
+            auto fill = [&](auto tv) {
+                using TT = decltype(tv);
+                if (L.ci)
+                    hipLaunchKernelGGL((k_ell_fill<TT, 1>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
+                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (TT*)v, L.ecols);
+                else
+                    hipLaunchKernelGGL((k_ell_fill<TT, 0>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
+                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (TT*)v, L.ecols);
+                hipLaunchKernelGGL((k_cast_inv<TT>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (TT*)d);
+            };
+            if (f32) fill(0.0f);
+            else fill(0.0);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(s));
             L.evals = v;
@@ -385,7 +398,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     };
     hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
-        hipLaunchKernelGGL((kc_resid<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_resid<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        else hipLaunchKernelGGL((kc_resid<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
                            (T*)A.lv[l + 1].b);
     }
@@ -393,7 +407,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     for (size_t l = nl - 1; l-- > 1;) {
         hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w,
                            (const T*)A.lv[l + 1].z);
-        hipLaunchKernelGGL((kc_post<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        else hipLaunchKernelGGL((kc_post<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
     }
     hipLaunchKernelGGL((kc_up0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
                        (const T*)L0.dinv, w, r, (const int32_t*)L0.agg, (const T*)L1.z, z);
