@@ -431,7 +431,16 @@ struct CycLevel {
     T* b;                    // right-hand side on this level
     T* z;                    // correction on this level
     T* t;                    // work vector on this level
+    const int32_t* chart;    // row -> chart
+    const int32_t* done;     // per chart: B_RUNNING while its solve runs
 };
+
+// Rows of charts whose solve has stopped are skipped on every level: a chart's rows only couple to rows of the
+// same chart, and the running charts never read what a stopped chart's rows hold.
+template <typename T>
+__device__ __forceinline__ bool live_row(const CycLevel<T>& L, int64_t i) {
+    return L.done[L.chart[i]] == B_RUNNING;
+}
 
 // slice, lane, slice height of row i of a level's sliced ELL
 struct EllRow {
@@ -448,7 +457,7 @@ __device__ __forceinline__ EllRow ell_row(const CycLevel<T>& L, int64_t i) {
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n) return;
+    if (i >= L.n || !live_row(L, i)) return;
     const EllRow e = ell_row(L, i);
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     T s[R];
@@ -467,11 +476,12 @@ __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
     st<T, R>(L.t, i, o);
 }
 
-// level >= 1, per aggregate I of the next level: b_next[I] = sum over its member rows of t
+// per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t
 template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, T* __restrict__ bnext) {
+__global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, const int32_t* __restrict__ cnext,
+                                                   T* __restrict__ bnext) {
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (I >= nc) return;
+    if (I >= nc || L.done[cnext[I]] != B_RUNNING) return;
     T acc[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) acc[c] = T(0);
@@ -490,7 +500,7 @@ __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, T*
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* __restrict__ znext) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n) return;
+    if (i >= L.n || !live_row(L, i)) return;
     const T di = w * L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     const Vr<T, R> zc = ld<T, R>(znext, L.agg[i]);
@@ -504,7 +514,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* _
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n) return;
+    if (i >= L.n || !live_row(L, i)) return;
     const EllRow e = ell_row(L, i);
     T s[R];
 #pragma unroll
@@ -529,7 +539,7 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n) return;
+    if (i >= L.n || !live_row(L, i)) return;
     const T d = L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     Vr<T, R> o;

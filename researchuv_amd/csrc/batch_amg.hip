@@ -27,6 +27,7 @@ struct BamgLevel {
     void* b = nullptr;
     void* z = nullptr;
     void* t = nullptr;
+    const int32_t* chart = nullptr;  // row -> chart
 };
 
 struct Bamg {
@@ -234,6 +235,7 @@ int bamg_build(BatchImpl& Q) {
     for (int l = 0;; ++l) {
         BamgLevel L;
         L.n = cur.n;
+        L.chart = group;
         // cycle data of this level
         if (l == 0) {
             L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
@@ -344,8 +346,8 @@ int bamg_build(BatchImpl& Q) {
         HIPCHK(nk->alloc(&ncols, nnz2));
         HIPCHK(nk->alloc(&nvals, nnz2));
         HIPCHK(nk->alloc(&ndiag, M2.nc));
-        int32_t* ngroup = nullptr;
-        HIPCHK(nk->alloc(&ngroup, M2.nc));
+        int32_t* ngroup = nullptr;  // kept: the cycle skips the rows of stopped charts by it
+        HIPCHK(A->alloc(&ngroup, M2.nc));
         HIPCHK(hipMemcpyAsync(nptr, M2.C.ptr, sizeof(int64_t) * (M2.nc + 1), hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(ncols, M2.C.cols, sizeof(int32_t) * nnz2, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipMemcpyAsync(nvals, M2.C.vals, sizeof(double) * nnz2, hipMemcpyDeviceToDevice, s));
@@ -394,14 +396,14 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
-                           (T*)L.b, (T*)L.z, (T*)L.t};
+                           (T*)L.b, (T*)L.z, (T*)L.t, L.chart, Q.B.c_done};
     };
-    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, (T*)L1.b);
+    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_resid<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         else hipLaunchKernelGGL((kc_resid<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
-                           (T*)A.lv[l + 1].b);
+                           A.lv[l + 1].chart, (T*)A.lv[l + 1].b);
     }
     hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
     for (size_t l = nl - 1; l-- > 1;) {
