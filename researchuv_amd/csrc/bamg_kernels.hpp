@@ -548,6 +548,31 @@ __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
     st<T, R>(L.z, i, o);
 }
 
+// coarsest level with each chart's block inverted: z = inv(A_c) b over the rows of the row's chart (the inverse of
+// chart c is row-major at minv + moff[c], over the level's rows crow[c] .. crow[c + 1])
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kc_coarsest_dense(CycLevel<T> L, const int64_t* __restrict__ crow,
+                                                         const int64_t* __restrict__ moff, const T* __restrict__ minv) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= L.n || !live_row(L, i)) return;
+    const int c = L.chart[i];
+    const int64_t r0 = crow[c], nc = crow[c + 1] - r0;
+    const T* m = minv + moff[c] + (i - r0) * nc;
+    T acc[R];
+#pragma unroll
+    for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
+    for (int64_t j = 0; j < nc; ++j) {
+        const T v = m[j];
+        const Vr<T, R> bj = ld<T, R>(L.b, r0 + j);
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, bj.c[cc], acc[cc]);
+    }
+    Vr<T, R> o;
+#pragma unroll
+    for (int cc = 0; cc < R; ++cc) o.c[cc] = acc[cc];
+    st<T, R>(L.z, i, o);
+}
+
 // Finest level (the batch's sliced-ELL operator), per live tile.  b is the PCG residual r.
 //   down: t = r - A (w D^-1 r) for the tile's rows; kc_restrict then sums t over each aggregate.
 template <typename T, int R, int CI>

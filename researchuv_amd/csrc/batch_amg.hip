@@ -32,6 +32,9 @@ struct BamgLevel {
 
 struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
+    void* cinv = nullptr;       // T: each chart's inverted block of the coarsest operator (null: Jacobi there)
+    int64_t* cinv_off = nullptr;
+    int64_t* crow_c = nullptr;  // chart row offsets on the coarsest level
     double omega = 2.0 / 3.0;
     double setup_seconds = 0.0;
     DevPool* pool = nullptr;
@@ -185,10 +188,108 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     return RUV_OK;
 }
 
+// chart row offsets on the next level: through one matching (cid2 null) or two
 __global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, const int64_t* __restrict__ cid1,
                                  const int64_t* __restrict__ cid2, int64_t* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (k < m) out[k] = cid2[cid1[at[k]]];
+    if (k < m) out[k] = cid2 ? cid2[cid1[at[k]]] : cid1[at[k]];
+}
+
+// Invert each chart's block of the coarsest operator (FP64 CSR on the device, chart row offsets crow) on the host:
+// Cholesky, then the inverse of the factor; a block that is not numerically positive definite keeps the
+// inverse of its diagonal.  The inverses go to the device in the cycle's precision.
+int coarsest_dense(BatchImpl& Q, Bamg& A, const SetupCsr& cur, const std::vector<int64_t>& crow, bool f32) {
+    hipStream_t s = Q.stream;
+    const int64_t n = cur.n;
+    const int C = Q.nchart;
+    std::vector<int64_t> ptr((size_t)n + 1);
+    HIPCHK(hipMemcpyAsync(ptr.data(), cur.ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int64_t nnz = ptr[n];
+    std::vector<int32_t> cols((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<double> vals((size_t)std::max<int64_t>(nnz, 1));
+    HIPCHK(hipMemcpyAsync(cols.data(), cur.cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vals.data(), cur.vals, sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int64_t> off((size_t)C + 1, 0);
+    for (int c = 0; c < C; ++c) {
+        const int64_t m = crow[c + 1] - crow[c];
+        off[c + 1] = off[c] + m * m;
+    }
+    std::vector<double> inv((size_t)std::max<int64_t>(off[C], 1), 0.0);
+    std::vector<double> a, g;
+    for (int c = 0; c < C; ++c) {
+        const int64_t r0 = crow[c], m = crow[c + 1] - r0;
+        if (m == 0) continue;
+        a.assign((size_t)(m * m), 0.0);
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t k = ptr[r0 + i]; k < ptr[r0 + i + 1]; ++k) {
+                const int64_t j = cols[k] - r0;
+                if (j < 0 || j >= m) return fail(RUV_ERR_PRECOND, "batch amg: coarsest block couples two charts");
+                a[i * m + j] += vals[k];
+            }
+        // lower Cholesky factor in g
+        g.assign((size_t)(m * m), 0.0);
+        bool spd = true;
+        for (int64_t k = 0; k < m && spd; ++k) {
+            double d = a[k * m + k];
+            for (int64_t p = 0; p < k; ++p) d -= g[k * m + p] * g[k * m + p];
+            if (!(d > 1e-13 * std::fabs(a[k * m + k])) || !std::isfinite(d)) {
+                spd = false;
+                break;
+            }
+            const double lk = std::sqrt(d);
+            g[k * m + k] = lk;
+            for (int64_t i = k + 1; i < m; ++i) {
+                double v = a[i * m + k];
+                for (int64_t p = 0; p < k; ++p) v -= g[i * m + p] * g[k * m + p];
+                g[i * m + k] = v / lk;
+            }
+        }
+        double* out = inv.data() + off[c];
+        if (!spd) {
+            for (int64_t i = 0; i < m; ++i) out[i * m + i] = (a[i * m + i] > 0.0) ? 1.0 / a[i * m + i] : 0.0;
+            continue;
+        }
+        // h = inverse of the factor (lower), in a; then inv = h^T h
+        std::fill(a.begin(), a.end(), 0.0);
+        for (int64_t j = 0; j < m; ++j) {
+            a[j * m + j] = 1.0 / g[j * m + j];
+            for (int64_t i = j + 1; i < m; ++i) {
+                double v = 0.0;
+                for (int64_t p = j; p < i; ++p) v -= g[i * m + p] * a[p * m + j];
+                a[i * m + j] = v / g[i * m + i];
+            }
+        }
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t j = 0; j <= i; ++j) {
+                double v = 0.0;
+                for (int64_t p = i; p < m; ++p) v += a[p * m + i] * a[p * m + j];
+                out[i * m + j] = v;
+                out[j * m + i] = v;
+            }
+    }
+    const size_t ne = inv.size();
+    if (f32) {
+        std::vector<float> f(ne);
+        for (size_t k = 0; k < ne; ++k) f[k] = (float)inv[k];
+        float* d = nullptr;
+        HIPCHK(A.alloc(&d, ne));
+        HIPCHK(hipMemcpyAsync(d, f.data(), sizeof(float) * ne, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        A.cinv = d;
+    } else {
+        double* d = nullptr;
+        HIPCHK(A.alloc(&d, ne));
+        HIPCHK(hipMemcpyAsync(d, inv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s));
+        A.cinv = d;
+    }
+    HIPCHK(A.alloc(&A.cinv_off, (size_t)C + 1));
+    HIPCHK(A.alloc(&A.crow_c, (size_t)C + 1));
+    HIPCHK(hipMemcpyAsync(A.cinv_off, off.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(A.crow_c, crow.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
 }
 
 }  // namespace
@@ -221,17 +322,30 @@ int bamg_build(BatchImpl& Q) {
     const size_t sv = f32 ? sizeof(float) : sizeof(double);
     const int R = Q.R;
     // matching: coupling threshold relative to a row's strongest, handshake rounds (overridable for tuning)
+    // and matchings per level: one (aggregates of about two rows) converges in about half the iterations of two
+    // (aggregates of about four) on the heterogeneous weights of jittered meshes, for about 1.5x the coarse work
     double theta = 0.1;
-    int rounds = 4;
+    int rounds = 4, passes = 1;
     if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("RUV_BATCH_AMG_PASSES")) passes = std::min(2, std::max(1, std::atoi(e)));
+    // coarsening stops once every chart has at most dense_rows rows; the coarsest level is then solved exactly
+    // with each chart's inverted block (up to dense_max rows a chart, else Jacobi)
+    int64_t dense_rows = 64;
+    const int64_t dense_max = 256;
+    if (const char* e = std::getenv("RUV_BATCH_AMG_DENSE")) dense_rows = std::max(0, std::atoi(e));
+    auto max_chart_rows = [&](const std::vector<int64_t>& cr) {
+        int64_t m = 0;
+        for (int c = 0; c < Q.nchart; ++c) m = std::max(m, cr[c + 1] - cr[c]);
+        return m;
+    };
     const int64_t n0 = Q.nrows;
     const int32_t* group = Q.amg_group;
     SetupCsr cur{n0, Q.amg_ptr, Q.amg_cols, Q.amg_vals};
     const double* cur_diag = nullptr;
     std::vector<int64_t> crow = Q.crow;
     std::unique_ptr<Scratch> keep(new Scratch(&Q.pool));  // owns the current level's setup CSR (levels >= 1)
-    const int max_levels = 24;
+    const int max_levels = 40;
     for (int l = 0;; ++l) {
         BamgLevel L;
         L.n = cur.n;
@@ -297,27 +411,34 @@ int bamg_build(BatchImpl& Q) {
             L.z = z;
             L.t = t;
         }
-        const bool last = (l + 1 >= max_levels) || (l > 0 && cur.n <= (int64_t)Q.nchart);
+        const bool last = (l + 1 >= max_levels) ||
+                          (l > 0 && (cur.n <= (int64_t)Q.nchart || max_chart_rows(crow) <= dense_rows));
         if (last) {
             A->lv.push_back(L);
             break;
         }
-        // two matchings make the next level
+        // one or two matchings make the next level
         std::unique_ptr<Scratch> S1(new Scratch(&Q.pool)), S2(new Scratch(&Q.pool));
         Matched M1, M2;
         int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
         if (rc) return rc;
-        rc = match_and_coarsen(s, M1.C, M1.group, theta, rounds, *S2, &M2);
-        if (rc) return rc;
-        if (l > 0 && M2.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
+        if (passes > 1) {
+            rc = match_and_coarsen(s, M1.C, M1.group, theta, rounds, *S2, &M2);
+            if (rc) return rc;
+        }
+        const Matched& MF = (passes > 1) ? M2 : M1;
+        if (l > 0 && MF.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
             A->lv.push_back(L);
             break;
         }
         HIPCHK(A->alloc(&L.agg, cur.n));
-        hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
-                           (const int32_t*)M2.agg, L.agg);
+        if (passes > 1)
+            hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
+                               (const int32_t*)M2.agg, L.agg);
+        else
+            HIPCHK(hipMemcpyAsync(L.agg, M1.agg, sizeof(int32_t) * cur.n, hipMemcpyDeviceToDevice, s));
         HIPCHK(hipGetLastError());
-        rc = build_members(s, cur.n, L.agg, M2.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows, (int32_t*)nullptr);
+        rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows, (int32_t*)nullptr);
         if (rc) return rc;
         // chart offsets on the next level
         {
@@ -327,7 +448,8 @@ int bamg_build(BatchImpl& Q) {
             HIPCHK(S1->alloc(&d_out, at.size()));
             HIPCHK(hipMemcpyAsync(d_at, at.data(), sizeof(int64_t) * at.size(), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_gather_offsets, rows_grid((int64_t)at.size()), dim3(BLK), 0, s, (int64_t)at.size(),
-                               (const int64_t*)d_at, (const int64_t*)M1.cid, (const int64_t*)M2.cid, d_out);
+                               (const int64_t*)d_at, (const int64_t*)M1.cid,
+                               (const int64_t*)(passes > 1 ? M2.cid : nullptr), d_out);
             std::vector<int64_t> out(at.size());
             HIPCHK(hipMemcpyAsync(out.data(), d_out, sizeof(int64_t) * at.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -340,26 +462,26 @@ int bamg_build(BatchImpl& Q) {
         int32_t* ncols = nullptr;
         double *nvals = nullptr, *ndiag = nullptr;
         int64_t nnz2 = 0;
-        HIPCHK(hipMemcpyAsync(&nnz2, M2.C.ptr + M2.nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&nnz2, MF.C.ptr + MF.nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(nk->alloc(&nptr, M2.nc + 1));
+        HIPCHK(nk->alloc(&nptr, MF.nc + 1));
         HIPCHK(nk->alloc(&ncols, nnz2));
         HIPCHK(nk->alloc(&nvals, nnz2));
-        HIPCHK(nk->alloc(&ndiag, M2.nc));
+        HIPCHK(nk->alloc(&ndiag, MF.nc));
         int32_t* ngroup = nullptr;  // kept: the cycle skips the rows of stopped charts by it
-        HIPCHK(A->alloc(&ngroup, M2.nc));
-        HIPCHK(hipMemcpyAsync(nptr, M2.C.ptr, sizeof(int64_t) * (M2.nc + 1), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(ncols, M2.C.cols, sizeof(int32_t) * nnz2, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(nvals, M2.C.vals, sizeof(double) * nnz2, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(ndiag, M2.cdiag, sizeof(double) * M2.nc, hipMemcpyDeviceToDevice, s));
+        HIPCHK(A->alloc(&ngroup, MF.nc));
+        HIPCHK(hipMemcpyAsync(nptr, MF.C.ptr, sizeof(int64_t) * (MF.nc + 1), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ncols, MF.C.cols, sizeof(int32_t) * nnz2, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(nvals, MF.C.vals, sizeof(double) * nnz2, hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(ndiag, MF.cdiag, sizeof(double) * MF.nc, hipMemcpyDeviceToDevice, s));
         int64_t* d_crow = nullptr;
         HIPCHK(nk->alloc(&d_crow, crow.size()));
         HIPCHK(hipMemcpyAsync(d_crow, crow.data(), sizeof(int64_t) * crow.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_chart_groups, rows_grid(M2.nc), dim3(BLK), 0, s, M2.nc, (const int64_t*)d_crow,
+        hipLaunchKernelGGL(k_chart_groups, rows_grid(MF.nc), dim3(BLK), 0, s, MF.nc, (const int64_t*)d_crow,
                            Q.nchart, ngroup);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
-        cur = SetupCsr{M2.nc, nptr, ncols, nvals};
+        cur = SetupCsr{MF.nc, nptr, ncols, nvals};
         cur_diag = ndiag;
         group = ngroup;
         keep = std::move(nk);  // frees the previous level's setup copy
@@ -368,6 +490,10 @@ int bamg_build(BatchImpl& Q) {
     }
     // finest level's aggregate map and members live on; level 1 must exist
     if (A->lv.size() < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
+    if (max_chart_rows(crow) <= dense_max) {
+        int rc = coarsest_dense(Q, *A, cur, crow, f32);
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -405,7 +531,11 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
                            A.lv[l + 1].chart, (T*)A.lv[l + 1].b);
     }
-    hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
+    if (A.cinv)
+        hipLaunchKernelGGL((kc_coarsest_dense<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1),
+                           (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const T*)A.cinv);
+    else
+        hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
     for (size_t l = nl - 1; l-- > 1;) {
         hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w,
                            (const T*)A.lv[l + 1].z);
