@@ -251,16 +251,19 @@ __global__ __launch_bounds__(BLK) void k_member_sort(int64_t nc, const int64_t* 
 constexpr int GAL_BLK = 64;
 constexpr int GAL_CAP = 64;
 
+// With `list`, the kernel does the rows list[0 .. nc) (those k_galerkin_reg could not).
 template <bool FILL>
 __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
                                                       const int64_t* __restrict__ mrows,
                                                       const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
                                                       const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
-                                                      double* __restrict__ cvals, double* __restrict__ cdiag) {
+                                                      double* __restrict__ cvals, double* __restrict__ cdiag,
+                                                      const int64_t* __restrict__ list) {
     __shared__ int32_t skey[GAL_CAP * GAL_BLK];
     __shared__ double sval[GAL_CAP * GAL_BLK];
-    const int64_t I = (int64_t)blockIdx.x * GAL_BLK + threadIdx.x;
-    if (I >= nc) return;
+    const int64_t idx = (int64_t)blockIdx.x * GAL_BLK + threadIdx.x;
+    if (idx >= nc) return;
+    const int64_t I = list ? list[idx] : idx;
     const int tid = threadIdx.x;
     const int64_t m0 = mptr[I], m1 = mptr[I + 1];
     int m = 0;
@@ -331,6 +334,87 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
     }
     if (!FILL) cnt[I] = count;
     else cdiag[I] = diag;
+}
+
+// The same Galerkin rows, one thread a row with the distinct columns in registers (up to GR_SLOTS of them): each
+// entry adds into the slot of its column, in row order, so the sums equal k_galerkin's; the output position of
+// a slot is the number of smaller columns.  A row with more distinct columns goes to `over` (counted by nover)
+// for k_galerkin.
+constexpr int GR_SLOTS = 32;
+
+template <bool FILL>
+__global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
+                                                      const int64_t* __restrict__ mrows,
+                                                      const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
+                                                      const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
+                                                      double* __restrict__ cvals, double* __restrict__ cdiag,
+                                                      unsigned long long* __restrict__ nover,
+                                                      int64_t* __restrict__ over) {
+    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (I >= nc) return;
+    int32_t key[GR_SLOTS];
+    double val[GR_SLOTS];
+#pragma unroll
+    for (int k = 0; k < GR_SLOTS; ++k) {
+        key[k] = INT32_MAX;
+        val[k] = 0.0;
+    }
+    int n = 0;
+    bool full = false;
+    const int64_t m1 = mptr[I + 1];
+    for (int64_t q = mptr[I]; q < m1; ++q) {
+        const int64_t r = mrows[q];
+        const int64_t e1 = A.ptr[r + 1];
+        for (int64_t e = A.ptr[r]; e < e1; ++e) {
+            const int32_t c = agg[A.cols[e]];
+            const double v = A.vals[e];
+            bool hit = false;
+#pragma unroll
+            for (int k = 0; k < GR_SLOTS; ++k)
+                if (key[k] == c) {
+                    val[k] += v;
+                    hit = true;
+                }
+            if (!hit) {
+                if (n < GR_SLOTS) {
+#pragma unroll
+                    for (int k = 0; k < GR_SLOTS; ++k)
+                        if (k == n) {
+                            key[k] = c;
+                            val[k] += v;
+                        }
+                    ++n;
+                } else {
+                    full = true;
+                }
+            }
+        }
+    }
+    if (full) {
+        if (!FILL) {
+            cnt[I] = 0;
+            over[atomicAdd(nover, 1ull)] = I;
+        }
+        return;
+    }
+    if (!FILL) {
+        cnt[I] = n;
+        return;
+    }
+    const int64_t out = cptr[I];
+    double diag = 0.0;
+#pragma unroll
+    for (int k = 0; k < GR_SLOTS; ++k) {
+        if (k < n) {
+            int rank = 0;
+#pragma unroll
+            for (int t = 0; t < GR_SLOTS; ++t) rank += (key[t] < key[k]) ? 1 : 0;
+            ccols[out + rank] = key[k];
+            cvals[out + rank] = val[k];
+            if (key[k] == I) diag = val[k];
+        }
+    }
+    cdiag[I] = diag;
 }
 
 // sliced ELL of a coarse operator: slice widths, then the fill (padding: value 0, the row itself)

@@ -161,13 +161,26 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&M->group, nc));
     rc = build_members(s, n, M->agg, nc, group, S, &M->mptr, &M->mrows, M->group);
     if (rc) return rc;
-    int64_t *cnt = nullptr, *cptr = nullptr;
+    // Galerkin operator: rows of few distinct columns in registers, the others (listed) through LDS
+    int64_t *cnt = nullptr, *cptr = nullptr, *over = nullptr;
+    unsigned long long* nover = nullptr;
     HIPCHK(S.alloc(&cnt, nc));
     HIPCHK(S.alloc(&cptr, nc + 1));
-    const dim3 gg((unsigned)std::max<int64_t>(1, (nc + GAL_BLK - 1) / GAL_BLK));
-    hipLaunchKernelGGL((k_galerkin<false>), gg, dim3(GAL_BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+    HIPCHK(S.alloc(&over, nc));
+    HIPCHK(S.alloc(&nover, 1));
+    HIPCHK(hipMemsetAsync(nover, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL((k_galerkin_reg<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
                        (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
-                       (int32_t*)nullptr, (double*)nullptr, (double*)nullptr);
+                       (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, nover, over);
+    unsigned long long hover = 0;
+    HIPCHK(hipMemcpyAsync(&hover, nover, sizeof(hover), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int64_t no = (int64_t)hover;
+    const dim3 go((unsigned)std::max<int64_t>(1, (no + GAL_BLK - 1) / GAL_BLK));
+    if (no > 0)
+        hipLaunchKernelGGL((k_galerkin<false>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
+                           (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
+                           (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, (const int64_t*)over);
     HIPCHK(hipGetLastError());
     rc = scan(s, nc, cnt, cptr, S);
     if (rc) return rc;
@@ -180,9 +193,13 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
     HIPCHK(S.alloc(&M->cdiag, nc));
-    hipLaunchKernelGGL((k_galerkin<true>), gg, dim3(GAL_BLK), 0, s, nc, A, (const int64_t*)M->mptr,
+    hipLaunchKernelGGL((k_galerkin_reg<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
                        (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
-                       ccols, cvals, M->cdiag);
+                       ccols, cvals, M->cdiag, (unsigned long long*)nullptr, (int64_t*)nullptr);
+    if (no > 0)
+        hipLaunchKernelGGL((k_galerkin<true>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
+                           (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr,
+                           (const int64_t*)cptr, ccols, cvals, M->cdiag, (const int64_t*)over);
     HIPCHK(hipGetLastError());
     M->C = SetupCsr{nc, cptr, ccols, cvals};
     return RUV_OK;
