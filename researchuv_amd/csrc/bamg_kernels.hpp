@@ -103,16 +103,6 @@ struct SetupCsr {
     const double* vals;
 };
 
-// strongest coupling of each row: max over off-diagonal a_ij < 0 of -a_ij
-__global__ __launch_bounds__(BLK) void k_smax(SetupCsr A, double* __restrict__ smax) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= A.n) return;
-    double m = 0.0;
-    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
-        if (A.cols[k] != i && A.vals[k] < 0.0) m = fmax(m, -A.vals[k]);
-    smax[i] = m;
-}
-
 // Preference of the coupling (i, j): its strength in bins of a quarter octave, then a hash of the unordered
 // pair.  Couplings that vary smoothly along a line (uniform or geometric spacing) fall into one bin, where the
 // hash -- the same seen from either end -- acts as a random priority; ordering by exact strength or by index
@@ -129,27 +119,84 @@ __device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
     return ((uint64_t)(uint32_t)max(bin, 0) << 32) | h;
 }
 
-// an unmatched row proposes its preferred unmatched neighbour of the same group among the couplings
-// >= theta * its strongest
-__global__ __launch_bounds__(BLK) void k_propose(SetupCsr A, const double* __restrict__ smax,
-                                                 const int32_t* __restrict__ group, const int64_t* __restrict__ mate,
-                                                 int64_t* __restrict__ prop, double theta) {
+// Handshake matching.  Each row ranks its eligible neighbours once -- same group, coupling a_ij < 0 with
+// -a_ij >= theta * the row's strongest -- by pair_pref (ties to the lower row), keeping the best MCAND; every round
+// an unmatched row proposes the first of them still unmatched, and mutual proposals become pairs.
+constexpr int MCAND = 8;
+
+__device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, int32_t k) {
+    return p > q || (p == q && j < k);
+}
+
+// ranking and the first round's proposals (every row is unmatched then)
+__global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, const int32_t* __restrict__ group, double theta,
+                                                    int32_t* __restrict__ cand, int32_t* __restrict__ ncand,
+                                                    int64_t* __restrict__ prop) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= A.n) return;
+    const int64_t e0 = A.ptr[i], e1 = A.ptr[i + 1];
+    double smax = 0.0;
+    for (int64_t e = e0; e < e1; ++e)
+        if (A.cols[e] != i && A.vals[e] < 0.0) smax = fmax(smax, -A.vals[e]);
+    const double thr = theta * smax;
+    const int32_t g = group[i];
+    uint64_t kp[MCAND];
+    int32_t kj[MCAND];
+#pragma unroll
+    for (int k = 0; k < MCAND; ++k) {
+        kp[k] = 0;
+        kj[k] = INT32_MAX;
+    }
+    int cnt = 0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int32_t j = A.cols[e];
+        if (j == i || !(A.vals[e] < 0.0) || group[j] != g) continue;
+        const double sv = -A.vals[e];
+        if (sv < thr) continue;
+        // carry (p, j) down the descending list: it takes the first slot it precedes, or the first free one
+        uint64_t cp = pair_pref(sv, i, j);
+        int32_t cj = j;
+        bool carry = true;
+#pragma unroll
+        for (int k = 0; k < MCAND; ++k) {
+            if (carry) {
+                if (k >= cnt) {
+                    kp[k] = cp;
+                    kj[k] = cj;
+                    carry = false;
+                } else if (pref_before(cp, cj, kp[k], kj[k])) {
+                    const uint64_t tp = kp[k];
+                    const int32_t tj = kj[k];
+                    kp[k] = cp;
+                    kj[k] = cj;
+                    cp = tp;
+                    cj = tj;
+                }
+            }
+        }
+        if (cnt < MCAND) ++cnt;
+    }
+#pragma unroll
+    for (int k = 0; k < MCAND; ++k)
+        if (k < cnt) cand[i * MCAND + k] = kj[k];
+    ncand[i] = cnt;
+    prop[i] = cnt ? kj[0] : -1;
+}
+
+// later rounds: an unmatched row proposes its first candidate still unmatched
+__global__ __launch_bounds__(BLK) void k_propose(int64_t n, const int32_t* __restrict__ cand,
+                                                 const int32_t* __restrict__ ncand,
+                                                 const int64_t* __restrict__ mate, int64_t* __restrict__ prop) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
     int64_t best = -1;
     if (mate[i] < 0) {
-        const double thr = theta * smax[i];
-        const int32_t g = group[i];
-        uint64_t bp = 0;
-        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
-            const int64_t j = A.cols[k];
-            if (j == i || !(A.vals[k] < 0.0) || mate[j] >= 0 || group[j] != g) continue;
-            const double s = -A.vals[k];
-            if (s < thr) continue;
-            const uint64_t p = pair_pref(s, i, j);
-            if (best < 0 || p > bp || (p == bp && j < best)) {
-                bp = p;
+        const int c = ncand[i];
+        for (int k = 0; k < c; ++k) {
+            const int32_t j = cand[i * MCAND + k];
+            if (mate[j] < 0) {
                 best = j;
+                break;
             }
         }
     }
@@ -164,29 +211,23 @@ __global__ __launch_bounds__(BLK) void k_accept(int64_t n, const int64_t* __rest
     if (j >= 0 && prop[j] == i) mate[i] = j;
 }
 
-// a row still unmatched joins the pair it is most strongly coupled to (same group, coupling >= theta * its
-// strongest, ties to the lower row); join = that pair's lower row, or -1 (the row stays alone)
-__global__ __launch_bounds__(BLK) void k_join(SetupCsr A, const double* __restrict__ smax,
-                                              const int32_t* __restrict__ group, const int64_t* __restrict__ mate,
-                                              int64_t* __restrict__ join, double theta) {
+// a row still unmatched joins the pair of its first candidate that is matched; join = that pair's lower row, or
+// -1 (the row stays alone)
+__global__ __launch_bounds__(BLK) void k_join(int64_t n, const int32_t* __restrict__ cand,
+                                              const int32_t* __restrict__ ncand, const int64_t* __restrict__ mate,
+                                              int64_t* __restrict__ join) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= A.n) return;
+    if (i >= n) return;
     int64_t best = -1;
     if (mate[i] < 0) {
-        const double thr = theta * smax[i];
-        const int32_t g = group[i];
-        double bs = 0.0;
-        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
-            const int64_t j = A.cols[k];
-            if (j == i || !(A.vals[k] < 0.0) || mate[j] < 0 || group[j] != g) continue;
-            const double s = -A.vals[k];
-            if (s < thr) continue;
-            if (best < 0 || s > bs || (s == bs && j < best)) {
-                bs = s;
-                best = j;
+        const int c = ncand[i];
+        for (int k = 0; k < c; ++k) {
+            const int64_t j = cand[i * MCAND + k];
+            if (mate[j] >= 0) {
+                best = min(j, mate[j]);
+                break;
             }
         }
-        if (best >= 0) best = min(best, mate[best]);
     }
     join[i] = best;
 }

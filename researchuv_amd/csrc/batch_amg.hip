@@ -77,6 +77,12 @@ struct Scratch {
     }
 };
 
+// move the buffer p from one owner's list to another's
+void hand_over(Scratch& from, void* p, std::vector<void*>& to) {
+    from.mem.erase(std::remove(from.mem.begin(), from.mem.end(), p), from.mem.end());
+    to.push_back(p);
+}
+
 // out[0..n) = exclusive scan of in, out[n] = total
 int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) {
     const int64_t nb = std::max<int64_t>(1, (n + SCAN_BLK - 1) / SCAN_BLK);
@@ -130,9 +136,10 @@ struct Matched {
 int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, double theta, int rounds,
                       Scratch& S, Matched* M) {
     const int64_t n = A.n;
-    double* smax = nullptr;
+    int32_t *cand = nullptr, *ncand = nullptr;
     int64_t *prop = nullptr, *flag = nullptr, *mate = nullptr, *join = nullptr;
-    HIPCHK(S.alloc(&smax, n));
+    HIPCHK(S.alloc(&cand, (size_t)n * MCAND));
+    HIPCHK(S.alloc(&ncand, n));
     HIPCHK(S.alloc(&prop, n));
     HIPCHK(S.alloc(&flag, n));
     HIPCHK(S.alloc(&mate, n));
@@ -140,14 +147,15 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&M->cid, n + 1));
     HIPCHK(S.alloc(&M->agg, n));
     HIPCHK(hipMemsetAsync(mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
-    hipLaunchKernelGGL(k_smax, rows_grid(n), dim3(BLK), 0, s, A, smax);
+    hipLaunchKernelGGL(k_candidates, rows_grid(n), dim3(BLK), 0, s, A, group, theta, cand, ncand, prop);
     for (int r = 0; r < rounds; ++r) {
-        hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, A, (const double*)smax, group,
-                           (const int64_t*)mate, prop, theta);
+        if (r > 0)
+            hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)cand,
+                               (const int32_t*)ncand, (const int64_t*)mate, prop);
         hipLaunchKernelGGL(k_accept, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)prop, mate);
     }
-    hipLaunchKernelGGL(k_join, rows_grid(n), dim3(BLK), 0, s, A, (const double*)smax, group, (const int64_t*)mate,
-                       join, theta);
+    hipLaunchKernelGGL(k_join, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)cand, (const int32_t*)ncand,
+                       (const int64_t*)mate, join);
     hipLaunchKernelGGL(k_roots, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join, flag);
     HIPCHK(hipGetLastError());
     int rc = scan(s, n, flag, M->cid, S);
@@ -448,15 +456,21 @@ int bamg_build(BatchImpl& Q) {
             A->lv.push_back(L);
             break;
         }
-        HIPCHK(A->alloc(&L.agg, cur.n));
-        if (passes > 1)
+        Scratch& SF = (passes > 1) ? *S2 : *S1;
+        if (passes > 1) {
+            HIPCHK(A->alloc(&L.agg, cur.n));
             hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
                                (const int32_t*)M2.agg, L.agg);
-        else
-            HIPCHK(hipMemcpyAsync(L.agg, M1.agg, sizeof(int32_t) * cur.n, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipGetLastError());
-        rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows, (int32_t*)nullptr);
-        if (rc) return rc;
+            HIPCHK(hipGetLastError());
+            rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows,
+                               (int32_t*)nullptr);
+            if (rc) return rc;
+        } else {  // the matching's own aggregate map and member lists
+            for (void* p : {(void*)M1.agg, (void*)M1.mptr, (void*)M1.mrows}) hand_over(*S1, p, A->mem);
+            L.agg = M1.agg;
+            L.mptr = M1.mptr;
+            L.mrows = M1.mrows;
+        }
         // chart offsets on the next level
         {
             std::vector<int64_t> at(crow.begin(), crow.end());
@@ -475,22 +489,11 @@ int bamg_build(BatchImpl& Q) {
         A->lv.push_back(L);
         // the next level's operator (kept), its diagonal, and chart groups
         std::unique_ptr<Scratch> nk(new Scratch(&Q.pool));
-        int64_t* nptr = nullptr;
-        int32_t* ncols = nullptr;
-        double *nvals = nullptr, *ndiag = nullptr;
-        int64_t nnz2 = 0;
-        HIPCHK(hipMemcpyAsync(&nnz2, MF.C.ptr + MF.nc, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(nk->alloc(&nptr, MF.nc + 1));
-        HIPCHK(nk->alloc(&ncols, nnz2));
-        HIPCHK(nk->alloc(&nvals, nnz2));
-        HIPCHK(nk->alloc(&ndiag, MF.nc));
+        for (const void* p : {(const void*)MF.C.ptr, (const void*)MF.C.cols, (const void*)MF.C.vals,
+                              (const void*)MF.cdiag})
+            hand_over(SF, (void*)p, nk->mem);
         int32_t* ngroup = nullptr;  // kept: the cycle skips the rows of stopped charts by it
         HIPCHK(A->alloc(&ngroup, MF.nc));
-        HIPCHK(hipMemcpyAsync(nptr, MF.C.ptr, sizeof(int64_t) * (MF.nc + 1), hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(ncols, MF.C.cols, sizeof(int32_t) * nnz2, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(nvals, MF.C.vals, sizeof(double) * nnz2, hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipMemcpyAsync(ndiag, MF.cdiag, sizeof(double) * MF.nc, hipMemcpyDeviceToDevice, s));
         int64_t* d_crow = nullptr;
         HIPCHK(nk->alloc(&d_crow, crow.size()));
         HIPCHK(hipMemcpyAsync(d_crow, crow.data(), sizeof(int64_t) * crow.size(), hipMemcpyHostToDevice, s));
@@ -498,8 +501,8 @@ int bamg_build(BatchImpl& Q) {
                            Q.nchart, ngroup);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
-        cur = SetupCsr{MF.nc, nptr, ncols, nvals};
-        cur_diag = ndiag;
+        cur = MF.C;
+        cur_diag = MF.cdiag;
         group = ngroup;
         keep = std::move(nk);  // frees the previous level's setup copy
         S1.reset();
