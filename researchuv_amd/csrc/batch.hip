@@ -366,7 +366,9 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
     }
     HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    int chunk = 8;
+    // steps between looks at which charts still run: growing up to 64 for the cheap Jacobi/plain steps; the AMG
+    // step costs a V-cycle, so a look (one small copy and a wait) every few steps wastes less than overshooting
+    int chunk = (PC == 2) ? Q.amg_chunk : 8;
     uint32_t iter = 0;
     int par = 0;
     for (;;) {
@@ -402,7 +404,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        chunk = std::min(chunk * 2, 64);
+        if (PC != 2) chunk = std::min(chunk * 2, 64);
     }
     HIPCHK(hipMemcpyAsync(Q.h_iters, B.c_iters, sizeof(unsigned) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -556,6 +558,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
         const double v = std::atof(e);
         if (v > 0.0 && v < 1.0) Q->inner_tol = v;
     }
+    if (const char* e = std::getenv("RUV_BATCH_AMG_CHUNK")) Q->amg_chunk = std::min(64, std::max(1, std::atoi(e)));
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

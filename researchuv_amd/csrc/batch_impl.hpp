@@ -96,6 +96,7 @@ struct BatchImpl {
     std::vector<uint64_t> iters_total;
     double inner_tol = 1e-5;
     int max_rounds = 40;
+    int amg_chunk = 4;  // AMG-preconditioned steps between convergence looks
     // step-kernel accounting (optional): an event pair around every launch, summed after each solve
     bool ktime = false;
     std::vector<hipEvent_t> kev;
