@@ -458,6 +458,24 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, co
     cdiag[I] = diag;
 }
 
+// Gershgorin bound of the Jacobi-scaled operator, max over rows of sum_j |a_ij| / a_ii, into *g (as the bits of
+// a non-negative double, which order like the double)
+__global__ __launch_bounds__(BLK) void k_gershgorin(SetupCsr A, unsigned long long* __restrict__ g) {
+    __shared__ double sh[WAVES];
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    double q = 0.0;
+    if (i < A.n) {
+        double d = 0.0, sum = 0.0;
+        for (int64_t e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+            sum += fabs(A.vals[e]);
+            if (A.cols[e] == i) d += A.vals[e];
+        }
+        q = (d > 0.0) ? sum / d : 0.0;
+    }
+    const double m = block_max_all(q, sh);
+    if (threadIdx.x == 0) atomicMax(g, (unsigned long long)__double_as_longlong(m));
+}
+
 // sliced ELL of a coarse operator: slice widths, then the fill (padding: value 0, the row itself)
 __global__ __launch_bounds__(BLK) void k_slice_sizes(int64_t n, const int64_t* __restrict__ ptr,
                                                      int32_t* __restrict__ w, int64_t* __restrict__ sz) {
@@ -736,6 +754,75 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
         for (int cc = 0; cc < R; ++cc) o.c[cc] = acc[cc];
         st<T, R>(t_out, i, o);
     }
+}
+
+//   up, in two passes: e1 = w D^-1 r + P z1 into e (kc_prolong0), then z = e1 + w D^-1 (r - A e1) with the
+//   neighbours' e1 read back (kc_post0), partial r.z of the tile into part_c[par]
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict__ act, int na,
+                                                   const T* __restrict__ dinv, T w, const T* __restrict__ r,
+                                                   const int32_t* __restrict__ agg, const T* __restrict__ z1,
+                                                   T* __restrict__ e) {
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    if (B.c_done[B.tile_chart[t]] != B_RUNNING) return;
+    const int64_t r1 = B.tile_r1[t];
+    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
+        const T di = w * dinv[i];
+        const Vr<T, R> ri = ld<T, R>(r, i);
+        const Vr<T, R> zc = ld<T, R>(z1, agg[i]);
+        Vr<T, R> o;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) o.c[cc] = fma(di, ri.c[cc], zc.c[cc]);
+        st<T, R>(e, i, o);
+    }
+}
+
+template <typename T, int R, int CI>
+__global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ act, int na, int par,
+                                                const T* __restrict__ vals, const T* __restrict__ dinv, T w,
+                                                const T* __restrict__ r, const T* __restrict__ e, T* __restrict__ z) {
+    __shared__ double sh[WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    const int c = B.tile_chart[t];
+    if (B.c_done[c] != B_RUNNING) return;
+    const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
+    double arz = 0.0;
+    const int s1 = B.tile_s0[t + 1];
+    for (int s = B.tile_s0[t]; s < s1; ++s) {
+        const int n = B.sl_n[s];
+        if ((int)threadIdx.x >= n) continue;
+        const int64_t i = B.sl_row0[s] + threadIdx.x;
+        const int64_t e0 = B.sl_off[s] + threadIdx.x;
+        const int64_t c0 = B.sl_coff[s];
+        const int wd = B.sl_w[s];
+        T acc[R];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
+        for (int k = 0; k < wd; ++k) {
+            const T v = vals[e0 + (int64_t)k * n];
+            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
+                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
+            const Vr<T, R> ej = ld<T, R>(e, j);
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, ej.c[cc], acc[cc]);
+        }
+        const T di = w * dinv[i];
+        const Vr<T, R> ri = ld<T, R>(r, i);
+        const Vr<T, R> ei = ld<T, R>(e, i);
+        Vr<T, R> o;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) {
+            o.c[cc] = fma(di, ri.c[cc] - acc[cc], ei.c[cc]);
+            arz += (double)ri.c[cc] * (double)o.c[cc];
+        }
+        st<T, R>(z, i, o);
+    }
+    const double srz = block_sum_all(arz, sh);
+    if (threadIdx.x == 0) B.part_c[(int64_t)par * B.ntile + t] = srz;
 }
 
 //   up: e1 = w D^-1 r + P z1; z = e1 + w D^-1 (r - A e1) for the tile's rows; partial r.z of the tile into

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,7 @@ struct BamgLevel {
     void* z = nullptr;
     void* t = nullptr;
     const int32_t* chart = nullptr;  // row -> chart
+    double omega = 2.0 / 3.0;        // Jacobi weight of this level's smoothing steps
 };
 
 struct Bamg {
@@ -35,7 +37,10 @@ struct Bamg {
     void* cinv = nullptr;       // T: each chart's inverted block of the coarsest operator (null: Jacobi there)
     int64_t* cinv_off = nullptr;
     int64_t* crow_c = nullptr;  // chart row offsets on the coarsest level
-    double omega = 2.0 / 3.0;
+    // Jacobi weight of the smoothing steps: omega_max, or less where a level's Gershgorin bound g on D^-1 A
+    // asks for it (omega <= 1.9 / g keeps each smoother an A-norm contraction, so the cycle stays SPD)
+    double omega_max = 0.9;
+    bool split_up = true;  // level 0's way up in two passes (kc_prolong0, kc_post0) or one (kc_up0)
     double setup_seconds = 0.0;
     DevPool* pool = nullptr;
     std::vector<void*> mem;
@@ -343,6 +348,11 @@ int bamg_build(BatchImpl& Q) {
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, s));
     std::unique_ptr<Bamg> A(new Bamg(&Q.pool));
+    if (const char* e = std::getenv("RUV_BATCH_AMG_OMEGA")) {
+        const double v = std::atof(e);
+        if (v > 0.0 && v < 1.0) A->omega_max = v;
+    }
+    if (const char* e = std::getenv("RUV_BATCH_AMG_SPLIT_UP")) A->split_up = std::atoi(e) != 0;
     const bool f32 = Q.mode == RUV_MODE_MIXED;
     const size_t sv = f32 ? sizeof(float) : sizeof(double);
     const int R = Q.R;
@@ -375,6 +385,20 @@ int bamg_build(BatchImpl& Q) {
         BamgLevel L;
         L.n = cur.n;
         L.chart = group;
+        {
+            unsigned long long* g = nullptr;
+            Scratch T(&Q.pool);
+            HIPCHK(T.alloc(&g, 1));
+            HIPCHK(hipMemsetAsync(g, 0, sizeof(unsigned long long), s));
+            hipLaunchKernelGGL(k_gershgorin, rows_grid(cur.n), dim3(BLK), 0, s, cur, g);
+            HIPCHK(hipGetLastError());
+            unsigned long long hg = 0;
+            HIPCHK(hipMemcpyAsync(&hg, g, sizeof(hg), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            double gb = 0.0;
+            std::memcpy(&gb, &hg, sizeof(gb));
+            L.omega = (gb > 0.0) ? std::min(A->omega_max, 1.9 / gb) : A->omega_max;
+        }
         // cycle data of this level
         if (l == 0) {
             L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
@@ -531,14 +555,14 @@ template <typename T, int R, int CI>
 int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     Bamg& A = *Q.amg;
     hipStream_t s = Q.stream;
-    const T w = (T)A.omega;
     const size_t nl = A.lv.size();
     BamgLevel& L0 = A.lv[0];
     BamgLevel& L1 = A.lv[1];
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
+    const T w0 = (T)L0.omega;
     hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals,
-                       (const T*)L0.dinv, w, r, (T*)L0.t);
+                       (const T*)L0.dinv, w0, r, (T*)L0.t);
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
@@ -546,6 +570,7 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     };
     hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
+        const T w = (T)A.lv[l].omega;
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_resid<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         else hipLaunchKernelGGL((kc_resid<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
@@ -557,13 +582,21 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     else
         hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
     for (size_t l = nl - 1; l-- > 1;) {
+        const T w = (T)A.lv[l].omega;
         hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w,
                            (const T*)A.lv[l + 1].z);
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         else hipLaunchKernelGGL((kc_post<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
     }
-    hipLaunchKernelGGL((kc_up0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
-                       (const T*)L0.dinv, w, r, (const int32_t*)L0.agg, (const T*)L1.z, z);
+    if (A.split_up) {  // e1 in level 0's work vector (free once restricted), then the post-smoothing
+        hipLaunchKernelGGL((kc_prolong0<T, R>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na,
+                           (const T*)L0.dinv, w0, r, (const int32_t*)L0.agg, (const T*)L1.z, (T*)L0.t);
+        hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
+                           (const T*)L0.dinv, w0, r, (const T*)L0.t, z);
+    } else {
+        hipLaunchKernelGGL((kc_up0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
+                           (const T*)L0.dinv, w0, r, (const int32_t*)L0.agg, (const T*)L1.z, z);
+    }
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }

@@ -29,6 +29,19 @@ __device__ __forceinline__ double block_sum_all(double v, double* sh) {
     return s;
 }
 
+// Maximum over the workgroup; every thread returns the same value.  sh holds WAVES doubles.
+__device__ __forceinline__ double block_max_all(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = sh[0];
+#pragma unroll
+    for (int i = 1; i < WAVES; ++i) m = fmax(m, sh[i]);
+    return m;
+}
+
 // Sum of partials part[a, b) in a fixed order; identical in every workgroup.
 __device__ __forceinline__ double sum_range(const double* __restrict__ part, int64_t a, int64_t b, double* sh) {
     double v = 0.0;
