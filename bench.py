@@ -254,7 +254,7 @@ def run_batch(args):
     log(f"fp64 drop-in per chart: {len(prepared)} charts, {unk_d} unknowns in {el_d:.2f}s -> {value_d:.3e} /s")
 
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel="kb_dir_spmv<float, 2, %d, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1, sched["nontemporal"]),
+                traffic=None, kernel="kb_spmv<float, 2, %d, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1, sched["nontemporal"]),
                 launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
                 us_per_launch=1e6 * ks["seconds"] / ks["launches"],
                 whole_batch_launch={"us": sec * 1e6, "bytes": byt, "GBps": byt / sec / 1e9})
@@ -349,7 +349,7 @@ def run_single(args):
     achieved = ks["bytes"] / ks["seconds"]
     sched = bt.schedule()
     roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK,
-                traffic=None, kernel="kb_dir_spmv<float, 2, %d, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1, sched["nontemporal"]),
+                traffic=None, kernel="kb_spmv<float, 2, %d, %d, 0>" % (0 if sched["col_bytes"] == 2 else 1, sched["nontemporal"]),
                 launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
                 us_per_launch=1e6 * ks["seconds"] / ks["launches"])
     bt.close()

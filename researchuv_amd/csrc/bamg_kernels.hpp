@@ -596,26 +596,62 @@ __device__ __forceinline__ EllRow ell_row(const CycLevel<T>& L, int64_t i) {
     return EllRow{L.eoff[s] + (i - s * SLICE), min((int64_t)SLICE, L.n - s * SLICE), L.ew[s]};
 }
 
+// acc[c] += sum over the ELL row of a level operator of a_ij * g(j)[c], the slot loop unrolled for the slice
+// width (a switch, uniform over the workgroup) as in row_slots
+template <int W, typename T, int R, int CI, typename G>
+__device__ __forceinline__ void ell_slots(const CycLevel<T>& L, int64_t e0, int64_t ns, int64_t i, G g,
+                                          T (&acc)[R]) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int64_t e = e0 + k * ns;
+        const T a = L.evals[e];
+        const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
+    }
+}
+template <typename T, int R, int CI, typename G>
+__device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r, int64_t i, G g, T (&acc)[R]) {
+    switch (r.w) {
+#define RUV_EL(W) \
+    case W: ell_slots<W, T, R, CI>(L, r.e0, r.ns, i, g, acc); break;
+        RUV_EL(1) RUV_EL(2) RUV_EL(3) RUV_EL(4) RUV_EL(5) RUV_EL(6) RUV_EL(7) RUV_EL(8) RUV_EL(9) RUV_EL(10)
+        RUV_EL(11) RUV_EL(12) RUV_EL(13) RUV_EL(14) RUV_EL(15) RUV_EL(16) RUV_EL(17) RUV_EL(18) RUV_EL(19)
+        RUV_EL(20)
+#undef RUV_EL
+        default:
+            for (int k = 0; k < r.w; ++k) {
+                const int64_t e = r.e0 + k * r.ns;
+                const T a = L.evals[e];
+                const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
+            }
+    }
+}
+
 // level >= 1: t = b - A (w D^-1 b)
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= L.n || !live_row(L, i)) return;
     const EllRow e = ell_row(L, i);
-    const Vr<T, R> bi = ld<T, R>(L.b, i);
     T s[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) s[c] = bi.c[c];
-    for (int k = 0; k < e.w; ++k) {
-        const int64_t j = col_of<CI>(L.ecols, e.e0 + k * e.ns, i);
-        const T v = L.evals[e.e0 + k * e.ns] * (w * L.dinv[j]);
-        const Vr<T, R> bj = ld<T, R>(L.b, j);
+    for (int c = 0; c < R; ++c) s[c] = T(0);
+    ell_apply<T, R, CI>(L, e, i,
+                        [&](int64_t j) {
+                            const T d = w * L.dinv[j];
+                            Vr<T, R> x = ld<T, R>(L.b, j);
 #pragma unroll
-        for (int c = 0; c < R; ++c) s[c] = fma(-v, bj.c[c], s[c]);
-    }
+                            for (int c = 0; c < R; ++c) x.c[c] *= d;
+                            return x;
+                        },
+                        s);
+    const Vr<T, R> bi = ld<T, R>(L.b, i);
     Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = s[c];
+    for (int c = 0; c < R; ++c) o.c[c] = bi.c[c] - s[c];
     st<T, R>(L.t, i, o);
 }
 
@@ -662,13 +698,7 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
     T s[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) s[c] = T(0);
-    for (int k = 0; k < e.w; ++k) {
-        const int64_t j = col_of<CI>(L.ecols, e.e0 + k * e.ns, i);
-        const T v = L.evals[e.e0 + k * e.ns];
-        const Vr<T, R> ej = ld<T, R>(L.t, j);
-#pragma unroll
-        for (int c = 0; c < R; ++c) s[c] = fma(v, ej.c[c], s[c]);
-    }
+    ell_apply<T, R, CI>(L, e, i, [&](int64_t j) { return ld<T, R>(L.t, j); }, s);
     const T di = w * L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     const Vr<T, R> ei = ld<T, R>(L.t, i);
@@ -733,25 +763,22 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
-        const int64_t e0 = B.sl_off[s] + threadIdx.x;
-        const int64_t c0 = B.sl_coff[s];
-        const int wd = B.sl_w[s];
-        const Vr<T, R> ri = ld<T, R>(r, i);
         T acc[R];
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc[cc] = ri.c[cc];
-        for (int k = 0; k < wd; ++k) {
-            const T v = vals[e0 + (int64_t)k * n];
-            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
-                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
-            const T dj = w * dinv[j];
-            const Vr<T, R> rj = ld<T, R>(r, j);
+        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
+        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals,
+                               [&](int64_t j) {
+                                   const T d = w * dinv[j];
+                                   Vr<T, R> x = ld<T, R>(r, j);
 #pragma unroll
-            for (int cc = 0; cc < R; ++cc) acc[cc] = fma(-v * dj, rj.c[cc], acc[cc]);
-        }
+                                   for (int cc = 0; cc < R; ++cc) x.c[cc] *= d;
+                                   return x;
+                               },
+                               acc);
+        const Vr<T, R> ri = ld<T, R>(r, i);
         Vr<T, R> o;
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) o.c[cc] = acc[cc];
+        for (int cc = 0; cc < R; ++cc) o.c[cc] = ri.c[cc] - acc[cc];
         st<T, R>(t_out, i, o);
     }
 }
@@ -796,20 +823,10 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
-        const int64_t e0 = B.sl_off[s] + threadIdx.x;
-        const int64_t c0 = B.sl_coff[s];
-        const int wd = B.sl_w[s];
         T acc[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        for (int k = 0; k < wd; ++k) {
-            const T v = vals[e0 + (int64_t)k * n];
-            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
-                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
-            const Vr<T, R> ej = ld<T, R>(e, j);
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, ej.c[cc], acc[cc]);
-        }
+        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals, [&](int64_t j) { return ld<T, R>(e, j); }, acc);
         const T di = w * dinv[i];
         const Vr<T, R> ri = ld<T, R>(r, i);
         const Vr<T, R> ei = ld<T, R>(e, i);
@@ -845,22 +862,19 @@ __global__ __launch_bounds__(BLK) void kc_up0(BDev B, const int* __restrict__ ac
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
-        const int64_t e0 = B.sl_off[s] + threadIdx.x;
-        const int64_t c0 = B.sl_coff[s];
-        const int wd = B.sl_w[s];
         T acc[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        for (int k = 0; k < wd; ++k) {
-            const T v = vals[e0 + (int64_t)k * n];
-            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
-                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
-            const T dj = w * dinv[j];
-            const Vr<T, R> rj = ld<T, R>(r, j);
-            const Vr<T, R> zj = ld<T, R>(z1, agg[j]);
+        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals,
+                               [&](int64_t j) {
+                                   const T d = w * dinv[j];
+                                   const Vr<T, R> rj = ld<T, R>(r, j);
+                                   Vr<T, R> x = ld<T, R>(z1, agg[j]);
 #pragma unroll
-            for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, fma(dj, rj.c[cc], zj.c[cc]), acc[cc]);
-        }
+                                   for (int cc = 0; cc < R; ++cc) x.c[cc] = fma(d, rj.c[cc], x.c[cc]);
+                                   return x;
+                               },
+                               acc);
         const T di = w * dinv[i];
         const Vr<T, R> ri = ld<T, R>(r, i);
         const Vr<T, R> zc = ld<T, R>(z1, agg[i]);

@@ -1,4 +1,4 @@
-// batch.hip -- batched LSCM solves: thousands of independent charts, two launches per PCG step.
+// batch.hip -- batched LSCM solves: thousands of independent charts, three launches per PCG step.
 //
 // The reference solved charts one at a time from worker threads (crates/researchuv-unwrap/src/pipeline.rs:
 // 171-194 -> lscm.rs:471-506 -> solver.rs:284-565), each solve paying its own uploads, host-synchronised dot
@@ -194,7 +194,7 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
         const double sv = mode == RUV_MODE_MIXED ? 4.0 : 8.0, sc = narrow ? 2.0 : 4.0;
         Q.chart_step_bytes.assign(nchart, 0.0);
         for (int c = 0; c < nchart; ++c) {
-            double b = 4.0 * (double)(crow[c + 1] - crow[c]) * R * sv;
+            double b = 2.0 * (double)(crow[c + 1] - crow[c]) * R * sv;  // p' read, q written
             for (int s = Q.chart_s0[c]; s < Q.chart_s0[c + 1]; ++s)
                 b += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + 32.0 +
                      (stencil[s] ? 4.0 * SLOTS : 0.0);
@@ -329,14 +329,14 @@ int set_active(BatchImpl& Q, Pred pred, int* count) {
 inline dim3 grid_of(int na) { return dim3((unsigned)(((na + XCDS - 1) / XCDS) * XCDS)); }
 
 template <typename T, int R, int CI, int PROBE = 0>
-void launch_dir_spmv(BatchImpl& Q, int na, int par, uint32_t iter, const T* z, const T* pold, T* pnew, T* q) {
+void launch_spmv(BatchImpl& Q, int na, const T* p, T* q) {
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
     if (Q.nt)
-        hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, 1, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
-                           (const int*)Q.act, na, par, iter, vals, z, pold, pnew, q);
+        hipLaunchKernelGGL((kb_spmv<T, R, CI, 1, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
+                           (const int*)Q.act, na, vals, p, q);
     else
-        hipLaunchKernelGGL((kb_dir_spmv<T, R, CI, 0, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
-                           (const int*)Q.act, na, par, iter, vals, z, pold, pnew, q);
+        hipLaunchKernelGGL((kb_spmv<T, R, CI, 0, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
+                           (const int*)Q.act, na, vals, p, q);
 }
 
 template <typename T, int R, int PC, int CI>
@@ -376,6 +376,8 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
         if (rc) return rc;
         if (na == 0) break;
         for (int it = 0; it < chunk; ++it) {
+            hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
+                               (const T*)pb[par ^ 1], pb[par]);
             if (Q.ktime) {
                 if (Q.kev_used + 2 > Q.kev.size()) {
                     hipEvent_t e0, e1;
@@ -386,7 +388,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
                 }
                 HIPCHK(hipEventRecord(Q.kev[Q.kev_used], s));
             }
-            launch_dir_spmv<T, R, CI>(Q, na, par, iter, z, pb[par ^ 1], pb[par], q);
+            launch_spmv<T, R, CI>(Q, na, pb[par], q);
             if (Q.ktime) {
                 HIPCHK(hipEventRecord(Q.kev[Q.kev_used + 1], s));
                 Q.kev_used += 2;
@@ -467,7 +469,7 @@ int run_ci(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, doub
     return Q.ci == 0 ? run_pc<T, R, 0>(Q, d_b, d_x, max_iter, tol) : run_pc<T, R, 1>(Q, d_b, d_x, max_iter, tol);
 }
 
-// The step kernel over every tile with never-converging state: mean seconds per launch.
+// The step's SpMV kernel over every tile with never-converging state: mean seconds per launch.
 template <typename T, int R, int CI>
 int time_step_kernel(BatchImpl& Q, int reps, double* sec) {
     hipStream_t s = Q.stream;
@@ -488,11 +490,10 @@ int time_step_kernel(BatchImpl& Q, int reps, double* sec) {
     int rc = set_active(Q, [](int) { return true; }, &na);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(s));
-    launch_dir_spmv<T, R, CI, 1>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+    launch_spmv<T, R, CI, 1>(Q, na, (const T*)Q.pv0, (T*)Q.qv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(Q.ev0, s));
-    for (int i = 0; i < reps; ++i)
-        launch_dir_spmv<T, R, CI, 1>(Q, na, 0, 0, (const T*)Q.zv, (const T*)Q.pv1, (T*)Q.pv0, (T*)Q.qv);
+    for (int i = 0; i < reps; ++i) launch_spmv<T, R, CI, 1>(Q, na, (const T*)Q.pv0, (T*)Q.qv);
     HIPCHK(hipEventRecord(Q.ev1, s));
     HIPCHK(hipEventSynchronize(Q.ev1));
     float ms = 0;
@@ -511,7 +512,7 @@ int time_step(BatchImpl& Q, int reps, double* sec) {
     return Q.ci == 0 ? time_step_kernel<double, 1, 0>(Q, reps, sec) : time_step_kernel<double, 1, 1>(Q, reps, sec);
 }
 
-// Pick slices-per-tile and the value-load hint by timing the step kernel over the whole batch.
+// Pick slices-per-tile and the value-load hint by timing the step's SpMV kernel over the whole batch.
 int autotune(BatchImpl& Q) {
     const char* env = std::getenv("RUV_BATCH_SLICES_PER_TILE");
     if (env) {
@@ -617,9 +618,9 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
     return RUV_OK;
 }
 
-// Mean duration and algorithmic bytes of the step kernel (direction + SpMV) over the whole batch.  Bytes:
-// the sliced-ELL values and column offsets as stored (padding included), the slice table, and four vectors
-// streamed once (z, old p read; new p, q written); neighbour gathers are served by the caches.
+// Mean duration and algorithmic bytes of the step's SpMV kernel (kb_spmv) over the whole batch.  Bytes: the
+// sliced-ELL values and column offsets as stored (padding included), the slice table, and two vectors streamed
+// once (p read, q written); neighbour gathers are served by the caches.
 int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
     if (reps <= 0) return fail(RUV_ERR_INVALID, "reps must be positive");
     int rc = time_step(*Q, reps, sec);

@@ -16,11 +16,12 @@
 //
 // Iteration.  With z = M r (Jacobi) the reference recurrence (solver.rs:405-552)
 //     alpha = rz / pAp;  x += alpha p;  r -= alpha q;  z = M r;  beta = rz' / rz;  p = z + beta p
-// needs two chart-wide reductions per step, so it runs as two kernels:
-//   kb_dir_spmv  opens the step (convergence on ||r||, beta from the r.z partials), forms the new direction
-//                p' = z + beta p for its own rows AND for every neighbour it gathers (from z and the old p, so no
-//                third kernel and no extra pass), q = A p', partials p'.q;
-//   kb_update    alpha = rz / p'.q; x, r, z updated; partials r.r and r.z.
+// needs two chart-wide reductions per step, so it runs as three kernels:
+//   kb_dir     opens the step (convergence on ||r||, beta from the r.z partials), p' = z + beta p (a stream);
+//   kb_spmv    q = A p', partials p'.q;
+//   kb_update  alpha = rz / p'.q; x, r, z updated; partials r.r and r.z.
+// (Forming p' for every gathered neighbour inside the SpMV instead, from z and the old p, saves the direction
+// pass but doubles the gathers; measured slower.)
 // The old and new directions live in two buffers swapped every step, and the per-step scalars (r.r and r.z
 // partials, chart r.z) are double-buffered by step parity, so no workgroup reads a value another workgroup of
 // the same launch writes.  All workgroups of a chart sum the same partials in the same order, so they take the
@@ -131,28 +132,80 @@ __device__ __forceinline__ void sum_range2(const double* __restrict__ pa, const 
     *sb = s1;
 }
 
-// the new search direction of one row; used for own rows and for gathered neighbours alike, so both see
-// bit-identical values
-template <typename T>
-__device__ __forceinline__ T dir(T z, T p, T beta) {
-    return fma(beta, p, z);
-}
-
-// Opens step `iter` (parity par) of every live chart, then p' = z + beta p and q = A p' over one tile.
 // NT = 1 loads the matrix values with the non-temporal hint (they are read once per step; the gathered vectors
-// are the data worth keeping in L2); the autotuner picks it or not.  PROBE = 1 is the same code under its own
-// symbol for the schedule autotuner and the measurement hook, so a kernel trace of the solves is not mixed with
-// whole-batch timing launches.
+// are the data worth keeping in L2); the autotuner picks it or not.
 template <typename T>
 __device__ __forceinline__ T ld_val(const T* __restrict__ p, int nt) {
     return nt ? __builtin_nontemporal_load(p) : *p;
 }
 
-template <typename T, int R, int CI, int NT, int PROBE = 0>
-__global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict__ act, int na, int par,
-                                                   uint32_t iter, const T* __restrict__ vals,
-                                                   const T* __restrict__ z, const T* __restrict__ pold,
-                                                   T* __restrict__ pnew, T* __restrict__ q) {
+// Row i of slice s (lane threadIdx.x): acc[c] += sum over the row's slots of a_ij * g(j)[c].  The slot loop is
+// unrolled for the slice's width through a switch (the width is uniform over the workgroup): a loop of run-time
+// length issues its loads a few at a time, and the kernels built on it measured 20-35 % slower.
+template <int W, typename T, int R, int NT, typename G>
+__device__ __forceinline__ void stencil_slots(const T* __restrict__ v, int n, const int32_t* __restrict__ dt,
+                                              int64_t i, int64_t lo, int64_t hi, G g, T (&acc)[R]) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const T a = ld_val(v + k * n, NT);
+        const Vr<T, R> x = g(min(max(i + (int64_t)dt[k], lo), hi));
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
+    }
+}
+template <int W, typename T, int R, int CI, int NT, typename G>
+__device__ __forceinline__ void general_slots(const T* __restrict__ v, int n, const void* __restrict__ ec, int64_t f0,
+                                              int64_t i, G g, T (&acc)[R]) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const T a = ld_val(v + k * n, NT);
+        const Vr<T, R> x = g(col_of<CI>(ec, f0 + k * n, i));
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
+    }
+}
+constexpr int UNROLL_W = 16;  // general slices wider than this take a plain loop
+
+template <typename T, int R, int CI, int NT, typename G>
+__device__ __forceinline__ void row_slots(const BDev& B, int s, int n, int64_t i, int64_t lo, int64_t hi,
+                                          const T* __restrict__ vals, G g, T (&acc)[R]) {
+    const T* v = vals + B.sl_off[s] + threadIdx.x;
+    const int64_t c0 = B.sl_coff[s];
+    const int w = B.sl_w[s];
+    if (c0 < 0) {
+        const int32_t* dt = B.sl_dtab + (int64_t)s * SLOTS;
+        switch (w) {
+#define RUV_ST(W) \
+    case W: stencil_slots<W, T, R, NT>(v, n, dt, i, lo, hi, g, acc); break;
+            RUV_ST(1) RUV_ST(2) RUV_ST(3) RUV_ST(4) RUV_ST(5) RUV_ST(6) RUV_ST(7) RUV_ST(8)
+#undef RUV_ST
+            default: break;
+        }
+    } else {
+        const int64_t f0 = c0 + threadIdx.x;
+        switch (w) {
+#define RUV_GS(W) \
+    case W: general_slots<W, T, R, CI, NT>(v, n, B.ecol, f0, i, g, acc); break;
+            RUV_GS(1) RUV_GS(2) RUV_GS(3) RUV_GS(4) RUV_GS(5) RUV_GS(6) RUV_GS(7) RUV_GS(8)
+            RUV_GS(9) RUV_GS(10) RUV_GS(11) RUV_GS(12) RUV_GS(13) RUV_GS(14) RUV_GS(15) RUV_GS(16)
+#undef RUV_GS
+            default:
+                for (int k = 0; k < w; ++k) {
+                    const T a = ld_val(v + k * n, NT);
+                    const Vr<T, R> x = g(col_of<CI>(B.ecol, f0 + (int64_t)k * n, i));
+#pragma unroll
+                    for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
+                }
+        }
+    }
+}
+
+// Opens step `iter` (parity par) of every live chart -- convergence on ||r||, max_iter, breakdown, beta from the
+// r.z partials -- then p' = z + beta p over the tile's rows.
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ act, int na, int par, uint32_t iter,
+                                              const T* __restrict__ z, const T* __restrict__ pold,
+                                              T* __restrict__ pnew) {
     __shared__ double sh[2 * WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
@@ -181,6 +234,29 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
         B.c_iters[c] = iter + 1;
     }
     const T beta = (T)(rz / rzold);
+    const int64_t r1 = B.tile_r1[t];
+    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
+        const Vr<T, R> zi = ld<T, R>(z, i), pi = ld<T, R>(pold, i);
+        Vr<T, R> pn;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) pn.c[cc] = fma(beta, pi.c[cc], zi.c[cc]);
+        st<T, R>(pnew, i, pn);
+    }
+}
+
+// q = A p' over one tile of a running chart, partial p'.q of the tile.  PROBE = 1 is the same code under its own
+// symbol for the schedule autotuner and the measurement hook, so a kernel trace of the solves is not mixed with
+// whole-batch timing launches.
+template <typename T, int R, int CI, int NT, int PROBE = 0>
+__global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ act, int na,
+                                               const T* __restrict__ vals, const T* __restrict__ p,
+                                               T* __restrict__ q) {
+    __shared__ double sh[WAVES];
+    const int slot = xcd_slot(na);
+    if (slot < 0) return;
+    const int t = act[slot];
+    const int c = B.tile_chart[t];
+    if (B.c_done[c] != B_RUNNING) return;
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     double acc = 0.0;
     const int s1 = B.tile_s0[t + 1];
@@ -188,49 +264,21 @@ __global__ __launch_bounds__(BLK) void kb_dir_spmv(BDev B, const int* __restrict
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
-        const int64_t e0 = B.sl_off[s] + threadIdx.x;
-        const int64_t c0 = B.sl_coff[s];
-        const int w = B.sl_w[s];
-        T acc_q[R];
+        T aq[R];
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc_q[cc] = T(0);
-        if (c0 < 0) {
-            const int32_t* dt = B.sl_dtab + (int64_t)s * SLOTS;
-#pragma unroll 4
-            for (int k = 0; k < w; ++k) {
-                const T v = ld_val(vals + e0 + (int64_t)k * n, NT);
-                const int64_t j = min(max(i + dt[k], lo), hi);
-                const Vr<T, R> zj = ld<T, R>(z, j);
-                const Vr<T, R> pj = ld<T, R>(pold, j);
-#pragma unroll
-                for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
-            }
-        } else {
-            const int64_t f0 = c0 + threadIdx.x;
-#pragma unroll 4
-            for (int k = 0; k < w; ++k) {
-                const T v = ld_val(vals + e0 + (int64_t)k * n, NT);
-                const int64_t j = col_of<CI>(B.ecol, f0 + (int64_t)k * n, i);
-                const Vr<T, R> zj = ld<T, R>(z, j);
-                const Vr<T, R> pj = ld<T, R>(pold, j);
-#pragma unroll
-                for (int cc = 0; cc < R; ++cc) acc_q[cc] = fma(v, dir(zj.c[cc], pj.c[cc], beta), acc_q[cc]);
-            }
-        }
-        const Vr<T, R> zi = ld<T, R>(z, i);
-        const Vr<T, R> pi = ld<T, R>(pold, i);
-        Vr<T, R> pn, qv;
+        for (int cc = 0; cc < R; ++cc) aq[cc] = T(0);
+        row_slots<T, R, CI, NT>(B, s, n, i, lo, hi, vals, [&](int64_t j) { return ld<T, R>(p, j); }, aq);
+        const Vr<T, R> pi = ld<T, R>(p, i);
+        Vr<T, R> qv;
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) {
-            pn.c[cc] = dir(zi.c[cc], pi.c[cc], beta);
-            qv.c[cc] = acc_q[cc];
-            acc += (double)pn.c[cc] * (double)acc_q[cc];
+            qv.c[cc] = aq[cc];
+            acc += (double)pi.c[cc] * (double)aq[cc];
         }
-        st<T, R>(pnew, i, pn);
         st<T, R>(q, i, qv);
     }
-    const double s = block_sum_all(acc, sh);
-    if (threadIdx.x == 0) B.part_a[t] = s;
+    const double sum = block_sum_all(acc, sh);
+    if (threadIdx.x == 0) B.part_a[t] = sum;
 }
 
 // alpha = rz / p.q;  x += alpha p;  r -= alpha q;  z = D^-1 r (PC 1; z aliases r for PC 0; PC 2 leaves z and
@@ -413,20 +461,11 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
-        const int64_t e0 = B.sl_off[s] + threadIdx.x;
-        const int64_t c0 = B.sl_coff[s];
-        const int w = B.sl_w[s];
         double sum[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) sum[cc] = 0.0;
-        for (int k = 0; k < w; ++k) {
-            const double v = vals64[e0 + (int64_t)k * n];
-            const int64_t j = (c0 < 0) ? min(max(i + B.sl_dtab[(int64_t)s * SLOTS + k], lo), hi)
-                                       : col_of<CI>(B.ecol, c0 + threadIdx.x + (int64_t)k * n, i);
-            const Vr<double, R> xj = ld<double, R>(x64, j);
-#pragma unroll
-            for (int cc = 0; cc < R; ++cc) sum[cc] = fma(v, xj.c[cc], sum[cc]);
-        }
+        row_slots<double, R, CI, 0>(B, s, n, i, lo, hi, vals64, [&](int64_t j) { return ld<double, R>(x64, j); },
+                                    sum);
         const Vr<double, R> bi = ld<double, R>(b, i);
         Vr<double, R> ri;
 #pragma unroll

@@ -2,7 +2,7 @@
 
 Usage (from the repository root, after the two profiled runs of the same command):
 
-    python -m researchuv_amd.traffic --fetch DIR_A --write DIR_B --kernel kb_dir_spmv<float, 2, 0, 0> \
+    python -m researchuv_amd.traffic --fetch DIR_A --write DIR_B --kernel 'kb_spmv<float, 2, 0, 0, 0>' \
         --copy-kernel copyBuffer --copy-bytes 1073741824 --algorithmic BYTES_PER_LAUNCH --out FILE.json
 
 FETCH_SIZE and WRITE_SIZE are read in KiB per dispatch and averaged over the dispatches whose name contains
