@@ -129,15 +129,25 @@ __device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, i
 }
 
 // ranking and the first round's proposals (every row is unmatched then)
+// Also the level's Gershgorin bound on D^-1 A, max over rows of sum_j |a_ij| / a_ii, into *gmax (the bits of a
+// non-negative double order like the double).
 __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, const int32_t* __restrict__ group, double theta,
                                                     int32_t* __restrict__ cand, int32_t* __restrict__ ncand,
-                                                    int64_t* __restrict__ prop) {
+                                                    int64_t* __restrict__ prop, unsigned long long* __restrict__ gmax) {
+    __shared__ double sh[WAVES];
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= A.n) return;
-    const int64_t e0 = A.ptr[i], e1 = A.ptr[i + 1];
-    double smax = 0.0;
-    for (int64_t e = e0; e < e1; ++e)
-        if (A.cols[e] != i && A.vals[e] < 0.0) smax = fmax(smax, -A.vals[e]);
+    const bool in = i < A.n;
+    const int64_t e0 = in ? A.ptr[i] : 0, e1 = in ? A.ptr[i + 1] : 0;
+    double smax = 0.0, asum = 0.0, d = 0.0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const double v = A.vals[e];
+        asum += fabs(v);
+        if (A.cols[e] == i) d += v;
+        else if (v < 0.0) smax = fmax(smax, -v);
+    }
+    const double gb = block_max_all((d > 0.0) ? asum / d : 0.0, sh);
+    if (threadIdx.x == 0) atomicMax(gmax, (unsigned long long)__double_as_longlong(gb));
+    if (!in) return;
     const double thr = theta * smax;
     const int32_t g = group[i];
     uint64_t kp[MCAND];
@@ -458,22 +468,18 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, co
     cdiag[I] = diag;
 }
 
-// Gershgorin bound of the Jacobi-scaled operator, max over rows of sum_j |a_ij| / a_ii, into *g (as the bits of
-// a non-negative double, which order like the double)
-__global__ __launch_bounds__(BLK) void k_gershgorin(SetupCsr A, unsigned long long* __restrict__ g) {
-    __shared__ double sh[WAVES];
+// first and last chart of every BLK rows of a level
+__global__ __launch_bounds__(BLK) void k_block_charts(int64_t n, const int32_t* __restrict__ chart,
+                                                      int2* __restrict__ blk) {
+    const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (b * BLK >= n) return;
+    blk[b] = make_int2(chart[b * BLK], chart[min(n, (b + 1) * BLK) - 1]);
+}
+
+// int64 -> int32 (row indices of one level, below 2^31)
+__global__ __launch_bounds__(BLK) void k_narrow(int64_t n, const int64_t* __restrict__ in, int32_t* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    double q = 0.0;
-    if (i < A.n) {
-        double d = 0.0, sum = 0.0;
-        for (int64_t e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
-            sum += fabs(A.vals[e]);
-            if (A.cols[e] == i) d += A.vals[e];
-        }
-        q = (d > 0.0) ? sum / d : 0.0;
-    }
-    const double m = block_max_all(q, sh);
-    if (threadIdx.x == 0) atomicMax(g, (unsigned long long)__double_as_longlong(m));
+    if (i < n) out[i] = (int32_t)in[i];
 }
 
 // sliced ELL of a coarse operator: slice widths, then the fill (padding: value 0, the row itself)
@@ -569,20 +575,30 @@ struct CycLevel {
     const T* evals;
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
-    const int64_t* mptr;     // next level's aggregates -> member rows here
-    const int64_t* mrows;
+    const int32_t* mptr;     // next level's aggregates -> member rows here
+    const int32_t* mrows;
     T* b;                    // right-hand side on this level
     T* z;                    // correction on this level
     T* t;                    // work vector on this level
     const int32_t* chart;    // row -> chart
+    const int2* blk;         // per BLK rows: first and last chart
     const int32_t* done;     // per chart: B_RUNNING while its solve runs
 };
 
 // Rows of charts whose solve has stopped are skipped on every level: a chart's rows only couple to rows of the
 // same chart, and the running charts never read what a stopped chart's rows hold.
+// Decided per workgroup of BLK rows from its first and last chart (charts are contiguous on every level): all
+// stopped (the workgroup leaves), all running (no per-row test), or mixed (each row tests its own chart).
+__device__ __forceinline__ int block_state(const int2* __restrict__ blk, const int32_t* __restrict__ done) {
+    const int2 b = blk[blockIdx.x];
+    int run = 0;
+    for (int c = b.x; c <= b.y; ++c) run += (done[c] == B_RUNNING) ? 1 : 0;
+    return run == 0 ? 0 : (run == b.y - b.x + 1 ? 1 : 2);
+}
 template <typename T>
 __device__ __forceinline__ bool live_row(const CycLevel<T>& L, int64_t i) {
-    return L.done[L.chart[i]] == B_RUNNING;
+    const int bs = block_state(L.blk, L.done);
+    return bs == 1 || (bs == 2 && L.done[L.chart[i]] == B_RUNNING);
 }
 
 // slice, lane, slice height of row i of a level's sliced ELL
@@ -658,13 +674,16 @@ __global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
 // per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, const int32_t* __restrict__ cnext,
-                                                   T* __restrict__ bnext) {
+                                                   const int2* __restrict__ bnext_blk, T* __restrict__ bnext) {
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (I >= nc || L.done[cnext[I]] != B_RUNNING) return;
+    if (I >= nc) return;
+    const int bs = block_state(bnext_blk, L.done);
+    if (bs == 0 || (bs == 2 && L.done[cnext[I]] != B_RUNNING)) return;
     T acc[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) acc[c] = T(0);
-    for (int64_t q = L.mptr[I]; q < L.mptr[I + 1]; ++q) {
+    const int32_t q1 = L.mptr[I + 1];
+    for (int32_t q = L.mptr[I]; q < q1; ++q) {
         const Vr<T, R> tm = ld<T, R>(L.t, L.mrows[q]);
 #pragma unroll
         for (int c = 0; c < R; ++c) acc[c] += tm.c[c];

@@ -23,12 +23,13 @@ struct BamgLevel {
     void* evals = nullptr;    // T
     void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal)
     int32_t* agg = nullptr;   // row -> next-level aggregate
-    int64_t* mptr = nullptr;  // next level's aggregates -> member rows here
-    int64_t* mrows = nullptr;
+    int32_t* mptr = nullptr;  // next level's aggregates -> member rows here
+    int32_t* mrows = nullptr;
     void* b = nullptr;
     void* z = nullptr;
     void* t = nullptr;
     const int32_t* chart = nullptr;  // row -> chart
+    int2* blk = nullptr;             // per BLK rows: first and last chart
     double omega = 2.0 / 3.0;        // Jacobi weight of this level's smoothing steps
 };
 
@@ -136,6 +137,7 @@ struct Matched {
     int32_t* group = nullptr;  // of the aggregates
     SetupCsr C{};
     double* cdiag = nullptr;
+    double gersh = 0.0;  // Gershgorin bound on D^-1 A of the matched operator
 };
 
 int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, double theta, int rounds,
@@ -143,6 +145,9 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     const int64_t n = A.n;
     int32_t *cand = nullptr, *ncand = nullptr;
     int64_t *prop = nullptr, *flag = nullptr, *mate = nullptr, *join = nullptr;
+    unsigned long long* gmax = nullptr;
+    HIPCHK(S.alloc(&gmax, 1));
+    HIPCHK(hipMemsetAsync(gmax, 0, sizeof(unsigned long long), s));
     HIPCHK(S.alloc(&cand, (size_t)n * MCAND));
     HIPCHK(S.alloc(&ncand, n));
     HIPCHK(S.alloc(&prop, n));
@@ -152,7 +157,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&M->cid, n + 1));
     HIPCHK(S.alloc(&M->agg, n));
     HIPCHK(hipMemsetAsync(mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
-    hipLaunchKernelGGL(k_candidates, rows_grid(n), dim3(BLK), 0, s, A, group, theta, cand, ncand, prop);
+    hipLaunchKernelGGL(k_candidates, rows_grid(n), dim3(BLK), 0, s, A, group, theta, cand, ncand, prop, gmax);
     for (int r = 0; r < rounds; ++r) {
         if (r > 0)
             hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)cand,
@@ -165,8 +170,11 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(hipGetLastError());
     int rc = scan(s, n, flag, M->cid, S);
     if (rc) return rc;
+    unsigned long long hg = 0;
     HIPCHK(hipMemcpyAsync(&M->nc, M->cid + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hg, gmax, sizeof(hg), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    std::memcpy(&M->gersh, &hg, sizeof(double));
     const int64_t nc = M->nc;
     hipLaunchKernelGGL(k_root_agg, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join,
                        (const int64_t*)M->cid, M->agg);
@@ -386,18 +394,10 @@ int bamg_build(BatchImpl& Q) {
         L.n = cur.n;
         L.chart = group;
         {
-            unsigned long long* g = nullptr;
-            Scratch T(&Q.pool);
-            HIPCHK(T.alloc(&g, 1));
-            HIPCHK(hipMemsetAsync(g, 0, sizeof(unsigned long long), s));
-            hipLaunchKernelGGL(k_gershgorin, rows_grid(cur.n), dim3(BLK), 0, s, cur, g);
+            const int64_t nb = (cur.n + BLK - 1) / BLK;
+            HIPCHK(A->alloc(&L.blk, nb));
+            hipLaunchKernelGGL(k_block_charts, rows_grid(nb), dim3(BLK), 0, s, cur.n, group, L.blk);
             HIPCHK(hipGetLastError());
-            unsigned long long hg = 0;
-            HIPCHK(hipMemcpyAsync(&hg, g, sizeof(hg), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            double gb = 0.0;
-            std::memcpy(&gb, &hg, sizeof(gb));
-            L.omega = (gb > 0.0) ? std::min(A->omega_max, 1.9 / gb) : A->omega_max;
         }
         // cycle data of this level
         if (l == 0) {
@@ -476,25 +476,34 @@ int bamg_build(BatchImpl& Q) {
             if (rc) return rc;
         }
         const Matched& MF = (passes > 1) ? M2 : M1;
+        L.omega = (M1.gersh > 0.0) ? std::min(A->omega_max, 1.9 / M1.gersh) : A->omega_max;
         if (l > 0 && MF.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
             A->lv.push_back(L);
             break;
         }
         Scratch& SF = (passes > 1) ? *S2 : *S1;
+        const int64_t* mptr64 = M1.mptr;
+        const int64_t* mrows64 = M1.mrows;
         if (passes > 1) {
             HIPCHK(A->alloc(&L.agg, cur.n));
             hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
                                (const int32_t*)M2.agg, L.agg);
             HIPCHK(hipGetLastError());
-            rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *A, &L.mptr, &L.mrows,
-                               (int32_t*)nullptr);
+            int64_t *p64 = nullptr, *r64 = nullptr;
+            rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *S2, &p64, &r64, (int32_t*)nullptr);
             if (rc) return rc;
+            mptr64 = p64;
+            mrows64 = r64;
         } else {  // the matching's own aggregate map and member lists
-            for (void* p : {(void*)M1.agg, (void*)M1.mptr, (void*)M1.mrows}) hand_over(*S1, p, A->mem);
+            hand_over(*S1, M1.agg, A->mem);
             L.agg = M1.agg;
-            L.mptr = M1.mptr;
-            L.mrows = M1.mrows;
         }
+        // the cycle's member lists in 32 bits (a level's rows are below 2^31)
+        HIPCHK(A->alloc(&L.mptr, MF.nc + 1));
+        HIPCHK(A->alloc(&L.mrows, cur.n));
+        hipLaunchKernelGGL(k_narrow, rows_grid(MF.nc + 1), dim3(BLK), 0, s, MF.nc + 1, mptr64, L.mptr);
+        hipLaunchKernelGGL(k_narrow, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, mrows64, L.mrows);
+        HIPCHK(hipGetLastError());
         // chart offsets on the next level
         {
             std::vector<int64_t> at(crow.begin(), crow.end());
@@ -566,15 +575,16 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
-                           (T*)L.b, (T*)L.z, (T*)L.t, L.chart, Q.B.c_done};
+                           (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done};
     };
-    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart, (T*)L1.b);
+    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
+                       (const int2*)L1.blk, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
         const T w = (T)A.lv[l].omega;
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_resid<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         else hipLaunchKernelGGL((kc_resid<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
-                           A.lv[l + 1].chart, (T*)A.lv[l + 1].b);
+                           A.lv[l + 1].chart, (const int2*)A.lv[l + 1].blk, (T*)A.lv[l + 1].b);
     }
     if (A.cinv)
         hipLaunchKernelGGL((kc_coarsest_dense<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1),
