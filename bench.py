@@ -259,8 +259,10 @@ def run_batch(args):
                 us_per_launch=1e6 * ks["seconds"] / ks["launches"],
                 whole_batch_launch={"us": sec * 1e6, "bytes": byt, "GBps": byt / sec / 1e9})
     tf = os.path.join(ROOT, "profiles", f"traffic_batch_{args.batch_precond.lower()}.json")
-    if os.path.exists(tf):
-        roof["traffic"] = json.load(open(tf)).get("hbm_bytes_per_launch")
+    if os.path.exists(tf):  # counter-derived HBM bytes per launch, when profiled for this same kernel
+        tj = json.load(open(tf))
+        if tj.get("kernel") == roof["kernel"]:
+            roof["traffic"] = tj.get("hbm_bytes_per_launch")
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
         config={"workload": f"batch: {lb.nchart} uneven height-field patch charts (64-65536 vertices, log-uniform), "

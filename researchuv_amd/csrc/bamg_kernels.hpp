@@ -103,6 +103,30 @@ struct SetupCsr {
     const double* vals;
 };
 
+// Coarse-level operator values are stored as bfloat16 under an FP32 cycle (half the bytes of the level sweeps,
+// 8 significant bits: the operator stays symmetric, since a_ij and a_ji round alike, and the cycle a fixed SPD
+// preconditioner) and as double under an FP64 cycle.
+struct Bf16 {
+    uint16_t bits;
+};
+template <typename T>
+struct CoarseVal {
+    using type = T;
+};
+template <>
+struct CoarseVal<float> {
+    using type = Bf16;
+};
+__device__ __forceinline__ float cval(Bf16 v) { return __uint_as_float((uint32_t)v.bits << 16); }
+__device__ __forceinline__ double cval(double v) { return v; }
+// double -> stored value: round to nearest, ties to even, through FP32 for bfloat16
+__device__ __forceinline__ void cput(Bf16* p, double v) {
+    const uint32_t u = __float_as_uint((float)v);
+    p->bits = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ void cput(double* p, double v) { *p = v; }
+
+
 // Preference of the coupling (i, j): its strength in bins of a quarter octave, then a hash of the unordered
 // pair.  Couplings that vary smoothly along a line (uniform or geometric spacing) fall into one bin, where the
 // hash -- the same seen from either end -- acts as a random priority; ordering by exact strength or by index
@@ -512,7 +536,8 @@ __global__ __launch_bounds__(BLK) void k_offsets_wide(int64_t n, const int64_t* 
 // CI 0: 16-bit offsets from the row; CI 1: 32-bit columns
 template <typename T, int CI>
 __global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const int64_t* __restrict__ off,
-                                                  const int32_t* __restrict__ w, T* __restrict__ ev,
+                                                  const int32_t* __restrict__ w,
+                                                  typename CoarseVal<T>::type* __restrict__ ev,
                                                   void* __restrict__ ec) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
@@ -521,12 +546,12 @@ __global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const i
     const int64_t e0 = off[s] + lane;
     int k = 0;
     for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
-        ev[e0 + k * ns] = (T)A.vals[a];
+        cput(ev + e0 + k * ns, A.vals[a]);
         if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)((int64_t)A.cols[a] - i);
         else static_cast<int32_t*>(ec)[e0 + k * ns] = A.cols[a];
     }
     for (; k < w[s]; ++k) {
-        ev[e0 + k * ns] = T(0);
+        cput(ev + e0 + k * ns, 0.0);
         if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = 0;
         else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)i;
     }
@@ -572,7 +597,7 @@ struct CycLevel {
     const int64_t* eoff;     // sliced ELL of the level operator (levels >= 1)
     const int32_t* ew;
     const void* ecols;       // int16 offsets (CI 0) or int32 columns (CI 1)
-    const T* evals;
+    const typename CoarseVal<T>::type* evals;
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
     const int32_t* mptr;     // next level's aggregates -> member rows here
@@ -620,7 +645,7 @@ __device__ __forceinline__ void ell_slots(const CycLevel<T>& L, int64_t e0, int6
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const int64_t e = e0 + k * ns;
-        const T a = L.evals[e];
+        const T a = (T)cval(L.evals[e]);
         const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
@@ -638,7 +663,7 @@ __device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r,
         default:
             for (int k = 0; k < r.w; ++k) {
                 const int64_t e = r.e0 + k * r.ns;
-                const T a = L.evals[e];
+                const T a = (T)cval(L.evals[e]);
                 const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
 #pragma unroll
                 for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);

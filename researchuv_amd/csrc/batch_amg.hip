@@ -431,16 +431,17 @@ int bamg_build(BatchImpl& Q) {
             char *v = nullptr, *d = nullptr, *ec = nullptr;
             HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
             L.ecols = ec;
-            HIPCHK(A->alloc(&v, L.nell * sv));
+            HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
             HIPCHK(A->alloc(&d, cur.n * sv));
             auto fill = [&](auto tv) {
                 using TT = decltype(tv);
+                using VT = typename CoarseVal<TT>::type;
                 if (L.ci)
                     hipLaunchKernelGGL((k_ell_fill<TT, 1>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (TT*)v, L.ecols);
+                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
                 else
                     hipLaunchKernelGGL((k_ell_fill<TT, 0>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (TT*)v, L.ecols);
+                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
                 hipLaunchKernelGGL((k_cast_inv<TT>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (TT*)d);
             };
             if (f32) fill(0.0f);
@@ -574,7 +575,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
                        (const T*)L0.dinv, w0, r, (T*)L0.t);
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
-        return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const T*)L.evals, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
+        return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const typename CoarseVal<T>::type*)L.evals,
+                           (const T*)L.dinv, L.agg, L.mptr, L.mrows,
                            (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done};
     };
     hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,

@@ -50,6 +50,8 @@ def main(argv=None):
     ap.add_argument("--copy-bytes", type=float, required=True)
     ap.add_argument("--algorithmic", type=float, default=None, help="algorithmic bytes per launch, for the ratio")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--table", nargs="*", default=[],
+                    help="also print read / write MB per dispatch of the kernels whose names contain these")
     a = ap.parse_args(argv)
     f_k, n_f = mean_counter(a.fetch, "FETCH_SIZE", a.kernel)
     w_k, n_w = mean_counter(a.write, "WRITE_SIZE", a.kernel)
@@ -70,6 +72,13 @@ def main(argv=None):
         json.dump(out, fh, indent=1)
         fh.write("\n")
     print(json.dumps(out))
+    if a.table:
+        print("| Kernel | Dispatches | Read (MB) | Write (MB) |")
+        print("|---|---:|---:|---:|")
+        for k in a.table:
+            fk, nk = mean_counter(a.fetch, "FETCH_SIZE", k)
+            wk, _ = mean_counter(a.write, "WRITE_SIZE", k)
+            print(f"| `{k}` | {nk} | {fk * 1024.0 * sf / 1e6:.1f} | {wk * 1024.0 * sw / 1e6:.1f} |")
 
 
 if __name__ == "__main__":
