@@ -103,12 +103,10 @@ struct SetupCsr {
     const double* vals;
 };
 
-// Coarse-level operator values are stored as bfloat16 under an FP32 cycle (half the bytes of the level sweeps,
-// 8 significant bits: the operator stays symmetric, since a_ij and a_ji round alike, and the cycle a fixed SPD
-// preconditioner) and as double under an FP64 cycle.
-struct Bf16 {
-    uint16_t bits;
-};
+// The cycle's operator values (coarse levels, and the finest level's copy for the smoothing legs) are stored as
+// bfloat16 under an FP32 cycle (half the bytes of the level sweeps, 8 significant bits: the operators stay
+// symmetric, since a_ij and a_ji round alike, and the cycle a fixed SPD preconditioner) and as double under an
+// FP64 cycle.
 template <typename T>
 struct CoarseVal {
     using type = T;
@@ -117,8 +115,6 @@ template <>
 struct CoarseVal<float> {
     using type = Bf16;
 };
-__device__ __forceinline__ float cval(Bf16 v) { return __uint_as_float((uint32_t)v.bits << 16); }
-__device__ __forceinline__ double cval(double v) { return v; }
 // double -> stored value: round to nearest, ties to even, through FP32 for bfloat16
 __device__ __forceinline__ void cput(Bf16* p, double v) {
     const uint32_t u = __float_as_uint((float)v);
@@ -500,6 +496,12 @@ __global__ __launch_bounds__(BLK) void k_block_charts(int64_t n, const int32_t* 
     blk[b] = make_int2(chart[b * BLK], chart[min(n, (b + 1) * BLK) - 1]);
 }
 
+// the finest operator's values for the cycle's smoothing legs
+__global__ __launch_bounds__(BLK) void k_to_bf16(int64_t n, const float* __restrict__ in, Bf16* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) cput(out + i, (double)in[i]);
+}
+
 // int64 -> int32 (row indices of one level, below 2^31)
 __global__ __launch_bounds__(BLK) void k_narrow(int64_t n, const int64_t* __restrict__ in, int32_t* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -794,7 +796,8 @@ __global__ __launch_bounds__(BLK) void kc_coarsest_dense(CycLevel<T> L, const in
 //   down: t = r - A (w D^-1 r) for the tile's rows; kc_restrict then sums t over each aggregate.
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__ act, int na,
-                                                 const T* __restrict__ vals, const T* __restrict__ dinv, T w,
+                                                 const typename CoarseVal<T>::type* __restrict__ vals,
+                                                 const T* __restrict__ dinv, T w,
                                                  const T* __restrict__ r, T* __restrict__ t_out) {
     const int slot = xcd_slot(na);
     if (slot < 0) return;
@@ -852,7 +855,8 @@ __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict
 
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ act, int na, int par,
-                                                const T* __restrict__ vals, const T* __restrict__ dinv, T w,
+                                                const typename CoarseVal<T>::type* __restrict__ vals,
+                                                const T* __restrict__ dinv, T w,
                                                 const T* __restrict__ r, const T* __restrict__ e, T* __restrict__ z) {
     __shared__ double sh[WAVES];
     const int slot = xcd_slot(na);
@@ -890,7 +894,8 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
 //       part_c[par] (the r.z the next step opens with).
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_up0(BDev B, const int* __restrict__ act, int na, int par,
-                                              const T* __restrict__ vals, const T* __restrict__ dinv, T w,
+                                              const typename CoarseVal<T>::type* __restrict__ vals,
+                                              const T* __restrict__ dinv, T w,
                                               const T* __restrict__ r, const int32_t* __restrict__ agg,
                                               const T* __restrict__ z1, T* __restrict__ z) {
     __shared__ double sh[WAVES];

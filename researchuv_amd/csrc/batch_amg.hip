@@ -35,6 +35,7 @@ struct BamgLevel {
 
 struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
+    void* ev0 = nullptr;        // the finest operator's values for the smoothing legs (CoarseVal<T>::type)
     void* cinv = nullptr;       // T: each chart's inverted block of the coarsest operator (null: Jacobi there)
     int64_t* cinv_off = nullptr;
     int64_t* crow_c = nullptr;  // chart row offsets on the coarsest level
@@ -402,6 +403,15 @@ int bamg_build(BatchImpl& Q) {
         // cycle data of this level
         if (l == 0) {
             L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
+            if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build)
+                Bf16* e16 = nullptr;
+                HIPCHK(A->alloc(&e16, (size_t)Q.nell));
+                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
+                HIPCHK(hipGetLastError());
+                A->ev0 = e16;
+            } else {
+                A->ev0 = Q.ev64;
+            }
             char* t = nullptr;
             HIPCHK(A->alloc(&t, cur.n * R * sv));
             HIPCHK(hipMemsetAsync(t, 0, cur.n * R * sv, s));  // rows of converged charts keep finite values
@@ -568,7 +578,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     const size_t nl = A.lv.size();
     BamgLevel& L0 = A.lv[0];
     BamgLevel& L1 = A.lv[1];
-    const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
+    using VT = typename CoarseVal<T>::type;
+    const VT* vals = (const VT*)A.ev0;
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
     const T w0 = (T)L0.omega;
     hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals,

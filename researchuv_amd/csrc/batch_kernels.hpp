@@ -139,26 +139,35 @@ __device__ __forceinline__ T ld_val(const T* __restrict__ p, int nt) {
     return nt ? __builtin_nontemporal_load(p) : *p;
 }
 
+// bfloat16 matrix values (the AMG cycle's copies of the operators under FP32): the upper 16 bits of an FP32
+struct Bf16 {
+    uint16_t bits;
+};
+__device__ __forceinline__ float cval(Bf16 v) { return __uint_as_float((uint32_t)v.bits << 16); }
+__device__ __forceinline__ double cval(double v) { return v; }
+__device__ __forceinline__ float cval(float v) { return v; }
+__device__ __forceinline__ float ld_val(const Bf16* __restrict__ p, int) { return cval(*p); }
+
 // Row i of slice s (lane threadIdx.x): acc[c] += sum over the row's slots of a_ij * g(j)[c].  The slot loop is
 // unrolled for the slice's width through a switch (the width is uniform over the workgroup): a loop of run-time
 // length issues its loads a few at a time, and the kernels built on it measured 20-35 % slower.
-template <int W, typename T, int R, int NT, typename G>
-__device__ __forceinline__ void stencil_slots(const T* __restrict__ v, int n, const int32_t* __restrict__ dt,
+template <int W, typename T, int R, int NT, typename V, typename G>
+__device__ __forceinline__ void stencil_slots(const V* __restrict__ v, int n, const int32_t* __restrict__ dt,
                                               int64_t i, int64_t lo, int64_t hi, G g, T (&acc)[R]) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const T a = ld_val(v + k * n, NT);
+        const T a = (T)ld_val(v + k * n, NT);
         const Vr<T, R> x = g(min(max(i + (int64_t)dt[k], lo), hi));
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
     }
 }
-template <int W, typename T, int R, int CI, int NT, typename G>
-__device__ __forceinline__ void general_slots(const T* __restrict__ v, int n, const void* __restrict__ ec, int64_t f0,
+template <int W, typename T, int R, int CI, int NT, typename V, typename G>
+__device__ __forceinline__ void general_slots(const V* __restrict__ v, int n, const void* __restrict__ ec, int64_t f0,
                                               int64_t i, G g, T (&acc)[R]) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const T a = ld_val(v + k * n, NT);
+        const T a = (T)ld_val(v + k * n, NT);
         const Vr<T, R> x = g(col_of<CI>(ec, f0 + k * n, i));
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
@@ -166,10 +175,10 @@ __device__ __forceinline__ void general_slots(const T* __restrict__ v, int n, co
 }
 constexpr int UNROLL_W = 16;  // general slices wider than this take a plain loop
 
-template <typename T, int R, int CI, int NT, typename G>
+template <typename T, int R, int CI, int NT, typename V, typename G>
 __device__ __forceinline__ void row_slots(const BDev& B, int s, int n, int64_t i, int64_t lo, int64_t hi,
-                                          const T* __restrict__ vals, G g, T (&acc)[R]) {
-    const T* v = vals + B.sl_off[s] + threadIdx.x;
+                                          const V* __restrict__ vals, G g, T (&acc)[R]) {
+    const V* v = vals + B.sl_off[s] + threadIdx.x;
     const int64_t c0 = B.sl_coff[s];
     const int w = B.sl_w[s];
     if (c0 < 0) {
@@ -191,7 +200,7 @@ __device__ __forceinline__ void row_slots(const BDev& B, int s, int n, int64_t i
 #undef RUV_GS
             default:
                 for (int k = 0; k < w; ++k) {
-                    const T a = ld_val(v + k * n, NT);
+                    const T a = (T)ld_val(v + k * n, NT);
                     const Vr<T, R> x = g(col_of<CI>(B.ecol, f0 + (int64_t)k * n, i));
 #pragma unroll
                     for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
