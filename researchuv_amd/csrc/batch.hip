@@ -551,6 +551,8 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (precond != RUV_PC_NONE && precond != RUV_PC_JACOBI && precond != RUV_PC_AMG)
         return fail(RUV_ERR_INVALID, "the batched solver supports the none, jacobi and amg preconditioners");
     if (mode != RUV_MODE_FP64 && mode != RUV_MODE_MIXED) return fail(RUV_ERR_INVALID, "unknown mode");
+    if (nrows < 0 || nrows >= (int64_t)INT32_MAX)  // the kernels index rows in 32 bits
+        return fail(RUV_ERR_INVALID, "batch: the row count must be below 2^31 - 1");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RUV_ERR_UNAVAILABLE, "no HIP device");
     std::unique_ptr<BatchImpl> Q(new BatchImpl());

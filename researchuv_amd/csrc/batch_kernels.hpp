@@ -151,13 +151,16 @@ __device__ __forceinline__ float ld_val(const Bf16* __restrict__ p, int) { retur
 // Row i of slice s (lane threadIdx.x): acc[c] += sum over the row's slots of a_ij * g(j)[c].  The slot loop is
 // unrolled for the slice's width through a switch (the width is uniform over the workgroup): a loop of run-time
 // length issues its loads a few at a time, and the kernels built on it measured 20-35 % slower.
-template <int W, typename T, int R, int NT, typename V, typename G>
+// Stencil slots in 32-bit row arithmetic (a batch has fewer than 2^31 rows); CLAMP = 0 for a slice whose every
+// offset stays inside its chart, which skips the clamp.
+template <int W, int CLAMP, typename T, int R, int NT, typename V, typename G>
 __device__ __forceinline__ void stencil_slots(const V* __restrict__ v, int n, const int32_t* __restrict__ dt,
-                                              int64_t i, int64_t lo, int64_t hi, G g, T (&acc)[R]) {
+                                              int32_t i, int32_t lo, int32_t hi, G g, T (&acc)[R]) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         const T a = (T)ld_val(v + k * n, NT);
-        const Vr<T, R> x = g(min(max(i + (int64_t)dt[k], lo), hi));
+        const int32_t j = CLAMP ? min(max(i + dt[k], lo), hi) : i + dt[k];
+        const Vr<T, R> x = g((int64_t)j);
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
     }
@@ -182,13 +185,27 @@ __device__ __forceinline__ void row_slots(const BDev& B, int s, int n, int64_t i
     const int64_t c0 = B.sl_coff[s];
     const int w = B.sl_w[s];
     if (c0 < 0) {
+        if (w <= 0) return;
         const int32_t* dt = B.sl_dtab + (int64_t)s * SLOTS;
-        switch (w) {
+        const int64_t r0 = B.sl_row0[s];
+        const bool inner = r0 + dt[0] >= lo && r0 + n - 1 + dt[w - 1] <= hi;  // offsets ascend
+        const int32_t i32 = (int32_t)i, lo32 = (int32_t)lo, hi32 = (int32_t)hi;
+        if (inner) {
+            switch (w) {
 #define RUV_ST(W) \
-    case W: stencil_slots<W, T, R, NT>(v, n, dt, i, lo, hi, g, acc); break;
-            RUV_ST(1) RUV_ST(2) RUV_ST(3) RUV_ST(4) RUV_ST(5) RUV_ST(6) RUV_ST(7) RUV_ST(8)
+    case W: stencil_slots<W, 0, T, R, NT>(v, n, dt, i32, lo32, hi32, g, acc); break;
+                RUV_ST(1) RUV_ST(2) RUV_ST(3) RUV_ST(4) RUV_ST(5) RUV_ST(6) RUV_ST(7) RUV_ST(8)
 #undef RUV_ST
-            default: break;
+                default: break;
+            }
+        } else {
+            switch (w) {
+#define RUV_ST(W) \
+    case W: stencil_slots<W, 1, T, R, NT>(v, n, dt, i32, lo32, hi32, g, acc); break;
+                RUV_ST(1) RUV_ST(2) RUV_ST(3) RUV_ST(4) RUV_ST(5) RUV_ST(6) RUV_ST(7) RUV_ST(8)
+#undef RUV_ST
+                default: break;
+            }
         }
     } else {
         const int64_t f0 = c0 + threadIdx.x;
