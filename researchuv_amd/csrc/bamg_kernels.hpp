@@ -559,13 +559,6 @@ __global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const i
     }
 }
 
-// composed map of two matchings: fine row -> level aggregate
-__global__ __launch_bounds__(BLK) void k_compose_agg(int64_t n, const int32_t* __restrict__ agg1,
-                                                     const int32_t* __restrict__ agg2, int32_t* __restrict__ agg) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < n) agg[i] = agg2[agg1[i]];
-}
-
 // group (chart) of every row from the chart row offsets
 __global__ __launch_bounds__(BLK) void k_chart_groups(int64_t n, const int64_t* __restrict__ crow, int nchart,
                                                       int32_t* __restrict__ group) {
@@ -882,56 +875,6 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) {
             o.c[cc] = fma(di, ri.c[cc] - acc[cc], ei.c[cc]);
-            arz += (double)ri.c[cc] * (double)o.c[cc];
-        }
-        st<T, R>(z, i, o);
-    }
-    const double srz = block_sum_all(arz, sh);
-    if (threadIdx.x == 0) B.part_c[(int64_t)par * B.ntile + t] = srz;
-}
-
-//   up: e1 = w D^-1 r + P z1; z = e1 + w D^-1 (r - A e1) for the tile's rows; partial r.z of the tile into
-//       part_c[par] (the r.z the next step opens with).
-template <typename T, int R, int CI>
-__global__ __launch_bounds__(BLK) void kc_up0(BDev B, const int* __restrict__ act, int na, int par,
-                                              const typename CoarseVal<T>::type* __restrict__ vals,
-                                              const T* __restrict__ dinv, T w,
-                                              const T* __restrict__ r, const int32_t* __restrict__ agg,
-                                              const T* __restrict__ z1, T* __restrict__ z) {
-    __shared__ double sh[WAVES];
-    const int slot = xcd_slot(na);
-    if (slot < 0) return;
-    const int t = act[slot];
-    const int c = B.tile_chart[t];
-    if (B.c_done[c] != B_RUNNING) return;
-    const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
-    double arz = 0.0;
-    const int s1 = B.tile_s0[t + 1];
-    for (int s = B.tile_s0[t]; s < s1; ++s) {
-        const int n = B.sl_n[s];
-        if ((int)threadIdx.x >= n) continue;
-        const int64_t i = B.sl_row0[s] + threadIdx.x;
-        T acc[R];
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals,
-                               [&](int64_t j) {
-                                   const T d = w * dinv[j];
-                                   const Vr<T, R> rj = ld<T, R>(r, j);
-                                   Vr<T, R> x = ld<T, R>(z1, agg[j]);
-#pragma unroll
-                                   for (int cc = 0; cc < R; ++cc) x.c[cc] = fma(d, rj.c[cc], x.c[cc]);
-                                   return x;
-                               },
-                               acc);
-        const T di = w * dinv[i];
-        const Vr<T, R> ri = ld<T, R>(r, i);
-        const Vr<T, R> zc = ld<T, R>(z1, agg[i]);
-        Vr<T, R> o;
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) {
-            const T e1 = fma(di, ri.c[cc], zc.c[cc]);
-            o.c[cc] = fma(di, ri.c[cc] - acc[cc], e1);
             arz += (double)ri.c[cc] * (double)o.c[cc];
         }
         st<T, R>(z, i, o);

@@ -42,7 +42,6 @@ struct Bamg {
     // Jacobi weight of the smoothing steps: omega_max, or less where a level's Gershgorin bound g on D^-1 A
     // asks for it (omega <= 1.9 / g keeps each smoother an A-norm contraction, so the cycle stays SPD)
     double omega_max = 0.9;
-    bool split_up = true;  // level 0's way up in two passes (kc_prolong0, kc_post0) or one (kc_up0)
     double setup_seconds = 0.0;
     DevPool* pool = nullptr;
     std::vector<void*> mem;
@@ -227,11 +226,11 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     return RUV_OK;
 }
 
-// chart row offsets on the next level: through one matching (cid2 null) or two
-__global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, const int64_t* __restrict__ cid1,
-                                 const int64_t* __restrict__ cid2, int64_t* __restrict__ out) {
+// chart row offsets on the next level
+__global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, const int64_t* __restrict__ cid,
+                                 int64_t* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (k < m) out[k] = cid2 ? cid2[cid1[at[k]]] : cid1[at[k]];
+    if (k < m) out[k] = cid[at[k]];
 }
 
 // Invert each chart's block of the coarsest operator (FP64 CSR on the device, chart row offsets crow) on the host:
@@ -361,7 +360,6 @@ int bamg_build(BatchImpl& Q) {
         const double v = std::atof(e);
         if (v > 0.0 && v < 1.0) A->omega_max = v;
     }
-    if (const char* e = std::getenv("RUV_BATCH_AMG_SPLIT_UP")) A->split_up = std::atoi(e) != 0;
     const bool f32 = Q.mode == RUV_MODE_MIXED;
     const size_t sv = f32 ? sizeof(float) : sizeof(double);
     const int R = Q.R;
@@ -369,10 +367,9 @@ int bamg_build(BatchImpl& Q) {
     // and matchings per level: one (aggregates of about two rows) converges in about half the iterations of two
     // (aggregates of about four) on the heterogeneous weights of jittered meshes, for about 1.5x the coarse work
     double theta = 0.1;
-    int rounds = 4, passes = 1;
+    int rounds = 4;
     if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));
-    if (const char* e = std::getenv("RUV_BATCH_AMG_PASSES")) passes = std::min(2, std::max(1, std::atoi(e)));
     // coarsening stops once every chart has at most dense_rows rows; the coarsest level is then solved exactly
     // with each chart's inverted block (up to dense_max rows a chart, else Jacobi)
     int64_t dense_rows = 64;
@@ -477,43 +474,26 @@ int bamg_build(BatchImpl& Q) {
             A->lv.push_back(L);
             break;
         }
-        // one or two matchings make the next level
-        std::unique_ptr<Scratch> S1(new Scratch(&Q.pool)), S2(new Scratch(&Q.pool));
-        Matched M1, M2;
+        // one matching makes the next level (aggregates of about two rows: a second matching of the Galerkin
+        // operator, aggregates of about four, doubled the iterations on jittered meshes)
+        std::unique_ptr<Scratch> S1(new Scratch(&Q.pool));
+        Matched M1;
         int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
         if (rc) return rc;
-        if (passes > 1) {
-            rc = match_and_coarsen(s, M1.C, M1.group, theta, rounds, *S2, &M2);
-            if (rc) return rc;
-        }
-        const Matched& MF = (passes > 1) ? M2 : M1;
+        const Matched& MF = M1;
         L.omega = (M1.gersh > 0.0) ? std::min(A->omega_max, 1.9 / M1.gersh) : A->omega_max;
         if (l > 0 && MF.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
             A->lv.push_back(L);
             break;
         }
-        Scratch& SF = (passes > 1) ? *S2 : *S1;
-        const int64_t* mptr64 = M1.mptr;
-        const int64_t* mrows64 = M1.mrows;
-        if (passes > 1) {
-            HIPCHK(A->alloc(&L.agg, cur.n));
-            hipLaunchKernelGGL(k_compose_agg, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int32_t*)M1.agg,
-                               (const int32_t*)M2.agg, L.agg);
-            HIPCHK(hipGetLastError());
-            int64_t *p64 = nullptr, *r64 = nullptr;
-            rc = build_members(s, cur.n, L.agg, MF.nc, (const int32_t*)nullptr, *S2, &p64, &r64, (int32_t*)nullptr);
-            if (rc) return rc;
-            mptr64 = p64;
-            mrows64 = r64;
-        } else {  // the matching's own aggregate map and member lists
-            hand_over(*S1, M1.agg, A->mem);
-            L.agg = M1.agg;
-        }
+        hand_over(*S1, M1.agg, A->mem);  // the matching's aggregate map serves the cycle
+        L.agg = M1.agg;
         // the cycle's member lists in 32 bits (a level's rows are below 2^31)
         HIPCHK(A->alloc(&L.mptr, MF.nc + 1));
         HIPCHK(A->alloc(&L.mrows, cur.n));
-        hipLaunchKernelGGL(k_narrow, rows_grid(MF.nc + 1), dim3(BLK), 0, s, MF.nc + 1, mptr64, L.mptr);
-        hipLaunchKernelGGL(k_narrow, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, mrows64, L.mrows);
+        hipLaunchKernelGGL(k_narrow, rows_grid(MF.nc + 1), dim3(BLK), 0, s, MF.nc + 1, (const int64_t*)M1.mptr,
+                           L.mptr);
+        hipLaunchKernelGGL(k_narrow, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int64_t*)M1.mrows, L.mrows);
         HIPCHK(hipGetLastError());
         // chart offsets on the next level
         {
@@ -523,8 +503,7 @@ int bamg_build(BatchImpl& Q) {
             HIPCHK(S1->alloc(&d_out, at.size()));
             HIPCHK(hipMemcpyAsync(d_at, at.data(), sizeof(int64_t) * at.size(), hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_gather_offsets, rows_grid((int64_t)at.size()), dim3(BLK), 0, s, (int64_t)at.size(),
-                               (const int64_t*)d_at, (const int64_t*)M1.cid,
-                               (const int64_t*)(passes > 1 ? M2.cid : nullptr), d_out);
+                               (const int64_t*)d_at, (const int64_t*)M1.cid, d_out);
             std::vector<int64_t> out(at.size());
             HIPCHK(hipMemcpyAsync(out.data(), d_out, sizeof(int64_t) * at.size(), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -535,7 +514,7 @@ int bamg_build(BatchImpl& Q) {
         std::unique_ptr<Scratch> nk(new Scratch(&Q.pool));
         for (const void* p : {(const void*)MF.C.ptr, (const void*)MF.C.cols, (const void*)MF.C.vals,
                               (const void*)MF.cdiag})
-            hand_over(SF, (void*)p, nk->mem);
+            hand_over(*S1, (void*)p, nk->mem);
         int32_t* ngroup = nullptr;  // kept: the cycle skips the rows of stopped charts by it
         HIPCHK(A->alloc(&ngroup, MF.nc));
         int64_t* d_crow = nullptr;
@@ -550,7 +529,6 @@ int bamg_build(BatchImpl& Q) {
         group = ngroup;
         keep = std::move(nk);  // frees the previous level's setup copy
         S1.reset();
-        S2.reset();
     }
     // finest level's aggregate map and members live on; level 1 must exist
     if (A->lv.size() < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
@@ -611,15 +589,12 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
         else hipLaunchKernelGGL((kc_post<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
     }
-    if (A.split_up) {  // e1 in level 0's work vector (free once restricted), then the post-smoothing
-        hipLaunchKernelGGL((kc_prolong0<T, R>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na,
-                           (const T*)L0.dinv, w0, r, (const int32_t*)L0.agg, (const T*)L1.z, (T*)L0.t);
-        hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
-                           (const T*)L0.dinv, w0, r, (const T*)L0.t, z);
-    } else {
-        hipLaunchKernelGGL((kc_up0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
-                           (const T*)L0.dinv, w0, r, (const int32_t*)L0.agg, (const T*)L1.z, z);
-    }
+    // up: e1 in level 0's work vector (free once restricted), then the post-smoothing over it (two passes:
+    // recomputing e1 for every gathered neighbour inside the post-smoothing measured slower)
+    hipLaunchKernelGGL((kc_prolong0<T, R>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, (const T*)L0.dinv, w0,
+                       r, (const int32_t*)L0.agg, (const T*)L1.z, (T*)L0.t);
+    hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
+                       (const T*)L0.dinv, w0, r, (const T*)L0.t, z);
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }
