@@ -275,6 +275,22 @@ int ruv_lscm_diff_vjp(ruv_lscm_diff* h, const double* g_uv, double energy_weight
                       ruv_solve_info* info);
 void ruv_lscm_diff_destroy(ruv_lscm_diff* h);
 
+/* ---------------------------------------------------------------------------------------------------------
+ * Post-solve chart metrics, many charts per call on the device (the unfold stage runs both on every chart
+ * after its solve, pipeline.rs:171-182).  Chart c owns vertices [v_off[c], v_off[c+1]) of p3 (xyz: the mesh
+ * positions of the chart's vertex_ids, chart-local order) and uv (u, v); v_off[0] = 0.  Host pointers.
+ * ------------------------------------------------------------------------------------------------------- */
+/* chart_distortion (crates/researchuv-unwrap/src/metrics.rs:73-141; svd2 :59-71, winding_flips :39-56): chart
+ * c's triangles are [t_off[c], t_off[c+1]) of tris, chart-local corner triples in mesh face order.
+ * metrics (nchart*3): conformal_mean, conformal_max, area_ratio_mean; counts (nchart*2): flips, folds. */
+int ruv_chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                      const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
+/* rectangularize (metrics.rs:147-227): chart c's border vertices are [b_off[c], b_off[c+1]) of bnd (distinct
+ * chart-local ids).  q (2 per vertex): the rectified uv in [0,1]^2; ext (nchart*2): the extent before
+ * normalisation (the pack rectangle, pipeline.rs:178). */
+int ruv_rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
+                       const int64_t* bnd, double* q, double* ext);
+
 #ifdef __cplusplus
 }
 #endif

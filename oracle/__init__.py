@@ -70,6 +70,10 @@ def lib():
         L.orc_ic0_apply.argtypes = [C.c_int32, P, P, P, P, P, P, P, P]
         L.orc_transpose_lower.argtypes = [C.c_int32, P, P, P, P, P, P]
         L.orc_coloring.argtypes = [C.c_int32, P, P, P]
+        L.orc_winding_flips.restype = C.c_int64
+        L.orc_winding_flips.argtypes = [P, P, C.c_int64]
+        L.orc_chart_distortion.argtypes = [P, P, C.c_int64, P, P, P]
+        L.orc_rectangularize.argtypes = [P, C.c_int64, P, C.c_int64, P, P]
         _lib = L
     return _lib
 
@@ -310,3 +314,33 @@ def lscm_loss_frozen(p3, tris, pin_ids, pin_uv, target):
     loss = L.orc_lscm_loss_frozen(_p(p3), len(p3), _p(tris), len(tris), _p(pin_ids), _p(pin_uv), len(pin_ids),
                                   _p(target), _p(uv), C.byref(ok))
     return (loss, uv) if ok.value else None
+
+
+# ------------------------------------------------------------------ post-solve chart metrics
+def winding_flips(uv, tris):
+    """metrics.rs:39-56"""
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    return int(lib().orc_winding_flips(_p(uv), _p(tris), len(tris)))
+
+
+def chart_distortion(p3, tris, uv):
+    """metrics.rs:73-141 on chart-local positions / triangles / uv -> dict of the ChartMetrics fields."""
+    p3 = np.ascontiguousarray(p3, np.float64).reshape(-1, 3)
+    tris = np.ascontiguousarray(tris, np.uint32).reshape(-1, 3)
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    out = np.zeros(3)
+    cnt = np.zeros(2, np.int64)
+    lib().orc_chart_distortion(_p(p3), _p(tris), len(tris), _p(uv), _p(out), _p(cnt))
+    return dict(conformal_mean=float(out[0]), conformal_max=float(out[1]), area_ratio_mean=float(out[2]),
+                flips=int(cnt[0]), folds=int(cnt[1]))
+
+
+def rectangularize(uv, bnd):
+    """metrics.rs:147-227 -> (q (nv, 2), ext (2,))"""
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    bnd = np.ascontiguousarray(bnd, np.int64).reshape(-1)
+    q = np.zeros_like(uv)
+    ext = np.zeros(2)
+    lib().orc_rectangularize(_p(uv), len(uv), _p(bnd), len(bnd), _p(q), _p(ext))
+    return q, ext

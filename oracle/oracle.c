@@ -25,6 +25,13 @@
  *   orc_ic0_* ...................... L*L^T == A on the pattern (precond.rs:708-737), level partition (:739-755),
  *                                    dense triangular solve of the same factor
  *   orc_coloring ................... distance-1 validity, colour bound (solver.rs:889-909)
+ *   orc_winding_flips .............. recut.rs:155-169 known answers, a planted fold
+ *   orc_chart_distortion ........... identity unfold of a triangle is conformal (metrics.rs:239-257); affine maps
+ *                                    of a planar mesh (closed-form singular values and area ratio); degenerate
+ *                                    triangles counted as flips
+ *   orc_rectangularize ............. unit square on its border (metrics.rs:258-279); a rectangle rotated by a
+ *                                    ladder angle comes back axis-aligned with its side lengths; border-free
+ *                                    normalisation to [0,1]^2
  *   Parity unpinned (no independent check exists without running the Rust code): the tie-break order inside
  *   chart_border_loops' straight walk and the eigenvector sign convention of eig3.  Both are restated literally.
  */
@@ -1500,4 +1507,153 @@ void orc_coloring(int32_t n, const int32_t* cols, const int32_t* row_ptr, uint32
         color[v] = k;
     }
     free(used);
+}
+
+/* ------------------------------------------------------------------ post-solve chart metrics */
+/* tri_signed (metrics.rs:31-34): signed doubled area of one UV triangle */
+static double tri_signed2(v2 u, v2 v, v2 w) { return (v.u - u.u) * (w.v - u.v) - (w.u - u.u) * (v.v - u.v); }
+static v2 UV(const double* uv, size_t i) { v2 r = {uv[2 * i], uv[2 * i + 1]}; return r; }
+
+/* winding_flips (metrics.rs:39-56): triangles whose UV winding disagrees with the signed-area majority */
+int64_t orc_winding_flips(const double* uv, const uint32_t* tris, int64_t ntri) {
+    if (ntri == 0) return 0;
+    double sum = 0.0;
+    for (int64_t t = 0; t < ntri; ++t)
+        sum += tri_signed2(UV(uv, tris[3 * t]), UV(uv, tris[3 * t + 1]), UV(uv, tris[3 * t + 2]));
+    const double dom = (sum < 0.0) ? -1.0 : 1.0;
+    int64_t n = 0;
+    for (int64_t t = 0; t < ntri; ++t)
+        if (tri_signed2(UV(uv, tris[3 * t]), UV(uv, tris[3 * t + 1]), UV(uv, tris[3 * t + 2])) * dom <= 0.0) ++n;
+    return n;
+}
+
+/* svd2 (metrics.rs:59-71): singular values of a 2x2 matrix through the eigenvalues of M^T M */
+static void svd2(const double m[2][2], double* s1, double* s2) {
+    const double a = m[0][0] * m[0][0] + m[1][0] * m[1][0];
+    const double b = m[0][0] * m[0][1] + m[1][0] * m[1][1];
+    const double c = m[0][1] * m[0][1] + m[1][1] * m[1][1];
+    const double tr = a + c;
+    const double d = 0.5 * (a - c);
+    const double disc = sqrt(fmax(d * d + b * b, 0.0));
+    const double l1 = 0.5 * tr + disc;
+    const double l2 = fmax(0.5 * tr - disc, 0.0);
+    *s1 = sqrt(fmax(l1, 0.0));
+    *s2 = sqrt(l2);
+}
+
+/* chart_distortion (metrics.rs:73-141).  p3: the chart's vertex positions in chart-local order (the mesh
+ * positions of vertex_ids), tris: chart-local triangles (the mesh faces' corners in face order), uv: the
+ * unfold.  out = {conformal_mean, conformal_max, area_ratio_mean}, cnt = {flips, folds}. */
+void orc_chart_distortion(const double* p3, const uint32_t* tris, int64_t ntri, const double* uv, double* out,
+                          int64_t* cnt) {
+    double conf_sum = 0.0, conf_max = 0.0, area_sum = 0.0;
+    int64_t nconf = 0, flips = 0;
+    for (int64_t t = 0; t < ntri; ++t) {
+        const uint32_t a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+        const v3 a3 = P3(p3, a), b3 = P3(p3, b), c3 = P3(p3, c);
+        const v2 u0 = UV(uv, a), u1 = UV(uv, b), u2 = UV(uv, c);
+        const v3 e1 = v3sub(b3, a3);
+        const v3 e2 = v3sub(c3, a3);
+        const v3 nrm = v3cross(e1, e2);
+        const double ln = v3len(nrm);
+        if (ln < 1e-30) continue;
+        const v3 n = v3div(nrm, ln);
+        const v3 t1 = v3div(e1, fmax(v3len(e1), 1e-30));
+        const v3 t2 = v3cross(n, t1);
+        const double e1t1 = v3dot(e1, t1), e2t1 = v3dot(e2, t1), e1t2 = v3dot(e1, t2), e2t2 = v3dot(e2, t2);
+        const double det = e1t1 * e2t2 - e2t1 * e1t2;
+        if (fabs(det) < 1e-30) {
+            ++flips;
+            continue;
+        }
+        const double ip00 = e2t2 / det, ip01 = -e2t1 / det, ip10 = -e1t2 / det, ip11 = e1t1 / det;
+        const v2 du1 = {u1.u - u0.u, u1.v - u0.v}, du2 = {u2.u - u0.u, u2.v - u0.v};
+        const double f[2][2] = {{du1.u * ip00 + du2.u * ip10, du1.u * ip01 + du2.u * ip11},
+                                {du1.v * ip00 + du2.v * ip10, du1.v * ip01 + du2.v * ip11}};
+        double s1, s2;
+        svd2(f, &s1, &s2);
+        const double conf = s1 / fmax(s2, 1e-30);
+        conf_sum += conf;
+        conf_max = fmax(conf_max, conf);
+        ++nconf;
+        const double uv_area = 0.5 * fabs(du1.u * du2.v - du2.u * du1.v);
+        area_sum += uv_area / (0.5 * ln);
+    }
+    const double den = (double)(nconf > 0 ? nconf : 1);
+    out[0] = conf_sum / den;
+    out[1] = conf_max;
+    out[2] = area_sum / den;
+    cnt[0] = flips;
+    cnt[1] = orc_winding_flips(uv, tris, ntri);
+}
+
+/* rectangularize (metrics.rs:147-227): the rotation of a 72-step ladder over [0, pi] with the smallest
+ * sqrt(ext0 (ext0 + 1e-30) + ext1 (ext1 + 1e-30)) of the border's bounding box (first one on ties), the whole
+ * chart rotated and normalised into [0,1]^2, border points snapped onto the nearest rectangle side.  q (nv*2)
+ * the rectified uv, ext (2) the extent before normalisation.  Without border the chart is only normalised. */
+void orc_rectangularize(const double* uv, int64_t nv, const int64_t* bnd, int64_t nb, double* q, double* ext) {
+    if (nv == 0) {
+        ext[0] = ext[1] = 0.0;
+        return;
+    }
+    if (nb == 0) {
+        v2 mn = UV(uv, 0), mx = UV(uv, 0);
+        for (int64_t i = 0; i < nv; ++i) {
+            const v2 p = UV(uv, i);
+            mn.u = fmin(mn.u, p.u); mn.v = fmin(mn.v, p.v);
+            mx.u = fmax(mx.u, p.u); mx.v = fmax(mx.v, p.v);
+        }
+        const v2 e = {mx.u - mn.u, mx.v - mn.v};
+        for (int64_t i = 0; i < nv; ++i) {
+            const v2 p = UV(uv, i);
+            q[2 * i] = (p.u - mn.u) / fmax(e.u, 1e-30);
+            q[2 * i + 1] = (p.v - mn.v) / fmax(e.v, 1e-30);
+        }
+        ext[0] = e.u;
+        ext[1] = e.v;
+        return;
+    }
+    const int n_steps = 72;
+    int have = 0;
+    double bc = 1.0, bs = 0.0, bscore = INFINITY;
+    for (int k = 0; k < n_steps; ++k) {
+        const double ang = 3.14159265358979323846 * (double)k / (double)(n_steps - 1);
+        const double cs = cos(ang), sn = sin(ang);
+        double mn = INFINITY, mx = -INFINITY, mnv = INFINITY, mxv = -INFINITY;
+        for (int64_t j = 0; j < nb; ++j) {
+            const v2 p = UV(uv, (size_t)bnd[j]);
+            const double qu = cs * p.u + sn * p.v;
+            const double qv = -sn * p.u + cs * p.v;
+            mn = fmin(mn, qu); mx = fmax(mx, qu);
+            mnv = fmin(mnv, qv); mxv = fmax(mxv, qv);
+        }
+        const double e0 = mx - mn, e1 = mxv - mnv;
+        const double score = sqrt(e0 * (e0 + 1e-30) + e1 * (e1 + 1e-30));
+        if (have && score >= bscore) continue;
+        have = 1; bc = cs; bs = sn; bscore = score;
+    }
+    for (int64_t i = 0; i < nv; ++i) {
+        const v2 p = UV(uv, i);
+        q[2 * i] = bc * p.u + bs * p.v;
+        q[2 * i + 1] = -bs * p.u + bc * p.v;
+    }
+    double mnu = q[0], mnv = q[1], mxu = q[0], mxv = q[1];
+    for (int64_t i = 0; i < nv; ++i) {
+        mnu = fmin(mnu, q[2 * i]); mnv = fmin(mnv, q[2 * i + 1]);
+        mxu = fmax(mxu, q[2 * i]); mxv = fmax(mxv, q[2 * i + 1]);
+    }
+    const double eu = mxu - mnu, ev = mxv - mnv;
+    for (int64_t i = 0; i < nv; ++i) {
+        q[2 * i] = (q[2 * i] - mnu) / fmax(eu, 1e-30);
+        q[2 * i + 1] = (q[2 * i + 1] - mnv) / fmax(ev, 1e-30);
+    }
+    for (int64_t j = 0; j < nb; ++j) {
+        const int64_t bi = bnd[j];
+        const double pu = q[2 * bi], pv = q[2 * bi + 1];
+        const double dx = fmin(pu, 1.0 - pu), dy = fmin(pv, 1.0 - pv);
+        if (dx <= dy) q[2 * bi] = (pu < 0.5) ? 0.0 : 1.0;
+        else q[2 * bi + 1] = (pv < 0.5) ? 0.0 : 1.0;
+    }
+    ext[0] = eu;
+    ext[1] = ev;
 }

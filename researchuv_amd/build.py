@@ -25,7 +25,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature"
              "-munsafe-fp-atomics"]
 HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["pcg.hip", "batch.hip", "batch_amg.hip", "assemble.hip", "adjoint.hip", "diff_batch.hip"]
+HIP_SOURCES = ["pcg.hip", "batch.hip", "batch_amg.hip", "assemble.hip", "adjoint.hip", "diff_batch.hip", "metrics.hip"]
 HOST_SOURCES = ["host_precond.cpp", "host_amg.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp", "api.cpp"]
 
 

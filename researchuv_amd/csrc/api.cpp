@@ -544,4 +544,14 @@ void ruv_lscm_diff_destroy(ruv_lscm_diff* h) {
     delete h;
 }
 
+int ruv_chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                      const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts) {
+    return ruv::chart_metrics(nchart, v_off, p3, uv, t_off, tris, metrics, counts);
+}
+
+int ruv_rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
+                       const int64_t* bnd, double* q, double* ext) {
+    return ruv::rectangularize(nchart, v_off, uv, b_off, bnd, q, ext);
+}
+
 }  // extern "C"

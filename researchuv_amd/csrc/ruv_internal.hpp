@@ -73,4 +73,10 @@ int device_solve(const Csr& a, int R, const double* b, uint64_t max_iter, double
                  double* x, ruv_solve_info* info);
 int device_apply_ic0(const Csr& a, int R, const double* r, double* z);
 
+// post-solve chart metrics on the device (metrics.hip)
+int chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
+                  const uint32_t* tris, double* metrics, int64_t* counts);
+int rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off, const int64_t* bnd,
+                   double* q, double* ext);
+
 }  // namespace ruv

@@ -1,0 +1,126 @@
+"""GPU parity of the post-solve chart metrics (ruv_chart_metrics, ruv_rectangularize) against the oracle's
+restatement of metrics.rs:31-227: the reference's own cases, every chart of real meshes (cube charts,
+uneven height-field patches unfolded by the oracle's LU), a 512-chart batch at full patch sizes (with the
+tangent-plane projection as the unfold), and the error paths."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def _charts(orc, meshes, unfold=True):
+    """(p3, tris, uv, bnd) per chart, chart-local, as pipeline.rs:171-177 hands them to the metrics."""
+    out = []
+    for pos, faces in meshes:
+        pw, fw = orc.weld(pos, faces, 1e-12)
+        ch = orc.Charts(pw, fw, 180.0)
+        for i in range(len(ch)):
+            c = ch.chart(i)
+            vids = c["vertex_ids"].astype(np.int64)
+            p3 = pw[vids]
+            uv = ch.unfold(i)[0] if unfold else orc.tangent_frame(p3)
+            bnd = np.searchsorted(vids, c["border"].astype(np.int64))
+            out.append((p3, c["tris"], uv, bnd))
+    return out
+
+
+def _same_metrics(got, ref, tag):
+    assert got.flips == ref["flips"] and got.folds == ref["folds"], tag
+    assert got.conformal_max == pytest.approx(ref["conformal_max"], rel=1e-14, abs=0), tag
+    assert got.conformal_mean == pytest.approx(ref["conformal_mean"], rel=1e-12), tag
+    assert got.area_ratio_mean == pytest.approx(ref["area_ratio_mean"], rel=1e-12), tag
+
+
+def test_reference_cases():
+    from researchuv_amd import metrics as M
+
+    uv = np.array([[0.0, 0.0], [0.5, 0.0], [0.0, 0.5], [0.5, 0.5]])
+    assert M.winding_flips(uv, [[0, 1, 2], [1, 2, 3]]) == 1  # recut.rs:155-169
+    assert M.winding_flips(uv, [[0, 2, 1], [1, 2, 3]]) == 0
+    assert M.winding_flips(uv, np.zeros((0, 3))) == 0
+    p = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    m = M.chart_distortion(p, [[0, 1, 2]], p[:, :2])  # metrics.rs:239-257
+    assert abs(m.conformal_mean - 1.0) < 1e-9 and m.flips == 0
+    q, ext = M.rectangularize(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]), [0, 1, 2, 3])
+    assert abs(ext[0] - 1.0) < 1e-9 and abs(ext[1] - 1.0) < 1e-9  # metrics.rs:258-279
+    for u, v in q:
+        assert min(abs(u), abs(1 - u), abs(v), abs(1 - v)) < 1e-9
+
+
+@pytest.mark.parametrize("which", ["cube", "patches"])
+def test_every_chart_matches_the_oracle(orc, which):
+    from researchuv_amd import meshgen
+    from researchuv_amd import metrics as M
+
+    if which == "cube":
+        meshes = [orc.cube(6)]
+    else:
+        sizes = meshgen.patch_sizes(24, vmin=64, vmax=1500, seed=5)
+        meshes = [meshgen.heightfield(int(a), int(b), seed=5000 + k) for k, (a, b) in enumerate(sizes)]
+    charts = _charts(orc, meshes)
+    say(f"{which}: {len(charts)} charts")
+    got = M.chart_distortion_batch([(p3, t, uv) for p3, t, uv, _ in charts])
+    rect = M.rectangularize_batch([(uv, b) for _, _, uv, b in charts])
+    for k, (p3, t, uv, b) in enumerate(charts):
+        _same_metrics(got[k], orc.chart_distortion(p3, t, uv), f"chart {k}")
+        q0, e0 = orc.rectangularize(uv, b)
+        np.testing.assert_array_equal(rect[k][0], q0, err_msg=f"chart {k}")
+        np.testing.assert_array_equal(rect[k][1], e0, err_msg=f"chart {k}")
+
+
+def test_full_size_patch_batch(orc):
+    from researchuv_amd import meshgen
+    from researchuv_amd import metrics as M
+
+    say("512 patches of 64-65536 vertices, tangent-plane unfold")
+    meshes = list(meshgen.patch_batch(512, seed=2))
+    charts = _charts(orc, meshes, unfold=False)
+    got = M.chart_distortion_batch([(p3, t, uv) for p3, t, uv, _ in charts])
+    rect = M.rectangularize_batch([(uv, b) for _, _, uv, b in charts])
+    for k in range(0, len(charts), 7):
+        p3, t, uv, b = charts[k]
+        _same_metrics(got[k], orc.chart_distortion(p3, t, uv), f"chart {k}")
+        q0, e0 = orc.rectangularize(uv, b)
+        np.testing.assert_array_equal(rect[k][0], q0, err_msg=f"chart {k}")
+        np.testing.assert_array_equal(rect[k][1], e0, err_msg=f"chart {k}")
+
+
+def test_batch_equals_single_calls_and_edge_cases(orc):
+    from researchuv_amd import metrics as M
+
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=(30, 3))
+    t = rng.integers(0, 30, size=(40, 3)).astype(np.uint32)
+    uv = rng.normal(size=(30, 2))
+    b = np.arange(0, 30, 3)
+    empty = (np.zeros((0, 3)), np.zeros((0, 3), np.uint32), np.zeros((0, 2)))
+    no_tris = (p[:5], np.zeros((0, 3), np.uint32), uv[:5])
+    batch = M.chart_distortion_batch([(p, t, uv), empty, no_tris, (p, t[::-1], uv)])
+    assert batch[0] == M.chart_distortion(p, t, uv)
+    _same_metrics(batch[0], orc.chart_distortion(p, t, uv), "random")
+    _same_metrics(batch[3], orc.chart_distortion(p, t[::-1], uv), "random reversed")
+    for m in batch[1:3]:
+        assert (m.conformal_mean, m.conformal_max, m.area_ratio_mean, m.flips, m.folds) == (0.0, 0.0, 0.0, 0, 0)
+    rect = M.rectangularize_batch([(uv, b), (np.zeros((0, 2)), np.zeros(0, np.int64)), (uv, np.zeros(0, np.int64))])
+    for (q, e), (u, bb) in zip([rect[0], rect[2]], [(uv, b), (uv, np.zeros(0, np.int64))]):
+        q0, e0 = orc.rectangularize(u, bb)
+        np.testing.assert_array_equal(q, q0)
+        np.testing.assert_array_equal(e, e0)
+    assert rect[1][0].shape == (0, 2) and list(rect[1][1]) == [0.0, 0.0]
+
+
+def test_errors_are_reported():
+    from researchuv_amd import RuvError
+    from researchuv_amd import metrics as M
+
+    p = np.zeros((3, 3))
+    with pytest.raises(RuvError):
+        M.chart_distortion(p, [[0, 1, 3]], np.zeros((3, 2)))  # corner outside the chart
+    with pytest.raises(RuvError):
+        M.rectangularize(np.zeros((3, 2)), [0, 1, 1])  # repeated border vertex
+    with pytest.raises(RuvError):
+        M.rectangularize(np.zeros((3, 2)), [0, 3])  # border vertex outside the chart
