@@ -9,7 +9,8 @@ to a true FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 per chart, with the
 solution resident in HBM.
 
 value = LSCM unknowns (2 x free vertices, summed over the charts) solved per second by the mixed-precision
-batched solve (FP32 Jacobi-PCG inside per-chart FP64 iterative refinement).  Beside it: the same batch in plain
+batched solve (FP32 PCG preconditioned by an aggregation AMG built on the device inside every step, within
+per-chart FP64 iterative refinement).  Beside it: the same batch in plain
 FP64, the plain FP64 drop-in called once per chart as the reference's driver does (on an evenly spaced sample
 of the charts), the UV and LSCM-energy error of the mixed solution against a CPU FP64 sparse LU (SuperLU) on a
 sample of charts, the step kernel's roofline (timed with HIP events on the solver's stream) and the CPU
@@ -18,6 +19,9 @@ oracle's worker-thread sparse-LU path on a bounded sample of the same patch fami
 `--workload chart512`: the single 512 x 512 vertex height-field chart; `--workload stretched`: the
 ill-conditioned 2048 x 2048 planar grid (column spacing 1 -> 100, 5% flipped diagonals).  Both solve one chart
 through the batched engine (a batch of one) with the device-built AMG, the plan drop-in timed beside it.
+`--workload diffbatch`: forward + adjoint solves of 256 height-field charts of 128 x 128 (the differentiable
+batch).  `--workload metrics`: the post-solve chart metrics (chart_distortion and rectangularize) of the 4096
+patch charts on the device, in triangles per second.
 """
 from __future__ import annotations
 
@@ -468,12 +472,106 @@ def run_diffbatch(args):
         roofline=None, meshes=None)
 
 
+def run_metrics(args):
+    """The post-solve metrics stage (pipeline.rs:171-182) over the 4096 patch charts: chart_distortion
+    (metrics.rs:73-141) and rectangularize (:147-227) on the device, inputs resident in HBM.  The charts come
+    from the product's host charting (weld, segment at 180 degrees); the unfold is stood in for by each patch's
+    planar (x, y) coordinates, which the metrics treat like any unfold."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from researchuv_amd import _lib, lscm, meshgen
+    from researchuv_amd import metrics as M
+
+    t0 = time.perf_counter()
+    p3s, tris, uvs, bnds = [], [], [], []
+    for pos, faces in meshgen.patch_batch(args.charts, seed=0):
+        pw, fw = lscm.weld(pos, faces, 1e-12)
+        seg = lscm.segment(pw, fw, 180.0)
+        for i in range(len(seg)):
+            ch = seg.chart(i)
+            vids = np.asarray(ch.vertex_ids, np.int64)
+            p3s.append(pw[vids])
+            tris.append(np.asarray(ch.tris, np.uint32))
+            uvs.append(np.ascontiguousarray(pw[vids][:, :2]))
+            bnds.append(np.searchsorted(vids, np.asarray(ch.border, np.int64)))
+    n = len(p3s)
+    off = lambda xs: np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.int64)  # noqa: E731
+    v_off, t_off, b_off = off(p3s), off(tris), off(bnds)
+    nv, nt, nb = int(v_off[-1]), int(t_off[-1]), int(b_off[-1])
+    dev = torch.device("cuda")
+    d = {k: torch.from_numpy(np.ascontiguousarray(a)).to(dev) for k, a in [
+        ("v_off", v_off), ("t_off", t_off), ("b_off", b_off), ("p3", np.concatenate(p3s)),
+        ("uv", np.concatenate(uvs)), ("tris", np.concatenate(tris).view(np.int32)),
+        ("bnd", np.concatenate(bnds).astype(np.int64))]}
+    met = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    cnt = torch.zeros((n, 2), dtype=torch.int64, device=dev)
+    q = torch.zeros((nv, 2), dtype=torch.float64, device=dev)
+    ext = torch.zeros((n, 2), dtype=torch.float64, device=dev)
+    L = M.lib()
+    V = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    log(f"metrics: {n} charts, {nv} vertices, {nt} triangles, {nb} border vertices, host charting "
+        f"{time.perf_counter() - t0:.1f}s")
+
+    def step():
+        _lib.check(L.ruv_chart_metrics_dev(n, V(d["v_off"]), V(d["p3"]), V(d["uv"]), V(d["t_off"]), V(d["tris"]),
+                                           V(met), V(cnt)))
+        _lib.check(L.ruv_rectangularize_dev(n, V(d["v_off"]), V(d["uv"]), V(d["b_off"]), V(d["bnd"]), V(q),
+                                            V(ext)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / args.steps
+    # algorithmic bytes: every input read once (offsets, positions, uv, triangles, border ids), outputs written
+    byt = 3 * 8 * (n + 1) + nv * (24 + 16) + nt * 12 + nb * 8 + n * (24 + 16) + nv * 16 + n * 16
+    log(f"metrics: {1e3 * sec:.2f} ms per pass, {nt / sec:.3e} triangles/s, {byt / sec / 1e9:.1f} GB/s "
+        f"(parity with the oracle is tests/test_gpu_metrics.py)")
+    return dict(
+        value=nt / sec, ms_per_step=1e3 * sec, dtype="f64", metric="chart metrics triangles/s (chart_distortion + "
+        "rectangularize)", unit="triangles/s",
+        config={"workload": f"metrics: {n} patch charts, {nv} vertices, {nt} triangles, {nb} border vertices",
+                "charts": n, "triangles": nt},
+        extra={},
+        roofline=dict(bound="hbm", achieved=byt / sec / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                      frac=byt / sec / HBM_PEAK, traffic=None, kernel="k_chart_distortion + k_rectangularize"),
+        meshes=None, cpu_sample=(p3s, tris, uvs, bnds))
+
+
+def cpu_baseline_metrics(sample, budget=2_000_000):
+    """The oracle's chart_distortion + rectangularize, one thread, over the first charts up to `budget`
+    triangles."""
+    import oracle
+
+    p3s, tris, uvs, bnds = sample
+    t0 = time.perf_counter()
+    nt = 0
+    k = 0
+    while k < len(p3s) and nt < budget:
+        oracle.chart_distortion(p3s[k], tris[k], uvs[k])
+        oracle.rectangularize(uvs[k], bnds[k])
+        nt += len(tris[k])
+        k += 1
+    sec = time.perf_counter() - t0
+    return dict(value=nt / sec, unit="triangles/s", cores=1, kind="port",
+                sample=f"oracle chart_distortion + rectangularize over the first {k} charts ({nt} triangles), "
+                       f"{sec:.2f} s on one thread")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched", "diffbatch"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched", "diffbatch", "metrics"])
     ap.add_argument("--side", type=int, default=0, help="chart512 / stretched: vertices per side (512 / 2048)")
     ap.add_argument("--compare", nargs="*", default=["JACOBI"], help="single chart: plan preconditioners timed beside")
     ap.add_argument("--compare-iters", type=int, default=20000, help="single chart: iteration cap of the comparisons")
@@ -501,17 +599,23 @@ def main():
         out = run_batch(args)
     elif args.workload == "diffbatch":
         out = run_diffbatch(args)
+    elif args.workload == "metrics":
+        out = run_metrics(args)
     else:
         out = run_single(args)
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline (oracle sparse LU)")
-        cpu = cpu_baseline_batch(out["meshes"]) if out["meshes"] is not None else cpu_baseline_chart()
-        log(f"cpu: {cpu['value']:.0f} unknowns/s ({cpu['sample']})")
+        if args.workload == "metrics":
+            log("cpu baseline (oracle metrics)")
+            cpu = cpu_baseline_metrics(out["cpu_sample"])
+        else:
+            log("cpu baseline (oracle sparse LU)")
+            cpu = cpu_baseline_batch(out["meshes"]) if out["meshes"] is not None else cpu_baseline_chart()
+        log(f"cpu: {cpu['value']:.0f} {cpu['unit']} ({cpu['sample']})")
     result = {
-        "metric": METRIC,
+        "metric": out.get("metric", METRIC),
         "value": out["value"],
-        "unit": "unknowns/s",
+        "unit": out.get("unit", "unknowns/s"),
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,

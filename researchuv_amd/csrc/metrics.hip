@@ -4,11 +4,13 @@
 //   solve (pipeline.rs:171-182).
 // Every operation follows the reference's order with multiply-add contraction disabled in this file, so each
 // triangle's terms, every min / max and the rectangularized coordinates are the reference's bit for bit; the
-// per-chart means sum the triangles in a fixed tree order instead of the reference's left-to-right order.
+// per-chart means sum the triangles in a fixed order (tiles of 1024 triangles, each a tree) instead of the
+// reference's left to right.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "dev_common.hpp"
@@ -60,24 +62,38 @@ __device__ __forceinline__ long long block_count(long long v, long long* sh) {
     return s;
 }
 
-// metrics (3 per chart): conformal mean, conformal max, area-ratio mean; counts (2 per chart): flips, folds
-__global__ __launch_bounds__(BLK) void k_chart_distortion(const int64_t* __restrict__ v_off,
+// Per tile of at most TILE_TRIS triangles of one chart (a big chart spans many tiles, so its triangles spread
+// over the whole device): the tile's sums of conformal ratio, area ratio and signed area, its largest conformal
+// ratio and its counts (measured triangles, flips, signed areas <= 0 and >= 0).
+constexpr int TILE_TRIS = 1024;
+struct TilePart {
+    double conf_sum, conf_max, area_sum, signed_sum;
+    long long nconf, flips, le0, ge0;
+};
+
+__global__ __launch_bounds__(BLK) void k_distortion_tiles(const int64_t* __restrict__ v_off,
                                                           const double* __restrict__ p3,
                                                           const double* __restrict__ uv,
-                                                          const int64_t* __restrict__ t_off,
+                                                          const int32_t* __restrict__ tile_chart,
+                                                          const int64_t* __restrict__ tile_t0,
+                                                          const int64_t* __restrict__ tile_t1,
                                                           const uint32_t* __restrict__ tris,
-                                                          double* __restrict__ metrics,
-                                                          long long* __restrict__ counts) {
+                                                          TilePart* __restrict__ part, int* __restrict__ err) {
     __shared__ double sh[WAVES];
     __shared__ long long shc[WAVES];
-    const int c = blockIdx.x;
-    const int64_t v0 = v_off[c];
+    const int tile = blockIdx.x;
+    const int c = tile_chart[tile];
+    const int64_t v0 = v_off[c], nvc = v_off[c + 1] - v0;
     const double* P = p3 + 3 * v0;
     const double* U = uv + 2 * v0;
     double conf_sum = 0.0, conf_max = 0.0, area_sum = 0.0, signed_sum = 0.0;
     long long nconf = 0, flips = 0, le0 = 0, ge0 = 0;
-    for (int64_t t = t_off[c] + threadIdx.x; t < t_off[c + 1]; t += BLK) {
+    for (int64_t t = tile_t0[tile] + threadIdx.x; t < tile_t1[tile]; t += BLK) {
         const uint32_t a = tris[3 * t], b = tris[3 * t + 1], cc = tris[3 * t + 2];
+        if ((int64_t)a >= nvc || (int64_t)b >= nvc || (int64_t)cc >= nvc) {  // reported, never read
+            atomicOr(err, 1);
+            continue;
+        }
         const double u0u = U[2 * a], u0v = U[2 * a + 1], u1u = U[2 * b], u1v = U[2 * b + 1];
         const double u2u = U[2 * cc], u2v = U[2 * cc + 1];
         // winding_flips' signed area, for every triangle
@@ -120,114 +136,277 @@ __global__ __launch_bounds__(BLK) void k_chart_distortion(const int64_t* __restr
         const double uv_area = 0.5 * fabs(d1u * d2v - d2u * d1v);
         area_sum += uv_area / (0.5 * ln);
     }
-    const double cs = block_sum_all(conf_sum, sh);
-    const double cm = block_max(conf_max, sh);
-    const double as = block_sum_all(area_sum, sh);
-    const double ss = block_sum_all(signed_sum, sh);
-    const long long nc = block_count(nconf, shc);
-    const long long fl = block_count(flips, shc);
-    const long long nle = block_count(le0, shc);
-    const long long nge = block_count(ge0, shc);
-    if (threadIdx.x == 0) {
-        const double den = (double)(nc > 0 ? nc : 1);
-        metrics[3 * c] = cs / den;
-        metrics[3 * c + 1] = cm;
-        metrics[3 * c + 2] = as / den;
-        counts[2 * c] = fl;
-        // dominant winding = sign of the signed-area sum; folds disagree with it (zero areas count)
-        counts[2 * c + 1] = (t_off[c + 1] == t_off[c]) ? 0 : ((ss < 0.0) ? nge : nle);
+    TilePart r;
+    r.conf_sum = block_sum_all(conf_sum, sh);
+    r.conf_max = block_max(conf_max, sh);
+    r.area_sum = block_sum_all(area_sum, sh);
+    r.signed_sum = block_sum_all(signed_sum, sh);
+    r.nconf = block_count(nconf, shc);
+    r.flips = block_count(flips, shc);
+    r.le0 = block_count(le0, shc);
+    r.ge0 = block_count(ge0, shc);
+    if (threadIdx.x == 0) part[tile] = r;
+}
+
+// per chart: its tiles' parts summed in tile order.  metrics (3 per chart): conformal mean, conformal max,
+// area-ratio mean; counts (2 per chart): flips, folds
+__global__ __launch_bounds__(BLK) void k_distortion_finish(int32_t nchart, const int32_t* __restrict__ chart_tile0,
+                                                           const TilePart* __restrict__ part,
+                                                           double* __restrict__ metrics,
+                                                           long long* __restrict__ counts) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= nchart) return;
+    double cs = 0.0, cm = 0.0, as = 0.0, ss = 0.0;
+    long long nc = 0, fl = 0, nle = 0, nge = 0;
+    for (int t = chart_tile0[c]; t < chart_tile0[c + 1]; ++t) {
+        const TilePart r = part[t];
+        cs += r.conf_sum;
+        cm = fmax(cm, r.conf_max);
+        as += r.area_sum;
+        ss += r.signed_sum;
+        nc += r.nconf;
+        fl += r.flips;
+        nle += r.le0;
+        nge += r.ge0;
     }
+    const double den = (double)(nc > 0 ? nc : 1);
+    metrics[3 * c] = cs / den;
+    metrics[3 * c + 1] = cm;
+    metrics[3 * c + 2] = as / den;
+    counts[2 * c] = fl;
+    // dominant winding = sign of the signed-area sum; folds disagree with it (zero areas count)
+    counts[2 * c + 1] = (chart_tile0[c + 1] == chart_tile0[c]) ? 0 : ((ss < 0.0) ? nge : nle);
 }
 
 // rectangularize (metrics.rs:147-227), chart c: q (2 per vertex) and ext (2 per chart); ladder[2k], [2k+1]
 // the cosine and sine of pi k / 71 as the host's libm gives them
-__global__ __launch_bounds__(BLK) void k_rectangularize(const int64_t* __restrict__ v_off,
-                                                        const double* __restrict__ uv,
-                                                        const int64_t* __restrict__ b_off,
-                                                        const int64_t* __restrict__ bnd,
-                                                        const double* __restrict__ ladder,
-                                                        double* __restrict__ q, double* __restrict__ ext) {
-    __shared__ double sh[WAVES];
-    const int c = blockIdx.x;
+// N maxima over the workgroup at once (one pair of barriers); every thread receives them in v
+template <int N>
+__device__ __forceinline__ void block_max_vec(double (&v)[N], double* sh /* [WAVES * N] */) {
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[(threadIdx.x >> 6) * N + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double m = sh[k];
+#pragma unroll
+        for (int w = 1; w < WAVES; ++w) m = fmax(m, sh[w * N + k]);
+        v[k] = m;
+    }
+}
+
+constexpr int LADDER_GROUP = 8;  // divides LADDER
+static_assert(LADDER % LADDER_GROUP == 0, "ladder groups");
+
+// rectangularize in six passes so a big chart spreads over the device:
+//   k_rect_scores  one wave per (chart, 8 ladder angles): the border's bounding-box score of each angle
+//   k_rect_pick    per chart: the chosen rotation (or none)
+//   k_rect_box     per vertex tile: the rotated coordinates' bounding box
+//   k_rect_finish  per chart: the box of its tiles -> minimum and extent (ext)
+//   k_rect_write   per vertex tile: q = (rotated - minimum) / max(extent, 1e-30)
+//   k_rect_snap    per border tile: border points onto the nearest side
+// Min / max are exact in any order, so every pass gives the reference's values bit for bit.
+struct RectChart {
+    double c, s;     // rotation (cos, sin); rot = 0: none
+    double mnu, mnv, eu, ev;
+    int rot, pad;
+};
+
+// the ladder's scores: one wave per (chart c, group g) takes angles g*LADDER_GROUP.. over c's whole border
+constexpr int LGROUPS = LADDER / LADDER_GROUP;
+__global__ __launch_bounds__(BLK) void k_rect_scores(int32_t nchart, const int64_t* __restrict__ v_off,
+                                                     const double* __restrict__ uv, const int64_t* __restrict__ b_off,
+                                                     const int64_t* __restrict__ bnd,
+                                                     const double* __restrict__ ladder, double* __restrict__ score,
+                                                     int* __restrict__ bad_chart) {
+    const int64_t pair = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (pair >= (int64_t)nchart * LGROUPS) return;  // whole waves; no workgroup barrier below
+    const int c = (int)(pair / LGROUPS), k0 = (int)(pair % LGROUPS) * LADDER_GROUP;
     const int64_t v0 = v_off[c], nv = v_off[c + 1] - v0;
     const int64_t b0 = b_off[c], nb = b_off[c + 1] - b0;
     const double* U = uv + 2 * v0;
-    double* Q = q + 2 * v0;
-    if (nv == 0) {
-        if (threadIdx.x == 0) ext[2 * c] = ext[2 * c + 1] = 0.0;
-        return;
+    double cs[LADDER_GROUP], sn[LADDER_GROUP];
+#pragma unroll
+    for (int g = 0; g < LADDER_GROUP; ++g) {
+        cs[g] = ladder[2 * (k0 + g)];
+        sn[g] = ladder[2 * (k0 + g) + 1];
     }
-    if (nb == 0) {
-        double mnu = INFINITY, mnv = INFINITY, mxu = -INFINITY, mxv = -INFINITY;
-        for (int64_t i = threadIdx.x; i < nv; i += BLK) {
-            mnu = fmin(mnu, U[2 * i]); mnv = fmin(mnv, U[2 * i + 1]);
-            mxu = fmax(mxu, U[2 * i]); mxv = fmax(mxv, U[2 * i + 1]);
-        }
-        mnu = block_min(mnu, sh); mnv = block_min(mnv, sh);
-        mxu = block_max(mxu, sh); mxv = block_max(mxv, sh);
-        const double eu = mxu - mnu, ev = mxv - mnv;
-        for (int64_t i = threadIdx.x; i < nv; i += BLK) {
-            Q[2 * i] = (U[2 * i] - mnu) / fmax(eu, 1e-30);
-            Q[2 * i + 1] = (U[2 * i + 1] - mnv) / fmax(ev, 1e-30);
-        }
-        if (threadIdx.x == 0) {
-            ext[2 * c] = eu;
-            ext[2 * c + 1] = ev;
-        }
-        return;
-    }
-    // the ladder: the first angle of smallest score (a later one replaces it only when strictly smaller)
-    double bc = 1.0, bs = 0.0, bscore = INFINITY;
-    bool have = false;
-    for (int k = 0; k < LADDER; ++k) {
-        const double cs = ladder[2 * k], sn = ladder[2 * k + 1];
-        double mn = INFINITY, mx = -INFINITY, mnv = INFINITY, mxv = -INFINITY;
-        for (int64_t j = threadIdx.x; j < nb; j += BLK) {
-            const int64_t bi = bnd[b0 + j];
-            const double pu = U[2 * bi], pv = U[2 * bi + 1];
-            const double qu = cs * pu + sn * pv;
-            const double qv = -sn * pu + cs * pv;
-            mn = fmin(mn, qu); mx = fmax(mx, qu);
-            mnv = fmin(mnv, qv); mxv = fmax(mxv, qv);
-        }
-        mn = block_min(mn, sh); mx = block_max(mx, sh);
-        mnv = block_min(mnv, sh); mxv = block_max(mxv, sh);
-        const double e0 = mx - mn, e1 = mxv - mnv;
-        const double score = sqrt(e0 * (e0 + 1e-30) + e1 * (e1 + 1e-30));
-        if (!(have && score >= bscore)) {  // every thread holds the same values
-            have = true;
-            bc = cs;
-            bs = sn;
-            bscore = score;
-        }
-    }
-    double mnu = INFINITY, mnv = INFINITY, mxu = -INFINITY, mxv = -INFINITY;
-    for (int64_t i = threadIdx.x; i < nv; i += BLK) {
-        const double pu = U[2 * i], pv = U[2 * i + 1];
-        const double qu = bc * pu + bs * pv, qv = -bs * pu + bc * pv;
-        mnu = fmin(mnu, qu); mnv = fmin(mnv, qv);
-        mxu = fmax(mxu, qu); mxv = fmax(mxv, qv);
-    }
-    mnu = block_min(mnu, sh); mnv = block_min(mnv, sh);
-    mxu = block_max(mxu, sh); mxv = block_max(mxv, sh);
-    const double eu = mxu - mnu, ev = mxv - mnv;
-    for (int64_t i = threadIdx.x; i < nv; i += BLK) {
-        const double pu = U[2 * i], pv = U[2 * i + 1];
-        const double qu = bc * pu + bs * pv, qv = -bs * pu + bc * pv;
-        Q[2 * i] = (qu - mnu) / fmax(eu, 1e-30);
-        Q[2 * i + 1] = (qv - mnv) / fmax(ev, 1e-30);
-    }
-    __syncthreads();  // the border snap reads what other threads wrote (border entries are distinct)
-    for (int64_t j = threadIdx.x; j < nb; j += BLK) {
+    double m[4 * LADDER_GROUP];  // per angle: -min u, max u, -min v, max v
+#pragma unroll
+    for (int g = 0; g < 4 * LADDER_GROUP; ++g) m[g] = -INFINITY;
+    int bad = 0;
+    for (int64_t j = lane; j < nb; j += 64) {
         const int64_t bi = bnd[b0 + j];
+        if (bi < 0 || bi >= nv) {  // reported, never read
+            bad = 1;
+            continue;
+        }
+        const double pu = U[2 * bi], pv = U[2 * bi + 1];
+#pragma unroll
+        for (int g = 0; g < LADDER_GROUP; ++g) {
+            const double qu = cs[g] * pu + sn[g] * pv;
+            const double qv = -sn[g] * pu + cs[g] * pv;
+            m[4 * g] = fmax(m[4 * g], -qu);
+            m[4 * g + 1] = fmax(m[4 * g + 1], qu);
+            m[4 * g + 2] = fmax(m[4 * g + 2], -qv);
+            m[4 * g + 3] = fmax(m[4 * g + 3], qv);
+        }
+    }
+    if (bad) bad_chart[c] = 1;
+#pragma unroll
+    for (int g = 0; g < 4 * LADDER_GROUP; ++g)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[g] = fmax(m[g], __shfl_xor(m[g], o, 64));
+    if (lane < LADDER_GROUP) {
+        double e0 = 0.0, e1 = 0.0;
+#pragma unroll
+        for (int g = 0; g < LADDER_GROUP; ++g)
+            if (g == lane) {
+                e0 = m[4 * g + 1] - (-m[4 * g]);
+                e1 = m[4 * g + 3] - (-m[4 * g + 2]);
+            }
+        score[(int64_t)c * LADDER + k0 + lane] = sqrt(e0 * (e0 + 1e-30) + e1 * (e1 + 1e-30));
+    }
+}
+
+// per chart: the first angle of smallest score, a later one replacing it only when strictly smaller
+__global__ __launch_bounds__(BLK) void k_rect_pick(int32_t nchart, const int64_t* __restrict__ b_off,
+                                                   const double* __restrict__ ladder,
+                                                   const double* __restrict__ score,
+                                                   const int* __restrict__ bad_chart, RectChart* __restrict__ rc,
+                                                   int* __restrict__ err) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= nchart) return;
+    const bool border = b_off[c + 1] > b_off[c];
+    const int bad = border ? bad_chart[c] : 0;
+    if (bad) atomicOr(err, 2);
+    RectChart r{};
+    r.c = 1.0;
+    r.s = 0.0;
+    if (border && !bad) {
+        double bscore = INFINITY;
+        bool have = false;
+        for (int k = 0; k < LADDER; ++k) {
+            const double sc = score[(int64_t)c * LADDER + k];
+            if (!(have && sc >= bscore)) {
+                have = true;
+                r.c = ladder[2 * k];
+                r.s = ladder[2 * k + 1];
+                bscore = sc;
+            }
+        }
+    }
+    r.rot = (border && !bad) ? 1 : 0;
+    r.pad = bad;
+    rc[c] = r;
+}
+
+// tiles: tile_chart, [tile_i0, tile_i1) of the chart's local vertex (or border-entry) indices
+__device__ __forceinline__ void rect_rotated(const RectChart& r, double pu, double pv, double* qu, double* qv) {
+    if (r.rot) {
+        *qu = r.c * pu + r.s * pv;
+        *qv = -r.s * pu + r.c * pv;
+    } else {  // without border the chart is only normalised (metrics.rs:152-172)
+        *qu = pu;
+        *qv = pv;
+    }
+}
+
+struct Box {
+    double nu, xu, nv, xv;  // -min u, max u, -min v, max v
+};
+
+__global__ __launch_bounds__(BLK) void k_rect_box(const int32_t* __restrict__ tile_chart,
+                                                  const int64_t* __restrict__ tile_i0,
+                                                  const int64_t* __restrict__ tile_i1,
+                                                  const int64_t* __restrict__ v_off, const double* __restrict__ uv,
+                                                  const RectChart* __restrict__ rc, Box* __restrict__ box) {
+    __shared__ double shv[WAVES * 4];
+    const int tile = blockIdx.x;
+    const int c = tile_chart[tile];
+    const RectChart r = rc[c];
+    const double* U = uv + 2 * v_off[c];
+    double m[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (!r.pad)
+        for (int64_t i = tile_i0[tile] + threadIdx.x; i < tile_i1[tile]; i += BLK) {
+            double qu, qv;
+            rect_rotated(r, U[2 * i], U[2 * i + 1], &qu, &qv);
+            m[0] = fmax(m[0], -qu);
+            m[1] = fmax(m[1], qu);
+            m[2] = fmax(m[2], -qv);
+            m[3] = fmax(m[3], qv);
+        }
+    block_max_vec<4>(m, shv);
+    if (threadIdx.x == 0) box[tile] = Box{m[0], m[1], m[2], m[3]};
+}
+
+__global__ __launch_bounds__(BLK) void k_rect_finish(int32_t nchart, const int32_t* __restrict__ chart_tile0,
+                                                     const Box* __restrict__ box, RectChart* __restrict__ rc,
+                                                     double* __restrict__ ext) {
+    const int c = blockIdx.x * BLK + threadIdx.x;
+    if (c >= nchart) return;
+    double nu = -INFINITY, xu = -INFINITY, nv = -INFINITY, xv = -INFINITY;
+    for (int t = chart_tile0[c]; t < chart_tile0[c + 1]; ++t) {
+        nu = fmax(nu, box[t].nu);
+        xu = fmax(xu, box[t].xu);
+        nv = fmax(nv, box[t].nv);
+        xv = fmax(xv, box[t].xv);
+    }
+    RectChart r = rc[c];
+    r.mnu = -nu;
+    r.mnv = -nv;
+    r.eu = xu - r.mnu;
+    r.ev = xv - r.mnv;
+    if (chart_tile0[c + 1] == chart_tile0[c] || r.pad) r.eu = r.ev = 0.0;  // empty chart (or a reported error)
+    rc[c] = r;
+    ext[2 * c] = r.eu;
+    ext[2 * c + 1] = r.ev;
+}
+
+__global__ __launch_bounds__(BLK) void k_rect_write(const int32_t* __restrict__ tile_chart,
+                                                    const int64_t* __restrict__ tile_i0,
+                                                    const int64_t* __restrict__ tile_i1,
+                                                    const int64_t* __restrict__ v_off, const double* __restrict__ uv,
+                                                    const RectChart* __restrict__ rc, double* __restrict__ q) {
+    const int tile = blockIdx.x;
+    const int c = tile_chart[tile];
+    const RectChart r = rc[c];
+    if (r.pad) return;
+    const double* U = uv + 2 * v_off[c];
+    double* Q = q + 2 * v_off[c];
+    for (int64_t i = tile_i0[tile] + threadIdx.x; i < tile_i1[tile]; i += BLK) {
+        double qu, qv;
+        rect_rotated(r, U[2 * i], U[2 * i + 1], &qu, &qv);
+        Q[2 * i] = (qu - r.mnu) / fmax(r.eu, 1e-30);
+        Q[2 * i + 1] = (qv - r.mnv) / fmax(r.ev, 1e-30);
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_rect_snap(const int32_t* __restrict__ tile_chart,
+                                                   const int64_t* __restrict__ tile_i0,
+                                                   const int64_t* __restrict__ tile_i1,
+                                                   const int64_t* __restrict__ v_off,
+                                                   const int64_t* __restrict__ b_off,
+                                                   const int64_t* __restrict__ bnd,
+                                                   const RectChart* __restrict__ rc, double* __restrict__ q) {
+    const int tile = blockIdx.x;
+    const int c = tile_chart[tile];
+    if (!rc[c].rot) return;
+    double* Q = q + 2 * v_off[c];
+    const int64_t* B = bnd + b_off[c];
+    for (int64_t j = tile_i0[tile] + threadIdx.x; j < tile_i1[tile]; j += BLK) {
+        const int64_t bi = B[j];
         const double pu = Q[2 * bi], pv = Q[2 * bi + 1];
         const double dx = fmin(pu, 1.0 - pu), dy = fmin(pv, 1.0 - pv);
         if (dx <= dy) Q[2 * bi] = (pu < 0.5) ? 0.0 : 1.0;
         else Q[2 * bi + 1] = (pv < 0.5) ? 0.0 : 1.0;
-    }
-    if (threadIdx.x == 0) {
-        ext[2 * c] = eu;
-        ext[2 * c + 1] = ev;
     }
 }
 
@@ -261,6 +440,216 @@ int check_charts(int32_t nchart, const int64_t* v_off, const int64_t* x_off, int
 
 }  // namespace
 
+namespace {
+
+// the ladder's angles, pi k / 71 (metrics.rs:176-179), through the host's libm like the reference
+void ladder_table(double* lad) {
+    for (int k = 0; k < LADDER; ++k) {
+        const double ang = 3.14159265358979323846 * (double)k / (double)(LADDER - 1);
+        lad[2 * k] = std::cos(ang);
+        lad[2 * k + 1] = std::sin(ang);
+    }
+}
+
+// calls share their device's flag, so they take turns
+std::mutex& aux_call_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+// per device, allocated on first use and kept: the ladder table and the error flag of the device-pointer entry
+// points (so a call allocates nothing after the first on its device)
+struct MetricsAux {
+    double* ladder = nullptr;
+    int* err = nullptr;
+    void* buf = nullptr;  // tile tables and parts, grown on demand
+    size_t cap = 0;
+};
+int aux_for_device(MetricsAux** out) {
+    static std::mutex mu;
+    static std::vector<MetricsAux> aux;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)aux.size() <= dev) aux.resize((size_t)dev + 1);
+    MetricsAux& a = aux[(size_t)dev];
+    if (!a.ladder) {
+        double lad[2 * LADDER];
+        ladder_table(lad);
+        void* p = nullptr;
+        HIPCHK(hipMalloc(&p, sizeof(lad) + 256));
+        HIPCHK(hipMemcpy(p, lad, sizeof(lad), hipMemcpyHostToDevice));
+        a.ladder = (double*)p;
+        a.err = (int*)((char*)p + sizeof(lad));
+    }
+    *out = &a;
+    return RUV_OK;
+}
+
+int read_err(int* derr, const char* what) {
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, derr, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return fail(RUV_ERR_BAD_DIMS, std::string(what) + ": an index lies outside its chart");
+    return RUV_OK;
+}
+
+}  // namespace
+
+int chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
+                      const uint32_t* tris, double* metrics, int64_t* counts) {
+    if (nchart < 0) return fail(RUV_ERR_INVALID, "metrics: negative chart count");
+    if (nchart == 0) return RUV_OK;
+    if (!v_off || !t_off || !metrics || !counts) return fail(RUV_ERR_INVALID, "metrics: null argument");
+    MetricsAux* aux = nullptr;
+    int rc = aux_for_device(&aux);
+    if (rc) return rc;
+    // tiles of at most TILE_TRIS triangles, none across charts (built from a host copy of t_off)
+    std::vector<int64_t> toff((size_t)nchart + 1);
+    HIPCHK(hipMemcpy(toff.data(), t_off, sizeof(int64_t) * toff.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> chart_tile0((size_t)nchart + 1, 0), tile_chart;
+    std::vector<int64_t> tile_t0, tile_t1;
+    for (int32_t c = 0; c < nchart; ++c) {
+        if (toff[c + 1] < toff[c]) return fail(RUV_ERR_BAD_DIMS, "metrics: offsets not monotone");
+        for (int64_t t = toff[c]; t < toff[c + 1]; t += TILE_TRIS) {
+            tile_chart.push_back(c);
+            tile_t0.push_back(t);
+            tile_t1.push_back(std::min<int64_t>(toff[c + 1], t + TILE_TRIS));
+        }
+        chart_tile0[(size_t)c + 1] = (int32_t)tile_chart.size();
+    }
+    const size_t ntile = tile_chart.size();
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_ct = up(sizeof(int32_t) * chart_tile0.size()), b_tc = up(sizeof(int32_t) * std::max<size_t>(ntile, 1));
+    const size_t b_t = up(sizeof(int64_t) * std::max<size_t>(ntile, 1)), b_p = up(sizeof(TilePart) * std::max<size_t>(ntile, 1));
+    const size_t need = b_ct + b_tc + 2 * b_t + b_p;
+    std::lock_guard<std::mutex> lock(aux_call_mutex());
+    if (aux->cap < need) {
+        if (aux->buf) HIPCHK(hipFree(aux->buf));
+        aux->buf = nullptr;
+        aux->cap = 0;
+        HIPCHK(hipMalloc(&aux->buf, need));
+        aux->cap = need;
+    }
+    char* base = (char*)aux->buf;
+    int32_t* d_ct = (int32_t*)base;
+    int32_t* d_tc = (int32_t*)(base + b_ct);
+    int64_t* d_t0 = (int64_t*)(base + b_ct + b_tc);
+    int64_t* d_t1 = (int64_t*)(base + b_ct + b_tc + b_t);
+    TilePart* d_p = (TilePart*)(base + b_ct + b_tc + 2 * b_t);
+    HIPCHK(hipMemcpyAsync(d_ct, chart_tile0.data(), sizeof(int32_t) * chart_tile0.size(), hipMemcpyHostToDevice, 0));
+    if (ntile) {
+        HIPCHK(hipMemcpyAsync(d_tc, tile_chart.data(), sizeof(int32_t) * ntile, hipMemcpyHostToDevice, 0));
+        HIPCHK(hipMemcpyAsync(d_t0, tile_t0.data(), sizeof(int64_t) * ntile, hipMemcpyHostToDevice, 0));
+        HIPCHK(hipMemcpyAsync(d_t1, tile_t1.data(), sizeof(int64_t) * ntile, hipMemcpyHostToDevice, 0));
+    }
+    HIPCHK(hipMemsetAsync(aux->err, 0, sizeof(int), 0));
+    if (ntile)
+        hipLaunchKernelGGL(k_distortion_tiles, dim3((unsigned)ntile), dim3(BLK), 0, 0, v_off, p3, uv,
+                           (const int32_t*)d_tc, (const int64_t*)d_t0, (const int64_t*)d_t1, tris, d_p, aux->err);
+    hipLaunchKernelGGL(k_distortion_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart,
+                       (const int32_t*)d_ct, (const TilePart*)d_p, metrics, (long long*)counts);
+    HIPCHK(hipGetLastError());
+    return read_err(aux->err, "metrics");
+}
+
+// tiles of at most `size` items of each chart from its offsets (host copy): chart_tile0 (nchart + 1), per tile
+// its chart and local range
+void make_tiles(int32_t nchart, const std::vector<int64_t>& off, int64_t size, std::vector<int32_t>* chart_tile0,
+                std::vector<int32_t>* tile_chart, std::vector<int64_t>* i0, std::vector<int64_t>* i1) {
+    chart_tile0->assign((size_t)nchart + 1, 0);
+    for (int32_t c = 0; c < nchart; ++c) {
+        for (int64_t t = 0; t < off[c + 1] - off[c]; t += size) {
+            tile_chart->push_back(c);
+            i0->push_back(t);
+            i1->push_back(std::min<int64_t>(off[c + 1] - off[c], t + size));
+        }
+        (*chart_tile0)[(size_t)c + 1] = (int32_t)tile_chart->size();
+    }
+}
+
+// carve the per-device growable buffer into the given byte sizes (256-aligned)
+int carve(MetricsAux* aux, const std::vector<size_t>& bytes, std::vector<char*>* ptrs) {
+    size_t need = 0;
+    for (size_t b : bytes) need += (std::max<size_t>(b, 1) + 255) & ~(size_t)255;
+    if (aux->cap < need) {
+        if (aux->buf) HIPCHK(hipFree(aux->buf));
+        aux->buf = nullptr;
+        aux->cap = 0;
+        HIPCHK(hipMalloc(&aux->buf, need));
+        aux->cap = need;
+    }
+    char* p = (char*)aux->buf;
+    ptrs->clear();
+    for (size_t b : bytes) {
+        ptrs->push_back(p);
+        p += (std::max<size_t>(b, 1) + 255) & ~(size_t)255;
+    }
+    return RUV_OK;
+}
+
+int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
+                       const int64_t* bnd, double* q, double* ext) {
+    if (nchart < 0) return fail(RUV_ERR_INVALID, "rectangularize: negative chart count");
+    if (nchart == 0) return RUV_OK;
+    if (!v_off || !b_off || !ext) return fail(RUV_ERR_INVALID, "rectangularize: null argument");
+    MetricsAux* aux = nullptr;
+    int rc = aux_for_device(&aux);
+    if (rc) return rc;
+    std::vector<int64_t> voff((size_t)nchart + 1), boff((size_t)nchart + 1);
+    HIPCHK(hipMemcpy(voff.data(), v_off, sizeof(int64_t) * voff.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(boff.data(), b_off, sizeof(int64_t) * boff.size(), hipMemcpyDeviceToHost));
+    for (int32_t c = 0; c < nchart; ++c)
+        if (voff[c + 1] < voff[c] || boff[c + 1] < boff[c]) return fail(RUV_ERR_BAD_DIMS, "rectangularize: offsets not monotone");
+    constexpr int64_t VT = 4096, BT = 4096;
+    std::vector<int32_t> vct0, vtc, bct0, btc;
+    std::vector<int64_t> vi0, vi1, bi0, bi1;
+    make_tiles(nchart, voff, VT, &vct0, &vtc, &vi0, &vi1);
+    make_tiles(nchart, boff, BT, &bct0, &btc, &bi0, &bi1);
+    const size_t nvt = vtc.size(), nbt = btc.size();
+    std::lock_guard<std::mutex> lock(aux_call_mutex());
+    std::vector<char*> P;
+    rc = carve(aux, {sizeof(RectChart) * (size_t)nchart, sizeof(int32_t) * vct0.size(), sizeof(int32_t) * nvt,
+                     sizeof(int64_t) * nvt, sizeof(int64_t) * nvt, sizeof(Box) * nvt, sizeof(int32_t) * nbt,
+                     sizeof(int64_t) * nbt, sizeof(int64_t) * nbt, sizeof(double) * LADDER * (size_t)nchart,
+                     sizeof(int) * (size_t)nchart}, &P);
+    if (rc) return rc;
+    double* d_score = (double*)P[9];
+    int* d_bad = (int*)P[10];
+    RectChart* d_rc = (RectChart*)P[0];
+    int32_t *d_vct0 = (int32_t*)P[1], *d_vtc = (int32_t*)P[2], *d_btc = (int32_t*)P[6];
+    int64_t *d_vi0 = (int64_t*)P[3], *d_vi1 = (int64_t*)P[4], *d_bi0 = (int64_t*)P[7], *d_bi1 = (int64_t*)P[8];
+    Box* d_box = (Box*)P[5];
+    auto h2d = [](void* d, const void* h, size_t b) {
+        return b ? hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, 0) : hipSuccess;
+    };
+    HIPCHK(h2d(d_vct0, vct0.data(), sizeof(int32_t) * vct0.size()));
+    HIPCHK(h2d(d_vtc, vtc.data(), sizeof(int32_t) * nvt));
+    HIPCHK(h2d(d_vi0, vi0.data(), sizeof(int64_t) * nvt));
+    HIPCHK(h2d(d_vi1, vi1.data(), sizeof(int64_t) * nvt));
+    HIPCHK(h2d(d_btc, btc.data(), sizeof(int32_t) * nbt));
+    HIPCHK(h2d(d_bi0, bi0.data(), sizeof(int64_t) * nbt));
+    HIPCHK(h2d(d_bi1, bi1.data(), sizeof(int64_t) * nbt));
+    HIPCHK(hipMemsetAsync(aux->err, 0, sizeof(int), 0));
+    HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int) * (size_t)nchart, 0));
+    hipLaunchKernelGGL(k_rect_scores, dim3((unsigned)(((int64_t)nchart * LGROUPS + WAVES - 1) / WAVES)), dim3(BLK), 0, 0,
+                       nchart, v_off, uv, b_off, bnd, (const double*)aux->ladder, d_score, d_bad);
+    hipLaunchKernelGGL(k_rect_pick, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart, b_off,
+                       (const double*)aux->ladder, (const double*)d_score, (const int*)d_bad, d_rc, aux->err);
+    if (nvt)
+        hipLaunchKernelGGL(k_rect_box, dim3((unsigned)nvt), dim3(BLK), 0, 0, (const int32_t*)d_vtc,
+                           (const int64_t*)d_vi0, (const int64_t*)d_vi1, v_off, uv, (const RectChart*)d_rc, d_box);
+    hipLaunchKernelGGL(k_rect_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart,
+                       (const int32_t*)d_vct0, (const Box*)d_box, d_rc, ext);
+    if (nvt)
+        hipLaunchKernelGGL(k_rect_write, dim3((unsigned)nvt), dim3(BLK), 0, 0, (const int32_t*)d_vtc,
+                           (const int64_t*)d_vi0, (const int64_t*)d_vi1, v_off, uv, (const RectChart*)d_rc, q);
+    if (nbt)
+        hipLaunchKernelGGL(k_rect_snap, dim3((unsigned)nbt), dim3(BLK), 0, 0, (const int32_t*)d_btc,
+                           (const int64_t*)d_bi0, (const int64_t*)d_bi1, v_off, b_off, bnd, (const RectChart*)d_rc, q);
+    HIPCHK(hipGetLastError());
+    return read_err(aux->err, "rectangularize");
+}
+
 int chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
                   const uint32_t* tris, double* metrics, int64_t* counts) {
     if (!v_off || !t_off || (nchart > 0 && (!metrics || !counts))) return fail(RUV_ERR_INVALID, "metrics: null argument");
@@ -275,24 +664,20 @@ int chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const 
             if ((int64_t)tris[k] >= n) return fail(RUV_ERR_BAD_DIMS, "metrics: triangle corner outside its chart");
     }
     DevSet D;
-    int64_t *dv = nullptr, *dt = nullptr;
+    int64_t *dv = nullptr, *dt = nullptr, *dc = nullptr;
     double *dp = nullptr, *du = nullptr, *dm = nullptr;
     uint32_t* dtri = nullptr;
-    long long* dc = nullptr;
     HIPCHK(D.put(&dv, v_off, (size_t)nchart + 1));
     HIPCHK(D.put(&dt, t_off, (size_t)nchart + 1));
     HIPCHK(D.put(&dp, p3, 3 * (size_t)nv));
     HIPCHK(D.put(&du, uv, 2 * (size_t)nv));
     HIPCHK(D.put(&dtri, tris, 3 * (size_t)nt));
     HIPCHK(D.put(&dm, (const double*)nullptr, 3 * (size_t)nchart));
-    HIPCHK(D.put(&dc, (const long long*)nullptr, 2 * (size_t)nchart));
-    hipLaunchKernelGGL(k_chart_distortion, dim3((unsigned)nchart), dim3(BLK), 0, 0, (const int64_t*)dv,
-                       (const double*)dp, (const double*)du, (const int64_t*)dt, (const uint32_t*)dtri, dm, dc);
-    HIPCHK(hipGetLastError());
+    HIPCHK(D.put(&dc, (const int64_t*)nullptr, 2 * (size_t)nchart));
+    rc = chart_metrics_dev(nchart, dv, dp, du, dt, dtri, dm, dc);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(metrics, dm, sizeof(double) * 3 * (size_t)nchart, hipMemcpyDeviceToHost));
-    std::vector<long long> hc(2 * (size_t)nchart);
-    HIPCHK(hipMemcpy(hc.data(), dc, sizeof(long long) * hc.size(), hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < hc.size(); ++k) counts[k] = (int64_t)hc[k];
+    HIPCHK(hipMemcpy(counts, dc, sizeof(int64_t) * 2 * (size_t)nchart, hipMemcpyDeviceToHost));
     return RUV_OK;
 }
 
@@ -314,26 +699,17 @@ int rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const
             seen[(size_t)bnd[j]] = 1;
         }
     }
-    // the ladder's angles, pi k / 71 (metrics.rs:176-179), through the host's libm like the reference
-    double lad[2 * LADDER];
-    for (int k = 0; k < LADDER; ++k) {
-        const double ang = 3.14159265358979323846 * (double)k / (double)(LADDER - 1);
-        lad[2 * k] = std::cos(ang);
-        lad[2 * k + 1] = std::sin(ang);
-    }
     DevSet D;
     int64_t *dv = nullptr, *db = nullptr, *dbn = nullptr;
-    double *du = nullptr, *dq = nullptr, *de = nullptr, *dl = nullptr;
+    double *du = nullptr, *dq = nullptr, *de = nullptr;
     HIPCHK(D.put(&dv, v_off, (size_t)nchart + 1));
     HIPCHK(D.put(&db, b_off, (size_t)nchart + 1));
     HIPCHK(D.put(&dbn, bnd, (size_t)nb));
     HIPCHK(D.put(&du, uv, 2 * (size_t)nv));
-    HIPCHK(D.put(&dl, (const double*)lad, 2 * (size_t)LADDER));
     HIPCHK(D.put(&dq, (const double*)nullptr, 2 * (size_t)nv));
     HIPCHK(D.put(&de, (const double*)nullptr, 2 * (size_t)nchart));
-    hipLaunchKernelGGL(k_rectangularize, dim3((unsigned)nchart), dim3(BLK), 0, 0, (const int64_t*)dv,
-                       (const double*)du, (const int64_t*)db, (const int64_t*)dbn, (const double*)dl, dq, de);
-    HIPCHK(hipGetLastError());
+    rc = rectangularize_dev(nchart, dv, du, db, dbn, dq, de);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(q, dq, sizeof(double) * 2 * (size_t)nv, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ext, de, sizeof(double) * 2 * (size_t)nchart, hipMemcpyDeviceToHost));
     return RUV_OK;

@@ -124,3 +124,53 @@ def test_errors_are_reported():
         M.rectangularize(np.zeros((3, 2)), [0, 1, 1])  # repeated border vertex
     with pytest.raises(RuvError):
         M.rectangularize(np.zeros((3, 2)), [0, 3])  # border vertex outside the chart
+
+
+def test_device_pointer_entry_points_match(orc):
+    import ctypes as C
+
+    import torch
+
+    from researchuv_amd import _lib
+    from researchuv_amd import metrics as M
+
+    rng = np.random.default_rng(1)
+    charts = []
+    for n in (5, 40, 300):
+        p = rng.normal(size=(n, 3))
+        t = rng.integers(0, n, size=(2 * n, 3)).astype(np.uint32)
+        uv = rng.normal(size=(n, 2))
+        charts.append((p, t, uv, np.arange(0, n, 2)))
+    host = M.chart_distortion_batch([(p, t, uv) for p, t, uv, _ in charts])
+    hrect = M.rectangularize_batch([(uv, b) for _, _, uv, b in charts])
+    off = lambda xs: torch.tensor(np.concatenate([[0], np.cumsum([len(x) for x in xs])]), dtype=torch.int64,  # noqa: E731
+                                  device="cuda")
+    v_off, t_off, b_off = off([c[0] for c in charts]), off([c[1] for c in charts]), off([c[3] for c in charts])
+    cat = lambda k, dt: torch.tensor(np.concatenate([c[k] for c in charts]), dtype=dt, device="cuda")  # noqa: E731
+    p3, uv = cat(0, torch.float64), cat(2, torch.float64)
+    tris = torch.tensor(np.concatenate([c[1] for c in charts]).view(np.int32), device="cuda")
+    bnd = cat(3, torch.int64)
+    n = len(charts)
+    met = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    cnt = torch.zeros((n, 2), dtype=torch.int64, device="cuda")
+    q = torch.zeros_like(uv)
+    ext = torch.zeros((n, 2), dtype=torch.float64, device="cuda")
+    V = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    L = M.lib()
+    _lib.check(L.ruv_chart_metrics_dev(n, V(v_off), V(p3), V(uv), V(t_off), V(tris), V(met), V(cnt)))
+    _lib.check(L.ruv_rectangularize_dev(n, V(v_off), V(uv), V(b_off), V(bnd), V(q), V(ext)))
+    m, k = met.cpu().numpy(), cnt.cpu().numpy()
+    for i, h in enumerate(host):
+        assert (m[i, 0], m[i, 1], m[i, 2], k[i, 0], k[i, 1]) == (h.conformal_mean, h.conformal_max, h.area_ratio_mean,
+                                                                 h.flips, h.folds)
+    np.testing.assert_array_equal(q.cpu().numpy(), np.concatenate([r[0] for r in hrect]))
+    np.testing.assert_array_equal(ext.cpu().numpy(), np.stack([r[1] for r in hrect]))
+    # an index outside its chart is skipped on the device and reported
+    bad = tris.clone()
+    bad[0] = 7
+    with pytest.raises(_lib.RuvError):
+        _lib.check(L.ruv_chart_metrics_dev(n, V(v_off), V(p3), V(uv), V(t_off), V(bad), V(met), V(cnt)))
+    badb = bnd.clone()
+    badb[0] = 99
+    with pytest.raises(_lib.RuvError):
+        _lib.check(L.ruv_rectangularize_dev(n, V(v_off), V(uv), V(b_off), V(badb), V(q), V(ext)))
