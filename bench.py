@@ -58,7 +58,7 @@ def host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline_batch(meshes, budget_verts=250_000, vmax=4096):
+def cpu_baseline_batch(meshes, budget_verts=750_000, vmax=4096):
     """The CPU oracle's worker-thread path (pipeline.rs:171-194 over exec.rs:28-60, each chart one sparse-LU
     factor+solve, sparse.rs:91-265) on the first patches of the batch with at most `vmax` vertices, up to
     `budget_verts` vertices in total (the LU of the largest patches would take minutes each)."""
@@ -546,7 +546,7 @@ def run_metrics(args):
         meshes=None, cpu_sample=(p3s, tris, uvs, bnds))
 
 
-def cpu_baseline_metrics(sample, budget=2_000_000):
+def cpu_baseline_metrics(sample, budget=200_000_000):
     """The oracle's chart_distortion + rectangularize, one thread, over the first charts up to `budget`
     triangles."""
     import oracle
