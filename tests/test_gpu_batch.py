@@ -308,3 +308,47 @@ def test_batch_amg_is_deterministic(monkeypatch):
         xs.append(x1)
         bt.close()
     np.testing.assert_array_equal(xs[0], xs[1])
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_batch_amg_degenerate_charts(mixed):
+    """The device AMG over charts of 0, 1, 2 and 3 rows, a permuted chart (32-bit columns on every level) and a
+    trailing empty chart: every chart against SuperLU."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from researchuv_amd import batch as B
+    from researchuv_amd import lscm, meshgen
+    from researchuv_amd.solver import Precond
+
+    say(f"AMG with degenerate charts mixed={mixed}")
+    pos, faces = meshgen.heightfield(120, 120, seed=8)
+    s = lscm.segment(*lscm.weld(pos, faces, 1e-12), 180.0).system(0)
+    n = s["nfree"]
+    L = sp.csr_matrix((s["l_vals"], s["l_cols"], s["l_ptr"]), shape=(n, n))
+    perm = np.random.default_rng(4).permutation(n)
+    Lp = L[perm][:, perm].tocsr()
+    rhs_p = s["rhs"].reshape(-1, 2)[perm]
+    blocks = [sp.csr_matrix((0, 0)), sp.csr_matrix([[4.0]]), sp.csr_matrix([[3.0, -1.0], [-1.0, 2.0]]),
+              sp.csr_matrix([[2.0, -1.0, 0.0], [-1.0, 2.0, -1.0], [0.0, -1.0, 2.0]]), Lp, sp.csr_matrix((0, 0))]
+    rng = np.random.default_rng(7)
+    rhs = [rng.normal(size=(b.shape[0], 2)) for b in blocks]
+    rhs[4] = rhs_p
+    A = sp.block_diag([b for b in blocks if b.shape[0]], format="csr")
+    A.sort_indices()
+    off = np.concatenate([[0], np.cumsum([b.shape[0] for b in blocks])])
+    b = np.concatenate([r.reshape(-1) for r in rhs])
+    bt = B.Batch.from_csr(A.indptr.astype(np.int64), A.indices, A.data, off, precond=Precond.AMG, mixed=mixed)
+    assert bt.precond_info()["levels"] >= 2
+    x, info, it, res = bt.solve(b, max_iter=20000, tol=1e-10)
+    for c, blk in enumerate(blocks):
+        m = blk.shape[0]
+        if m == 0:
+            assert it[c] == 0
+            continue
+        lu = spla.splu(blk.tocsc())
+        xr = np.stack([lu.solve(rhs[c][:, 0]), lu.solve(rhs[c][:, 1])], axis=1)
+        xc = x[2 * off[c]:2 * off[c + 1]].reshape(-1, 2)
+        err = np.linalg.norm(xc - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max()
+        say(f"  chart {c} ({m} rows): uv err {err:.2e}, iters {it[c]}")
+        assert err < 1e-6, c
