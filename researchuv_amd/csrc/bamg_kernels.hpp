@@ -714,9 +714,9 @@ __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, co
     st<T, R>(bnext, I, o);
 }
 
-// level >= 1: t = e1 = w D^-1 b + P z_next
+// level >= 1: t = e1 = w D^-1 b + alpha P z_next
 template <typename T, int R>
-__global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* __restrict__ znext) {
+__global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, const T* __restrict__ znext) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= L.n || !live_row(L, i)) return;
     const T di = w * L.dinv[i];
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, const T* _
     const Vr<T, R> zc = ld<T, R>(znext, L.agg[i]);
     Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c], zc.c[c]);
+    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c], alpha * zc.c[c]);
     st<T, R>(L.t, i, o);
 }
 
@@ -823,13 +823,13 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
     }
 }
 
-//   up, in two passes: e1 = w D^-1 r + P z1 into e (kc_prolong0), then z = e1 + w D^-1 (r - A e1) with the
+//   up, in two passes: e1 = w D^-1 r + alpha P z1 into e (kc_prolong0), then z = e1 + w D^-1 (r - A e1) with the
 //   neighbours' e1 read back (kc_post0), partial r.z of the tile into part_c[par]
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict__ act, int na,
-                                                   const T* __restrict__ dinv, T w, const T* __restrict__ r,
-                                                   const int32_t* __restrict__ agg, const T* __restrict__ z1,
-                                                   T* __restrict__ e) {
+                                                   const T* __restrict__ dinv, T w, T alpha,
+                                                   const T* __restrict__ r, const int32_t* __restrict__ agg,
+                                                   const T* __restrict__ z1, T* __restrict__ e) {
     const int slot = xcd_slot(na);
     if (slot < 0) return;
     const int t = act[slot];
@@ -841,7 +841,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict
         const Vr<T, R> zc = ld<T, R>(z1, agg[i]);
         Vr<T, R> o;
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) o.c[cc] = fma(di, ri.c[cc], zc.c[cc]);
+        for (int cc = 0; cc < R; ++cc) o.c[cc] = fma(di, ri.c[cc], alpha * zc.c[cc]);
         st<T, R>(e, i, o);
     }
 }
