@@ -290,6 +290,13 @@ int ruv_chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, co
  * normalisation (the pack rectangle, pipeline.rs:178). */
 int ruv_rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                        const int64_t* bnd, double* q, double* ext);
+/* The same two with every array a device pointer (inputs resident in HBM, results written there), launched on
+ * the null stream and synchronous.  Offsets are trusted; a triangle corner or border id outside its chart is
+ * skipped on the device and reported as RUV_ERR_BAD_DIMS; repeated border ids give an unspecified snap. */
+int ruv_chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                          const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
+int ruv_rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
+                           const int64_t* bnd, double* q, double* ext);
 
 #ifdef __cplusplus
 }
