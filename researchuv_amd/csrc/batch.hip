@@ -512,13 +512,16 @@ int time_step(BatchImpl& Q, int reps, double* sec) {
     return Q.ci == 0 ? time_step_kernel<double, 1, 0>(Q, reps, sec) : time_step_kernel<double, 1, 1>(Q, reps, sec);
 }
 
-// Pick slices-per-tile and the value-load hint by timing the step's SpMV kernel over the whole batch.
+// Pick slices-per-tile and the value-load hint by timing the step's SpMV kernel over the whole batch.  Both can
+// be pinned instead (RUV_BATCH_SLICES_PER_TILE = 1, 2, 4 or 8, RUV_BATCH_NONTEMPORAL = 0 or 1): a profiler that
+// serialises launches and reads counters distorts the timings, so its runs pin the choice of the timed run.
 int autotune(BatchImpl& Q) {
     const char* env = std::getenv("RUV_BATCH_SLICES_PER_TILE");
     if (env) {
         const int v = std::atoi(env);
         if (v == 1 || v == 2 || v == 4 || v == 8) {
-            Q.nt = 0;
+            const char* nt = std::getenv("RUV_BATCH_NONTEMPORAL");
+            Q.nt = (nt && std::atoi(nt) != 0) ? 1 : 0;
             return build_tiles(Q, v);
         }
     }

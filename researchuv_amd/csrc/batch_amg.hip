@@ -42,7 +42,8 @@ struct Bamg {
     // Jacobi weight of the smoothing steps: omega_max, or less where a level's Gershgorin bound g on D^-1 A
     // asks for it (omega <= 1.9 / g keeps each smoother an A-norm contraction, so the cycle stays SPD)
     double omega_max = 0.9;
-    // coarse-grid corrections enter scaled by alpha (over-correction; the cycle stays SPD for 0 < alpha < 2):
+    // coarse-grid corrections enter scaled by alpha (over-correction; the cycle stays SPD for any alpha > 0, since
+    // its error operator S (I - alpha P Bc P^T A) S has every eigenvalue below 1 when S is a strict A-contraction):
     // with aggregates of about two rows the plain correction is too short, and alpha = 1.2 cut the PCG
     // iterations by about a third on height fields (49 -> 34 on a 256^2 patch in a CPU model of the cycle)
     double alpha = 1.2;
@@ -374,7 +375,7 @@ int bamg_build(BatchImpl& Q) {
     // matching: coupling threshold relative to a row's strongest, handshake rounds (overridable for tuning)
     // and matchings per level: one (aggregates of about two rows) converges in about half the iterations of two
     // (aggregates of about four) on the heterogeneous weights of jittered meshes, for about 1.5x the coarse work
-    double theta = 0.1;
+    double theta = 0.25;
     int rounds = 4;
     if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));

@@ -49,7 +49,6 @@ __device__ __forceinline__ double tri_signed(double uu, double uv, double vu, do
 
 // block reductions in a fixed order
 __device__ __forceinline__ double block_max(double v, double* sh) { return block_max_all(v, sh); }
-__device__ __forceinline__ double block_min(double v, double* sh) { return -block_max_all(-v, sh); }
 __device__ __forceinline__ long long block_count(long long v, long long* sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -373,7 +373,7 @@ void apply_external(PlanImpl& P, const T* r, T* z, T* y, T* t, const SolveState*
 // PCG on the device.  On entry r holds the right-hand side (x0 = 0), x is overwritten.
 // tol_abs_from: tol relative to ||r0|| (tol absolute when ||r0|| <= 1e-300), as solver.rs:465-468.
 template <typename T, int R>
-int run_pcg(PlanImpl& P, const T* vals, const T* dinv, const DevTri<T>& tri, T* x, T* r, T* z, T* p, T* q,
+int run_pcg(PlanImpl& P, const T* vals, const T* dinv, T* x, T* r, T* z, T* p, T* q,
             T* y, T* t, uint64_t max_iter, double tol, uint64_t* iters_out, int* done_out) {
     const int G = P.G, n = P.n;
     hipStream_t s = P.stream;
@@ -502,7 +502,7 @@ int solve_fp64(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, d
     HIPCHK(hipEventRecord(P.ev0, s));
     uint64_t iters = 0;
     int done = 0;
-    int rc = run_pcg<double, R>(P, P.vals64, P.dinv64, P.tri64, d_x, P.r64, P.z64, P.p64, P.q64, P.y64, P.t64,
+    int rc = run_pcg<double, R>(P, P.vals64, P.dinv64, d_x, P.r64, P.z64, P.p64, P.q64, P.y64, P.t64,
                                 max_iter, tol, &iters, &done);
     if (rc) return rc;
     HIPCHK(hipEventRecord(P.ev1, s));
@@ -547,7 +547,7 @@ int solve_mixed(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, 
         uint64_t it = 0;
         int done = 0;
         double itol = std::max(P.inner_tol, 0.25 * target / std::max(std::sqrt(rr), 1e-300));
-        rc = run_pcg<float, R>(P, P.vals32, P.dinv32, P.tri32, P.x32, P.r32, P.z32, P.p32, P.q32, P.y32, P.t32,
+        rc = run_pcg<float, R>(P, P.vals32, P.dinv32, P.x32, P.r32, P.z32, P.p32, P.q32, P.y32, P.t32,
                                max_iter - total, itol, &it, &done);
         if (rc) return rc;
         total += it;
