@@ -580,7 +580,6 @@ def main():
     ap.add_argument("--batch-precond", default="AMG", choices=["JACOBI", "AMG"],
                     help="batch: preconditioner of the mixed batched solve (AMG is built on the device each step)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="batch: stream 1 GiB through one kernel first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()

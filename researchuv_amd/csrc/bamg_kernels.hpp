@@ -798,28 +798,34 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
     const int c = B.tile_chart[t];
     if (B.c_done[c] != B_RUNNING) return;
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
+    const auto g = [&](int64_t j) {
+        const T d = w * dinv[j];
+        Vr<T, R> x = ld<T, R>(r, j);
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) x.c[cc] *= d;
+        return x;
+    };
+    const auto fin = [&](int64_t i, const T (&acc)[R]) {
+        const Vr<T, R> ri = ld<T, R>(r, i);
+        Vr<T, R> o;
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) o.c[cc] = ri.c[cc] - acc[cc];
+        st<T, R>(t_out, i, o);
+    };
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
+        if (s + 1 < s1 && spmv_group<2, T, R, 0>(B, s, lo, hi, vals, g, fin)) {
+            ++s;
+            continue;
+        }
         const int n = B.sl_n[s];
         if ((int)threadIdx.x >= n) continue;
         const int64_t i = B.sl_row0[s] + threadIdx.x;
         T acc[R];
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals,
-                               [&](int64_t j) {
-                                   const T d = w * dinv[j];
-                                   Vr<T, R> x = ld<T, R>(r, j);
-#pragma unroll
-                                   for (int cc = 0; cc < R; ++cc) x.c[cc] *= d;
-                                   return x;
-                               },
-                               acc);
-        const Vr<T, R> ri = ld<T, R>(r, i);
-        Vr<T, R> o;
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) o.c[cc] = ri.c[cc] - acc[cc];
-        st<T, R>(t_out, i, o);
+        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals, g, acc);
+        fin(i, acc);
     }
 }
 
@@ -859,15 +865,8 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
     if (B.c_done[c] != B_RUNNING) return;
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     double arz = 0.0;
-    const int s1 = B.tile_s0[t + 1];
-    for (int s = B.tile_s0[t]; s < s1; ++s) {
-        const int n = B.sl_n[s];
-        if ((int)threadIdx.x >= n) continue;
-        const int64_t i = B.sl_row0[s] + threadIdx.x;
-        T acc[R];
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
-        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals, [&](int64_t j) { return ld<T, R>(e, j); }, acc);
+    const auto g = [&](int64_t j) { return ld<T, R>(e, j); };
+    const auto fin = [&](int64_t i, const T (&acc)[R]) {
         const T di = w * dinv[i];
         const Vr<T, R> ri = ld<T, R>(r, i);
         const Vr<T, R> ei = ld<T, R>(e, i);
@@ -878,6 +877,21 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
             arz += (double)ri.c[cc] * (double)o.c[cc];
         }
         st<T, R>(z, i, o);
+    };
+    const int s1 = B.tile_s0[t + 1];
+    for (int s = B.tile_s0[t]; s < s1; ++s) {
+        if (s + 1 < s1 && spmv_group<2, T, R, 0>(B, s, lo, hi, vals, g, fin)) {
+            ++s;
+            continue;
+        }
+        const int n = B.sl_n[s];
+        if ((int)threadIdx.x >= n) continue;
+        const int64_t i = B.sl_row0[s] + threadIdx.x;
+        T acc[R];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
+        row_slots<T, R, CI, 0>(B, s, n, i, lo, hi, vals, g, acc);
+        fin(i, acc);
     }
     const double srz = block_sum_all(arz, sh);
     if (threadIdx.x == 0) B.part_c[(int64_t)par * B.ntile + t] = srz;

@@ -226,6 +226,43 @@ __device__ __forceinline__ void row_slots(const BDev& B, int s, int n, int64_t i
     }
 }
 
+// P full inner stencil slices of one width at once (row threadIdx.x of each): P times the loads in flight per
+// thread before the first one is waited for.  v[p], d[p], i[p]: values, offsets and row of slice p.
+template <int P, int W, typename T, int R, int NT, typename V, typename G>
+__device__ __forceinline__ void stencil_group(const V* const (&v)[P], const int32_t* const (&d)[P],
+                                              const int32_t (&i)[P], G g, T (&acc)[P][R]) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        T u[P];
+        Vr<T, R> x[P];
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            u[q] = (T)ld_val(v[q] + k * SLICE, NT);
+            x[q] = g((int64_t)(i[q] + d[q][k]));
+        }
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) acc[q][cc] = fma(u[q], x[q].c[cc], acc[q][cc]);
+    }
+}
+
+// Width of slices s .. s + P - 1 when all are full stencil slices of that width whose every offset stays inside
+// the chart [lo, hi], else 0.
+template <int P>
+__device__ __forceinline__ int group_width(const BDev& B, int s, int64_t lo, int64_t hi) {
+    const int w = B.sl_w[s];
+    if (w <= 0) return 0;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        if (B.sl_coff[s + q] >= 0 || B.sl_n[s + q] != SLICE || B.sl_w[s + q] != w) return 0;
+        const int32_t* d = B.sl_dtab + (int64_t)(s + q) * SLOTS;
+        const int64_t r0 = B.sl_row0[s + q];
+        if (r0 + d[0] < lo || r0 + SLICE - 1 + d[w - 1] > hi) return 0;
+    }
+    return w;
+}
+
 // Opens step `iter` (parity par) of every live chart -- convergence on ||r||, max_iter, breakdown, beta from the
 // r.z partials -- then p' = z + beta p over the tile's rows.
 template <typename T, int R>
@@ -270,6 +307,37 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
     }
 }
 
+// Rows threadIdx.x of slices s .. s + P - 1 through stencil_group when group_width allows, each finished by fin;
+// false (nothing done) otherwise.
+template <int P, typename T, int R, int NT, typename V, typename G, typename F>
+__device__ __forceinline__ bool spmv_group(const BDev& B, int s, int64_t lo, int64_t hi, const V* __restrict__ vals,
+                                           G g, F fin) {
+    const int w = group_width<P>(B, s, lo, hi);
+    if (w == 0) return false;
+    const V* v[P];
+    const int32_t* d[P];
+    int32_t i[P];
+    T acc[P][R];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        v[q] = vals + B.sl_off[s + q] + threadIdx.x;
+        d[q] = B.sl_dtab + (int64_t)(s + q) * SLOTS;
+        i[q] = (int32_t)(B.sl_row0[s + q] + threadIdx.x);
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) acc[q][cc] = T(0);
+    }
+    switch (w) {
+#define RUV_SG(W) \
+    case W: stencil_group<P, W, T, R, NT>(v, d, i, g, acc); break;
+        RUV_SG(1) RUV_SG(2) RUV_SG(3) RUV_SG(4) RUV_SG(5) RUV_SG(6) RUV_SG(7) RUV_SG(8)
+#undef RUV_SG
+        default: break;
+    }
+#pragma unroll
+    for (int q = 0; q < P; ++q) fin((int64_t)i[q], acc[q]);
+    return true;
+}
+
 // q = A p' over one tile of a running chart, partial p'.q of the tile.  PROBE = 1 is the same code under its own
 // symbol for the schedule autotuner and the measurement hook, so a kernel trace of the solves is not mixed with
 // whole-batch timing launches.
@@ -285,15 +353,8 @@ __global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ a
     if (B.c_done[c] != B_RUNNING) return;
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     double acc = 0.0;
-    const int s1 = B.tile_s0[t + 1];
-    for (int s = B.tile_s0[t]; s < s1; ++s) {
-        const int n = B.sl_n[s];
-        if ((int)threadIdx.x >= n) continue;
-        const int64_t i = B.sl_row0[s] + threadIdx.x;
-        T aq[R];
-#pragma unroll
-        for (int cc = 0; cc < R; ++cc) aq[cc] = T(0);
-        row_slots<T, R, CI, NT>(B, s, n, i, lo, hi, vals, [&](int64_t j) { return ld<T, R>(p, j); }, aq);
+    const auto gp = [&](int64_t j) { return ld<T, R>(p, j); };
+    const auto finish = [&](int64_t i, const T (&aq)[R]) {
         const Vr<T, R> pi = ld<T, R>(p, i);
         Vr<T, R> qv;
 #pragma unroll
@@ -302,6 +363,21 @@ __global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ a
             acc += (double)pi.c[cc] * (double)aq[cc];
         }
         st<T, R>(q, i, qv);
+    };
+    const int s1 = B.tile_s0[t + 1];
+    for (int s = B.tile_s0[t]; s < s1; ++s) {
+        if (s + 1 < s1 && spmv_group<2, T, R, NT>(B, s, lo, hi, vals, gp, finish)) {
+            ++s;
+            continue;
+        }
+        const int n = B.sl_n[s];
+        if ((int)threadIdx.x >= n) continue;
+        const int64_t i = B.sl_row0[s] + threadIdx.x;
+        T aq[R];
+#pragma unroll
+        for (int cc = 0; cc < R; ++cc) aq[cc] = T(0);
+        row_slots<T, R, CI, NT>(B, s, n, i, lo, hi, vals, gp, aq);
+        finish(i, aq);
     }
     const double sum = block_sum_all(acc, sh);
     if (threadIdx.x == 0) B.part_a[t] = sum;
