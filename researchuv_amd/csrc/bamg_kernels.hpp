@@ -585,18 +585,177 @@ __global__ __launch_bounds__(BLK) void k_cast(int64_t n, const double* __restric
     if (i < n) b[i] = (T)a[i];
 }
 
+// ---------------------------------------------------------------- stored row order of the coarse levels
+// The cycle stores every level between the finest and the coarsest with the member rows of each next-level
+// aggregate contiguous, the aggregates in the next level's stored order (the coarsest keeps the order of its
+// construction; the order is laid down from the coarsest level towards the finest).  Then the restriction of a
+// level is a sum over runs of consecutive rows, done in the pass that forms the residual, and the prolongation
+// reads the coarse correction in ascending order.  A level's rows are cut into slices that start where an
+// aggregate starts: slice b holds rows srow[b] .. srow[b + 1], srow[b] = the first aggregate start at or after
+// b * SLICE, and sagg[b] is that aggregate.  Permutations map a row of the constructed order to its stored row;
+// a null permutation is the identity.  Charts keep their row ranges (aggregates never leave their chart).
+constexpr int MAX_SLICE_ROWS = 4 * SLICE;  // rows of one slice, bounded by the LDS of the restriction
+
+// sz[J] = member count of aggregate I, J = its stored index
+__global__ __launch_bounds__(BLK) void k_perm_sizes(int64_t nI, const int32_t* __restrict__ pos,
+                                                    const int64_t* __restrict__ mptr, int64_t* __restrict__ sz) {
+    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (I < nI) sz[pos ? pos[I] : I] = mptr[I + 1] - mptr[I];
+}
+
+// the k-th member m of aggregate I (stored at J) goes to stored row start[J] + k: rpos[m] = start[J] + k, and
+// mout[start[J] + k] = m (each skipped when null)
+__global__ __launch_bounds__(BLK) void k_perm_expand(int64_t nI, const int32_t* __restrict__ pos,
+                                                     const int64_t* __restrict__ mptr, const int64_t* __restrict__ mrows,
+                                                     const int64_t* __restrict__ start, int32_t* __restrict__ rpos,
+                                                     int32_t* __restrict__ mout) {
+    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (I >= nI) return;
+    const int64_t J = pos ? pos[I] : I, s0 = start[J], m0 = mptr[I], m1 = mptr[I + 1];
+    for (int64_t k = 0; k < m1 - m0; ++k) {
+        const int64_t m = mrows[m0 + k];
+        if (rpos) rpos[m] = (int32_t)(s0 + k);
+        if (mout) mout[s0 + k] = (int32_t)m;
+    }
+}
+
+// out[stored row of i] = stored index of agg[i]
+__global__ __launch_bounds__(BLK) void k_perm_agg(int64_t n, const int32_t* __restrict__ agg,
+                                                  const int32_t* __restrict__ rpos, const int32_t* __restrict__ npos,
+                                                  int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t a = agg[i];
+    out[rpos ? rpos[i] : i] = npos ? npos[a] : a;
+}
+
+__global__ __launch_bounds__(BLK) void k_perm_inverse(int64_t n, const int32_t* __restrict__ rpos,
+                                                      int32_t* __restrict__ ipos) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) ipos[rpos[i]] = (int32_t)i;
+}
+
+// first aggregate start at or after min(b * SLICE, n) (start ascends over nI + 1 entries, start[nI] = n)
+__device__ __forceinline__ int64_t first_start_at(int64_t target, int64_t nI, const int64_t* __restrict__ start) {
+    int64_t lo = 0, hi = nI;  // smallest J with start[J] >= target; start[nI] = n >= target
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (start[mid] >= target) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+// srow, sagg for b = 0 .. nsl; 1 into *tall when a slice holds more than MAX_SLICE_ROWS rows
+__global__ __launch_bounds__(BLK) void k_slice_rows(int64_t nsl, int64_t n, int64_t nI,
+                                                    const int64_t* __restrict__ start, int32_t* __restrict__ srow,
+                                                    int32_t* __restrict__ sagg, int32_t* __restrict__ tall) {
+    const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (b > nsl) return;
+    const int64_t J = first_start_at(min(b * SLICE, n), nI, start);
+    srow[b] = (int32_t)start[J];
+    sagg[b] = (int32_t)J;
+    if (b < nsl) {
+        const int64_t J1 = first_start_at(min((b + 1) * SLICE, n), nI, start);
+        if (start[J1] - start[J] > MAX_SLICE_ROWS) atomicOr(tall, 1);
+    }
+}
+
+// slice of stored row p: the largest b with srow[b] <= p (empty slices are passed over).  srow[b] >= b * SLICE,
+// so it is p / SLICE or, where p lies before that slice's first aggregate start, one of the slices before it
+// (srow[0] = 0 ends the walk).
+__device__ __forceinline__ int64_t slice_of(int64_t p, int64_t nsl, const int32_t* __restrict__ srow) {
+    int64_t b = min(p / SLICE, nsl - 1);
+    while (srow[b] > p) --b;
+    return b;
+}
+
+// one workgroup per slice: width (longest row) and size of the slice's ELL block; first and last chart
+__global__ __launch_bounds__(BLK) void k_pslice_sizes(const int32_t* __restrict__ srow, const int32_t* __restrict__ ipos,
+                                                      const int64_t* __restrict__ ptr, const int32_t* __restrict__ chart,
+                                                      int32_t* __restrict__ w, int64_t* __restrict__ sz,
+                                                      int2* __restrict__ sblk) {
+    __shared__ double sh[WAVES];
+    const int64_t b = blockIdx.x, r0 = srow[b], r1 = srow[b + 1];
+    double m = 0.0;
+    for (int64_t p = r0 + threadIdx.x; p < r1; p += BLK) {
+        const int64_t i = ipos ? ipos[p] : p;
+        m = fmax(m, (double)(ptr[i + 1] - ptr[i]));
+    }
+    m = block_max_all(m, sh);
+    if (threadIdx.x == 0) {
+        w[b] = (int32_t)m;
+        sz[b] = (int64_t)m * (r1 - r0);
+        sblk[b] = (r1 > r0) ? make_int2(chart[r0], chart[r1 - 1]) : make_int2(0, -1);
+    }
+}
+
+// 1 into *wide when some column is more than 32767 stored rows from its row
+__global__ __launch_bounds__(BLK) void k_poffsets_wide(int64_t n, const int64_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ cols, const int32_t* __restrict__ rpos,
+                                                       int32_t* __restrict__ wide) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = rpos ? rpos[i] : i;
+    for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+        const int64_t d = (int64_t)(rpos ? rpos[cols[a]] : cols[a]) - p;
+        if (d > 32767 || d < -32767) {
+            atomicOr(wide, 1);
+            return;
+        }
+    }
+}
+
+// the operator's rows into the ELL blocks of their stored slices, columns as stored rows (CI 0: 16-bit offsets
+// from the row; CI 1: 32-bit rows); padding: value 0, the row itself.  One thread per stored row p (constructed
+// row ipos[p]), so the stores of a slot are contiguous.
+template <typename T, int CI>
+__global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const int32_t* __restrict__ rpos,
+                                                   const int32_t* __restrict__ ipos, int64_t nsl,
+                                                   const int32_t* __restrict__ srow, const int64_t* __restrict__ off,
+                                                   const int32_t* __restrict__ w,
+                                                   typename CoarseVal<T>::type* __restrict__ ev, void* __restrict__ ec) {
+    const int64_t p = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (p >= n) return;
+    const int64_t i = ipos ? ipos[p] : p;
+    const int64_t b = slice_of(p, nsl, srow);
+    const int64_t ns = srow[b + 1] - srow[b];
+    const int64_t e0 = off[b] + (p - srow[b]);
+    int k = 0;
+    for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
+        const int64_t c = rpos ? rpos[A.cols[a]] : A.cols[a];
+        cput(ev + e0 + k * ns, A.vals[a]);
+        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)(c - p);
+        else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)c;
+    }
+    for (; k < w[b]; ++k) {
+        cput(ev + e0 + k * ns, 0.0);
+        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = 0;
+        else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)p;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_pcast_inv(int64_t n, const double* __restrict__ d,
+                                                   const int32_t* __restrict__ rpos, T* __restrict__ dinv) {
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (i < n) dinv[rpos ? rpos[i] : i] = (T)(1.0 / d[i]);
+}
+
 // ---------------------------------------------------------------- cycle
 template <typename T>
 struct CycLevel {
     int64_t n;
-    const int64_t* eoff;     // sliced ELL of the level operator (levels >= 1)
+    const int64_t* eoff;     // sliced ELL of the level operator (levels between the finest and the coarsest)
     const int32_t* ew;
     const void* ecols;       // int16 offsets (CI 0) or int32 columns (CI 1)
     const typename CoarseVal<T>::type* evals;
+    const int32_t* srow;     // slice row starts (aligned to aggregate starts)
+    const int32_t* sagg;     // first next-level aggregate of every slice
+    const int2* sblk;        // per slice: first and last chart
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
-    const int32_t* mptr;     // next level's aggregates -> member rows here
-    const int32_t* mrows;
+    const int32_t* mptr;     // next level's aggregates -> member rows here (levels >= 1: first stored row)
+    const int32_t* mrows;    // (the finest level: the member rows)
     T* b;                    // right-hand side on this level
     T* z;                    // correction on this level
     T* t;                    // work vector on this level
@@ -610,7 +769,7 @@ struct CycLevel {
 // Decided per workgroup of BLK rows from its first and last chart (charts are contiguous on every level): all
 // stopped (the workgroup leaves), all running (no per-row test), or mixed (each row tests its own chart).
 __device__ __forceinline__ int block_state(const int2* __restrict__ blk, const int32_t* __restrict__ done) {
-    const int2 b = blk[blockIdx.x];
+    const int2 b = blk[blockIdx.x];  // (first, last) chart of this workgroup's rows; last < first: no rows
     int run = 0;
     for (int c = b.x; c <= b.y; ++c) run += (done[c] == B_RUNNING) ? 1 : 0;
     return run == 0 ? 0 : (run == b.y - b.x + 1 ? 1 : 2);
@@ -621,16 +780,11 @@ __device__ __forceinline__ bool live_row(const CycLevel<T>& L, int64_t i) {
     return bs == 1 || (bs == 2 && L.done[L.chart[i]] == B_RUNNING);
 }
 
-// slice, lane, slice height of row i of a level's sliced ELL
+// first entry, slice height and width of a row of a level's sliced ELL
 struct EllRow {
     int64_t e0, ns;
     int w;
 };
-template <typename T>
-__device__ __forceinline__ EllRow ell_row(const CycLevel<T>& L, int64_t i) {
-    const int64_t s = i / SLICE;
-    return EllRow{L.eoff[s] + (i - s * SLICE), min((int64_t)SLICE, L.n - s * SLICE), L.ew[s]};
-}
 
 // acc[c] += sum over the ELL row of a level operator of a_ij * g(j)[c], the slot loop unrolled for the slice
 // width (a switch, uniform over the workgroup) as in row_slots
@@ -666,29 +820,56 @@ __device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r,
     }
 }
 
-// level >= 1: t = b - A (w D^-1 b)
+// Levels between the finest and the coarsest, one workgroup per slice: t = b - A (w D^-1 b) over the slice's rows
+// (into LDS), then b_next[J] = the sum of t over the members of every next-level aggregate J starting in the slice
+// (its members are the consecutive rows mptr[J] .. mptr[J + 1], all in the slice), in member order.
 template <typename T, int R, int CI>
-__global__ __launch_bounds__(BLK) void kc_resid(CycLevel<T> L, T w) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n || !live_row(L, i)) return;
-    const EllRow e = ell_row(L, i);
-    T s[R];
+__global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict__ bnext) {
+    __shared__ T tl[MAX_SLICE_ROWS * R];
+    const int bs = block_state(L.sblk, L.done);
+    if (bs == 0) return;
+    const int64_t sb = blockIdx.x;
+    const int64_t r0 = L.srow[sb], h = L.srow[sb + 1] - r0;
+    const int wd = L.ew[sb];
+    const int64_t e0 = L.eoff[sb];
+    for (int64_t r = threadIdx.x; r < h; r += BLK) {
+        const int64_t i = r0 + r;
+        Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) s[c] = T(0);
-    ell_apply<T, R, CI>(L, e, i,
-                        [&](int64_t j) {
-                            const T d = w * L.dinv[j];
-                            Vr<T, R> x = ld<T, R>(L.b, j);
+        for (int c = 0; c < R; ++c) o.c[c] = T(0);
+        if (bs == 1 || L.done[L.chart[i]] == B_RUNNING) {
+            T acc[R];
 #pragma unroll
-                            for (int c = 0; c < R; ++c) x.c[c] *= d;
-                            return x;
-                        },
-                        s);
-    const Vr<T, R> bi = ld<T, R>(L.b, i);
-    Vr<T, R> o;
+            for (int c = 0; c < R; ++c) acc[c] = T(0);
+            ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i,
+                                [&](int64_t j) {
+                                    const T d = w * L.dinv[j];
+                                    Vr<T, R> x = ld<T, R>(L.b, j);
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = bi.c[c] - s[c];
-    st<T, R>(L.t, i, o);
+                                    for (int c = 0; c < R; ++c) x.c[c] *= d;
+                                    return x;
+                                },
+                                acc);
+            const Vr<T, R> bi = ld<T, R>(L.b, i);
+#pragma unroll
+            for (int c = 0; c < R; ++c) o.c[c] = bi.c[c] - acc[c];
+        }
+#pragma unroll
+        for (int c = 0; c < R; ++c) tl[r * R + c] = o.c[c];
+    }
+    __syncthreads();
+    const int64_t J0 = L.sagg[sb], J1 = L.sagg[sb + 1];
+    for (int64_t J = J0 + threadIdx.x; J < J1; J += BLK) {
+        const int64_t m0 = L.mptr[J], m1 = L.mptr[J + 1];
+        if (bs == 2 && L.done[L.chart[m0]] != B_RUNNING) continue;
+        Vr<T, R> o;
+#pragma unroll
+        for (int c = 0; c < R; ++c) o.c[c] = T(0);
+        for (int64_t m = m0; m < m1; ++m)
+#pragma unroll
+            for (int c = 0; c < R; ++c) o.c[c] += tl[(m - r0) * R + c];
+        st<T, R>(bnext, J, o);
+    }
 }
 
 // per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t
@@ -728,23 +909,30 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, c
     st<T, R>(L.t, i, o);
 }
 
-// level >= 1: z = e1 + w D^-1 (b - A e1), e1 in t
+// levels between the finest and the coarsest, one workgroup per slice: z = e1 + w D^-1 (b - A e1), e1 in t
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n || !live_row(L, i)) return;
-    const EllRow e = ell_row(L, i);
-    T s[R];
+    const int bs = block_state(L.sblk, L.done);
+    if (bs == 0) return;
+    const int64_t sb = blockIdx.x;
+    const int64_t r0 = L.srow[sb], h = L.srow[sb + 1] - r0;
+    const int wd = L.ew[sb];
+    const int64_t e0 = L.eoff[sb];
+    for (int64_t r = threadIdx.x; r < h; r += BLK) {
+        const int64_t i = r0 + r;
+        if (bs == 2 && L.done[L.chart[i]] != B_RUNNING) continue;
+        T s[R];
 #pragma unroll
-    for (int c = 0; c < R; ++c) s[c] = T(0);
-    ell_apply<T, R, CI>(L, e, i, [&](int64_t j) { return ld<T, R>(L.t, j); }, s);
-    const T di = w * L.dinv[i];
-    const Vr<T, R> bi = ld<T, R>(L.b, i);
-    const Vr<T, R> ei = ld<T, R>(L.t, i);
-    Vr<T, R> o;
+        for (int c = 0; c < R; ++c) s[c] = T(0);
+        ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i, [&](int64_t j) { return ld<T, R>(L.t, j); }, s);
+        const T di = w * L.dinv[i];
+        const Vr<T, R> bi = ld<T, R>(L.b, i);
+        const Vr<T, R> ei = ld<T, R>(L.t, i);
+        Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c] - s[c], ei.c[c]);
-    st<T, R>(L.z, i, o);
+        for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c] - s[c], ei.c[c]);
+        st<T, R>(L.z, i, o);
+    }
 }
 
 // coarsest level: z = D^-1 b (exact for a chart coarsened to one row)

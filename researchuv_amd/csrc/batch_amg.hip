@@ -15,16 +15,19 @@
 namespace ruv {
 
 struct BamgLevel {
-    int64_t n = 0, nell = 0;
-    int64_t* eoff = nullptr;  // sliced ELL of the operator (levels >= 1)
+    int64_t n = 0, nell = 0, nsl = 0;
+    int64_t* eoff = nullptr;  // sliced ELL of the operator (levels between the finest and the coarsest)
     int32_t* ew = nullptr;
+    int32_t* srow = nullptr;  // slice row starts, at aggregate starts
+    int32_t* sagg = nullptr;  // first next-level aggregate of every slice
+    int2* sblk = nullptr;     // per slice: first and last chart
     void* ecols = nullptr;
     int ci = 1;               // 0: 16-bit column offsets
     void* evals = nullptr;    // T
     void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal)
     int32_t* agg = nullptr;   // row -> next-level aggregate
-    int32_t* mptr = nullptr;  // next level's aggregates -> member rows here
-    int32_t* mrows = nullptr;
+    int32_t* mptr = nullptr;  // next level's aggregates -> member rows here (levels >= 1: first stored row)
+    int32_t* mrows = nullptr; // (the finest level: the member rows)
     void* b = nullptr;
     void* z = nullptr;
     void* t = nullptr;
@@ -390,6 +393,19 @@ int bamg_build(BatchImpl& Q) {
         return m;
     };
     const int64_t n0 = Q.nrows;
+    // Bottom-up: the matchings and Galerkin operators, every level in the order of its construction.
+    struct NatLevel {
+        int64_t n = 0;
+        SetupCsr A{};
+        const double* diag = nullptr;     // levels >= 1 (level 0: the batch's Jacobi diagonal)
+        const int32_t* group = nullptr;   // row -> chart
+        int32_t* agg = nullptr;           // row -> next-level aggregate (null on the coarsest level)
+        int64_t* mptr = nullptr;          // next level's aggregates -> member rows (ascending)
+        int64_t* mrows = nullptr;
+        double omega = 2.0 / 3.0;
+        std::unique_ptr<Scratch> own;     // the level's operator (levels >= 1) and matching
+    };
+    std::vector<NatLevel> nat;
     const int32_t* group = Q.amg_group;
     SetupCsr cur{n0, Q.amg_ptr, Q.amg_cols, Q.amg_vals};
     const double* cur_diag = nullptr;
@@ -397,90 +413,16 @@ int bamg_build(BatchImpl& Q) {
     std::unique_ptr<Scratch> keep(new Scratch(&Q.pool));  // owns the current level's setup CSR (levels >= 1)
     const int max_levels = 40;
     for (int l = 0;; ++l) {
-        BamgLevel L;
-        L.n = cur.n;
-        L.chart = group;
-        {
-            const int64_t nb = (cur.n + BLK - 1) / BLK;
-            HIPCHK(A->alloc(&L.blk, nb));
-            hipLaunchKernelGGL(k_block_charts, rows_grid(nb), dim3(BLK), 0, s, cur.n, group, L.blk);
-            HIPCHK(hipGetLastError());
-        }
-        // cycle data of this level
-        if (l == 0) {
-            L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
-            if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build)
-                Bf16* e16 = nullptr;
-                HIPCHK(A->alloc(&e16, (size_t)Q.nell));
-                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
-                HIPCHK(hipGetLastError());
-                A->ev0 = e16;
-            } else {
-                A->ev0 = Q.ev64;
-            }
-            char* t = nullptr;
-            HIPCHK(A->alloc(&t, cur.n * R * sv));
-            HIPCHK(hipMemsetAsync(t, 0, cur.n * R * sv, s));  // rows of converged charts keep finite values
-            L.t = t;
-        } else {
-            // sliced ELL of the level operator
-            const int64_t nsl = (cur.n + SLICE - 1) / SLICE;
-            int64_t* sz = nullptr;
-            Scratch T(&Q.pool);
-            HIPCHK(T.alloc(&sz, nsl));
-            HIPCHK(A->alloc(&L.ew, nsl));
-            HIPCHK(A->alloc(&L.eoff, nsl + 1));
-            hipLaunchKernelGGL(k_slice_sizes, rows_grid(nsl), dim3(BLK), 0, s, cur.n, cur.ptr, L.ew, sz);
-            HIPCHK(hipGetLastError());
-            int rc = scan(s, nsl, sz, L.eoff, T);
-            if (rc) return rc;
-            HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            int32_t* wide = nullptr;
-            HIPCHK(T.alloc(&wide, 1));
-            HIPCHK(hipMemsetAsync(wide, 0, sizeof(int32_t), s));
-            hipLaunchKernelGGL(k_offsets_wide, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur.ptr, cur.cols, wide);
-            int32_t hw = 0;
-            HIPCHK(hipMemcpyAsync(&hw, wide, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            L.ci = hw ? 1 : 0;
-            char *v = nullptr, *d = nullptr, *ec = nullptr;
-            HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
-            L.ecols = ec;
-            HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
-            HIPCHK(A->alloc(&d, cur.n * sv));
-            auto fill = [&](auto tv) {
-                using TT = decltype(tv);
-                using VT = typename CoarseVal<TT>::type;
-                if (L.ci)
-                    hipLaunchKernelGGL((k_ell_fill<TT, 1>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
-                else
-                    hipLaunchKernelGGL((k_ell_fill<TT, 0>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur,
-                                       (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
-                hipLaunchKernelGGL((k_cast_inv<TT>), rows_grid(cur.n), dim3(BLK), 0, s, cur.n, cur_diag, (TT*)d);
-            };
-            if (f32) fill(0.0f);
-            else fill(0.0);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipStreamSynchronize(s));
-            L.evals = v;
-            L.dinv = d;
-            char *b = nullptr, *z = nullptr, *t = nullptr;
-            HIPCHK(A->alloc(&b, cur.n * R * sv));
-            HIPCHK(A->alloc(&z, cur.n * R * sv));
-            HIPCHK(A->alloc(&t, cur.n * R * sv));
-            HIPCHK(hipMemsetAsync(b, 0, cur.n * R * sv, s));
-            HIPCHK(hipMemsetAsync(z, 0, cur.n * R * sv, s));
-            HIPCHK(hipMemsetAsync(t, 0, cur.n * R * sv, s));
-            L.b = b;
-            L.z = z;
-            L.t = t;
-        }
+        NatLevel N;
+        N.n = cur.n;
+        N.A = cur;
+        N.diag = cur_diag;
+        N.group = group;
         const bool last = (l + 1 >= max_levels) ||
                           (l > 0 && (cur.n <= (int64_t)Q.nchart || max_chart_rows(crow) <= dense_rows));
         if (last) {
-            A->lv.push_back(L);
+            N.own = std::move(keep);
+            nat.push_back(std::move(N));
             break;
         }
         // one matching makes the next level (aggregates of about two rows: a second matching of the Galerkin
@@ -490,20 +432,18 @@ int bamg_build(BatchImpl& Q) {
         int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
         if (rc) return rc;
         const Matched& MF = M1;
-        L.omega = (M1.gersh > 0.0) ? std::min(A->omega_max, 1.9 / M1.gersh) : A->omega_max;
+        N.omega = (M1.gersh > 0.0) ? std::min(A->omega_max, 1.9 / M1.gersh) : A->omega_max;
         if (l > 0 && MF.nc > (int64_t)(0.9 * (double)cur.n)) {  // coarsening stalled: this level is the coarsest
-            A->lv.push_back(L);
+            N.own = std::move(keep);
+            nat.push_back(std::move(N));
             break;
         }
-        hand_over(*S1, M1.agg, A->mem);  // the matching's aggregate map serves the cycle
-        L.agg = M1.agg;
-        // the cycle's member lists in 32 bits (a level's rows are below 2^31)
-        HIPCHK(A->alloc(&L.mptr, MF.nc + 1));
-        HIPCHK(A->alloc(&L.mrows, cur.n));
-        hipLaunchKernelGGL(k_narrow, rows_grid(MF.nc + 1), dim3(BLK), 0, s, MF.nc + 1, (const int64_t*)M1.mptr,
-                           L.mptr);
-        hipLaunchKernelGGL(k_narrow, rows_grid(cur.n), dim3(BLK), 0, s, cur.n, (const int64_t*)M1.mrows, L.mrows);
-        HIPCHK(hipGetLastError());
+        N.agg = M1.agg;
+        N.mptr = M1.mptr;
+        N.mrows = M1.mrows;
+        N.own = std::move(keep);
+        for (const void* p : {(const void*)M1.agg, (const void*)M1.mptr, (const void*)M1.mrows})
+            hand_over(*S1, (void*)p, N.own->mem);
         // chart offsets on the next level
         {
             std::vector<int64_t> at(crow.begin(), crow.end());
@@ -518,7 +458,7 @@ int bamg_build(BatchImpl& Q) {
             HIPCHK(hipStreamSynchronize(s));
             crow.swap(out);
         }
-        A->lv.push_back(L);
+        nat.push_back(std::move(N));
         // the next level's operator (kept), its diagonal, and chart groups
         std::unique_ptr<Scratch> nk(new Scratch(&Q.pool));
         for (const void* p : {(const void*)MF.C.ptr, (const void*)MF.C.cols, (const void*)MF.C.vals,
@@ -536,13 +476,173 @@ int bamg_build(BatchImpl& Q) {
         cur = MF.C;
         cur_diag = MF.cdiag;
         group = ngroup;
-        keep = std::move(nk);  // frees the previous level's setup copy
+        keep = std::move(nk);
         S1.reset();
     }
-    // finest level's aggregate map and members live on; level 1 must exist
-    if (A->lv.size() < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
+    const size_t nl = nat.size();
+    if (nl < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
+    A->lv.resize(nl);
+    for (size_t l = 0; l < nl; ++l) {
+        BamgLevel& L = A->lv[l];
+        L.n = nat[l].n;
+        L.chart = nat[l].group;
+        L.omega = nat[l].omega;
+        const int64_t nb = (L.n + BLK - 1) / BLK;
+        HIPCHK(A->alloc(&L.blk, nb));
+        if (nb > 0)
+            hipLaunchKernelGGL(k_block_charts, rows_grid(nb), dim3(BLK), 0, s, L.n, L.chart, L.blk);
+        HIPCHK(hipGetLastError());
+    }
+    // Top-down: the stored order of every level between the finest and the coarsest, and the cycle's data.
+    Scratch P(&Q.pool);
+    std::vector<int32_t*> pos(nl, nullptr);  // constructed row -> stored row (null: the identity)
+    auto alloc_vectors = [&](BamgLevel& L) -> int {
+        char *b = nullptr, *z = nullptr, *t = nullptr;
+        const size_t bytes = (size_t)L.n * R * sv;
+        HIPCHK(A->alloc(&b, bytes));
+        HIPCHK(A->alloc(&z, bytes));
+        HIPCHK(A->alloc(&t, bytes));
+        HIPCHK(hipMemsetAsync(b, 0, bytes, s));
+        HIPCHK(hipMemsetAsync(z, 0, bytes, s));
+        HIPCHK(hipMemsetAsync(t, 0, bytes, s));
+        L.b = b;
+        L.z = z;
+        L.t = t;
+        return RUV_OK;
+    };
+    // member offsets, in the next level's stored order, of the aggregates of constructed level l
+    auto stored_starts = [&](size_t l, int64_t** start) -> int {
+        const int64_t nI = nat[l + 1].n;
+        int64_t* sz = nullptr;
+        HIPCHK(P.alloc(&sz, nI));
+        HIPCHK(P.alloc(start, nI + 1));
+        hipLaunchKernelGGL(k_perm_sizes, rows_grid(nI), dim3(BLK), 0, s, nI, (const int32_t*)pos[l + 1],
+                           (const int64_t*)nat[l].mptr, sz);
+        HIPCHK(hipGetLastError());
+        return scan(s, nI, sz, *start, P);
+    };
+    {  // coarsest: constructed order
+        BamgLevel& L = A->lv[nl - 1];
+        const NatLevel& N = nat[nl - 1];
+        char* d = nullptr;
+        HIPCHK(A->alloc(&d, (size_t)L.n * sv));
+        if (f32) hipLaunchKernelGGL((k_cast_inv<float>), rows_grid(L.n), dim3(BLK), 0, s, L.n, N.diag, (float*)d);
+        else hipLaunchKernelGGL((k_cast_inv<double>), rows_grid(L.n), dim3(BLK), 0, s, L.n, N.diag, (double*)d);
+        HIPCHK(hipGetLastError());
+        L.dinv = d;
+        int rc = alloc_vectors(L);
+        if (rc) return rc;
+    }
+    for (size_t l = nl - 1; l-- > 1;) {
+        BamgLevel& L = A->lv[l];
+        const NatLevel& N = nat[l];
+        const int64_t n = L.n, nI = nat[l + 1].n;
+        int64_t* start = nullptr;
+        int rc = stored_starts(l, &start);
+        if (rc) return rc;
+        int32_t *rpos = nullptr, *ipos = nullptr, *flags = nullptr;
+        HIPCHK(P.alloc(&rpos, n));
+        HIPCHK(P.alloc(&ipos, n));
+        HIPCHK(P.alloc(&flags, 2));
+        HIPCHK(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s));
+        hipLaunchKernelGGL(k_perm_expand, rows_grid(nI), dim3(BLK), 0, s, nI, (const int32_t*)pos[l + 1],
+                           (const int64_t*)N.mptr, (const int64_t*)N.mrows, (const int64_t*)start, rpos,
+                           (int32_t*)nullptr);
+        hipLaunchKernelGGL(k_perm_inverse, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)rpos, ipos);
+        pos[l] = rpos;
+        HIPCHK(A->alloc(&L.agg, n));
+        hipLaunchKernelGGL(k_perm_agg, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)N.agg, (const int32_t*)rpos,
+                           (const int32_t*)pos[l + 1], L.agg);
+        HIPCHK(A->alloc(&L.mptr, nI + 1));
+        hipLaunchKernelGGL(k_narrow, rows_grid(nI + 1), dim3(BLK), 0, s, nI + 1, (const int64_t*)start, L.mptr);
+        // slices aligned to aggregate starts, their ELL blocks
+        L.nsl = (n + SLICE - 1) / SLICE;
+        const int64_t nsl = L.nsl;
+        int64_t* sz = nullptr;
+        HIPCHK(A->alloc(&L.srow, nsl + 1));
+        HIPCHK(A->alloc(&L.sagg, nsl + 1));
+        HIPCHK(A->alloc(&L.sblk, nsl));
+        HIPCHK(A->alloc(&L.ew, nsl));
+        HIPCHK(A->alloc(&L.eoff, nsl + 1));
+        HIPCHK(P.alloc(&sz, nsl));
+        hipLaunchKernelGGL(k_slice_rows, rows_grid(nsl + 1), dim3(BLK), 0, s, nsl, n, nI, (const int64_t*)start,
+                           L.srow, L.sagg, flags);
+        if (nsl > 0)
+            hipLaunchKernelGGL(k_pslice_sizes, dim3((unsigned)nsl), dim3(BLK), 0, s, (const int32_t*)L.srow,
+                               (const int32_t*)ipos, N.A.ptr, N.group, L.ew, sz, L.sblk);
+        hipLaunchKernelGGL(k_poffsets_wide, rows_grid(n), dim3(BLK), 0, s, n, N.A.ptr, N.A.cols, (const int32_t*)rpos,
+                           flags + 1);
+        HIPCHK(hipGetLastError());
+        rc = scan(s, nsl, sz, L.eoff, P);
+        if (rc) return rc;
+        int32_t hf[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hf[0]) return fail(RUV_ERR_PRECOND, "batch amg: an aggregate holds more rows than a slice allows");
+        L.ci = hf[1] ? 1 : 0;
+        char *v = nullptr, *d = nullptr, *ec = nullptr;
+        HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
+        L.ecols = ec;
+        HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
+        HIPCHK(A->alloc(&d, n * sv));
+        auto fill = [&](auto tv) {
+            using TT = decltype(tv);
+            using VT = typename CoarseVal<TT>::type;
+            if (L.ci)
+                hipLaunchKernelGGL((k_pell_fill<TT, 1>), rows_grid(n), dim3(BLK), 0, s, n, N.A, (const int32_t*)rpos,
+                                   (const int32_t*)ipos, nsl, (const int32_t*)L.srow, (const int64_t*)L.eoff,
+                                   (const int32_t*)L.ew, (VT*)v, L.ecols);
+            else
+                hipLaunchKernelGGL((k_pell_fill<TT, 0>), rows_grid(n), dim3(BLK), 0, s, n, N.A, (const int32_t*)rpos,
+                                   (const int32_t*)ipos, nsl, (const int32_t*)L.srow, (const int64_t*)L.eoff,
+                                   (const int32_t*)L.ew, (VT*)v, L.ecols);
+            hipLaunchKernelGGL((k_pcast_inv<TT>), rows_grid(n), dim3(BLK), 0, s, n, N.diag, (const int32_t*)rpos,
+                               (TT*)d);
+        };
+        if (f32) fill(0.0f);
+        else fill(0.0);
+        HIPCHK(hipGetLastError());
+        L.evals = v;
+        L.dinv = d;
+        rc = alloc_vectors(L);
+        if (rc) return rc;
+    }
+    {  // finest level: member lists and aggregate map into level 1's stored order
+        BamgLevel& L = A->lv[0];
+        const NatLevel& N = nat[0];
+        const int64_t n1 = nat[1].n;
+        L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
+        if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build)
+            Bf16* e16 = nullptr;
+            HIPCHK(A->alloc(&e16, (size_t)Q.nell));
+            hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
+            HIPCHK(hipGetLastError());
+            A->ev0 = e16;
+        } else {
+            A->ev0 = Q.ev64;
+        }
+        char* t = nullptr;
+        HIPCHK(A->alloc(&t, L.n * R * sv));
+        HIPCHK(hipMemsetAsync(t, 0, L.n * R * sv, s));  // rows of converged charts keep finite values
+        L.t = t;
+        int64_t* start = nullptr;
+        int rc = stored_starts(0, &start);
+        if (rc) return rc;
+        HIPCHK(A->alloc(&L.mrows, L.n));
+        hipLaunchKernelGGL(k_perm_expand, rows_grid(n1), dim3(BLK), 0, s, n1, (const int32_t*)pos[1],
+                           (const int64_t*)N.mptr, (const int64_t*)N.mrows, (const int64_t*)start, (int32_t*)nullptr,
+                           L.mrows);
+        HIPCHK(A->alloc(&L.mptr, n1 + 1));
+        hipLaunchKernelGGL(k_narrow, rows_grid(n1 + 1), dim3(BLK), 0, s, n1 + 1, (const int64_t*)start, L.mptr);
+        HIPCHK(A->alloc(&L.agg, L.n));
+        hipLaunchKernelGGL(k_perm_agg, rows_grid(L.n), dim3(BLK), 0, s, L.n, (const int32_t*)N.agg,
+                           (const int32_t*)nullptr, (const int32_t*)pos[1], L.agg);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+    }
     if (max_chart_rows(crow) <= dense_max) {
-        int rc = coarsest_dense(Q, *A, cur, crow, f32);
+        int rc = coarsest_dense(Q, *A, nat[nl - 1].A, crow, f32);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(e1, s));
@@ -574,17 +674,17 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const typename CoarseVal<T>::type*)L.evals,
-                           (const T*)L.dinv, L.agg, L.mptr, L.mrows,
+                           L.srow, L.sagg, L.sblk, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
                            (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done};
     };
     hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
                        (const int2*)L1.blk, (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
         const T w = (T)A.lv[l].omega;
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_resid<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
-        else hipLaunchKernelGGL((kc_resid<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
-        hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(A.lv[l + 1].n), dim3(BLK), 0, s, cyc(l), A.lv[l + 1].n,
-                           A.lv[l + 1].chart, (const int2*)A.lv[l + 1].blk, (T*)A.lv[l + 1].b);
+        const dim3 gs((unsigned)A.lv[l].nsl);
+        if (A.lv[l].nsl == 0) continue;
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_down<T, R, 1>), gs, dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        else hipLaunchKernelGGL((kc_down<T, R, 0>), gs, dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
     }
     if (A.cinv)
         hipLaunchKernelGGL((kc_coarsest_dense<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1),
@@ -593,10 +693,12 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
     for (size_t l = nl - 1; l-- > 1;) {
         const T w = (T)A.lv[l].omega;
+        if (A.lv[l].nsl == 0) continue;
         hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha,
                            (const T*)A.lv[l + 1].z);
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
-        else hipLaunchKernelGGL((kc_post<T, R, 0>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w);
+        const dim3 gs((unsigned)A.lv[l].nsl);
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), gs, dim3(BLK), 0, s, cyc(l), w);
+        else hipLaunchKernelGGL((kc_post<T, R, 0>), gs, dim3(BLK), 0, s, cyc(l), w);
     }
     // up: e1 in level 0's work vector (free once restricted), then the post-smoothing over it (two passes:
     // recomputing e1 for every gathered neighbour inside the post-smoothing measured slower)
