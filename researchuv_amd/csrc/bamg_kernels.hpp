@@ -407,26 +407,31 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
     else cdiag[I] = diag;
 }
 
-// The same Galerkin rows, one thread a row with the distinct columns in registers (up to GR_SLOTS of them): each
+// The same Galerkin rows, one thread a row with the distinct columns in registers (up to SL of them): each
 // entry adds into the slot of its column, in row order, so the sums equal k_galerkin's; the output position of
 // a slot is the number of smaller columns.  A row with more distinct columns goes to `over` (counted by nover)
-// for k_galerkin.
+// for the next stage: 16 slots over all rows, then 32 over the rows listed by the first, then k_galerkin (the
+// search and the ranking cost grow with the slots, and most coarse rows have at most 16 distinct columns).
+constexpr int GR_SLOTS_FIRST = 16;
 constexpr int GR_SLOTS = 32;
+constexpr int GR_BATCH = 8;  // entries whose loads are issued together
 
-template <bool FILL>
-__global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, const int64_t* __restrict__ mptr,
+template <bool FILL, int SL>
+__global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_t* __restrict__ list, SetupCsr A,
+                                                      const int64_t* __restrict__ mptr,
                                                       const int64_t* __restrict__ mrows,
                                                       const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
                                                       const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
                                                       double* __restrict__ cvals, double* __restrict__ cdiag,
                                                       unsigned long long* __restrict__ nover,
                                                       int64_t* __restrict__ over) {
-    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (I >= nc) return;
-    int32_t key[GR_SLOTS];
-    double val[GR_SLOTS];
+    const int64_t x = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (x >= nrow) return;
+    const int64_t I = list ? list[x] : x;
+    int32_t key[SL];
+    double val[SL];
 #pragma unroll
-    for (int k = 0; k < GR_SLOTS; ++k) {
+    for (int k = 0; k < SL; ++k) {
         key[k] = INT32_MAX;
         val[k] = 0.0;
     }
@@ -436,27 +441,43 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, co
     for (int64_t q = mptr[I]; q < m1; ++q) {
         const int64_t r = mrows[q];
         const int64_t e1 = A.ptr[r + 1];
-        for (int64_t e = A.ptr[r]; e < e1; ++e) {
-            const int32_t c = agg[A.cols[e]];
-            const double v = A.vals[e];
-            bool hit = false;
+        // entries in batches of GR_BATCH: the column loads, then the aggregate lookups, are issued together
+        for (int64_t e0 = A.ptr[r]; e0 < e1; e0 += GR_BATCH) {
+            int32_t cb[GR_BATCH];
+            double vb[GR_BATCH];
 #pragma unroll
-            for (int k = 0; k < GR_SLOTS; ++k)
-                if (key[k] == c) {
-                    val[k] += v;
-                    hit = true;
+            for (int u = 0; u < GR_BATCH; ++u)
+                if (e0 + u < e1) {
+                    cb[u] = A.cols[e0 + u];
+                    vb[u] = A.vals[e0 + u];
                 }
-            if (!hit) {
-                if (n < GR_SLOTS) {
 #pragma unroll
-                    for (int k = 0; k < GR_SLOTS; ++k)
-                        if (k == n) {
-                            key[k] = c;
-                            val[k] += v;
-                        }
-                    ++n;
-                } else {
-                    full = true;
+            for (int u = 0; u < GR_BATCH; ++u)
+                if (e0 + u < e1) cb[u] = agg[cb[u]];
+#pragma unroll
+            for (int u = 0; u < GR_BATCH; ++u) {
+                if (e0 + u >= e1) break;
+                const int32_t c = cb[u];
+                const double v = vb[u];
+                bool hit = false;
+#pragma unroll
+                for (int k = 0; k < SL; ++k)
+                    if (key[k] == c) {
+                        val[k] += v;
+                        hit = true;
+                    }
+                if (!hit) {
+                    if (n < SL) {
+#pragma unroll
+                        for (int k = 0; k < SL; ++k)
+                            if (k == n) {
+                                key[k] = c;
+                                val[k] += v;
+                            }
+                        ++n;
+                    } else {
+                        full = true;
+                    }
                 }
             }
         }
@@ -475,11 +496,11 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nc, SetupCsr A, co
     const int64_t out = cptr[I];
     double diag = 0.0;
 #pragma unroll
-    for (int k = 0; k < GR_SLOTS; ++k) {
+    for (int k = 0; k < SL; ++k) {
         if (k < n) {
             int rank = 0;
 #pragma unroll
-            for (int t = 0; t < GR_SLOTS; ++t) rank += (key[t] < key[k]) ? 1 : 0;
+            for (int t = 0; t < SL; ++t) rank += (key[t] < key[k]) ? 1 : 0;
             ccols[out + rank] = key[k];
             cvals[out + rank] = val[k];
             if (key[k] == I) diag = val[k];

@@ -190,26 +190,38 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&M->group, nc));
     rc = build_members(s, n, M->agg, nc, group, S, &M->mptr, &M->mrows, M->group);
     if (rc) return rc;
-    // Galerkin operator: rows of few distinct columns in registers, the others (listed) through LDS
-    int64_t *cnt = nullptr, *cptr = nullptr, *over = nullptr;
+    // Galerkin operator: rows of few distinct columns in registers (16 slots, then 32 for the rows listed by the
+    // first pass), the others (listed again) through LDS
+    int64_t *cnt = nullptr, *cptr = nullptr, *over1 = nullptr, *over2 = nullptr;
     unsigned long long* nover = nullptr;
     HIPCHK(S.alloc(&cnt, nc));
     HIPCHK(S.alloc(&cptr, nc + 1));
-    HIPCHK(S.alloc(&over, nc));
-    HIPCHK(S.alloc(&nover, 1));
-    HIPCHK(hipMemsetAsync(nover, 0, sizeof(unsigned long long), s));
-    hipLaunchKernelGGL((k_galerkin_reg<false>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
-                       (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
-                       (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, nover, over);
-    unsigned long long hover = 0;
-    HIPCHK(hipMemcpyAsync(&hover, nover, sizeof(hover), hipMemcpyDeviceToHost, s));
+    HIPCHK(S.alloc(&over1, nc));
+    HIPCHK(S.alloc(&over2, nc));
+    HIPCHK(S.alloc(&nover, 2));
+    HIPCHK(hipMemsetAsync(nover, 0, 2 * sizeof(unsigned long long), s));
+    unsigned long long hover[2] = {0, 0};
+    hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
+                       (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
+                       (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (double*)nullptr,
+                       (double*)nullptr, nover, over1);
+    HIPCHK(hipMemcpyAsync(&hover[0], nover, sizeof(hover[0]), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    const int64_t no = (int64_t)hover;
+    const int64_t n1 = (int64_t)hover[0];
+    if (n1 > 0) {
+        hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
+                           (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
+                           (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (double*)nullptr,
+                           (double*)nullptr, nover + 1, over2);
+        HIPCHK(hipMemcpyAsync(&hover[1], nover + 1, sizeof(hover[1]), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    const int64_t no = (int64_t)hover[1];
     const dim3 go((unsigned)std::max<int64_t>(1, (no + GAL_BLK - 1) / GAL_BLK));
     if (no > 0)
         hipLaunchKernelGGL((k_galerkin<false>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
                            (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
-                           (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, (const int64_t*)over);
+                           (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, (const int64_t*)over2);
     HIPCHK(hipGetLastError());
     rc = scan(s, nc, cnt, cptr, S);
     if (rc) return rc;
@@ -222,13 +234,19 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
     HIPCHK(S.alloc(&M->cdiag, nc));
-    hipLaunchKernelGGL((k_galerkin_reg<true>), rows_grid(nc), dim3(BLK), 0, s, nc, A, (const int64_t*)M->mptr,
-                       (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr,
-                       ccols, cvals, M->cdiag, (unsigned long long*)nullptr, (int64_t*)nullptr);
+    hipLaunchKernelGGL((k_galerkin_reg<true, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
+                       (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
+                       (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr, ccols, cvals, M->cdiag,
+                       (unsigned long long*)nullptr, (int64_t*)nullptr);
+    if (n1 > 0)
+        hipLaunchKernelGGL((k_galerkin_reg<true, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
+                           (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
+                           (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr, ccols, cvals, M->cdiag,
+                           (unsigned long long*)nullptr, (int64_t*)nullptr);
     if (no > 0)
         hipLaunchKernelGGL((k_galerkin<true>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
                            (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr,
-                           (const int64_t*)cptr, ccols, cvals, M->cdiag, (const int64_t*)over);
+                           (const int64_t*)cptr, ccols, cvals, M->cdiag, (const int64_t*)over2);
     HIPCHK(hipGetLastError());
     M->C = SetupCsr{nc, cptr, ccols, cvals};
     return RUV_OK;
