@@ -303,6 +303,7 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     HIPCHK(Q.alloc(&B.c_tol, nchart));
     HIPCHK(Q.alloc(&B.c_bnorm, nchart));
     HIPCHK(Q.alloc(&B.c_true, nchart));
+    HIPCHK(Q.alloc(&B.c_rrec, nchart));
     HIPCHK(Q.alloc(&B.c_done, nchart));
     HIPCHK(Q.alloc(&B.c_final, nchart));
     HIPCHK(Q.alloc(&B.c_iters, nchart));
@@ -339,8 +340,10 @@ void launch_spmv(BatchImpl& Q, int na, const T* p, T* q) {
                            (const int*)Q.act, na, vals, p, q);
 }
 
+// One inner PCG over the charts not yet final.  cont: go on from the previous inner solve's last direction (the
+// residual replaced by the refined FP64 one) instead of starting afresh.
 template <typename T, int R, int PC, int CI>
-int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
+int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     hipStream_t s = Q.stream;
     BDev& B = Q.B;
     B.max_iter = (unsigned)std::min<uint64_t>(max_iter, 0xffffffffu);
@@ -356,9 +359,9 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter) {
     if (na == 0) return RUV_OK;
     // step 0 reads its old direction from pb[1]
     hipLaunchKernelGGL((kb_prep<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
-                       (const double*)Q.r64, dinv, x, r, z, pb[1]);
+                       (const double*)Q.r64, dinv, x, r, z, (const T*)pb[0], pb[1], cont);
     hipLaunchKernelGGL(kb_init, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol, Q.inner_tol,
-                       Q.mode == RUV_MODE_MIXED ? 1 : 0);
+                       Q.mode == RUV_MODE_MIXED ? 1 : 0, cont, Q.cont_gap);
     HIPCHK(hipGetLastError());
     if (PC == 2) {  // z0 = M^-1 r0 and its r.z partials (parity 0); kb_init has marked the running charts
         rc = bamg_apply<T, R, CI>(Q, na, 0, (const T*)r, z);
@@ -434,7 +437,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
     int na = 0;
     for (int round = 0;; ++round) {
         if (round >= Q.max_rounds) return fail(RUV_ERR_NOT_CONVERGED, "batch: refinement did not converge");
-        int rc = inner_pcg<T, R, PC, CI>(Q, tol, max_iter);
+        int rc = inner_pcg<T, R, PC, CI>(Q, tol, max_iter, (round > 0 && Q.continue_rounds) ? 1 : 0);
         if (rc) return rc;
         rc = set_active(Q, [&](int c) { return Q.h_final[c] == 0; }, &na);
         if (rc) return rc;
@@ -565,6 +568,8 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
         if (v > 0.0 && v < 1.0) Q->inner_tol = v;
     }
     if (const char* e = std::getenv("RUV_BATCH_AMG_CHUNK")) Q->amg_chunk = std::min(64, std::max(1, std::atoi(e)));
+    if (const char* e = std::getenv("RUV_BATCH_RESTART")) Q->continue_rounds = std::atoi(e) ? 0 : 1;
+    if (const char* e = std::getenv("RUV_BATCH_CONT_GAP")) Q->cont_gap = std::atof(e);
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

@@ -96,6 +96,12 @@ struct BatchImpl {
     std::vector<uint64_t> iters_total;
     double inner_tol = 1e-4;  // inner FP32 solve: reduce ||r|| by this much (two rounds reach 1e-8)
     int max_rounds = 40;
+    // rounds after the first go on from the previous round's last direction (1) or start afresh (0)
+    int continue_rounds = 1;
+    // ... when |refined - recursive residual| <= cont_gap * |recursive residual| (measured: the 4096-chart batch's
+    // charts are nearly all below 0.03, the 2048^2 stretched grid's single chart between 0.1 and 0.3, where going
+    // on took 1017 steps against 292 for a fresh start)
+    double cont_gap = 0.05;
     int amg_chunk = 4;  // AMG-preconditioned steps between convergence looks
     // step-kernel accounting (optional): an event pair around every launch, summed after each solve
     bool ktime = false;

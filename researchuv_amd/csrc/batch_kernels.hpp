@@ -63,6 +63,7 @@ struct BDev {
     double* c_tol;
     double* c_bnorm;
     double* c_true;
+    double* c_rrec;  // recursive ||r|| of the step an inner solve converged at
     int32_t* c_done;
     int32_t* c_final;
     uint32_t* c_iters;
@@ -280,7 +281,10 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
     sum_range2(B.part_b + (int64_t)par * B.ntile, B.part_c + (int64_t)par * B.ntile, t0, t1, sh, &rr, &rz);
     const bool lead = (t == t0) && threadIdx.x == 0;
     if (sqrt(rr) <= B.c_tol[c]) {
-        if (lead) B.c_done[c] = B_CONVERGED;
+        if (lead) {
+            B.c_done[c] = B_CONVERGED;
+            B.c_rrec[c] = sqrt(rr);
+        }
         return;
     }
     if (iter >= B.max_iter) {
@@ -445,25 +449,34 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
     }
 }
 
-// inner start on the live charts: x = 0, r = T(r64), z = D^-1 r, p = 0 (the first step's old direction);
-// partials r.r / r.z for parity 0 (PC 2: z and r.z come from the AMG cycle)
+// inner start on the live charts: x = 0, r = T(r64), z = D^-1 r; partials r.r / r.z for parity 0 (PC 2: z
+// and r.z come from the AMG cycle).  The first step's old direction p (= p1, read by the step of parity 0): 0
+// for a fresh start; with cont, the chart's last direction of the previous inner solve (the residual is
+// replaced by the refined one and the iteration goes on, as in reliable-update CG), which ended in p0 or p1 by
+// the parity of its last step.
 template <typename T, int R, int PC>
 __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ act, int na,
                                                const double* __restrict__ r64, const T* __restrict__ dinv,
                                                T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                                               T* __restrict__ p) {
+                                               const T* __restrict__ p0, T* __restrict__ p, int cont) {
     __shared__ double sh[WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
     const int t = act[slot];
-    double arr = 0.0, arz = 0.0;
+    const uint32_t last = B.c_iters[B.tile_chart[t]];  // steps of the previous inner solve (read before kb_init)
+    // 0: zero direction; 1: keep p (the last step had parity 1); 2: copy p0 (parity 0)
+    const int keep = (cont && last > 0) ? (((last - 1) & 1) ? 1 : 2) : 0;
+    double arr = 0.0, arz = 0.0, agap = 0.0;
     const int64_t r1 = B.tile_r1[t];
     for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
         const T d = (PC == 1) ? dinv[i] : T(1);
         Vr<T, R> xi, ri, zi;
+        Vr<T, R> rold;  // the recursive residual the previous inner solve ended with
+        if (cont) rold = ld<T, R>(r, i);
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) {
             ri.c[cc] = (T)r64[i * R + cc];
+            if (cont) agap += ((double)ri.c[cc] - (double)rold.c[cc]) * ((double)ri.c[cc] - (double)rold.c[cc]);
             zi.c[cc] = d * ri.c[cc];
             xi.c[cc] = T(0);
             arr += (double)ri.c[cc] * (double)ri.c[cc];
@@ -472,19 +485,23 @@ __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ a
         st<T, R>(x, i, xi);
         st<T, R>(r, i, ri);
         if (PC == 1) st<T, R>(z, i, zi);
-        st<T, R>(p, i, xi);  // xi is zero
+        if (keep == 0) st<T, R>(p, i, xi);  // xi is zero
+        else if (keep == 2) st<T, R>(p, i, ld<T, R>(p0, i));
     }
     const double srr = block_sum_all(arr, sh);
     const double srz = block_sum_all(arz, sh);
+    const double sgap = block_sum_all(agap, sh);
     if (threadIdx.x == 0) {
         B.part_b[t] = srr;
         if (PC != 2) B.part_c[t] = srz;
+        B.part_o[t] = sgap;  // ||refined - recursive residual||^2 of the tile (part_o is free until the next outer test)
     }
 }
 
 // per chart (not final): inner tolerance, step state.  plain: tol * ||b|| on the recursive residual
 // (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
-static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed) {
+static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed, int cont,
+                                                      double gap) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= B.nchart) return;
     if (B.c_final[c]) {
@@ -494,12 +511,24 @@ static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double
     double s = 0.0;
     for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
     const double rn = sqrt(s), bn = B.c_bnorm[c];
+    // the "old" r.z of step 0.  Fresh start: 1 (the direction is zero, only finiteness matters).  cont: that of
+    // the previous inner solve's last step, whose direction kb_prep kept, when the refined residual differs from
+    // the recursive one the solve stopped at by at most gap times the latter's norm; a larger difference (FP32
+    // drift on a badly conditioned chart) makes the kept direction a poor one, so the step restarts instead:
+    // infinity makes beta 0 and p' = z exactly.
+    const uint32_t last = B.c_iters[c];
+    double rz_old = 1.0;
+    if (cont && last > 0) {
+        double g = 0.0;
+        for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) g += B.part_o[t];
+        rz_old = (sqrt(g) <= gap * B.c_rrec[c]) ? B.c_rz[(int64_t)((last - 1) & 1) * B.nchart + c] : INFINITY;
+    }
     double ta;
     if (mixed) ta = fmax(inner_tol * rn, 0.25 * ((bn <= 1e-300) ? tol : tol * bn));
     else ta = (bn <= 1e-300) ? tol : tol * bn;
     B.c_tol[c] = ta;
     B.c_iters[c] = 0;
-    B.c_rz[B.nchart + c] = 1.0;  // "old" r.z of step 0; its direction is zero, so only finiteness matters
+    B.c_rz[B.nchart + c] = rz_old;
     B.c_done[c] = (rn <= ta) ? B_CONVERGED : B_RUNNING;
 }
 
