@@ -131,8 +131,7 @@ int build_members(hipStream_t s, int64_t n, const int32_t* agg, int64_t nc, cons
     hipLaunchKernelGGL(k_member_sort, rows_grid(nc), dim3(BLK), 0, s, nc, (const int64_t*)*mptr, *mrows, group,
                        group_c);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));  // the scratch counters die here
-    return RUV_OK;
+    return RUV_OK;  // the counters go back to the pool; their next user is ordered after these kernels
 }
 
 // One matching of A within groups (pairs, then lone rows joining pairs) and its Galerkin operator.
@@ -490,7 +489,7 @@ int bamg_build(BatchImpl& Q) {
         hipLaunchKernelGGL(k_chart_groups, rows_grid(MF.nc), dim3(BLK), 0, s, MF.nc, (const int64_t*)d_crow,
                            Q.nchart, ngroup);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(s));
+        // (crow, the copy's source, changes next after the next level's matching, which waits on the stream)
         cur = MF.C;
         cur_diag = MF.cdiag;
         group = ngroup;
