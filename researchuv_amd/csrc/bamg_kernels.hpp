@@ -773,6 +773,8 @@ struct CycLevel {
     const int32_t* srow;     // slice row starts (aligned to aggregate starts)
     const int32_t* sagg;     // first next-level aggregate of every slice
     const int2* sblk;        // per slice: first and last chart
+    const int32_t* slist;    // slices of the running charts, one per workgroup (null: workgroup b is slice b)
+    const int32_t* blist;    // the same for BLK-row blocks
     const T* dinv;
     const int32_t* agg;      // row -> aggregate on the next level
     const int32_t* mptr;     // next level's aggregates -> member rows here (levels >= 1: first stored row)
@@ -789,15 +791,19 @@ struct CycLevel {
 // same chart, and the running charts never read what a stopped chart's rows hold.
 // Decided per workgroup of BLK rows from its first and last chart (charts are contiguous on every level): all
 // stopped (the workgroup leaves), all running (no per-row test), or mixed (each row tests its own chart).
-__device__ __forceinline__ int block_state(const int2* __restrict__ blk, const int32_t* __restrict__ done) {
-    const int2 b = blk[blockIdx.x];  // (first, last) chart of this workgroup's rows; last < first: no rows
+// The workgroups of a launch cover either every slice (block) of a level or, late in a solve, only those of the
+// running charts, listed: workgroup -> slice (block).
+__device__ __forceinline__ int64_t wg_item(const int32_t* __restrict__ list) {
+    return list ? (int64_t)list[blockIdx.x] : (int64_t)blockIdx.x;
+}
+__device__ __forceinline__ int block_state(const int2* __restrict__ blk, int64_t b, const int32_t* __restrict__ done) {
+    const int2 fl = blk[b];  // (first, last) chart of the rows of slice or block b; last < first: no rows
     int run = 0;
-    for (int c = b.x; c <= b.y; ++c) run += (done[c] == B_RUNNING) ? 1 : 0;
-    return run == 0 ? 0 : (run == b.y - b.x + 1 ? 1 : 2);
+    for (int c = fl.x; c <= fl.y; ++c) run += (done[c] == B_RUNNING) ? 1 : 0;
+    return run == 0 ? 0 : (run == fl.y - fl.x + 1 ? 1 : 2);
 }
 template <typename T>
-__device__ __forceinline__ bool live_row(const CycLevel<T>& L, int64_t i) {
-    const int bs = block_state(L.blk, L.done);
+__device__ __forceinline__ bool live_row(const CycLevel<T>& L, int bs, int64_t i) {
     return bs == 1 || (bs == 2 && L.done[L.chart[i]] == B_RUNNING);
 }
 
@@ -847,9 +853,9 @@ __device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r,
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict__ bnext) {
     __shared__ T tl[MAX_SLICE_ROWS * R];
-    const int bs = block_state(L.sblk, L.done);
+    const int64_t sb = wg_item(L.slist);
+    const int bs = block_state(L.sblk, sb, L.done);
     if (bs == 0) return;
-    const int64_t sb = blockIdx.x;
     const int64_t r0 = L.srow[sb], h = L.srow[sb + 1] - r0;
     const int wd = L.ew[sb];
     const int64_t e0 = L.eoff[sb];
@@ -896,10 +902,12 @@ __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict
 // per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, const int32_t* __restrict__ cnext,
-                                                   const int2* __restrict__ bnext_blk, T* __restrict__ bnext) {
-    const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
+                                                   const int2* __restrict__ bnext_blk,
+                                                   const int32_t* __restrict__ bnext_list, T* __restrict__ bnext) {
+    const int64_t nb = wg_item(bnext_list);
+    const int64_t I = nb * BLK + threadIdx.x;
     if (I >= nc) return;
-    const int bs = block_state(bnext_blk, L.done);
+    const int bs = block_state(bnext_blk, nb, L.done);
     if (bs == 0 || (bs == 2 && L.done[cnext[I]] != B_RUNNING)) return;
     T acc[R];
 #pragma unroll
@@ -919,8 +927,8 @@ __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, co
 // level >= 1: t = e1 = w D^-1 b + alpha P z_next
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, const T* __restrict__ znext) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n || !live_row(L, i)) return;
+    const int64_t b = wg_item(L.blist), i = b * BLK + threadIdx.x;
+    if (i >= L.n || !live_row(L, block_state(L.blk, b, L.done), i)) return;
     const T di = w * L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     const Vr<T, R> zc = ld<T, R>(znext, L.agg[i]);
@@ -933,9 +941,9 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, c
 // levels between the finest and the coarsest, one workgroup per slice: z = e1 + w D^-1 (b - A e1), e1 in t
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
-    const int bs = block_state(L.sblk, L.done);
+    const int64_t sb = wg_item(L.slist);
+    const int bs = block_state(L.sblk, sb, L.done);
     if (bs == 0) return;
-    const int64_t sb = blockIdx.x;
     const int64_t r0 = L.srow[sb], h = L.srow[sb + 1] - r0;
     const int wd = L.ew[sb];
     const int64_t e0 = L.eoff[sb];
@@ -959,8 +967,8 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
 // coarsest level: z = D^-1 b (exact for a chart coarsened to one row)
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n || !live_row(L, i)) return;
+    const int64_t b = wg_item(L.blist), i = b * BLK + threadIdx.x;
+    if (i >= L.n || !live_row(L, block_state(L.blk, b, L.done), i)) return;
     const T d = L.dinv[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     Vr<T, R> o;
@@ -974,8 +982,8 @@ __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_coarsest_dense(CycLevel<T> L, const int64_t* __restrict__ crow,
                                                          const int64_t* __restrict__ moff, const T* __restrict__ minv) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= L.n || !live_row(L, i)) return;
+    const int64_t b = wg_item(L.blist), i = b * BLK + threadIdx.x;
+    if (i >= L.n || !live_row(L, block_state(L.blk, b, L.done), i)) return;
     const int c = L.chart[i];
     const int64_t r0 = crow[c], nc = crow[c + 1] - r0;
     const T* m = minv + moff[c] + (i - r0) * nc;

@@ -34,6 +34,7 @@ int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, cons
 int bamg_build(BatchImpl& Q);
 template <typename T, int R, int CI>
 int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z);
+int bamg_set_live(BatchImpl& Q, const int32_t* done);
 int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows);
 int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap);
 
@@ -364,7 +365,8 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
                        Q.mode == RUV_MODE_MIXED ? 1 : 0, cont, Q.cont_gap);
     HIPCHK(hipGetLastError());
     if (PC == 2) {  // z0 = M^-1 r0 and its r.z partials (parity 0); kb_init has marked the running charts
-        rc = bamg_apply<T, R, CI>(Q, na, 0, (const T*)r, z);
+        rc = bamg_set_live(Q, nullptr);
+        if (!rc) rc = bamg_apply<T, R, CI>(Q, na, 0, (const T*)r, z);
         if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
@@ -378,6 +380,9 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
         rc = set_active(Q, [&](int c) { return Q.h_done[c] == B_RUNNING; }, &na);
         if (rc) return rc;
         if (na == 0) break;
+        // once most charts have stopped, the coarse levels launch only the running charts' slices and blocks
+        if (PC == 2) rc = bamg_set_live(Q, 2 * (int64_t)na < (int64_t)Q.ntile ? Q.h_done : nullptr);
+        if (rc) return rc;
         for (int it = 0; it < chunk; ++it) {
             hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
                                (const T*)pb[par ^ 1], pb[par]);

@@ -51,11 +51,20 @@ struct Bamg {
     // iterations by about a third on height fields (49 -> 34 on a 256^2 patch in a CPU model of the cycle)
     double alpha = 1.2;
     double setup_seconds = 0.0;
+    // Lists of the slices and BLK-row blocks of the running charts per level, for launches late in a solve
+    // (empty vectors / null pointers: launches cover the whole level).  cs0/cs1 (cb0/cb1): per level, each chart's
+    // slices (blocks) [first, last + 1); the lists live in one device buffer filled from pinned host memory.
+    std::vector<std::vector<int32_t>> cs0, cs1, cb0, cb1;
+    std::vector<int32_t*> slist, blist;
+    std::vector<int64_t> nslist, nblist;
+    int32_t* d_lists = nullptr;
+    int32_t* h_lists = nullptr;
     DevPool* pool = nullptr;
     std::vector<void*> mem;
     explicit Bamg(DevPool* p) : pool(p) {}
     ~Bamg() {
         for (void* p : mem) pool->put(p);
+        if (h_lists) (void)hipHostFree(h_lists);
     }
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
@@ -420,6 +429,7 @@ int bamg_build(BatchImpl& Q) {
         int64_t* mptr = nullptr;          // next level's aggregates -> member rows (ascending)
         int64_t* mrows = nullptr;
         double omega = 2.0 / 3.0;
+        std::vector<int64_t> crow;        // chart row ranges
         std::unique_ptr<Scratch> own;     // the level's operator (levels >= 1) and matching
     };
     std::vector<NatLevel> nat;
@@ -435,6 +445,7 @@ int bamg_build(BatchImpl& Q) {
         N.A = cur;
         N.diag = cur_diag;
         N.group = group;
+        N.crow = crow;
         const bool last = (l + 1 >= max_levels) ||
                           (l > 0 && (cur.n <= (int64_t)Q.nchart || max_chart_rows(crow) <= dense_rows));
         if (last) {
@@ -499,6 +510,11 @@ int bamg_build(BatchImpl& Q) {
     const size_t nl = nat.size();
     if (nl < 2) return fail(RUV_ERR_PRECOND, "batch amg: no coarse level");
     A->lv.resize(nl);
+    for (auto* v : {&A->cs0, &A->cs1, &A->cb0, &A->cb1}) v->assign(nl, {});
+    A->slist.assign(nl, nullptr);
+    A->blist.assign(nl, nullptr);
+    A->nslist.assign(nl, 0);
+    A->nblist.assign(nl, 0);
     for (size_t l = 0; l < nl; ++l) {
         BamgLevel& L = A->lv[l];
         L.n = nat[l].n;
@@ -593,9 +609,26 @@ int bamg_build(BatchImpl& Q) {
         rc = scan(s, nsl, sz, L.eoff, P);
         if (rc) return rc;
         int32_t hf[2] = {0, 0};
+        std::vector<int32_t> hsrow((size_t)nsl + 1);
         HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hsrow.data(), L.srow, sizeof(int32_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        {  // each chart's slices: those holding its first and last rows, and all between
+            const std::vector<int64_t>& cr = nat[l].crow;
+            auto slice_of = [&](int64_t p) {
+                int64_t b = std::min<int64_t>(p / SLICE, nsl - 1);
+                while (hsrow[b] > p) --b;
+                return (int32_t)b;
+            };
+            A->cs0[l].assign(Q.nchart, 0);
+            A->cs1[l].assign(Q.nchart, 0);
+            for (int c = 0; c < Q.nchart; ++c)
+                if (cr[c + 1] > cr[c]) {
+                    A->cs0[l][c] = slice_of(cr[c]);
+                    A->cs1[l][c] = slice_of(cr[c + 1] - 1) + 1;
+                }
+        }
         if (hf[0]) return fail(RUV_ERR_PRECOND, "batch amg: an aggregate holds more rows than a slice allows");
         L.ci = hf[1] ? 1 : 0;
         char *v = nullptr, *d = nullptr, *ec = nullptr;
@@ -662,6 +695,22 @@ int bamg_build(BatchImpl& Q) {
         int rc = coarsest_dense(Q, *A, nat[nl - 1].A, crow, f32);
         if (rc) return rc;
     }
+    {  // each chart's BLK-row blocks on levels >= 1, and room for the lists of running slices and blocks
+        size_t cap = 0;
+        for (size_t l = 1; l < nl; ++l) {
+            const std::vector<int64_t>& cr = nat[l].crow;
+            A->cb0[l].assign(Q.nchart, 0);
+            A->cb1[l].assign(Q.nchart, 0);
+            for (int c = 0; c < Q.nchart; ++c)
+                if (cr[c + 1] > cr[c]) {
+                    A->cb0[l][c] = (int32_t)(cr[c] / BLK);
+                    A->cb1[l][c] = (int32_t)((cr[c + 1] - 1) / BLK + 1);
+                }
+            cap += (size_t)A->lv[l].nsl + (size_t)((A->lv[l].n + BLK - 1) / BLK);
+        }
+        HIPCHK(A->alloc(&A->d_lists, cap));
+        HIPCHK(hipHostMalloc((void**)&A->h_lists, sizeof(int32_t) * std::max<size_t>(cap, 1), hipHostMallocDefault));
+    }
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -671,6 +720,32 @@ int bamg_build(BatchImpl& Q) {
     A->setup_seconds = ms * 1e-3;
     if (Q.amg) bamg_destroy(Q.amg);
     Q.amg = A.release();
+    return RUV_OK;
+}
+
+// Launches of the coarse levels cover every slice and block (done == nullptr) or only those of the charts with
+// done[c] == B_RUNNING (the lists are uploaded on the solver's stream).
+int bamg_set_live(BatchImpl& Q, const int32_t* done) {
+    Bamg& A = *Q.amg;
+    const size_t nl = A.lv.size();
+    std::fill(A.slist.begin(), A.slist.end(), nullptr);
+    std::fill(A.blist.begin(), A.blist.end(), nullptr);
+    if (!done) return RUV_OK;
+    size_t pos = 0;
+    auto add = [&](const std::vector<int32_t>& f, const std::vector<int32_t>& e, int32_t** list, int64_t* count) {
+        const size_t start = pos;
+        int32_t last = -1;
+        for (int c = 0; c < Q.nchart; ++c)
+            if (done[c] == B_RUNNING)
+                for (int32_t b = std::max(f[c], last + 1); b < e[c]; ++b) A.h_lists[pos++] = last = b;
+        *list = A.d_lists + start;
+        *count = (int64_t)(pos - start);
+    };
+    for (size_t l = 1; l < nl; ++l) {
+        if (l + 1 < nl) add(A.cs0[l], A.cs1[l], &A.slist[l], &A.nslist[l]);
+        add(A.cb0[l], A.cb1[l], &A.blist[l], &A.nblist[l]);
+    }
+    if (pos) HIPCHK(hipMemcpyAsync(A.d_lists, A.h_lists, sizeof(int32_t) * pos, hipMemcpyHostToDevice, Q.stream));
     return RUV_OK;
 }
 
@@ -691,31 +766,39 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const typename CoarseVal<T>::type*)L.evals,
-                           L.srow, L.sagg, L.sblk, (const T*)L.dinv, L.agg, L.mptr, L.mrows,
+                           L.srow, L.sagg, L.sblk, A.slist[l], A.blist[l], (const T*)L.dinv, L.agg, L.mptr, L.mrows,
                            (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done};
     };
-    hipLaunchKernelGGL((kc_restrict<T, R>), rows_grid(L1.n), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
-                       (const int2*)L1.blk, (T*)L1.b);
+    // grids: every slice / block of a level, or the listed ones of the running charts
+    auto sgrid = [&](size_t l) -> int64_t { return A.slist[l] ? A.nslist[l] : A.lv[l].nsl; };
+    auto bgrid = [&](size_t l) -> int64_t { return A.blist[l] ? A.nblist[l] : (A.lv[l].n + BLK - 1) / BLK; };
+    auto g1 = [](int64_t n) { return dim3((unsigned)n); };
+    if (bgrid(1) > 0)
+        hipLaunchKernelGGL((kc_restrict<T, R>), g1(bgrid(1)), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
+                           (const int2*)L1.blk, (const int32_t*)A.blist[1], (T*)L1.b);
     for (size_t l = 1; l + 1 < nl; ++l) {
         const T w = (T)A.lv[l].omega;
-        const dim3 gs((unsigned)A.lv[l].nsl);
-        if (A.lv[l].nsl == 0) continue;
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_down<T, R, 1>), gs, dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
-        else hipLaunchKernelGGL((kc_down<T, R, 0>), gs, dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        const int64_t ns = sgrid(l);
+        if (ns == 0) continue;
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_down<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        else hipLaunchKernelGGL((kc_down<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
     }
-    if (A.cinv)
-        hipLaunchKernelGGL((kc_coarsest_dense<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1),
-                           (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const T*)A.cinv);
-    else
-        hipLaunchKernelGGL((kc_coarsest<T, R>), rows_grid(A.lv[nl - 1].n), dim3(BLK), 0, s, cyc(nl - 1));
+    const int64_t nbc = bgrid(nl - 1);
+    if (nbc > 0) {
+        if (A.cinv)
+            hipLaunchKernelGGL((kc_coarsest_dense<T, R>), g1(nbc), dim3(BLK), 0, s, cyc(nl - 1),
+                               (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const T*)A.cinv);
+        else
+            hipLaunchKernelGGL((kc_coarsest<T, R>), g1(nbc), dim3(BLK), 0, s, cyc(nl - 1));
+    }
     for (size_t l = nl - 1; l-- > 1;) {
         const T w = (T)A.lv[l].omega;
-        if (A.lv[l].nsl == 0) continue;
-        hipLaunchKernelGGL((kc_prolong<T, R>), rows_grid(A.lv[l].n), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha,
+        const int64_t ns = sgrid(l), nb = bgrid(l);
+        if (ns == 0 || nb == 0) continue;
+        hipLaunchKernelGGL((kc_prolong<T, R>), g1(nb), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha,
                            (const T*)A.lv[l + 1].z);
-        const dim3 gs((unsigned)A.lv[l].nsl);
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), gs, dim3(BLK), 0, s, cyc(l), w);
-        else hipLaunchKernelGGL((kc_post<T, R, 0>), gs, dim3(BLK), 0, s, cyc(l), w);
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w);
+        else hipLaunchKernelGGL((kc_post<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w);
     }
     // up: e1 in level 0's work vector (free once restricted), then the post-smoothing over it (two passes:
     // recomputing e1 for every gathered neighbour inside the post-smoothing measured slower)
