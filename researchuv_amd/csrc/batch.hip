@@ -381,7 +381,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
         if (rc) return rc;
         if (na == 0) break;
         // once most charts have stopped, the coarse levels launch only the running charts' slices and blocks
-        if (PC == 2) rc = bamg_set_live(Q, 2 * (int64_t)na < (int64_t)Q.ntile ? Q.h_done : nullptr);
+        if (PC == 2) rc = bamg_set_live(Q, (double)na < Q.list_below * (double)Q.ntile ? Q.h_done : nullptr);
         if (rc) return rc;
         for (int it = 0; it < chunk; ++it) {
             hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
@@ -575,6 +575,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (const char* e = std::getenv("RUV_BATCH_AMG_CHUNK")) Q->amg_chunk = std::min(64, std::max(1, std::atoi(e)));
     if (const char* e = std::getenv("RUV_BATCH_RESTART")) Q->continue_rounds = std::atoi(e) ? 0 : 1;
     if (const char* e = std::getenv("RUV_BATCH_CONT_GAP")) Q->cont_gap = std::atof(e);
+    if (const char* e = std::getenv("RUV_BATCH_LIST_BELOW")) Q->list_below = std::atof(e);
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

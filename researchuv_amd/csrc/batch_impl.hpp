@@ -102,6 +102,8 @@ struct BatchImpl {
     // charts are nearly all below 0.03, the 2048^2 stretched grid's single chart between 0.1 and 0.3, where going
     // on took 1017 steps against 292 for a fresh start)
     double cont_gap = 0.05;
+    // the AMG's coarse launches list the running charts' slices once fewer than this fraction of tiles run
+    double list_below = 0.5;
     int amg_chunk = 4;  // AMG-preconditioned steps between convergence looks
     // step-kernel accounting (optional): an event pair around every launch, summed after each solve
     bool ktime = false;

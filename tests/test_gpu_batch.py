@@ -352,3 +352,45 @@ def test_batch_amg_degenerate_charts(mixed):
         err = np.linalg.norm(xc - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max()
         say(f"  chart {c} ({m} rows): uv err {err:.2e}, iters {it[c]}")
         assert err < 1e-6, c
+
+
+def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
+    """Refinement rounds that go on from the previous round's direction reach the same solutions as fresh
+    restarts; with the continuation test never passing (gap 0, beta = 0) they are bitwise the restarts.  Listing
+    only the running charts' slices for the coarse launches changes no arithmetic: bitwise the full launches."""
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    say("batch AMG refinement: continued / restarted rounds, listed / full coarse launches")
+    meshes = _meshes(24, seed=5)
+    lb = B.LscmBatchBuilder(meshes, angle_min_deg=180.0, threads=4)
+    a = lb.arrays()
+    tol = 1e-9
+
+    def solve(**env):
+        for k in ("RUV_BATCH_RESTART", "RUV_BATCH_CONT_GAP", "RUV_BATCH_LIST_BELOW"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        bt = lb.device_batch(precond=Precond.AMG, mixed=True)
+        return bt.solve(a["rhs"], max_iter=20000, tol=tol)
+
+    x_c, i_c, it_c, r_c = solve()
+    x_r, i_r, it_r, r_r = solve(RUV_BATCH_RESTART="1")
+    x_g, i_g, it_g, r_g = solve(RUV_BATCH_CONT_GAP="0")
+    x_n, _, it_n, _ = solve(RUV_BATCH_LIST_BELOW="0")
+    x_a, _, it_a, _ = solve(RUV_BATCH_LIST_BELOW="2")
+    say(f"  max iters continued {i_c.iters} restarted {i_r.iters}; worst resid {r_c.max():.2e} / {r_r.max():.2e}")
+    assert (r_c <= tol).all() and (r_r <= tol).all()
+    assert np.array_equal(x_g, x_r) and np.array_equal(it_g, it_r)
+    assert np.array_equal(x_n, x_a) and np.array_equal(it_n, it_a)
+    assert np.array_equal(x_a, x_c) and np.array_equal(it_a, it_c)
+    ref = _oracle_solutions(orc, meshes)
+    off = a["chart_row_off"]
+    for x in (x_c, x_r):
+        worst = 0.0
+        for c in range(lb.nchart):
+            xc = x[2 * off[c]:2 * off[c + 1]].reshape(-1, 2)
+            rc = ref[c].reshape(-1, 2)
+            worst = max(worst, np.linalg.norm(xc - rc, axis=1).max() / max(np.linalg.norm(rc, axis=1).max(), 1e-30))
+        assert worst < 1e-6
