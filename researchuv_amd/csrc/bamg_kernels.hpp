@@ -96,11 +96,15 @@ __global__ __launch_bounds__(BLK) void k_scan_add(int64_t n, int64_t* __restrict
 }
 
 // ---------------------------------------------------------------- matching
+// A level's operator during the setup (CSR).  Values in FP32: the cycle stores them in bfloat16 (FP64 under the
+// FP64 cycle, where FP32 setup values still make a fixed SPD preconditioner), and the setup passes read them
+// once or twice each, so half the bytes of FP64 shorten every pass; sums (Galerkin entries, strengths, diagonals)
+// are formed in FP64.
 struct SetupCsr {
     int64_t n;
     const int64_t* ptr;
     const int32_t* cols;
-    const double* vals;
+    const float* vals;
 };
 
 // The cycle's operator values (coarse levels, and the finest level's copy for the smoothing legs) are stored as
@@ -328,7 +332,7 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
                                                       const int64_t* __restrict__ mrows,
                                                       const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
                                                       const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
-                                                      double* __restrict__ cvals, double* __restrict__ cdiag,
+                                                      float* __restrict__ cvals, double* __restrict__ cdiag,
                                                       const int64_t* __restrict__ list) {
     __shared__ int32_t skey[GAL_CAP * GAL_BLK];
     __shared__ double sval[GAL_CAP * GAL_BLK];
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
             while (p < m && skey[p * GAL_BLK + tid] == key) v += sval[(p++) * GAL_BLK + tid];
             if (FILL) {
                 ccols[out] = key;
-                cvals[out] = v;
+                cvals[out] = (float)v;
                 ++out;
                 if (key == I) diag = v;
             }
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
                         if (agg[A.cols[e]] == key) v += A.vals[e];
                 }
                 ccols[out] = (int32_t)key;
-                cvals[out] = v;
+                cvals[out] = (float)v;
                 ++out;
                 if (key == I) diag = v;
             }
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
                                                       const int64_t* __restrict__ mrows,
                                                       const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
                                                       const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
-                                                      double* __restrict__ cvals, double* __restrict__ cdiag,
+                                                      float* __restrict__ cvals, double* __restrict__ cdiag,
                                                       unsigned long long* __restrict__ nover,
                                                       int64_t* __restrict__ over) {
     const int64_t x = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
 #pragma unroll
             for (int t = 0; t < SL; ++t) rank += (key[t] < key[k]) ? 1 : 0;
             ccols[out + rank] = key[k];
-            cvals[out + rank] = val[k];
+            cvals[out + rank] = (float)val[k];
             if (key[k] == I) diag = val[k];
         }
     }

@@ -32,6 +32,7 @@ namespace ruv {
 
 int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals);
 int bamg_build(BatchImpl& Q);
+int bamg_set_fine_values(BatchImpl& Q, const double* d_vals);
 template <typename T, int R, int CI>
 int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z);
 int bamg_set_live(BatchImpl& Q, const int32_t* done);
@@ -714,8 +715,8 @@ int batch_update_values(BatchImpl* Q, const double* d_vals, const int64_t* d_map
         hipLaunchKernelGGL(kb_inv_diag, gr, dim3(BLK), 0, s, Q->nrows, d_vals, d_diagpos, Q->dinv64, Q->dinv32);
     HIPCHK(hipGetLastError());
     if (Q->precond == RUV_PC_AMG) {
-        HIPCHK(hipMemcpyAsync(Q->amg_vals, d_vals, sizeof(double) * Q->nnz, hipMemcpyDeviceToDevice, s));
-        return bamg_build(*Q);
+        int rc = bamg_set_fine_values(*Q, d_vals);
+        return rc ? rc : bamg_build(*Q);
     }
     return RUV_OK;
 }

@@ -211,7 +211,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     unsigned long long hover[2] = {0, 0};
     hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
                        (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                       (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (double*)nullptr,
+                       (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
                        (double*)nullptr, nover, over1);
     HIPCHK(hipMemcpyAsync(&hover[0], nover, sizeof(hover[0]), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -219,7 +219,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     if (n1 > 0) {
         hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
                            (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                           (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (double*)nullptr,
+                           (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
                            (double*)nullptr, nover + 1, over2);
         HIPCHK(hipMemcpyAsync(&hover[1], nover + 1, sizeof(hover[1]), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -229,7 +229,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     if (no > 0)
         hipLaunchKernelGGL((k_galerkin<false>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
                            (const int64_t*)M->mrows, (const int32_t*)M->agg, cnt, (const int64_t*)nullptr,
-                           (int32_t*)nullptr, (double*)nullptr, (double*)nullptr, (const int64_t*)over2);
+                           (int32_t*)nullptr, (float*)nullptr, (double*)nullptr, (const int64_t*)over2);
     HIPCHK(hipGetLastError());
     rc = scan(s, nc, cnt, cptr, S);
     if (rc) return rc;
@@ -238,7 +238,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(hipStreamSynchronize(s));
     if (cnnz >= INT32_MAX) return fail(RUV_ERR_PRECOND, "batch amg: coarse operator too large");
     int32_t* ccols = nullptr;
-    double* cvals = nullptr;
+    float* cvals = nullptr;
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
     HIPCHK(S.alloc(&M->cdiag, nc));
@@ -279,9 +279,9 @@ int coarsest_dense(BatchImpl& Q, Bamg& A, const SetupCsr& cur, const std::vector
     HIPCHK(hipStreamSynchronize(s));
     const int64_t nnz = ptr[n];
     std::vector<int32_t> cols((size_t)std::max<int64_t>(nnz, 1));
-    std::vector<double> vals((size_t)std::max<int64_t>(nnz, 1));
+    std::vector<float> vals((size_t)std::max<int64_t>(nnz, 1));
     HIPCHK(hipMemcpyAsync(cols.data(), cur.cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(vals.data(), cur.vals, sizeof(double) * nnz, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vals.data(), cur.vals, sizeof(float) * nnz, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     std::vector<int64_t> off((size_t)C + 1, 0);
     for (int c = 0; c < C; ++c) {
@@ -372,12 +372,20 @@ int bamg_upload_fine(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, cons
     const int64_t n0 = Q.nrows;
     HIPCHK(Q.upload(&Q.amg_ptr, ptr, (size_t)n0 + 1));
     HIPCHK(Q.upload(&Q.amg_cols, cols, (size_t)Q.nnz));
-    HIPCHK(Q.upload(&Q.amg_vals, vals, (size_t)Q.nnz));
+    std::vector<float> v32(vals, vals + Q.nnz);  // the setup reads FP32 values (SetupCsr)
+    HIPCHK(Q.upload(&Q.amg_vals, v32.data(), v32.size()));
     std::vector<int32_t> g(n0);
     for (int c = 0; c < Q.nchart; ++c)
         for (int64_t i = Q.crow[c]; i < Q.crow[c + 1]; ++i) g[i] = c;
     HIPCHK(Q.upload(&Q.amg_group, g.data(), g.size()));
     HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
+// New finest values (FP64, device, CSR order of bamg_upload_fine) for the next build.
+int bamg_set_fine_values(BatchImpl& Q, const double* d_vals) {
+    hipLaunchKernelGGL((k_cast<float>), rows_grid(Q.nnz), dim3(BLK), 0, Q.stream, Q.nnz, d_vals, Q.amg_vals);
+    HIPCHK(hipGetLastError());
     return RUV_OK;
 }
 

@@ -116,7 +116,7 @@ struct BatchImpl {
     Bamg* amg = nullptr;                   // precond AMG: the hierarchy
     int64_t* amg_ptr = nullptr;            // precond AMG: the finest operator in FP64 CSR for (re)builds,
     int32_t* amg_cols = nullptr;           // and each row's finest-level matching group (its tile)
-    double* amg_vals = nullptr;
+    float* amg_vals = nullptr;
     int32_t* amg_group = nullptr;
 
     template <typename T>
