@@ -146,7 +146,7 @@ __device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
 // Handshake matching.  Each row ranks its eligible neighbours once -- same group, coupling a_ij < 0 with
 // -a_ij >= theta * the row's strongest -- by pair_pref (ties to the lower row), keeping the best MCAND; every round
 // an unmatched row proposes the first of them still unmatched, and mutual proposals become pairs.
-constexpr int MCAND = 8;
+constexpr int MCAND = 4;
 
 __device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, int32_t k) {
     return p > q || (p == q && j < k);
