@@ -26,6 +26,7 @@ patch charts on the device, in triangles per second.
 from __future__ import annotations
 
 import argparse
+import glob
 import json
 import os
 import sys
@@ -262,11 +263,12 @@ def run_batch(args):
                 launches=ks["launches"], bytes_per_launch=ks["bytes"] / ks["launches"],
                 us_per_launch=1e6 * ks["seconds"] / ks["launches"],
                 whole_batch_launch={"us": sec * 1e6, "bytes": byt, "GBps": byt / sec / 1e9})
-    tf = os.path.join(ROOT, "profiles", f"traffic_batch_{args.batch_precond.lower()}.json")
-    if os.path.exists(tf):  # counter-derived HBM bytes per launch, when profiled for this same kernel
+    # counter-derived HBM bytes per launch, from the profile of this same kernel variant (the autotuner picks it)
+    for tf in sorted(glob.glob(os.path.join(ROOT, "profiles", f"traffic_batch_{args.batch_precond.lower()}*.json"))):
         tj = json.load(open(tf))
         if tj.get("kernel") == roof["kernel"]:
             roof["traffic"] = tj.get("hbm_bytes_per_launch")
+            break
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
         config={"workload": f"batch: {lb.nchart} uneven height-field patch charts (64-65536 vertices, log-uniform), "
