@@ -58,13 +58,12 @@ struct Bamg {
     std::vector<int32_t*> slist, blist;
     std::vector<int64_t> nslist, nblist;
     int32_t* d_lists = nullptr;
-    int32_t* h_lists = nullptr;
+    int32_t* h_lists = nullptr;  // the batch's pinned staging buffer (not owned)
     DevPool* pool = nullptr;
     std::vector<void*> mem;
     explicit Bamg(DevPool* p) : pool(p) {}
     ~Bamg() {
         for (void* p : mem) pool->put(p);
-        if (h_lists) (void)hipHostFree(h_lists);
     }
     template <typename T>
     hipError_t alloc(T** p, size_t n) {
@@ -717,7 +716,14 @@ int bamg_build(BatchImpl& Q) {
             cap += (size_t)A->lv[l].nsl + (size_t)((A->lv[l].n + BLK - 1) / BLK);
         }
         HIPCHK(A->alloc(&A->d_lists, cap));
-        HIPCHK(hipHostMalloc((void**)&A->h_lists, sizeof(int32_t) * std::max<size_t>(cap, 1), hipHostMallocDefault));
+        if (cap > Q.h_lists_cap) {  // no copy from it is in flight: every solve ends with a wait on the stream
+            if (Q.h_lists) (void)hipHostFree(Q.h_lists);
+            Q.h_lists = nullptr;
+            Q.h_lists_cap = 0;
+            HIPCHK(hipHostMalloc((void**)&Q.h_lists, sizeof(int32_t) * std::max<size_t>(cap, 1), hipHostMallocDefault));
+            Q.h_lists_cap = cap;
+        }
+        A->h_lists = Q.h_lists;
     }
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));

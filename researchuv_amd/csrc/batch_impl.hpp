@@ -90,6 +90,8 @@ struct BatchImpl {
     double *x64 = nullptr, *r64 = nullptr;
     void *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv0 = nullptr, *pv1 = nullptr, *qv = nullptr;
     int* h_done = nullptr;
+    int32_t* h_lists = nullptr;  // pinned staging of the AMG's running-chart launch lists (grown, kept)
+    size_t h_lists_cap = 0;
     int* h_final = nullptr;
     unsigned* h_iters = nullptr;
     double* h_true = nullptr;
@@ -149,7 +151,7 @@ struct BatchImpl {
         if (amg) bamg_destroy(amg);
         free_tiles();
         for (void* p : mem) (void)hipFree(p);
-        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true})
+        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true, (void*)h_lists})
             if (p) (void)hipHostFree(p);
         for (hipEvent_t e : kev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
