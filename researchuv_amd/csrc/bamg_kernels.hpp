@@ -533,57 +533,6 @@ __global__ __launch_bounds__(BLK) void k_narrow(int64_t n, const int64_t* __rest
     if (i < n) out[i] = (int32_t)in[i];
 }
 
-// sliced ELL of a coarse operator: slice widths, then the fill (padding: value 0, the row itself)
-__global__ __launch_bounds__(BLK) void k_slice_sizes(int64_t n, const int64_t* __restrict__ ptr,
-                                                     int32_t* __restrict__ w, int64_t* __restrict__ sz) {
-    const int64_t s = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    const int64_t nsl = (n + SLICE - 1) / SLICE;
-    if (s >= nsl) return;
-    const int64_t r0 = s * SLICE, r1 = min(n, r0 + SLICE);
-    int64_t m = 0;
-    for (int64_t i = r0; i < r1; ++i) m = max(m, ptr[i + 1] - ptr[i]);
-    w[s] = (int32_t)m;
-    sz[s] = m * (r1 - r0);
-}
-
-// 1 into *wide when some column of the operator is more than 32767 rows from its row
-__global__ __launch_bounds__(BLK) void k_offsets_wide(int64_t n, const int64_t* __restrict__ ptr,
-                                                      const int32_t* __restrict__ cols, int32_t* __restrict__ wide) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= n) return;
-    for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
-        const int64_t d = (int64_t)cols[a] - i;
-        if (d > 32767 || d < -32767) {
-            atomicOr(wide, 1);
-            return;
-        }
-    }
-}
-
-// CI 0: 16-bit offsets from the row; CI 1: 32-bit columns
-template <typename T, int CI>
-__global__ __launch_bounds__(BLK) void k_ell_fill(int64_t n, SetupCsr A, const int64_t* __restrict__ off,
-                                                  const int32_t* __restrict__ w,
-                                                  typename CoarseVal<T>::type* __restrict__ ev,
-                                                  void* __restrict__ ec) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= n) return;
-    const int64_t s = i / SLICE, lane = i - s * SLICE;
-    const int64_t ns = min((int64_t)SLICE, n - s * SLICE);
-    const int64_t e0 = off[s] + lane;
-    int k = 0;
-    for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
-        cput(ev + e0 + k * ns, A.vals[a]);
-        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)((int64_t)A.cols[a] - i);
-        else static_cast<int32_t*>(ec)[e0 + k * ns] = A.cols[a];
-    }
-    for (; k < w[s]; ++k) {
-        cput(ev + e0 + k * ns, 0.0);
-        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = 0;
-        else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)i;
-    }
-}
-
 // group (chart) of every row from the chart row offsets
 __global__ __launch_bounds__(BLK) void k_chart_groups(int64_t n, const int64_t* __restrict__ crow, int nchart,
                                                       int32_t* __restrict__ group) {
