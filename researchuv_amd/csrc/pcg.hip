@@ -76,7 +76,34 @@ struct DevTri {
     T *l_vals = nullptr, *lt_vals = nullptr;
     int *fwd_rows = nullptr, *fwd_off = nullptr, *bwd_rows = nullptr, *bwd_off = nullptr;
     int n_fwd = 0, n_bwd = 0;
+    // Sweep schedule: a level of at least `wide` rows runs as its own many-workgroup launch (k_tri_level), the
+    // runs of narrower levels in single-workgroup launches (k_tri_sweep).  Each segment: first level, end level,
+    // and for a wide level its row range.
+    struct Seg {
+        int l0, l1, a, b;
+        bool wide;
+    };
+    std::vector<Seg> fwd_seg, bwd_seg;
 };
+
+// cut the levels (offsets off, nlev + 1 entries) into wide single levels and runs of narrow ones
+inline std::vector<typename DevTri<float>::Seg> tri_segments(const std::vector<int32_t>& off, int wide) {
+    std::vector<typename DevTri<float>::Seg> out;
+    const int nlev = (int)off.size() - 1;
+    int l = 0;
+    while (l < nlev) {
+        if (off[l + 1] - off[l] >= wide) {
+            out.push_back({l, l + 1, off[l], off[l + 1], true});
+            ++l;
+            continue;
+        }
+        int e = l;
+        while (e < nlev && off[e + 1] - off[e] < wide) ++e;
+        out.push_back({l, e, off[l], off[e], false});
+        l = e;
+    }
+    return out;
+}
 
 template <typename T>
 int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
@@ -94,6 +121,12 @@ int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
     HIPCHK(m.upload(&d->bwd_off, f.bwd_off.data(), f.bwd_off.size(), s));
     d->n_fwd = (int)f.fwd_off.size() - 1;
     d->n_bwd = (int)f.bwd_off.size() - 1;
+    int wide = 16384;  // rows from which one level fills enough workgroups to beat the single-workgroup sweep
+    if (const char* e = std::getenv("RUV_TRI_WIDE")) wide = std::max(1, std::atoi(e));
+    d->fwd_seg.clear();
+    d->bwd_seg.clear();
+    for (const auto& g : tri_segments(f.fwd_off, wide)) d->fwd_seg.push_back({g.l0, g.l1, g.a, g.b, g.wide});
+    for (const auto& g : tri_segments(f.bwd_off, wide)) d->bwd_seg.push_back({g.l0, g.l1, g.a, g.b, g.wide});
     // the host vectors lv/ltv must outlive the async copies
     HIPCHK(hipStreamSynchronize(s));
     return RUV_OK;
@@ -318,10 +351,19 @@ void apply_tri(PlanImpl& P, const DevTri<T>& f, const T* r, T* z, T* y, T* t, co
         hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_orig, r, t, st);
         in = t;
     }
-    hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, f.n_fwd, f.fwd_off, f.fwd_rows,
-                       f.l_ptr, f.l_cols, f.l_vals, in, y, st);
-    hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, f.n_bwd, f.bwd_off, f.bwd_rows,
-                       f.lt_ptr, f.lt_cols, f.lt_vals, (const T*)y, P.permuted ? t : out, st);
+    auto sweep = [&](const std::vector<typename DevTri<T>::Seg>& segs, const int* off, const int* rows,
+                     const int* ptr, const int* cols, const T* vals, const T* x, T* o) {
+        for (const auto& g : segs) {
+            if (g.wide)
+                hipLaunchKernelGGL((k_tri_level<T, R>), dim3((unsigned)((g.b - g.a + BLK - 1) / BLK)), dim3(BLK), 0,
+                                   P.stream, g.a, g.b, rows, ptr, cols, vals, x, o, st);
+            else
+                hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, g.l1 - g.l0, off + g.l0,
+                                   rows, ptr, cols, vals, x, o, st);
+        }
+    };
+    sweep(f.fwd_seg, f.fwd_off, f.fwd_rows, f.l_ptr, f.l_cols, f.l_vals, in, y);
+    sweep(f.bwd_seg, f.bwd_off, f.bwd_rows, f.lt_ptr, f.lt_cols, f.lt_vals, (const T*)y, P.permuted ? t : out);
     if (P.permuted) hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_perm, (const T*)t, out, st);
 }
 

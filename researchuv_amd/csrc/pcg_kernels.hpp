@@ -239,6 +239,32 @@ __global__ __launch_bounds__(TRI_BLK) void k_tri_sweep(int nlev, const int* __re
     }
 }
 
+// One wide level of a triangular sweep over many workgroups: rows rows[a .. b), the same arithmetic per row as
+// k_tri_sweep (a level's rows depend only on earlier levels).
+template <typename T, int R>
+__global__ __launch_bounds__(BLK) void k_tri_level(int a, int b, const int* __restrict__ rows,
+                                                   const int* __restrict__ ptr, const int* __restrict__ cols,
+                                                   const T* __restrict__ vals, const T* __restrict__ in, T* out,
+                                                   const SolveState* st) {
+    if (st && st->done) return;
+    const int t = a + (int)(blockIdx.x * BLK + threadIdx.x);
+    if (t >= b) return;
+    const int i = rows[t];
+    const int e = ptr[i + 1] - 1;  // diagonal last
+    T s[R];
+#pragma unroll
+    for (int c = 0; c < R; ++c) s[c] = in[(int64_t)i * R + c];
+    for (int k = ptr[i]; k < e; ++k) {
+        const T v = vals[k];
+        const int64_t col = (int64_t)cols[k] * R;
+#pragma unroll
+        for (int c = 0; c < R; ++c) s[c] -= v * out[col + c];
+    }
+    const T d = vals[e];
+#pragma unroll
+    for (int c = 0; c < R; ++c) out[(int64_t)i * R + c] = s[c] / d;
+}
+
 // dst[i] = src[idx[i]] per row (permutation round trips, kernels.cu:98-102)
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void k_gather(int n, const int* __restrict__ idx, const T* __restrict__ src,

@@ -317,3 +317,35 @@ def test_graph_replay_matches_direct_launches(gpu, orc, pc, monkeypatch):
         assert i1.iters == idd.iters
     direct.close()
     graph.close()
+
+
+@pytest.mark.parametrize("pc", ["IC0", "IC0_COLOR"])
+def test_triangular_sweep_wide_levels_match_single_workgroup(gpu, orc, pc, monkeypatch):
+    """A triangular-sweep level of many rows runs as its own many-workgroup launch, the narrow levels in one
+    single-workgroup launch per run; every row keeps its arithmetic, so the solves are bitwise those of the
+    all-single-workgroup sweeps (levels forced wide from 32 rows here)."""
+    import researchuv_amd as ra
+
+    say(f"triangular sweeps, wide levels vs single workgroup, {pc}")
+    _, s, _ = lscm_case(orc, orc.grid(64, 64))
+    n = len(s["row_ptr"]) - 1
+    lv, lc, lp = [], [], [0]
+    for i in range(0, n, 2):
+        for k in range(s["row_ptr"][i], s["row_ptr"][i + 1]):
+            lc.append(s["cols"][k] // 2)
+            lv.append(s["vals"][k])
+        lp.append(len(lv))
+    Lm = csr(lv, lc, lp)
+    b = s["rhs"][:n]
+    out = {}
+    for wide in ("32", "1000000000"):
+        monkeypatch.setenv("RUV_TRI_WIDE", wide)
+        for mixed in (False, True):
+            plan = ra.Plan(Lm, 2, getattr(ra.Precond, pc), mixed=mixed)
+            x, info = plan.solve(b, 100_000, 1e-10)
+            plan.close()
+            out[(wide, mixed)] = (x, info.iters)
+    for mixed in (False, True):
+        xw, iw = out[("32", mixed)]
+        xs, is_ = out[("1000000000", mixed)]
+        assert np.array_equal(xw, xs) and iw == is_, mixed
