@@ -1,25 +1,26 @@
 // bamg_kernels.hpp -- device-built aggregation AMG for the batched solver: setup (pairwise matching, Galerkin
 // products) and the V(1,1) cycle.
 //
-// The hierarchy follows the reference's aggregation AMG (crates/researchuv-gpu/src/amg.rs:127-455): rows are
-// matched with their strongest unmatched neighbour, two matchings make one level (amg.rs SA_AGG = 4), the
-// prolongator is the piecewise-constant aggregate map, the coarse operator is the Galerkin product P^T A P,
-// and the cycle smooths with damped Jacobi (omega = 2/3).  What differs: the matching runs on the device by
-// handshakes (a row proposes its preferred unmatched neighbour among those coupled at least 0.1 as strongly as
-// its strongest -- strength in quarter-octave bins, then a pair hash -- and mutual proposals pair up, four
-// rounds), a row left unmatched joins the pair it is most strongly coupled to, every aggregate stays inside
-// one chart, so the hierarchy of a batch is block-diagonal by chart, and
-// the coarsest level is one Jacobi step (exact once a chart has coarsened to a single row).  No inner
-// products: the cycle is a fixed linear SPD operator, so each chart's PCG stays an independent PCG.
+// The hierarchy follows the reference's aggregation AMG (crates/researchuv-gpu/src/amg.rs:127-455) in kind:
+// aggregates of rows strongly coupled to each other, the piecewise-constant aggregate map as the prolongator, the
+// Galerkin product P^T A P as the coarse operator, damped Jacobi smoothing, a direct solve on the coarsest level.
+// What differs, for the device and for batches of charts:
+//  * one pairwise matching per level, by handshakes on the device: each row ranks its neighbours coupled at least
+//    theta (0.25) times as strongly as its strongest -- strength in quarter-octave bins, then a pair hash -- keeps
+//    the best MCAND, and in four rounds proposes the first still unmatched; mutual proposals pair up, and a row
+//    left unmatched joins the pair of its first matched candidate.  Aggregates never leave their chart, so the
+//    hierarchy of a batch is block-diagonal by chart;
+//  * the coarse levels are stored with each next-level aggregate's rows consecutive (the restriction is a sum
+//    over runs of rows, fused into the residual pass) as sliced ELL whose slices start at aggregate starts
+//    (16-bit offsets from the row, or 32-bit columns when some offset does not fit; bfloat16 values under the
+//    FP32 cycle), so every coarse pass streams coalesced;
+//  * V(1,1) with Jacobi weight min(0.9, 1.9 / g) per level (g the level's Gershgorin bound on D^-1 A), coarse
+//    corrections over-corrected by a fixed factor, and each chart's coarsest block inverted (dense) when small.
+// No inner products: the cycle is a fixed linear SPD operator, so each chart's PCG stays an independent PCG.
 //
-// Aggregates are kept as member lists (CSR over the finer level's rows, ascending).  Coarse operators are
-// stored as sliced ELL (slices of 256 consecutive rows padded to their longest row, value k of row lane at
-// off + k * n_s + lane; 16-bit offsets from the row, or 32-bit columns when some offset does not fit), so
-// every coarse pass streams coalesced.
-//
-// Setup works on FP64 CSR levels (64-bit row pointers, 32-bit level-global columns).  Ties in the matching are
-// broken by the lower row index, and every Galerkin row is summed in a fixed order, so the hierarchy is
-// deterministic.
+// Setup works on CSR levels with FP32 values (64-bit row pointers, 32-bit level-global columns; sums in FP64).
+// Ties in the matching are broken by a pair hash and the lower row, and every Galerkin row is summed in a fixed
+// order, so the hierarchy is deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
