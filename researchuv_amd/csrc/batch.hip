@@ -8,7 +8,9 @@
 // host compacts the list of live tiles, so late steps launch only the tiles of the charts still running.
 //
 // Mixed precision: the inner PCG runs in FP32 (matrix, Jacobi diagonal, Krylov vectors); the outer loop keeps
-// x and r = b - A x in FP64 per chart and stops a chart when its true residual ||b - A x|| <= tol ||b||.
+// x and r = b - A x in FP64 per chart and stops a chart when its true residual ||b - A x|| <= tol ||b||.  An
+// inner solve after the first goes on from the previous one's last direction (the residual replaced by the
+// refined one) when the two residuals agree closely, else starts afresh.
 //
 // Schedule: the number of slices per tile (rows per workgroup) is picked per batch by timing the step kernel
 // over the whole batch for each candidate (RUV_BATCH_SLICES_PER_TILE=1|2|4|8 forces one).  The storage form of

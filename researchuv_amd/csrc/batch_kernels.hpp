@@ -1,4 +1,4 @@
-// batch_kernels.hpp -- device side of the batched solver: sliced-ELL storage and a two-kernel PCG iteration.
+// batch_kernels.hpp -- device side of the batched solver: sliced-ELL storage and a three-kernel PCG iteration.
 //
 // Storage.  Every chart is one diagonal block; its rows are cut into slices of at most SLICE (= BLK) rows, one
 // row per thread.  Slice s holds rows [sl_row0, sl_row0 + n_s) in w_s value columns; value k of row lane sits at
