@@ -7,7 +7,7 @@
 // What differs, for the device and for batches of charts:
 //  * one pairwise matching per level, by handshakes on the device: each row ranks its neighbours coupled at least
 //    theta (0.25) times as strongly as its strongest -- strength in quarter-octave bins, then a pair hash -- keeps
-//    the best MCAND, and in four rounds proposes the first still unmatched; mutual proposals pair up, and a row
+//    the best MCAND, and in three rounds proposes the first still unmatched; mutual proposals pair up, and a row
 //    left unmatched joins the pair of its first matched candidate.  Aggregates never leave their chart, so the
 //    hierarchy of a batch is block-diagonal by chart;
 //  * the coarse levels are stored with each next-level aggregate's rows consecutive (the restriction is a sum

@@ -412,7 +412,7 @@ int bamg_build(BatchImpl& Q) {
     // and matchings per level: one (aggregates of about two rows) converges in about half the iterations of two
     // (aggregates of about four) on the heterogeneous weights of jittered meshes, for about 1.5x the coarse work
     double theta = 0.25;
-    int rounds = 4;
+    int rounds = 3;
     if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));
     // coarsening stops once every chart has at most dense_rows rows; the coarsest level is then solved exactly
