@@ -81,6 +81,9 @@ int build_tiles(BatchImpl& Q, int spt) {
     }
     Q.chart_t0[Q.nchart] = (int)tile_chart.size();
     const int T = (int)tile_chart.size();
+    int most = 0;  // tiles of the largest chart
+    for (int c = 0; c < Q.nchart; ++c) most = std::max(most, Q.chart_t0[c + 1] - Q.chart_t0[c]);
+    Q.B.big = most >= 1024 ? 1 : 0;
     std::vector<int32_t> ts0(tile_s0);
     ts0.push_back(Q.nslice);  // tile t spans slices [ts0[t], ts0[t + 1])
     Q.free_tiles();
@@ -308,6 +311,9 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     HIPCHK(Q.alloc(&B.c_bnorm, nchart));
     HIPCHK(Q.alloc(&B.c_true, nchart));
     HIPCHK(Q.alloc(&B.c_rrec, nchart));
+    HIPCHK(Q.alloc(&B.c_sa, nchart));
+    HIPCHK(Q.alloc(&B.c_sb, 2 * (size_t)nchart));
+    HIPCHK(Q.alloc(&B.c_sc, 2 * (size_t)nchart));
     HIPCHK(Q.alloc(&B.c_done, nchart));
     HIPCHK(Q.alloc(&B.c_final, nchart));
     HIPCHK(Q.alloc(&B.c_iters, nchart));
@@ -364,7 +370,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     // step 0 reads its old direction from pb[1]
     hipLaunchKernelGGL((kb_prep<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
                        (const double*)Q.r64, dinv, x, r, z, (const T*)pb[0], pb[1], cont);
-    hipLaunchKernelGGL(kb_init, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol, Q.inner_tol,
+    hipLaunchKernelGGL(kb_init, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol, Q.inner_tol,
                        Q.mode == RUV_MODE_MIXED ? 1 : 0, cont, Q.cont_gap);
     HIPCHK(hipGetLastError());
     if (PC == 2) {  // z0 = M^-1 r0 and its r.z partials (parity 0); kb_init has marked the running charts
@@ -372,6 +378,12 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
         if (!rc) rc = bamg_apply<T, R, CI>(Q, na, 0, (const T*)r, z);
         if (rc) return rc;
     }
+    // charts of very many tiles: the chart-wide sums once per producer, not in every consuming workgroup
+    auto chart_sums = [&](const double* pa, const double* pb, double* sa, double* sb) {
+        if (B.big)
+            hipLaunchKernelGGL(kb_chart_sums, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, pa, pb, sa, sb);
+    };
+    chart_sums(B.part_b, B.part_c, B.c_sb, B.c_sc);  // parity 0
     HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     // steps between looks at which charts still run: growing up to 64 for the cheap Jacobi/plain steps; the AMG
@@ -405,12 +417,15 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
                 Q.kev_used += 2;
                 Q.k_launches += 1;
             }
+            chart_sums(B.part_a, nullptr, B.c_sa, nullptr);
             hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
                                dinv, (const T*)pb[par], (const T*)q, x, r, z);
             if (PC == 2) {
                 rc = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z);
                 if (rc) return rc;
             }
+            chart_sums(B.part_b + (int64_t)(par ^ 1) * Q.ntile, B.part_c + (int64_t)(par ^ 1) * Q.ntile,
+                       B.c_sb + (int64_t)(par ^ 1) * Q.nchart, B.c_sc + (int64_t)(par ^ 1) * Q.nchart);
             par ^= 1;
             ++iter;
         }
@@ -439,7 +454,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
     const bool mixed = Q.mode == RUV_MODE_MIXED;
     Q.iters_total.assign(Q.nchart, 0);
     hipLaunchKernelGGL((kb_start<R>), dim3(Q.ntile), dim3(BLK), 0, s, B, d_b, d_x, Q.r64);
-    hipLaunchKernelGGL(kb_bnorm, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B);
+    hipLaunchKernelGGL(kb_bnorm, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B);
     HIPCHK(hipGetLastError());
     std::fill(Q.h_final, Q.h_final + Q.nchart, 0);
     int na = 0;
@@ -455,7 +470,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
                            (const T*)Q.xv, d_x);
         hipLaunchKernelGGL((kb_residual<R, CI>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
                            (const double*)Q.ev64, d_b, (const double*)d_x, Q.r64);
-        hipLaunchKernelGGL(kb_outer, dim3((Q.nchart + BLK - 1) / BLK), dim3(BLK), 0, s, B, tol);
+        hipLaunchKernelGGL(kb_outer, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_final, B.c_final, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));

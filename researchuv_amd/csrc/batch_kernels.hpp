@@ -64,6 +64,12 @@ struct BDev {
     double* c_bnorm;
     double* c_true;
     double* c_rrec;  // recursive ||r|| of the step an inner solve converged at
+    // big: charts with very many tiles; the step's chart-wide sums come from kb_chart_sums (p.q in c_sa, r.r and
+    // r.z by parity in c_sb, c_sc) instead of every workgroup summing the chart's partials
+    int32_t big;
+    double* c_sa;
+    double* c_sb;
+    double* c_sc;
     int32_t* c_done;
     int32_t* c_final;
     uint32_t* c_iters;
@@ -278,7 +284,12 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
     if (B.c_done[c] != B_RUNNING) return;
     const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
     double rr, rz;
-    sum_range2(B.part_b + (int64_t)par * B.ntile, B.part_c + (int64_t)par * B.ntile, t0, t1, sh, &rr, &rz);
+    if (B.big) {  // summed per chart by kb_chart_sums
+        rr = B.c_sb[(int64_t)par * B.nchart + c];
+        rz = B.c_sc[(int64_t)par * B.nchart + c];
+    } else {
+        sum_range2(B.part_b + (int64_t)par * B.ntile, B.part_c + (int64_t)par * B.ntile, t0, t1, sh, &rr, &rz);
+    }
     const bool lead = (t == t0) && threadIdx.x == 0;
     if (sqrt(rr) <= B.c_tol[c]) {
         if (lead) {
@@ -401,7 +412,7 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
     const int c = B.tile_chart[t];
     if (B.c_done[c] != B_RUNNING) return;
     const int t0 = B.chart_t0[c];
-    const double pq = sum_range(B.part_a, t0, B.chart_t0[c + 1], sh);
+    const double pq = B.big ? B.c_sa[c] : sum_range(B.part_a, t0, B.chart_t0[c + 1], sh);
     if (!(fabs(pq) > 1e-300) || !isfinite(pq)) {
         if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_BREAKDOWN;
         return;
@@ -500,17 +511,17 @@ __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ a
 
 // per chart (not final): inner tolerance, step state.  plain: tol * ||b|| on the recursive residual
 // (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
+// One workgroup per chart (the chart's tile partials summed in a fixed order).
 static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed, int cont,
                                                       double gap) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart) return;
+    __shared__ double sh[WAVES];
+    const int c = blockIdx.x;
     if (B.c_final[c]) {
-        B.c_done[c] = B_SKIP;
+        if (threadIdx.x == 0) B.c_done[c] = B_SKIP;
         return;
     }
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_b[t];
-    const double rn = sqrt(s), bn = B.c_bnorm[c];
+    const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
+    const double rn = sqrt(sum_range(B.part_b, t0, t1, sh)), bn = B.c_bnorm[c];
     // the "old" r.z of step 0.  Fresh start: 1 (the direction is zero, only finiteness matters).  cont: that of
     // the previous inner solve's last step, whose direction kb_prep kept, when the refined residual differs from
     // the recursive one the solve stopped at by at most gap times the latter's norm; a larger difference (FP32
@@ -519,10 +530,10 @@ static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double
     const uint32_t last = B.c_iters[c];
     double rz_old = 1.0;
     if (cont && last > 0) {
-        double g = 0.0;
-        for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) g += B.part_o[t];
+        const double g = sum_range(B.part_o, t0, t1, sh);
         rz_old = (sqrt(g) <= gap * B.c_rrec[c]) ? B.c_rz[(int64_t)((last - 1) & 1) * B.nchart + c] : INFINITY;
     }
+    if (threadIdx.x != 0) return;
     double ta;
     if (mixed) ta = fmax(inner_tol * rn, 0.25 * ((bn <= 1e-300) ? tol : tol * bn));
     else ta = (bn <= 1e-300) ? tol : tol * bn;
@@ -551,11 +562,11 @@ __global__ __launch_bounds__(BLK) void kb_start(BDev B, const double* __restrict
 }
 
 // per chart: ||b||, not final
-static __global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart) return;
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_o[t];
+static __global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {  // one workgroup per chart
+    __shared__ double sh[WAVES];
+    const int c = blockIdx.x;
+    const double s = sum_range(B.part_o, B.chart_t0[c], B.chart_t0[c + 1], sh);
+    if (threadIdx.x != 0) return;
     B.c_bnorm[c] = sqrt(s);
     B.c_final[c] = 0;
     B.c_true[c] = 1.0;
@@ -611,14 +622,37 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
 }
 
 // per chart: true residual; final when ||b - A x|| <= tol ||b||
+// one workgroup per chart
 static __global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
-    const int c = blockIdx.x * BLK + threadIdx.x;
-    if (c >= B.nchart || B.c_final[c]) return;
-    double s = 0.0;
-    for (int t = B.chart_t0[c]; t < B.chart_t0[c + 1]; ++t) s += B.part_o[t];
-    const double rn = sqrt(s), bn = B.c_bnorm[c];
+    __shared__ double sh[WAVES];
+    const int c = blockIdx.x;
+    if (B.c_final[c]) return;
+    const double rn = sqrt(sum_range(B.part_o, B.chart_t0[c], B.chart_t0[c + 1], sh)), bn = B.c_bnorm[c];
+    if (threadIdx.x != 0) return;
     B.c_true[c] = (bn > 0.0) ? rn / bn : rn;
     if (rn <= ((bn <= 1e-300) ? tol : tol * bn)) B.c_final[c] = 1;
+}
+
+// Per running chart (one workgroup each), the fixed-order sums of the tile partials pa (and pb) into sa[c] (sb[c]):
+// for batches with charts of very many tiles, whose every workgroup would otherwise sum them all again.
+static __global__ __launch_bounds__(BLK) void kb_chart_sums(BDev B, const double* __restrict__ pa,
+                                                            const double* __restrict__ pb, double* __restrict__ sa,
+                                                            double* __restrict__ sb) {
+    __shared__ double sh[2 * WAVES];
+    const int c = blockIdx.x;
+    if (B.c_done[c] != B_RUNNING) return;
+    const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
+    if (pb) {
+        double a, b;
+        sum_range2(pa, pb, t0, t1, sh, &a, &b);
+        if (threadIdx.x == 0) {
+            sa[c] = a;
+            sb[c] = b;
+        }
+    } else {
+        const double a = sum_range(pa, t0, t1, sh);
+        if (threadIdx.x == 0) sa[c] = a;
+    }
 }
 
 }  // namespace ruv
