@@ -892,9 +892,12 @@ __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, c
     st<T, R>(L.t, i, o);
 }
 
-// levels between the finest and the coarsest, one workgroup per slice: z = e1 + w D^-1 (b - A e1), e1 in t
-template <typename T, int R, int CI>
-__global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
+// levels between the finest and the coarsest, one workgroup per slice: z = e1 + w D^-1 (b - A e1), e1 in t; or,
+// with FUSED (small levels, where a launch costs more than the extra gathers), e1 formed for every row read
+// exactly as kc_prolong forms it (same operations, so the same values) instead of a kc_prolong pass before.
+template <typename T, int R, int CI, int FUSED = 0>
+__global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w, T alpha = T(0),
+                                               const T* __restrict__ znext = nullptr) {
     const int64_t sb = wg_item(L.slist);
     const int bs = block_state(L.sblk, sb, L.done);
     if (bs == 0) return;
@@ -907,10 +910,20 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w) {
         T s[R];
 #pragma unroll
         for (int c = 0; c < R; ++c) s[c] = T(0);
-        ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i, [&](int64_t j) { return ld<T, R>(L.t, j); }, s);
+        const auto e1 = [&](int64_t j) {
+            if (!FUSED) return ld<T, R>(L.t, j);
+            const T dj = w * L.dinv[j];
+            const Vr<T, R> bj = ld<T, R>(L.b, j);
+            const Vr<T, R> zc = ld<T, R>(znext, L.agg[j]);
+            Vr<T, R> o;
+#pragma unroll
+            for (int c = 0; c < R; ++c) o.c[c] = fma(dj, bj.c[c], alpha * zc.c[c]);
+            return o;
+        };
+        ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i, e1, s);
         const T di = w * L.dinv[i];
         const Vr<T, R> bi = ld<T, R>(L.b, i);
-        const Vr<T, R> ei = ld<T, R>(L.t, i);
+        const Vr<T, R> ei = e1(i);
         Vr<T, R> o;
 #pragma unroll
         for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c] - s[c], ei.c[c]);

@@ -50,6 +50,8 @@ struct Bamg {
     // with aggregates of about two rows the plain correction is too short, and alpha = 1.2 cut the PCG
     // iterations by about a third on height fields (49 -> 34 on a 256^2 patch in a CPU model of the cycle)
     double alpha = 1.2;
+    // levels of at most this many rows fold the prolongation into the post-smoothing pass (kc_post<FUSED>)
+    int64_t fuse_rows = 65536;
     double setup_seconds = 0.0;
     // Lists of the slices and BLK-row blocks of the running charts per level, for launches late in a solve
     // (empty vectors / null pointers: launches cover the whole level).  cs0/cs1 (cb0/cb1): per level, each chart's
@@ -401,6 +403,7 @@ int bamg_build(BatchImpl& Q) {
         const double v = std::atof(e);
         if (v > 0.0 && v < 1.0) A->omega_max = v;
     }
+    if (const char* e = std::getenv("RUV_BATCH_AMG_FUSE_ROWS")) A->fuse_rows = std::atoll(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ALPHA")) {
         const double v = std::atof(e);
         if (v > 0.0 && v < 2.0) A->alpha = v;
@@ -809,10 +812,17 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         const T w = (T)A.lv[l].omega;
         const int64_t ns = sgrid(l), nb = bgrid(l);
         if (ns == 0 || nb == 0) continue;
-        hipLaunchKernelGGL((kc_prolong<T, R>), g1(nb), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha,
-                           (const T*)A.lv[l + 1].z);
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w);
-        else hipLaunchKernelGGL((kc_post<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w);
+        const T* zn = (const T*)A.lv[l + 1].z;
+        if (A.lv[l].n <= A.fuse_rows) {  // small level: the prolongation inside the post-smoothing pass
+            if (A.lv[l].ci)
+                hipLaunchKernelGGL((kc_post<T, R, 1, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+            else
+                hipLaunchKernelGGL((kc_post<T, R, 0, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+            continue;
+        }
+        hipLaunchKernelGGL((kc_prolong<T, R>), g1(nb), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
+        else hipLaunchKernelGGL((kc_post<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
     }
     // up: e1 in level 0's work vector (free once restricted), then the post-smoothing over it (two passes:
     // recomputing e1 for every gathered neighbour inside the post-smoothing measured slower)

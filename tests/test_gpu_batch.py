@@ -357,7 +357,8 @@ def test_batch_amg_degenerate_charts(mixed):
 def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     """Refinement rounds that go on from the previous round's direction reach the same solutions as fresh
     restarts; with the continuation test never passing (gap 0, beta = 0) they are bitwise the restarts.  Listing
-    only the running charts' slices for the coarse launches changes no arithmetic: bitwise the full launches."""
+    only the running charts' slices for the coarse launches, and folding the coarse prolongation into the
+    post-smoothing pass, change no arithmetic: bitwise the full launches and the separate passes."""
     from researchuv_amd import batch as B
     from researchuv_amd.solver import Precond
 
@@ -368,7 +369,7 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     tol = 1e-9
 
     def solve(**env):
-        for k in ("RUV_BATCH_RESTART", "RUV_BATCH_CONT_GAP", "RUV_BATCH_LIST_BELOW"):
+        for k in ("RUV_BATCH_RESTART", "RUV_BATCH_CONT_GAP", "RUV_BATCH_LIST_BELOW", "RUV_BATCH_AMG_FUSE_ROWS"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -380,11 +381,15 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     x_g, i_g, it_g, r_g = solve(RUV_BATCH_CONT_GAP="0")
     x_n, _, it_n, _ = solve(RUV_BATCH_LIST_BELOW="0")
     x_a, _, it_a, _ = solve(RUV_BATCH_LIST_BELOW="2")
+    x_u, _, it_u, _ = solve(RUV_BATCH_AMG_FUSE_ROWS="0")  # prolongation always a pass of its own
+    x_v, _, it_v, _ = solve(RUV_BATCH_AMG_FUSE_ROWS="1000000000")  # always inside the post-smoothing
     say(f"  max iters continued {i_c.iters} restarted {i_r.iters}; worst resid {r_c.max():.2e} / {r_r.max():.2e}")
     assert (r_c <= tol).all() and (r_r <= tol).all()
     assert np.array_equal(x_g, x_r) and np.array_equal(it_g, it_r)
     assert np.array_equal(x_n, x_a) and np.array_equal(it_n, it_a)
     assert np.array_equal(x_a, x_c) and np.array_equal(it_a, it_c)
+    assert np.array_equal(x_u, x_c) and np.array_equal(it_u, it_c)
+    assert np.array_equal(x_v, x_c) and np.array_equal(it_v, it_c)
     ref = _oracle_solutions(orc, meshes)
     off = a["chart_row_off"]
     for x in (x_c, x_r):
