@@ -83,7 +83,9 @@ int build_tiles(BatchImpl& Q, int spt) {
     const int T = (int)tile_chart.size();
     int most = 0;  // tiles of the largest chart
     for (int c = 0; c < Q.nchart; ++c) most = std::max(most, Q.chart_t0[c + 1] - Q.chart_t0[c]);
-    Q.B.big = most >= 1024 ? 1 : 0;
+    int big_tiles = 1024;  // from this many tiles on, a chart's sums are formed once (kb_chart_sums)
+    if (const char* e = std::getenv("RUV_BATCH_BIG_TILES")) big_tiles = std::max(1, std::atoi(e));
+    Q.B.big = most >= big_tiles ? 1 : 0;
     std::vector<int32_t> ts0(tile_s0);
     ts0.push_back(Q.nslice);  // tile t spans slices [ts0[t], ts0[t + 1])
     Q.free_tiles();

@@ -358,7 +358,8 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     """Refinement rounds that go on from the previous round's direction reach the same solutions as fresh
     restarts; with the continuation test never passing (gap 0, beta = 0) they are bitwise the restarts.  Listing
     only the running charts' slices for the coarse launches, and folding the coarse prolongation into the
-    post-smoothing pass, change no arithmetic: bitwise the full launches and the separate passes."""
+    post-smoothing pass, and forming the chart-wide sums once per chart, change no arithmetic: bitwise the full
+    launches, the separate passes and the per-workgroup sums."""
     from researchuv_amd import batch as B
     from researchuv_amd.solver import Precond
 
@@ -369,7 +370,8 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     tol = 1e-9
 
     def solve(**env):
-        for k in ("RUV_BATCH_RESTART", "RUV_BATCH_CONT_GAP", "RUV_BATCH_LIST_BELOW", "RUV_BATCH_AMG_FUSE_ROWS"):
+        for k in ("RUV_BATCH_RESTART", "RUV_BATCH_CONT_GAP", "RUV_BATCH_LIST_BELOW", "RUV_BATCH_AMG_FUSE_ROWS",
+                  "RUV_BATCH_BIG_TILES"):
             monkeypatch.delenv(k, raising=False)
         for k, v in env.items():
             monkeypatch.setenv(k, v)
@@ -383,6 +385,7 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     x_a, _, it_a, _ = solve(RUV_BATCH_LIST_BELOW="2")
     x_u, _, it_u, _ = solve(RUV_BATCH_AMG_FUSE_ROWS="0")  # prolongation always a pass of its own
     x_v, _, it_v, _ = solve(RUV_BATCH_AMG_FUSE_ROWS="1000000000")  # always inside the post-smoothing
+    x_s, _, it_s, _ = solve(RUV_BATCH_BIG_TILES="1")  # chart-wide sums once per chart (kb_chart_sums)
     say(f"  max iters continued {i_c.iters} restarted {i_r.iters}; worst resid {r_c.max():.2e} / {r_r.max():.2e}")
     assert (r_c <= tol).all() and (r_r <= tol).all()
     assert np.array_equal(x_g, x_r) and np.array_equal(it_g, it_r)
@@ -390,6 +393,7 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
     assert np.array_equal(x_a, x_c) and np.array_equal(it_a, it_c)
     assert np.array_equal(x_u, x_c) and np.array_equal(it_u, it_c)
     assert np.array_equal(x_v, x_c) and np.array_equal(it_v, it_c)
+    assert np.array_equal(x_s, x_c) and np.array_equal(it_s, it_c)
     ref = _oracle_solutions(orc, meshes)
     off = a["chart_row_off"]
     for x in (x_c, x_r):
