@@ -15,12 +15,20 @@
  *   orc_cube / orc_grid ............ counts 294/432, 11*9/160 (meshgen.rs:330-368)
  *   orc_weld ....................... 218/432 cube (parity.rs:31-33), seam + degenerate tests (weld.rs:69-125)
  *   orc_segment .................... 6x49 cube charts, flat grid / triangle / tetra loops (segment.rs:456-510),
- *                                    cylinder 2 loops (meshgen.rs:370-380), annulus border 128 (parity.rs:155)
- *   orc_eig3 / orc_tangent_frame ... known spectra (lscm_eig.rs:72-120), numpy.linalg.eigh, planar-isometry check
+ *                                    cylinder 2 loops (meshgen.rs:370-380), annulus border 128 (parity.rs:155),
+ *                                    hand-walked loops through a pinch vertex: straightest turn, strict-max
+ *                                    tie keeping the smaller id, smallest first step (segment.rs:258-330)
+ *   orc_eig3 / orc_tangent_frame ... known spectra (lscm_eig.rs:72-120), numpy.linalg.eigh, planar-isometry check,
+ *                                    one-rotation sign convention in each plane, a hand-derived two-rotation
+ *                                    case fixing the accumulation order V' = Q V (lscm_eig.rs:57-62)
  *   orc_chart_system (pins, weights, assembly) ... cube closed form (L = 72 x 5-point Laplacian, pins on the
  *                                    unit-square perimeter), symmetric/row-sum invariants, dense numpy rebuild
  *   orc_sparse_solve ............... sparse.rs:272-423 known answers, numpy.linalg.solve brute force
  *   orc_unfold_chart ............... cube affine closed form, iters == 2, dense-solve parity
+ *   orc_rect_point / orc_anchor_ids  perimeter walk known answers, farthest-point anchors on a square
+ *   orc_par_unfold ................. bitwise equal for 1, 2, 3 and 8 workers and to the serial driver
+ *   orc_lscm_loss_frozen ........... affine maps on a right-triangle grid: uv == A x + t, closed-form energy
+ *                                    2 nx ny (hy/hx |A e_x|^2 + hx/hy |A e_y|^2) plus the target term
  *   orc_cg_solve ................... 1-D Laplacian x == 1 (solver.rs:767-779)
  *   orc_ic0_* ...................... L*L^T == A on the pattern (precond.rs:708-737), level partition (:739-755),
  *                                    dense triangular solve of the same factor
@@ -32,8 +40,7 @@
  *   orc_rectangularize ............. unit square on its border (metrics.rs:258-279); a rectangle rotated by a
  *                                    ladder angle comes back axis-aligned with its side lengths; border-free
  *                                    normalisation to [0,1]^2
- *   Parity unpinned (no independent check exists without running the Rust code): the tie-break order inside
- *   chart_border_loops' straight walk and the eigenvector sign convention of eig3.  Both are restated literally.
+ *   Every function above has a check independent of its restatement.
  */
 #include <math.h>
 #include <stdint.h>

@@ -77,6 +77,29 @@ def cylinder(r, height, nu, nv):
     return np.array(p, np.float64), np.array(f, np.uint32)
 
 
+def pinched_grid():
+    """A flat 3 x 3 quad grid (vertex i + 4 j at (i, j, 0)) without the centre quad and the corner quad
+    (2, 2): the hole and the notch meet at vertex 10 = (2, 2), which has four boundary edges.  The seven
+    remaining quads form one edge-connected ring."""
+    p = np.array([(i, j, 0.0) for j in range(4) for i in range(4)], np.float64)
+    f = []
+    for j in range(3):
+        for i in range(3):
+            if (i, j) in ((1, 1), (2, 2)):
+                continue
+            a = j * 4 + i
+            f += [(a, a + 1, a + 5), (a, a + 5, a + 4)]
+    return p, np.array(f, np.uint32)
+
+
+def pinched_grid_tie():
+    """The pinched grid with its top row moved one unit to -x: vertex 14 sits at (1, 3), so from the
+    pinch vertex 10 = (2, 2) the neighbours 9 = (1, 2) and 14 lie at the same distance along -x."""
+    p, f = pinched_grid()
+    p[12:, 0] -= 1.0
+    return p, f
+
+
 def laplacian_1d(n):
     """solver.rs:743-765: tridiagonal (-1, 2, -1); x = 1 solves b = e_0 + e_{n-1}."""
     vals, cols, ptr = [], [], [0]

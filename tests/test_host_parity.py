@@ -22,6 +22,8 @@ CASES = {
     "annulus": lambda orc: meshes.torus_annulus(2.0, 0.7, 24, 16),
     "sphere": lambda orc: meshes.uv_sphere(16, 8),
     "torus": lambda orc: meshes.torus(2.0, 0.7, 16, 10),
+    "pinched": lambda orc: meshes.pinched_grid(),
+    "pinched_tie": lambda orc: meshes.pinched_grid_tie(),
 }
 ANGLES = {"sphere": 180.0, "torus": 180.0}
 

@@ -121,6 +121,44 @@ def test_annulus_and_sphere_counts(orc):
     assert len(pw) == 1036 and len(fw) == 2068
 
 
+def test_border_walk_goes_straight_through_a_pinch_vertex(orc):
+    """Known answer for the keep-going-straight rule (segment.rs:277-320), worked by hand.  Vertex
+    10 = (2, 2) of the pinched grid has boundary neighbours 11, 14, 6 and 9.  The walk starts at the
+    smallest boundary vertex 0 and steps to its smallest neighbour 1 (segment.rs:272-279), runs along
+    y = 0 and x = 3 and enters 10 from 11, heading -x: 9 continues straight (dot 1), 14 and 6 turn
+    (dot 0).  It then circles the hole 9, 5, 6, enters 10 again heading +y, where 14 is the one
+    unused edge left, and closes along y = 3 and x = 0: one loop through all 16 boundary edges.  A
+    walk that turned at 10 would give two loops (13 and 5 entries)."""
+    ch = orc.Charts(*meshes.pinched_grid(), 30.0)
+    assert len(ch) == 1 and len(ch.cut()) == 0
+    c = ch.chart(0)
+    assert len(c["border"]) == 15
+    assert [list(x) for x in c["border_loops"]] == [[0, 1, 2, 3, 7, 11, 10, 9, 5, 6, 10, 14, 13, 12, 8, 4, 0]]
+
+
+def test_border_walk_prefers_the_straightest_of_unequal_turns(orc):
+    """The same pinch, sheared by x' = x + y / 2: entering 10 from 11 heads -x, and the candidates
+    9 (-1, 0), 14 (1/2, 1) and 6 (-1/2, -1) score 1, -1/2 and 1/2.  The largest wins, so the loop is
+    the one above; the smallest (a sign slip in the rule) would turn to 14 and split it."""
+    p, f = meshes.pinched_grid()
+    p = p.copy()
+    p[:, 0] += 0.5 * p[:, 1]
+    c = orc.Charts(p, f, 30.0).chart(0)
+    assert [list(x) for x in c["border_loops"]] == [[0, 1, 2, 3, 7, 11, 10, 9, 5, 6, 10, 14, 13, 12, 8, 4, 0]]
+
+
+
+def test_border_walk_tie_keeps_the_smaller_vertex(orc):
+    """Ties (segment.rs:300-306): a candidate replaces the best only with a strictly larger dot, and
+    the candidates come in ascending id order (the adjacency is filled from a sorted edge set,
+    segment.rs:235-254).  In the tie variant, entering 10 from 11 heading -x, both 9 = (1, 2) and
+    14 = (1, 3) score exactly 1 and 6 scores 0: 9 is kept and the single 17-entry loop results.
+    Keeping the later one (>=) would close the outer loop through 14 first and leave the hole as a
+    second loop [5, 6, 10, 9, 5]."""
+    c = orc.Charts(*meshes.pinched_grid_tie(), 30.0).chart(0)
+    assert [list(x) for x in c["border_loops"]] == [[0, 1, 2, 3, 7, 11, 10, 9, 5, 6, 10, 14, 13, 12, 8, 4, 0]]
+
+
 # ------------------------------------------------------------------ eig3 / frame / pins
 def test_eig3_known_spectra(orc):
     w, _ = orc.eig3(np.diag([2.0, 5.0, 1.0]))  # lscm_eig.rs:72-79
@@ -153,6 +191,59 @@ def test_eig3_eigenvalues_match_numpy_eigh(orc):
         m[2, 2] = 7.0
         w, v = orc.eig3(m)
         np.testing.assert_allclose(m @ v, v * w, atol=1e-9 * np.abs(w).max())
+
+
+def test_eig3_single_rotation_sign_convention(orc):
+    """Sign convention of one Jacobi rotation (lscm_eig.rs:33-62).  A matrix coupled in one plane
+    (p, q) is diagonalised by a single rotation through theta = atan2(2 a_pq, a_pp - a_qq) / 2, which
+    lies in (-pi/2, pi/2]: so V is the identity outside the plane, V[p, p] = V[q, q] = cos(theta) >= 0,
+    V[q, p] = -V[p, q] = sin(theta) has the sign of a_pq, det V = +1, and column k is the eigenvector
+    of w[k].  Rotating V the other way (V' = Q^T V) or by the wrong angle fails one of these."""
+    rng = np.random.default_rng(5)
+    for p, q in ((0, 1), (0, 2), (1, 2)):
+        r = 3 - p - q
+        for _ in range(8):
+            m = np.zeros((3, 3))
+            m[p, p], m[q, q], m[r, r] = rng.uniform(0.5, 4.0, size=3)
+            m[p, q] = m[q, p] = rng.choice([-1.0, 1.0]) * rng.uniform(0.1, 2.0)
+            w, v = orc.eig3(m)
+            assert v[r, r] == 1.0 and v[r, p] == v[r, q] == v[p, r] == v[q, r] == 0.0
+            assert v[p, p] == v[q, q] and v[p, p] >= 0.0
+            assert v[q, p] == -v[p, q] and np.sign(v[q, p]) == np.sign(m[p, q])
+            assert abs(np.linalg.det(v) - 1.0) < 1e-12
+            np.testing.assert_allclose(m @ v, v * w, atol=1e-12 * np.abs(w).max())
+    # a coupling in every plane needs several rotations: V stays a proper rotation
+    for _ in range(10):
+        b = rng.normal(size=(3, 3))
+        _, v = orc.eig3(b @ b.T)
+        assert abs(np.linalg.det(v) - 1.0) < 1e-12
+
+
+def test_eig3_accumulates_two_rotations_in_the_reference_order(orc):
+    """M = [[3, 1, e], [1, 3, e], [e, e, 1]], e = 0.3, is diagonalised by two rotations worked by hand:
+    the largest off-diagonal (0, 1) has a_00 = a_11, so theta_1 = pi/4 and the rotated matrix has
+    a_00 = 4, a_11 = 2, a_20 = sqrt(2) e, a_21 = 0; the second rotation is in the (0, 2) plane with
+    theta_2 = atan2(2 sqrt(2) e, 4 - 1) / 2 and leaves nothing off the diagonal (later rotations act on
+    rounding residue only).  The eigenvector matrix is Q1 Q2; the reference accumulates V' = Q V
+    (lscm_eig.rs:57-62) and returns Q2 Q1, whose columns are not eigenvectors."""
+    e = 0.3
+    m = np.array([[3.0, 1.0, e], [1.0, 3.0, e], [e, e, 1.0]])
+
+    def rot(p, q, t):
+        r = np.eye(3)
+        r[p, p] = r[q, q] = np.cos(t)
+        r[p, q], r[q, p] = -np.sin(t), np.sin(t)
+        return r
+
+    q1 = rot(0, 1, np.pi / 4)
+    q2 = rot(0, 2, 0.5 * np.arctan2(2.0 * np.sqrt(2.0) * e, 3.0))
+    eig = q1 @ q2
+    d = eig.T @ m @ eig
+    np.testing.assert_allclose(d - np.diag(np.diag(d)), 0.0, atol=1e-14)  # the hand derivation
+    w, v = orc.eig3(m)
+    np.testing.assert_allclose(w, np.diag(d), atol=1e-13)
+    np.testing.assert_allclose(v, q2 @ q1, atol=1e-13)
+    assert np.abs(v - eig).max() > 0.05 and np.abs(m @ v - v * w).max() > 0.05
 
 
 def test_tangent_frame_is_an_isometry_of_planar_points(orc):
@@ -372,6 +463,39 @@ def test_worker_thread_path_is_bitwise_stable(orc):
 
 
 # ------------------------------------------------------------------ CG / IC(0) / colouring
+def test_frozen_pin_loss_closed_form_on_affine_maps(orc):
+    """Closed form for the frozen-pin loss used as the adjoint's reference (weights of
+    lscm.rs:252-274: w_e sums the cotangents of the angles opposite e, c_e = w_e / |e|^2).  On an
+    nx x ny grid of hx x hy cells split into right triangles, every x-edge gets cot = hy/hx from each
+    triangle beside it, every y-edge hx/hy, every diagonal 0 (its opposite angle is right).  Opposite
+    edges at an interior vertex carry equal weights, so affine maps are reproduced: with the border
+    pinned at A x + t the solve returns A x + t everywhere, and E = sum_e c_e |A d_e|^2
+    = 2 nx ny (hy/hx |A e_x|^2 + hx/hy |A e_y|^2).  The target term adds |uv - target|^2; a rigid motion
+    of the 3-D positions changes nothing.  A cotangent of the wrong angle, a weight on the wrong edge or
+    a lost factor fails one of these."""
+    nx, ny, hx, hy = 7, 5, 0.5, 1.25
+    pos = np.array([(i * hx, j * hy, 0.0) for j in range(ny + 1) for i in range(nx + 1)])
+    tris = []
+    for j in range(ny):
+        for i in range(nx):
+            a = j * (nx + 1) + i
+            tris += [(a, a + 1, a + nx + 2), (a, a + nx + 2, a + nx + 1)]
+    tris = np.array(tris, np.uint32)
+    ij = np.array([(i, j) for j in range(ny + 1) for i in range(nx + 1)])
+    border = np.flatnonzero((ij[:, 0] == 0) | (ij[:, 0] == nx) | (ij[:, 1] == 0) | (ij[:, 1] == ny)).astype(np.int32)
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    for A in (np.eye(2), np.array([[2.0, 0.5], [-0.3, 0.7]]), np.array([[0.0, 1.0], [-1.0, 0.0]])):
+        t = np.array([0.3, -1.1])
+        lin = pos[:, :2] @ A.T + t
+        delta = rng.normal(size=lin.shape) * 0.1
+        for p3 in (pos, pos @ q.T + 5.0):
+            loss, uv = orc.lscm_loss_frozen(p3, tris, border, lin[border], lin + delta)
+            np.testing.assert_allclose(uv, lin, atol=1e-12)
+            e_exact = 2.0 * nx * ny * (hy / hx * np.sum(A[:, 0] ** 2) + hx / hy * np.sum(A[:, 1] ** 2))
+            np.testing.assert_allclose(loss, e_exact + np.sum(delta * delta), rtol=1e-12)
+
+
 def test_cpu_cg_1d_laplacian(orc):
     v, c, p, b = meshes.laplacian_1d(200)
     x = orc.cg_solve(v, c, p, b, 2000)
