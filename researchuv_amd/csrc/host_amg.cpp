@@ -235,7 +235,7 @@ bool amg_hierarchy(const Csr& a, bool smooth, std::vector<AmgHostLevel>* levels,
         L.op = cur;
         if (coarsest) {
             if (!L.diag_only) {
-                if (!dense_inverse(cur, &L.dense_chol)) {
+                if (!dense_inverse(cur, &L.dense_inv)) {
                     *why = "coarsest level is not positive definite";
                     return false;
                 }

@@ -183,7 +183,7 @@ int upload_amg(DevArena& m, const std::vector<AmgHostLevel>& H, int R, hipStream
             HIPCHK(m.upload(&L.r_cols, h.r.cols.data(), h.r.cols.size(), s));
             HIPCHK(up(&L.r_vals, h.r.vals));
         } else if (h.has_factor) {
-            HIPCHK(m.upload(&L.dense_inv, h.dense_chol.data(), h.dense_chol.size(), s));
+            HIPCHK(m.upload(&L.dense_inv, h.dense_inv.data(), h.dense_inv.size(), s));
         }
     }
     HIPCHK(hipStreamSynchronize(s));

@@ -45,9 +45,6 @@ Csr transpose_lower(const Csr& l);
 std::vector<uint32_t> graph_coloring(const Csr& a);
 void color_permutation(const std::vector<uint32_t>& color, std::vector<int32_t>* to_perm, std::vector<int32_t>* to_orig);
 Csr permute_csr(const Csr& a, const std::vector<int32_t>& to_perm, const std::vector<int32_t>& to_orig);
-// nd_order / factor_exact (precond.rs:300-498)
-std::vector<int32_t> nd_order(const Csr& a);
-bool factor_exact(const Csr& a, const std::vector<int32_t>& to_orig, TriFactor* f);
 
 // AMG hierarchy (amg.rs:127-455), host side.
 struct AmgHostLevel {
@@ -57,9 +54,9 @@ struct AmgHostLevel {
     int32_t n_coarse = 0;
     bool diag_only = false;
     bool has_factor = false;
-    TriFactor factor;              // exact coarse factor (coarsest level)
-    std::vector<int32_t> to_orig;  // its elimination order
-    std::vector<double> dense_chol;  // coarsest level: dense Cholesky factor (row-major lower), n <= COARSEST
+    // coarsest level (n <= 512): the dense inverse A^-1 (row-major n x n) from a dense Cholesky, in place of
+    // the reference's sparse exact factor under a nested-dissection order (precond.rs:300-498)
+    std::vector<double> dense_inv;
 };
 bool amg_hierarchy(const Csr& a, bool smooth, std::vector<AmgHostLevel>* levels, std::string* why);
 void amg_aggregate(const Csr& a, int max_members, std::vector<int32_t>* agg, int32_t* n_coarse);
