@@ -68,9 +68,12 @@ int ruv_available(void);
 const char* ruv_last_error(void);
 
 /* GpuSolver::cg_solve_precond (solver.rs:284-565): PCG in FP64, x0 = 0, stop when the recursive residual
- * satisfies ||r|| <= tol*||b|| (tol absolute when ||b|| <= 1e-300), at most max_iter iterations; flexible CG
- * for the AMG preconditioners (solver.rs:474-481).  A is SPD CSR (vals/cols of length nnz, row_ptr of length
- * n+1), b of length b_len (must equal n).  x_out (n) and iters_out are written on success.  An interleaved
+ * satisfies ||r|| <= tol*||b|| (tol absolute when ||b|| <= 1e-300), at most max_iter iterations.  The
+ * reference ran flexible CG for its AMG preconditioners (solver.rs:474-481) because its K-cycle varies from
+ * one application to the next; here AMG is a fixed symmetric V(1,1) cycle (damped Jacobi smoothing, exact
+ * coarsest solve), an SPD operator, so every preconditioner runs under plain PCG (beta = rz_new / rz).
+ * A is SPD CSR (vals/cols of length nnz, row_ptr of length n+1), b of length b_len (must equal n).
+ * x_out (n) and iters_out are written on success.  An interleaved
  * A = L (x) I2 (the LSCM free block of lscm.rs:605-640 as produced by Sparse::to_csr) is detected and
  * streamed as L against two right-hand sides; the iterates are those of CG on A. */
 int ruv_cg_solve_precond(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
@@ -139,10 +142,12 @@ int ruv_batch_solve_dev(ruv_batch* batch, const double* d_b, double* d_x, uint64
                         ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
 int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid);
-/* Measurement hook: mean duration and algorithmic bytes of the batch's inner-precision step kernel (new
- * search direction + SpMV) over every tile: the stored values and column indices, the slice tables, and four
- * vectors (z, old p read; new p, q written) streamed once. */
-int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch);
+/* Measurement hook: mean duration of the batch's inner-precision SpMV kernel (q = A p' with the p'.q partials)
+ * over every tile, and its bytes per launch: algorithmic (every nonzero's value and, in a general slice, its
+ * stored column index; the slice tables; p' read and q written once) and as stored (the zero slots that pad a
+ * sliced-ELL slice to its width included; null to skip). */
+int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch,
+                        double* stored_bytes_per_launch);
 /* The batch's storage and the schedule its autotuner chose: rows per workgroup tile; bytes per stored column
  * index of the general slices (2: 16-bit offsets from the row, 4: 32-bit rows); stored values (padding
  * included) and stored column indices; slices in stencil form (column offsets per slice, none per entry) and
@@ -150,9 +155,9 @@ int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_laun
 int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
                        int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal);
 /* Step-kernel accounting over the solves that follow (measurement hook): with enable != 0 every launch of the
- * step kernel is bracketed by HIP events on the batch's stream.  ruv_batch_kernel_stats returns, and resets,
- * the launch count, the summed device time of those launches and their algorithmic bytes (per chart: its
- * sliced-ELL entries as stored, slice table and four vectors, times the steps the chart ran). */
+ * SpMV kernel is bracketed by HIP events on the batch's stream.  ruv_batch_kernel_stats returns, and resets,
+ * the launch count, the summed device time of those launches, their algorithmic bytes and their bytes as
+ * stored (per chart as in ruv_batch_spmv_time, times the steps the chart ran; any pointer may be null). */
 int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable);
 /* precond amg: levels of the device-built aggregation hierarchy (finest included), the device time its setup
  * took inside ruv_batch_create, and the rows of all coarse levels together; zeros for the other
@@ -164,7 +169,8 @@ int ruv_batch_setup_precond(ruv_batch* batch);
 /* precond amg: rows of each level, finest first, into rows[0..cap); returns the number of levels (0 for the
  * other preconditioners). */
 int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap);
-int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes);
+int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes,
+                           double* stored_bytes);
 void ruv_batch_destroy(ruv_batch* batch);
 
 /* Build the LSCM systems of nmesh meshes on host worker threads (the worker-thread unfold stage,
@@ -290,9 +296,13 @@ int ruv_chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, co
  * normalisation (the pack rectangle, pipeline.rs:178). */
 int ruv_rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                        const int64_t* bnd, double* q, double* ext);
-/* The same two with every array a device pointer (inputs resident in HBM, results written there), launched on
- * the null stream and synchronous.  Offsets are trusted; a triangle corner or border id outside its chart is
- * skipped on the device and reported as RUV_ERR_BAD_DIMS; repeated border ids give an unspecified snap. */
+/* The same two with every array a device pointer (inputs resident in HBM, results written there).  Each call
+ * runs on a stream of its own (leased for the call; concurrent calls lease different streams and scratch
+ * buffers) that first waits for the work issued so far on the null stream, so inputs written there or on any
+ * blocking stream are read complete; inputs written on another non-blocking stream must be complete (that
+ * stream synchronised) before the call.  The call returns when its results are written.  Offsets are trusted;
+ * a triangle corner or border id outside its chart is skipped on the device and reported as RUV_ERR_BAD_DIMS;
+ * repeated border ids give an unspecified snap. */
 int ruv_chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
                           const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
 int ruv_rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,

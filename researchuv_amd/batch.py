@@ -29,7 +29,7 @@ def _setup(L):
     L.ruv_batch_solve.restype = C.c_int
     L.ruv_batch_solve.argtypes = [P, P, P, C.c_uint64, C.c_double, P, P, P]
     L.ruv_batch_spmv_time.restype = C.c_int
-    L.ruv_batch_spmv_time.argtypes = [P, C.c_int32, P, P]
+    L.ruv_batch_spmv_time.argtypes = [P, C.c_int32, P, P, P]
     L.ruv_batch_schedule.restype = C.c_int
     L.ruv_batch_kernel_timing.restype = C.c_int
     L.ruv_batch_precond_info.restype = C.c_int
@@ -40,7 +40,7 @@ def _setup(L):
     L.ruv_batch_precond_info.argtypes = [P, P, P, P]
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
     L.ruv_batch_kernel_stats.restype = C.c_int
-    L.ruv_batch_kernel_stats.argtypes = [P, P, P, P]
+    L.ruv_batch_kernel_stats.argtypes = [P, P, P, P, P]
     L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
@@ -116,11 +116,14 @@ class Batch:
                                              C.byref(info), _p(it), _p(res)))
         return (info, it, res) if per_chart else info
 
-    def spmv_time(self, reps=50) -> Tuple[float, float]:
+    def spmv_time(self, reps=50, stored=False):
+        """Mean seconds per launch of the SpMV kernel over the whole batch and its algorithmic bytes (nonzeros
+        only); with stored=True also the bytes as stored (padding slots included)."""
         sec = C.c_double(0)
         byt = C.c_double(0)
-        _lib.check(lib().ruv_batch_spmv_time(self.h, reps, C.byref(sec), C.byref(byt)))
-        return sec.value, byt.value
+        sto = C.c_double(0)
+        _lib.check(lib().ruv_batch_spmv_time(self.h, reps, C.byref(sec), C.byref(byt), C.byref(sto)))
+        return (sec.value, byt.value, sto.value) if stored else (sec.value, byt.value)
 
     def schedule(self) -> dict:
         """The autotuned schedule: rows per workgroup tile, bytes per column index, stored ELL entries."""
@@ -156,12 +159,14 @@ class Batch:
         _lib.check(lib().ruv_batch_kernel_timing(self.h, 1 if enable else 0))
 
     def kernel_stats(self) -> dict:
-        """Launches, summed device seconds and algorithmic bytes of the step kernel since the last call."""
+        """Launches, summed device seconds, algorithmic bytes (nonzeros only) and bytes as stored (padding
+        included) of the SpMV kernel since the last call."""
         n = C.c_int64(0)
         sec = C.c_double(0)
         byt = C.c_double(0)
-        _lib.check(lib().ruv_batch_kernel_stats(self.h, C.byref(n), C.byref(sec), C.byref(byt)))
-        return dict(launches=n.value, seconds=sec.value, bytes=byt.value)
+        sto = C.c_double(0)
+        _lib.check(lib().ruv_batch_kernel_stats(self.h, C.byref(n), C.byref(sec), C.byref(byt), C.byref(sto)))
+        return dict(launches=n.value, seconds=sec.value, bytes=byt.value, stored_bytes=sto.value)
 
     def close(self):
         if self.h:

@@ -20,14 +20,14 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
 void batch_destroy(BatchImpl* q);
 int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_iter, double tol,
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_resid);
-int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes);
+int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes, double* stored);
 int batch_schedule(BatchImpl* Q, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
                    int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal);
 int batch_kernel_timing(BatchImpl* Q, int enable);
 int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
 int batch_setup_precond(BatchImpl* Q);
 int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap);
-int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes);
+int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes, double* stored);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
                      LscmBatch** out);
@@ -331,9 +331,10 @@ int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_i
     return rc;
 }
 
-int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch) {
+int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_launch, double* bytes_per_launch,
+                        double* stored_bytes_per_launch) {
     if (!batch || !seconds_per_launch || !bytes_per_launch) return ruv::fail(RUV_ERR_INVALID, "null argument");
-    return ruv::batch_spmv_time(batch->impl, reps, seconds_per_launch, bytes_per_launch);
+    return ruv::batch_spmv_time(batch->impl, reps, seconds_per_launch, bytes_per_launch, stored_bytes_per_launch);
 }
 
 int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
@@ -363,9 +364,10 @@ int ruv_batch_kernel_timing(ruv_batch* batch, int32_t enable) {
     return ruv::batch_kernel_timing(batch->impl, enable);
 }
 
-int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes) {
+int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes,
+                           double* stored_bytes) {
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
-    return ruv::batch_kernel_stats(batch->impl, launches, seconds, bytes);
+    return ruv::batch_kernel_stats(batch->impl, launches, seconds, bytes, stored_bytes);
 }
 
 void ruv_batch_destroy(ruv_batch* batch) {

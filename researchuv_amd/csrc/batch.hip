@@ -200,14 +200,24 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     Q.nell = nell;
     Q.ncol = ncol;
     {
+        // algorithmic bytes of one step's SpMV per chart: every nonzero's value (and, in a general slice, its
+        // stored column), the slice table (and a stencil slice's offsets), p' read and q written once.  The
+        // stored form also streams the zero slots of its padding; that figure is kept beside it.
         const double sv = mode == RUV_MODE_MIXED ? 4.0 : 8.0, sc = narrow ? 2.0 : 4.0;
         Q.chart_step_bytes.assign(nchart, 0.0);
+        Q.chart_step_stored.assign(nchart, 0.0);
         for (int c = 0; c < nchart; ++c) {
-            double b = 2.0 * (double)(crow[c + 1] - crow[c]) * R * sv;  // p' read, q written
-            for (int s = Q.chart_s0[c]; s < Q.chart_s0[c + 1]; ++s)
-                b += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + 32.0 +
-                     (stencil[s] ? 4.0 * SLOTS : 0.0);
+            const double vec = 2.0 * (double)(crow[c + 1] - crow[c]) * R * sv;  // p' read, q written
+            double b = vec, st = vec;
+            for (int s = Q.chart_s0[c]; s < Q.chart_s0[c + 1]; ++s) {
+                const int64_t r0 = Q.h_sl_row0[s];
+                const double nz = (double)(ptr[r0 + Q.h_sl_n[s]] - ptr[r0]);
+                const double meta = 32.0 + (stencil[s] ? 4.0 * SLOTS : 0.0);
+                b += nz * (stencil[s] ? sv : sv + sc) + meta;
+                st += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + meta;
+            }
             Q.chart_step_bytes[c] = b;
+            Q.chart_step_stored[c] = st;
         }
     }
     // fill the values (and the general slices' columns, and the Jacobi diagonal) on the host threads
@@ -632,7 +642,10 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
             Q->k_seconds += kms * 1e-3;
         }
         Q->kev_used = 0;
-        for (int c = 0; c < Q->nchart; ++c) Q->k_bytes += Q->chart_step_bytes[c] * (double)Q->iters_total[c];
+        for (int c = 0; c < Q->nchart; ++c) {
+            Q->k_bytes += Q->chart_step_bytes[c] * (double)Q->iters_total[c];
+            Q->k_stored += Q->chart_step_stored[c] * (double)Q->iters_total[c];
+        }
     }
     if (info) {
         uint64_t mx = 0;
@@ -654,16 +667,20 @@ int batch_solve_dev(BatchImpl* Q, const double* d_b, double* d_x, uint64_t max_i
     return RUV_OK;
 }
 
-// Mean duration and algorithmic bytes of the step's SpMV kernel (kb_spmv) over the whole batch.  Bytes: the
-// sliced-ELL values and column offsets as stored (padding included), the slice table, and two vectors streamed
-// once (p read, q written); neighbour gathers are served by the caches.
-int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes) {
+// Mean duration and bytes of the step's SpMV kernel (kb_spmv) over the whole batch: algorithmic (every nonzero's
+// value and stored column, the slice table, two vectors streamed once: p read, q written; neighbour gathers are
+// served by the caches) and as stored (the padding slots of the sliced ELL included).
+int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes, double* stored) {
     if (reps <= 0) return fail(RUV_ERR_INVALID, "reps must be positive");
     int rc = time_step(*Q, reps, sec);
     if (rc) return rc;
-    double b = 0.0;
-    for (int c = 0; c < Q->nchart; ++c) b += Q->chart_step_bytes[c];
-    *bytes = b;
+    double b = 0.0, st = 0.0;
+    for (int c = 0; c < Q->nchart; ++c) {
+        b += Q->chart_step_bytes[c];
+        st += Q->chart_step_stored[c];
+    }
+    if (bytes) *bytes = b;
+    if (stored) *stored = st;
     return RUV_OK;
 }
 
@@ -746,16 +763,19 @@ int batch_kernel_timing(BatchImpl* Q, int enable) {
     Q->k_launches = 0;
     Q->k_seconds = 0.0;
     Q->k_bytes = 0.0;
+    Q->k_stored = 0.0;
     return RUV_OK;
 }
 
-int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes) {
+int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes, double* stored) {
     if (launches) *launches = Q->k_launches;
     if (seconds) *seconds = Q->k_seconds;
     if (bytes) *bytes = Q->k_bytes;
+    if (stored) *stored = Q->k_stored;
     Q->k_launches = 0;
     Q->k_seconds = 0.0;
     Q->k_bytes = 0.0;
+    Q->k_stored = 0.0;
     return RUV_OK;
 }
 

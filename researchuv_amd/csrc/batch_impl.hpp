@@ -112,8 +112,9 @@ struct BatchImpl {
     std::vector<hipEvent_t> kev;
     size_t kev_used = 0;
     int64_t k_launches = 0;
-    double k_seconds = 0.0, k_bytes = 0.0;
-    std::vector<double> chart_step_bytes;  // algorithmic bytes of one step of chart c
+    double k_seconds = 0.0, k_bytes = 0.0, k_stored = 0.0;
+    std::vector<double> chart_step_bytes;   // algorithmic bytes of one SpMV of chart c (nonzeros only)
+    std::vector<double> chart_step_stored;  // bytes of one SpMV of chart c as stored (padding slots included)
     DevPool pool;                          // setup and hierarchy blocks (precond AMG)
     Bamg* amg = nullptr;                   // precond AMG: the hierarchy
     int64_t* amg_ptr = nullptr;            // precond AMG: the finest operator in FP64 CSR for (re)builds,

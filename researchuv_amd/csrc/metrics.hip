@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -70,24 +71,74 @@ struct TilePart {
     long long nconf, flips, le0, ge0;
 };
 
-__global__ __launch_bounds__(BLK) void k_distortion_tiles(const int64_t* __restrict__ v_off,
+// chart_tile0[c] (c = 0..nchart) = the first tile of chart c, tiles of TILE_TRIS triangles from each chart's first
+// triangle; one workgroup scans the charts in runs of BLK.  The kernel first clears the call's error flag; bit 4
+// reports triangle offsets that do not start at 0 or decrease (such a chart gets no tiles).
+__global__ __launch_bounds__(BLK) void k_tile_prefix(int32_t nchart, const int64_t* __restrict__ t_off,
+                                                     int32_t* __restrict__ chart_tile0, int* __restrict__ err) {
+    __shared__ int32_t sh[WAVES];
+    if (threadIdx.x == 0) {
+        *err = (t_off[0] != 0) ? 4 : 0;
+        chart_tile0[0] = 0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int32_t base = 0;
+    for (int32_t c0 = 0; c0 < nchart; c0 += BLK) {
+        const int32_t c = c0 + (int32_t)threadIdx.x;
+        int32_t n = 0;
+        if (c < nchart) {
+            const int64_t a = t_off[c], b = t_off[c + 1];
+            if (b < a) atomicOr(err, 4);
+            else n = (int32_t)((b - a + TILE_TRIS - 1) / TILE_TRIS);
+        }
+        int32_t inc = n;  // inclusive scan over the wave, then over the waves
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        if (lane == 63) sh[wv] = inc;
+        __syncthreads();
+        int32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) {
+            before += (w < wv) ? sh[w] : 0;
+            total += sh[w];
+        }
+        if (c < nchart) chart_tile0[c + 1] = base + before + inc;
+        base += total;
+        __syncthreads();
+    }
+}
+
+// One workgroup per tile (grid: an upper bound on the tiles; the ones past chart_tile0[nchart] leave at once).
+__global__ __launch_bounds__(BLK) void k_distortion_tiles(int32_t nchart, const int64_t* __restrict__ v_off,
                                                           const double* __restrict__ p3,
                                                           const double* __restrict__ uv,
-                                                          const int32_t* __restrict__ tile_chart,
-                                                          const int64_t* __restrict__ tile_t0,
-                                                          const int64_t* __restrict__ tile_t1,
+                                                          const int64_t* __restrict__ t_off,
+                                                          const int32_t* __restrict__ chart_tile0,
                                                           const uint32_t* __restrict__ tris,
                                                           TilePart* __restrict__ part, int* __restrict__ err) {
     __shared__ double sh[WAVES];
     __shared__ long long shc[WAVES];
-    const int tile = blockIdx.x;
-    const int c = tile_chart[tile];
+    const int32_t tile = (int32_t)blockIdx.x;
+    if (tile >= chart_tile0[nchart]) return;
+    int32_t lo = 0, hi = nchart;  // the chart of the tile: the last c with chart_tile0[c] <= tile
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (chart_tile0[mid] <= tile) lo = mid;
+        else hi = mid;
+    }
+    const int c = lo;
+    const int64_t tt0 = t_off[c] + (int64_t)(tile - chart_tile0[c]) * TILE_TRIS;
+    const int64_t tt1 = min(t_off[c + 1], tt0 + TILE_TRIS);
     const int64_t v0 = v_off[c], nvc = v_off[c + 1] - v0;
     const double* P = p3 + 3 * v0;
     const double* U = uv + 2 * v0;
     double conf_sum = 0.0, conf_max = 0.0, area_sum = 0.0, signed_sum = 0.0;
     long long nconf = 0, flips = 0, le0 = 0, ge0 = 0;
-    for (int64_t t = tile_t0[tile] + threadIdx.x; t < tile_t1[tile]; t += BLK) {
+    for (int64_t t = tt0 + threadIdx.x; t < tt1; t += BLK) {
         const uint32_t a = tris[3 * t], b = tris[3 * t + 1], cc = tris[3 * t + 2];
         if ((int64_t)a >= nvc || (int64_t)b >= nvc || (int64_t)cc >= nvc) {  // reported, never read
             atomicOr(err, 1);
@@ -148,16 +199,18 @@ __global__ __launch_bounds__(BLK) void k_distortion_tiles(const int64_t* __restr
 }
 
 // per chart: its tiles' parts summed in tile order.  metrics (3 per chart): conformal mean, conformal max,
-// area-ratio mean; counts (2 per chart): flips, folds
+// area-ratio mean; counts (2 per chart): flips, folds.  Tiles at or past ntile_cap (the launched tiles, which
+// offsets that decrease can leave short of the table; the call then reports an error) are not read.
 __global__ __launch_bounds__(BLK) void k_distortion_finish(int32_t nchart, const int32_t* __restrict__ chart_tile0,
-                                                           const TilePart* __restrict__ part,
+                                                           int32_t ntile_cap, const TilePart* __restrict__ part,
                                                            double* __restrict__ metrics,
                                                            long long* __restrict__ counts) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= nchart) return;
     double cs = 0.0, cm = 0.0, as = 0.0, ss = 0.0;
     long long nc = 0, fl = 0, nle = 0, nge = 0;
-    for (int t = chart_tile0[c]; t < chart_tile0[c + 1]; ++t) {
+    const int t1 = min(chart_tile0[c + 1], ntile_cap);
+    for (int t = chart_tile0[c]; t < t1; ++t) {
         const TilePart r = part[t];
         cs += r.conf_sum;
         cm = fmax(cm, r.conf_max);
@@ -450,105 +503,145 @@ void ladder_table(double* lad) {
     }
 }
 
-// calls share their device's flag, so they take turns
-std::mutex& aux_call_mutex() {
-    static std::mutex m;
-    return m;
-}
-
-// per device, allocated on first use and kept: the ladder table and the error flag of the device-pointer entry
-// points (so a call allocates nothing after the first on its device)
-struct MetricsAux {
-    double* ladder = nullptr;
-    int* err = nullptr;
-    void* buf = nullptr;  // tile tables and parts, grown on demand
-    size_t cap = 0;
-};
-int aux_for_device(MetricsAux** out) {
+// The ladder table of each device: uploaded on the first call on that device, read-only afterwards.
+int ladder_for_device(int dev, const double** out) {
     static std::mutex mu;
-    static std::vector<MetricsAux> aux;
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
+    static std::vector<double*> tab;
     std::lock_guard<std::mutex> lock(mu);
-    if ((int)aux.size() <= dev) aux.resize((size_t)dev + 1);
-    MetricsAux& a = aux[(size_t)dev];
-    if (!a.ladder) {
+    if ((int)tab.size() <= dev) tab.resize((size_t)dev + 1, nullptr);
+    if (!tab[(size_t)dev]) {
         double lad[2 * LADDER];
         ladder_table(lad);
         void* p = nullptr;
-        HIPCHK(hipMalloc(&p, sizeof(lad) + 256));
+        HIPCHK(hipMalloc(&p, sizeof(lad)));
         HIPCHK(hipMemcpy(p, lad, sizeof(lad), hipMemcpyHostToDevice));
-        a.ladder = (double*)p;
-        a.err = (int*)((char*)p + sizeof(lad));
+        tab[(size_t)dev] = (double*)p;
     }
-    *out = &a;
+    *out = tab[(size_t)dev];
     return RUV_OK;
 }
 
-int read_err(int* derr, const char* what) {
-    int herr = 0;
-    HIPCHK(hipMemcpy(&herr, derr, sizeof(int), hipMemcpyDeviceToHost));
-    if (herr) return fail(RUV_ERR_BAD_DIMS, std::string(what) + ": an index lies outside its chart");
-    return RUV_OK;
+// The resources of one call of a device-pointer entry point: a non-blocking stream of default priority, an event,
+// pinned host words for the results read back, and a scratch block grown on demand.  A call leases a context of
+// the current device for its duration; concurrent calls lease different ones, so they neither share a stream nor
+// a buffer nor wait on each other.  Released contexts are kept (per device) for later calls until the process
+// ends, so a call allocates nothing once its device has as many contexts as calls have overlapped.
+struct CallCtx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ready = nullptr;
+    int64_t* h_word = nullptr;  // pinned: [0] the error flag, [1] a count read back
+    void* buf = nullptr;
+    size_t cap = 0;
+};
+
+struct CtxPool {
+    std::mutex mu;
+    std::vector<std::vector<CallCtx*>> free_by_dev;
+};
+CtxPool& ctx_pool() {
+    static CtxPool* p = new CtxPool();  // never destroyed: contexts live until the process ends
+    return *p;
 }
 
-}  // namespace
+struct CtxLease {
+    CallCtx* c = nullptr;
+    ~CtxLease() {
+        if (!c) return;
+        CtxPool& P = ctx_pool();
+        std::lock_guard<std::mutex> lock(P.mu);
+        P.free_by_dev[(size_t)c->dev].push_back(c);
+    }
+};
 
-int chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
-                      const uint32_t* tris, double* metrics, int64_t* counts) {
-    if (nchart < 0) return fail(RUV_ERR_INVALID, "metrics: negative chart count");
-    if (nchart == 0) return RUV_OK;
-    if (!v_off || !t_off || !metrics || !counts) return fail(RUV_ERR_INVALID, "metrics: null argument");
-    MetricsAux* aux = nullptr;
-    int rc = aux_for_device(&aux);
-    if (rc) return rc;
-    // tiles of at most TILE_TRIS triangles, none across charts (built from a host copy of t_off)
-    std::vector<int64_t> toff((size_t)nchart + 1);
-    HIPCHK(hipMemcpy(toff.data(), t_off, sizeof(int64_t) * toff.size(), hipMemcpyDeviceToHost));
-    std::vector<int32_t> chart_tile0((size_t)nchart + 1, 0), tile_chart;
-    std::vector<int64_t> tile_t0, tile_t1;
-    for (int32_t c = 0; c < nchart; ++c) {
-        if (toff[c + 1] < toff[c]) return fail(RUV_ERR_BAD_DIMS, "metrics: offsets not monotone");
-        for (int64_t t = toff[c]; t < toff[c + 1]; t += TILE_TRIS) {
-            tile_chart.push_back(c);
-            tile_t0.push_back(t);
-            tile_t1.push_back(std::min<int64_t>(toff[c + 1], t + TILE_TRIS));
+int lease_ctx(CtxLease* L) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    CtxPool& P = ctx_pool();
+    {
+        std::lock_guard<std::mutex> lock(P.mu);
+        if ((int)P.free_by_dev.size() <= dev) P.free_by_dev.resize((size_t)dev + 1);
+        auto& fl = P.free_by_dev[(size_t)dev];
+        if (!fl.empty()) {
+            L->c = fl.back();
+            fl.pop_back();
+            return RUV_OK;
         }
-        chart_tile0[(size_t)c + 1] = (int32_t)tile_chart.size();
     }
-    const size_t ntile = tile_chart.size();
-    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t b_ct = up(sizeof(int32_t) * chart_tile0.size()), b_tc = up(sizeof(int32_t) * std::max<size_t>(ntile, 1));
-    const size_t b_t = up(sizeof(int64_t) * std::max<size_t>(ntile, 1)), b_p = up(sizeof(TilePart) * std::max<size_t>(ntile, 1));
-    const size_t need = b_ct + b_tc + 2 * b_t + b_p;
-    std::lock_guard<std::mutex> lock(aux_call_mutex());
-    if (aux->cap < need) {
-        if (aux->buf) HIPCHK(hipFree(aux->buf));
-        aux->buf = nullptr;
-        aux->cap = 0;
-        HIPCHK(hipMalloc(&aux->buf, need));
-        aux->cap = need;
-    }
-    char* base = (char*)aux->buf;
-    int32_t* d_ct = (int32_t*)base;
-    int32_t* d_tc = (int32_t*)(base + b_ct);
-    int64_t* d_t0 = (int64_t*)(base + b_ct + b_tc);
-    int64_t* d_t1 = (int64_t*)(base + b_ct + b_tc + b_t);
-    TilePart* d_p = (TilePart*)(base + b_ct + b_tc + 2 * b_t);
-    HIPCHK(hipMemcpyAsync(d_ct, chart_tile0.data(), sizeof(int32_t) * chart_tile0.size(), hipMemcpyHostToDevice, 0));
-    if (ntile) {
-        HIPCHK(hipMemcpyAsync(d_tc, tile_chart.data(), sizeof(int32_t) * ntile, hipMemcpyHostToDevice, 0));
-        HIPCHK(hipMemcpyAsync(d_t0, tile_t0.data(), sizeof(int64_t) * ntile, hipMemcpyHostToDevice, 0));
-        HIPCHK(hipMemcpyAsync(d_t1, tile_t1.data(), sizeof(int64_t) * ntile, hipMemcpyHostToDevice, 0));
-    }
-    HIPCHK(hipMemsetAsync(aux->err, 0, sizeof(int), 0));
-    if (ntile)
-        hipLaunchKernelGGL(k_distortion_tiles, dim3((unsigned)ntile), dim3(BLK), 0, 0, v_off, p3, uv,
-                           (const int32_t*)d_tc, (const int64_t*)d_t0, (const int64_t*)d_t1, tris, d_p, aux->err);
-    hipLaunchKernelGGL(k_distortion_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart,
-                       (const int32_t*)d_ct, (const TilePart*)d_p, metrics, (long long*)counts);
+    std::unique_ptr<CallCtx> c(new CallCtx());
+    c->dev = dev;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void**)&c->h_word, 2 * sizeof(int64_t), hipHostMallocDefault));
+    L->c = c.release();
+    return RUV_OK;
+}
+
+// the context's scratch block, at least `need` bytes (the stream has finished with the old block: every call
+// ends with a wait on its stream)
+int ctx_scratch(CallCtx* c, size_t need) {
+    if (c->cap >= need) return RUV_OK;
+    if (c->buf) HIPCHK(hipFree(c->buf));
+    c->buf = nullptr;
+    c->cap = 0;
+    HIPCHK(hipMalloc(&c->buf, need));
+    c->cap = need;
+    return RUV_OK;
+}
+
+// The call's stream waits for the work issued so far on the null stream (which itself follows the blocking
+// streams): inputs a caller wrote there, or on any blocking stream, are complete before the call reads them.
+int ctx_order_after_caller(CallCtx* c) {
+    HIPCHK(hipEventRecord(c->ready, nullptr));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->ready, 0));
+    return RUV_OK;
+}
+
+// the error flag at `derr`, read back on `s` after the call's kernels (host word `h`, pinned or not)
+int finish_call(hipStream_t s, const int* derr, int64_t* h, const char* what) {
+    *h = 0;
+    HIPCHK(hipMemcpyAsync(h, derr, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if ((int)*h & 4) return fail(RUV_ERR_BAD_DIMS, std::string(what) + ": chart offsets not monotone from 0");
+    if (*h) return fail(RUV_ERR_BAD_DIMS, std::string(what) + ": an index lies outside its chart");
+    return RUV_OK;
+}
+
+inline size_t al256(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+
+// Scratch of chart_distortion for nchart charts with `ntri` triangles in all: [error flag | chart_tile0 | one
+// TilePart per tile], tiles bounded by nchart + ntri / TILE_TRIS.
+struct DistortionScratch {
+    int64_t tiles_bound = 0;
+    size_t off_ct = 0, off_part = 0, bytes = 0;
+};
+DistortionScratch distortion_scratch(int32_t nchart, int64_t ntri) {
+    DistortionScratch d;
+    d.tiles_bound = (int64_t)nchart + (std::max<int64_t>(ntri, 0) + TILE_TRIS - 1) / TILE_TRIS;
+    d.off_ct = 256;
+    d.off_part = d.off_ct + al256(sizeof(int32_t) * ((size_t)nchart + 1));
+    d.bytes = d.off_part + al256(sizeof(TilePart) * (size_t)std::max<int64_t>(d.tiles_bound, 1));
+    return d;
+}
+
+// The chart_distortion launches on stream s with scratch at `base` (laid out by distortion_scratch): tile table
+// on the device, one workgroup per tile, per-chart finish.  Shared by the leased-context and static-scratch entry
+// points.
+int distortion_launch(hipStream_t s, char* base, const DistortionScratch& d, int32_t nchart, const int64_t* v_off,
+                      const double* p3, const double* uv, const int64_t* t_off, const uint32_t* tris,
+                      double* metrics, int64_t* counts, int64_t* h_word) {
+    int* d_err = (int*)base;
+    int32_t* d_ct = (int32_t*)(base + d.off_ct);
+    TilePart* d_p = (TilePart*)(base + d.off_part);
+    hipLaunchKernelGGL(k_tile_prefix, dim3(1), dim3(BLK), 0, s, nchart, t_off, d_ct, d_err);
+    if (d.tiles_bound > 0)
+        hipLaunchKernelGGL(k_distortion_tiles, dim3((unsigned)d.tiles_bound), dim3(BLK), 0, s, nchart, v_off, p3, uv,
+                           t_off, (const int32_t*)d_ct, tris, d_p, d_err);
+    hipLaunchKernelGGL(k_distortion_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, s, nchart,
+                       (const int32_t*)d_ct, (int32_t)d.tiles_bound, (const TilePart*)d_p, metrics,
+                       (long long*)counts);
     HIPCHK(hipGetLastError());
-    return read_err(aux->err, "metrics");
+    return finish_call(s, d_err, h_word, "metrics");
 }
 
 // tiles of at most `size` items of each chart from its offsets (host copy): chart_tile0 (nchart + 1), per tile
@@ -566,24 +659,28 @@ void make_tiles(int32_t nchart, const std::vector<int64_t>& off, int64_t size, s
     }
 }
 
-// carve the per-device growable buffer into the given byte sizes (256-aligned)
-int carve(MetricsAux* aux, const std::vector<size_t>& bytes, std::vector<char*>* ptrs) {
-    size_t need = 0;
-    for (size_t b : bytes) need += (std::max<size_t>(b, 1) + 255) & ~(size_t)255;
-    if (aux->cap < need) {
-        if (aux->buf) HIPCHK(hipFree(aux->buf));
-        aux->buf = nullptr;
-        aux->cap = 0;
-        HIPCHK(hipMalloc(&aux->buf, need));
-        aux->cap = need;
-    }
-    char* p = (char*)aux->buf;
-    ptrs->clear();
-    for (size_t b : bytes) {
-        ptrs->push_back(p);
-        p += (std::max<size_t>(b, 1) + 255) & ~(size_t)255;
-    }
-    return RUV_OK;
+}  // namespace
+
+int chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
+                      const uint32_t* tris, double* metrics, int64_t* counts) {
+    if (nchart < 0) return fail(RUV_ERR_INVALID, "metrics: negative chart count");
+    if (nchart == 0) return RUV_OK;
+    if (!v_off || !t_off || !metrics || !counts) return fail(RUV_ERR_INVALID, "metrics: null argument");
+    CtxLease L;
+    int rc = lease_ctx(&L);
+    if (rc) return rc;
+    CallCtx* C = L.c;
+    rc = ctx_order_after_caller(C);
+    if (rc) return rc;
+    // the triangle count bounds the tiles (one word read back; the tile table itself is built on the device)
+    HIPCHK(hipMemcpyAsync(&C->h_word[1], t_off + nchart, sizeof(int64_t), hipMemcpyDeviceToHost, C->stream));
+    HIPCHK(hipStreamSynchronize(C->stream));
+    const DistortionScratch d = distortion_scratch(nchart, C->h_word[1]);
+    if (d.tiles_bound > (int64_t)INT32_MAX) return fail(RUV_ERR_BAD_DIMS, "metrics: too many triangles");
+    rc = ctx_scratch(C, d.bytes);
+    if (rc) return rc;
+    return distortion_launch(C->stream, (char*)C->buf, d, nchart, v_off, p3, uv, t_off, tris, metrics, counts,
+                             &C->h_word[0]);
 }
 
 int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
@@ -591,12 +688,20 @@ int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, c
     if (nchart < 0) return fail(RUV_ERR_INVALID, "rectangularize: negative chart count");
     if (nchart == 0) return RUV_OK;
     if (!v_off || !b_off || !ext) return fail(RUV_ERR_INVALID, "rectangularize: null argument");
-    MetricsAux* aux = nullptr;
-    int rc = aux_for_device(&aux);
+    CtxLease L;
+    int rc = lease_ctx(&L);
+    if (rc) return rc;
+    CallCtx* C = L.c;
+    const double* ladder = nullptr;
+    rc = ladder_for_device(C->dev, &ladder);
+    if (rc) return rc;
+    const hipStream_t s = C->stream;
+    rc = ctx_order_after_caller(C);
     if (rc) return rc;
     std::vector<int64_t> voff((size_t)nchart + 1), boff((size_t)nchart + 1);
-    HIPCHK(hipMemcpy(voff.data(), v_off, sizeof(int64_t) * voff.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(boff.data(), b_off, sizeof(int64_t) * boff.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(voff.data(), v_off, sizeof(int64_t) * voff.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(boff.data(), b_off, sizeof(int64_t) * boff.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     for (int32_t c = 0; c < nchart; ++c)
         if (voff[c + 1] < voff[c] || boff[c + 1] < boff[c]) return fail(RUV_ERR_BAD_DIMS, "rectangularize: offsets not monotone");
     constexpr int64_t VT = 4096, BT = 4096;
@@ -605,21 +710,30 @@ int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, c
     make_tiles(nchart, voff, VT, &vct0, &vtc, &vi0, &vi1);
     make_tiles(nchart, boff, BT, &bct0, &btc, &bi0, &bi1);
     const size_t nvt = vtc.size(), nbt = btc.size();
-    std::lock_guard<std::mutex> lock(aux_call_mutex());
-    std::vector<char*> P;
-    rc = carve(aux, {sizeof(RectChart) * (size_t)nchart, sizeof(int32_t) * vct0.size(), sizeof(int32_t) * nvt,
-                     sizeof(int64_t) * nvt, sizeof(int64_t) * nvt, sizeof(Box) * nvt, sizeof(int32_t) * nbt,
-                     sizeof(int64_t) * nbt, sizeof(int64_t) * nbt, sizeof(double) * LADDER * (size_t)nchart,
-                     sizeof(int) * (size_t)nchart}, &P);
+    const std::vector<size_t> bytes = {sizeof(int), sizeof(RectChart) * (size_t)nchart, sizeof(int32_t) * vct0.size(),
+                                       sizeof(int32_t) * nvt, sizeof(int64_t) * nvt, sizeof(int64_t) * nvt,
+                                       sizeof(Box) * nvt, sizeof(int32_t) * nbt, sizeof(int64_t) * nbt,
+                                       sizeof(int64_t) * nbt, sizeof(double) * LADDER * (size_t)nchart,
+                                       sizeof(int) * (size_t)nchart};
+    size_t need = 0;
+    for (size_t b : bytes) need += al256(b);
+    rc = ctx_scratch(C, need);
     if (rc) return rc;
-    double* d_score = (double*)P[9];
-    int* d_bad = (int*)P[10];
-    RectChart* d_rc = (RectChart*)P[0];
-    int32_t *d_vct0 = (int32_t*)P[1], *d_vtc = (int32_t*)P[2], *d_btc = (int32_t*)P[6];
-    int64_t *d_vi0 = (int64_t*)P[3], *d_vi1 = (int64_t*)P[4], *d_bi0 = (int64_t*)P[7], *d_bi1 = (int64_t*)P[8];
-    Box* d_box = (Box*)P[5];
-    auto h2d = [](void* d, const void* h, size_t b) {
-        return b ? hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, 0) : hipSuccess;
+    std::vector<char*> P;
+    char* p = (char*)C->buf;
+    for (size_t b : bytes) {
+        P.push_back(p);
+        p += al256(b);
+    }
+    int* d_err = (int*)P[0];
+    RectChart* d_rc = (RectChart*)P[1];
+    int32_t *d_vct0 = (int32_t*)P[2], *d_vtc = (int32_t*)P[3], *d_btc = (int32_t*)P[7];
+    int64_t *d_vi0 = (int64_t*)P[4], *d_vi1 = (int64_t*)P[5], *d_bi0 = (int64_t*)P[8], *d_bi1 = (int64_t*)P[9];
+    Box* d_box = (Box*)P[6];
+    double* d_score = (double*)P[10];
+    int* d_bad = (int*)P[11];
+    auto h2d = [s](void* d, const void* h, size_t b) {
+        return b ? hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, s) : hipSuccess;
     };
     HIPCHK(h2d(d_vct0, vct0.data(), sizeof(int32_t) * vct0.size()));
     HIPCHK(h2d(d_vtc, vtc.data(), sizeof(int32_t) * nvt));
@@ -628,25 +742,25 @@ int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, c
     HIPCHK(h2d(d_btc, btc.data(), sizeof(int32_t) * nbt));
     HIPCHK(h2d(d_bi0, bi0.data(), sizeof(int64_t) * nbt));
     HIPCHK(h2d(d_bi1, bi1.data(), sizeof(int64_t) * nbt));
-    HIPCHK(hipMemsetAsync(aux->err, 0, sizeof(int), 0));
-    HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int) * (size_t)nchart, 0));
-    hipLaunchKernelGGL(k_rect_scores, dim3((unsigned)(((int64_t)nchart * LGROUPS + WAVES - 1) / WAVES)), dim3(BLK), 0, 0,
-                       nchart, v_off, uv, b_off, bnd, (const double*)aux->ladder, d_score, d_bad);
-    hipLaunchKernelGGL(k_rect_pick, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart, b_off,
-                       (const double*)aux->ladder, (const double*)d_score, (const int*)d_bad, d_rc, aux->err);
+    HIPCHK(hipMemsetAsync(d_err, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int) * (size_t)nchart, s));
+    hipLaunchKernelGGL(k_rect_scores, dim3((unsigned)(((int64_t)nchart * LGROUPS + WAVES - 1) / WAVES)), dim3(BLK), 0, s,
+                       nchart, v_off, uv, b_off, bnd, ladder, d_score, d_bad);
+    hipLaunchKernelGGL(k_rect_pick, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, s, nchart, b_off, ladder,
+                       (const double*)d_score, (const int*)d_bad, d_rc, d_err);
     if (nvt)
-        hipLaunchKernelGGL(k_rect_box, dim3((unsigned)nvt), dim3(BLK), 0, 0, (const int32_t*)d_vtc,
+        hipLaunchKernelGGL(k_rect_box, dim3((unsigned)nvt), dim3(BLK), 0, s, (const int32_t*)d_vtc,
                            (const int64_t*)d_vi0, (const int64_t*)d_vi1, v_off, uv, (const RectChart*)d_rc, d_box);
-    hipLaunchKernelGGL(k_rect_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, 0, nchart,
+    hipLaunchKernelGGL(k_rect_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, s, nchart,
                        (const int32_t*)d_vct0, (const Box*)d_box, d_rc, ext);
     if (nvt)
-        hipLaunchKernelGGL(k_rect_write, dim3((unsigned)nvt), dim3(BLK), 0, 0, (const int32_t*)d_vtc,
+        hipLaunchKernelGGL(k_rect_write, dim3((unsigned)nvt), dim3(BLK), 0, s, (const int32_t*)d_vtc,
                            (const int64_t*)d_vi0, (const int64_t*)d_vi1, v_off, uv, (const RectChart*)d_rc, q);
     if (nbt)
-        hipLaunchKernelGGL(k_rect_snap, dim3((unsigned)nbt), dim3(BLK), 0, 0, (const int32_t*)d_btc,
+        hipLaunchKernelGGL(k_rect_snap, dim3((unsigned)nbt), dim3(BLK), 0, s, (const int32_t*)d_btc,
                            (const int64_t*)d_bi0, (const int64_t*)d_bi1, v_off, b_off, bnd, (const RectChart*)d_rc, q);
     HIPCHK(hipGetLastError());
-    return read_err(aux->err, "rectangularize");
+    return finish_call(s, d_err, &C->h_word[0], "rectangularize");
 }
 
 int chart_metrics(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,

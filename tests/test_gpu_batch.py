@@ -56,6 +56,48 @@ def test_batch_matches_per_chart_lu(orc, mixed):
     assert (it > 0).sum() > 0
 
 
+@pytest.mark.parametrize("mixed", [True, False])
+def test_batch_largest_patches_at_bench_tolerance(mixed):
+    """The four largest patches of the 4096-patch workload (64 227-65 404 vertices, the same seeds and shapes),
+    with eight small patches beside them, solved with the device AMG at the workload's tolerance (true relative
+    residual 1e-8) and checked per chart against SuperLU: the UV metric of lscm.rs:749-758 stays below 1e-6.
+    A residual test bounds the UV error least on the largest charts."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from researchuv_amd import batch as B
+    from researchuv_amd import meshgen
+    from researchuv_amd.solver import Precond
+
+    sizes = meshgen.patch_sizes(4096, seed=0)
+    top = np.argsort(-(sizes[:, 0] * sizes[:, 1]), kind="stable")[:4]
+    meshes = [meshgen.heightfield(int(sizes[k][0]), int(sizes[k][1]), seed=int(k) + 1) for k in top]
+    meshes += _meshes(8, seed=9)
+    say(f"largest patches {[tuple(sizes[k]) for k in top]} + 8 small, mixed={mixed}, tol 1e-8")
+    lb = B.LscmBatchBuilder(meshes, angle_min_deg=180.0, threads=8)
+    a = lb.arrays(csr=True)
+    bt = lb.device_batch(precond=Precond.AMG, mixed=mixed)
+    x, info, it, res = bt.solve(a["rhs"], max_iter=100000, tol=1e-8)
+    bt.close()
+    off, ptr = a["chart_row_off"], a["row_ptr"]
+    worst = 0.0
+    for c in range(lb.nchart):
+        r0, r1 = int(off[c]), int(off[c + 1])
+        L = sp.csr_matrix((a["vals"][ptr[r0]:ptr[r1]], a["cols"][ptr[r0]:ptr[r1]] - r0, ptr[r0:r1 + 1] - ptr[r0]),
+                          shape=(r1 - r0, r1 - r0))
+        xr = spla.splu(L.tocsc()).solve(a["rhs"][2 * r0:2 * r1].reshape(-1, 2).copy())
+        xg = x[2 * r0:2 * r1].reshape(-1, 2)
+        err = np.linalg.norm(xg - xr, axis=1).max() / np.linalg.norm(xr, axis=1).max()
+        worst = max(worst, err)
+        if c < 4:
+            say(f"  chart {c}: {r1 - r0} free vertices, {int(it[c])} iters, true resid {res[c]:.2e}, uv err {err:.2e}")
+    say(f"  worst uv err {worst:.2e}, worst true resid {info.true_rel_resid:.2e}")
+    assert max(np.diff(off)[:4]) > 60000
+    assert worst < 1e-6
+    if mixed:
+        assert (res <= 1e-8).all()
+
+
 def test_batch_is_independent_of_composition(orc, monkeypatch):
     """Each chart runs its own PCG: solving a chart alone or inside a batch gives the same iterates (with the
     tile schedule held fixed, since the tiles decide the order of the partial sums)."""
