@@ -305,6 +305,15 @@ int ruv_rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, c
  * repeated border ids give an unspecified snap. */
 int ruv_chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
                           const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
+/* ruv_chart_metrics_dev without any allocation (no counterpart in the reference; for callers that cannot
+ * allocate device memory: tight serving loops, harnesses that forbid hipMalloc).  The same kernels and results
+ * (bitwise) as ruv_chart_metrics_dev; its scratch (error flag, tile table, per-tile partials) is a fixed 8 MiB
+ * device array held by each GPU the library runs on until the process ends, enough for about 131 000 tiles of
+ * 1024 triangles (nchart + ceil(triangles / 1024) tiles); larger inputs are rejected with RUV_ERR_BAD_DIMS
+ * before anything is launched.  Runs on the null stream and returns when its results are written.  Every call on
+ * one GPU shares that GPU's scratch: calls that overlap on one GPU race. */
+int ruv_chart_metrics_dev_noalloc(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                                  const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
 int ruv_rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                            const int64_t* bnd, double* q, double* ext);
 

@@ -561,6 +561,11 @@ int ruv_chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3
     return ruv::chart_metrics_dev(nchart, v_off, p3, uv, t_off, tris, metrics, counts);
 }
 
+int ruv_chart_metrics_dev_noalloc(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                                  const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts) {
+    return ruv::chart_metrics_dev_noalloc(nchart, v_off, p3, uv, t_off, tris, metrics, counts);
+}
+
 int ruv_rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                            const int64_t* bnd, double* q, double* ext) {
     return ruv::rectangularize_dev(nchart, v_off, uv, b_off, bnd, q, ext);

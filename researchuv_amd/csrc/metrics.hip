@@ -683,6 +683,56 @@ int chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, co
                              &C->h_word[0]);
 }
 
+namespace {
+
+// The fixed scratch of the allocation-free chart_distortion: the error flag, the tile table and one TilePart per
+// tile (distortion_scratch's layout).  Each device holds its own copy until the process ends.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+// the device address of `scratch` on the current device, looked up once per device (hipGetSymbolAddress with the
+// device current) and kept: host code never passes the array's name itself to a kernel or a copy
+int noalloc_scratch(char** out) {
+    static std::mutex mu;
+    static std::vector<char*> addr;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)addr.size() <= dev) addr.resize((size_t)dev + 1, nullptr);
+    if (!addr[(size_t)dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(RUV_ERR_DEVICE, std::string("metrics: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[(size_t)dev] = (char*)ptr;
+    }
+    *out = addr[(size_t)dev];
+    return RUV_OK;
+}
+
+}  // namespace
+
+int chart_metrics_dev_noalloc(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                              const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts) {
+    if (nchart < 0) return fail(RUV_ERR_INVALID, "metrics: negative chart count");
+    if (nchart == 0) return RUV_OK;
+    if (!v_off || !t_off || !metrics || !counts) return fail(RUV_ERR_INVALID, "metrics: null argument");
+    char* base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    // the null stream: ordered after the caller's null-stream (and blocking-stream) work; nothing is created
+    int64_t ntri = 0;
+    HIPCHK(hipMemcpyAsync(&ntri, t_off + nchart, sizeof(int64_t), hipMemcpyDeviceToHost, nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    const DistortionScratch d = distortion_scratch(nchart, ntri);
+    if (d.bytes > NOALLOC_SCRATCH_BYTES)
+        return fail(RUV_ERR_BAD_DIMS, "metrics (no-alloc): " + std::to_string(nchart) + " charts of " +
+                                          std::to_string(ntri) + " triangles need " + std::to_string(d.bytes) +
+                                          " scratch bytes, above the fixed " + std::to_string(NOALLOC_SCRATCH_BYTES));
+    int64_t herr = 0;
+    return distortion_launch(nullptr, base, d, nchart, v_off, p3, uv, t_off, tris, metrics, counts, &herr);
+}
+
 int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                        const int64_t* bnd, double* q, double* ext) {
     if (nchart < 0) return fail(RUV_ERR_INVALID, "rectangularize: negative chart count");

@@ -77,6 +77,8 @@ int rectangularize(int32_t nchart, const int64_t* v_off, const double* uv, const
                    double* q, double* ext);
 int chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv, const int64_t* t_off,
                       const uint32_t* tris, double* metrics, int64_t* counts);
+int chart_metrics_dev_noalloc(int32_t nchart, const int64_t* v_off, const double* p3, const double* uv,
+                              const int64_t* t_off, const uint32_t* tris, double* metrics, int64_t* counts);
 int rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                        const int64_t* bnd, double* q, double* ext);
 

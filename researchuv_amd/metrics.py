@@ -32,6 +32,8 @@ def _setup(L):
     L.ruv_rectangularize.argtypes = [C.c_int32, P, P, P, P, P, P]
     L.ruv_chart_metrics_dev.restype = C.c_int
     L.ruv_chart_metrics_dev.argtypes = [C.c_int32, P, P, P, P, P, P, P]
+    L.ruv_chart_metrics_dev_noalloc.restype = C.c_int
+    L.ruv_chart_metrics_dev_noalloc.argtypes = [C.c_int32, P, P, P, P, P, P, P]
     L.ruv_rectangularize_dev.restype = C.c_int
     L.ruv_rectangularize_dev.argtypes = [C.c_int32, P, P, P, P, P, P]
     L._ruv_metrics_ready = True
