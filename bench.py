@@ -5,8 +5,8 @@ Default workload (`batch`): 4096 open height-field patch charts, vertex counts l
 aspect ratio uniform in [1, 8], z = sum of 8 seeded sinusoids (amplitude 0.1, frequencies 1-16), xy jitter
 +-0.2 spacing; each patch is welded, segmented at 180 degrees into one chart and pinned on its border by the
 reference's border pass, and all charts form one batch.  One step = solving every chart's LSCM system (u and v)
-to a true FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 per chart, with the matrix, right-hand sides and
-solution resident in HBM.
+to a true FP64 relative residual ||b - A x|| / ||b|| <= 1e-8 per chart (the mixed solve also until its estimated
+forward error is <= 1e-7 max|x|), with the matrix, right-hand sides and solution resident in HBM.
 
 value = LSCM unknowns (2 x free vertices, summed over the charts) solved per second by the mixed-precision
 batched solve (FP32 PCG preconditioned by an aggregation AMG built on the device inside every step, within
@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 METRIC = "LSCM unknowns solved/s @ rel. residual 1e-8; SpMV HBM GB/s % peak vs FP64 PCG"
 TOL = 1e-8
+UV_EST_TOL = 1e-7  # the mixed solves' per-chart forward-error estimate bound (relative to max |x|), library default
 
 
 def log(*a):
@@ -317,7 +318,7 @@ def run_batch(args):
         config={"workload": f"batch: {lb.nchart} uneven height-field patch charts (64-65536 vertices, log-uniform), "
                             f"{lb.nv_total} vertices, {unknowns} unknowns", "charts": lb.nchart,
                 "unknowns": unknowns, "nnz_L": lb.nnz, "precond": args.batch_precond.lower(),
-                "tol_true_rel_resid": TOL, "schedule": sched},
+                "tol_true_rel_resid": TOL, "tol_forward_error_estimate": UV_EST_TOL, "schedule": sched},
         extra={"setup_s": setup_s, "precond_setup_in_step": pinfo, "max_inner_iters": int(info.iters),
                "mean_inner_iters": float(np.mean(it)),
                "worst_true_rel_resid": float(info.true_rel_resid),
@@ -413,6 +414,7 @@ def run_single(args):
     extra = {"setup_s": setup_s, "precond_setup_in_step": pinfo, "iters": int(info.iters),
              "true_rel_resid": float(info.true_rel_resid), "schedule": sched}
     # UV / energy error against SuperLU where its factorisation is affordable
+    xr = None
     if nfree <= 400_000:
         L = sp.csr_matrix((s["l_vals"], s["l_cols"], s["l_ptr"]), shape=(nfree, nfree))
         r = s["rhs"].reshape(-1, 2)
@@ -425,31 +427,50 @@ def run_single(args):
 
         extra["energy_rel_err_vs_cpu_lu"] = abs(energy(xa) - energy(xr)) / abs(energy(xr))
         log(f"vs SuperLU: uv err {extra['uv_rel_err_vs_cpu_lu']:.2e}, energy {extra['energy_rel_err_vs_cpu_lu']:.2e}")
-    # the plan drop-in beside it (plain FP64 PCG, the reference's preconditioners, capped)
+    # the plan drop-in beside it, one call per preconditioner: plain FP64 PCG (the reference's recurrence and
+    # recursive-residual test) and, for IC(0), the configuration's FP32 PCG + IC(0) inside FP64 refinement (true
+    # residual test); each timed after one warm solve, host setup (factorisation, levels) timed apart
     Lp = solver.CsrMatrix(s["l_vals"], s["l_cols"], s["l_ptr"])
-    x64 = torch.zeros_like(b)
-    for name in args.compare:
-        try:
-            t0 = time.perf_counter()
-            plan = solver.Plan(Lp, 2, getattr(solver.Precond, name), mixed=False)
-            st = time.perf_counter() - t0
-            t0 = time.perf_counter()
-            i64 = plan.solve_dev(b.data_ptr(), x64.data_ptr(), args.compare_iters, TOL)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t0
-            extra[f"fp64_plan_{name.lower()}"] = {"value": unknowns / (el + st), "solve_s": el, "host_setup_s": st,
-                                                  "iters": int(i64.iters), "true_rel_resid": float(i64.true_rel_resid)}
-            log(f"fp64 plan {name.lower()}: setup {st:.2f}s solve {el:.2f}s iters {i64.iters}")
-            plan.close()
-        except Exception as e:  # not converged within the cap, or a failed setup: reported, not fatal
-            extra[f"fp64_plan_{name.lower()}"] = {"error": str(e)[:200], "iter_cap": args.compare_iters}
-            log(f"fp64 plan {name.lower()}: {e}")
+    xp = torch.zeros_like(b)
+    compare = args.compare if args.compare is not None else (
+        ["IC0", "IC0_COLOR", "JACOBI"] if kind == "chart512" else ["JACOBI"])
+    for name in compare:
+        for mixed in ((False, True) if name in ("IC0", "IC0_COLOR") else (False,)):
+            key = f"{'mixed' if mixed else 'fp64'}_plan_{name.lower()}"
+            try:
+                t0 = time.perf_counter()
+                plan = solver.Plan(Lp, 2, getattr(solver.Precond, name), mixed=mixed)
+                st = time.perf_counter() - t0
+                plan.solve_dev(b.data_ptr(), xp.data_ptr(), args.compare_iters, TOL)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ip = plan.solve_dev(b.data_ptr(), xp.data_ptr(), args.compare_iters, TOL)
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t0
+                plan.close()
+                rec = {"value": unknowns / el, "solve_s": el, "host_setup_s": st, "iters": int(ip.iters),
+                       "refinements": int(ip.refinements), "true_rel_resid": float(ip.true_rel_resid)}
+                if xr is not None:
+                    xq = xp.cpu().numpy().reshape(-1, 2)
+                    rec["uv_rel_err_vs_cpu_lu"] = float(np.linalg.norm(xq - xr, axis=1).max() /
+                                                        np.linalg.norm(xr, axis=1).max())
+                extra[key] = rec
+                log(f"{key}: setup {st:.2f}s solve {el * 1e3:.1f} ms iters {ip.iters} true resid "
+                    f"{ip.true_rel_resid:.2e} uv err {rec.get('uv_rel_err_vs_cpu_lu', float('nan')):.2e}")
+            except Exception as e:  # not converged within the cap, or a failed setup: reported, not fatal
+                extra[key] = {"error": str(e)[:200], "iter_cap": args.compare_iters}
+                log(f"{key}: {e}")
+    for name in ("ic0", "ic0_color"):
+        a64, a32 = extra.get(f"fp64_plan_{name}", {}), extra.get(f"mixed_plan_{name}", {})
+        if "solve_s" in a64 and "solve_s" in a32:
+            extra[f"speedup_mixed_vs_fp64_plan_{name}"] = a64["solve_s"] / a32["solve_s"]
     what = ("one 512x512-vertex height-field chart" if kind == "chart512" else
             f"one {side}x{side}-vertex planar grid, column spacing 1 -> 100 geometric, 5% flipped diagonals")
     return dict(
         value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
         config={"workload": f"{kind}: {what}, {unknowns} unknowns", "unknowns": unknowns,
-                "nnz_L": int(len(s["l_vals"])), "precond": args.batch_precond.lower(), "tol_true_rel_resid": TOL},
+                "nnz_L": int(len(s["l_vals"])), "precond": args.batch_precond.lower(), "tol_true_rel_resid": TOL,
+                "tol_forward_error_estimate": UV_EST_TOL},
         extra=extra, roofline=roof, meshes=None)
 
 
@@ -623,7 +644,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch", choices=["batch", "chart512", "stretched", "diffbatch", "metrics"])
     ap.add_argument("--side", type=int, default=0, help="chart512 / stretched: vertices per side (512 / 2048)")
-    ap.add_argument("--compare", nargs="*", default=["JACOBI"], help="single chart: plan preconditioners timed beside")
+    ap.add_argument("--compare", nargs="*", default=None,
+                    help="single chart: plan preconditioners timed beside (chart512: IC0 IC0_COLOR JACOBI, FP64 and "
+                         "mixed for IC(0); stretched: JACOBI)")
     ap.add_argument("--compare-iters", type=int, default=20000, help="single chart: iteration cap of the comparisons")
     ap.add_argument("--charts", type=int, default=4096, help="batch workload: number of charts")
     ap.add_argument("--dropin-charts", type=int, default=64, help="batch: charts in the per-chart drop-in sample")

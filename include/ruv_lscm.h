@@ -92,7 +92,12 @@ int ruv_apply_ic0(const double* vals, const int32_t* cols, const int32_t* row_pt
 /* Mixed-precision form (no reference counterpart; the reference GPU path is FP64 throughout,
  * solver.rs:284-565): the SpMV, preconditioner and Krylov vectors run in FP32 inside an FP64
  * iterative-refinement loop (r = b - A x, the x update and the CG alpha/beta reductions in FP64) until the
- * true FP64 residual satisfies ||b - A x|| <= tol*||b||.  max_iter bounds the summed inner iterations. */
+ * true FP64 residual satisfies ||b - A x|| <= tol*||b|| and the forward-error estimate does not exceed
+ * 1e-7 max|x|: each refinement round solves A d = r_start, so gamma = max|d| / ||r_start|| (the largest of
+ * the rounds) measures how far a residual reaches in x, and gamma ||r|| estimates the error left.  A residual
+ * test alone does not bound the UV error on large charts (the 512^2 height field at ||r|| 2.5e-9 ||b|| was
+ * 2e-6 from the LU).  The environment variable RUV_UV_TOL replaces 1e-7 (0: residual test only).  max_iter
+ * bounds the summed inner iterations. */
 int ruv_cg_solve_mixed(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
                        const double* b, int64_t b_len, uint64_t max_iter, double tol, int32_t precond,
                        double* x_out, ruv_solve_info* info);
@@ -131,7 +136,8 @@ int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch,
  * alpha/beta/stopping test) inside shared launches; precond none, jacobi or amg (an aggregation hierarchy
  * built on the device at create time, block-diagonal by chart, applied as a V(1,1) cycle).  mode 0: FP64 PCG per chart with
  * the recursive-residual test tol*||b_c||; mode 1: FP32 PCG inside per-chart FP64 refinement until
- * ||b_c - A_c x_c|| <= tol*||b_c||.  info->iters is the largest per-chart iteration count and
+ * ||b_c - A_c x_c|| <= tol*||b_c|| and the chart's forward-error estimate is at most 1e-7 max|x_c| (the test
+ * of ruv_cg_solve_mixed, per chart).  info->iters is the largest per-chart iteration count and
  * info->true_rel_resid the worst chart; chart_iters / chart_true_resid (nchart, optional) give every chart.
  * ------------------------------------------------------------------------------------------------------- */
 typedef struct ruv_batch ruv_batch;

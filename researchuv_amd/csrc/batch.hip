@@ -108,6 +108,7 @@ int build_tiles(BatchImpl& Q, int spt) {
     HIPCHK(hipMalloc((void**)&Q.d_part_b, sizeof(double) * 2 * (size_t)T));
     HIPCHK(hipMalloc((void**)&Q.d_part_c, sizeof(double) * 2 * (size_t)T));
     HIPCHK(hipMalloc((void**)&Q.d_part_o, sizeof(double) * (size_t)T));
+    HIPCHK(hipMalloc((void**)&Q.d_part_d, sizeof(double) * 2 * (size_t)T));
     HIPCHK(hipMalloc((void**)&Q.act, sizeof(int) * (size_t)T));
     HIPCHK(hipHostMalloc((void**)&Q.h_act, sizeof(int) * (size_t)T, hipHostMallocDefault));
     HIPCHK(hipStreamSynchronize(Q.stream));  // host staging vectors die at return
@@ -121,6 +122,7 @@ int build_tiles(BatchImpl& Q, int spt) {
     B.part_b = Q.d_part_b;
     B.part_c = Q.d_part_c;
     B.part_o = Q.d_part_o;
+    B.part_d = Q.d_part_d;
     B.ntile = T;
     return RUV_OK;
 }
@@ -323,6 +325,9 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     HIPCHK(Q.alloc(&B.c_bnorm, nchart));
     HIPCHK(Q.alloc(&B.c_true, nchart));
     HIPCHK(Q.alloc(&B.c_rrec, nchart));
+    HIPCHK(Q.alloc(&B.c_rin, nchart));
+    HIPCHK(Q.alloc(&B.c_gamma, nchart));
+    HIPCHK(Q.alloc(&B.c_tgt, nchart));
     HIPCHK(Q.alloc(&B.c_sa, nchart));
     HIPCHK(Q.alloc(&B.c_sb, 2 * (size_t)nchart));
     HIPCHK(Q.alloc(&B.c_sc, 2 * (size_t)nchart));
@@ -482,7 +487,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
                            (const T*)Q.xv, d_x);
         hipLaunchKernelGGL((kb_residual<R, CI>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
                            (const double*)Q.ev64, d_b, (const double*)d_x, Q.r64);
-        hipLaunchKernelGGL(kb_outer, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol);
+        hipLaunchKernelGGL(kb_outer, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol, mixed ? Q.uv_tol : 0.0);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_final, B.c_final, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -606,6 +611,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (const char* e = std::getenv("RUV_BATCH_RESTART")) Q->continue_rounds = std::atoi(e) ? 0 : 1;
     if (const char* e = std::getenv("RUV_BATCH_CONT_GAP")) Q->cont_gap = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_LIST_BELOW")) Q->list_below = std::atof(e);
+    if (const char* e = std::getenv("RUV_UV_TOL")) Q->uv_tol = std::atof(e);
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

@@ -64,6 +64,11 @@ struct BDev {
     double* c_bnorm;
     double* c_true;
     double* c_rrec;  // recursive ||r|| of the step an inner solve converged at
+    // forward-error test of the mixed refinement: c_rin the FP64 ||r|| each round starts from, c_gamma the largest
+    // ratio max|correction| / ||r_start|| of a round so far, c_tgt the residual norm the chart must reach
+    double* c_rin;
+    double* c_gamma;
+    double* c_tgt;
     // big: charts with very many tiles; the step's chart-wide sums come from kb_chart_sums (p.q in c_sa, r.r and
     // r.z by parity in c_sb, c_sc) instead of every workgroup summing the chart's partials
     int32_t big;
@@ -78,6 +83,7 @@ struct BDev {
     double* part_b;  // [2][ntile] r.r by parity
     double* part_c;  // [2][ntile] r.z by parity
     double* part_o;  // outer FP64 residual / ||b||
+    double* part_d;  // [2][ntile] largest |correction| and |x| of the tile after a refinement round
     uint32_t max_iter;
     int32_t nchart;
     int32_t ntile;
@@ -535,8 +541,9 @@ static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double
     }
     if (threadIdx.x != 0) return;
     double ta;
-    if (mixed) ta = fmax(inner_tol * rn, 0.25 * ((bn <= 1e-300) ? tol : tol * bn));
-    else ta = (bn <= 1e-300) ? tol : tol * bn;
+    const double tres = (bn <= 1e-300) ? tol : tol * bn;
+    if (mixed) ta = fmax(inner_tol * rn, 0.25 * (B.c_tgt[c] > 0.0 ? B.c_tgt[c] : tres));
+    else ta = tres;
     B.c_tol[c] = ta;
     B.c_iters[c] = 0;
     B.c_rz[B.nchart + c] = rz_old;
@@ -570,18 +577,35 @@ static __global__ __launch_bounds__(BLK) void kb_bnorm(BDev B) {  // one workgro
     B.c_bnorm[c] = sqrt(s);
     B.c_final[c] = 0;
     B.c_true[c] = 1.0;
+    B.c_rin[c] = sqrt(s);
+    B.c_gamma[c] = 0.0;
+    B.c_tgt[c] = -1.0;  // before the first round: the residual test alone
 }
 
 // x64 += x (the inner correction) on the live charts
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kb_axpy(BDev B, const int* __restrict__ act, int na,
                                                const T* __restrict__ xin, double* __restrict__ x64) {
+    __shared__ double sh[WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
     const int t = act[slot];
     if (B.c_final[B.tile_chart[t]]) return;
+    double dmax = 0.0, xmax = 0.0;
     const int64_t r1 = B.tile_r1[t] * R;
-    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1; k += BLK) x64[k] += (double)xin[k];
+    for (int64_t k = B.tile_r0[t] * R + threadIdx.x; k < r1; k += BLK) {
+        const double d = (double)xin[k];
+        const double v = x64[k] + d;
+        x64[k] = v;
+        dmax = fmax(dmax, fabs(d));
+        xmax = fmax(xmax, fabs(v));
+    }
+    dmax = block_max_all(dmax, sh);
+    xmax = block_max_all(xmax, sh);
+    if (threadIdx.x == 0) {
+        B.part_d[t] = dmax;
+        B.part_d[B.ntile + t] = xmax;
+    }
 }
 
 // r64 = b - A64 x64 in FP64 on the live charts, partials of r64.r64 (after kb_axpy: a tile's rows read x64 of
@@ -623,14 +647,38 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
 
 // per chart: true residual; final when ||b - A x|| <= tol ||b||
 // one workgroup per chart
-static __global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol) {
+// The per-chart outer test after a refinement round.  The round's correction d solved A d = r_start, so
+// gamma = max|d| / ||r_start|| is how far that residual's solution reaches per unit of residual; the error left,
+// A^-1 r, is estimated as gamma ||r|| with the largest gamma of the rounds so far.  The chart is final when
+// ||r|| <= tol ||b|| and that estimate is at most uv_tol max|x| (uv_tol <= 0: the residual test alone), i.e. when
+// ||r|| <= c_tgt = min(tol ||b||, uv_tol max|x| / gamma), never asked below 1e-4 tol ||b||; a round that fails to
+// halve a residual already below tol ||b|| also ends the chart (the FP64 residual has stagnated).
+static __global__ __launch_bounds__(BLK) void kb_outer(BDev B, double tol, double uv_tol) {
     __shared__ double sh[WAVES];
     const int c = blockIdx.x;
     if (B.c_final[c]) return;
-    const double rn = sqrt(sum_range(B.part_o, B.chart_t0[c], B.chart_t0[c + 1], sh)), bn = B.c_bnorm[c];
+    const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
+    const double rn = sqrt(sum_range(B.part_o, t0, t1, sh)), bn = B.c_bnorm[c];
+    double dmax = 0.0, xmax = 0.0;
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += BLK) {
+        dmax = fmax(dmax, B.part_d[t]);
+        xmax = fmax(xmax, B.part_d[B.ntile + t]);
+    }
+    dmax = block_max_all(dmax, sh);
+    xmax = block_max_all(xmax, sh);
     if (threadIdx.x != 0) return;
     B.c_true[c] = (bn > 0.0) ? rn / bn : rn;
-    if (rn <= ((bn <= 1e-300) ? tol : tol * bn)) B.c_final[c] = 1;
+    const double tres = (bn <= 1e-300) ? tol : tol * bn;
+    const double rin = B.c_rin[c];
+    double tgt = tres;
+    if (uv_tol > 0.0) {
+        const double g = fmax(B.c_gamma[c], rin > 0.0 ? dmax / rin : 0.0);
+        B.c_gamma[c] = g;
+        if (g > 0.0) tgt = fmax(fmin(tres, uv_tol * xmax / g), 1e-4 * tres);
+    }
+    B.c_tgt[c] = tgt;
+    B.c_rin[c] = rn;
+    if (rn <= tgt || (rn <= tres && rn > 0.5 * rin)) B.c_final[c] = 1;
 }
 
 // Per running chart (one workgroup each), the fixed-order sums of the tile partials pa (and pb) into sa[c] (sb[c]):

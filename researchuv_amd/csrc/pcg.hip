@@ -226,6 +226,9 @@ struct PlanImpl {
     double* h_part = nullptr;
     double inner_tol = 1e-4;
     int max_refine = 40;
+    // mixed refinement: also stop only once the estimated forward error is at most uv_tol max|x| (a tenth of the
+    // 1e-6 UV bound, as the batched solver; 0 turns the test off)
+    double uv_tol = 1e-7;
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
 
@@ -245,6 +248,7 @@ namespace {
 int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
     const char* ng = std::getenv("RUV_NO_GRAPH");  // direct launches instead of graph replay (A/B checks)
     P.use_graph = !(ng && ng[0] == '1');
+    if (const char* e = std::getenv("RUV_UV_TOL")) P.uv_tol = std::atof(e);
     P.R = R;
     P.n = a.n;
     P.nnz = a.nnz();
@@ -339,6 +343,15 @@ int host_sum(PlanImpl& P, const double* part, double* out) {
     double s = 0.0;
     for (int i = 0; i < P.G; ++i) s += P.h_part[i];
     *out = s;
+    return RUV_OK;
+}
+
+int host_max(PlanImpl& P, const double* part, double* out) {
+    HIPCHK(hipMemcpyAsync(P.h_part, part, sizeof(double) * P.G, hipMemcpyDeviceToHost, P.stream));
+    HIPCHK(hipStreamSynchronize(P.stream));
+    double m = 0.0;
+    for (int i = 0; i < P.G; ++i) m = std::max(m, P.h_part[i]);
+    *out = m;
     return RUV_OK;
 }
 
@@ -577,18 +590,26 @@ int solve_mixed(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, 
     double rr = bb;
     uint64_t total = 0;
     uint32_t refs = 0;
+    // Outer test: ||r|| <= tol ||b|| and, when uv_tol > 0, the forward-error estimate gamma ||r|| <= uv_tol max|x|,
+    // gamma the largest ratio max|correction| / ||r_start|| of a round so far (the correction of a round solved
+    // A d = r_start); i.e. ||r|| <= goal = min(target, uv_tol max|x| / gamma), never below 1e-4 target.  A round
+    // that fails to halve a residual already at the target ends the loop (the FP64 residual has stagnated).
+    double goal = target, gamma = 0.0, rstart = std::sqrt(bb);
     for (;;) {
-        if (std::sqrt(rr) <= target) break;
+        const double rn = std::sqrt(rr);
+        if (rn <= goal) break;
+        if (refs > 0 && rn <= target && rn > 0.5 * rstart) break;
         if (refs >= (uint32_t)P.max_refine) {
             return fail(RUV_ERR_NOT_CONVERGED, "mixed: refinement stalled at rel. residual " +
                                                    std::to_string(std::sqrt(rr) / std::max(bnorm, 1e-300)));
         }
         if (total >= max_iter) return fail(RUV_ERR_NOT_CONVERGED, "mixed: inner iteration budget exhausted");
+        rstart = rn;
         // r32 = fp32(r64); inner FP32 PCG to inner_tol relative to ||r32||
         hipLaunchKernelGGL((k_to_f32), dim3(G), dim3(BLK), 0, s, N, (const double*)P.r64, P.r32);
         uint64_t it = 0;
         int done = 0;
-        double itol = std::max(P.inner_tol, 0.25 * target / std::max(std::sqrt(rr), 1e-300));
+        double itol = std::max(P.inner_tol, 0.25 * goal / std::max(rn, 1e-300));
         rc = run_pcg<float, R>(P, P.vals32, P.dinv32, P.x32, P.r32, P.z32, P.p32, P.q32, P.y32, P.t32,
                                max_iter - total, itol, &it, &done);
         if (rc) return rc;
@@ -596,13 +617,21 @@ int solve_mixed(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, 
         refs++;
         if (done == BREAKDOWN && it <= 1) return status_of(done, it);
         // x += d; r = b - A x (FP64) ; rr
-        hipLaunchKernelGGL((k_axpy_f32), dim3(G), dim3(BLK), 0, s, N, (const float*)P.x32, d_x);
+        hipLaunchKernelGGL((k_axpy_f32), dim3(G), dim3(BLK), 0, s, N, (const float*)P.x32, d_x, P.part_b, P.part_c);
         hipLaunchKernelGGL((k_residual<R>), dim3(G), dim3(BLK), 0, s, n, P.ptr, P.cols, P.vals64, (const double*)d_x,
                            d_b, P.r64, P.part_a);
         HIPCHK(hipGetLastError());
         rc = host_sum(P, P.part_a, &rr);
         if (rc) return rc;
         if (!std::isfinite(rr)) return fail(RUV_ERR_NONFINITE, "mixed: non-finite residual");
+        if (P.uv_tol > 0.0) {
+            double dmax = 0.0, xmax = 0.0;
+            rc = host_max(P, P.part_b, &dmax);
+            if (!rc) rc = host_max(P, P.part_c, &xmax);
+            if (rc) return rc;
+            if (rstart > 0.0) gamma = std::max(gamma, dmax / rstart);
+            if (gamma > 0.0) goal = std::max(std::min(target, P.uv_tol * xmax / gamma), 1e-4 * target);
+        }
     }
     HIPCHK(hipEventRecord(P.ev1, s));
     HIPCHK(hipEventSynchronize(P.ev1));

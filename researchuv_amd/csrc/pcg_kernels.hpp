@@ -49,9 +49,25 @@ __global__ __launch_bounds__(BLK) void k_to_f32(int64_t n, const double* __restr
     for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) dst[i] = (float)src[i];
 }
 
-// x64 += d32 (the FP64 solution update of the refinement loop)
-__global__ __launch_bounds__(BLK) void k_axpy_f32(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
-    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) x[i] += (double)d[i];
+// x64 += d32 (the FP64 solution update of the refinement loop); per-workgroup maxima of |d| and of the updated
+// |x| into pd[blockIdx.x], px[blockIdx.x]
+__global__ __launch_bounds__(BLK) void k_axpy_f32(int64_t n, const float* __restrict__ d, double* __restrict__ x,
+                                                  double* __restrict__ pd, double* __restrict__ px) {
+    __shared__ double sh[WAVES];
+    double dmax = 0.0, xmax = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLK) {
+        const double di = (double)d[i];
+        const double v = x[i] + di;
+        x[i] = v;
+        dmax = fmax(dmax, fabs(di));
+        xmax = fmax(xmax, fabs(v));
+    }
+    dmax = block_max_all(dmax, sh);
+    xmax = block_max_all(xmax, sh);
+    if (threadIdx.x == 0) {
+        pd[blockIdx.x] = dmax;
+        px[blockIdx.x] = xmax;
+    }
 }
 
 // per-workgroup partials of a.b (FP64 accumulation)
