@@ -198,6 +198,15 @@ def run_batch(args):
     log(f"SpMV over the whole batch: {sec * 1e3:.3f} ms/launch, {byt / 1e9:.3f} GB algorithmic ({sto / 1e9:.3f} GB "
         f"as stored) -> {byt / sec / 1e9:.0f} GB/s")
     bt.close()
+    if args.no_compare:
+        return dict(value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
+                    config={"workload": f"batch: {lb.nchart} uneven height-field patch charts", "charts": lb.nchart,
+                            "unknowns": unknowns, "tol_true_rel_resid": TOL,
+                            "tol_forward_error_estimate": UV_EST_TOL, "schedule": sched},
+                    extra={"max_inner_iters": int(info.iters), "precond_setup_in_step": pinfo,
+                           "note": "comparison legs skipped (--no-compare)"},
+                    roofline=dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                                  frac=achieved / HBM_PEAK, traffic=None), meshes=meshes)
 
     # the same batch in plain FP64: with the same device AMG (isolates the mixed-precision gain), and with
     # Jacobi (the reference's default preconditioner, lscm.rs:488); both stop on the recursive residual at tol
@@ -653,6 +662,8 @@ def main():
     ap.add_argument("--batch-precond", default="AMG", choices=["JACOBI", "AMG"],
                     help="batch: preconditioner of the mixed batched solve (AMG is built on the device each step)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="batch: skip the legs after the timed mixed solves (FP64 batches, LU checks, drop-in loop)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="batch: stream 1 GiB through one kernel first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()

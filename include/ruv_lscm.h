@@ -175,6 +175,17 @@ int ruv_batch_setup_precond(ruv_batch* batch);
 /* precond amg: rows of each level, finest first, into rows[0..cap); returns the number of levels (0 for the
  * other preconditioners). */
 int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap);
+/* precond amg: charts of the last build whose coarsest block is applied through its diagonal instead of its exact
+ * inverse (the coarsest blocks are factored on the device in FP64, Cholesky then the inverse of the factor,
+ * amg.rs:685-773 and precond.rs:439-498 factor theirs exactly on the host): blocks that are not numerically
+ * positive definite, blocks coupled to another chart, and blocks of more than 88 rows.  Zeros for the other
+ * preconditioners.  Any pointer may be null. */
+int ruv_batch_precond_fallbacks(ruv_batch* batch, int64_t* not_spd_charts, int64_t* coupled_charts,
+                                int64_t* too_large_charts);
+/* precond amg, inspection hook: chart's block of the coarsest operator (dense, row-major, FP64 from the stored
+ * values) and the FP64 inverse the device factorisation produced for it; *rows receives the block size (call with
+ * null block and inverse to size the buffers).  RUV_ERR_PRECOND for a chart kept on its diagonal. */
+int ruv_batch_coarsest_block(ruv_batch* batch, int32_t chart, int64_t* rows, double* block, double* inverse);
 int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes,
                            double* stored_bytes);
 void ruv_batch_destroy(ruv_batch* batch);

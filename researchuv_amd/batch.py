@@ -36,6 +36,10 @@ def _setup(L):
     L.ruv_batch_setup_precond.restype = C.c_int
     L.ruv_batch_precond_levels.restype = C.c_int
     L.ruv_batch_precond_levels.argtypes = [P, P, C.c_int32]
+    L.ruv_batch_precond_fallbacks.restype = C.c_int
+    L.ruv_batch_precond_fallbacks.argtypes = [P, P, P, P]
+    L.ruv_batch_coarsest_block.restype = C.c_int
+    L.ruv_batch_coarsest_block.argtypes = [P, C.c_int32, P, P, P]
     L.ruv_batch_setup_precond.argtypes = [P]
     L.ruv_batch_precond_info.argtypes = [P, P, P, P]
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
@@ -151,8 +155,21 @@ class Batch:
         _lib.check(lib().ruv_batch_precond_info(self.h, C.byref(lv), C.byref(sec), C.byref(rows)))
         per = np.zeros(64, np.int64)
         nl = lib().ruv_batch_precond_levels(self.h, _p(per), 64)
+        fb = np.zeros(3, np.int64)
+        _lib.check(lib().ruv_batch_precond_fallbacks(self.h, _p(fb[0:1]), _p(fb[1:2]), _p(fb[2:3])))
         return dict(levels=lv.value, setup_seconds=sec.value, coarse_rows=rows.value,
-                    level_rows=[int(v) for v in per[:max(nl, 0)]])
+                    level_rows=[int(v) for v in per[:max(nl, 0)]],
+                    coarsest_diagonal_charts={"not_spd": int(fb[0]), "coupled": int(fb[1]), "too_large": int(fb[2])})
+
+    def coarsest_block(self, chart: int):
+        """AMG: chart's dense block of the coarsest operator and the inverse the device factorisation made."""
+        m = C.c_int64(0)
+        _lib.check(lib().ruv_batch_coarsest_block(self.h, chart, C.byref(m), None, None))
+        blk = np.zeros((m.value, m.value))
+        inv = np.zeros((m.value, m.value))
+        if m.value:
+            _lib.check(lib().ruv_batch_coarsest_block(self.h, chart, C.byref(m), _p(blk), _p(inv)))
+        return blk, inv
 
     def kernel_timing(self, enable=True):
         """Bracket every step-kernel launch of the following solves with HIP events."""

@@ -27,6 +27,8 @@ int batch_kernel_timing(BatchImpl* Q, int enable);
 int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int64_t* coarse_rows);
 int batch_setup_precond(BatchImpl* Q);
 int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap);
+int batch_precond_fallbacks(BatchImpl* Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large);
+int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, double* inverse);
 int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes, double* stored);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
@@ -342,6 +344,17 @@ int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* 
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
                                slices, nontemporal);
+}
+
+int ruv_batch_precond_fallbacks(ruv_batch* batch, int64_t* not_spd_charts, int64_t* coupled_charts,
+                                int64_t* too_large_charts) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_precond_fallbacks(batch->impl, not_spd_charts, coupled_charts, too_large_charts);
+}
+
+int ruv_batch_coarsest_block(ruv_batch* batch, int32_t chart, int64_t* rows, double* block, double* inverse) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_coarsest_block(batch->impl, chart, rows, block, inverse);
 }
 
 int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap) {

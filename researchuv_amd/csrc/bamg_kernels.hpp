@@ -944,29 +944,121 @@ __global__ __launch_bounds__(BLK) void kc_coarsest(CycLevel<T> L) {
     st<T, R>(L.z, i, o);
 }
 
-// coarsest level with each chart's block inverted: z = inv(A_c) b over the rows of the row's chart (the inverse of
-// chart c is row-major at minv + moff[c], over the level's rows crow[c] .. crow[c + 1])
+// Coarsest level, each chart's block inverted: z = inv(A_c) b over the rows of the row's chart, accumulated in FP64
+// from the FP64 inverse (row-major at minv + moff[c], over the level's rows crow[c] .. crow[c + 1]); a chart with
+// moff[c] < 0 (a block too large for the device factorisation) is solved by its diagonal instead.
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_coarsest_dense(CycLevel<T> L, const int64_t* __restrict__ crow,
-                                                         const int64_t* __restrict__ moff, const T* __restrict__ minv) {
+                                                         const int64_t* __restrict__ moff,
+                                                         const double* __restrict__ minv) {
     const int64_t b = wg_item(L.blist), i = b * BLK + threadIdx.x;
     if (i >= L.n || !live_row(L, block_state(L.blk, b, L.done), i)) return;
     const int c = L.chart[i];
-    const int64_t r0 = crow[c], nc = crow[c + 1] - r0;
-    const T* m = minv + moff[c] + (i - r0) * nc;
-    T acc[R];
+    const int64_t r0 = crow[c], nc = crow[c + 1] - r0, mo = moff[c];
+    Vr<T, R> o;
+    if (mo < 0) {
+        const T d = L.dinv[i];
+        const Vr<T, R> bi = ld<T, R>(L.b, i);
 #pragma unroll
-    for (int cc = 0; cc < R; ++cc) acc[cc] = T(0);
+        for (int cc = 0; cc < R; ++cc) o.c[cc] = d * bi.c[cc];
+        st<T, R>(L.z, i, o);
+        return;
+    }
+    const double* m = minv + mo + (i - r0) * nc;
+    double acc[R];
+#pragma unroll
+    for (int cc = 0; cc < R; ++cc) acc[cc] = 0.0;
     for (int64_t j = 0; j < nc; ++j) {
-        const T v = m[j];
+        const double v = m[j];
         const Vr<T, R> bj = ld<T, R>(L.b, r0 + j);
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, bj.c[cc], acc[cc]);
+        for (int cc = 0; cc < R; ++cc) acc[cc] = fma(v, (double)bj.c[cc], acc[cc]);
     }
-    Vr<T, R> o;
 #pragma unroll
-    for (int cc = 0; cc < R; ++cc) o.c[cc] = acc[cc];
+    for (int cc = 0; cc < R; ++cc) o.c[cc] = (T)acc[cc];
     st<T, R>(L.z, i, o);
+}
+
+// The coarsest operator's block of every chart factored and inverted on the device in FP64, one workgroup per chart
+// (blocks of at most CHOL_MAX rows; the host marks larger ones moff[c] < 0 and they keep their diagonal).  The block
+// (summed from the coarsest operator's CSR, FP32 values) goes to LDS; a right-looking Cholesky A = G G^T (pivot test
+// d > 1e-13 |a_kk| on the original diagonal), the factor inverted in place from its last column (column j of
+// H = G^-1 is -H22 g_j / g_jj, H22 the already inverted trailing block), and inv = H^T H written out in FP64.  A
+// block that is not numerically positive definite keeps the inverse of its diagonal, and is counted in
+// fallback[0]; an entry whose column leaves the chart (which the aggregation never produces) is counted in
+// fallback[1] and the chart keeps its diagonal too.
+constexpr int CHOL_MAX = 88;  // (88^2 + 2 * 88) doubles of LDS stay below 64 KiB
+__global__ __launch_bounds__(BLK) void k_coarsest_chol(SetupCsr A, const int64_t* __restrict__ crow,
+                                                       const int64_t* __restrict__ moff, double* __restrict__ minv,
+                                                       unsigned long long* __restrict__ fallback) {
+    extern __shared__ double sm[];
+    __shared__ int bad;
+    const int c = blockIdx.x;
+    const int64_t r0 = crow[c], mo = moff[c];
+    const int m = (int)(crow[c + 1] - r0);
+    if (m == 0 || mo < 0) return;
+    double* a = sm;              // m x m, row-major, lower triangle used
+    double* d0 = sm + m * m;     // original diagonal
+    double* col = d0 + m;        // a column of the factor
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int q = tid; q < m * m; q += BLK) a[q] = 0.0;
+    __syncthreads();
+    for (int i = tid; i < m; i += BLK)
+        for (int64_t k = A.ptr[r0 + i]; k < A.ptr[r0 + i + 1]; ++k) {
+            const int64_t j = (int64_t)A.cols[k] - r0;
+            if (j < 0 || j >= m) bad = 2;  // (any writer; read after the barrier)
+            else a[i * m + j] += (double)A.vals[k];
+        }
+    __syncthreads();
+    for (int i = tid; i < m; i += BLK) d0[i] = a[i * m + i];
+    __syncthreads();
+    for (int k = 0; k < m && bad == 0; ++k) {
+        if (tid == 0) {
+            const double d = a[k * m + k];
+            if (!(d > 1e-13 * fabs(d0[k])) || !isfinite(d)) bad = 1;
+            else a[k * m + k] = sqrt(d);
+        }
+        __syncthreads();
+        if (bad) break;
+        const double g = a[k * m + k];
+        for (int i = k + 1 + tid; i < m; i += BLK) a[i * m + k] /= g;
+        __syncthreads();
+        const int w = m - k - 1;
+        for (int q = tid; q < w * w; q += BLK) {
+            const int ii = k + 1 + q / w, jj = k + 1 + q % w;
+            if (jj <= ii) a[ii * m + jj] -= a[ii * m + k] * a[jj * m + k];
+        }
+        __syncthreads();
+    }
+    double* out = minv + mo;
+    if (bad) {
+        if (tid == 0) atomicAdd(&fallback[bad == 1 ? 0 : 1], 1ull);
+        for (int q = tid; q < m * m; q += BLK) {
+            const int i = q / m, j = q % m;
+            out[q] = (i == j && d0[i] > 0.0) ? 1.0 / d0[i] : 0.0;
+        }
+        return;
+    }
+    for (int j = m - 1; j >= 0; --j) {  // a (lower) becomes H = G^-1
+        for (int p = j + 1 + tid; p < m; p += BLK) col[p] = a[p * m + j];
+        __syncthreads();
+        const double gjj = a[j * m + j];
+        for (int i = j + 1 + tid; i < m; i += BLK) {
+            double sacc = 0.0;
+            for (int p = j + 1; p <= i; ++p) sacc = fma(a[i * m + p], col[p], sacc);
+            a[i * m + j] = -sacc / gjj;
+        }
+        __syncthreads();
+        if (tid == 0) a[j * m + j] = 1.0 / gjj;
+        __syncthreads();
+    }
+    for (int q = tid; q < m * m; q += BLK) {  // inv = H^T H
+        const int i = q / m, j = q % m;
+        double sacc = 0.0;
+        for (int p = max(i, j); p < m; ++p) sacc = fma(a[p * m + i], a[p * m + j], sacc);
+        out[q] = sacc;
+    }
 }
 
 // Finest level (the batch's sliced-ELL operator), per live tile.  b is the PCG residual r.

@@ -40,6 +40,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z);
 int bamg_set_live(BatchImpl& Q, const int32_t* done);
 int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* coarse_rows);
 int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap);
+int bamg_fallbacks(BatchImpl& Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large);
+int bamg_coarsest_block(BatchImpl& Q, int c, int64_t* m_out, double* block, double* inverse);
 
 namespace {
 
@@ -691,6 +693,14 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes, double* 
 }
 
 int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap) { return bamg_level_rows(*Q, rows, cap); }
+
+int batch_precond_fallbacks(BatchImpl* Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large) {
+    return bamg_fallbacks(*Q, not_spd, coupled, too_large);
+}
+
+int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, double* inverse) {
+    return bamg_coarsest_block(*Q, c, m_out, block, inverse);
+}
 
 int batch_setup_precond(BatchImpl* Q) { return Q->precond == RUV_PC_AMG ? bamg_build(*Q) : RUV_OK; }
 

@@ -39,9 +39,13 @@ struct BamgLevel {
 struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
     void* ev0 = nullptr;        // the finest operator's values for the smoothing legs (CoarseVal<T>::type)
-    void* cinv = nullptr;       // T: each chart's inverted block of the coarsest operator (null: Jacobi there)
-    int64_t* cinv_off = nullptr;
+    double* cinv = nullptr;     // each chart's inverted block of the coarsest operator, FP64 (null: Jacobi there)
+    int64_t* cinv_off = nullptr;  // per chart: its block's first entry, -1 for a chart kept on its diagonal
     int64_t* crow_c = nullptr;  // chart row offsets on the coarsest level
+    std::vector<int64_t> h_cinv_off, h_crow_c;  // their host sources (alive until the build's last wait)
+    unsigned long long* chol_fallback = nullptr;  // [2] charts left on their diagonal: not SPD, coupled
+    SetupCsr coarse{};  // the coarsest operator (its memory owned here, for the inspection hook)
+    int64_t chol_too_large = 0;  // charts whose coarsest block exceeds the device factorisation
     // Jacobi weight of the smoothing steps: omega_max, or less where a level's Gershgorin bound g on D^-1 A
     // asks for it (omega <= 1.9 / g keeps each smoother an A-norm contraction, so the cycle stays SPD)
     double omega_max = 0.9;
@@ -268,100 +272,41 @@ __global__ void k_gather_offsets(int64_t m, const int64_t* __restrict__ at, cons
     if (k < m) out[k] = cid[at[k]];
 }
 
-// Invert each chart's block of the coarsest operator (FP64 CSR on the device, chart row offsets crow) on the host:
-// Cholesky, then the inverse of the factor; a block that is not numerically positive definite keeps the
-// inverse of its diagonal.  The inverses go to the device in the cycle's precision.
-int coarsest_dense(BatchImpl& Q, Bamg& A, const SetupCsr& cur, const std::vector<int64_t>& crow, bool f32) {
+// Factor and invert each chart's block of the coarsest operator on the device (k_coarsest_chol, FP64): offsets of
+// the blocks and of the chart rows from the host's copy of the level's chart offsets, nothing read back.
+int coarsest_device(BatchImpl& Q, Bamg& A, const SetupCsr& cur, const std::vector<int64_t>& crow) {
     hipStream_t s = Q.stream;
-    const int64_t n = cur.n;
     const int C = Q.nchart;
-    std::vector<int64_t> ptr((size_t)n + 1);
-    HIPCHK(hipMemcpyAsync(ptr.data(), cur.ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    const int64_t nnz = ptr[n];
-    std::vector<int32_t> cols((size_t)std::max<int64_t>(nnz, 1));
-    std::vector<float> vals((size_t)std::max<int64_t>(nnz, 1));
-    HIPCHK(hipMemcpyAsync(cols.data(), cur.cols, sizeof(int32_t) * nnz, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(vals.data(), cur.vals, sizeof(float) * nnz, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::vector<int64_t> off((size_t)C + 1, 0);
+    A.h_cinv_off.assign((size_t)C + 1, 0);
+    A.h_crow_c.assign(crow.begin(), crow.end());
+    int64_t total = 0;
+    int maxm = 0;
+    A.chol_too_large = 0;
     for (int c = 0; c < C; ++c) {
         const int64_t m = crow[c + 1] - crow[c];
-        off[c + 1] = off[c] + m * m;
-    }
-    std::vector<double> inv((size_t)std::max<int64_t>(off[C], 1), 0.0);
-    std::vector<double> a, g;
-    for (int c = 0; c < C; ++c) {
-        const int64_t r0 = crow[c], m = crow[c + 1] - r0;
-        if (m == 0) continue;
-        a.assign((size_t)(m * m), 0.0);
-        for (int64_t i = 0; i < m; ++i)
-            for (int64_t k = ptr[r0 + i]; k < ptr[r0 + i + 1]; ++k) {
-                const int64_t j = cols[k] - r0;
-                if (j < 0 || j >= m) return fail(RUV_ERR_PRECOND, "batch amg: coarsest block couples two charts");
-                a[i * m + j] += vals[k];
-            }
-        // lower Cholesky factor in g
-        g.assign((size_t)(m * m), 0.0);
-        bool spd = true;
-        for (int64_t k = 0; k < m && spd; ++k) {
-            double d = a[k * m + k];
-            for (int64_t p = 0; p < k; ++p) d -= g[k * m + p] * g[k * m + p];
-            if (!(d > 1e-13 * std::fabs(a[k * m + k])) || !std::isfinite(d)) {
-                spd = false;
-                break;
-            }
-            const double lk = std::sqrt(d);
-            g[k * m + k] = lk;
-            for (int64_t i = k + 1; i < m; ++i) {
-                double v = a[i * m + k];
-                for (int64_t p = 0; p < k; ++p) v -= g[i * m + p] * g[k * m + p];
-                g[i * m + k] = v / lk;
-            }
-        }
-        double* out = inv.data() + off[c];
-        if (!spd) {
-            for (int64_t i = 0; i < m; ++i) out[i * m + i] = (a[i * m + i] > 0.0) ? 1.0 / a[i * m + i] : 0.0;
+        if (m > CHOL_MAX) {
+            A.h_cinv_off[c] = -1;
+            ++A.chol_too_large;
             continue;
         }
-        // h = inverse of the factor (lower), in a; then inv = h^T h
-        std::fill(a.begin(), a.end(), 0.0);
-        for (int64_t j = 0; j < m; ++j) {
-            a[j * m + j] = 1.0 / g[j * m + j];
-            for (int64_t i = j + 1; i < m; ++i) {
-                double v = 0.0;
-                for (int64_t p = j; p < i; ++p) v -= g[i * m + p] * a[p * m + j];
-                a[i * m + j] = v / g[i * m + i];
-            }
-        }
-        for (int64_t i = 0; i < m; ++i)
-            for (int64_t j = 0; j <= i; ++j) {
-                double v = 0.0;
-                for (int64_t p = i; p < m; ++p) v += a[p * m + i] * a[p * m + j];
-                out[i * m + j] = v;
-                out[j * m + i] = v;
-            }
+        A.h_cinv_off[c] = total;
+        total += m * m;
+        maxm = std::max(maxm, (int)m);
     }
-    const size_t ne = inv.size();
-    if (f32) {
-        std::vector<float> f(ne);
-        for (size_t k = 0; k < ne; ++k) f[k] = (float)inv[k];
-        float* d = nullptr;
-        HIPCHK(A.alloc(&d, ne));
-        HIPCHK(hipMemcpyAsync(d, f.data(), sizeof(float) * ne, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
-        A.cinv = d;
-    } else {
-        double* d = nullptr;
-        HIPCHK(A.alloc(&d, ne));
-        HIPCHK(hipMemcpyAsync(d, inv.data(), sizeof(double) * ne, hipMemcpyHostToDevice, s));
-        A.cinv = d;
-    }
+    A.h_cinv_off[C] = total;
+    HIPCHK(A.alloc(&A.cinv, (size_t)std::max<int64_t>(total, 1)));
     HIPCHK(A.alloc(&A.cinv_off, (size_t)C + 1));
     HIPCHK(A.alloc(&A.crow_c, (size_t)C + 1));
-    HIPCHK(hipMemcpyAsync(A.cinv_off, off.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(A.crow_c, crow.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(A.alloc(&A.chol_fallback, 2));
+    HIPCHK(hipMemcpyAsync(A.cinv_off, A.h_cinv_off.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(A.crow_c, A.h_crow_c.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(A.chol_fallback, 0, 2 * sizeof(unsigned long long), s));
+    if (C > 0 && maxm > 0) {
+        const size_t lds = sizeof(double) * ((size_t)maxm * maxm + 2 * (size_t)maxm);
+        hipLaunchKernelGGL(k_coarsest_chol, dim3((unsigned)C), dim3(BLK), lds, s, cur, (const int64_t*)A.crow_c,
+                           (const int64_t*)A.cinv_off, A.cinv, A.chol_fallback);
+    }
+    HIPCHK(hipGetLastError());
     return RUV_OK;
 }
 
@@ -419,9 +364,8 @@ int bamg_build(BatchImpl& Q) {
     if (const char* e = std::getenv("RUV_BATCH_AMG_THETA")) theta = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_AMG_ROUNDS")) rounds = std::max(1, std::atoi(e));
     // coarsening stops once every chart has at most dense_rows rows; the coarsest level is then solved exactly
-    // with each chart's inverted block (up to dense_max rows a chart, else Jacobi)
+    // with each chart's inverted block (up to CHOL_MAX rows a chart, else its diagonal)
     int64_t dense_rows = 64;
-    const int64_t dense_max = 256;
     if (const char* e = std::getenv("RUV_BATCH_AMG_DENSE")) dense_rows = std::max(0, std::atoi(e));
     auto max_chart_rows = [&](const std::vector<int64_t>& cr) {
         int64_t m = 0;
@@ -701,9 +645,14 @@ int bamg_build(BatchImpl& Q) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(s));
     }
-    if (max_chart_rows(crow) <= dense_max) {
-        int rc = coarsest_dense(Q, *A, nat[nl - 1].A, crow, f32);
+    {
+        int rc = coarsest_device(Q, *A, nat[nl - 1].A, crow);
         if (rc) return rc;
+        A->coarse = nat[nl - 1].A;  // kept with the hierarchy
+        if (nat[nl - 1].own) {
+            for (void* p : nat[nl - 1].own->mem) A->mem.push_back(p);
+            nat[nl - 1].own->mem.clear();
+        }
     }
     {  // each chart's BLK-row blocks on levels >= 1, and room for the lists of running slices and blocks
         size_t cap = 0;
@@ -804,7 +753,7 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     if (nbc > 0) {
         if (A.cinv)
             hipLaunchKernelGGL((kc_coarsest_dense<T, R>), g1(nbc), dim3(BLK), 0, s, cyc(nl - 1),
-                               (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const T*)A.cinv);
+                               (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const double*)A.cinv);
         else
             hipLaunchKernelGGL((kc_coarsest<T, R>), g1(nbc), dim3(BLK), 0, s, cyc(nl - 1));
     }
@@ -842,6 +791,53 @@ template int bamg_apply<double, 1, 0>(BatchImpl&, int, int, const double*, doubl
 template int bamg_apply<double, 1, 1>(BatchImpl&, int, int, const double*, double*);
 template int bamg_apply<double, 2, 0>(BatchImpl&, int, int, const double*, double*);
 template int bamg_apply<double, 2, 1>(BatchImpl&, int, int, const double*, double*);
+
+// Charts whose coarsest block kept its diagonal instead of the exact inverse: not numerically positive definite,
+// coupled to another chart, or larger than the device factorisation takes (reads two counters back).
+int bamg_fallbacks(BatchImpl& Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large) {
+    unsigned long long h[2] = {0, 0};
+    if (Q.amg && Q.amg->chol_fallback) {
+        HIPCHK(hipMemcpyAsync(h, Q.amg->chol_fallback, sizeof(h), hipMemcpyDeviceToHost, Q.stream));
+        HIPCHK(hipStreamSynchronize(Q.stream));
+    }
+    if (not_spd) *not_spd = (int64_t)h[0];
+    if (coupled) *coupled = (int64_t)h[1];
+    if (too_large) *too_large = Q.amg ? Q.amg->chol_too_large : 0;
+    return RUV_OK;
+}
+
+// Chart c's block of the coarsest operator (dense, FP64 from the stored values) and the inverse the device
+// factorisation produced for it (row-major, m x m each; m alone when both pointers are null).  An inspection hook.
+int bamg_coarsest_block(BatchImpl& Q, int c, int64_t* m_out, double* block, double* inverse) {
+    if (!Q.amg || !Q.amg->cinv) return fail(RUV_ERR_INVALID, "batch amg: no coarsest inverse");
+    Bamg& A = *Q.amg;
+    if (c < 0 || c >= Q.nchart) return fail(RUV_ERR_INVALID, "batch amg: chart out of range");
+    const int64_t r0 = A.h_crow_c[c], m = A.h_crow_c[c + 1] - r0, mo = A.h_cinv_off[c];
+    if (m_out) *m_out = m;
+    if ((!block && !inverse) || m == 0) return RUV_OK;
+    if (mo < 0) return fail(RUV_ERR_PRECOND, "batch amg: this chart's coarsest block keeps its diagonal");
+    hipStream_t s = Q.stream;
+    if (inverse) HIPCHK(hipMemcpyAsync(inverse, A.cinv + mo, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+    if (block) {
+        std::vector<int64_t> ptr((size_t)m + 1);
+        HIPCHK(hipMemcpyAsync(ptr.data(), A.coarse.ptr + r0, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t nz = ptr[m] - ptr[0];
+        std::vector<int32_t> cols((size_t)std::max<int64_t>(nz, 1));
+        std::vector<float> vals((size_t)std::max<int64_t>(nz, 1));
+        HIPCHK(hipMemcpyAsync(cols.data(), A.coarse.cols + ptr[0], sizeof(int32_t) * nz, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(vals.data(), A.coarse.vals + ptr[0], sizeof(float) * nz, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::fill(block, block + m * m, 0.0);
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t k = ptr[i] - ptr[0]; k < ptr[i + 1] - ptr[0]; ++k) {
+                const int64_t j = cols[k] - r0;
+                if (j >= 0 && j < m) block[i * m + j] += (double)vals[k];
+            }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
 
 int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap) {
     if (!Q.amg) return 0;

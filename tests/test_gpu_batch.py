@@ -242,6 +242,36 @@ def test_batch_amg_matches_per_chart_lu(orc, mixed):
     assert info.iters * 2 < info_j.iters
 
 
+@pytest.mark.parametrize("mixed", [True, False])
+def test_batch_amg_coarsest_inverse_on_device(mixed):
+    """The coarsest level's blocks are factored (Cholesky) and inverted on the device in FP64, one workgroup per
+    chart: every chart's inverse against numpy's LU inverse of the same dense block, and no chart left on its
+    diagonal (amg.rs:685-773 and precond.rs:439-498 factor the coarsest level exactly)."""
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    meshes = _meshes(24, seed=6)
+    lb = B.LscmBatchBuilder(meshes, angle_min_deg=180.0, threads=4)
+    bt = lb.device_batch(precond=Precond.AMG, mixed=mixed)
+    pc = bt.precond_info()
+    say(f"coarsest blocks mixed={mixed}: {pc['level_rows'][-1]} rows over {lb.nchart} charts, {pc}")
+    assert pc["coarsest_diagonal_charts"] == {"not_spd": 0, "coupled": 0, "too_large": 0}
+    worst, sizes = 0.0, []
+    for c in range(lb.nchart):
+        blk, inv = bt.coarsest_block(c)
+        if blk.size == 0:
+            continue
+        sizes.append(len(blk))
+        np.testing.assert_array_equal(blk, blk.T)
+        ref = np.linalg.inv(blk)
+        worst = max(worst, np.abs(inv - ref).max() / np.abs(ref).max())
+        assert np.abs(inv @ blk - np.eye(len(blk))).max() < 1e-10
+    bt.close()
+    say(f"  {len(sizes)} blocks of {min(sizes)}..{max(sizes)} rows, worst inverse rel diff vs numpy {worst:.2e}")
+    assert max(sizes) >= 8
+    assert worst < 1e-11
+
+
 def test_batch_amg_general_slices_and_small_systems():
     """AMG over general (per-entry column) slices: the permuted chart of the wide-column test, and the golden
     small SPD systems with one right-hand side per row."""
