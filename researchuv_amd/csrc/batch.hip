@@ -112,7 +112,7 @@ int build_tiles(BatchImpl& Q, int spt) {
     HIPCHK(hipMalloc((void**)&Q.d_part_o, sizeof(double) * (size_t)T));
     HIPCHK(hipMalloc((void**)&Q.d_part_d, sizeof(double) * 2 * (size_t)T));
     HIPCHK(hipMalloc((void**)&Q.act, sizeof(int) * (size_t)T));
-    HIPCHK(hipHostMalloc((void**)&Q.h_act, sizeof(int) * (size_t)T, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&Q.h_act, sizeof(int) * 2 * (size_t)T, hipHostMallocDefault));  // two halves
     HIPCHK(hipStreamSynchronize(Q.stream));  // host staging vectors die at return
     BDev& B = Q.B;
     B.tile_s0 = Q.d_tile_s0;
@@ -337,6 +337,9 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     HIPCHK(Q.alloc(&B.c_final, nchart));
     HIPCHK(Q.alloc(&B.c_iters, nchart));
     HIPCHK(hipHostMalloc((void**)&Q.h_done, sizeof(int) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&Q.h_done_b, sizeof(int) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&Q.fev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&Q.fev[1], hipEventDisableTiming));
     HIPCHK(hipHostMalloc((void**)&Q.h_final, sizeof(int) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_iters, sizeof(unsigned) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&Q.h_true, sizeof(double) * (size_t)std::max(nchart, 1), hipHostMallocDefault));
@@ -344,15 +347,20 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     return build_tiles(Q, 2);
 }
 
-// upload the tiles of the charts whose flag[c] satisfies pred; returns the count
+// upload the tiles of the charts whose flag[c] satisfies pred; returns the count.  The pinned staging alternates
+// between two halves, so a list can be written while the copy of the previous one is still queued (the copy of the
+// one before that has completed: every caller waits on the stream, or on a look recorded after that copy, at
+// least once between two lists).
 template <typename Pred>
 int set_active(BatchImpl& Q, Pred pred, int* count) {
+    Q.act_half ^= 1;
+    int* h = Q.h_act + (size_t)Q.act_half * (size_t)Q.ntile;
     int n = 0;
     for (int c = 0; c < Q.nchart; ++c)
         if (pred(c))
-            for (int t = Q.chart_t0[c]; t < Q.chart_t0[c + 1]; ++t) Q.h_act[n++] = t;
+            for (int t = Q.chart_t0[c]; t < Q.chart_t0[c + 1]; ++t) h[n++] = t;
     *count = n;
-    if (n) HIPCHK(hipMemcpyAsync(Q.act, Q.h_act, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, Q.stream));
+    if (n) HIPCHK(hipMemcpyAsync(Q.act, h, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, Q.stream));
     return RUV_OK;
 }
 
@@ -403,21 +411,22 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
             hipLaunchKernelGGL(kb_chart_sums, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, pa, pb, sa, sb);
     };
     chart_sums(B.part_b, B.part_c, B.c_sb, B.c_sc);  // parity 0
-    HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
+    // Which charts still run is looked at after every chunk of steps: the flags after chunk j are copied to a pinned
+    // buffer, marked by an event, and chunk j + 1 launches the tiles of the charts still running.  With look_lag the
+    // look is one chunk late while most tiles run: chunk j + 1 is queued with the charts that ran after chunk j - 1
+    // while the host waits for chunk j's flags, so the device never idles for a look (the lists only shrink and
+    // every kernel leaves at once for a stopped chart, so an older list is a superset that costs early exits).
+    // Flags alternate between two pinned buffers, each copy with its own event.
+    int* flags[2] = {Q.h_done, Q.h_done_b};
+    HIPCHK(hipMemcpyAsync(flags[0], B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    // steps between looks at which charts still run: growing up to 64 for the cheap Jacobi/plain steps; the AMG
-    // step costs a V-cycle, so a look (one small copy and a wait) every few steps wastes less than overshooting
+    // steps between looks: growing up to 64 for the cheap Jacobi/plain steps; the AMG step costs a V-cycle, so a
+    // look every few steps keeps the lists tight
     int chunk = (PC == 2) ? Q.amg_chunk : 8;
     uint32_t iter = 0;
     int par = 0;
-    for (;;) {
-        rc = set_active(Q, [&](int c) { return Q.h_done[c] == B_RUNNING; }, &na);
-        if (rc) return rc;
-        if (na == 0) break;
-        // once most charts have stopped, the coarse levels launch only the running charts' slices and blocks
-        if (PC == 2) rc = bamg_set_live(Q, (double)na < Q.list_below * (double)Q.ntile ? Q.h_done : nullptr);
-        if (rc) return rc;
-        for (int it = 0; it < chunk; ++it) {
+    auto run_chunk = [&](int steps) -> int {
+        for (int it = 0; it < steps; ++it) {
             hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
                                (const T*)pb[par ^ 1], pb[par]);
             if (Q.ktime) {
@@ -440,8 +449,8 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
             hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
                                dinv, (const T*)pb[par], (const T*)q, x, r, z);
             if (PC == 2) {
-                rc = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z);
-                if (rc) return rc;
+                int rc2 = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z);
+                if (rc2) return rc2;
             }
             chart_sums(B.part_b + (int64_t)(par ^ 1) * Q.ntile, B.part_c + (int64_t)(par ^ 1) * Q.ntile,
                        B.c_sb + (int64_t)(par ^ 1) * Q.nchart, B.c_sc + (int64_t)(par ^ 1) * Q.nchart);
@@ -449,10 +458,38 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
             ++iter;
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        return RUV_OK;
+    };
+    // flags[j % 2] receives the flags after chunk j (chunk 0: the state kb_init left, in flags[0]); `known` is the
+    // buffer of the latest flags waited for
+    int known = 0;
+    for (int64_t j = 1;; ++j) {
+        const int* kf = flags[known];
+        rc = set_active(Q, [&](int c) { return kf[c] == B_RUNNING; }, &na);
+        if (rc) return rc;
+        if (na == 0) break;
+        // once most charts have stopped, the coarse levels launch only the running charts' slices and blocks
+        if (PC == 2) rc = bamg_set_live(Q, (double)na < Q.list_below * (double)Q.ntile ? kf : nullptr);
+        if (rc) return rc;
+        rc = run_chunk(chunk);
+        if (rc) return rc;
+        // the flags after chunk j go to the buffer read above (the list is built); read one iteration later
+        HIPCHK(hipMemcpyAsync(flags[j & 1], B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(Q.fev[j & 1], s));
+        // the look one chunk late pays while most tiles run (a chunk then takes milliseconds, and the lists barely
+        // change); late in a solve it would queue a chunk of early exits after the last chart stops
+        if (!Q.look_lag || 2 * (int64_t)na < (int64_t)Q.ntile) {  // this chunk's flags (the device waits)
+            HIPCHK(hipEventSynchronize(Q.fev[j & 1]));
+            known = (int)(j & 1);
+        } else if (j > 1) {  // chunk j - 1's flags, while chunk j runs
+            HIPCHK(hipEventSynchronize(Q.fev[(j - 1) & 1]));
+            known = (int)((j - 1) & 1);
+        }
         if (PC != 2) chunk = std::min(chunk * 2, 64);
     }
+    // every queued chunk done, and the final states
+    HIPCHK(hipMemcpyAsync(Q.h_done, B.c_done, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipMemcpyAsync(Q.h_iters, B.c_iters, sizeof(unsigned) * Q.nchart, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     for (int c = 0; c < Q.nchart; ++c) {
@@ -614,6 +651,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (const char* e = std::getenv("RUV_BATCH_CONT_GAP")) Q->cont_gap = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_LIST_BELOW")) Q->list_below = std::atof(e);
     if (const char* e = std::getenv("RUV_UV_TOL")) Q->uv_tol = std::atof(e);
+    if (const char* e = std::getenv("RUV_BATCH_LOOK_LAG")) Q->look_lag = std::atoi(e) ? 1 : 0;
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;
     rc = autotune(*Q);

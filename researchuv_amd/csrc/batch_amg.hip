@@ -64,7 +64,9 @@ struct Bamg {
     std::vector<int32_t*> slist, blist;
     std::vector<int64_t> nslist, nblist;
     int32_t* d_lists = nullptr;
-    int32_t* h_lists = nullptr;  // the batch's pinned staging buffer (not owned)
+    int32_t* h_lists = nullptr;  // the batch's pinned staging buffer (not owned): two halves of lists_cap
+    size_t lists_cap = 0;
+    int lists_half = 0;
     DevPool* pool = nullptr;
     std::vector<void*> mem;
     explicit Bamg(DevPool* p) : pool(p) {}
@@ -668,14 +670,16 @@ int bamg_build(BatchImpl& Q) {
             cap += (size_t)A->lv[l].nsl + (size_t)((A->lv[l].n + BLK - 1) / BLK);
         }
         HIPCHK(A->alloc(&A->d_lists, cap));
-        if (cap > Q.h_lists_cap) {  // no copy from it is in flight: every solve ends with a wait on the stream
+        if (2 * cap > Q.h_lists_cap) {  // no copy from it is in flight: every solve ends with a wait on the stream
             if (Q.h_lists) (void)hipHostFree(Q.h_lists);
             Q.h_lists = nullptr;
             Q.h_lists_cap = 0;
-            HIPCHK(hipHostMalloc((void**)&Q.h_lists, sizeof(int32_t) * std::max<size_t>(cap, 1), hipHostMallocDefault));
-            Q.h_lists_cap = cap;
+            HIPCHK(hipHostMalloc((void**)&Q.h_lists, sizeof(int32_t) * std::max<size_t>(2 * cap, 1),
+                                 hipHostMallocDefault));
+            Q.h_lists_cap = 2 * cap;
         }
         A->h_lists = Q.h_lists;
+        A->lists_cap = cap;
     }
     HIPCHK(hipEventRecord(e1, s));
     HIPCHK(hipEventSynchronize(e1));
@@ -697,13 +701,17 @@ int bamg_set_live(BatchImpl& Q, const int32_t* done) {
     std::fill(A.slist.begin(), A.slist.end(), nullptr);
     std::fill(A.blist.begin(), A.blist.end(), nullptr);
     if (!done) return RUV_OK;
+    // staged in the half not used by the previous lists, whose copy may still be queued (the copy before that has
+    // completed: the solver waits for a look recorded after it before it builds the next lists)
+    A.lists_half ^= 1;
+    int32_t* h = A.h_lists + (size_t)A.lists_half * A.lists_cap;
     size_t pos = 0;
     auto add = [&](const std::vector<int32_t>& f, const std::vector<int32_t>& e, int32_t** list, int64_t* count) {
         const size_t start = pos;
         int32_t last = -1;
         for (int c = 0; c < Q.nchart; ++c)
             if (done[c] == B_RUNNING)
-                for (int32_t b = std::max(f[c], last + 1); b < e[c]; ++b) A.h_lists[pos++] = last = b;
+                for (int32_t b = std::max(f[c], last + 1); b < e[c]; ++b) h[pos++] = last = b;
         *list = A.d_lists + start;
         *count = (int64_t)(pos - start);
     };
@@ -711,7 +719,7 @@ int bamg_set_live(BatchImpl& Q, const int32_t* done) {
         if (l + 1 < nl) add(A.cs0[l], A.cs1[l], &A.slist[l], &A.nslist[l]);
         add(A.cb0[l], A.cb1[l], &A.blist[l], &A.nblist[l]);
     }
-    if (pos) HIPCHK(hipMemcpyAsync(A.d_lists, A.h_lists, sizeof(int32_t) * pos, hipMemcpyHostToDevice, Q.stream));
+    if (pos) HIPCHK(hipMemcpyAsync(A.d_lists, h, sizeof(int32_t) * pos, hipMemcpyHostToDevice, Q.stream));
     return RUV_OK;
 }
 

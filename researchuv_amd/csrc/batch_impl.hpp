@@ -80,7 +80,7 @@ struct BatchImpl {
     int64_t *d_tile_r0 = nullptr, *d_tile_r1 = nullptr;
     double *d_part_a = nullptr, *d_part_b = nullptr, *d_part_c = nullptr, *d_part_o = nullptr, *d_part_d = nullptr;
     int* act = nullptr;
-    int* h_act = nullptr;
+    int* h_act = nullptr;  // pinned staging of the active-tile lists: two halves of ntile
     // matrix
     float* ev32 = nullptr;
     double* ev64 = nullptr;
@@ -90,7 +90,13 @@ struct BatchImpl {
     double *x64 = nullptr, *r64 = nullptr;
     void *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv0 = nullptr, *pv1 = nullptr, *qv = nullptr;
     int* h_done = nullptr;
-    int32_t* h_lists = nullptr;  // pinned staging of the AMG's running-chart launch lists (grown, kept)
+    int* h_done_b = nullptr;           // second pinned flag buffer (the look one chunk late)
+    hipEvent_t fev[2] = {nullptr, nullptr};
+    int act_half = 0;                  // half of h_act the last list was staged in
+    // 1: look at which charts run one chunk late while most tiles run, so the device never waits for a look
+    // (measured on the 4096-patch batch: 154.5 ms against 152.9 ms for looking right after each chunk)
+    int look_lag = 0;
+    int32_t* h_lists = nullptr;  // pinned staging of the AMG's running-chart launch lists: two halves (grown, kept)
     size_t h_lists_cap = 0;
     int* h_final = nullptr;
     unsigned* h_iters = nullptr;
@@ -157,8 +163,11 @@ struct BatchImpl {
         if (amg) bamg_destroy(amg);
         free_tiles();
         for (void* p : mem) (void)hipFree(p);
-        for (void* p : {(void*)h_done, (void*)h_final, (void*)h_iters, (void*)h_true, (void*)h_lists})
+        for (void* p : {(void*)h_done, (void*)h_done_b, (void*)h_final, (void*)h_iters, (void*)h_true,
+                        (void*)h_lists})
             if (p) (void)hipHostFree(p);
+        for (hipEvent_t e : fev)
+            if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : kev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
