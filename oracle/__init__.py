@@ -316,6 +316,45 @@ def lscm_loss_frozen(p3, tris, pin_ids, pin_uv, target):
     return (loss, uv) if ok.value else None
 
 
+def _vjp_setup(L):
+    if not getattr(L, "_vjp_ready", False):
+        P = C.c_void_p
+        L.orc_lscm_vjp_frozen.restype = C.c_int
+        L.orc_lscm_vjp_frozen.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_int, P, C.c_double, P, P]
+        L.orc_lscm_grad_scatter.restype = None
+        L.orc_lscm_grad_scatter.argtypes = [P, C.c_int, P, C.c_int, P, P, C.c_double, P]
+        L._vjp_ready = True
+    return L
+
+
+def lscm_vjp_frozen(p3, tris, pin_ids, pin_uv, g_uv, energy_weight=1.0):
+    """dJ/dp for J = energy_weight * E(uv(p), p) + <g_uv, uv(p)> (frozen pins) by two sparse-LU solves with the
+    same matrix (L^T = L) -> (uv, grad (nv, 3)); None when an LU solve fails."""
+    L = _vjp_setup(lib())
+    p3 = np.ascontiguousarray(p3, np.float64)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    pin_ids = np.ascontiguousarray(pin_ids, np.int32)
+    pin_uv = np.ascontiguousarray(pin_uv, np.float64)
+    g_uv = np.ascontiguousarray(g_uv, np.float64)
+    uv = np.zeros((len(p3), 2))
+    grad = np.zeros((len(p3), 3))
+    rc = L.orc_lscm_vjp_frozen(_p(p3), len(p3), _p(tris), len(tris), _p(pin_ids), _p(pin_uv), len(pin_ids), _p(g_uv),
+                               float(energy_weight), _p(uv), _p(grad))
+    return (uv, grad) if rc == 0 else None
+
+
+def lscm_grad_scatter(p3, tris, uv, lam, energy_weight=1.0):
+    """The assembly half of the VJP given the solution uv and the adjoint lam (zero at pins) -> grad (nv, 3)."""
+    L = _vjp_setup(lib())
+    p3 = np.ascontiguousarray(p3, np.float64)
+    tris = np.ascontiguousarray(tris, np.uint32)
+    uv = np.ascontiguousarray(uv, np.float64)
+    lam = np.ascontiguousarray(lam, np.float64)
+    grad = np.zeros((len(p3), 3))
+    L.orc_lscm_grad_scatter(_p(p3), len(p3), _p(tris), len(tris), _p(uv), _p(lam), float(energy_weight), _p(grad))
+    return grad
+
+
 # ------------------------------------------------------------------ post-solve chart metrics
 def winding_flips(uv, tris):
     """metrics.rs:39-56"""

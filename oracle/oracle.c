@@ -29,6 +29,8 @@
  *   orc_par_unfold ................. bitwise equal for 1, 2, 3 and 8 workers and to the serial driver
  *   orc_lscm_loss_frozen ........... affine maps on a right-triangle grid: uv == A x + t, closed-form energy
  *                                    2 nx ny (hy/hx |A e_x|^2 + hx/hy |A e_y|^2) plus the target term
+ *   orc_lscm_vjp_frozen, orc_lscm_grad_scatter .. central differences of orc_lscm_loss_frozen; rigid-motion
+ *                                    invariance (sum of gradients 0, sum p x grad 0); scale: sum p . grad = -2 E
  *   orc_cg_solve ................... 1-D Laplacian x == 1 (solver.rs:767-779)
  *   orc_ic0_* ...................... L*L^T == A on the pattern (precond.rs:708-737), level partition (:739-755),
  *                                    dense triangular solve of the same factor
@@ -1199,6 +1201,133 @@ double orc_lscm_loss_frozen(const double* p3, int nv, const uint32_t* tris, int 
     sparse_free(&m);
     free(sol); free(rhs); free(cval); free(ew); free(free_pos); free(isfree); free(pin_lookup);
     return loss;
+}
+
+/* Gradient of the frozen-pin loss through the assembly, given the solution uv (nv*2, pins at their targets) and
+ * the adjoint lam (nv*2, zero at pins; L lam = g_uv on the free vertices, L^T = L): for every assembled edge e =
+ * (i, j) (w_e > EPS, c_e = w_e / max(|p_i - p_j|^2, EPS), lscm.rs:252-274,610)
+ *   G_e = energy_weight |uv_i - uv_j|^2 - (lam_i - lam_j) . (uv_i - uv_j)
+ * (dE/dc_e at fixed uv, and lam^T d(b - L x)/dc_e), and grad (nv*3) += G_e dc_e/dp: through |p_i - p_j|^2 and,
+ * for every triangle corner o opposite e with cot > 0, through cot = (a1 . a2) / |a1 x a2| (a1 = p_e1 - p_o,
+ * a2 = p_e0 - p_o; the corners that lscm.rs:252-274 skips or clamps to 0 contribute nothing). */
+static int cmp_edge_key(const void* a, const void* b) {
+    const ewrec* x = (const ewrec*)a; const ewrec* y = (const ewrec*)b;
+    if (x->i != y->i) return x->i < y->i ? -1 : 1;
+    if (x->j != y->j) return x->j < y->j ? -1 : 1;
+    return 0;
+}
+void orc_lscm_grad_scatter(const double* p3, int nv, const uint32_t* tris, int ntri, const double* uv,
+                           const double* lam, double energy_weight, double* grad) {
+    ewrec* ew;
+    size_t ne = edge_weights(p3, tris, ntri, &ew);
+    double* G = (double*)calloc(ne ? ne : 1, sizeof(double));     /* dJ/dc_e */
+    double* dd = (double*)calloc(ne ? ne : 1, sizeof(double));
+    memset(grad, 0, sizeof(double) * 3 * (size_t)nv);
+    for (size_t e = 0; e < ne; ++e) {
+        uint32_t i = ew[e].i, j = ew[e].j;
+        if (ew[e].w <= EPS_DEGENERATE) continue;
+        v3 d3 = v3sub(P3(p3, j), P3(p3, i));
+        double q = v3dot(d3, d3);
+        dd[e] = q > EPS_DEGENERATE ? q : EPS_DEGENERATE;
+        double du = uv[2 * i] - uv[2 * j], dv = uv[2 * i + 1] - uv[2 * j + 1];
+        double lu = lam[2 * i] - lam[2 * j], lv = lam[2 * i + 1] - lam[2 * j + 1];
+        G[e] = energy_weight * (du * du + dv * dv) - (lu * du + lv * dv);
+        if (q > EPS_DEGENERATE) { /* d(w / |p_j - p_i|^2) through the length */
+            double f = G[e] * 2.0 * ew[e].w / (q * q);
+            grad[3 * j] -= f * d3.x; grad[3 * j + 1] -= f * d3.y; grad[3 * j + 2] -= f * d3.z;
+            grad[3 * i] += f * d3.x; grad[3 * i + 1] += f * d3.y; grad[3 * i + 2] += f * d3.z;
+        }
+    }
+    for (int t = 0; t < ntri; ++t) {
+        uint32_t a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+        uint32_t trip[3][3] = {{a, b, c}, {b, c, a}, {c, a, b}};
+        for (int k = 0; k < 3; ++k) {
+            uint32_t e0 = trip[k][0], e1 = trip[k][1], o = trip[k][2];
+            v3 a1 = v3sub(P3(p3, e1), P3(p3, o));
+            v3 a2 = v3sub(P3(p3, e0), P3(p3, o));
+            double la = v3len(a1), lb = v3len(a2);
+            if (la <= EPS_DEGENERATE || lb <= EPS_DEGENERATE) continue;
+            v3 n = v3cross(a1, a2);
+            double N = v3len(n);
+            if (N / (la * lb) <= EPS_DEGENERATE) continue;
+            double D = v3dot(a1, a2);
+            if (!(D / N > 0.0)) continue; /* clamped to 0 */
+            ewrec key;
+            key.i = e0 < e1 ? e0 : e1; key.j = e0 < e1 ? e1 : e0;
+            ewrec* hit = (ewrec*)bsearch(&key, ew, ne, sizeof(ewrec), cmp_edge_key);
+            if (!hit || hit->w <= EPS_DEGENERATE) continue;
+            size_t e = (size_t)(hit - ew);
+            double gw = G[e] / dd[e]; /* dJ/dw_e */
+            v3 nh = {n.x / N, n.y / N, n.z / N};
+            v3 c1 = v3cross(a2, nh), c2 = v3cross(nh, a1); /* dN/da1, dN/da2 */
+            double N2 = N * N;
+            v3 g1 = {(a2.x * N - D * c1.x) / N2, (a2.y * N - D * c1.y) / N2, (a2.z * N - D * c1.z) / N2};
+            v3 g2 = {(a1.x * N - D * c2.x) / N2, (a1.y * N - D * c2.y) / N2, (a1.z * N - D * c2.z) / N2};
+            grad[3 * e1] += gw * g1.x; grad[3 * e1 + 1] += gw * g1.y; grad[3 * e1 + 2] += gw * g1.z;
+            grad[3 * e0] += gw * g2.x; grad[3 * e0 + 1] += gw * g2.y; grad[3 * e0 + 2] += gw * g2.z;
+            grad[3 * o] -= gw * (g1.x + g2.x); grad[3 * o + 1] -= gw * (g1.y + g2.y);
+            grad[3 * o + 2] -= gw * (g1.z + g2.z);
+        }
+    }
+    free(G); free(dd); free(ew);
+}
+
+/* The frozen-pin VJP by the sparse LU (sparse.rs:219-265): uv from the forward solve, lam from a second solve of
+ * the same matrix with the free rows of g_uv as the right-hand side (the slack row 0), then
+ * orc_lscm_grad_scatter.  uv_out (nv*2), grad (nv*3); returns 0 on success, -1 when an LU solve fails. */
+int orc_lscm_vjp_frozen(const double* p3, int nv, const uint32_t* tris, int ntri, const int32_t* pin_ids,
+                        const double* pin_uv, int npins, const double* g_uv, double energy_weight, double* uv_out,
+                        double* grad) {
+    double* target = (double*)calloc(2 * (size_t)(nv ? nv : 1), sizeof(double));
+    int ok = 0;
+    (void)orc_lscm_loss_frozen(p3, nv, tris, ntri, pin_ids, pin_uv, npins, target, uv_out, &ok);
+    free(target);
+    if (!ok) return -1;
+    uint8_t* isfree = (uint8_t*)malloc((size_t)(nv ? nv : 1));
+    memset(isfree, 1, (size_t)(nv ? nv : 1));
+    for (int q = 0; q < npins; ++q) isfree[pin_ids[q]] = 0;
+    int32_t* free_pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nv ? nv : 1));
+    int nfree = 0;
+    for (int i = 0; i < nv; ++i) free_pos[i] = isfree[i] ? nfree++ : -1;
+    ewrec* ew;
+    size_t ne = edge_weights(p3, tris, ntri, &ew);
+    int32_t nvar = 2 * nfree + 1, aux = 2 * nfree;
+    sparse m;
+    sparse_init(&m, nvar);
+    for (size_t e = 0; e < ne; ++e) { /* the matrix of orc_lscm_loss_frozen */
+        uint32_t i = ew[e].i, j = ew[e].j;
+        if (ew[e].w <= EPS_DEGENERATE) continue;
+        v3 d3 = v3sub(P3(p3, j), P3(p3, i));
+        double dd = v3dot(d3, d3);
+        double c = ew[e].w / (dd > EPS_DEGENERATE ? dd : EPS_DEGENERATE);
+        for (int side = 0; side < 2; ++side) {
+            uint32_t aa = side ? j : i, bb = side ? i : j;
+            if (!isfree[aa]) continue;
+            int32_t ra = 2 * free_pos[aa];
+            srow_add(&m.rows[ra], ra, c);
+            srow_add(&m.rows[ra + 1], ra + 1, c);
+            if (isfree[bb]) {
+                srow_add(&m.rows[ra], 2 * free_pos[bb], -c);
+                srow_add(&m.rows[ra + 1], 2 * free_pos[bb] + 1, -c);
+            }
+        }
+    }
+    srow_add(&m.rows[aux], aux, 1.0);
+    double* rhs = (double*)calloc((size_t)nvar, sizeof(double));
+    double* sol = (double*)malloc(sizeof(double) * (size_t)nvar);
+    for (int v = 0; v < nv; ++v)
+        if (isfree[v]) { rhs[2 * free_pos[v]] = g_uv[2 * v]; rhs[2 * free_pos[v] + 1] = g_uv[2 * v + 1]; }
+    int rc = sparse_solve(&m, rhs, sol) == 0 ? 0 : -1;
+    if (rc == 0) {
+        double* lam = (double*)calloc(2 * (size_t)(nv ? nv : 1), sizeof(double));
+        for (int v = 0; v < nv; ++v)
+            if (isfree[v]) { lam[2 * v] = sol[2 * free_pos[v]]; lam[2 * v + 1] = sol[2 * free_pos[v] + 1]; }
+        orc_lscm_grad_scatter(p3, nv, tris, ntri, uv_out, lam, energy_weight, grad);
+        free(lam);
+    }
+    sparse_free(&m);
+    free(rhs); free(sol); free(ew); free(free_pos); free(isfree);
+    return rc;
 }
 
 /* Export the chart system: sizes[0]=nv, [1]=nfree, [2]=npins, [3]=nnz of the leading 2*nfree block.

@@ -196,3 +196,114 @@ def test_device_diff_batch_matches_union_solve(orc, pc, mixed):
         assert abs(e_dev - e_ref) <= 1e-9 * abs(e_ref)
         assert np.abs(pt.grad.numpy() - g_ref).max() <= 1e-6 * np.abs(g_ref).max()
     db.close()
+
+
+def _frozen_system(P, tris, pin_ids, pin_uv):
+    """The frozen-pin free block L (cotan weights c_e = w_e / |e|^2, lscm.rs:252-274,610, over the free vertices)
+    and its right-hand side (pin terms), assembled with numpy -> (L csr, rhs (nfree, 2), free ids)."""
+    import scipy.sparse as sp
+
+    t = np.asarray(tris, np.int64)
+    e0 = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+    e1 = np.concatenate([t[:, 1], t[:, 2], t[:, 0]])
+    o = np.concatenate([t[:, 2], t[:, 0], t[:, 1]])
+    a1, a2 = P[e1] - P[o], P[e0] - P[o]
+    cot = np.maximum(np.einsum("ij,ij->i", a1, a2) / np.linalg.norm(np.cross(a1, a2), axis=1), 0.0)
+    lo, hi = np.minimum(e0, e1), np.maximum(e0, e1)
+    key = lo * len(P) + hi
+    uk, inv = np.unique(key, return_inverse=True)
+    w = np.bincount(inv, weights=cot)
+    i, j = uk // len(P), uk % len(P)
+    keep = w > 0.0
+    i, j, w = i[keep], j[keep], w[keep]
+    c = w / ((P[i] - P[j]) ** 2).sum(axis=1)
+    nv = len(P)
+    free = np.ones(nv, bool)
+    free[np.asarray(pin_ids)] = False
+    fid = np.flatnonzero(free)
+    pos = -np.ones(nv, np.int64)
+    pos[fid] = np.arange(len(fid))
+    pin_val = np.zeros((nv, 2))
+    pin_val[np.asarray(pin_ids)] = pin_uv
+    rows, cols, vals = [], [], []
+    rhs = np.zeros((len(fid), 2))
+    for a, b in ((i, j), (j, i)):
+        fa = free[a]
+        rows += [pos[a[fa]]]
+        cols += [pos[a[fa]]]
+        vals += [c[fa]]
+        ff = fa & free[b]
+        rows += [pos[a[ff]]]
+        cols += [pos[b[ff]]]
+        vals += [-c[ff]]
+        fp = fa & ~free[b]
+        np.add.at(rhs, pos[a[fp]], c[fp, None] * pin_val[b[fp]])
+    L = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(len(fid),) * 2)
+    return L, rhs, fid
+
+
+def test_device_diff_batch_128_charts_match_lu_gradient(orc):
+    """Four 128 x 128 height-field charts with positions rounded to FP32 (the differentiable workload's charts) in
+    the device differentiable batch (device AMG, mixed precision, tol 1e-12): uv against SuperLU, and dJ/dp for
+    J = E + |uv - target|^2 against the CPU-LU gradient -- SuperLU for the forward and the adjoint solve (L^T = L)
+    and the oracle's gradient through the assembly (orc_lscm_grad_scatter)."""
+    import scipy.sparse.linalg as spla
+
+    from researchuv_amd import diff, lscm, meshgen
+    from researchuv_amd.solver import Precond
+
+    charts, p3s = [], []
+    for k in range(4):
+        pos, faces = meshgen.heightfield(128, 128, seed=7000 + k)
+        pos = pos.astype(np.float32).astype(np.float64)
+        pw, fw = lscm.weld(pos, faces, 1e-12)
+        seg = lscm.segment(pw, fw, 180.0)
+        ch = seg.chart(0)
+        sy = seg.system(0)
+        charts.append((len(ch.vertex_ids), ch.tris, sy["pin_ids"], sy["pin_uv"]))
+        p3s.append(pw[ch.vertex_ids])
+    P = np.concatenate(p3s)
+    say(f"4 charts of 128^2 (FP32-rounded positions): {len(P)} vertices")
+    target = np.random.default_rng(11).normal(size=(len(P), 2)) * 0.1
+    db = diff.DiffBatch(P, charts, precond=Precond.AMG, mixed=True, tol=1e-12)
+    uv, e = db.forward(P)
+    g = db.vjp(P, uv, 2.0 * (uv - target), energy_weight=1.0)
+    db.close()
+    uvs, gs, ts = db.split(uv), db.split(g), db.split(target)
+    worst_uv, worst_g = 0.0, 0.0
+    for c, (nv, tris, pin_ids, pin_uv) in enumerate(charts):
+        Pc = p3s[c]
+        L, rhs, fid = _frozen_system(Pc, tris, pin_ids, pin_uv)
+        lu = spla.splu(L.tocsc())
+        uref = np.zeros((nv, 2))
+        uref[np.asarray(pin_ids)] = pin_uv
+        uref[fid] = lu.solve(rhs)
+        lam = np.zeros((nv, 2))
+        lam[fid] = lu.solve(np.ascontiguousarray(2.0 * (uref - ts[c])[fid]))
+        gref = orc.lscm_grad_scatter(Pc, tris, uref, lam, energy_weight=1.0)
+        eu = np.linalg.norm(uvs[c] - uref, axis=1).max() / np.linalg.norm(uref, axis=1).max()
+        eg = np.abs(gs[c] - gref).max() / np.abs(gref).max()
+        say(f"  chart {c}: uv err {eu:.2e}, grad rel diff {eg:.2e}")
+        worst_uv, worst_g = max(worst_uv, eu), max(worst_g, eg)
+    assert worst_uv < 1e-9
+    assert worst_g < 1e-8
+
+
+def test_numpy_lu_gradient_equals_oracle_vjp(orc):
+    """The SuperLU + scatter gradient used above equals the oracle's all-LU VJP on a small chart."""
+    p3, tris, pin_ids, pin_uv = _chart(orc, n=12, seed=31)
+    import scipy.sparse.linalg as spla
+
+    rng = np.random.default_rng(1)
+    target = rng.normal(size=(len(p3), 2))
+    L, rhs, fid = _frozen_system(p3, tris, pin_ids, pin_uv)
+    lu = spla.splu(L.tocsc())
+    u = np.zeros((len(p3), 2))
+    u[np.asarray(pin_ids)] = pin_uv
+    u[fid] = lu.solve(rhs)
+    lam = np.zeros_like(u)
+    lam[fid] = lu.solve(np.ascontiguousarray(2.0 * (u - target)[fid]))
+    g1 = orc.lscm_grad_scatter(p3, tris, u, lam, 1.0)
+    u2, g2 = orc.lscm_vjp_frozen(p3, tris, pin_ids, pin_uv, 2.0 * (u - target), 1.0)
+    assert np.abs(u2 - u).max() <= 1e-12 * np.abs(u).max()
+    assert np.abs(g1 - g2).max() <= 1e-10 * np.abs(g2).max()

@@ -496,6 +496,98 @@ def test_frozen_pin_loss_closed_form_on_affine_maps(orc):
             np.testing.assert_allclose(loss, e_exact + np.sum(delta * delta), rtol=1e-12)
 
 
+def _vjp_case(n=9, seed=3):
+    """A jittered height-field grid with its border pinned at the jittered xy (a frozen-pin chart)."""
+    pos, faces = meshes_hf(n, seed)
+    side = int(round(np.sqrt(len(pos))))
+    ij = np.array([(i, j) for j in range(side) for i in range(side)])
+    border = np.flatnonzero((ij[:, 0] == 0) | (ij[:, 0] == side - 1) | (ij[:, 1] == 0) |
+                            (ij[:, 1] == side - 1)).astype(np.int32)
+    return pos, faces, border, np.ascontiguousarray(pos[border, :2] * np.array([1.1, 0.9]))
+
+
+def meshes_hf(n, seed):
+    from researchuv_amd import meshgen
+
+    return meshgen.heightfield(n, n, seed=seed)
+
+
+def test_frozen_pin_vjp_matches_central_differences(orc):
+    """The LU-based VJP (two solves with the same matrix, then the gradient through the cotangent weights and edge
+    lengths) against FP64 central differences (h = 1e-6) of the frozen-pin loss J = E + |uv - target|^2,
+    on every coordinate of a jittered, non-planar 9 x 9 chart."""
+    p3, tris, pins, pin_uv = _vjp_case()
+    rng = np.random.default_rng(2)
+    target = rng.normal(size=(len(p3), 2)) * 0.2
+    loss, uv = orc.lscm_loss_frozen(p3, tris, pins, pin_uv, target)
+    uv2, grad = orc.lscm_vjp_frozen(p3, tris, pins, pin_uv, 2.0 * (uv - target), energy_weight=1.0)
+    np.testing.assert_array_equal(uv2, uv)
+    h = 1e-6
+    fd = np.zeros(p3.size)
+    for k in range(p3.size):
+        pp, pm = p3.copy().reshape(-1), p3.copy().reshape(-1)
+        pp[k] += h
+        pm[k] -= h
+        fd[k] = (orc.lscm_loss_frozen(pp.reshape(-1, 3), tris, pins, pin_uv, target)[0] -
+                 orc.lscm_loss_frozen(pm.reshape(-1, 3), tris, pins, pin_uv, target)[0]) / (2 * h)
+    err = np.abs(fd - grad.reshape(-1)).max() / np.abs(grad).max()
+    assert err < 1e-7, err
+    assert np.abs(grad[pins]).max() > 0.0  # pinned vertices move the weights too
+
+
+def test_frozen_pin_vjp_invariants(orc):
+    """J = w E + <g, uv> does not change under a rigid motion of the 3-D positions, and uv does not change under
+    a scaling while E scales as 1 / s^2 (c_e = w_e / |e|^2 with w_e scale-free), so the gradient satisfies
+    sum_v grad_v = 0, sum_v p_v x grad_v = 0 and sum_v p_v . grad_v = -2 w E.  A wrong cross-product order in the
+    derivative of the cotangent, a lost length term or a sign error in the adjoint term fails one of these."""
+    p3, tris, pins, pin_uv = _vjp_case(n=11, seed=5)
+    rng = np.random.default_rng(4)
+    g = rng.normal(size=(len(p3), 2))
+    for w in (0.0, 1.0, 2.5):
+        uv, grad = orc.lscm_vjp_frozen(p3, tris, pins, pin_uv, g, energy_weight=w)
+        scale = np.abs(grad).max() * np.abs(p3).max() * len(p3)
+        np.testing.assert_allclose(grad.sum(axis=0), 0.0, atol=1e-12 * scale)
+        np.testing.assert_allclose(np.cross(p3, grad).sum(axis=0), 0.0, atol=1e-12 * scale)
+        e = orc.lscm_loss_frozen(p3, tris, pins, pin_uv, uv)[0]  # target = uv: the loss is the energy alone
+        assert abs(float((p3 * grad).sum()) + 2.0 * w * e) <= 1e-10 * max(scale, 1.0)
+
+
+def test_grad_scatter_with_zero_adjoint_is_the_energy_gradient(orc):
+    """With lam = 0 the scatter gives dE/dp at fixed uv; central differences of E(uv fixed, p) agree."""
+    p3, tris, pins, pin_uv = _vjp_case(n=7, seed=8)
+    _, uv = orc.lscm_loss_frozen(p3, tris, pins, pin_uv, np.zeros((len(p3), 2)))
+    grad = orc.lscm_grad_scatter(p3, tris, uv, np.zeros_like(uv), energy_weight=1.0)
+
+    def energy(P):  # E at fixed uv: lscm_loss_frozen's energy term is sum_e c_e(P) |uv_i - uv_j|^2
+        tot = 0.0
+        for (i, j), c in _cot_weights(P, tris).items():
+            tot += c * float(((uv[i] - uv[j]) ** 2).sum())
+        return tot
+
+    h = 1e-6
+    worst = 0.0
+    for k in range(0, p3.size, 3):
+        pp, pm = p3.copy().reshape(-1), p3.copy().reshape(-1)
+        pp[k] += h
+        pm[k] -= h
+        fd = (energy(pp.reshape(-1, 3)) - energy(pm.reshape(-1, 3))) / (2 * h)
+        worst = max(worst, abs(fd - grad.reshape(-1)[k]) / np.abs(grad).max())
+    assert worst < 1e-7
+
+
+def _cot_weights(P, tris):
+    """c_e = sum of max(0, cot) of the angles opposite e over |e|^2 (lscm.rs:252-274,610), numpy."""
+    w = {}
+    for t in tris:
+        for q in range(3):
+            e0, e1, o = int(t[q]), int(t[(q + 1) % 3]), int(t[(q + 2) % 3])
+            a1, a2 = P[e1] - P[o], P[e0] - P[o]
+            cot = float(a1 @ a2) / np.linalg.norm(np.cross(a1, a2))
+            key = (min(e0, e1), max(e0, e1))
+            w[key] = w.get(key, 0.0) + max(cot, 0.0)
+    return {k: v / float(((P[k[0]] - P[k[1]]) ** 2).sum()) for k, v in w.items() if v > 0.0}
+
+
 def test_cpu_cg_1d_laplacian(orc):
     v, c, p, b = meshes.laplacian_1d(200)
     x = orc.cg_solve(v, c, p, b, 2000)
