@@ -103,6 +103,35 @@ int ruv_cg_solve_mixed(const double* vals, const int32_t* cols, const int32_t* r
                        double* x_out, ruv_solve_info* info);
 
 /* ---------------------------------------------------------------------------------------------------------
+ * AMG setup inspection (test hooks; the reference checks its setup in amg.rs:799-942).  The plan path builds
+ * the reference's hierarchy on the host (amg.rs:127-455: matched aggregation, tentative or smoothed
+ * prolongator, Galerkin operators, exactly factored coarsest level); these return it level by level.
+ * ------------------------------------------------------------------------------------------------------- */
+typedef struct ruv_amg_host ruv_amg_host;
+/* aggregate (amg.rs:254-291): agg (n) row -> aggregate, *n_coarse aggregates */
+int ruv_amg_aggregate(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                      int32_t max_members, int32_t* agg, int32_t* n_coarse);
+/* prolongator (amg.rs:308-362): the tentative (smoothed = 0) or smoothed P = (I - 2/3 D^-1 A) B, truncated to 32
+ * entries a row; sizes (2): rows, entries; arrays null: sizes only. */
+int ruv_amg_prolongator(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                        const int32_t* agg, int32_t n_coarse, int32_t smoothed, int64_t* sizes, double* p_vals,
+                        int32_t* p_cols, int32_t* p_ptr);
+/* galerkin_p (amg.rs:384-431): A_c = P^T A P; sizes (2) as above. */
+int ruv_amg_galerkin(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                     const double* p_vals, const int32_t* p_cols, const int32_t* p_ptr, int32_t n_coarse,
+                     int64_t* sizes, double* c_vals, int32_t* c_cols, int32_t* c_ptr);
+/* hierarchy (amg.rs:127-164); smoothed = 0: Amg (pairs, tentative P), 1: AmgSa (SA_AGG members, smoothed P) */
+int ruv_amg_host_build(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                       int32_t smoothed, ruv_amg_host** out);
+int32_t ruv_amg_host_levels(const ruv_amg_host* h);
+/* sizes (5): rows, nnz of the level operator, nnz of P (0 on the last level), coarse rows, 1 when the level is
+ * factored exactly (the last).  Any array may be null: op (CSR, rows + 1 pointers), P (rows x coarse rows, CSR),
+ * agg (rows; absent on the last level). */
+int ruv_amg_host_level(const ruv_amg_host* h, int32_t level, int64_t* sizes, double* op_vals, int32_t* op_cols,
+                       int32_t* op_ptr, double* p_vals, int32_t* p_cols, int32_t* p_ptr, int32_t* agg);
+void ruv_amg_host_destroy(ruv_amg_host* h);
+
+/* ---------------------------------------------------------------------------------------------------------
  * Device-resident plans: upload and set up once, solve many times on device buffers.  A plan holds the
  * system L (x) I_R for R = 1 (generic CSR) or R = 2 (the LSCM free block: L is the cotan Laplacian on the
  * free vertices, vectors interleaved (u0,v0,u1,v1,...) as in lscm.rs:616-629).  mode 0 = plain FP64 PCG,
@@ -186,6 +215,13 @@ int ruv_batch_precond_fallbacks(ruv_batch* batch, int64_t* not_spd_charts, int64
  * values) and the FP64 inverse the device factorisation produced for it; *rows receives the block size (call with
  * null block and inverse to size the buffers).  RUV_ERR_PRECOND for a chart kept on its diagonal. */
 int ruv_batch_coarsest_block(ruv_batch* batch, int32_t chart, int64_t* rows, double* block, double* inverse);
+/* precond amg, inspection hook: level `level` of the device hierarchy in its stored order.  sizes (3): rows, stored
+ * operator entries (padding dropped; 0 for level 0, whose operator is the batch's own), rows of the next level
+ * (0 on the coarsest).  Any array may be null: the operator as CSR (row_ptr rows + 1, cols, vals: the values the
+ * cycle applies), agg (rows: each row's aggregate, a row of the next level; absent on the coarsest) and chart
+ * (rows: each row's chart). */
+int ruv_batch_amg_level(ruv_batch* batch, int32_t level, int64_t* sizes, int64_t* row_ptr, int32_t* cols,
+                        double* vals, int32_t* agg, int32_t* chart);
 int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes,
                            double* stored_bytes);
 void ruv_batch_destroy(ruv_batch* batch);

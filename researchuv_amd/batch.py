@@ -40,6 +40,8 @@ def _setup(L):
     L.ruv_batch_precond_fallbacks.argtypes = [P, P, P, P]
     L.ruv_batch_coarsest_block.restype = C.c_int
     L.ruv_batch_coarsest_block.argtypes = [P, C.c_int32, P, P, P]
+    L.ruv_batch_amg_level.restype = C.c_int
+    L.ruv_batch_amg_level.argtypes = [P, C.c_int32, P, P, P, P, P, P]
     L.ruv_batch_setup_precond.argtypes = [P]
     L.ruv_batch_precond_info.argtypes = [P, P, P, P]
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
@@ -170,6 +172,21 @@ class Batch:
         if m.value:
             _lib.check(lib().ruv_batch_coarsest_block(self.h, chart, C.byref(m), _p(blk), _p(inv)))
         return blk, inv
+
+    def amg_level(self, level: int) -> dict:
+        """AMG: level `level` of the device hierarchy in its stored order: op (scipy CSR, None for level 0), agg
+        (row -> next-level row; None on the coarsest), chart (row -> chart), rows of the next level."""
+        import scipy.sparse as sp
+
+        sz = np.zeros(3, np.int64)
+        _lib.check(lib().ruv_batch_amg_level(self.h, level, _p(sz), None, None, None, None, None))
+        n, nz, nn = (int(v) for v in sz)
+        ptr, cols, vals = np.zeros(n + 1, np.int64), np.zeros(max(nz, 1), np.int32), np.zeros(max(nz, 1))
+        agg = np.zeros(n, np.int32) if nn else None
+        chart = np.zeros(n, np.int32)
+        _lib.check(lib().ruv_batch_amg_level(self.h, level, _p(sz), _p(ptr), _p(cols), _p(vals), _p(agg), _p(chart)))
+        op = sp.csr_matrix((vals[:nz], cols[:nz], ptr), shape=(n, n)) if level > 0 else None
+        return dict(op=op, agg=agg, chart=chart, n_next=nn)
 
     def kernel_timing(self, enable=True):
         """Bracket every step-kernel launch of the following solves with HIP events."""

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <memory>
 #include <string>
 
 #include "host_lscm.hpp"
@@ -29,6 +30,8 @@ int batch_setup_precond(BatchImpl* Q);
 int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap);
 int batch_precond_fallbacks(BatchImpl* Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large);
 int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, double* inverse);
+int batch_amg_level(BatchImpl* Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
+                    int32_t* chart);
 int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes, double* stored);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
@@ -43,6 +46,10 @@ struct ruv_batch {
 
 struct ruv_plan {
     ruv::PlanImpl* impl;
+};
+
+struct ruv_amg_host {
+    std::vector<ruv::AmgHostLevel> levels;
 };
 
 namespace {
@@ -105,6 +112,96 @@ int ruv_cg_solve(const double* vals, const int32_t* cols, const int32_t* row_ptr
                  const double* b, int64_t b_len, uint64_t max_iter, double tol, double* x_out) {
     return ruv_cg_solve_precond(vals, cols, row_ptr, n, nnz, b, b_len, max_iter, tol, RUV_PC_NONE, x_out, nullptr);
 }
+
+int ruv_amg_aggregate(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                      int32_t max_members, int32_t* agg, int32_t* n_coarse) {
+    if (!agg || !n_coarse || max_members < 1) return ruv::fail(RUV_ERR_INVALID, "null argument or max_members < 1");
+    ruv::Csr a;
+    int rc = take_csr(vals, cols, row_ptr, n, nnz, &a);
+    if (rc) return rc;
+    std::vector<int32_t> g;
+    ruv::amg_aggregate(a, max_members, &g, n_coarse);
+    std::copy(g.begin(), g.end(), agg);
+    return RUV_OK;
+}
+
+// out CSR of a host matrix into caller arrays (sizes[0] rows, sizes[1] entries; arrays null: sizes only)
+static int put_csr(const ruv::Csr& m, int64_t* sizes, double* vals, int32_t* cols, int32_t* ptr) {
+    sizes[0] = m.n;
+    sizes[1] = m.nnz();
+    if (vals) std::copy(m.vals.begin(), m.vals.end(), vals);
+    if (cols) std::copy(m.cols.begin(), m.cols.end(), cols);
+    if (ptr) std::copy(m.ptr.begin(), m.ptr.end(), ptr);
+    return RUV_OK;
+}
+
+int ruv_amg_prolongator(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                        const int32_t* agg, int32_t n_coarse, int32_t smoothed, int64_t* sizes, double* p_vals,
+                        int32_t* p_cols, int32_t* p_ptr) {
+    if (!agg || !sizes) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    ruv::Csr a;
+    int rc = take_csr(vals, cols, row_ptr, n, nnz, &a);
+    if (rc) return rc;
+    for (int32_t i = 0; i < n; ++i)
+        if (agg[i] < 0 || agg[i] >= n_coarse) return ruv::fail(RUV_ERR_BAD_DIMS, "aggregate out of range");
+    const ruv::Csr p = ruv::amg_prolongator(a, std::vector<int32_t>(agg, agg + n), smoothed != 0);
+    return put_csr(p, sizes, p_vals, p_cols, p_ptr);
+}
+
+int ruv_amg_galerkin(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                     const double* p_vals, const int32_t* p_cols, const int32_t* p_ptr, int32_t n_coarse,
+                     int64_t* sizes, double* c_vals, int32_t* c_cols, int32_t* c_ptr) {
+    if (!p_vals || !p_cols || !p_ptr || !sizes) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    ruv::Csr a;
+    int rc = take_csr(vals, cols, row_ptr, n, nnz, &a);
+    if (rc) return rc;
+    ruv::Csr p;
+    p.n = n;
+    p.ptr.assign(p_ptr, p_ptr + n + 1);
+    p.cols.assign(p_cols, p_cols + p_ptr[n]);
+    p.vals.assign(p_vals, p_vals + p_ptr[n]);
+    for (int32_t c : p.cols)
+        if (c < 0 || c >= n_coarse) return ruv::fail(RUV_ERR_BAD_DIMS, "P column out of range");
+    const ruv::Csr ac = ruv::amg_galerkin(a, p, n_coarse);
+    return put_csr(ac, sizes, c_vals, c_cols, c_ptr);
+}
+
+int ruv_amg_host_build(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                       int32_t smoothed, ruv_amg_host** out) {
+    if (!out) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    ruv::Csr a;
+    int rc = take_csr(vals, cols, row_ptr, n, nnz, &a);
+    if (rc) return rc;
+    std::unique_ptr<ruv_amg_host> h(new ruv_amg_host());
+    std::string why;
+    if (!ruv::amg_hierarchy(a, smoothed != 0, &h->levels, &why)) return ruv::fail(RUV_ERR_PRECOND, "amg: " + why);
+    *out = h.release();
+    return RUV_OK;
+}
+
+int32_t ruv_amg_host_levels(const ruv_amg_host* h) { return h ? (int32_t)h->levels.size() : 0; }
+
+int ruv_amg_host_level(const ruv_amg_host* h, int32_t level, int64_t* sizes, double* op_vals, int32_t* op_cols,
+                       int32_t* op_ptr, double* p_vals, int32_t* p_cols, int32_t* p_ptr, int32_t* agg) {
+    if (!h || !sizes) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    if (level < 0 || level >= (int32_t)h->levels.size()) return ruv::fail(RUV_ERR_INVALID, "level out of range");
+    const ruv::AmgHostLevel& L = h->levels[(size_t)level];
+    sizes[0] = L.op.n;
+    sizes[1] = L.op.nnz();
+    sizes[2] = L.p.nnz();
+    sizes[3] = L.n_coarse;
+    sizes[4] = L.has_factor ? 1 : 0;
+    if (op_vals) std::copy(L.op.vals.begin(), L.op.vals.end(), op_vals);
+    if (op_cols) std::copy(L.op.cols.begin(), L.op.cols.end(), op_cols);
+    if (op_ptr) std::copy(L.op.ptr.begin(), L.op.ptr.end(), op_ptr);
+    if (p_vals) std::copy(L.p.vals.begin(), L.p.vals.end(), p_vals);
+    if (p_cols) std::copy(L.p.cols.begin(), L.p.cols.end(), p_cols);
+    if (p_ptr && !L.p.ptr.empty()) std::copy(L.p.ptr.begin(), L.p.ptr.end(), p_ptr);
+    if (agg) std::copy(L.agg.begin(), L.agg.end(), agg);
+    return RUV_OK;
+}
+
+void ruv_amg_host_destroy(ruv_amg_host* h) { delete h; }
 
 int ruv_cg_solve_mixed(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
                        const double* b, int64_t b_len, uint64_t max_iter, double tol, int32_t precond,
@@ -355,6 +452,12 @@ int ruv_batch_precond_fallbacks(ruv_batch* batch, int64_t* not_spd_charts, int64
 int ruv_batch_coarsest_block(ruv_batch* batch, int32_t chart, int64_t* rows, double* block, double* inverse) {
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_coarsest_block(batch->impl, chart, rows, block, inverse);
+}
+
+int ruv_batch_amg_level(ruv_batch* batch, int32_t level, int64_t* sizes, int64_t* row_ptr, int32_t* cols,
+                        double* vals, int32_t* agg, int32_t* chart) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_amg_level(batch->impl, level, sizes, row_ptr, cols, vals, agg, chart);
 }
 
 int ruv_batch_precond_levels(ruv_batch* batch, int64_t* rows, int32_t cap) {

@@ -42,6 +42,8 @@ int bamg_info(const BatchImpl& Q, int* levels, double* setup_seconds, int64_t* c
 int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap);
 int bamg_fallbacks(BatchImpl& Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large);
 int bamg_coarsest_block(BatchImpl& Q, int c, int64_t* m_out, double* block, double* inverse);
+int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
+               int32_t* chart);
 
 namespace {
 
@@ -738,6 +740,11 @@ int batch_precond_fallbacks(BatchImpl* Q, int64_t* not_spd, int64_t* coupled, in
 
 int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, double* inverse) {
     return bamg_coarsest_block(*Q, c, m_out, block, inverse);
+}
+
+int batch_amg_level(BatchImpl* Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
+                    int32_t* chart) {
+    return bamg_level(*Q, l, sizes, row_ptr, cols, vals, agg, chart);
 }
 
 int batch_setup_precond(BatchImpl* Q) { return Q->precond == RUV_PC_AMG ? bamg_build(*Q) : RUV_OK; }

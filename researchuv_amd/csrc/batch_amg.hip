@@ -847,6 +847,89 @@ int bamg_coarsest_block(BatchImpl& Q, int c, int64_t* m_out, double* block, doub
     return RUV_OK;
 }
 
+// Level l of the hierarchy in its stored order (the inspection hook): sizes, the operator as CSR from the sliced
+// ELL (levels between the finest and the coarsest) or the kept coarsest CSR, the aggregate map and row charts.
+int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
+               int32_t* chart) {
+    if (!Q.amg) return fail(RUV_ERR_INVALID, "batch amg: no hierarchy");
+    Bamg& A = *Q.amg;
+    const int nl = (int)A.lv.size();
+    if (l < 0 || l >= nl) return fail(RUV_ERR_INVALID, "batch amg: level out of range");
+    const BamgLevel& L = A.lv[(size_t)l];
+    hipStream_t s = Q.stream;
+    const int64_t n = L.n;
+    std::vector<int64_t> ptr(1, 0);
+    std::vector<int32_t> cc;
+    std::vector<double> vv;
+    const bool f32 = Q.mode == RUV_MODE_MIXED;
+    if (l == nl - 1) {  // the coarsest: the setup CSR (FP32 values)
+        std::vector<int64_t> p((size_t)n + 1);
+        HIPCHK(hipMemcpyAsync(p.data(), A.coarse.ptr, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        const int64_t nz = p[n];
+        cc.resize((size_t)std::max<int64_t>(nz, 1));
+        std::vector<float> fv((size_t)std::max<int64_t>(nz, 1));
+        HIPCHK(hipMemcpyAsync(cc.data(), A.coarse.cols, sizeof(int32_t) * nz, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(fv.data(), A.coarse.vals, sizeof(float) * nz, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        cc.resize((size_t)nz);
+        vv.assign(fv.begin(), fv.begin() + nz);
+        ptr = p;
+    } else if (l > 0) {  // sliced ELL, padding (0, the row itself) dropped
+        const int64_t nsl = L.nsl, ne = L.nell;
+        std::vector<int64_t> eoff((size_t)nsl + 1);
+        std::vector<int32_t> ew((size_t)std::max<int64_t>(nsl, 1)), srow((size_t)nsl + 1);
+        HIPCHK(hipMemcpyAsync(eoff.data(), L.eoff, sizeof(int64_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(ew.data(), L.ew, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(srow.data(), L.srow, sizeof(int32_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
+        std::vector<int32_t> c32(L.ci ? (size_t)std::max<int64_t>(ne, 1) : 0);
+        std::vector<int16_t> c16(L.ci ? 0 : (size_t)std::max<int64_t>(ne, 1));
+        if (L.ci) HIPCHK(hipMemcpyAsync(c32.data(), L.ecols, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, s));
+        else HIPCHK(hipMemcpyAsync(c16.data(), L.ecols, sizeof(int16_t) * ne, hipMemcpyDeviceToHost, s));
+        std::vector<uint16_t> b16(f32 ? (size_t)std::max<int64_t>(ne, 1) : 0);
+        std::vector<double> d64(f32 ? 0 : (size_t)std::max<int64_t>(ne, 1));
+        if (f32) HIPCHK(hipMemcpyAsync(b16.data(), L.evals, sizeof(uint16_t) * ne, hipMemcpyDeviceToHost, s));
+        else HIPCHK(hipMemcpyAsync(d64.data(), L.evals, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int64_t b = 0; b < nsl; ++b) {
+            const int64_t r0 = srow[b], ns = srow[b + 1] - r0;
+            for (int64_t p = r0; p < r0 + ns; ++p) {
+                for (int k = 0; k < ew[b]; ++k) {
+                    const int64_t e = eoff[b] + (p - r0) + (int64_t)k * ns;
+                    const int64_t c = L.ci ? (int64_t)c32[e] : p + (int64_t)c16[e];
+                    double v;
+                    if (f32) {
+                        const uint32_t u = (uint32_t)b16[e] << 16;
+                        float f;
+                        std::memcpy(&f, &u, sizeof(f));
+                        v = f;
+                    } else {
+                        v = d64[e];
+                    }
+                    if (c == p && v == 0.0) continue;  // padding
+                    cc.push_back((int32_t)c);
+                    vv.push_back(v);
+                }
+                ptr.push_back((int64_t)cc.size());
+            }
+        }
+    } else {
+        ptr.assign((size_t)n + 1, 0);
+    }
+    if (sizes) {
+        sizes[0] = n;
+        sizes[1] = (int64_t)cc.size();
+        sizes[2] = l + 1 < nl ? A.lv[(size_t)l + 1].n : 0;
+    }
+    if (row_ptr && l > 0) std::copy(ptr.begin(), ptr.end(), row_ptr);
+    if (cols) std::copy(cc.begin(), cc.end(), cols);
+    if (vals) std::copy(vv.begin(), vv.end(), vals);
+    if (agg && l + 1 < nl) HIPCHK(hipMemcpyAsync(agg, L.agg, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    if (chart) HIPCHK(hipMemcpyAsync(chart, L.chart, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
 int bamg_level_rows(const BatchImpl& Q, int64_t* rows, int cap) {
     if (!Q.amg) return 0;
     const int nl = (int)Q.amg->lv.size();

@@ -247,6 +247,7 @@ bool amg_hierarchy(const Csr& a, bool smooth, std::vector<AmgHostLevel>* levels,
         L.p = amg_prolongator(cur, agg, smooth);
         L.r = transpose_rect(L.p, nc);
         L.n_coarse = nc;
+        L.agg = agg;
         Csr next = amg_galerkin(cur, L.p, nc);
         levels->push_back(std::move(L));
         cur = std::move(next);

@@ -52,6 +52,7 @@ struct AmgHostLevel {
     Csr p;         // prolongator (n x n_coarse), empty on the last level
     Csr r;         // restriction R = P^T
     int32_t n_coarse = 0;
+    std::vector<int32_t> agg;  // row -> aggregate (the next level's row), empty on the last level
     bool diag_only = false;
     bool has_factor = false;
     // coarsest level (n <= 512): the dense inverse A^-1 (row-major n x n) from a dense Cholesky, in place of

@@ -174,3 +174,108 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class AmgHost:
+    """The plan path's host AMG hierarchy (amg.rs:127-455), level by level (an inspection hook for tests)."""
+
+    def __init__(self, a: CsrMatrix, smoothed: bool):
+        L = _lib.lib()
+        P = C.c_void_p
+        L.ruv_amg_host_build.restype = C.c_int
+        L.ruv_amg_host_build.argtypes = [P, P, P, C.c_int32, C.c_int64, C.c_int32, P]
+        L.ruv_amg_host_levels.restype = C.c_int32
+        L.ruv_amg_host_levels.argtypes = [P]
+        L.ruv_amg_host_level.restype = C.c_int
+        L.ruv_amg_host_level.argtypes = [P, C.c_int32, P, P, P, P, P, P, P, P]
+        L.ruv_amg_host_destroy.restype = None
+        L.ruv_amg_host_destroy.argtypes = [P]
+        self._L = L
+        self.h = C.c_void_p()
+        _lib.check(L.ruv_amg_host_build(_p(a.vals), _p(a.cols), _p(a.row_ptr), a.rows(), len(a.vals),
+                                        1 if smoothed else 0, C.byref(self.h)))
+
+    def __len__(self):
+        return int(self._L.ruv_amg_host_levels(self.h))
+
+    def level(self, k: int) -> dict:
+        """op / p as scipy CSR, agg, coarse rows, factored flag."""
+        import scipy.sparse as sp
+
+        sz = np.zeros(5, np.int64)
+        nul = None
+        _lib.check(self._L.ruv_amg_host_level(self.h, k, _p(sz), nul, nul, nul, nul, nul, nul, nul))
+        n, nnz, pnz, nc, fac = (int(v) for v in sz)
+        ov, oc, op = np.zeros(nnz), np.zeros(nnz, np.int32), np.zeros(n + 1, np.int32)
+        pv, pc, pp = np.zeros(pnz), np.zeros(pnz, np.int32), np.zeros(n + 1, np.int32)
+        agg = np.zeros(n if nc else 0, np.int32)
+        _lib.check(self._L.ruv_amg_host_level(self.h, k, _p(sz), _p(ov), _p(oc), _p(op), _p(pv), _p(pc), _p(pp),
+                                              _p(agg) if nc else None))
+        out = dict(op=sp.csr_matrix((ov, oc, op), shape=(n, n)), n_coarse=nc, factored=bool(fac), agg=agg)
+        out["p"] = sp.csr_matrix((pv, pc, pp), shape=(n, nc)) if nc else None
+        return out
+
+    def close(self):
+        if self.h:
+            self._L.ruv_amg_host_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def amg_aggregate(a: CsrMatrix, max_members: int):
+    """aggregate (amg.rs:254-291) -> (agg, n_coarse)"""
+    L = _lib.lib()
+    L.ruv_amg_aggregate.restype = C.c_int
+    L.ruv_amg_aggregate.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
+    agg = np.zeros(a.rows(), np.int32)
+    nc = C.c_int32(0)
+    _lib.check(L.ruv_amg_aggregate(_p(a.vals), _p(a.cols), _p(a.row_ptr), a.rows(), len(a.vals), max_members,
+                                   _p(agg), C.byref(nc)))
+    return agg, int(nc.value)
+
+
+def _csr_call(fn, args):
+    """Two-call protocol of the host CSR hooks: sizes first, then the arrays -> scipy CSR."""
+    import scipy.sparse as sp
+
+    sz = np.zeros(2, np.int64)
+    _lib.check(fn(*args, _p(sz), None, None, None))
+    n, nnz = int(sz[0]), int(sz[1])
+    v, c, p = np.zeros(nnz), np.zeros(nnz, np.int32), np.zeros(n + 1, np.int32)
+    _lib.check(fn(*args, _p(sz), _p(v), _p(c), _p(p)))
+    return v, c, p, n
+
+
+def amg_prolongator(a: CsrMatrix, agg, n_coarse: int, smoothed: bool):
+    """prolongator (amg.rs:308-362) -> scipy CSR (rows x n_coarse)"""
+    import scipy.sparse as sp
+
+    L = _lib.lib()
+    L.ruv_amg_prolongator.restype = C.c_int
+    L.ruv_amg_prolongator.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_int32] + \
+        [C.c_void_p] * 4
+    agg = np.ascontiguousarray(agg, np.int32)
+    v, c, p, n = _csr_call(L.ruv_amg_prolongator, (_p(a.vals), _p(a.cols), _p(a.row_ptr), a.rows(), len(a.vals),
+                                                   _p(agg), n_coarse, 1 if smoothed else 0))
+    return sp.csr_matrix((v, c, p), shape=(n, n_coarse))
+
+
+def amg_galerkin(a: CsrMatrix, p_csr, n_coarse: int):
+    """galerkin_p (amg.rs:384-431) -> scipy CSR (n_coarse x n_coarse)"""
+    import scipy.sparse as sp
+
+    L = _lib.lib()
+    L.ruv_amg_galerkin.restype = C.c_int
+    L.ruv_amg_galerkin.argtypes = [C.c_void_p] * 3 + [C.c_int32, C.c_int64] + [C.c_void_p] * 3 + [C.c_int32] + \
+        [C.c_void_p] * 4
+    pv = np.ascontiguousarray(p_csr.data, np.float64)
+    pc = np.ascontiguousarray(p_csr.indices, np.int32)
+    pp = np.ascontiguousarray(p_csr.indptr, np.int32)
+    v, c, p, n = _csr_call(L.ruv_amg_galerkin, (_p(a.vals), _p(a.cols), _p(a.row_ptr), a.rows(), len(a.vals),
+                                                _p(pv), _p(pc), _p(pp), n_coarse))
+    return sp.csr_matrix((v, c, p), shape=(n, n))

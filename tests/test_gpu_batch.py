@@ -272,6 +272,55 @@ def test_batch_amg_coarsest_inverse_on_device(mixed):
     assert worst < 1e-11
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_amg_device_hierarchy_invariants(mixed):
+    """The device-built hierarchy level by level (its stored order): every row in exactly one aggregate, of its own
+    chart, and every aggregate used; each coarse operator the Galerkin product P^T A P of the level above with the
+    aggregate map as P (the tentative transfer of amg.rs:308-362, product of amg.rs:384-431); the entry sum kept;
+    every level symmetric with a positive diagonal; the coarsest at most 64 rows a chart (the checks of
+    amg.rs:799-942, applied to the device hierarchy).  Values as the cycle stores them: FP32-computed, held in
+    FP64 (mixed=False) or bfloat16 (mixed=True), hence the tolerances."""
+    import scipy.sparse as sp
+
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    meshes = _meshes(24, seed=12)
+    lb = B.LscmBatchBuilder(meshes, angle_min_deg=180.0, threads=4)
+    a = lb.arrays(csr=True)
+    n0 = lb.nrows
+    bt = lb.device_batch(precond=Precond.AMG, mixed=mixed)
+    nl = bt.precond_info()["levels"]
+    say(f"device hierarchy mixed={mixed}: {bt.precond_info()['level_rows']}")
+    assert nl >= 3
+    rtol = 4e-3 if mixed else 2e-6
+    op = sp.csr_matrix((a["vals"], a["cols"], a["row_ptr"]), shape=(n0, n0))
+    prev = bt.amg_level(0)
+    off = a["chart_row_off"]
+    np.testing.assert_array_equal(prev["chart"], np.repeat(np.arange(lb.nchart), np.diff(off)))
+    for lvl in range(1, nl):
+        cur = bt.amg_level(lvl)
+        agg, n = prev["agg"], cur["op"].shape[0]
+        assert prev["n_next"] == n
+        assert agg.min() >= 0 and agg.max() < n
+        assert (np.bincount(agg, minlength=n) > 0).all()
+        np.testing.assert_array_equal(cur["chart"][agg], prev["chart"])
+        P = sp.csr_matrix((np.ones(len(agg)), (np.arange(len(agg)), agg)), shape=(len(agg), n))
+        want = (P.T @ op @ P).tocsr()
+        got = cur["op"]
+        scale = abs(want).max()
+        diff = abs(got - want).max()
+        say(f"  level {lvl}: {n} rows, {got.nnz} entries, Galerkin max diff {diff / scale:.2e} of max |A|")
+        assert diff <= rtol * scale
+        assert abs(got.sum() - op.sum()) <= rtol * abs(op).sum()
+        assert (got.diagonal() > 0).all()
+        assert abs(got - got.T).max() <= rtol * scale
+        op, prev = got, cur
+    cnt = np.bincount(prev["chart"], minlength=lb.nchart)
+    assert cnt.max() <= 64
+    bt.close()
+
+
 def test_batch_amg_general_slices_and_small_systems():
     """AMG over general (per-entry column) slices: the permuted chart of the wide-column test, and the golden
     small SPD systems with one right-hand side per row."""
