@@ -313,6 +313,30 @@ int ruv_lscm_system_device(const double* mesh_pos, const ruv_charts* charts, int
 int ruv_unfold_chart(const double* mesh_pos, const ruv_charts* charts, int64_t i, const ruv_unfold_opts* opts,
                      double* uv_out, double* stats, ruv_solve_info* info);
 
+/* The unfold stage of pipeline::run (pipeline.rs:171-194) over nmesh meshes in one call: weld (weld_tol) and segment
+ * (angle_min_deg) each mesh and assemble its charts on host worker threads (threads 0 = all cores, input order as
+ * exec::par_map), solve every chart's system in one batched device solve (opts: precond none / jacobi / amg, mode,
+ * tol as ruv_batch_solve, max_iter, angle_mix, mix_w; the size gate min_vars does not apply: every chart goes to
+ * the device), run lscm::unfold_chart's driver iterations per chart (lscm.rs:602-718), and compute
+ * rectangularize and chart_distortion of every chart on the device (pipeline.rs:176-177).  The results per chart
+ * are those of the reference's stage: its uv (pins included), UnfoldResult (lscm.rs:104-116), rectified uv and
+ * extent, and ChartMetrics, whose fold count is recut::chart_folds' measure (recut.rs:71-80). */
+typedef struct ruv_unfold_batch ruv_unfold_batch;
+int ruv_unfold_batch_run(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                         const int64_t* face_off, double weld_tol, double angle_min_deg, const ruv_unfold_opts* opts,
+                         int32_t threads, ruv_unfold_batch** out);
+/* sizes (3): charts, chart vertices in all, chart triangles in all */
+int ruv_unfold_batch_sizes(const ruv_unfold_batch* u, int64_t* sizes);
+/* Any pointer may be null.  Charts in (mesh, chart) order: chart_mesh (nchart); v_off (nchart + 1) into the per-
+ * vertex arrays, chart c's vertices in ascending welded-vertex order; vertex_ids (the welded mesh's vertex of each);
+ * uv and rectified (2 per vertex); ext (2 per chart); metrics (3 per chart: conformal_mean, conformal_max,
+ * area_ratio_mean); counts (2 per chart: flips, folds); stats (4 per chart: iters, err_accum, target, area3d);
+ * info: the batched solve. */
+int ruv_unfold_batch_results(const ruv_unfold_batch* u, int32_t* chart_mesh, int64_t* v_off, uint32_t* vertex_ids,
+                             double* uv, double* rectified, double* ext, double* metrics, int64_t* counts,
+                             double* stats, ruv_solve_info* info);
+void ruv_unfold_batch_destroy(ruv_unfold_batch* u);
+
 /* ---------------------------------------------------------------------------------------------------------
  * The differentiable batch (no reference counterpart; the forward is lscm.rs:594-666 + the GPU solve, with pins
  * frozen as in ruv_lscm_forward).  nchart charts as one disjoint union: chart c owns vertices
@@ -361,7 +385,7 @@ int ruv_chart_metrics_dev(int32_t nchart, const int64_t* v_off, const double* p3
 /* ruv_chart_metrics_dev without any allocation (no counterpart in the reference; for callers that cannot
  * allocate device memory: tight serving loops, harnesses that forbid hipMalloc).  The same kernels and results
  * (bitwise) as ruv_chart_metrics_dev; its scratch (error flag, tile table, per-tile partials) is a fixed 8 MiB
- * device array held by each GPU the library runs on until the process ends, enough for about 131 000 tiles of
+ * device array held by each GPU the library runs on until the process ends, enough for about 116 000 tiles of
  * 1024 triangles (nchart + ceil(triangles / 1024) tiles); larger inputs are rejected with RUV_ERR_BAD_DIMS
  * before anything is launched.  Runs on the null stream and returns when its results are written.  Every call on
  * one GPU shares that GPU's scratch: calls that overlap on one GPU race. */

@@ -26,7 +26,8 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature"
 HOST_FLAGS = ["-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["pcg.hip", "batch.hip", "batch_amg.hip", "assemble.hip", "adjoint.hip", "diff_batch.hip", "metrics.hip"]
-HOST_SOURCES = ["host_precond.cpp", "host_amg.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp", "api.cpp"]
+HOST_SOURCES = ["host_precond.cpp", "host_amg.cpp", "host_mesh.cpp", "host_lscm.cpp", "host_batch.cpp",
+                "host_unfold_batch.cpp", "api.cpp"]
 
 
 def _compile(src: str) -> str:

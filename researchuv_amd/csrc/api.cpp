@@ -48,6 +48,22 @@ struct ruv_plan {
     ruv::PlanImpl* impl;
 };
 
+namespace ruv {
+struct UnfoldBatch;
+int unfold_batch_run(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                     const int64_t* face_off, double weld_tol, double angle_min_deg, const ruv_unfold_opts& o,
+                     int32_t threads, UnfoldBatch** out);
+void unfold_batch_destroy(UnfoldBatch* u);
+int unfold_batch_sizes(const UnfoldBatch* u, int64_t* sizes);
+int unfold_batch_results(const UnfoldBatch* u, int32_t* chart_mesh, int64_t* v_off, uint32_t* vertex_ids, double* uv,
+                         double* rectified, double* ext, double* metrics, int64_t* counts, double* stats,
+                         ruv_solve_info* info);
+}  // namespace ruv
+
+struct ruv_unfold_batch {
+    ruv::UnfoldBatch* impl;
+};
+
 struct ruv_amg_host {
     std::vector<ruv::AmgHostLevel> levels;
 };
@@ -685,6 +701,48 @@ int ruv_chart_metrics_dev_noalloc(int32_t nchart, const int64_t* v_off, const do
 int ruv_rectangularize_dev(int32_t nchart, const int64_t* v_off, const double* uv, const int64_t* b_off,
                            const int64_t* bnd, double* q, double* ext) {
     return ruv::rectangularize_dev(nchart, v_off, uv, b_off, bnd, q, ext);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int ruv_unfold_batch_run(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
+                         const int64_t* face_off, double weld_tol, double angle_min_deg, const ruv_unfold_opts* opts,
+                         int32_t threads, ruv_unfold_batch** out) {
+    if (!pos_off || !face_off || !opts || !out || (nmesh > 0 && (!pos || !faces)))
+        return ruv::fail(RUV_ERR_INVALID, "null argument");
+    for (int32_t m = 0; m < nmesh; ++m) {
+        if (pos_off[m + 1] < pos_off[m] || face_off[m + 1] < face_off[m])
+            return ruv::fail(RUV_ERR_BAD_DIMS, "unfold batch: offsets not monotone");
+        const int64_t nv = pos_off[m + 1] - pos_off[m];
+        for (int64_t k = 3 * face_off[m]; k < 3 * face_off[m + 1]; ++k)
+            if ((int64_t)faces[k] >= nv) return ruv::fail(RUV_ERR_BAD_DIMS, "unfold batch: face index outside its mesh");
+    }
+    ruv::UnfoldBatch* u = nullptr;
+    int rc = ruv::unfold_batch_run(nmesh, pos, pos_off, faces, face_off, weld_tol, angle_min_deg, *opts, threads, &u);
+    if (rc) return rc;
+    *out = new ruv_unfold_batch{u};
+    return RUV_OK;
+}
+
+int ruv_unfold_batch_sizes(const ruv_unfold_batch* u, int64_t* sizes) {
+    if (!u || !sizes) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::unfold_batch_sizes(u->impl, sizes);
+}
+
+int ruv_unfold_batch_results(const ruv_unfold_batch* u, int32_t* chart_mesh, int64_t* v_off, uint32_t* vertex_ids,
+                             double* uv, double* rectified, double* ext, double* metrics, int64_t* counts,
+                             double* stats, ruv_solve_info* info) {
+    if (!u) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::unfold_batch_results(u->impl, chart_mesh, v_off, vertex_ids, uv, rectified, ext, metrics, counts,
+                                     stats, info);
+}
+
+void ruv_unfold_batch_destroy(ruv_unfold_batch* u) {
+    if (!u) return;
+    ruv::unfold_batch_destroy(u->impl);
+    delete u;
 }
 
 }  // extern "C"

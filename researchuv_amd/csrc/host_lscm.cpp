@@ -456,40 +456,20 @@ int64_t Chart::local_of(uint32_t v) const {
     return (it != vertex_ids.end() && *it == v) ? (int64_t)(it - vertex_ids.begin()) : -1;
 }
 
-// unfold_chart (lscm.rs:512-727) with the device solve.  The default system (B = 0, no area row) does not
-// depend on uv, so every driver iteration re-assembles the same system; the device solve is bitwise
-// deterministic, so the solution of iteration 0 is reused instead of solving the identical system again.
-int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, double* uv_out, double* stats,
-                 ruv_solve_info* info_out) {
-    ChartSystem S;
-    if (o.device_assembly) {
-        int rc = device_assemble(pos, ch, o.angle_mix, &S);
-        if (rc) return rc;
-    } else {
-        build_system(pos, ch, o.angle_mix, &S);
-    }
+// The driver iterations of unfold_chart (lscm.rs:602-718) given the free-vertex solution `sol` (2 per free vertex)
+// of the chart's system: uv starts at the tangent frame, each iteration replaces it (damped) by the solution with
+// the pins at their targets, with the reference's stop rules (lscm.rs:697-718).  The default system (B = 0, no
+// area row) does not depend on uv, so every iteration re-assembles the same system, and the solution of the first
+// is the solution of all.  uv_out (nv*2); stats (4): iters, err_accum, target, area3d.
+void driver_iterations(const ChartSystem& S, const Chart& ch, const double* sol, int max_iter_opt, double* uv_out,
+                       double* stats) {
     const int64_t nv = S.nv;
     std::vector<double> uv(S.p2);
     double err_accum = 1.0, step = 1.0, prev_err = INFINITY;
     int iters_done = 0;
-    const int max_iter = o.max_iter > 0 ? o.max_iter : DEFAULT_MAX_ITER;
-    std::vector<double> sol;
-    bool have_sol = false;
-    ruv_solve_info info{};
+    const int max_iter = max_iter_opt > 0 ? max_iter_opt : DEFAULT_MAX_ITER;
     for (int it = 0; it < max_iter; ++it) {
         iters_done = it + 1;
-        if (!have_sol) {
-            // gpu_solve gates (lscm.rs:478-481): size gate, slack weight
-            if (2 * (int64_t)S.nfree < o.min_vars || o.mix_w <= 0.0)
-                return fail(RUV_ERR_BAD_DIMS, "chart below the GPU size gate (the caller runs its CPU solve)");
-            sol.assign(2 * (size_t)S.nfree, 0.0);
-            if (S.nfree > 0) {
-                const uint64_t mi = (uint64_t)std::max(max_iter, 1024) * 16;  // lscm.rs:501
-                int rc = device_solve(S.l, 2, S.rhs.data(), mi, o.tol, o.precond, o.mode, sol.data(), &info);
-                if (rc) return rc;
-            }
-            have_sol = true;
-        }
         std::vector<double> new_uv(uv);
         for (int32_t q = 0; q < S.nfree; ++q) {
             new_uv[2 * S.free_ids[q]] = sol[2 * q];
@@ -524,6 +504,31 @@ int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, d
         stats[2] = ch.area3d;
         stats[3] = ch.area3d;
     }
+}
+
+// unfold_chart (lscm.rs:512-727) with the device solve: the system of the first driver iteration solved on the
+// device, the iterations run by driver_iterations.
+int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, double* uv_out, double* stats,
+                 ruv_solve_info* info_out) {
+    ChartSystem S;
+    if (o.device_assembly) {
+        int rc = device_assemble(pos, ch, o.angle_mix, &S);
+        if (rc) return rc;
+    } else {
+        build_system(pos, ch, o.angle_mix, &S);
+    }
+    // gpu_solve gates (lscm.rs:478-481): size gate, slack weight
+    if (2 * (int64_t)S.nfree < o.min_vars || o.mix_w <= 0.0)
+        return fail(RUV_ERR_BAD_DIMS, "chart below the GPU size gate (the caller runs its CPU solve)");
+    const int max_iter = o.max_iter > 0 ? o.max_iter : DEFAULT_MAX_ITER;
+    std::vector<double> sol(2 * (size_t)S.nfree, 0.0);
+    ruv_solve_info info{};
+    if (S.nfree > 0) {
+        const uint64_t mi = (uint64_t)std::max(max_iter, 1024) * 16;  // lscm.rs:501
+        int rc = device_solve(S.l, 2, S.rhs.data(), mi, o.tol, o.precond, o.mode, sol.data(), &info);
+        if (rc) return rc;
+    }
+    driver_iterations(S, ch, sol.data(), o.max_iter, uv_out, stats);
     if (info_out) *info_out = info;
     return RUV_OK;
 }

@@ -43,5 +43,7 @@ void build_system(const double* pos, const Chart& ch, double angle_mix, ChartSys
 int device_assemble(const double* pos, const Chart& ch, double angle_mix, ChartSystem* S);
 int unfold_chart(const double* pos, const Chart& ch, const ruv_unfold_opts& o, double* uv_out, double* stats,
                  ruv_solve_info* info);
+void driver_iterations(const ChartSystem& S, const Chart& ch, const double* sol, int max_iter_opt, double* uv_out,
+                       double* stats);
 
 }  // namespace ruv

@@ -67,7 +67,7 @@ __device__ __forceinline__ long long block_count(long long v, long long* sh) {
 // ratio and its counts (measured triangles, flips, signed areas <= 0 and >= 0).
 constexpr int TILE_TRIS = 1024;
 struct TilePart {
-    double conf_sum, conf_max, area_sum, signed_sum;
+    double conf_sum, conf_max, area_sum, signed_sum, signed_abs;
     long long nconf, flips, le0, ge0;
 };
 
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BLK) void k_distortion_tiles(int32_t nchart, const 
     const int64_t v0 = v_off[c], nvc = v_off[c + 1] - v0;
     const double* P = p3 + 3 * v0;
     const double* U = uv + 2 * v0;
-    double conf_sum = 0.0, conf_max = 0.0, area_sum = 0.0, signed_sum = 0.0;
+    double conf_sum = 0.0, conf_max = 0.0, area_sum = 0.0, signed_sum = 0.0, signed_abs = 0.0;
     long long nconf = 0, flips = 0, le0 = 0, ge0 = 0;
     for (int64_t t = tt0 + threadIdx.x; t < tt1; t += BLK) {
         const uint32_t a = tris[3 * t], b = tris[3 * t + 1], cc = tris[3 * t + 2];
@@ -149,6 +149,7 @@ __global__ __launch_bounds__(BLK) void k_distortion_tiles(int32_t nchart, const 
         // winding_flips' signed area, for every triangle
         const double sa = tri_signed(u0u, u0v, u1u, u1v, u2u, u2v);
         signed_sum += sa;
+        signed_abs += fabs(sa);
         le0 += (sa <= 0.0) ? 1 : 0;
         ge0 += (sa >= 0.0) ? 1 : 0;
         const V3 a3 = ld3(P, a), b3 = ld3(P, b), c3 = ld3(P, cc);
@@ -191,6 +192,7 @@ __global__ __launch_bounds__(BLK) void k_distortion_tiles(int32_t nchart, const 
     r.conf_max = block_max(conf_max, sh);
     r.area_sum = block_sum_all(area_sum, sh);
     r.signed_sum = block_sum_all(signed_sum, sh);
+    r.signed_abs = block_sum_all(signed_abs, sh);
     r.nconf = block_count(nconf, shc);
     r.flips = block_count(flips, shc);
     r.le0 = block_count(le0, shc);
@@ -201,13 +203,22 @@ __global__ __launch_bounds__(BLK) void k_distortion_tiles(int32_t nchart, const 
 // per chart: its tiles' parts summed in tile order.  metrics (3 per chart): conformal mean, conformal max,
 // area-ratio mean; counts (2 per chart): flips, folds.  Tiles at or past ntile_cap (the launched tiles, which
 // offsets that decrease can leave short of the table; the call then reports an error) are not read.
+// The fold count needs the sign of the signed-area sum as winding_flips forms it, left to right from +0.0
+// (metrics.rs:39-56).  The tiles' sum has another rounding, but any order of summing n terms lands within
+// (n - 1) u sum|s| of the exact sum (u = 2^-53), so when the tiles' sum is farther than twice that from 0 both
+// sums have its sign; otherwise (terms that nearly cancel, as on a chart folded onto itself) the chart's thread
+// sums its triangles again in the reference's order.
 __global__ __launch_bounds__(BLK) void k_distortion_finish(int32_t nchart, const int32_t* __restrict__ chart_tile0,
                                                            int32_t ntile_cap, const TilePart* __restrict__ part,
+                                                           const int64_t* __restrict__ v_off,
+                                                           const double* __restrict__ uv,
+                                                           const int64_t* __restrict__ t_off,
+                                                           const uint32_t* __restrict__ tris,
                                                            double* __restrict__ metrics,
                                                            long long* __restrict__ counts) {
     const int c = blockIdx.x * BLK + threadIdx.x;
     if (c >= nchart) return;
-    double cs = 0.0, cm = 0.0, as = 0.0, ss = 0.0;
+    double cs = 0.0, cm = 0.0, as = 0.0, ss = 0.0, sa = 0.0;
     long long nc = 0, fl = 0, nle = 0, nge = 0;
     const int t1 = min(chart_tile0[c + 1], ntile_cap);
     for (int t = chart_tile0[c]; t < t1; ++t) {
@@ -216,10 +227,23 @@ __global__ __launch_bounds__(BLK) void k_distortion_finish(int32_t nchart, const
         cm = fmax(cm, r.conf_max);
         as += r.area_sum;
         ss += r.signed_sum;
+        sa += r.signed_abs;
         nc += r.nconf;
         fl += r.flips;
         nle += r.le0;
         nge += r.ge0;
+    }
+    const int64_t ntri = t_off[c + 1] - t_off[c];
+    if (ntri > 0 && !(fabs(ss) > 2.0 * (double)ntri * 0x1p-53 * sa)) {
+        const int64_t v0 = v_off[c], nvc = v_off[c + 1] - v0;
+        const double* U = uv + 2 * v0;
+        double seq = 0.0;
+        for (int64_t t = t_off[c]; t < t_off[c + 1]; ++t) {
+            const uint32_t a = tris[3 * t], b = tris[3 * t + 1], d = tris[3 * t + 2];
+            if ((int64_t)a >= nvc || (int64_t)b >= nvc || (int64_t)d >= nvc) continue;  // reported by the tiles
+            seq += tri_signed(U[2 * a], U[2 * a + 1], U[2 * b], U[2 * b + 1], U[2 * d], U[2 * d + 1]);
+        }
+        ss = seq;
     }
     const double den = (double)(nc > 0 ? nc : 1);
     metrics[3 * c] = cs / den;
@@ -638,8 +662,8 @@ int distortion_launch(hipStream_t s, char* base, const DistortionScratch& d, int
         hipLaunchKernelGGL(k_distortion_tiles, dim3((unsigned)d.tiles_bound), dim3(BLK), 0, s, nchart, v_off, p3, uv,
                            t_off, (const int32_t*)d_ct, tris, d_p, d_err);
     hipLaunchKernelGGL(k_distortion_finish, dim3((unsigned)((nchart + BLK - 1) / BLK)), dim3(BLK), 0, s, nchart,
-                       (const int32_t*)d_ct, (int32_t)d.tiles_bound, (const TilePart*)d_p, metrics,
-                       (long long*)counts);
+                       (const int32_t*)d_ct, (int32_t)d.tiles_bound, (const TilePart*)d_p, v_off, uv, t_off, tris,
+                       metrics, (long long*)counts);
     HIPCHK(hipGetLastError());
     return finish_call(s, d_err, h_word, "metrics");
 }

@@ -200,3 +200,77 @@ class Segmentation:
 
 def segment(positions, faces, angle_min_deg=30.0) -> Segmentation:
     return Segmentation(positions, faces, angle_min_deg)
+
+
+@dataclass
+class ChartResult:
+    """pipeline::ChartResult (pipeline.rs:180-189) of one chart of the batched unfold stage."""
+
+    mesh: int
+    vertex_ids: np.ndarray      # the welded mesh's vertex of each chart vertex (ascending)
+    uv: np.ndarray              # (nv, 2), pins at their targets
+    rectified: np.ndarray       # (nv, 2)
+    ext: np.ndarray             # (2,)
+    conformal_mean: float
+    conformal_max: float
+    area_ratio_mean: float
+    flips: int
+    folds: int                  # recut::chart_folds' measure (recut.rs:71-80)
+    iters: int                  # UnfoldResult (lscm.rs:104-116)
+    err_accum: float
+    target: float
+    area3d: float
+
+
+def unfold_batch(meshes, opts: Optional[UnfoldOptions] = None, weld_tol=1e-12, angle_min_deg=30.0, threads=0):
+    """The unfold stage of pipeline::run (pipeline.rs:171-194) over many meshes in one call: every chart's solve
+    in one batched device solve, the driver iterations per chart on host threads, rectangularize and
+    chart_distortion on the device.  meshes: iterable of (positions (V,3), faces (F,3)).  -> ([ChartResult], info)"""
+    opts = opts or UnfoldOptions(precond=Precond.AMG, mixed=True, tol=1e-8)
+    L = lib()
+    if not getattr(L, "_ruv_unfold_batch_ready", False):
+        P = C.c_void_p
+        L.ruv_unfold_batch_run.restype = C.c_int
+        L.ruv_unfold_batch_run.argtypes = [C.c_int32, P, P, P, P, C.c_double, C.c_double, P, C.c_int32, P]
+        L.ruv_unfold_batch_sizes.restype = C.c_int
+        L.ruv_unfold_batch_sizes.argtypes = [P, P]
+        L.ruv_unfold_batch_results.restype = C.c_int
+        L.ruv_unfold_batch_results.argtypes = [P] * 11
+        L.ruv_unfold_batch_destroy.restype = None
+        L.ruv_unfold_batch_destroy.argtypes = [P]
+        L._ruv_unfold_batch_ready = True
+    pos, faces, po, fo = [], [], [0], [0]
+    for p, f in meshes:
+        pos.append(np.ascontiguousarray(p, np.float64).reshape(-1, 3))
+        faces.append(np.ascontiguousarray(f, np.uint32).reshape(-1, 3))
+        po.append(po[-1] + len(pos[-1]))
+        fo.append(fo[-1] + len(faces[-1]))
+    Pp = np.ascontiguousarray(np.concatenate(pos)) if pos else np.zeros((0, 3))
+    F = np.ascontiguousarray(np.concatenate(faces)) if faces else np.zeros((0, 3), np.uint32)
+    po = np.asarray(po, np.int64)
+    fo = np.asarray(fo, np.int64)
+    h = C.c_void_p()
+    o = opts.c_opts()
+    _lib.check(L.ruv_unfold_batch_run(len(po) - 1, _p(Pp), _p(po), _p(F), _p(fo), weld_tol, angle_min_deg,
+                                      C.byref(o), threads, C.byref(h)))
+    try:
+        sz = np.zeros(3, np.int64)
+        _lib.check(L.ruv_unfold_batch_sizes(h, _p(sz)))
+        nc, nv = int(sz[0]), int(sz[1])
+        cm = np.zeros(nc, np.int32)
+        off = np.zeros(nc + 1, np.int64)
+        vid = np.zeros(nv, np.uint32)
+        uv, rq = np.zeros((nv, 2)), np.zeros((nv, 2))
+        ext, met, cnt, st = np.zeros((nc, 2)), np.zeros((nc, 3)), np.zeros((nc, 2), np.int64), np.zeros((nc, 4))
+        info = _lib.SolveInfo()
+        _lib.check(L.ruv_unfold_batch_results(h, _p(cm), _p(off), _p(vid), _p(uv), _p(rq), _p(ext), _p(met), _p(cnt),
+                                              _p(st), C.byref(info)))
+    finally:
+        L.ruv_unfold_batch_destroy(h)
+    out = []
+    for c in range(nc):
+        s = slice(int(off[c]), int(off[c + 1]))
+        out.append(ChartResult(int(cm[c]), vid[s].copy(), uv[s].copy(), rq[s].copy(), ext[c].copy(), float(met[c, 0]),
+                               float(met[c, 1]), float(met[c, 2]), int(cnt[c, 0]), int(cnt[c, 1]), int(st[c, 0]),
+                               float(st[c, 1]), float(st[c, 2]), float(st[c, 3])))
+    return out, info
