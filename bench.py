@@ -137,6 +137,38 @@ def chart_blocks(a, lb):
     return block
 
 
+def step_profile(bt, args, reps=5):
+    """Per-kernel roofline of the AMG-preconditioned step over the whole batch (every chart running): HIP events
+    around every launch of `reps` steps (ruv_batch_step_profile), the algorithmic bytes of each launch (nonzeros'
+    values and columns, each vector element read or written once, index arrays; stored bytes count the padding
+    slots), summed over a step per kernel (all levels of a coarse kernel together).  Outside the timed region."""
+    prof = bt.step_profile(reps)
+    for e in prof:
+        log(f"  step profile: {e['kernel']:12s} level {e['level']:2d} {e['seconds'] * 1e6:8.1f} us "
+            f"{e['bytes'] / 1e6:9.1f} MB ({e['stored_bytes'] / 1e6:9.1f} MB stored) -> "
+            f"{e['bytes'] / e['seconds'] / 1e9:6.0f} GB/s")
+    agg = {}
+    for e in prof:
+        a = agg.setdefault(e["kernel"], dict(kernel=e["kernel"], launches=0, us=0.0, bytes=0.0, stored_bytes=0.0))
+        a["launches"] += 1
+        a["us"] += e["seconds"] * 1e6
+        a["bytes"] += e["bytes"]
+        a["stored_bytes"] += e["stored_bytes"]
+    rows = []
+    for a in sorted(agg.values(), key=lambda a: -a["us"]):
+        gbps = a["bytes"] / (a["us"] * 1e-6) / 1e9
+        rows.append(dict(kernel=a["kernel"], launches_per_step=a["launches"], us_per_step=round(a["us"], 1),
+                         MB_per_step=round(a["bytes"] / 1e6, 1), GBps=round(gbps), frac=round(gbps * 1e9 / HBM_PEAK, 3),
+                         stored_GBps=round(a["stored_bytes"] / (a["us"] * 1e-6) / 1e9)))
+    total = sum(r["us_per_step"] for r in rows)
+    amg = [r for r in rows if r["kernel"].startswith("kc_")]
+    top = amg[0] if amg else None
+    log(f"step profile (whole batch): {total:.0f} us per step; top AMG kernel {top}")
+    return {"step_profile_whole_batch": {"us_per_step": round(total, 1), "reps": reps, "kernels": rows},
+            "amg_top_kernel": None if top is None else
+            {"kernel": top["kernel"], "us_per_step": top["us_per_step"], "GBps": top["GBps"], "frac": top["frac"]}}
+
+
 def run_batch(args):
     import numpy as np
     import scipy.sparse as sp
@@ -197,6 +229,7 @@ def run_batch(args):
     sec, byt, sto = bt.spmv_time(20, stored=True)
     log(f"SpMV over the whole batch: {sec * 1e3:.3f} ms/launch, {byt / 1e9:.3f} GB algorithmic ({sto / 1e9:.3f} GB "
         f"as stored) -> {byt / sec / 1e9:.0f} GB/s")
+    step_prof = step_profile(bt, args) if args.batch_precond == "AMG" and not args.no_step_profile else None
     bt.close()
     if args.no_compare:
         return dict(value=value, ms_per_step=ms_per_step, dtype="f32 inner / f64 refinement",
@@ -204,7 +237,7 @@ def run_batch(args):
                             "unknowns": unknowns, "tol_true_rel_resid": TOL,
                             "tol_forward_error_estimate": UV_EST_TOL, "schedule": sched},
                     extra={"max_inner_iters": int(info.iters), "precond_setup_in_step": pinfo,
-                           "note": "comparison legs skipped (--no-compare)"},
+                           "note": "comparison legs skipped (--no-compare)", **(step_prof or {})},
                     roofline=dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                                   frac=achieved / HBM_PEAK, traffic=None), meshes=meshes)
 
@@ -345,7 +378,7 @@ def run_batch(args):
                                          "seconds": el_d},
                "speedup_vs_fp64_amg_batched": value / (unknowns / el64a),
                "speedup_vs_fp64_batched": value / (unknowns / el64),
-               "speedup_vs_fp64_dropin_per_chart": value / value_d},
+               "speedup_vs_fp64_dropin_per_chart": value / value_d, **(step_prof or {})},
         roofline=roof, meshes=meshes)
 
 
@@ -664,6 +697,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-compare", action="store_true",
                     help="batch: skip the legs after the timed mixed solves (FP64 batches, LU checks, drop-in loop)")
+    ap.add_argument("--no-step-profile", action="store_true",
+                    help="batch: skip the whole-batch per-kernel profile of the step (kernel traces of the solves)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="batch: stream 1 GiB through one kernel first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE runs)")
     args = ap.parse_args()

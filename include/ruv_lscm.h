@@ -224,6 +224,17 @@ int ruv_batch_amg_level(ruv_batch* batch, int32_t level, int64_t* sizes, int64_t
                         double* vals, int32_t* agg, int32_t* chart);
 int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds, double* bytes,
                            double* stored_bytes);
+/* precond amg, measurement hook (no counterpart in the reference): reps steps of the AMG-preconditioned inner
+ * iteration over the whole batch (every chart running, every level whole), each kernel launch bracketed by HIP
+ * events on the batch's stream.  One entry per (kind, level): kind 0 direction update, 1 SpMV, 2 x/r update,
+ * 3 finest pre-smoothing + residual, 4 finest restriction, 5 coarse pre-smoothing + residual + restriction,
+ * 6 coarsest solve, 7 coarse prolongation, 8 coarse post-smoothing, 9 finest prolongation, 10 finest
+ * post-smoothing; the mean seconds per launch and the bytes of one launch, algorithmic (every nonzero's value and
+ * stored column, every vector element read or written once, the index arrays) and as stored (padding slots
+ * included).  Entries go to [0, cap) in launch order; *count receives their number.  The solver's state is
+ * overwritten (a later solve starts afresh).  RUV_ERR_INVALID without a built AMG hierarchy. */
+int ruv_batch_step_profile(ruv_batch* batch, int32_t reps, int32_t cap, int32_t* kind, int32_t* level,
+                           double* seconds, double* bytes, double* stored_bytes, int32_t* count);
 void ruv_batch_destroy(ruv_batch* batch);
 
 /* Build the LSCM systems of nmesh meshes on host worker threads (the worker-thread unfold stage,

@@ -47,6 +47,8 @@ def _setup(L):
     L.ruv_batch_kernel_timing.argtypes = [P, C.c_int32]
     L.ruv_batch_kernel_stats.restype = C.c_int
     L.ruv_batch_kernel_stats.argtypes = [P, P, P, P, P]
+    L.ruv_batch_step_profile.restype = C.c_int
+    L.ruv_batch_step_profile.argtypes = [P, C.c_int32, C.c_int32, P, P, P, P, P, P]
     L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
@@ -201,6 +203,23 @@ class Batch:
         sto = C.c_double(0)
         _lib.check(lib().ruv_batch_kernel_stats(self.h, C.byref(n), C.byref(sec), C.byref(byt), C.byref(sto)))
         return dict(launches=n.value, seconds=sec.value, bytes=byt.value, stored_bytes=sto.value)
+
+    STEP_KERNELS = ("kb_dir", "kb_spmv", "kb_update", "kc_resid0", "kc_restrict", "kc_down", "kc_coarsest",
+                    "kc_prolong", "kc_post", "kc_prolong0", "kc_post0")
+
+    def step_profile(self, reps=5) -> list:
+        """AMG: per kernel launch of the preconditioned step over the whole batch (every chart running), in launch
+        order: kernel, level, mean seconds per launch, algorithmic bytes and bytes as stored of one launch.  The
+        solver's state is overwritten (the next solve starts afresh)."""
+        cap = 256
+        kind, level = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        sec, byt, sto = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        cnt = C.c_int32(0)
+        _lib.check(lib().ruv_batch_step_profile(self.h, reps, cap, _p(kind), _p(level), _p(sec), _p(byt), _p(sto),
+                                                C.byref(cnt)))
+        n = min(cnt.value, cap)
+        return [dict(kernel=self.STEP_KERNELS[kind[j]], level=int(level[j]), seconds=float(sec[j]),
+                     bytes=float(byt[j]), stored_bytes=float(sto[j])) for j in range(n)]
 
     def close(self):
         if self.h:

@@ -33,6 +33,8 @@ int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, dou
 int batch_amg_level(BatchImpl* Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
                     int32_t* chart);
 int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double* bytes, double* stored);
+int batch_step_profile(BatchImpl* Q, int reps, int cap, int32_t* kind, int32_t* level, double* sec, double* bytes,
+                       double* stored, int32_t* count);
 int lscm_batch_build(int32_t nmesh, const double* pos, const int64_t* pos_off, const uint32_t* faces,
                      const int64_t* face_off, double tol, double angle, double angle_mix, int32_t threads,
                      LscmBatch** out);
@@ -500,6 +502,12 @@ int ruv_batch_kernel_stats(ruv_batch* batch, int64_t* launches, double* seconds,
                            double* stored_bytes) {
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_kernel_stats(batch->impl, launches, seconds, bytes, stored_bytes);
+}
+
+int ruv_batch_step_profile(ruv_batch* batch, int32_t reps, int32_t cap, int32_t* kind, int32_t* level,
+                           double* seconds, double* bytes, double* stored_bytes, int32_t* count) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_step_profile(batch->impl, reps, cap, kind, level, seconds, bytes, stored_bytes, count);
 }
 
 void ruv_batch_destroy(ruv_batch* batch) {

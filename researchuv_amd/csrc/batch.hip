@@ -212,6 +212,7 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
         const double sv = mode == RUV_MODE_MIXED ? 4.0 : 8.0, sc = narrow ? 2.0 : 4.0;
         Q.chart_step_bytes.assign(nchart, 0.0);
         Q.chart_step_stored.assign(nchart, 0.0);
+        Q.fin_colb = Q.fin_colb_st = Q.fin_meta = 0.0;
         for (int c = 0; c < nchart; ++c) {
             const double vec = 2.0 * (double)(crow[c + 1] - crow[c]) * R * sv;  // p' read, q written
             double b = vec, st = vec;
@@ -221,6 +222,11 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
                 const double meta = 32.0 + (stencil[s] ? 4.0 * SLOTS : 0.0);
                 b += nz * (stencil[s] ? sv : sv + sc) + meta;
                 st += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + meta;
+                Q.fin_meta += meta;
+                if (!stencil[s]) {
+                    Q.fin_colb += nz * sc;
+                    Q.fin_colb_st += (double)sl_w[s] * Q.h_sl_n[s] * sc;
+                }
             }
             Q.chart_step_bytes[c] = b;
             Q.chart_step_stored[c] = st;
@@ -434,8 +440,8 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
             if (Q.ktime) {
                 if (Q.kev_used + 2 > Q.kev.size()) {
                     hipEvent_t e0, e1;
-                    HIPCHK(hipEventCreate(&e0));
-                    HIPCHK(hipEventCreate(&e1));
+                    HIPCHK(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+                    HIPCHK(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
                     Q.kev.push_back(e0);
                     Q.kev.push_back(e1);
                 }
@@ -596,6 +602,67 @@ int time_step(BatchImpl& Q, int reps, double* sec) {
     return Q.ci == 0 ? time_step_kernel<double, 1, 0>(Q, reps, sec) : time_step_kernel<double, 1, 1>(Q, reps, sec);
 }
 
+// Whole-batch profile of the AMG-preconditioned step (every chart running, whole levels), reps times, each launch
+// marked (kind, level, bytes).  The scalars the step's kernels test are reset before each of them (outside the
+// marked intervals), so no chart stops; the traffic does not depend on the vectors' values.
+template <typename T, int R, int CI>
+int step_profile_kernel(BatchImpl& Q, int reps) {
+    hipStream_t s = Q.stream;
+    BDev& B = Q.B;
+    B.max_iter = 0xffffffffu;
+    const int C = Q.nchart;
+    const size_t nt = (size_t)Q.ntile;
+    std::vector<int> zeros(C, 0);
+    std::vector<double> ones(2 * std::max<size_t>((size_t)C, nt), 1.0), never(C, -1.0);
+    auto reset = [&]() -> int {
+        HIPCHK(hipMemcpyAsync(B.c_done, zeros.data(), sizeof(int) * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.c_tol, never.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.c_rz, ones.data(), sizeof(double) * 2 * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.c_sa, ones.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.c_sb, ones.data(), sizeof(double) * 2 * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.c_sc, ones.data(), sizeof(double) * 2 * C, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.part_a, ones.data(), sizeof(double) * nt, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.part_b, ones.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(B.part_c, ones.data(), sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s));
+        return Q.mark(-1, 0, 0.0, 0.0) == hipSuccess ? RUV_OK : fail(RUV_ERR_DEVICE, "profile event");
+    };
+    const size_t N = (size_t)Q.nrows * R * sizeof(T);
+    for (void* v : {Q.xv, Q.rv, Q.zv, Q.pv0, Q.pv1, Q.qv}) HIPCHK(hipMemsetAsync(v, 0, N, s));
+    int na = 0;
+    int rc = set_active(Q, [](int) { return true; }, &na);
+    if (!rc) rc = bamg_set_live(Q, nullptr);
+    if (rc) return rc;
+    const T* dinv = (sizeof(T) == 4) ? (const T*)(const void*)Q.dinv32 : (const T*)(const void*)Q.dinv64;
+    T *x = (T*)Q.xv, *r = (T*)Q.rv, *z = (T*)Q.zv, *q = (T*)Q.qv;
+    T* pb[2] = {(T*)Q.pv0, (T*)Q.pv1};
+    const double vb = (double)R * sizeof(T), n = (double)Q.nrows;
+    double spmv = 0.0, spmv_st = 0.0;
+    for (int c = 0; c < C; ++c) {
+        spmv += Q.chart_step_bytes[c];
+        spmv_st += Q.chart_step_stored[c];
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipEventRecord(Q.mev[0], s));
+    int par = 0;
+    for (int it = 0; it < reps; ++it) {
+        if ((rc = reset())) return rc;
+        hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, 0u, z,
+                           (const T*)pb[par ^ 1], pb[par]);
+        HIPCHK(Q.mark(BK_DIR, 0, 3.0 * n * vb, 3.0 * n * vb));  // z, old p read; p written
+        launch_spmv<T, R, CI>(Q, na, pb[par], q);
+        HIPCHK(Q.mark(BK_SPMV, 0, spmv, spmv_st));
+        if ((rc = reset())) return rc;
+        hipLaunchKernelGGL((kb_update<T, R, 2>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, dinv,
+                           (const T*)pb[par], (const T*)q, x, r, z);
+        HIPCHK(Q.mark(BK_UPDATE, 0, 6.0 * n * vb, 6.0 * n * vb));  // p, q, x, r read; x, r written
+        if ((rc = reset())) return rc;
+        if ((rc = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z))) return rc;
+        par ^= 1;
+    }
+    HIPCHK(hipGetLastError());
+    return RUV_OK;
+}
+
 // Pick slices-per-tile and the value-load hint by timing the step's SpMV kernel over the whole batch.  Both can
 // be pinned instead (RUV_BATCH_SLICES_PER_TILE = 1, 2, 4 or 8, RUV_BATCH_NONTEMPORAL = 0 or 1): a profiler that
 // serialises launches and reads counters distorts the timings, so its runs pin the choice of the timed run.
@@ -729,6 +796,59 @@ int batch_spmv_time(BatchImpl* Q, int reps, double* sec, double* bytes, double* 
     }
     if (bytes) *bytes = b;
     if (stored) *stored = st;
+    return RUV_OK;
+}
+
+int batch_step_profile(BatchImpl* Q, int reps, int cap, int32_t* kind, int32_t* level, double* sec, double* bytes,
+                       double* stored, int32_t* count) {
+    if (reps <= 0 || cap < 0) return fail(RUV_ERR_INVALID, "reps must be positive and cap non-negative");
+    if (Q->precond != RUV_PC_AMG || !Q->amg) return fail(RUV_ERR_INVALID, "the step profile needs a built AMG");
+    std::vector<BatchImpl::LaunchMark> marks;
+    if (Q->mev.empty()) {
+        hipEvent_t e = nullptr;
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+        Q->mev.push_back(e);
+    }
+    Q->marks = &marks;
+    int rc;
+    const bool f = Q->mode == RUV_MODE_MIXED;
+    if (Q->R == 2) {
+        if (f) rc = Q->ci == 0 ? step_profile_kernel<float, 2, 0>(*Q, reps) : step_profile_kernel<float, 2, 1>(*Q, reps);
+        else rc = Q->ci == 0 ? step_profile_kernel<double, 2, 0>(*Q, reps) : step_profile_kernel<double, 2, 1>(*Q, reps);
+    } else {
+        if (f) rc = Q->ci == 0 ? step_profile_kernel<float, 1, 0>(*Q, reps) : step_profile_kernel<float, 1, 1>(*Q, reps);
+        else rc = Q->ci == 0 ? step_profile_kernel<double, 1, 0>(*Q, reps) : step_profile_kernel<double, 1, 1>(*Q, reps);
+    }
+    Q->marks = nullptr;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(Q->stream));
+    // per (kind, level): mean interval from the previous event to the launch's own
+    struct Acc {
+        int kind, level;
+        double sec, bytes, stored;
+        int n;
+    };
+    std::vector<Acc> acc;
+    for (size_t k = 0; k < marks.size(); ++k) {
+        const BatchImpl::LaunchMark& m = marks[k];
+        if (m.kind < 0) continue;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, Q->mev[k], Q->mev[k + 1]));
+        size_t j = 0;
+        while (j < acc.size() && !(acc[j].kind == m.kind && acc[j].level == m.level)) ++j;
+        if (j == acc.size()) acc.push_back({m.kind, m.level, 0.0, m.bytes, m.stored, 0});
+        acc[j].sec += ms * 1e-3;
+        acc[j].n += 1;
+    }
+    const int nout = (int)std::min<size_t>(acc.size(), (size_t)cap);
+    for (int j = 0; j < nout; ++j) {
+        if (kind) kind[j] = acc[j].kind;
+        if (level) level[j] = acc[j].level;
+        if (sec) sec[j] = acc[j].sec / acc[j].n;
+        if (bytes) bytes[j] = acc[j].bytes;
+        if (stored) stored[j] = acc[j].stored;
+    }
+    if (count) *count = (int32_t)acc.size();
     return RUV_OK;
 }
 

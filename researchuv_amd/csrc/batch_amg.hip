@@ -16,6 +16,7 @@ namespace ruv {
 
 struct BamgLevel {
     int64_t n = 0, nell = 0, nsl = 0;
+    int64_t nnz = 0;          // the operator's nonzeros (levels between the finest and the coarsest)
     int64_t* eoff = nullptr;  // sliced ELL of the operator (levels between the finest and the coarsest)
     int32_t* ew = nullptr;
     int32_t* srow = nullptr;  // slice row starts, at aggregate starts
@@ -567,6 +568,7 @@ int bamg_build(BatchImpl& Q) {
         int32_t hf[2] = {0, 0};
         std::vector<int32_t> hsrow((size_t)nsl + 1);
         HIPCHK(hipMemcpyAsync(&L.nell, L.eoff + nsl, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&L.nnz, N.A.ptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(hsrow.data(), L.srow, sizeof(int32_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -724,6 +726,9 @@ int bamg_set_live(BatchImpl& Q, const int32_t* done) {
 }
 
 // z = M^-1 r (V(1,1) cycle) over the live tiles; partial r.z of every live tile into part_c[par].
+// With Q.marks set (profiling, every chart running, whole levels) each launch is marked with its kind and bytes:
+// algorithmic (every nonzero's value and stored column, every vector element read or written once -- neighbour
+// gathers are served by the caches -- and the level's index arrays) and as stored (padding slots included).
 template <typename T, int R, int CI>
 int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     Bamg& A = *Q.amg;
@@ -735,8 +740,22 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     const VT* vals = (const VT*)A.ev0;
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
     const T w0 = (T)L0.omega;
+    const bool prof = Q.marks != nullptr;
+    const double vb = (double)R * sizeof(T), db = (double)sizeof(T), cv = (double)sizeof(VT);
+    const double n0 = (double)L0.n, n1 = (double)L1.n;
+    const double fin = (double)Q.nnz * cv + Q.fin_colb + Q.fin_meta;
+    const double fin_st = (double)Q.nell * cv + Q.fin_colb_st + Q.fin_meta;
+    auto mark = [&](int kind, int level, double bytes, double stored) -> int {
+        HIPCHK(Q.mark(kind, level, bytes, stored));
+        return RUV_OK;
+    };
+    int rc = RUV_OK;
     hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals,
                        (const T*)L0.dinv, w0, r, (T*)L0.t);
+    if (prof) {  // r, dinv read, t written
+        const double v = n0 * (2.0 * vb + db);
+        if ((rc = mark(BK_RESID0, 0, fin + v, fin_st + v))) return rc;
+    }
     auto cyc = [&](size_t l) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const typename CoarseVal<T>::type*)L.evals,
@@ -747,15 +766,31 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     auto sgrid = [&](size_t l) -> int64_t { return A.slist[l] ? A.nslist[l] : A.lv[l].nsl; };
     auto bgrid = [&](size_t l) -> int64_t { return A.blist[l] ? A.nblist[l] : (A.lv[l].n + BLK - 1) / BLK; };
     auto g1 = [](int64_t n) { return dim3((unsigned)n); };
-    if (bgrid(1) > 0)
+    // a coarse level's operator: values and 16- or 32-bit columns, per nonzero or per stored slot, and the slice
+    // tables (row start, aggregate start, chart range, width, offset)
+    auto lmat = [&](size_t l, bool stored) {
+        const BamgLevel& L = A.lv[l];
+        const double e = cv + (L.ci ? 4.0 : 2.0);
+        return (stored ? (double)L.nell : (double)L.nnz) * e + (double)L.nsl * 28.0;
+    };
+    if (bgrid(1) > 0) {
         hipLaunchKernelGGL((kc_restrict<T, R>), g1(bgrid(1)), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
                            (const int2*)L1.blk, (const int32_t*)A.blist[1], (T*)L1.b);
+        if (prof) {  // t and the member rows read, member offsets read, b_1 written
+            const double v = n0 * (vb + 4.0) + n1 * (4.0 + vb);
+            if ((rc = mark(BK_RESTRICT, 0, v, v))) return rc;
+        }
+    }
     for (size_t l = 1; l + 1 < nl; ++l) {
         const T w = (T)A.lv[l].omega;
         const int64_t ns = sgrid(l);
         if (ns == 0) continue;
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_down<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
         else hipLaunchKernelGGL((kc_down<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        if (prof) {  // b, dinv read; next level's member offsets read, its b written
+            const double v = (double)A.lv[l].n * (vb + db) + (double)A.lv[l + 1].n * (4.0 + vb);
+            if ((rc = mark(BK_DOWN, (int)l, lmat(l, false) + v, lmat(l, true) + v))) return rc;
+        }
     }
     const int64_t nbc = bgrid(nl - 1);
     if (nbc > 0) {
@@ -764,29 +799,56 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
                                (const int64_t*)A.crow_c, (const int64_t*)A.cinv_off, (const double*)A.cinv);
         else
             hipLaunchKernelGGL((kc_coarsest<T, R>), g1(nbc), dim3(BLK), 0, s, cyc(nl - 1));
+        if (prof) {  // b read, z written, each chart's FP64 inverse (or the diagonal) read
+            const double nc = (double)A.lv[nl - 1].n;
+            const double inv = A.cinv ? 8.0 * (double)A.h_cinv_off.back() : nc * db;
+            const double v = nc * (2.0 * vb + 4.0) + inv;
+            if ((rc = mark(BK_COARSEST, (int)nl - 1, v, v))) return rc;
+        }
     }
     for (size_t l = nl - 1; l-- > 1;) {
         const T w = (T)A.lv[l].omega;
         const int64_t ns = sgrid(l), nb = bgrid(l);
         if (ns == 0 || nb == 0) continue;
         const T* zn = (const T*)A.lv[l + 1].z;
+        const double nL = (double)A.lv[l].n, nN = (double)A.lv[l + 1].n;
         if (A.lv[l].n <= A.fuse_rows) {  // small level: the prolongation inside the post-smoothing pass
             if (A.lv[l].ci)
                 hipLaunchKernelGGL((kc_post<T, R, 1, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
             else
                 hipLaunchKernelGGL((kc_post<T, R, 0, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+            if (prof) {  // b, dinv, agg read, z written; the next level's z read
+                const double v = nL * (2.0 * vb + db + 4.0) + nN * vb;
+                if ((rc = mark(BK_POST, (int)l, lmat(l, false) + v, lmat(l, true) + v))) return rc;
+            }
             continue;
         }
         hipLaunchKernelGGL((kc_prolong<T, R>), g1(nb), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+        if (prof) {  // b, dinv, agg read, t written; the next level's z read
+            const double v = nL * (2.0 * vb + db + 4.0) + nN * vb;
+            if ((rc = mark(BK_PROLONG, (int)l, v, v))) return rc;
+        }
         if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
         else hipLaunchKernelGGL((kc_post<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
+        if (prof) {  // b, t, dinv read, z written
+            const double v = nL * (3.0 * vb + db);
+            if ((rc = mark(BK_POST, (int)l, lmat(l, false) + v, lmat(l, true) + v))) return rc;
+        }
     }
     // up: e1 in level 0's work vector (free once restricted), then the post-smoothing over it (two passes:
     // recomputing e1 for every gathered neighbour inside the post-smoothing measured slower)
     hipLaunchKernelGGL((kc_prolong0<T, R>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, (const T*)L0.dinv, w0,
                        (T)A.alpha, r, (const int32_t*)L0.agg, (const T*)L1.z, (T*)L0.t);
+    if (prof) {  // r, dinv, agg read, e written; level 1's z read
+        const double v = n0 * (2.0 * vb + db + 4.0) + n1 * vb;
+        if ((rc = mark(BK_PROLONG0, 0, v, v))) return rc;
+    }
     hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
                        (const T*)L0.dinv, w0, r, (const T*)L0.t, z);
+    if (prof) {  // r, e, dinv read, z written, a partial per tile
+        const double v = n0 * (3.0 * vb + db) + 8.0 * Q.ntile;
+        if ((rc = mark(BK_POST0, 0, fin + v, fin_st + v))) return rc;
+    }
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }

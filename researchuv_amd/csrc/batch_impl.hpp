@@ -54,6 +54,12 @@ struct DevPool {
         if (e_ != hipSuccess) return fail(RUV_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// launch kinds of the profiling marks, numbered as ruv_batch_step_profile reports them
+enum : int {
+    BK_DIR = 0, BK_SPMV = 1, BK_UPDATE = 2, BK_RESID0 = 3, BK_RESTRICT = 4, BK_DOWN = 5, BK_COARSEST = 6,
+    BK_PROLONG = 7, BK_POST = 8, BK_PROLONG0 = 9, BK_POST0 = 10
+};
+
 struct BatchImpl {
     int R = 2, mode = 0, precond = 0, nchart = 0, ntile = 0, nslice = 0;
     int ci = 0;          // general slices: 0 = 16-bit column offsets, 1 = 32-bit columns
@@ -125,6 +131,29 @@ struct BatchImpl {
     double k_seconds = 0.0, k_bytes = 0.0, k_stored = 0.0;
     std::vector<double> chart_step_bytes;   // algorithmic bytes of one SpMV of chart c (nonzeros only)
     std::vector<double> chart_step_stored;  // bytes of one SpMV of chart c as stored (padding slots included)
+    // the finest operator's bytes besides its values: general slices' stored columns (per nonzero / per stored
+    // slot) and the slice tables
+    double fin_colb = 0.0, fin_colb_st = 0.0, fin_meta = 0.0;
+    // launch profiling (optional): with marks set, the step's launches each record an event after them and their
+    // kind, level and bytes (kind < 0: not a kernel; mev[0] is recorded before the first)
+    struct LaunchMark {
+        int kind, level;
+        double bytes, stored;
+    };
+    std::vector<LaunchMark>* marks = nullptr;
+    std::vector<hipEvent_t> mev;
+    hipError_t mark(int kind, int level, double bytes, double stored) {
+        if (!marks) return hipSuccess;
+        const size_t k = marks->size() + 1;
+        while (mev.size() <= k) {
+            hipEvent_t e = nullptr;
+            const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+            if (r != hipSuccess) return r;
+            mev.push_back(e);
+        }
+        marks->push_back({kind, level, bytes, stored});
+        return hipEventRecord(mev[k], stream);
+    }
     DevPool pool;                          // setup and hierarchy blocks (precond AMG)
     Bamg* amg = nullptr;                   // precond AMG: the hierarchy
     int64_t* amg_ptr = nullptr;            // precond AMG: the finest operator in FP64 CSR for (re)builds,
@@ -169,6 +198,7 @@ struct BatchImpl {
         for (hipEvent_t e : fev)
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : mev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);

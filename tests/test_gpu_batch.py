@@ -432,6 +432,40 @@ def test_batch_amg_is_deterministic(monkeypatch):
 
 
 @pytest.mark.parametrize("mixed", [True, False])
+def test_batch_step_profile(mixed):
+    """The whole-batch step profile marks every kernel of the preconditioned step once per level and step, with
+    positive times and the byte counts the level sizes give; a solve after it (state overwritten) is bitwise the
+    solve before it."""
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    say(f"batch step profile mixed={mixed}")
+    meshes = _meshes(12, seed=23)
+    lb = B.LscmBatchBuilder(meshes, threads=2)
+    a = lb.arrays()
+    bt = lb.device_batch(precond=Precond.AMG, mixed=mixed)
+    x1, _, _, _ = bt.solve(a["rhs"], max_iter=20000, tol=1e-10)
+    prof = bt.step_profile(3)
+    rows = bt.precond_info()["level_rows"]
+    nl = len(rows)
+    kinds = [e["kernel"] for e in prof]
+    for k in ("kb_dir", "kb_spmv", "kb_update", "kc_resid0", "kc_restrict", "kc_coarsest", "kc_prolong0", "kc_post0"):
+        assert kinds.count(k) == 1, (k, kinds)
+    assert sorted(e["level"] for e in prof if e["kernel"] == "kc_down") == list(range(1, nl - 1))
+    assert sorted(e["level"] for e in prof if e["kernel"] == "kc_post") == list(range(1, nl - 1))
+    assert all(e["seconds"] > 0 and e["bytes"] > 0 and e["stored_bytes"] >= e["bytes"] for e in prof)
+    vb = 2 * (4 if mixed else 8)
+    n = rows[0]
+    upd = [e for e in prof if e["kernel"] == "kb_update"][0]
+    assert upd["bytes"] == 6 * n * vb  # p, q, x, r read; x, r written
+    coarsest = [e for e in prof if e["kernel"] == "kc_coarsest"][0]
+    assert coarsest["level"] == nl - 1
+    x2, _, _, _ = bt.solve(a["rhs"], max_iter=20000, tol=1e-10)
+    np.testing.assert_array_equal(x1, x2)
+    bt.close()
+
+
+@pytest.mark.parametrize("mixed", [True, False])
 def test_batch_amg_degenerate_charts(mixed):
     """The device AMG over charts of 0, 1, 2 and 3 rows, a permuted chart (32-bit columns on every level) and a
     trailing empty chart: every chart against SuperLU."""
