@@ -218,7 +218,8 @@ int ruv_batch_coarsest_block(ruv_batch* batch, int32_t chart, int64_t* rows, dou
 /* precond amg, inspection hook: level `level` of the device hierarchy in its stored order.  sizes (3): rows, stored
  * operator entries (padding dropped; 0 for level 0, whose operator is the batch's own), rows of the next level
  * (0 on the coarsest).  Any array may be null: the operator as CSR (row_ptr rows + 1, cols, vals: the values the
- * cycle applies), agg (rows: each row's aggregate, a row of the next level; absent on the coarsest) and chart
+ * cycle applies, in FP64; the levels between the finest and the coarsest are stored symmetrically scaled,
+ * a_ij / sqrt(d_i d_j), and returned times the stored sqrt(d_i) sqrt(d_j)), agg (rows: each row's aggregate, a row of the next level; absent on the coarsest) and chart
  * (rows: each row's chart). */
 int ruv_batch_amg_level(ruv_batch* batch, int32_t level, int64_t* sizes, int64_t* row_ptr, int32_t* cols,
                         double* vals, int32_t* agg, int32_t* chart);

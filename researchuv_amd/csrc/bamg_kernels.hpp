@@ -680,11 +680,16 @@ __global__ __launch_bounds__(BLK) void k_poffsets_wide(int64_t n, const int64_t*
     }
 }
 
-// the operator's rows into the ELL blocks of their stored slices, columns as stored rows (CI 0: 16-bit offsets
-// from the row; CI 1: 32-bit rows); padding: value 0, the row itself.  One thread per stored row p (constructed
-// row ipos[p]), so the stores of a slot are contiguous.
+// 1 / sqrt(d) of a diagonal entry, 0 for a non-positive one (a row without couplings keeps zeros)
+__device__ __forceinline__ double rsqrt_or0(double d) { return d > 0.0 ? 1.0 / sqrt(d) : 0.0; }
+
+// the operator's rows into the ELL blocks of their stored slices, scaled symmetrically (a_ij / sqrt(d_i d_j), the
+// product of the two factors formed first so that (i, j) and (j, i) round alike), columns as stored rows (CI 0:
+// 16-bit offsets from the row; CI 1: 32-bit rows); padding: value 0, the row itself.  One thread per stored row p
+// (constructed row ipos[p]), so the stores of a slot are contiguous.
 template <typename T, int CI>
-__global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const int32_t* __restrict__ rpos,
+__global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const double* __restrict__ diag,
+                                                   const int32_t* __restrict__ rpos,
                                                    const int32_t* __restrict__ ipos, int64_t nsl,
                                                    const int32_t* __restrict__ srow, const int64_t* __restrict__ off,
                                                    const int32_t* __restrict__ w,
@@ -696,9 +701,10 @@ __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const 
     const int64_t ns = srow[b + 1] - srow[b];
     const int64_t e0 = off[b] + (p - srow[b]);
     int k = 0;
+    const double si = rsqrt_or0(diag[i]);
     for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
         const int64_t c = rpos ? rpos[A.cols[a]] : A.cols[a];
-        cput(ev + e0 + k * ns, A.vals[a]);
+        cput(ev + e0 + k * ns, (double)A.vals[a] * (si * rsqrt_or0(diag[A.cols[a]])));
         if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)(c - p);
         else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)c;
     }
@@ -709,11 +715,17 @@ __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const 
     }
 }
 
+// sqrt(d) and 1 / sqrt(d) of a scaled level's diagonal at the stored rows (zeros for a non-positive entry)
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_pcast_inv(int64_t n, const double* __restrict__ d,
-                                                   const int32_t* __restrict__ rpos, T* __restrict__ dinv) {
+__global__ __launch_bounds__(BLK) void k_pscale(int64_t n, const double* __restrict__ d,
+                                                const int32_t* __restrict__ rpos, T* __restrict__ q,
+                                                T* __restrict__ rs) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < n) dinv[rpos ? rpos[i] : i] = (T)(1.0 / d[i]);
+    if (i >= n) return;
+    const int64_t p = rpos ? rpos[i] : i;
+    const double r = rsqrt_or0(d[i]);
+    q[p] = (T)(r > 0.0 ? sqrt(d[i]) : 0.0);
+    rs[p] = (T)r;
 }
 
 // ---------------------------------------------------------------- cycle
@@ -739,6 +751,9 @@ struct CycLevel {
     const int32_t* chart;    // row -> chart
     const int2* blk;         // per BLK rows: first and last chart
     const int32_t* done;     // per chart: B_RUNNING while its solve runs
+    const T* q;              // sqrt(d_i) and 1 / sqrt(d_i) of the level's diagonal (scaled levels)
+    const T* rs;
+    const T* rsn;            // 1 / sqrt(d) of the next level when that one is scaled (else null)
 };
 
 // Rows of charts whose solve has stopped are skipped on every level: a chart's rows only couple to rows of the
@@ -801,9 +816,18 @@ __device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r,
     }
 }
 
-// Levels between the finest and the coarsest, one workgroup per slice: t = b - A (w D^-1 b) over the slice's rows
-// (into LDS), then b_next[J] = the sum of t over the members of every next-level aggregate J starting in the slice
-// (its members are the consecutive rows mptr[J] .. mptr[J + 1], all in the slice), in member order.
+// Levels between the finest and the coarsest are stored symmetrically scaled: the operator as
+// A_s = D^-1/2 A D^-1/2 (unit diagonal, the same bfloat16 value for (i, j) and (j, i)), the right-hand side as
+// b_s = D^-1/2 b and the work vector of the post-smoothing as e_s = D^1/2 e; the correction z is stored as it is.
+// Then the Jacobi step w D^-1 r is the scalar w r_s and every neighbour gather reads one vector (the unscaled form
+// gathers d_j as well), while the cycle is the same operator: t_s = b_s - w A_s b_s = D^-1/2 (b - A w D^-1 b),
+// the restriction sums sqrt(d_i) t_s,i = t_i, e_s = w b_s + alpha sqrt(d_i) (P z_next)_i, and
+// z = D^-1/2 (e_s + w (b_s - A_s e_s)).  The coarsest level keeps the unscaled form (its exact block solve).
+//
+// One workgroup per slice: t_s = b_s - A_s (w b_s) over the slice's rows (sqrt(d_i) t_s,i into LDS), then b_next[J]
+// = the sum over the members of every next-level aggregate J starting in the slice (its members are the
+// consecutive rows mptr[J] .. mptr[J + 1], all in the slice), in member order, times 1 / sqrt(d_next,J) when the
+// next level is scaled.
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict__ bnext) {
     __shared__ T tl[MAX_SLICE_ROWS * R];
@@ -824,16 +848,16 @@ __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict
             for (int c = 0; c < R; ++c) acc[c] = T(0);
             ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i,
                                 [&](int64_t j) {
-                                    const T d = w * L.dinv[j];
                                     Vr<T, R> x = ld<T, R>(L.b, j);
 #pragma unroll
-                                    for (int c = 0; c < R; ++c) x.c[c] *= d;
+                                    for (int c = 0; c < R; ++c) x.c[c] *= w;
                                     return x;
                                 },
                                 acc);
             const Vr<T, R> bi = ld<T, R>(L.b, i);
+            const T qi = L.q[i];
 #pragma unroll
-            for (int c = 0; c < R; ++c) o.c[c] = bi.c[c] - acc[c];
+            for (int c = 0; c < R; ++c) o.c[c] = qi * (bi.c[c] - acc[c]);
         }
 #pragma unroll
         for (int c = 0; c < R; ++c) tl[r * R + c] = o.c[c];
@@ -849,11 +873,17 @@ __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict
         for (int64_t m = m0; m < m1; ++m)
 #pragma unroll
             for (int c = 0; c < R; ++c) o.c[c] += tl[(m - r0) * R + c];
+        if (L.rsn) {
+            const T sj = L.rsn[J];
+#pragma unroll
+            for (int c = 0; c < R; ++c) o.c[c] *= sj;
+        }
         st<T, R>(bnext, J, o);
     }
 }
 
-// per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t
+// per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t (times
+// 1 / sqrt(d_next,I) when the next level is scaled)
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, const int32_t* __restrict__ cnext,
                                                    const int2* __restrict__ bnext_blk,
@@ -873,28 +903,29 @@ __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, co
         for (int c = 0; c < R; ++c) acc[c] += tm.c[c];
     }
     Vr<T, R> o;
+    const T sI = L.rsn ? L.rsn[I] : T(1);
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = acc[c];
+    for (int c = 0; c < R; ++c) o.c[c] = L.rsn ? acc[c] * sI : acc[c];
     st<T, R>(bnext, I, o);
 }
 
-// level >= 1: t = e1 = w D^-1 b + alpha P z_next
+// scaled level: t = e_s = w b_s + alpha sqrt(d_i) (P z_next)_i
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kc_prolong(CycLevel<T> L, T w, T alpha, const T* __restrict__ znext) {
     const int64_t b = wg_item(L.blist), i = b * BLK + threadIdx.x;
     if (i >= L.n || !live_row(L, block_state(L.blk, b, L.done), i)) return;
-    const T di = w * L.dinv[i];
+    const T aq = alpha * L.q[i];
     const Vr<T, R> bi = ld<T, R>(L.b, i);
     const Vr<T, R> zc = ld<T, R>(znext, L.agg[i]);
     Vr<T, R> o;
 #pragma unroll
-    for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c], alpha * zc.c[c]);
+    for (int c = 0; c < R; ++c) o.c[c] = fma(w, bi.c[c], aq * zc.c[c]);
     st<T, R>(L.t, i, o);
 }
 
-// levels between the finest and the coarsest, one workgroup per slice: z = e1 + w D^-1 (b - A e1), e1 in t; or,
-// with FUSED (small levels, where a launch costs more than the extra gathers), e1 formed for every row read
-// exactly as kc_prolong forms it (same operations, so the same values) instead of a kc_prolong pass before.
+// scaled levels, one workgroup per slice: z = D^-1/2 (e_s + w (b_s - A_s e_s)), e_s in t; or, with FUSED (small
+// levels, where a launch costs more than the extra gathers), e_s formed for every row read exactly as kc_prolong
+// forms it (same operations, so the same values) instead of a kc_prolong pass before.
 template <typename T, int R, int CI, int FUSED = 0>
 __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w, T alpha = T(0),
                                                const T* __restrict__ znext = nullptr) {
@@ -912,21 +943,21 @@ __global__ __launch_bounds__(BLK) void kc_post(CycLevel<T> L, T w, T alpha = T(0
         for (int c = 0; c < R; ++c) s[c] = T(0);
         const auto e1 = [&](int64_t j) {
             if (!FUSED) return ld<T, R>(L.t, j);
-            const T dj = w * L.dinv[j];
+            const T aq = alpha * L.q[j];
             const Vr<T, R> bj = ld<T, R>(L.b, j);
             const Vr<T, R> zc = ld<T, R>(znext, L.agg[j]);
             Vr<T, R> o;
 #pragma unroll
-            for (int c = 0; c < R; ++c) o.c[c] = fma(dj, bj.c[c], alpha * zc.c[c]);
+            for (int c = 0; c < R; ++c) o.c[c] = fma(w, bj.c[c], aq * zc.c[c]);
             return o;
         };
         ell_apply<T, R, CI>(L, EllRow{e0 + r, h, wd}, i, e1, s);
-        const T di = w * L.dinv[i];
+        const T si = L.rs[i];
         const Vr<T, R> bi = ld<T, R>(L.b, i);
         const Vr<T, R> ei = e1(i);
         Vr<T, R> o;
 #pragma unroll
-        for (int c = 0; c < R; ++c) o.c[c] = fma(di, bi.c[c] - s[c], ei.c[c]);
+        for (int c = 0; c < R; ++c) o.c[c] = si * fma(w, bi.c[c] - s[c], ei.c[c]);
         st<T, R>(L.z, i, o);
     }
 }

@@ -25,7 +25,9 @@ struct BamgLevel {
     void* ecols = nullptr;
     int ci = 1;               // 0: 16-bit column offsets
     void* evals = nullptr;    // T
-    void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal)
+    void* dinv = nullptr;     // T (level 0: the batch's Jacobi diagonal; the coarsest level: its inverse diagonal)
+    void* q = nullptr;        // T, scaled levels (between the finest and the coarsest): sqrt(d_i), 1 / sqrt(d_i)
+    void* rs = nullptr;
     int32_t* agg = nullptr;   // row -> next-level aggregate
     int32_t* mptr = nullptr;  // next level's aggregates -> member rows here (levels >= 1: first stored row)
     int32_t* mrows = nullptr; // (the finest level: the member rows)
@@ -589,30 +591,32 @@ int bamg_build(BatchImpl& Q) {
         }
         if (hf[0]) return fail(RUV_ERR_PRECOND, "batch amg: an aggregate holds more rows than a slice allows");
         L.ci = hf[1] ? 1 : 0;
-        char *v = nullptr, *d = nullptr, *ec = nullptr;
+        char *v = nullptr, *q = nullptr, *rs = nullptr, *ec = nullptr;
         HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
         L.ecols = ec;
         HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
-        HIPCHK(A->alloc(&d, n * sv));
+        HIPCHK(A->alloc(&q, n * sv));
+        HIPCHK(A->alloc(&rs, n * sv));
         auto fill = [&](auto tv) {
             using TT = decltype(tv);
             using VT = typename CoarseVal<TT>::type;
             if (L.ci)
-                hipLaunchKernelGGL((k_pell_fill<TT, 1>), rows_grid(n), dim3(BLK), 0, s, n, N.A, (const int32_t*)rpos,
-                                   (const int32_t*)ipos, nsl, (const int32_t*)L.srow, (const int64_t*)L.eoff,
-                                   (const int32_t*)L.ew, (VT*)v, L.ecols);
+                hipLaunchKernelGGL((k_pell_fill<TT, 1>), rows_grid(n), dim3(BLK), 0, s, n, N.A, N.diag,
+                                   (const int32_t*)rpos, (const int32_t*)ipos, nsl, (const int32_t*)L.srow,
+                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
             else
-                hipLaunchKernelGGL((k_pell_fill<TT, 0>), rows_grid(n), dim3(BLK), 0, s, n, N.A, (const int32_t*)rpos,
-                                   (const int32_t*)ipos, nsl, (const int32_t*)L.srow, (const int64_t*)L.eoff,
-                                   (const int32_t*)L.ew, (VT*)v, L.ecols);
-            hipLaunchKernelGGL((k_pcast_inv<TT>), rows_grid(n), dim3(BLK), 0, s, n, N.diag, (const int32_t*)rpos,
-                               (TT*)d);
+                hipLaunchKernelGGL((k_pell_fill<TT, 0>), rows_grid(n), dim3(BLK), 0, s, n, N.A, N.diag,
+                                   (const int32_t*)rpos, (const int32_t*)ipos, nsl, (const int32_t*)L.srow,
+                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
+            hipLaunchKernelGGL((k_pscale<TT>), rows_grid(n), dim3(BLK), 0, s, n, N.diag, (const int32_t*)rpos,
+                               (TT*)q, (TT*)rs);
         };
         if (f32) fill(0.0f);
         else fill(0.0);
         HIPCHK(hipGetLastError());
         L.evals = v;
-        L.dinv = d;
+        L.q = q;
+        L.rs = rs;
         rc = alloc_vectors(L);
         if (rc) return rc;
     }
@@ -760,7 +764,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         BamgLevel& L = A.lv[l];
         return CycLevel<T>{L.n, L.eoff, L.ew, L.ecols, (const typename CoarseVal<T>::type*)L.evals,
                            L.srow, L.sagg, L.sblk, A.slist[l], A.blist[l], (const T*)L.dinv, L.agg, L.mptr, L.mrows,
-                           (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done};
+                           (T*)L.b, (T*)L.z, (T*)L.t, L.chart, L.blk, Q.B.c_done, (const T*)L.q, (const T*)L.rs,
+                           (l + 2 < nl) ? (const T*)A.lv[l + 1].rs : nullptr};
     };
     // grids: every slice / block of a level, or the listed ones of the running charts
     auto sgrid = [&](size_t l) -> int64_t { return A.slist[l] ? A.nslist[l] : A.lv[l].nsl; };
@@ -776,8 +781,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     if (bgrid(1) > 0) {
         hipLaunchKernelGGL((kc_restrict<T, R>), g1(bgrid(1)), dim3(BLK), 0, s, cyc(0), L1.n, L1.chart,
                            (const int2*)L1.blk, (const int32_t*)A.blist[1], (T*)L1.b);
-        if (prof) {  // t and the member rows read, member offsets read, b_1 written
-            const double v = n0 * (vb + 4.0) + n1 * (4.0 + vb);
+        if (prof) {  // t and the member rows read, member offsets read, b_1 written (and its scale read)
+            const double v = n0 * (vb + 4.0) + n1 * (4.0 + vb + (nl > 2 ? db : 0.0));
             if ((rc = mark(BK_RESTRICT, 0, v, v))) return rc;
         }
     }
@@ -952,7 +957,13 @@ int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* c
         std::vector<double> d64(f32 ? 0 : (size_t)std::max<int64_t>(ne, 1));
         if (f32) HIPCHK(hipMemcpyAsync(b16.data(), L.evals, sizeof(uint16_t) * ne, hipMemcpyDeviceToHost, s));
         else HIPCHK(hipMemcpyAsync(d64.data(), L.evals, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        // the stored values are scaled by 1 / sqrt(d_i d_j): undone with the stored sqrt(d)
+        std::vector<float> q32(f32 ? (size_t)std::max<int64_t>(n, 1) : 0);
+        std::vector<double> q64(f32 ? 0 : (size_t)std::max<int64_t>(n, 1));
+        if (f32) HIPCHK(hipMemcpyAsync(q32.data(), L.q, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        else HIPCHK(hipMemcpyAsync(q64.data(), L.q, sizeof(double) * n, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        auto qv = [&](int64_t r) { return f32 ? (double)q32[(size_t)r] : q64[(size_t)r]; };
         for (int64_t b = 0; b < nsl; ++b) {
             const int64_t r0 = srow[b], ns = srow[b + 1] - r0;
             for (int64_t p = r0; p < r0 + ns; ++p) {
@@ -970,7 +981,7 @@ int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* c
                     }
                     if (c == p && v == 0.0) continue;  // padding
                     cc.push_back((int32_t)c);
-                    vv.push_back(v);
+                    vv.push_back(v * qv(p) * qv(c));
                 }
                 ptr.push_back((int64_t)cc.size());
             }
