@@ -471,7 +471,9 @@ def run_single(args):
         log(f"vs SuperLU: uv err {extra['uv_rel_err_vs_cpu_lu']:.2e}, energy {extra['energy_rel_err_vs_cpu_lu']:.2e}")
     # the plan drop-in beside it, one call per preconditioner: plain FP64 PCG (the reference's recurrence and
     # recursive-residual test) and, for IC(0), the configuration's FP32 PCG + IC(0) inside FP64 refinement (true
-    # residual test); each timed after one warm solve, host setup (factorisation, levels) timed apart
+    # residual test); each timed after one warm solve, setup timed apart (plan creation: for IC(0) the structure,
+    # levels and colouring on the host, the factor's values on the device) and, for the preconditioners a plan
+    # refreshes on the device, the values update alone (refactor_s: the IC(0) factorisation kernels)
     Lp = solver.CsrMatrix(s["l_vals"], s["l_cols"], s["l_ptr"])
     xp = torch.zeros_like(b)
     compare = args.compare if args.compare is not None else (
@@ -489,15 +491,20 @@ def run_single(args):
                 ip = plan.solve_dev(b.data_ptr(), xp.data_ptr(), args.compare_iters, TOL)
                 torch.cuda.synchronize()
                 el = time.perf_counter() - t0
+                rf = None
+                if name in ("IC0", "IC0_COLOR", "JACOBI"):
+                    t0 = time.perf_counter()
+                    plan.update_values(Lp.vals)
+                    rf = time.perf_counter() - t0
                 plan.close()
-                rec = {"value": unknowns / el, "solve_s": el, "host_setup_s": st, "iters": int(ip.iters),
+                rec = {"value": unknowns / el, "solve_s": el, "setup_s": st, "refactor_s": rf, "iters": int(ip.iters),
                        "refinements": int(ip.refinements), "true_rel_resid": float(ip.true_rel_resid)}
                 if xr is not None:
                     xq = xp.cpu().numpy().reshape(-1, 2)
                     rec["uv_rel_err_vs_cpu_lu"] = float(np.linalg.norm(xq - xr, axis=1).max() /
                                                         np.linalg.norm(xr, axis=1).max())
                 extra[key] = rec
-                log(f"{key}: setup {st:.2f}s solve {el * 1e3:.1f} ms iters {ip.iters} true resid "
+                log(f"{key}: setup {st:.2f}s (values update {rf}) solve {el * 1e3:.1f} ms iters {ip.iters} true resid "
                     f"{ip.true_rel_resid:.2e} uv err {rec.get('uv_rel_err_vs_cpu_lu', float('nan')):.2e}")
             except Exception as e:  # not converged within the cap, or a failed setup: reported, not fatal
                 extra[key] = {"error": str(e)[:200], "iter_cap": args.compare_iters}

@@ -88,6 +88,12 @@ int ruv_cg_solve(const double* vals, const int32_t* cols, const int32_t* row_ptr
  * (precond.rs:145-219) applied by the device triangular sweeps. */
 int ruv_apply_ic0(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
                   const double* r, double* z_out);
+/* Inspection hook: the IC(0) factor of A (factor_ic0, precond.rs:145-219) as the device factorisation computes it
+ * (structure on the host, values by level-scheduled device kernels in the reference's operation order): L's rows,
+ * strict-lower entries with ascending columns and the diagonal last (l_ptr n + 1, l_cols and l_vals *l_nnz).  Call
+ * with null arrays for *l_nnz.  RUV_ERR_PRECOND where factor_ic0 returns None (a vanishing pivot or diagonal). */
+int ruv_ic0_factor(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                   int64_t* l_nnz, int32_t* l_ptr, int32_t* l_cols, double* l_vals);
 
 /* Mixed-precision form (no reference counterpart; the reference GPU path is FP64 throughout,
  * solver.rs:284-565): the SpMV, preconditioner and Krylov vectors run in FP32 inside an FP64
@@ -151,6 +157,14 @@ int ruv_plan_solve_dev(ruv_plan* plan, const double* d_b, double* d_x, uint64_t 
 /* Host-buffer convenience around ruv_plan_solve_dev (copies b in, x out). */
 int ruv_plan_solve(ruv_plan* plan, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
 void ruv_plan_destroy(ruv_plan* plan);
+/* New values for the plan's operator (nnz doubles in the CSR order it was created with; same pattern): the
+ * operator and its preconditioner are refreshed on the device -- the Jacobi diagonal, or the IC(0) factor
+ * recomputed by the device factorisation from its kept structure (factor_ic0, precond.rs:145-219; the reference
+ * builds a new factor per solve, solver.rs:284-360) -- so refinement rounds and driver iterations reuse the plan.
+ * RUV_ERR_PRECOND on a vanishing pivot or diagonal (the plan then must not be solved with); RUV_ERR_INVALID for
+ * the amg preconditioners, whose hierarchy is built on the host (create a new plan).  Not concurrent with a solve
+ * of the same plan. */
+int ruv_plan_update_values(ruv_plan* plan, const double* vals);
 /* Measurement hook (no reference counterpart): run the plan's inner-precision SpMV kernel (the one its PCG
  * iterations launch) reps times back to back on the plan's stream and return the mean launch duration from
  * HIP events, plus the kernel's algorithmic bytes per launch: nnz*(s_v + 4) + (nrows + 1)*4 +

@@ -92,6 +92,10 @@ def lib():
         L.ruv_plan_solve.argtypes = [P, P, P, C.c_uint64, C.c_double, P]
         L.ruv_plan_destroy.restype = None
         L.ruv_plan_destroy.argtypes = [P]
+        L.ruv_plan_update_values.restype = C.c_int
+        L.ruv_plan_update_values.argtypes = [P, P]
+        L.ruv_ic0_factor.restype = C.c_int
+        L.ruv_ic0_factor.argtypes = [P, P, P, C.c_int32, C.c_int64, P, P, P, P]
         for name in ("ruv_unfold_chart",):
             if hasattr(L, name):
                 f = getattr(L, name)

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <memory>
 #include <string>
+#include <vector>
+#include <algorithm>
 
 #include "host_lscm.hpp"
 #include "ruv_internal.hpp"
@@ -15,6 +17,8 @@ void plan_destroy(PlanImpl* p);
 int plan_solve_dev(PlanImpl* P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info);
 int plan_solve_host(PlanImpl* P, const double* b, double* x, uint64_t max_iter, double tol, ruv_solve_info* info);
 int plan_spmv_time(PlanImpl* P, int reps, double* sec, double* bytes);
+int plan_update_values(PlanImpl* P, const double* vals);
+int ic0_factor_device(const Csr& a, std::vector<int32_t>* ptr, std::vector<int32_t>* cols, std::vector<double>* vals);
 struct BatchImpl;
 int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const double* vals, int nchart,
                  const int64_t* crow, int R, int precond, int mode, BatchImpl** out);
@@ -269,6 +273,27 @@ int ruv_plan_solve(ruv_plan* plan, const double* b, double* x, uint64_t max_iter
 int ruv_plan_spmv_time(ruv_plan* plan, int32_t reps, double* seconds_per_launch, double* bytes_per_launch) {
     if (!plan || !seconds_per_launch || !bytes_per_launch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::plan_spmv_time(plan->impl, reps, seconds_per_launch, bytes_per_launch);
+}
+
+int ruv_plan_update_values(ruv_plan* plan, const double* vals) {
+    if (!plan || !vals) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::plan_update_values(plan->impl, vals);
+}
+
+int ruv_ic0_factor(const double* vals, const int32_t* cols, const int32_t* row_ptr, int32_t n, int64_t nnz,
+                   int64_t* l_nnz, int32_t* l_ptr, int32_t* l_cols, double* l_vals) {
+    ruv::Csr a;
+    int rc = take_csr(vals, cols, row_ptr, n, nnz, &a);
+    if (rc) return rc;
+    std::vector<int32_t> p, c;
+    std::vector<double> v;
+    rc = ruv::ic0_factor_device(a, &p, &c, &v);
+    if (rc) return rc;
+    if (l_nnz) *l_nnz = (int64_t)v.size();
+    if (l_ptr) std::copy(p.begin(), p.end(), l_ptr);
+    if (l_cols) std::copy(c.begin(), c.end(), l_cols);
+    if (l_vals) std::copy(v.begin(), v.end(), l_vals);
+    return RUV_OK;
 }
 
 void ruv_plan_destroy(ruv_plan* plan) {

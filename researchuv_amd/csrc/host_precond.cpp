@@ -108,56 +108,60 @@ Csr transpose_lower(const Csr& l) {
     return t;
 }
 
-// factor_ic0 (precond.rs:145-219): L on A's strict-lower pattern, ascending columns, diagonal last;
-// L(i,k) = (A(i,k) - sum_{j<k} L(i,j) L(k,j)) / L(k,k); L(i,i) = sqrt(max(A(i,i) - sum L(i,k)^2, 1e-12|A(i,i)|)).
-bool factor_ic0(const Csr& a, TriFactor* f) {
+// The structure of factor_ic0's factor (precond.rs:145-219) without its values: L on A's strict-lower pattern,
+// columns ascending (a stable sort, as the reference's), the diagonal last; L^T (transpose_lower) and both
+// sweeps' levels (tri_levels).  For the device factorisation: src[q], the position in the values of A of each
+// strict-lower entry q of L (-1 at the diagonals; positions through val_of, the place of every entry of `a` in
+// the caller's value array, null: the identity); dsrc[i], that of row i's diagonal (its first entry, as
+// Csr::get reads it; -1 when absent); lt_src[p], the L entry each L^T entry copies.
+void ic0_pattern(const Csr& a, const std::vector<int64_t>* val_of, TriFactor* f, std::vector<int64_t>* src,
+                 std::vector<int64_t>* dsrc, std::vector<int32_t>* lt_src) {
     const int32_t n = a.n;
+    auto pos = [&](int32_t k) { return val_of ? (*val_of)[(size_t)k] : (int64_t)k; };
     Csr& L = f->l;
     L.n = n;
     L.ptr.assign(1, 0);
-    L.vals.clear();
     L.cols.clear();
-    std::vector<int32_t> sp(n + 1, 0);  // start of the strict-lower part of row i inside L
-    std::vector<double> diag(n, 0.0);
-    std::vector<std::pair<int32_t, double>> low;
+    L.vals.clear();
+    src->clear();
+    dsrc->assign((size_t)n, -1);
+    std::vector<std::pair<int32_t, int32_t>> low;
     for (int32_t i = 0; i < n; ++i) {
         low.clear();
-        for (int32_t k = a.ptr[i]; k < a.ptr[i + 1]; ++k)
-            if (a.cols[k] < i) low.emplace_back(a.cols[k], a.vals[k]);
-        std::stable_sort(low.begin(), low.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-        const int32_t start = (int32_t)L.vals.size();
-        sp[i] = start;
-        for (const auto& [k, aik] : low) {
-            double s = aik;
-            // rows[i].range(..k) x rows[k]: both ascending, merge
-            int32_t qi = start, qe = (int32_t)L.vals.size();
-            int32_t qk = sp[k], qke = L.ptr[k + 1] - 1;  // strict-lower part of row k
-            while (qi < qe && L.cols[qi] < k) {
-                const int32_t j = L.cols[qi];
-                while (qk < qke && L.cols[qk] < j) ++qk;
-                if (qk < qke && L.cols[qk] == j) s -= L.vals[qi] * L.vals[qk];
-                ++qi;
-            }
-            const double lkk = diag[k];
-            if (std::fabs(lkk) < 1e-300) return false;
-            L.cols.push_back(k);
-            L.vals.push_back(s / lkk);
+        for (int32_t k = a.ptr[i]; k < a.ptr[i + 1]; ++k) {
+            if (a.cols[k] < i) low.emplace_back(a.cols[k], k);
+            else if (a.cols[k] == i && (*dsrc)[i] < 0) (*dsrc)[i] = pos(k);
         }
-        const double aii = a.get(i, i);
-        if (std::fabs(aii) < 1e-300) return false;
-        double sum = aii;
-        for (int32_t q = start; q < (int32_t)L.vals.size(); ++q) sum -= L.vals[q] * L.vals[q];
-        const double floor_ = 1e-12 * std::max(std::fabs(aii), 1e-30);
-        if (sum < floor_) sum = floor_;
-        diag[i] = std::sqrt(sum);
+        std::stable_sort(low.begin(), low.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (const auto& [c, k] : low) {
+            L.cols.push_back(c);
+            src->push_back(pos(k));
+        }
         L.cols.push_back(i);
-        L.vals.push_back(diag[i]);
-        L.ptr.push_back((int32_t)L.vals.size());
+        src->push_back(-1);  // the diagonal slot (from dsrc)
+        L.ptr.push_back((int32_t)L.cols.size());
     }
+    // L^T with each entry's L position as its value (exact in a double), then the values dropped
+    L.vals.resize(L.cols.size());
+    for (size_t q = 0; q < L.vals.size(); ++q) L.vals[q] = (double)q;
     f->lt = transpose_lower(L);
+    lt_src->resize(f->lt.vals.size());
+    for (size_t p = 0; p < f->lt.vals.size(); ++p) (*lt_src)[p] = (int32_t)f->lt.vals[p];
+    L.vals.assign(L.cols.size(), 0.0);
+    f->lt.vals.assign(f->lt.cols.size(), 0.0);
     tri_levels(f->l, false, &f->fwd_rows, &f->fwd_off);
     tri_levels(f->lt, true, &f->bwd_rows, &f->bwd_off);
-    return true;
+}
+
+// The position of every entry of permute_csr(a, ...) in a's values (the same permutation applied to positions).
+std::vector<int64_t> permuted_positions(const Csr& a, const std::vector<int32_t>& to_perm,
+                                        const std::vector<int32_t>& to_orig) {
+    Csr ai = a;
+    for (size_t k = 0; k < ai.vals.size(); ++k) ai.vals[k] = (double)k;
+    const Csr pa = permute_csr(ai, to_perm, to_orig);
+    std::vector<int64_t> out(pa.vals.size());
+    for (size_t k = 0; k < out.size(); ++k) out[k] = (int64_t)pa.vals[k];
+    return out;
 }
 
 // graph_coloring (precond.rs:52-77): greedy, index order, smallest colour not used by a neighbour.

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "amg_kernels.hpp"
@@ -105,16 +106,16 @@ inline std::vector<typename DevTri<float>::Seg> tri_segments(const std::vector<i
     return out;
 }
 
+// the factor's structure and levels; its values are computed on the device (ic0_numeric)
 template <typename T>
 int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
     d->n = f.l.n;
-    std::vector<T> lv(f.l.vals.begin(), f.l.vals.end()), ltv(f.lt.vals.begin(), f.lt.vals.end());
     HIPCHK(m.upload(&d->l_ptr, f.l.ptr.data(), f.l.ptr.size(), s));
     HIPCHK(m.upload(&d->l_cols, f.l.cols.data(), f.l.cols.size(), s));
-    HIPCHK(m.upload(&d->l_vals, lv.data(), lv.size(), s));
+    HIPCHK(m.alloc(&d->l_vals, f.l.cols.size()));
     HIPCHK(m.upload(&d->lt_ptr, f.lt.ptr.data(), f.lt.ptr.size(), s));
     HIPCHK(m.upload(&d->lt_cols, f.lt.cols.data(), f.lt.cols.size(), s));
-    HIPCHK(m.upload(&d->lt_vals, ltv.data(), ltv.size(), s));
+    HIPCHK(m.alloc(&d->lt_vals, f.lt.cols.size()));
     HIPCHK(m.upload(&d->fwd_rows, f.fwd_rows.data(), f.fwd_rows.size(), s));
     HIPCHK(m.upload(&d->fwd_off, f.fwd_off.data(), f.fwd_off.size(), s));
     HIPCHK(m.upload(&d->bwd_rows, f.bwd_rows.data(), f.bwd_rows.size(), s));
@@ -127,7 +128,7 @@ int upload_tri(DevArena& m, const TriFactor& f, hipStream_t s, DevTri<T>* d) {
     d->bwd_seg.clear();
     for (const auto& g : tri_segments(f.fwd_off, wide)) d->fwd_seg.push_back({g.l0, g.l1, g.a, g.b, g.wide});
     for (const auto& g : tri_segments(f.bwd_off, wide)) d->bwd_seg.push_back({g.l0, g.l1, g.a, g.b, g.wide});
-    // the host vectors lv/ltv must outlive the async copies
+    // the host vectors must outlive the async copies
     HIPCHK(hipStreamSynchronize(s));
     return RUV_OK;
 }
@@ -215,6 +216,14 @@ struct PlanImpl {
     // permutation round trip for the colour-ordered IC(0) (precond.rs:613-630)
     int *to_orig = nullptr, *to_perm = nullptr;
     bool permuted = false;
+    // IC(0) computed on the device from vals64: each strict-lower L entry's and each diagonal's position in the
+    // values, each L^T entry's L entry, the FP64 workspace (mixed form), a failure flag
+    int64_t *ic0_src = nullptr, *ic0_dsrc = nullptr;
+    int32_t* ic0_ltsrc = nullptr;
+    int64_t ic0_nl = 0;
+    double* ic0_work = nullptr;
+    int* ic0_fail = nullptr;
+    std::vector<int64_t> jacobi_diag;  // Jacobi: each row's diagonal entry (-1: none)
     // work vectors
     double *x64 = nullptr, *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *q64 = nullptr, *y64 = nullptr,
            *t64 = nullptr;
@@ -245,6 +254,60 @@ struct PlanImpl {
 
 namespace {
 
+// The IC(0) factor's values on the device from the plan's FP64 values (k_ic0_sweep / k_ic0_level over the forward
+// levels, then the sweeps' copies); RUV_ERR_PRECOND on a vanishing pivot or diagonal (factor_ic0 returning None,
+// precond.rs:177-187).
+template <typename T>
+int ic0_numeric(PlanImpl& P, DevTri<T>& d) {
+    hipStream_t s = P.stream;
+    double* lv = std::is_same<T, double>::value ? (double*)(void*)d.l_vals : P.ic0_work;
+    HIPCHK(hipMemsetAsync(P.ic0_fail, 0, sizeof(int), s));
+    for (const auto& g : d.fwd_seg) {
+        if (g.wide)
+            hipLaunchKernelGGL(k_ic0_level, dim3((unsigned)((g.b - g.a + BLK - 1) / BLK)), dim3(BLK), 0, s, g.a, g.b,
+                               (const int*)d.fwd_rows, (const double*)P.vals64, (const int64_t*)P.ic0_src,
+                               (const int64_t*)P.ic0_dsrc, (const int*)d.l_ptr, (const int*)d.l_cols, lv, P.ic0_fail);
+        else
+            hipLaunchKernelGGL(k_ic0_sweep, dim3(1), dim3(TRI_BLK), 0, s, g.l1 - g.l0, (const int*)d.fwd_off + g.l0,
+                               (const int*)d.fwd_rows, (const double*)P.vals64, (const int64_t*)P.ic0_src,
+                               (const int64_t*)P.ic0_dsrc, (const int*)d.l_ptr, (const int*)d.l_cols, lv, P.ic0_fail);
+    }
+    if (P.ic0_nl > 0)
+        hipLaunchKernelGGL((k_ic0_store<T>), dim3((unsigned)((P.ic0_nl + BLK - 1) / BLK)), dim3(BLK), 0, s, P.ic0_nl,
+                           (const double*)lv, (const int32_t*)P.ic0_ltsrc, d.l_vals, d.lt_vals);
+    HIPCHK(hipGetLastError());
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, P.ic0_fail, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (bad) return fail(RUV_ERR_PRECOND, "ic0: vanishing pivot or diagonal");
+    return RUV_OK;
+}
+
+int ic0_refactor(PlanImpl& P) {
+    return P.mode == RUV_MODE_MIXED ? ic0_numeric<float>(P, P.tri32) : ic0_numeric<double>(P, P.tri64);
+}
+
+// Jacobi inverse diagonal (precond.rs:595-611) from the host values: a vanishing diagonal is a setup failure.
+int jacobi_upload(PlanImpl& P, const double* vals) {
+    hipStream_t s = P.stream;
+    std::vector<double> dinv((size_t)P.n, 1.0);
+    for (int32_t i = 0; i < P.n; ++i) {
+        const double d = P.jacobi_diag[(size_t)i] >= 0 ? vals[P.jacobi_diag[(size_t)i]] : 0.0;
+        if (std::fabs(d) < 1e-300) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(i));
+        dinv[(size_t)i] = 1.0 / d;
+    }
+    std::vector<float> dinv32;
+    if (!P.dinv64) HIPCHK(P.mem.alloc(&P.dinv64, dinv.size()));
+    HIPCHK(hipMemcpyAsync(P.dinv64, dinv.data(), sizeof(double) * dinv.size(), hipMemcpyHostToDevice, s));
+    if (P.mode == RUV_MODE_MIXED) {
+        dinv32.assign(dinv.begin(), dinv.end());
+        if (!P.dinv32) HIPCHK(P.mem.alloc(&P.dinv32, dinv32.size()));
+        HIPCHK(hipMemcpyAsync(P.dinv32, dinv32.data(), sizeof(float) * dinv32.size(), hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return RUV_OK;
+}
+
 int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
     const char* ng = std::getenv("RUV_NO_GRAPH");  // direct launches instead of graph replay (A/B checks)
     P.use_graph = !(ng && ng[0] == '1');
@@ -268,35 +331,42 @@ int plan_setup(PlanImpl& P, const Csr& a, int R, int precond, int mode) {
         HIPCHK(P.mem.upload(&P.vals32, v32.data(), v32.size(), s));
     }
     const size_t N = (size_t)a.n * R;
-    // Jacobi inverse diagonal (precond.rs:595-611): a vanishing diagonal is a setup failure.
-    std::vector<double> dinv(a.n, 1.0);
-    std::vector<float> dinv32;
     if (precond == RUV_PC_JACOBI) {
-        for (int32_t i = 0; i < a.n; ++i) {
-            double d = a.get(i, i);
-            if (std::fabs(d) < 1e-300) return fail(RUV_ERR_PRECOND, "jacobi: zero diagonal at row " + std::to_string(i));
-            dinv[i] = 1.0 / d;
-        }
-        HIPCHK(P.mem.upload(&P.dinv64, dinv.data(), dinv.size(), s));
-        if (mode == RUV_MODE_MIXED) {
-            dinv32.assign(dinv.begin(), dinv.end());
-            HIPCHK(P.mem.upload(&P.dinv32, dinv32.data(), dinv32.size(), s));
-        }
+        P.jacobi_diag.assign((size_t)a.n, -1);
+        for (int32_t i = 0; i < a.n; ++i)
+            for (int32_t k = a.ptr[i]; k < a.ptr[i + 1]; ++k)
+                if (a.cols[k] == i) {  // the first entry, as Csr::get
+                    P.jacobi_diag[(size_t)i] = k;
+                    break;
+                }
+        int rc = jacobi_upload(P, a.vals.data());
+        if (rc) return rc;
     } else if (precond == RUV_PC_IC0 || precond == RUV_PC_IC0COLOR) {
+        // the structure on the host, the values on the device (a values update refactors there)
         TriFactor f;
+        std::vector<int64_t> src, dsrc;
+        std::vector<int32_t> lt_src;
         if (precond == RUV_PC_IC0) {
-            if (!factor_ic0(a, &f)) return fail(RUV_ERR_PRECOND, "ic0: vanishing pivot or diagonal");
+            ic0_pattern(a, nullptr, &f, &src, &dsrc, &lt_src);
         } else {
             std::vector<uint32_t> color = graph_coloring(a);
             std::vector<int32_t> to_perm, to_orig;
             color_permutation(color, &to_perm, &to_orig);
-            Csr pa = permute_csr(a, to_perm, to_orig);
-            if (!factor_ic0(pa, &f)) return fail(RUV_ERR_PRECOND, "ic0color: vanishing pivot or diagonal");
+            const Csr pa = permute_csr(a, to_perm, to_orig);
+            const std::vector<int64_t> val_of = permuted_positions(a, to_perm, to_orig);
+            ic0_pattern(pa, &val_of, &f, &src, &dsrc, &lt_src);
             HIPCHK(P.mem.upload(&P.to_orig, to_orig.data(), to_orig.size(), s));
             HIPCHK(P.mem.upload(&P.to_perm, to_perm.data(), to_perm.size(), s));
             P.permuted = true;
         }
+        P.ic0_nl = (int64_t)lt_src.size();
+        HIPCHK(P.mem.upload(&P.ic0_src, src.data(), src.size(), s));
+        HIPCHK(P.mem.upload(&P.ic0_dsrc, dsrc.data(), dsrc.size(), s));
+        HIPCHK(P.mem.upload(&P.ic0_ltsrc, lt_src.data(), lt_src.size(), s));
+        HIPCHK(P.mem.alloc(&P.ic0_fail, 1));
+        if (mode == RUV_MODE_MIXED) HIPCHK(P.mem.alloc(&P.ic0_work, (size_t)P.ic0_nl));
         int rc = (mode == RUV_MODE_MIXED) ? upload_tri(P.mem, f, s, &P.tri32) : upload_tri(P.mem, f, s, &P.tri64);
+        if (!rc) rc = ic0_refactor(P);
         if (rc) return rc;
         P.has_tri = true;
     } else if (precond == RUV_PC_AMG || precond == RUV_PC_AMGSA) {
@@ -557,9 +627,12 @@ int solve_fp64(PlanImpl& P, const double* d_b, double* d_x, uint64_t max_iter, d
     HIPCHK(hipEventRecord(P.ev0, s));
     uint64_t iters = 0;
     int done = 0;
-    int rc = run_pcg<double, R>(P, P.vals64, P.dinv64, d_x, P.r64, P.z64, P.p64, P.q64, P.y64, P.t64,
+    // the iterate lives in the plan's own x64: the replayed graph's kernels keep the pointers of its capture, and
+    // the caller's d_x may differ from one solve to the next
+    int rc = run_pcg<double, R>(P, P.vals64, P.dinv64, P.x64, P.r64, P.z64, P.p64, P.q64, P.y64, P.t64,
                                 max_iter, tol, &iters, &done);
     if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(d_x, P.x64, sizeof(double) * (size_t)P.n * R, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipEventRecord(P.ev1, s));
     HIPCHK(hipEventSynchronize(P.ev1));
     float ms = 0;
@@ -660,6 +733,44 @@ int plan_create(const Csr& a, int R, int precond, int mode, PlanImpl** out) {
 }
 
 void plan_destroy(PlanImpl* p) { delete p; }
+
+// The IC(0) factor of a (natural order) as the device computes it: L's rows (strict-lower entries ascending, the
+// diagonal last).
+int ic0_factor_device(const Csr& a, std::vector<int32_t>* ptr, std::vector<int32_t>* cols, std::vector<double>* vals) {
+    PlanImpl* P = nullptr;
+    int rc = plan_create(a, 1, RUV_PC_IC0, RUV_MODE_FP64, &P);
+    if (rc) return rc;
+    std::unique_ptr<PlanImpl> hold(P);
+    const DevTri<double>& d = P->tri64;
+    const size_t nl = (size_t)P->ic0_nl;
+    ptr->resize((size_t)d.n + 1);
+    cols->resize(nl);
+    vals->resize(nl);
+    HIPCHK(hipMemcpyAsync(ptr->data(), d.l_ptr, sizeof(int32_t) * ptr->size(), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(cols->data(), d.l_cols, sizeof(int32_t) * nl, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(vals->data(), d.l_vals, sizeof(double) * nl, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return RUV_OK;
+}
+
+int plan_update_values(PlanImpl* P, const double* vals) {
+    if (P->precond == RUV_PC_AMG || P->precond == RUV_PC_AMGSA)
+        return fail(RUV_ERR_INVALID, "the amg preconditioners are built on the host: create a new plan");
+    hipStream_t s = P->stream;
+    const size_t nz = (size_t)P->nnz;
+    HIPCHK(hipStreamSynchronize(s));  // no solve of this plan is in flight
+    HIPCHK(hipMemcpyAsync(P->vals64, vals, sizeof(double) * nz, hipMemcpyHostToDevice, s));
+    std::vector<float> v32;
+    if (P->mode == RUV_MODE_MIXED) {
+        v32.assign(vals, vals + nz);
+        HIPCHK(hipMemcpyAsync(P->vals32, v32.data(), sizeof(float) * nz, hipMemcpyHostToDevice, s));
+    }
+    int rc = RUV_OK;
+    if (P->precond == RUV_PC_JACOBI) rc = jacobi_upload(*P, vals);
+    else if (P->has_tri) rc = ic0_refactor(*P);
+    HIPCHK(hipStreamSynchronize(s));
+    return rc;
+}
 
 int plan_solve_dev(PlanImpl* P, const double* d_b, double* d_x, uint64_t max_iter, double tol, ruv_solve_info* info) {
     ruv_solve_info tmp{};

@@ -322,4 +322,80 @@ __global__ __launch_bounds__(BLK) void k_residual(int n, const int* __restrict__
     if (threadIdx.x == 0) part_rr[blockIdx.x] = bs;
 }
 
+// ---------------------------------------------------------------- IC(0) factorisation on the device
+// Row i of the IC(0) factor (factor_ic0, precond.rs:145-219) in FP64 from A's values: for each strict-lower
+// entry (ascending column k) L(i,k) = (A(i,k) - sum_{j<k} L(i,j) L(k,j)) / L(k,k), the products subtracted in
+// ascending j (a merge of row i's entries so far with row k's); then L(i,i) = sqrt(max(A(i,i) - sum L(i,k)^2,
+// 1e-12 max(|A(i,i)|, 1e-30))).  Rows k < i are those of earlier levels.  The reference's operations in its order
+// and without fused multiply-adds, so the factor is the host's to the bit.  A vanishing pivot or diagonal sets
+// *fail (the host factorisation stops there; here the row goes on, and the caller discards the factor).
+__device__ __forceinline__ void ic0_row(int i, const double* __restrict__ av, const int64_t* __restrict__ src,
+                                        const int64_t* __restrict__ dsrc, const int* __restrict__ ptr,
+                                        const int* __restrict__ cols, double* lv, int* fail) {
+#pragma clang fp contract(off)
+    const int start = ptr[i], e = ptr[i + 1] - 1;
+    for (int q = start; q < e; ++q) {
+        const int k = cols[q];
+        double s = av[src[q]];
+        int qk = ptr[k];
+        const int qke = ptr[k + 1] - 1;  // row k's diagonal
+        for (int qi = start; qi < q; ++qi) {
+            const int j = cols[qi];
+            while (qk < qke && cols[qk] < j) ++qk;
+            if (qk < qke && cols[qk] == j) s -= lv[qi] * lv[qk];
+        }
+        const double lkk = lv[qke];
+        if (fabs(lkk) < 1e-300) {
+            atomicOr(fail, 1);
+            lv[q] = 0.0;
+            continue;
+        }
+        lv[q] = s / lkk;
+    }
+    const double aii = dsrc[i] >= 0 ? av[dsrc[i]] : 0.0;
+    if (fabs(aii) < 1e-300) {
+        atomicOr(fail, 1);
+        lv[e] = 1.0;
+        return;
+    }
+    double sum = aii;
+    for (int q = start; q < e; ++q) sum -= lv[q] * lv[q];
+    const double fl = 1e-12 * fmax(fabs(aii), 1e-30);
+    if (sum < fl) sum = fl;
+    lv[e] = sqrt(sum);
+}
+
+// a run of narrow levels in one workgroup (the levels' rows one per thread, a barrier between levels)
+__global__ __launch_bounds__(TRI_BLK) void k_ic0_sweep(int nlev, const int* __restrict__ off,
+                                                       const int* __restrict__ rows, const double* __restrict__ av,
+                                                       const int64_t* __restrict__ src, const int64_t* __restrict__ dsrc,
+                                                       const int* __restrict__ ptr, const int* __restrict__ cols,
+                                                       double* lv, int* fail) {
+    for (int l = 0; l < nlev; ++l) {
+        for (int t = off[l] + (int)threadIdx.x; t < off[l + 1]; t += TRI_BLK)
+            ic0_row(rows[t], av, src, dsrc, ptr, cols, lv, fail);
+        __syncthreads();
+    }
+}
+
+// one wide level over many workgroups: rows rows[a .. b)
+__global__ __launch_bounds__(BLK) void k_ic0_level(int a, int b, const int* __restrict__ rows,
+                                                   const double* __restrict__ av, const int64_t* __restrict__ src,
+                                                   const int64_t* __restrict__ dsrc, const int* __restrict__ ptr,
+                                                   const int* __restrict__ cols, double* lv, int* fail) {
+    const int t = a + (int)(blockIdx.x * BLK + threadIdx.x);
+    if (t < b) ic0_row(rows[t], av, src, dsrc, ptr, cols, lv, fail);
+}
+
+// the factor in the sweeps' precision: L (unless it is the FP64 workspace itself) and L^T (lt_src: each L^T entry's
+// L entry)
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_ic0_store(int64_t nl, const double* lv, const int32_t* __restrict__ lt_src,
+                                                   T* l_out, T* __restrict__ lt_out) {
+    const int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (q >= nl) return;
+    if ((const void*)l_out != (const void*)lv) l_out[q] = (T)lv[q];
+    lt_out[q] = (T)lv[lt_src[q]];
+}
+
 }  // namespace ruv

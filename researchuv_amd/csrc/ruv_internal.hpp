@@ -35,8 +35,11 @@ struct TriFactor {
     std::vector<int32_t> bwd_rows, bwd_off;  // over L^T, descending
 };
 
-// factor_ic0 (precond.rs:145-219); false on a vanishing pivot / diagonal.
-bool factor_ic0(const Csr& a, TriFactor* f);
+// factor_ic0's structure (precond.rs:145-219, the values are computed on the device): the L and L^T patterns and
+// levels, each strict-lower L entry's and each diagonal's position in the values of A (through val_of), and the
+// L entry of every L^T entry.
+void ic0_pattern(const Csr& a, const std::vector<int64_t>* val_of, TriFactor* f, std::vector<int64_t>* src,
+                 std::vector<int64_t>* dsrc, std::vector<int32_t>* lt_src);
 // tri_levels (precond.rs:227-249) flattened as (rows, offsets)
 void tri_levels(const Csr& l, bool descending, std::vector<int32_t>* rows, std::vector<int32_t>* off);
 // transpose_lower (precond.rs:253-290)
@@ -45,6 +48,9 @@ Csr transpose_lower(const Csr& l);
 std::vector<uint32_t> graph_coloring(const Csr& a);
 void color_permutation(const std::vector<uint32_t>& color, std::vector<int32_t>* to_perm, std::vector<int32_t>* to_orig);
 Csr permute_csr(const Csr& a, const std::vector<int32_t>& to_perm, const std::vector<int32_t>& to_orig);
+// the position in a's values of every entry of permute_csr(a, to_perm, to_orig)
+std::vector<int64_t> permuted_positions(const Csr& a, const std::vector<int32_t>& to_perm,
+                                        const std::vector<int32_t>& to_orig);
 
 // AMG hierarchy (amg.rs:127-455), host side.
 struct AmgHostLevel {

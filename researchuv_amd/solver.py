@@ -164,6 +164,12 @@ class Plan:
                                               C.byref(info)))
         return info
 
+    def update_values(self, vals):
+        """New values for the plan's pattern (the CSR order of creation): operator and Jacobi / IC(0)
+        preconditioner refreshed on the device."""
+        vals = np.ascontiguousarray(vals, np.float64)
+        _lib.check(self._L.ruv_plan_update_values(self.h, _p(vals)))
+
     def close(self):
         if self.h:
             self._L.ruv_plan_destroy(self.h)
@@ -174,6 +180,20 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def ic0_factor(a: CsrMatrix) -> dict:
+    """The IC(0) factor of `a` as the device factorisation computes it (factor_ic0, precond.rs:145-219): L's rows,
+    strict-lower entries ascending and the diagonal last."""
+    L = _lib.lib()
+    nl = C.c_int64(0)
+    args = (_p(a.vals), _p(a.cols), _p(a.row_ptr), a.rows(), len(a.vals))
+    _lib.check(L.ruv_ic0_factor(*args, C.byref(nl), None, None, None))
+    ptr = np.zeros(a.rows() + 1, np.int32)
+    cols = np.zeros(max(nl.value, 1), np.int32)
+    vals = np.zeros(max(nl.value, 1))
+    _lib.check(L.ruv_ic0_factor(*args, C.byref(nl), _p(ptr), _p(cols), _p(vals)))
+    return dict(l_ptr=ptr, l_cols=cols[:nl.value], l_vals=vals[:nl.value])
 
 
 class AmgHost:
