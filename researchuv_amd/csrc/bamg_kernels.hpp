@@ -275,33 +275,40 @@ __global__ __launch_bounds__(BLK) void k_roots(int64_t n, const int64_t* __restr
     flag[i] = (mate[i] >= 0) ? (i < mate[i] ? 1 : 0) : (join[i] < 0 ? 1 : 0);
 }
 
+// agg[i] = the aggregate of row i (the scan position cid of its root: the lower row of its pair, the pair it
+// joined, or itself), and the aggregates' sizes into cnt (zeroed before): each root adds its pair (1 or 2), each
+// row that joined a pair adds 1 -- one atomic per aggregate and per joined row, not per row
 __global__ __launch_bounds__(BLK) void k_root_agg(int64_t n, const int64_t* __restrict__ mate,
                                                   const int64_t* __restrict__ join, const int64_t* __restrict__ cid,
-                                                  int32_t* __restrict__ agg) {
+                                                  int32_t* __restrict__ agg, unsigned long long* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const int64_t root = (mate[i] >= 0) ? min(i, mate[i]) : (join[i] >= 0 ? join[i] : i);
-    agg[i] = (int32_t)cid[root];
+    const int64_t m = mate[i], jn = (m >= 0) ? -1 : join[i];
+    const int64_t root = (m >= 0) ? min(i, m) : (jn >= 0 ? jn : i);
+    const int64_t a = cid[root];
+    agg[i] = (int32_t)a;
+    if (root == i) atomicAdd(&cnt[a], (m >= 0) ? 2ull : 1ull);
+    else if (jn >= 0) atomicAdd(&cnt[a], 1ull);
 }
 
-// member lists of aggregates from a row -> aggregate map: counts, (scan), slots, then each list sorted
-__global__ __launch_bounds__(BLK) void k_member_count(int64_t n, const int32_t* __restrict__ agg,
-                                                      unsigned long long* __restrict__ cnt) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[agg[i]], 1ull);
-}
-
-__global__ __launch_bounds__(BLK) void k_member_fill(int64_t n, const int32_t* __restrict__ agg,
+// member lists of a matching's aggregates: a pair's rows at its first two slots (lower row first), rows that
+// joined it after them (slot order by an atomic cursor, zeroed before; k_member_sort orders the lists)
+__global__ __launch_bounds__(BLK) void k_member_fill(int64_t n, const int64_t* __restrict__ mate,
+                                                     const int64_t* __restrict__ join, const int32_t* __restrict__ agg,
                                                      const int64_t* __restrict__ mptr,
                                                      unsigned long long* __restrict__ cursor,
                                                      int64_t* __restrict__ mrows) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const int64_t a = agg[i];
-    const unsigned long long k = atomicAdd(&cursor[a], 1ull);
-    mrows[mptr[a] + (int64_t)k] = i;
+    const int64_t a = agg[i], m = mate[i];
+    int64_t slot;
+    if (m >= 0) slot = (i < m) ? 0 : 1;
+    else if (join[i] >= 0) slot = 2 + (int64_t)atomicAdd(&cursor[a], 1ull);
+    else slot = 0;
+    mrows[mptr[a] + slot] = i;
 }
 
+// each aggregate's member list sorted ascending; with group, the aggregate's group (that of its rows)
 __global__ __launch_bounds__(BLK) void k_member_sort(int64_t nc, const int64_t* __restrict__ mptr,
                                                      int64_t* __restrict__ mrows, const int32_t* __restrict__ group,
                                                      int32_t* __restrict__ group_c) {
@@ -417,6 +424,10 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
 // a slot is the number of smaller columns.  A row with more distinct columns goes to `over` (counted by nover)
 // for the next stage: 16 slots over all rows, then 32 over the rows listed by the first, then k_galerkin (the
 // search and the ranking cost grow with the slots, and most coarse rows have at most 16 distinct columns).
+// Without FILL a row is counted and, with tcols given, staged: its columns and values in column order at
+// tcols / tvals[rank * nrow + x] (slot-major over the stage's rows) and its diagonal into cdiag, so that
+// k_galerkin_unstage writes it out once the row offsets are known instead of a second pass recomputing it; a row
+// that does not fit is marked tcols[x] = -1.
 constexpr int GR_SLOTS_FIRST = 16;
 constexpr int GR_SLOTS = 32;
 constexpr int GR_BATCH = 8;  // entries whose loads are issued together
@@ -429,7 +440,8 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
                                                       const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
                                                       float* __restrict__ cvals, double* __restrict__ cdiag,
                                                       unsigned long long* __restrict__ nover,
-                                                      int64_t* __restrict__ over) {
+                                                      int64_t* __restrict__ over, int32_t* __restrict__ tcols = nullptr,
+                                                      float* __restrict__ tvals = nullptr) {
     const int64_t x = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (x >= nrow) return;
     const int64_t I = list ? list[x] : x;
@@ -491,14 +503,15 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
         if (!FILL) {
             cnt[I] = 0;
             over[atomicAdd(nover, 1ull)] = I;
+            if (tcols) tcols[x] = -1;
         }
         return;
     }
     if (!FILL) {
         cnt[I] = n;
-        return;
+        if (!tcols) return;
     }
-    const int64_t out = cptr[I];
+    const int64_t out = FILL ? cptr[I] : 0;
     double diag = 0.0;
 #pragma unroll
     for (int k = 0; k < SL; ++k) {
@@ -506,12 +519,33 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
             int rank = 0;
 #pragma unroll
             for (int t = 0; t < SL; ++t) rank += (key[t] < key[k]) ? 1 : 0;
-            ccols[out + rank] = key[k];
-            cvals[out + rank] = (float)val[k];
+            if (FILL) {
+                ccols[out + rank] = key[k];
+                cvals[out + rank] = (float)val[k];
+            } else {
+                tcols[(int64_t)rank * nrow + x] = key[k];
+                tvals[(int64_t)rank * nrow + x] = (float)val[k];
+            }
             if (key[k] == I) diag = val[k];
         }
     }
     cdiag[I] = diag;
+}
+
+// the staged rows of a k_galerkin_reg stage (rows list[x], or x) out to the coarse CSR at their offsets
+__global__ __launch_bounds__(BLK) void k_galerkin_unstage(int64_t nrow, const int64_t* __restrict__ list,
+                                                          const int32_t* __restrict__ tcols,
+                                                          const float* __restrict__ tvals,
+                                                          const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
+                                                          float* __restrict__ cvals) {
+    const int64_t x = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    if (x >= nrow || tcols[x] < 0) return;  // not staged: a later stage writes the row
+    const int64_t I = list ? list[x] : x;
+    const int64_t out = cptr[I], m = cptr[I + 1] - out;
+    for (int64_t k = 0; k < m; ++k) {
+        ccols[out + k] = tcols[k * nrow + x];
+        cvals[out + k] = tvals[k * nrow + x];
+    }
 }
 
 // first and last chart of every BLK rows of a level

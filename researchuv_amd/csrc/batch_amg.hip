@@ -129,30 +129,6 @@ int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) 
     return RUV_OK;
 }
 
-// Member lists (ascending, CSR over rows) of the nc aggregates of a row -> aggregate map; with group given,
-// also each aggregate's group.
-template <typename Alloc>
-int build_members(hipStream_t s, int64_t n, const int32_t* agg, int64_t nc, const int32_t* group, Alloc& S,
-                  int64_t** mptr, int64_t** mrows, int32_t* group_c) {
-    unsigned long long *cnt = nullptr, *cursor = nullptr;
-    Scratch T(S.pool);
-    HIPCHK(T.alloc(&cnt, nc));
-    HIPCHK(T.alloc(&cursor, nc));
-    HIPCHK(S.alloc(mptr, nc + 1));
-    HIPCHK(S.alloc(mrows, n));
-    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * nc, s));
-    HIPCHK(hipMemsetAsync(cursor, 0, sizeof(unsigned long long) * nc, s));
-    hipLaunchKernelGGL(k_member_count, rows_grid(n), dim3(BLK), 0, s, n, agg, cnt);
-    HIPCHK(hipGetLastError());
-    int rc = scan(s, nc, (const int64_t*)cnt, *mptr, T);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_member_fill, rows_grid(n), dim3(BLK), 0, s, n, agg, (const int64_t*)*mptr, cursor, *mrows);
-    hipLaunchKernelGGL(k_member_sort, rows_grid(nc), dim3(BLK), 0, s, nc, (const int64_t*)*mptr, *mrows, group,
-                       group_c);
-    HIPCHK(hipGetLastError());
-    return RUV_OK;  // the counters go back to the pool; their next user is ordered after these kernels
-}
-
 // One matching of A within groups (pairs, then lone rows joining pairs) and its Galerkin operator.
 struct Matched {
     int64_t nc = 0;
@@ -202,35 +178,59 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(hipStreamSynchronize(s));
     std::memcpy(&M->gersh, &hg, sizeof(double));
     const int64_t nc = M->nc;
-    hipLaunchKernelGGL(k_root_agg, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join,
-                       (const int64_t*)M->cid, M->agg);
-    HIPCHK(hipGetLastError());
-    HIPCHK(S.alloc(&M->group, nc));
-    rc = build_members(s, n, M->agg, nc, group, S, &M->mptr, &M->mrows, M->group);
-    if (rc) return rc;
+    {  // aggregate map, sizes, member lists (ascending) and each aggregate's group
+        unsigned long long *cnt = nullptr, *cursor = nullptr;
+        Scratch T(S.pool);
+        HIPCHK(T.alloc(&cnt, nc));
+        HIPCHK(T.alloc(&cursor, nc));
+        HIPCHK(S.alloc(&M->group, nc));
+        HIPCHK(S.alloc(&M->mptr, nc + 1));
+        HIPCHK(S.alloc(&M->mrows, n));
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * nc, s));
+        HIPCHK(hipMemsetAsync(cursor, 0, sizeof(unsigned long long) * nc, s));
+        hipLaunchKernelGGL(k_root_agg, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate, (const int64_t*)join,
+                           (const int64_t*)M->cid, M->agg, cnt);
+        HIPCHK(hipGetLastError());
+        rc = scan(s, nc, (const int64_t*)cnt, M->mptr, T);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_member_fill, rows_grid(n), dim3(BLK), 0, s, n, (const int64_t*)mate,
+                           (const int64_t*)join, (const int32_t*)M->agg, (const int64_t*)M->mptr, cursor, M->mrows);
+        hipLaunchKernelGGL(k_member_sort, rows_grid(nc), dim3(BLK), 0, s, nc, (const int64_t*)M->mptr, M->mrows,
+                           group, M->group);
+        HIPCHK(hipGetLastError());
+    }  // the counters go back to the pool; their next user is ordered after these kernels
     // Galerkin operator: rows of few distinct columns in registers (16 slots, then 32 for the rows listed by the
-    // first pass), the others (listed again) through LDS
+    // first pass), the others (listed again) through LDS; the register passes stage their rows while counting
+    // them, and the staged rows are written out once the offsets are known
     int64_t *cnt = nullptr, *cptr = nullptr, *over1 = nullptr, *over2 = nullptr;
     unsigned long long* nover = nullptr;
+    int32_t *t1c = nullptr, *t2c = nullptr;
+    float *t1v = nullptr, *t2v = nullptr;
     HIPCHK(S.alloc(&cnt, nc));
     HIPCHK(S.alloc(&cptr, nc + 1));
     HIPCHK(S.alloc(&over1, nc));
     HIPCHK(S.alloc(&over2, nc));
     HIPCHK(S.alloc(&nover, 2));
+    HIPCHK(S.alloc(&M->cdiag, nc));
+    Scratch TS(S.pool);  // staging, back to the pool when this level is built
+    HIPCHK(TS.alloc(&t1c, (size_t)GR_SLOTS_FIRST * nc));
+    HIPCHK(TS.alloc(&t1v, (size_t)GR_SLOTS_FIRST * nc));
     HIPCHK(hipMemsetAsync(nover, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long hover[2] = {0, 0};
     hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
                        (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
                        (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
-                       (double*)nullptr, nover, over1);
+                       M->cdiag, nover, over1, t1c, t1v);
     HIPCHK(hipMemcpyAsync(&hover[0], nover, sizeof(hover[0]), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const int64_t n1 = (int64_t)hover[0];
     if (n1 > 0) {
+        HIPCHK(TS.alloc(&t2c, (size_t)GR_SLOTS * n1));
+        HIPCHK(TS.alloc(&t2v, (size_t)GR_SLOTS * n1));
         hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
                            (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
                            (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
-                           (double*)nullptr, nover + 1, over2);
+                           M->cdiag, nover + 1, over2, t2c, t2v);
         HIPCHK(hipMemcpyAsync(&hover[1], nover + 1, sizeof(hover[1]), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
@@ -251,16 +251,11 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     float* cvals = nullptr;
     HIPCHK(S.alloc(&ccols, cnnz));
     HIPCHK(S.alloc(&cvals, cnnz));
-    HIPCHK(S.alloc(&M->cdiag, nc));
-    hipLaunchKernelGGL((k_galerkin_reg<true, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
-                       (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                       (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr, ccols, cvals, M->cdiag,
-                       (unsigned long long*)nullptr, (int64_t*)nullptr);
+    hipLaunchKernelGGL(k_galerkin_unstage, rows_grid(nc), dim3(BLK), 0, s, nc, (const int64_t*)nullptr,
+                       (const int32_t*)t1c, (const float*)t1v, (const int64_t*)cptr, ccols, cvals);
     if (n1 > 0)
-        hipLaunchKernelGGL((k_galerkin_reg<true, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
-                           (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                           (const int32_t*)M->agg, (int64_t*)nullptr, (const int64_t*)cptr, ccols, cvals, M->cdiag,
-                           (unsigned long long*)nullptr, (int64_t*)nullptr);
+        hipLaunchKernelGGL(k_galerkin_unstage, rows_grid(n1), dim3(BLK), 0, s, n1, (const int64_t*)over1,
+                           (const int32_t*)t2c, (const float*)t2v, (const int64_t*)cptr, ccols, cvals);
     if (no > 0)
         hipLaunchKernelGGL((k_galerkin<true>), go, dim3(GAL_BLK), 0, s, no, A, (const int64_t*)M->mptr,
                            (const int64_t*)M->mrows, (const int32_t*)M->agg, (int64_t*)nullptr,
