@@ -420,28 +420,26 @@ __global__ __launch_bounds__(GAL_BLK) void k_galerkin(int64_t nc, SetupCsr A, co
 }
 
 // The same Galerkin rows, one thread a row with the distinct columns in registers (up to SL of them): each
-// entry adds into the slot of its column, in row order, so the sums equal k_galerkin's; the output position of
-// a slot is the number of smaller columns.  A row with more distinct columns goes to `over` (counted by nover)
-// for the next stage: 16 slots over all rows, then 32 over the rows listed by the first, then k_galerkin (the
-// search and the ranking cost grow with the slots, and most coarse rows have at most 16 distinct columns).
-// Without FILL a row is counted and, with tcols given, staged: its columns and values in column order at
-// tcols / tvals[rank * nrow + x] (slot-major over the stage's rows) and its diagonal into cdiag, so that
-// k_galerkin_unstage writes it out once the row offsets are known instead of a second pass recomputing it; a row
-// that does not fit is marked tcols[x] = -1.
+// entry adds into the slot of its column, in row order, so the sums equal k_galerkin's.  A row that fits is counted
+// and staged in its slots' order (the order in which its columns first appear: the coarse rows are not sorted, and
+// nothing that reads them needs it) at tcols / tvals[k * nrow + x], slot-major over the stage's rows, and its
+// diagonal goes to cdiag; k_galerkin_unstage writes it out once the row offsets are known.  A row with more
+// distinct columns is marked tcols[x] = -1 and listed in `over` (counted by nover) for the next stage: 16 slots
+// over all rows, then 32 over the rows listed by the first, then k_galerkin (the search grows with the slots, and
+// most coarse rows have at most 16 distinct columns).
 constexpr int GR_SLOTS_FIRST = 16;
 constexpr int GR_SLOTS = 32;
 constexpr int GR_BATCH = 8;  // entries whose loads are issued together
 
-template <bool FILL, int SL>
-__global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_t* __restrict__ list, SetupCsr A,
-                                                      const int64_t* __restrict__ mptr,
-                                                      const int64_t* __restrict__ mrows,
-                                                      const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
-                                                      const int64_t* __restrict__ cptr, int32_t* __restrict__ ccols,
-                                                      float* __restrict__ cvals, double* __restrict__ cdiag,
-                                                      unsigned long long* __restrict__ nover,
-                                                      int64_t* __restrict__ over, int32_t* __restrict__ tcols = nullptr,
-                                                      float* __restrict__ tvals = nullptr) {
+template <int SL>
+__global__ __launch_bounds__(BLK) void k_galerkin_stage(int64_t nrow, const int64_t* __restrict__ list, SetupCsr A,
+                                                        const int64_t* __restrict__ mptr,
+                                                        const int64_t* __restrict__ mrows,
+                                                        const int32_t* __restrict__ agg, int64_t* __restrict__ cnt,
+                                                        double* __restrict__ cdiag,
+                                                        unsigned long long* __restrict__ nover,
+                                                        int64_t* __restrict__ over, int32_t* __restrict__ tcols,
+                                                        float* __restrict__ tvals) {
     const int64_t x = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (x >= nrow) return;
     const int64_t I = list ? list[x] : x;
@@ -455,7 +453,7 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
     int n = 0;
     bool full = false;
     const int64_t m1 = mptr[I + 1];
-    for (int64_t q = mptr[I]; q < m1; ++q) {
+    for (int64_t q = mptr[I]; q < m1 && !full; ++q) {
         const int64_t r = mrows[q];
         const int64_t e1 = A.ptr[r + 1];
         // entries in batches of GR_BATCH: the column loads, then the aggregate lookups, are issued together
@@ -500,39 +498,25 @@ __global__ __launch_bounds__(BLK) void k_galerkin_reg(int64_t nrow, const int64_
         }
     }
     if (full) {
-        if (!FILL) {
-            cnt[I] = 0;
-            over[atomicAdd(nover, 1ull)] = I;
-            if (tcols) tcols[x] = -1;
-        }
+        cnt[I] = 0;
+        over[atomicAdd(nover, 1ull)] = I;
+        tcols[x] = -1;
         return;
     }
-    if (!FILL) {
-        cnt[I] = n;
-        if (!tcols) return;
-    }
-    const int64_t out = FILL ? cptr[I] : 0;
+    cnt[I] = n;
     double diag = 0.0;
 #pragma unroll
     for (int k = 0; k < SL; ++k) {
         if (k < n) {
-            int rank = 0;
-#pragma unroll
-            for (int t = 0; t < SL; ++t) rank += (key[t] < key[k]) ? 1 : 0;
-            if (FILL) {
-                ccols[out + rank] = key[k];
-                cvals[out + rank] = (float)val[k];
-            } else {
-                tcols[(int64_t)rank * nrow + x] = key[k];
-                tvals[(int64_t)rank * nrow + x] = (float)val[k];
-            }
+            tcols[(int64_t)k * nrow + x] = key[k];
+            tvals[(int64_t)k * nrow + x] = (float)val[k];
             if (key[k] == I) diag = val[k];
         }
     }
     cdiag[I] = diag;
 }
 
-// the staged rows of a k_galerkin_reg stage (rows list[x], or x) out to the coarse CSR at their offsets
+// the staged rows of a k_galerkin_stage pass (rows list[x], or x) out to the coarse CSR at their offsets
 __global__ __launch_bounds__(BLK) void k_galerkin_unstage(int64_t nrow, const int64_t* __restrict__ list,
                                                           const int32_t* __restrict__ tcols,
                                                           const float* __restrict__ tvals,

@@ -201,7 +201,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     }  // the counters go back to the pool; their next user is ordered after these kernels
     // Galerkin operator: rows of few distinct columns in registers (16 slots, then 32 for the rows listed by the
     // first pass), the others (listed again) through LDS; the register passes stage their rows while counting
-    // them, and the staged rows are written out once the offsets are known
+    // them (columns in first-appearance order), and the staged rows are written out once the offsets are known
     int64_t *cnt = nullptr, *cptr = nullptr, *over1 = nullptr, *over2 = nullptr;
     unsigned long long* nover = nullptr;
     int32_t *t1c = nullptr, *t2c = nullptr;
@@ -217,20 +217,18 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(TS.alloc(&t1v, (size_t)GR_SLOTS_FIRST * nc));
     HIPCHK(hipMemsetAsync(nover, 0, 2 * sizeof(unsigned long long), s));
     unsigned long long hover[2] = {0, 0};
-    hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
+    hipLaunchKernelGGL((k_galerkin_stage<GR_SLOTS_FIRST>), rows_grid(nc), dim3(BLK), 0, s, nc,
                        (const int64_t*)nullptr, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                       (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
-                       M->cdiag, nover, over1, t1c, t1v);
+                       (const int32_t*)M->agg, cnt, M->cdiag, nover, over1, t1c, t1v);
     HIPCHK(hipMemcpyAsync(&hover[0], nover, sizeof(hover[0]), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     const int64_t n1 = (int64_t)hover[0];
     if (n1 > 0) {
         HIPCHK(TS.alloc(&t2c, (size_t)GR_SLOTS * n1));
         HIPCHK(TS.alloc(&t2v, (size_t)GR_SLOTS * n1));
-        hipLaunchKernelGGL((k_galerkin_reg<false, GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
+        hipLaunchKernelGGL((k_galerkin_stage<GR_SLOTS>), rows_grid(n1), dim3(BLK), 0, s, n1,
                            (const int64_t*)over1, A, (const int64_t*)M->mptr, (const int64_t*)M->mrows,
-                           (const int32_t*)M->agg, cnt, (const int64_t*)nullptr, (int32_t*)nullptr, (float*)nullptr,
-                           M->cdiag, nover + 1, over2, t2c, t2v);
+                           (const int32_t*)M->agg, cnt, M->cdiag, nover + 1, over2, t2c, t2v);
         HIPCHK(hipMemcpyAsync(&hover[1], nover + 1, sizeof(hover[1]), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
