@@ -61,6 +61,17 @@ def host_threads():
     return max(1, min(16, os.cpu_count() or 1))
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's record."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline_batch(meshes, budget_verts=750_000, vmax=4096):
     """The CPU oracle's worker-thread path (pipeline.rs:171-194 over exec.rs:28-60, each chart one sparse-LU
     factor+solve, sparse.rs:91-265) on the first patches of the batch with at most `vmax` vertices, up to
@@ -90,7 +101,7 @@ def cpu_baseline_batch(meshes, budget_verts=750_000, vmax=4096):
     t0 = time.perf_counter()
     ch.par_unfold(w, max_iter=1)
     took = time.perf_counter() - t0
-    return dict(value=unk / took, unit="unknowns/s", cores=w, kind="port",
+    return dict(value=unk / took, unit="unknowns/s", cores=w, kind="port", cpu=cpu_model(),
                 sample=f"oracle worker-thread sparse-LU path (one factor+solve per chart) over the first {picked} "
                        f"batch patches with <= {vmax} vertices ({len(ch)} charts, {unk} unknowns), {took:.2f} s "
                        f"on {w} threads; smaller charts than the batch average, so this favours the CPU")
@@ -115,7 +126,7 @@ def cpu_baseline_chart(seconds_target=12.0):
         if took > seconds_target / 4 or side >= 160:
             break
         side = int(side * 1.4)
-    return dict(value=unk / took, unit="unknowns/s", cores=1, kind="port",
+    return dict(value=unk / took, unit="unknowns/s", cores=1, kind="port", cpu=cpu_model(),
                 sample=f"oracle sparse-LU unfold (one factor+solve) of one {side}x{side} height-field chart "
                        f"({unk} unknowns), {took:.2f} s on 1 core")
 
@@ -316,6 +327,14 @@ def run_batch(args):
             f"{uvcheck[name]['energy_rel_err_max']:.2e}")
     log(f"LU checks of {len(pick)} charts ({sizes[largest[-1]]}..{sizes[largest[0]]} rows for the largest 64) "
         f"took {time.perf_counter() - t0:.1f} s")
+    # 12 mid-sized charts for the CPU leg's check against the repository's own FP64 sparse LU (the oracle's
+    # restatement of sparse.rs:91-265, too slow for the largest charts: 33 s at 16 000 rows)
+    mids = [int(c) for c in np.argsort(sizes, kind="stable") if 1000 <= sizes[c] <= 2500]
+    oracle_sample = []
+    for c in mids[:: max(1, len(mids) // 12)][:12]:
+        L, r = block(c)
+        r0 = int(off[c])
+        oracle_sample.append((L, r, sols["mixed"][r0:r0 + L.shape[0]].copy()))
     xa = sols["mixed"]
     uv_err, en_err = uvcheck["mixed"]["uv_rel_err_max"], uvcheck["mixed"]["energy_rel_err_max"]
 
@@ -379,7 +398,34 @@ def run_batch(args):
                "speedup_vs_fp64_amg_batched": value / (unknowns / el64a),
                "speedup_vs_fp64_batched": value / (unknowns / el64),
                "speedup_vs_fp64_dropin_per_chart": value / value_d, **(step_prof or {})},
-        roofline=roof, meshes=meshes)
+        roofline=roof, meshes=meshes, oracle_sample=oracle_sample)
+
+
+def oracle_lu_check(sample):
+    """Part of the CPU leg: the mixed batch's solution of each sampled chart, and SciPy's SuperLU (which the bench's
+    checks of the largest charts use), against the oracle's FP64 sparse LU (sparse.rs:91-265), the UV measure of
+    lscm.rs:749-758."""
+    import numpy as np
+    import scipy.sparse.linalg as spla
+
+    import oracle
+
+    def one(item):
+        L, r, xg = item
+        xo = np.stack([oracle.csr_sparse_solve(L.data, L.indices, L.indptr, r[:, k].copy()) for k in range(2)], 1)
+        xs = spla.splu(L.tocsc()).solve(r)
+        den = np.linalg.norm(xo, axis=1).max()
+        return np.linalg.norm(xg - xo, axis=1).max() / den, np.linalg.norm(xs - xo, axis=1).max() / den
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(host_threads()) as ex:
+        res = list(ex.map(one, sample))
+    out = {"charts": len(sample), "rows": [int(min(x[0].shape[0] for x in sample)), int(max(x[0].shape[0] for x in sample))],
+           "mixed_uv_rel_err_max": float(max(e[0] for e in res)),
+           "superlu_vs_oracle_uv_rel_diff_max": float(max(e[1] for e in res)),
+           "seconds": time.perf_counter() - t0}
+    log(f"vs the oracle's sparse LU: {out}")
+    return out
 
 
 def build_single(kind, side):
@@ -681,7 +727,7 @@ def cpu_baseline_metrics(sample, budget=200_000_000):
         nt += len(tris[k])
         k += 1
     sec = time.perf_counter() - t0
-    return dict(value=nt / sec, unit="triangles/s", cores=1, kind="port",
+    return dict(value=nt / sec, unit="triangles/s", cores=1, kind="port", cpu=cpu_model(),
                 sample=f"oracle chart_distortion + rectangularize over the first {k} charts ({nt} triangles), "
                        f"{sec:.2f} s on one thread")
 
@@ -736,6 +782,8 @@ def main():
         else:
             log("cpu baseline (oracle sparse LU)")
             cpu = cpu_baseline_batch(out["meshes"]) if out["meshes"] is not None else cpu_baseline_chart()
+            if out.get("oracle_sample"):
+                cpu["oracle_lu_check"] = oracle_lu_check(out["oracle_sample"])
         log(f"cpu: {cpu['value']:.0f} {cpu['unit']} ({cpu['sample']})")
     result = {
         "metric": out.get("metric", METRIC),
