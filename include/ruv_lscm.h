@@ -203,6 +203,12 @@ int ruv_batch_spmv_time(ruv_batch* batch, int32_t reps, double* seconds_per_laun
  * all slices; 1 when the step kernel loads the values with the non-temporal hint.  Any pointer may be null. */
 int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* col_bytes, int64_t* ell_entries,
                        int64_t* col_entries, int32_t* stencil_slices, int32_t* slices, int32_t* nontemporal);
+/* Charts the batch also keeps in band form -- free rows forming a grid of width W in row-major order (column
+ * offsets 0, +-1, +-(W - 1), +-W, +-(W + 1) only) with an exactly symmetric operator -- and the values that form
+ * stores (the diagonal and the upper offsets present, one slot per row each, zeros included).  The step's SpMV
+ * and the AMG's finest smoothing passes stream that form on those charts (RUV_BATCH_BAND=0 at creation turns it
+ * off).  Either pointer may be null.  No counterpart in the reference (its CSR SpMV stores every nonzero). */
+int ruv_batch_band_info(const ruv_batch* batch, int32_t* band_charts, int64_t* band_values);
 /* Step-kernel accounting over the solves that follow (measurement hook): with enable != 0 every launch of the
  * SpMV kernel is bracketed by HIP events on the batch's stream.  ruv_batch_kernel_stats returns, and resets,
  * the launch count, the summed device time of those launches, their algorithmic bytes and their bytes as

@@ -50,6 +50,8 @@ def _setup(L):
     L.ruv_batch_step_profile.restype = C.c_int
     L.ruv_batch_step_profile.argtypes = [P, C.c_int32, C.c_int32, P, P, P, P, P, P]
     L.ruv_batch_schedule.argtypes = [P, P, P, P, P, P, P, P]
+    L.ruv_batch_band_info.restype = C.c_int
+    L.ruv_batch_band_info.argtypes = [P, P, P]
     L.ruv_batch_destroy.restype = None
     L.ruv_batch_destroy.argtypes = [P]
     L.ruv_lscm_batch_build.restype = C.c_int
@@ -146,6 +148,13 @@ class Batch:
                                             C.byref(st), C.byref(ns), C.byref(nt)))
         return dict(rows_per_tile=rows.value, col_bytes=cb.value, ell_entries=ne.value, col_entries=nc.value,
                     stencil_slices=st.value, slices=ns.value, nontemporal=nt.value)
+
+    def band_info(self) -> dict:
+        """Charts kept in band form (grid charts with a symmetric operator) and the values that form stores."""
+        nb = C.c_int32(0)
+        nv = C.c_int64(0)
+        _lib.check(lib().ruv_batch_band_info(self.h, C.byref(nb), C.byref(nv)))
+        return dict(band_charts=nb.value, band_values=nv.value)
 
     def setup_precond(self):
         """AMG: rebuild the hierarchy on the device (no-op for the element-wise preconditioners)."""

@@ -33,6 +33,8 @@ int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int
 int batch_setup_precond(BatchImpl* Q);
 int batch_precond_levels(BatchImpl* Q, int64_t* rows, int32_t cap);
 int batch_precond_fallbacks(BatchImpl* Q, int64_t* not_spd, int64_t* coupled, int64_t* too_large);
+int batch_band_info(const BatchImpl* Q, int32_t* band_charts, int64_t* band_values);
+hipStream_t batch_stream(const BatchImpl* Q);
 int batch_coarsest_block(BatchImpl* Q, int c, int64_t* m_out, double* block, double* inverse);
 int batch_amg_level(BatchImpl* Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* cols, double* vals, int32_t* agg,
                     int32_t* chart);
@@ -455,14 +457,16 @@ int ruv_batch_solve(ruv_batch* batch, const double* b, double* x, uint64_t max_i
                     ruv_solve_info* info, uint64_t* chart_iters, double* chart_true_resid) {
     if (!batch || !b || !x) return ruv::fail(RUV_ERR_INVALID, "null argument");
     const size_t N = (size_t)batch->n;
+    const hipStream_t s = ruv::batch_stream(batch->impl);  // the copies on the batch's own stream
     double *db = nullptr, *dx = nullptr;
     hipError_t e = hipMalloc((void**)&db, N * sizeof(double));
     if (e == hipSuccess) e = hipMalloc((void**)&dx, N * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpy(db, b, N * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpyAsync(db, b, N * sizeof(double), hipMemcpyHostToDevice, s);
     int rc = (e == hipSuccess) ? ruv_batch_solve_dev(batch, db, dx, max_iter, tol, info, chart_iters, chart_true_resid)
                                : ruv::fail(RUV_ERR_DEVICE, hipGetErrorString(e));
     if (rc == RUV_OK) {
-        e = hipMemcpy(x, dx, N * sizeof(double), hipMemcpyDeviceToHost);
+        e = hipMemcpyAsync(x, dx, N * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) rc = ruv::fail(RUV_ERR_DEVICE, hipGetErrorString(e));
     }
     if (db) (void)hipFree(db);
@@ -484,6 +488,11 @@ int ruv_batch_schedule(const ruv_batch* batch, int32_t* rows_per_tile, int32_t* 
     if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
     return ruv::batch_schedule(batch->impl, rows_per_tile, col_bytes, ell_entries, col_entries, stencil_slices,
                                slices, nontemporal);
+}
+
+int ruv_batch_band_info(const ruv_batch* batch, int32_t* band_charts, int64_t* band_values) {
+    if (!batch) return ruv::fail(RUV_ERR_INVALID, "null argument");
+    return ruv::batch_band_info(batch->impl, band_charts, band_values);
 }
 
 int ruv_batch_precond_fallbacks(ruv_batch* batch, int64_t* not_spd_charts, int64_t* coupled_charts,

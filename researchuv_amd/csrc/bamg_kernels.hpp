@@ -1110,11 +1110,13 @@ __global__ __launch_bounds__(BLK) void k_coarsest_chol(SetupCsr A, const int64_t
     }
 }
 
-// Finest level (the batch's sliced-ELL operator), per live tile.  b is the PCG residual r.
+// Finest level (the batch's sliced-ELL operator, or the band form of a band chart), per live tile.  b is the PCG
+// residual r.
 //   down: t = r - A (w D^-1 r) for the tile's rows; kc_restrict then sums t over each aggregate.
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__ act, int na,
                                                  const typename CoarseVal<T>::type* __restrict__ vals,
+                                                 const typename CoarseVal<T>::type* __restrict__ bvals,
                                                  const T* __restrict__ dinv, T w,
                                                  const T* __restrict__ r, T* __restrict__ t_out) {
     const int slot = xcd_slot(na);
@@ -1137,6 +1139,11 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
         for (int cc = 0; cc < R; ++cc) o.c[cc] = ri.c[cc] - acc[cc];
         st<T, R>(t_out, i, o);
     };
+    if (B.c_bw && B.c_bw[c] > 0) {
+        const int t0 = B.chart_t0[c];
+        band_rows<T, R, 0>(B, c, t - t0, B.chart_t0[c + 1] - t0, bvals, g, fin);
+        return;
+    }
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
         if (s + 1 < s1 && spmv_group<2, T, R, 0>(B, s, lo, hi, vals, g, fin)) {
@@ -1180,6 +1187,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict
 template <typename T, int R, int CI>
 __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ act, int na, int par,
                                                 const typename CoarseVal<T>::type* __restrict__ vals,
+                                                const typename CoarseVal<T>::type* __restrict__ bvals,
                                                 const T* __restrict__ dinv, T w,
                                                 const T* __restrict__ r, const T* __restrict__ e, T* __restrict__ z) {
     __shared__ double sh[WAVES];
@@ -1203,6 +1211,13 @@ __global__ __launch_bounds__(BLK) void kc_post0(BDev B, const int* __restrict__ 
         }
         st<T, R>(z, i, o);
     };
+    if (B.c_bw && B.c_bw[c] > 0) {
+        const int t0 = B.chart_t0[c];
+        band_rows<T, R, 0>(B, c, t - t0, B.chart_t0[c + 1] - t0, bvals, g, fin);
+        const double srz = block_sum_all(arz, sh);
+        if (threadIdx.x == 0) B.part_c[(int64_t)par * B.ntile + t] = srz;
+        return;
+    }
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
         if (s + 1 < s1 && spmv_group<2, T, R, 0>(B, s, lo, hi, vals, g, fin)) {

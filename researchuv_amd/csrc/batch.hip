@@ -131,6 +131,130 @@ int build_tiles(BatchImpl& Q, int spt) {
     return RUV_OK;
 }
 
+// Band form of a chart (batch_kernels.hpp): the chart's positive column offsets, at most four, fit
+// {1, W - 1, W, W + 1} for one W >= 3, and the operator is exactly symmetric inside it.  Returns W (0: no) and the
+// offset mask.
+int band_of(int64_t lo, int64_t hi, const int64_t* ptr, const int32_t* cols, const double* vals, int* mask) {
+    int64_t ds[4];
+    int nd = 0;
+    int64_t upper = 0, lower = 0;
+    for (int64_t i = lo; i < hi; ++i)
+        for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+            const int64_t d = (int64_t)cols[a] - i;
+            if (cols[a] < lo || cols[a] >= hi) return 0;
+            if (d < 0) ++lower;
+            if (d <= 0) continue;
+            ++upper;
+            int k = 0;
+            while (k < nd && ds[k] != d) ++k;
+            if (k == nd) {
+                if (nd == 4) return 0;
+                ds[nd++] = d;
+            }
+        }
+    if (nd == 0 || upper != lower) return 0;
+    const int64_t dmax = *std::max_element(ds, ds + nd);
+    int W = 0, mk = 0;
+    for (const int64_t w : {dmax - 1, dmax, dmax + 1}) {
+        if (w < 3 || w > (int64_t)INT32_MAX / 16) continue;
+        int m = 0;
+        bool fits = true;
+        for (int k = 0; k < nd; ++k) {
+            const int64_t d = ds[k];
+            if (d == 1) m |= 1;
+            else if (d == w - 1) m |= 2;
+            else if (d == w) m |= 4;
+            else if (d == w + 1) m |= 8;
+            else fits = false;
+        }
+        if (fits) {
+            W = (int)w;
+            mk = m;
+            break;
+        }
+    }
+    if (!W) return 0;
+    // every upper entry (i, j) has its mirror (j, i) with the same value; with as many lower entries as upper ones,
+    // the lower triangle is exactly the mirror of the upper
+    for (int64_t i = lo; i < hi; ++i)
+        for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+            const int64_t j = cols[a];
+            if (j <= i) continue;
+            int64_t b = ptr[j];
+            while (b < ptr[j + 1] && cols[b] != (int32_t)i) ++b;
+            if (b == ptr[j + 1] || !(vals[b] == vals[a])) return 0;
+        }
+    *mask = mk;
+    return W;
+}
+
+// slot of offset d >= 0 in a band chart of width w and offset mask m (0: the diagonal)
+inline int band_slot(int64_t d, int w, int m) {
+    if (d == 0) return 0;
+    const int bit = d == 1 ? 0 : d == w - 1 ? 1 : d == w ? 2 : 3;
+    return 1 + __builtin_popcount((unsigned)(m & ((1 << bit) - 1)));
+}
+
+// Band form of every band chart: widths, masks, value offsets, and the values (the mode's precision) uploaded.
+// RUV_BATCH_BAND=0 keeps every chart in the sliced-ELL form.
+int band_setup(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const double* vals) {
+    const int nchart = Q.nchart;
+    Q.h_bw.assign(nchart, 0);
+    Q.h_bmask.assign(nchart, 0);
+    Q.h_bbase.assign(nchart + 1, 0);
+    const char* env = std::getenv("RUV_BATCH_BAND");
+    if (env && std::atoi(env) == 0) return RUV_OK;
+    parallel_for(nchart, [&](int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1; ++c) {
+            int m = 0;
+            const int w = band_of(Q.crow[c], Q.crow[c + 1], ptr, cols, vals, &m);
+            Q.h_bw[c] = w;
+            Q.h_bmask[c] = w ? m : 0;
+        }
+    });
+    Q.nband = 0;
+    for (int c = 0; c < nchart; ++c) {
+        const int64_t n = Q.crow[c + 1] - Q.crow[c];
+        Q.h_bbase[c + 1] = Q.h_bbase[c] + (Q.h_bw[c] ? (1 + __builtin_popcount((unsigned)Q.h_bmask[c])) * n : 0);
+        Q.nband += Q.h_bw[c] ? 1 : 0;
+    }
+    Q.nbv = Q.h_bbase[nchart];
+    if (Q.nband == 0) return RUV_OK;
+    std::vector<double> bv((size_t)Q.nbv, 0.0);
+    parallel_for(nchart, [&](int64_t c0, int64_t c1) {
+        for (int64_t c = c0; c < c1; ++c) {
+            const int w = Q.h_bw[c], m = Q.h_bmask[c];
+            if (!w) continue;
+            const int64_t lo = Q.crow[c], n = Q.crow[c + 1] - lo;
+            for (int64_t i = lo; i < lo + n; ++i)
+                for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+                    const int64_t d = (int64_t)cols[a] - i;
+                    if (d >= 0) bv[Q.h_bbase[c] + (int64_t)band_slot(d, w, m) * n + (i - lo)] += vals[a];
+                }
+        }
+    });
+    if (Q.mode == RUV_MODE_MIXED) {
+        std::vector<float> f((size_t)Q.nbv);
+        parallel_for(Q.nbv, [&](int64_t a, int64_t b) {
+            for (int64_t k = a; k < b; ++k) f[k] = (float)bv[k];
+        });
+        HIPCHK(Q.upload(&Q.bv32, f.data(), f.size()));
+        HIPCHK(hipStreamSynchronize(Q.stream));
+    } else {
+        HIPCHK(Q.upload(&Q.bv64, bv.data(), bv.size()));
+        HIPCHK(hipStreamSynchronize(Q.stream));
+    }
+    int32_t *d_bw = nullptr, *d_bm = nullptr;
+    int64_t* d_bb = nullptr;
+    HIPCHK(Q.upload(&d_bw, Q.h_bw.data(), Q.h_bw.size()));
+    HIPCHK(Q.upload(&d_bm, Q.h_bmask.data(), Q.h_bmask.size()));
+    HIPCHK(Q.upload(&d_bb, Q.h_bbase.data(), Q.h_bbase.size()));
+    Q.B.c_bw = d_bw;
+    Q.B.c_bmask = d_bm;
+    Q.B.c_bbase = d_bb;
+    return RUV_OK;
+}
+
 int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* cols, const double* vals,
                 int nchart, const int64_t* crow, int R, int precond, int mode) {
     Q.R = R;
@@ -154,6 +278,8 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     }
     Q.chart_s0[nchart] = (int)Q.h_sl_row0.size();
     const int nsl = Q.nslice = (int)Q.h_sl_row0.size();
+    int rc = band_setup(Q, ptr, cols, vals);
+    if (rc) return rc;
     // classify: a stencil slice has at most SLOTS distinct column offsets (col - row); its slot k holds the
     // entry at offset dtab[k] of every row (or 0), so it stores no column indices and its gathers coalesce.
     // Any other slice is padded to its longest row and stores a column (offset) per entry.
@@ -212,16 +338,24 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
         const double sv = mode == RUV_MODE_MIXED ? 4.0 : 8.0, sc = narrow ? 2.0 : 4.0;
         Q.chart_step_bytes.assign(nchart, 0.0);
         Q.chart_step_stored.assign(nchart, 0.0);
-        Q.fin_colb = Q.fin_colb_st = Q.fin_meta = 0.0;
+        Q.fin_colb = Q.fin_colb_st = Q.fin_meta = Q.fin_slots = 0.0;
         for (int c = 0; c < nchart; ++c) {
             const double vec = 2.0 * (double)(crow[c + 1] - crow[c]) * R * sv;  // p' read, q written
             double b = vec, st = vec;
+            if (Q.h_bw[c]) {  // band form: the nonzeros' values, as stored its slots (mirrored entries once)
+                const double slots = (double)(Q.h_bbase[c + 1] - Q.h_bbase[c]);
+                Q.chart_step_bytes[c] = b + (double)(ptr[crow[c + 1]] - ptr[crow[c]]) * sv;
+                Q.chart_step_stored[c] = st + slots * sv;
+                Q.fin_slots += slots;
+                continue;
+            }
             for (int s = Q.chart_s0[c]; s < Q.chart_s0[c + 1]; ++s) {
                 const int64_t r0 = Q.h_sl_row0[s];
                 const double nz = (double)(ptr[r0 + Q.h_sl_n[s]] - ptr[r0]);
                 const double meta = 32.0 + (stencil[s] ? 4.0 * SLOTS : 0.0);
                 b += nz * (stencil[s] ? sv : sv + sc) + meta;
                 st += (double)sl_w[s] * Q.h_sl_n[s] * (stencil[s] ? sv : sv + sc) + meta;
+                Q.fin_slots += (double)sl_w[s] * Q.h_sl_n[s];
                 Q.fin_meta += meta;
                 if (!stencil[s]) {
                     Q.fin_colb += nz * sc;
@@ -377,12 +511,13 @@ inline dim3 grid_of(int na) { return dim3((unsigned)(((na + XCDS - 1) / XCDS) * 
 template <typename T, int R, int CI, int PROBE = 0>
 void launch_spmv(BatchImpl& Q, int na, const T* p, T* q) {
     const T* vals = (sizeof(T) == 4) ? (const T*)(const void*)Q.ev32 : (const T*)(const void*)Q.ev64;
+    const T* bvals = (sizeof(T) == 4) ? (const T*)(const void*)Q.bv32 : (const T*)(const void*)Q.bv64;
     if (Q.nt)
         hipLaunchKernelGGL((kb_spmv<T, R, CI, 1, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
-                           (const int*)Q.act, na, vals, p, q);
+                           (const int*)Q.act, na, vals, bvals, p, q);
     else
         hipLaunchKernelGGL((kb_spmv<T, R, CI, 0, PROBE>), grid_of(na), dim3(BLK), 0, Q.stream, Q.B,
-                           (const int*)Q.act, na, vals, p, q);
+                           (const int*)Q.act, na, vals, bvals, p, q);
 }
 
 // One inner PCG over the charts not yet final.  cont: go on from the previous inner solve's last direction (the
@@ -876,9 +1011,21 @@ int batch_precond_info(BatchImpl* Q, int32_t* levels, double* setup_seconds, int
     return rc;
 }
 
-// Position of every CSR entry of the batch's operator in its value arrays (the storage form of batch_setup).
+// Position of every CSR entry of the batch's operator in its value arrays (the storage form of batch_setup): the
+// sliced-ELL position at map[a], and at map[nnz + a] the band position (-1 outside band charts and for the lower
+// entries, which the band form keeps as their mirrors).
 int batch_value_map(const BatchImpl& Q, const int64_t* ptr, const int32_t* cols, std::vector<int64_t>* map) {
-    map->assign((size_t)Q.nnz, -1);
+    map->assign(2 * (size_t)Q.nnz, -1);
+    for (int c = 0; c < Q.nchart; ++c) {
+        const int w = Q.h_bw[c];
+        if (!w) continue;
+        const int64_t lo = Q.crow[c], n = Q.crow[c + 1] - lo;
+        for (int64_t i = lo; i < lo + n; ++i)
+            for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
+                const int64_t d = (int64_t)cols[a] - i;
+                if (d >= 0) (*map)[Q.nnz + a] = Q.h_bbase[c] + (int64_t)band_slot(d, w, Q.h_bmask[c]) * n + (i - lo);
+            }
+    }
     for (int s = 0; s < Q.nslice; ++s) {
         const int64_t r0 = Q.h_sl_row0[s];
         const int n = Q.h_sl_n[s];
@@ -904,12 +1051,18 @@ namespace {
 // one CSR entry except with duplicate columns, which the assembly never produces) and the inverse diagonal
 __global__ __launch_bounds__(BLK) void kb_scatter_values(int64_t nnz, const double* __restrict__ v,
                                                          const int64_t* __restrict__ map, double* __restrict__ e64,
-                                                         float* __restrict__ e32) {
+                                                         float* __restrict__ e32, double* __restrict__ b64,
+                                                         float* __restrict__ b32) {
     const int64_t a = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (a >= nnz) return;
     const int64_t e = map[a];
     e64[e] = v[a];
     if (e32) e32[e] = (float)v[a];
+    const int64_t b = map[nnz + a];  // band form: upper entries of band charts
+    if (b >= 0) {
+        if (b64) b64[b] = v[a];
+        if (b32) b32[b] = (float)v[a];
+    }
 }
 __global__ __launch_bounds__(BLK) void kb_inv_diag(int64_t n, const double* __restrict__ v,
                                                    const int64_t* __restrict__ diagpos, double* __restrict__ d64,
@@ -922,12 +1075,14 @@ __global__ __launch_bounds__(BLK) void kb_inv_diag(int64_t n, const double* __re
 }
 }  // namespace
 
-// Replace the operator's values (same structure, CSR order, device) and rebuild what depends on them.
+// Replace the operator's values (same structure, CSR order, device) and rebuild what depends on them.  The band
+// form keeps the upper entries of its charts, so the new values must stay symmetric (the batch's operator is SPD).
 int batch_update_values(BatchImpl* Q, const double* d_vals, const int64_t* d_map, const int64_t* d_diagpos) {
     hipStream_t s = Q->stream;
     const dim3 ge((unsigned)std::max<int64_t>(1, (Q->nnz + BLK - 1) / BLK));
     const dim3 gr((unsigned)std::max<int64_t>(1, (Q->nrows + BLK - 1) / BLK));
-    hipLaunchKernelGGL(kb_scatter_values, ge, dim3(BLK), 0, s, Q->nnz, d_vals, d_map, Q->ev64, Q->ev32);
+    hipLaunchKernelGGL(kb_scatter_values, ge, dim3(BLK), 0, s, Q->nnz, d_vals, d_map, Q->ev64, Q->ev32, Q->bv64,
+                       Q->bv32);
     if (Q->dinv64)
         hipLaunchKernelGGL(kb_inv_diag, gr, dim3(BLK), 0, s, Q->nrows, d_vals, d_diagpos, Q->dinv64, Q->dinv32);
     HIPCHK(hipGetLastError());
@@ -957,6 +1112,14 @@ int batch_kernel_stats(BatchImpl* Q, int64_t* launches, double* seconds, double*
     Q->k_seconds = 0.0;
     Q->k_bytes = 0.0;
     Q->k_stored = 0.0;
+    return RUV_OK;
+}
+
+hipStream_t batch_stream(const BatchImpl* Q) { return Q->stream; }
+
+int batch_band_info(const BatchImpl* Q, int32_t* band_charts, int64_t* band_values) {
+    if (band_charts) *band_charts = Q->nband;
+    if (band_values) *band_values = Q->nbv;
     return RUV_OK;
 }
 

@@ -42,6 +42,7 @@ struct BamgLevel {
 struct Bamg {
     std::vector<BamgLevel> lv;  // lv.back() is the coarsest
     void* ev0 = nullptr;        // the finest operator's values for the smoothing legs (CoarseVal<T>::type)
+    void* bv0 = nullptr;        // ... and their band form (band charts)
     double* cinv = nullptr;     // each chart's inverted block of the coarsest operator, FP64 (null: Jacobi there)
     int64_t* cinv_off = nullptr;  // per chart: its block's first entry, -1 for a chart kept on its diagonal
     int64_t* crow_c = nullptr;  // chart row offsets on the coarsest level
@@ -622,10 +623,18 @@ int bamg_build(BatchImpl& Q) {
             Bf16* e16 = nullptr;
             HIPCHK(A->alloc(&e16, (size_t)Q.nell));
             hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
-            HIPCHK(hipGetLastError());
             A->ev0 = e16;
+            A->bv0 = nullptr;
+            if (Q.nbv) {
+                Bf16* b16 = nullptr;
+                HIPCHK(A->alloc(&b16, (size_t)Q.nbv));
+                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nbv), dim3(BLK), 0, s, Q.nbv, (const float*)Q.bv32, b16);
+                A->bv0 = b16;
+            }
+            HIPCHK(hipGetLastError());
         } else {
             A->ev0 = Q.ev64;
+            A->bv0 = Q.bv64;
         }
         char* t = nullptr;
         HIPCHK(A->alloc(&t, L.n * R * sv));
@@ -735,19 +744,20 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     BamgLevel& L1 = A.lv[1];
     using VT = typename CoarseVal<T>::type;
     const VT* vals = (const VT*)A.ev0;
+    const VT* bvals = (const VT*)A.bv0;
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
     const T w0 = (T)L0.omega;
     const bool prof = Q.marks != nullptr;
     const double vb = (double)R * sizeof(T), db = (double)sizeof(T), cv = (double)sizeof(VT);
     const double n0 = (double)L0.n, n1 = (double)L1.n;
     const double fin = (double)Q.nnz * cv + Q.fin_colb + Q.fin_meta;
-    const double fin_st = (double)Q.nell * cv + Q.fin_colb_st + Q.fin_meta;
+    const double fin_st = Q.fin_slots * cv + Q.fin_colb_st + Q.fin_meta;
     auto mark = [&](int kind, int level, double bytes, double stored) -> int {
         HIPCHK(Q.mark(kind, level, bytes, stored));
         return RUV_OK;
     };
     int rc = RUV_OK;
-    hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals,
+    hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals, bvals,
                        (const T*)L0.dinv, w0, r, (T*)L0.t);
     if (prof) {  // r, dinv read, t written
         const double v = n0 * (2.0 * vb + db);
@@ -841,7 +851,7 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         const double v = n0 * (2.0 * vb + db + 4.0) + n1 * vb;
         if ((rc = mark(BK_PROLONG0, 0, v, v))) return rc;
     }
-    hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals,
+    hipLaunchKernelGGL((kc_post0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, par, vals, bvals,
                        (const T*)L0.dinv, w0, r, (const T*)L0.t, z);
     if (prof) {  // r, e, dinv read, z written, a partial per tile
         const double v = n0 * (3.0 * vb + db) + 8.0 * Q.ntile;

@@ -90,6 +90,15 @@ struct BatchImpl {
     // matrix
     float* ev32 = nullptr;
     double* ev64 = nullptr;
+    // band form of the band charts (batch_kernels.hpp): per chart grid width (0: not in band form), offset mask and
+    // first value (nchart + 1); nbv values in FP32 (bv32, mixed) or FP64 (bv64)
+    std::vector<int32_t> h_bw, h_bmask;
+    std::vector<int64_t> h_bbase;
+    int nband = 0;
+    int64_t nbv = 0;
+    float* bv32 = nullptr;
+    double* bv64 = nullptr;
+    double fin_slots = 0.0;  // value slots a finest-operator pass streams: ELL slots of the other charts, band slots
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     // vectors

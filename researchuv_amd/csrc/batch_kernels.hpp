@@ -14,6 +14,18 @@
 //            (value 0, the row itself).
 // A tile is a run of slices of one chart; one workgroup owns one tile.
 //
+// Band charts.  A chart whose free-vertex rows form a grid of width W in row-major order -- every column offset
+// is one of 0, +-1, +-(W - 1), +-W, +-(W + 1), and the operator is exactly symmetric -- is also stored in band
+// form for the kernels that stream the finest operator once per step (the SpMV and the two finest smoothing
+// passes): only the diagonal and the upper offsets present in the chart (1, W - 1, W, W + 1, in that order;
+// bit b of the chart's mask says whether offset b is present), each as one slot-major column of the chart's n
+// rows, value 0 where a row has no entry.  Row i's entry at offset -d is row (i - d)'s entry at +d.  A workgroup
+// thread owns the rows r0, r0 + W, ..., r0 + (H - 1) W of a band of H = BAND_H grid rows and
+// walks down them with a three-row window of the gathered vector (rows r - W, r, r + W, each at columns -1, 0,
+// +1), so each row gathers three new vector elements that lie in the cache lines of its own row, and the lower
+// entry at -W is the upper entry at +W of the thread's previous row.  The chart's band columns are dealt to its
+// tiles in chunks of 256, chunk k to tile k, k + ntile, ...
+//
 // Iteration.  With z = M r (Jacobi) the reference recurrence (solver.rs:405-552)
 //     alpha = rz / pAp;  x += alpha p;  r -= alpha q;  z = M r;  beta = rz' / rz;  p = z + beta p
 // needs two chart-wide reductions per step, so it runs as three kernels:
@@ -38,6 +50,7 @@ namespace ruv {
 
 constexpr int SLICE = BLK;  // rows per slice (one per thread)
 constexpr int SLOTS = 8;    // column offsets of a stencil slice
+constexpr int BAND_H = 4;   // band charts: grid rows per thread
 constexpr int XCDS = 8;     // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
 enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4 };
@@ -87,6 +100,11 @@ struct BDev {
     uint32_t max_iter;
     int32_t nchart;
     int32_t ntile;
+    // band charts (null when the batch has none): grid width W (0: the chart is not in band form), offset mask,
+    // first value of the chart in the band arrays
+    const int32_t* c_bw;
+    const int32_t* c_bmask;
+    const int64_t* c_bbase;
 };
 
 template <typename T, int R>
@@ -276,6 +294,97 @@ __device__ __forceinline__ int group_width(const BDev& B, int s, int64_t lo, int
     return w;
 }
 
+// Band form (see the header): the chart's band columns of chunk k, k + ntl, ... of this workgroup; for each of
+// its rows, acc = sum over the row's entries of a_ij * g(j), then fin(row, acc).  NT: non-temporal upper values.
+// M: the chart's offset mask when known at compile time, so a row issues all its loads before any is waited for
+// (a load inside a branch is waited for inside it); M = 0 takes the mask at run time and loads every slot, an
+// absent one from the diagonal's slot, weighted 0.
+template <int H, int M, typename T, int R, int NT, typename V, typename G, typename F>
+__device__ __forceinline__ void band_rows_hm(int64_t lo, int n, int W, int m, const V* __restrict__ v0, int k,
+                                             int ntl, G g, F fin) {
+    const int mk = M ? M : m;
+    int slot = 1;
+    const V* v1 = (mk & 1) ? v0 + (int64_t)(slot++) * n : v0;   // +1
+    const V* vwm = (mk & 2) ? v0 + (int64_t)(slot++) * n : v0;  // +(W - 1)
+    const V* vw = (mk & 4) ? v0 + (int64_t)(slot++) * n : v0;   // +W
+    const V* vwp = (mk & 8) ? v0 + (int64_t)(slot++) * n : v0;  // +(W + 1)
+    // present offsets: compile-time with M, else weights 1 / 0 on every slot
+    constexpr bool ALL = M == 0;
+    const T f1 = (mk & 1) ? T(1) : T(0), fwm = (mk & 2) ? T(1) : T(0), fw = (mk & 4) ? T(1) : T(0),
+            fwp = (mk & 8) ? T(1) : T(0);
+    const auto gx = [&](int r) { return g(lo + (int64_t)min(max(r, 0), n - 1)); };
+    const auto low = [&](const V* a, int r) { return r >= 0 ? (T)cval(a[r]) : T(0); };  // r < n always
+    const auto up = [&](const V* a, int r) { return (T)ld_val(a + r, NT); };
+    const int hw = H * W;
+    const int64_t ncol = (int64_t)((n + hw - 1) / hw) * W;
+    const int nchunk = (int)((ncol + BLK - 1) / BLK);
+    for (int ch = k; ch < nchunk; ch += ntl) {
+        const int gi = ch * BLK + (int)threadIdx.x;
+        const int band = gi / W, j = gi - band * W;
+        const int r0 = band * hw + j;  // >= n for the columns past the chart's last band
+        if (r0 >= n) continue;
+        Vr<T, R> xm0 = gx(r0 - W - 1), xm1 = gx(r0 - W), xm2 = gx(r0 - W + 1);
+        Vr<T, R> xc0 = gx(r0 - 1), xc1 = gx(r0), xc2 = gx(r0 + 1);
+        T aw_prev = (ALL || (M & 4)) ? fw * low(vw, r0 - W) : T(0);
+        // row by row: each row's loads issue together; its store keeps the next row's loads behind it (the
+        // register use of hoisting them costs more occupancy than their latency)
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int r = r0 + h * W;
+            if (r >= n) break;
+            T acc[R];
+            const Vr<T, R> xn0 = gx(r + W - 1), xn1 = gx(r + W), xn2 = gx(r + W + 1);
+            const T a0 = up(v0, r);
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) acc[cc] = a0 * xc1.c[cc];
+            if (ALL || (M & 1)) {
+                const T u = f1 * up(v1, r), l = f1 * low(v1, r - 1);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc[cc] = fma(l, xc0.c[cc], fma(u, xc2.c[cc], acc[cc]));
+            }
+            if (ALL || (M & 2)) {
+                const T u = fwm * up(vwm, r), l = fwm * low(vwm, r - W + 1);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc[cc] = fma(l, xm2.c[cc], fma(u, xn0.c[cc], acc[cc]));
+            }
+            if (ALL || (M & 4)) {
+                const T u = fw * up(vw, r);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc[cc] = fma(aw_prev, xm1.c[cc], fma(u, xn1.c[cc], acc[cc]));
+                aw_prev = u;
+            }
+            if (ALL || (M & 8)) {
+                const T u = fwp * up(vwp, r), l = fwp * low(vwp, r - W - 1);
+#pragma unroll
+                for (int cc = 0; cc < R; ++cc) acc[cc] = fma(l, xm0.c[cc], fma(u, xn2.c[cc], acc[cc]));
+            }
+            xm0 = xc0;
+            xm1 = xc1;
+            xm2 = xc2;
+            xc0 = xn0;
+            xc1 = xn1;
+            xc2 = xn2;
+            fin(lo + (int64_t)r, acc);
+        }
+    }
+}
+// band_rows_hm with BAND_H grid rows per thread, the masks of triangulated grids (diagonals one way: 1, W, W + 1
+// or the other: 1, W - 1, W; both: all four) at compile time
+template <typename T, int R, int NT, typename V, typename G, typename F>
+__device__ __forceinline__ void band_rows(const BDev& B, int c, int k, int ntl, const V* __restrict__ bvals, G g,
+                                          F fin) {
+    const int64_t lo = B.crow[c];
+    const int n = (int)(B.crow[c + 1] - lo);
+    const int W = B.c_bw[c], m = B.c_bmask[c];
+    const V* v0 = bvals + B.c_bbase[c];
+    switch (m) {
+        case 13: band_rows_hm<BAND_H, 13, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
+        case 7: band_rows_hm<BAND_H, 7, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
+        case 15: band_rows_hm<BAND_H, 15, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
+        default: band_rows_hm<BAND_H, 0, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
+    }
+}
+
 // Opens step `iter` (parity par) of every live chart -- convergence on ||r||, max_iter, breakdown, beta from the
 // r.z partials -- then p' = z + beta p over the tile's rows.
 template <typename T, int R>
@@ -364,8 +473,8 @@ __device__ __forceinline__ bool spmv_group(const BDev& B, int s, int64_t lo, int
 // whole-batch timing launches.
 template <typename T, int R, int CI, int NT, int PROBE = 0>
 __global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ act, int na,
-                                               const T* __restrict__ vals, const T* __restrict__ p,
-                                               T* __restrict__ q) {
+                                               const T* __restrict__ vals, const T* __restrict__ bvals,
+                                               const T* __restrict__ p, T* __restrict__ q) {
     __shared__ double sh[WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
@@ -385,6 +494,13 @@ __global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ a
         }
         st<T, R>(q, i, qv);
     };
+    if (B.c_bw && B.c_bw[c] > 0) {
+        const int t0 = B.chart_t0[c];
+        band_rows<T, R, NT>(B, c, t - t0, B.chart_t0[c + 1] - t0, bvals, gp, finish);
+        const double sum = block_sum_all(acc, sh);
+        if (threadIdx.x == 0) B.part_a[t] = sum;
+        return;
+    }
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
         if (s + 1 < s1 && spmv_group<2, T, R, NT>(B, s, lo, hi, vals, gp, finish)) {

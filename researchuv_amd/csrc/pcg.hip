@@ -25,6 +25,8 @@
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "amg_kernels.hpp"
@@ -240,6 +242,14 @@ struct PlanImpl {
     double uv_tol = 1e-7;
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
+    // While a graph of iterations is built: the graph, its last node and the first error (plaunch adds kernel
+    // nodes instead of launching)
+    struct GraphRec {
+        hipGraph_t g = nullptr;
+        hipGraphNode_t last = nullptr;
+        hipError_t err = hipSuccess;
+    };
+    GraphRec* rec = nullptr;
 
     ~PlanImpl() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -253,6 +263,37 @@ struct PlanImpl {
 };
 
 namespace {
+
+// Launch kernel k on the plan's stream, or, while the plan records a graph, append it as a kernel node after the
+// last one (the node keeps a copy of the arguments).  Building the graph node by node instead of capturing the
+// stream keeps the plan clear of stream-capture rules, which other threads' work on the legacy stream would
+// otherwise break (a capture in progress makes such work fail, and fails itself).
+template <typename... KA, typename... A, size_t... I>
+void plaunch_rec(PlanImpl& P, void (*k)(KA...), dim3 grid, dim3 block, std::index_sequence<I...>, A&&... args) {
+    std::tuple<std::decay_t<KA>...> t(std::forward<A>(args)...);
+    void* params[] = {(void*)&std::get<I>(t)...};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(k);
+    kp.gridDim = grid;
+    kp.blockDim = block;
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = params;
+    kp.extra = nullptr;
+    PlanImpl::GraphRec& G = *P.rec;
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&node, G.g, G.last ? &G.last : nullptr, G.last ? 1 : 0, &kp);
+    if (e != hipSuccess) {
+        if (G.err == hipSuccess) G.err = e;
+        return;
+    }
+    G.last = node;
+}
+template <typename... KA, typename... A>
+void plaunch(PlanImpl& P, void (*k)(KA...), dim3 grid, dim3 block, A&&... args) {
+    static_assert(sizeof...(KA) == sizeof...(A), "kernel argument count");
+    if (P.rec) plaunch_rec(P, k, grid, block, std::index_sequence_for<KA...>{}, std::forward<A>(args)...);
+    else hipLaunchKernelGGL(k, grid, block, 0, P.stream, std::forward<A>(args)...);
+}
 
 // The IC(0) factor's values on the device from the plan's FP64 values (k_ic0_sweep / k_ic0_level over the forward
 // levels, then the sweeps' copies); RUV_ERR_PRECOND on a vanishing pivot or diagonal (factor_ic0 returning None,
@@ -431,23 +472,23 @@ void apply_tri(PlanImpl& P, const DevTri<T>& f, const T* r, T* z, T* y, T* t, co
     const T* in = r;
     T* out = z;
     if (P.permuted) {
-        hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_orig, r, t, st);
+        plaunch(P, k_gather<T, R>, dim3(P.G), dim3(BLK), P.n, P.to_orig, r, t, st);
         in = t;
     }
     auto sweep = [&](const std::vector<typename DevTri<T>::Seg>& segs, const int* off, const int* rows,
                      const int* ptr, const int* cols, const T* vals, const T* x, T* o) {
         for (const auto& g : segs) {
             if (g.wide)
-                hipLaunchKernelGGL((k_tri_level<T, R>), dim3((unsigned)((g.b - g.a + BLK - 1) / BLK)), dim3(BLK), 0,
-                                   P.stream, g.a, g.b, rows, ptr, cols, vals, x, o, st);
+                plaunch(P, k_tri_level<T, R>, dim3((unsigned)((g.b - g.a + BLK - 1) / BLK)), dim3(BLK), g.a, g.b, rows,
+                        ptr, cols, vals, x, o, st);
             else
-                hipLaunchKernelGGL((k_tri_sweep<T, R>), dim3(1), dim3(TRI_BLK), 0, P.stream, g.l1 - g.l0, off + g.l0,
-                                   rows, ptr, cols, vals, x, o, st);
+                plaunch(P, k_tri_sweep<T, R>, dim3(1), dim3(TRI_BLK), g.l1 - g.l0, off + g.l0, rows, ptr, cols, vals, x,
+                        o, st);
         }
     };
     sweep(f.fwd_seg, f.fwd_off, f.fwd_rows, f.l_ptr, f.l_cols, f.l_vals, in, y);
     sweep(f.bwd_seg, f.bwd_off, f.bwd_rows, f.lt_ptr, f.lt_cols, f.lt_vals, (const T*)y, P.permuted ? t : out);
-    if (P.permuted) hipLaunchKernelGGL((k_gather<T, R>), dim3(P.G), dim3(BLK), 0, P.stream, P.n, P.to_perm, (const T*)t, out, st);
+    if (P.permuted) plaunch(P, k_gather<T, R>, dim3(P.G), dim3(BLK), P.n, P.to_perm, (const T*)t, out, st);
 }
 
 template <typename T> DevTri<T>& tri_of(PlanImpl& P);
@@ -463,27 +504,26 @@ int grid_rows(int n) { return grid_for(n); }
 template <typename T, int R>
 void amg_cycle(PlanImpl& P, DevAmg<T>& A, size_t l, const T* r, T* e, const int32_t* done) {
     DevAmgLevel<T>& L = A.lv[l];
-    hipStream_t s = P.stream;
     const int g = grid_rows(L.n);
     if (L.coarsest) {
         if (L.dense_inv) {
-            hipLaunchKernelGGL((k_amg_dense<T, R>), dim3((L.n + WAVES - 1) / WAVES), dim3(BLK), 0, s, L.n,
-                               (const double*)L.dense_inv, r, e, done);
+            plaunch(P, k_amg_dense<T, R>, dim3((L.n + WAVES - 1) / WAVES), dim3(BLK), L.n, (const double*)L.dense_inv, r,
+                    e, done);
         } else {
-            hipLaunchKernelGGL((k_amg_diag<T, R>), dim3(g), dim3(BLK), 0, s, L.n, (const T*)L.dinv, r, e, done);
+            plaunch(P, k_amg_diag<T, R>, dim3(g), dim3(BLK), L.n, (const T*)L.dinv, r, e, done);
         }
         return;
     }
     DevAmgLevel<T>& C = A.lv[l + 1];
-    hipLaunchKernelGGL((k_amg_pre<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals,
-                       (const T*)L.dinv, A.omega, r, L.t, done);
-    hipLaunchKernelGGL((k_amg_spmv<T, R>), dim3(grid_rows(L.nc)), dim3(BLK), 0, s, L.nc, L.r_ptr, L.r_cols,
-                       (const T*)L.r_vals, (const T*)L.t, C.rin, done);
+    plaunch(P, k_amg_pre<T, R>, dim3(g), dim3(BLK), L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals, (const T*)L.dinv,
+            A.omega, r, L.t, done);
+    plaunch(P, k_amg_spmv<T, R>, dim3(grid_rows(L.nc)), dim3(BLK), L.nc, L.r_ptr, L.r_cols, (const T*)L.r_vals,
+            (const T*)L.t, C.rin, done);
     amg_cycle<T, R>(P, A, l + 1, C.rin, C.e, done);
-    hipLaunchKernelGGL((k_amg_prolong<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.p_ptr, L.p_cols, (const T*)L.p_vals,
-                       (const T*)C.e, (const T*)L.dinv, A.omega, r, L.e1, done);
-    hipLaunchKernelGGL((k_amg_post<T, R>), dim3(g), dim3(BLK), 0, s, L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals,
-                       (const T*)L.dinv, A.omega, r, (const T*)L.e1, e, done);
+    plaunch(P, k_amg_prolong<T, R>, dim3(g), dim3(BLK), L.n, L.p_ptr, L.p_cols, (const T*)L.p_vals, (const T*)C.e,
+            (const T*)L.dinv, A.omega, r, L.e1, done);
+    plaunch(P, k_amg_post<T, R>, dim3(g), dim3(BLK), L.n, L.a_ptr, L.a_cols, (const T*)L.a_vals, (const T*)L.dinv,
+            A.omega, r, (const T*)L.e1, e, done);
 }
 
 // z = M^-1 r for the preconditioners that are not element-wise (IC(0) variants, AMG)
@@ -520,43 +560,44 @@ int run_pcg(PlanImpl& P, const T* vals, const T* dinv, T* x, T* r, T* z, T* p, T
 
     auto body = [&](int iters) {
         for (int it = 0; it < iters; ++it) {
-            hipLaunchKernelGGL((k_spmv_pq<T, R>), dim3(G), dim3(BLK), 0, s, n, P.ptr, P.cols, vals, (const T*)p, q,
-                               P.part_a, (const double*)P.part_c, G, P.st);
+            plaunch(P, k_spmv_pq<T, R>, dim3(G), dim3(BLK), n, P.ptr, P.cols, vals, (const T*)p, q, P.part_a,
+                    (const double*)P.part_c, G, P.st);
             if (pc == RUV_PC_JACOBI) {
-                hipLaunchKernelGGL((k_update<T, R, 1>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
-                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
-                hipLaunchKernelGGL((k_pupdate<T, R, 1>), dim3(G), dim3(BLK), 0, s, n, (const T*)z, p,
-                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+                plaunch(P, k_update<T, R, 1>, dim3(G), dim3(BLK), n, (const T*)p, (const T*)q, x, r, z, dinv,
+                        (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                plaunch(P, k_pupdate<T, R, 1>, dim3(G), dim3(BLK), n, (const T*)z, p, (const double*)P.part_b,
+                        (const double*)P.part_c, G, P.st);
             } else if (pc == RUV_PC_NONE) {
-                hipLaunchKernelGGL((k_update<T, R, 0>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
-                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
-                hipLaunchKernelGGL((k_pupdate<T, R, 0>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, p,
-                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+                plaunch(P, k_update<T, R, 0>, dim3(G), dim3(BLK), n, (const T*)p, (const T*)q, x, r, z, dinv,
+                        (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                plaunch(P, k_pupdate<T, R, 0>, dim3(G), dim3(BLK), n, (const T*)r, p, (const double*)P.part_b,
+                        (const double*)P.part_c, G, P.st);
             } else {
-                hipLaunchKernelGGL((k_update<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)p, (const T*)q, x, r, z,
-                                   dinv, (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
+                plaunch(P, k_update<T, R, 2>, dim3(G), dim3(BLK), n, (const T*)p, (const T*)q, x, r, z, dinv,
+                        (const double*)P.part_a, P.part_b, P.part_c, G, P.st);
                 apply_external<T, R>(P, r, z, y, t, P.st);
-                hipLaunchKernelGGL((k_dot<T, R>), dim3(G), dim3(BLK), 0, s, n, (const T*)r, (const T*)z, P.part_c,
-                                   (const SolveState*)P.st);
-                hipLaunchKernelGGL((k_pupdate<T, R, 2>), dim3(G), dim3(BLK), 0, s, n, (const T*)z, p,
-                                   (const double*)P.part_b, (const double*)P.part_c, G, P.st);
+                plaunch(P, k_dot<T, R>, dim3(G), dim3(BLK), n, (const T*)r, (const T*)z, P.part_c,
+                        (const SolveState*)P.st);
+                plaunch(P, k_pupdate<T, R, 2>, dim3(G), dim3(BLK), n, (const T*)z, p, (const double*)P.part_b,
+                        (const double*)P.part_c, G, P.st);
             }
         }
     };
-    // A block of GRAPH_ITERS iterations is captured once per plan into a HIP graph and replayed: the
-    // kernel arguments (device pointers, the state block) never change between solves of a plan.  The graph
-    // holds kernel nodes only (no memset / memcpy nodes).
+    // A block of GRAPH_ITERS iterations is built once per plan into a HIP graph (one chain of kernel nodes, no
+    // memset / memcpy nodes) and replayed: the kernel arguments (device pointers, the state block) never change
+    // between solves of a plan.
     hipGraphExec_t gexec = nullptr;
     if (P.use_graph) {
         if (!P.gexec) {
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            PlanImpl::GraphRec rec;
+            HIPCHK(hipGraphCreate(&rec.g, 0));
+            P.rec = &rec;
             body(GRAPH_ITERS);
-            hipError_t e = hipStreamEndCapture(s, &g);
-            if (e != hipSuccess) return fail(RUV_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&P.gexec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(RUV_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+            P.rec = nullptr;
+            hipError_t e = rec.err;
+            if (e == hipSuccess) e = hipGraphInstantiate(&P.gexec, rec.g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(rec.g);
+            if (e != hipSuccess) return fail(RUV_ERR_DEVICE, std::string("graph build: ") + hipGetErrorString(e));
         }
         gexec = P.gexec;
     }

@@ -182,6 +182,80 @@ def test_batch_wide_columns_and_empty_chart():
     assert err < 1e-6 and err2 < 1e-6
 
 
+def _grid_operator(nx, ny, seed, diag_frac):
+    """SPD grid operator on nx x ny rows in row-major order: couplings to the right and lower neighbours, and to
+    each lower diagonal neighbour with probability diag_frac (offsets 1, nx, nx + 1, nx - 1), diagonally dominant."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    r, c, v = [], [], []
+    for y in range(ny):
+        for x in range(nx):
+            i = y * nx + x
+            for dx, dy in ((1, 0), (0, 1), (1, 1), (-1, 1)):
+                X, Y = x + dx, y + dy
+                if not (0 <= X < nx and Y < ny):
+                    continue
+                if dx * dy != 0 and rng.random() >= diag_frac:
+                    continue
+                w = 0.5 + rng.random()
+                r += [i, Y * nx + X]
+                c += [Y * nx + X, i]
+                v += [-w, -w]
+    A = sp.csr_matrix((v, (r, c)), shape=(nx * ny,) * 2)
+    A = (A + sp.diags(-np.asarray(A.sum(axis=1)).ravel() + 0.01 + 0.1 * rng.random(nx * ny))).tocsr()
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+@pytest.mark.parametrize("precond", ["JACOBI", "AMG"])
+def test_batch_band_form(mixed, precond, monkeypatch):
+    """Charts in band form (grid operators with offsets 1, W - 1, W, W + 1: all four, some, a 3-row-high grid
+    wider than a workgroup) beside charts that are not (a permuted grid, a 1-D chain, a one-row chart), every
+    tile schedule: each chart against SuperLU, and the batch against the same batch with the band form off."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    mats = [_grid_operator(50, 37, 1, 1.0), _grid_operator(31, 40, 2, 0.5), _grid_operator(300, 3, 3, 1.0),
+            _grid_operator(64, 64, 4, 0.0)]
+    P = _grid_operator(45, 45, 5, 1.0)
+    perm = np.random.default_rng(6).permutation(P.shape[0])
+    mats.append(P[perm][:, perm].tocsr())
+    chain = sp.diags([-np.ones(199), 2.01 * np.ones(200), -np.ones(199)], [-1, 0, 1], format="csr")
+    mats += [chain, sp.csr_matrix(np.array([[3.0]]))]
+    A = sp.block_diag(mats, format="csr")
+    A.sort_indices()
+    off = np.cumsum([0] + [m.shape[0] for m in mats])
+    b = np.random.default_rng(7).standard_normal(2 * A.shape[0])
+    refs = []
+    for k, m in enumerate(mats):
+        lu = spla.splu(sp.csc_matrix(m))
+        rb = b[2 * off[k]:2 * off[k + 1]].reshape(-1, 2)
+        refs.append(np.stack([lu.solve(rb[:, 0]), lu.solve(rb[:, 1])], axis=1))
+    pc = getattr(Precond, precond)
+    sols = {}
+    for band in ("1", "0"):
+        monkeypatch.setenv("RUV_BATCH_BAND", band)
+        for spt in ("1", "2", "4", "8"):
+            monkeypatch.setenv("RUV_BATCH_SLICES_PER_TILE", spt)
+            say(f"band form {band}, {spt} slices per tile, {precond} mixed={mixed}")
+            bt = B.Batch.from_csr(A.indptr.astype(np.int64), A.indices, A.data, off, precond=pc, mixed=mixed)
+            assert bt.band_info()["band_charts"] == (4 if band == "1" else 0)
+            x, info, it, res = bt.solve(b, max_iter=40000, tol=1e-10)
+            bt.close()
+            for k in range(len(mats)):
+                xr = refs[k]
+                err = np.linalg.norm(x[2 * off[k]:2 * off[k + 1]].reshape(-1, 2) - xr, axis=1).max()
+                assert err <= 1e-6 * np.linalg.norm(xr, axis=1).max(), (k, err)
+            sols[band, spt] = x
+    for key, x in sols.items():
+        assert np.abs(x - sols["0", "2"]).max() <= 1e-6 * np.abs(sols["0", "2"]).max(), key
+
+
 def test_batch_of_small_spd_systems_matches_golden():
     """One right-hand side per row (rhs_per_row = 1): the golden small SPD systems as one batch."""
     import os
@@ -453,7 +527,12 @@ def test_batch_step_profile(mixed):
         assert kinds.count(k) == 1, (k, kinds)
     assert sorted(e["level"] for e in prof if e["kernel"] == "kc_down") == list(range(1, nl - 1))
     assert sorted(e["level"] for e in prof if e["kernel"] == "kc_post") == list(range(1, nl - 1))
-    assert all(e["seconds"] > 0 and e["bytes"] > 0 and e["stored_bytes"] >= e["bytes"] for e in prof)
+    assert all(e["seconds"] > 0 and e["bytes"] > 0 for e in prof)
+    # stored bytes cover the padding slots; the finest-operator passes of band charts store each mirrored pair once
+    finest = ("kb_spmv", "kc_resid0", "kc_post0")
+    assert bt.band_info()["band_charts"] > 0
+    assert all(e["stored_bytes"] >= e["bytes"] for e in prof if e["kernel"] not in finest)
+    assert all(0 < e["stored_bytes"] < e["bytes"] for e in prof if e["kernel"] in finest)
     vb = 2 * (4 if mixed else 8)
     n = rows[0]
     upd = [e for e in prof if e["kernel"] == "kb_update"][0]
