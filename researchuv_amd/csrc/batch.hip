@@ -154,7 +154,10 @@ int band_of(int64_t lo, int64_t hi, const int64_t* ptr, const int32_t* cols, con
         }
     if (nd == 0 || upper != lower) return 0;
     const int64_t dmax = *std::max_element(ds, ds + nd);
+    // the width: the first candidate that fits all offsets, a candidate whose mask the kernels compile (the
+    // offsets of triangulated and plain grids) before any other
     int W = 0, mk = 0;
+    bool compiled = false;
     for (const int64_t w : {dmax - 1, dmax, dmax + 1}) {
         if (w < 3 || w > (int64_t)INT32_MAX / 16) continue;
         int m = 0;
@@ -167,10 +170,11 @@ int band_of(int64_t lo, int64_t hi, const int64_t* ptr, const int32_t* cols, con
             else if (d == w + 1) m |= 8;
             else fits = false;
         }
-        if (fits) {
+        const bool cm = band_mask_compiled(m);
+        if (fits && (!W || (cm && !compiled))) {
             W = (int)w;
             mk = m;
-            break;
+            compiled = cm;
         }
     }
     if (!W) return 0;
@@ -541,7 +545,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     hipLaunchKernelGGL((kb_prep<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
                        (const double*)Q.r64, dinv, x, r, z, (const T*)pb[0], pb[1], cont);
     hipLaunchKernelGGL(kb_init, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol, Q.inner_tol,
-                       Q.mode == RUV_MODE_MIXED ? 1 : 0, cont, Q.cont_gap);
+                       Q.mode == RUV_MODE_MIXED ? 1 : 0, cont, Q.cont_gap, Q.final_aim);
     HIPCHK(hipGetLastError());
     if (PC == 2) {  // z0 = M^-1 r0 and its r.z partials (parity 0); kb_init has marked the running charts
         rc = bamg_set_live(Q, nullptr);
@@ -855,6 +859,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (const char* e = std::getenv("RUV_BATCH_CONT_GAP")) Q->cont_gap = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_LIST_BELOW")) Q->list_below = std::atof(e);
     if (const char* e = std::getenv("RUV_UV_TOL")) Q->uv_tol = std::atof(e);
+    if (const char* e = std::getenv("RUV_BATCH_FINAL_AIM")) Q->final_aim = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_LOOK_LAG")) Q->look_lag = std::atoi(e) ? 1 : 0;
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;

@@ -619,11 +619,15 @@ int bamg_build(BatchImpl& Q) {
         const NatLevel& N = nat[0];
         const int64_t n1 = nat[1].n;
         L.dinv = f32 ? (void*)Q.dinv32 : (void*)Q.dinv64;
-        if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build)
-            Bf16* e16 = nullptr;
-            HIPCHK(A->alloc(&e16, (size_t)Q.nell));
-            hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
-            A->ev0 = e16;
+        if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build): the
+                    // sliced ELL unless every chart is in band form, and the band form
+            A->ev0 = nullptr;
+            if (Q.nband < Q.nchart) {
+                Bf16* e16 = nullptr;
+                HIPCHK(A->alloc(&e16, (size_t)Q.nell));
+                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
+                A->ev0 = e16;
+            }
             A->bv0 = nullptr;
             if (Q.nbv) {
                 Bf16* b16 = nullptr;

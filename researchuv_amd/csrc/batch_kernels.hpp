@@ -369,7 +369,8 @@ __device__ __forceinline__ void band_rows_hm(int64_t lo, int n, int W, int m, co
     }
 }
 // band_rows_hm with BAND_H grid rows per thread, the masks of triangulated grids (diagonals one way: 1, W, W + 1
-// or the other: 1, W - 1, W; both: all four) at compile time
+// or the other: 1, W - 1, W; both: all four) and of plain grids (1, W) at compile time
+constexpr bool band_mask_compiled(int m) { return m == 13 || m == 7 || m == 15 || m == 5; }
 template <typename T, int R, int NT, typename V, typename G, typename F>
 __device__ __forceinline__ void band_rows(const BDev& B, int c, int k, int ntl, const V* __restrict__ bvals, G g,
                                           F fin) {
@@ -381,6 +382,7 @@ __device__ __forceinline__ void band_rows(const BDev& B, int c, int k, int ntl, 
         case 13: band_rows_hm<BAND_H, 13, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
         case 7: band_rows_hm<BAND_H, 7, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
         case 15: band_rows_hm<BAND_H, 15, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
+        case 5: band_rows_hm<BAND_H, 5, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
         default: band_rows_hm<BAND_H, 0, T, R, NT>(lo, n, W, m, v0, k, ntl, g, fin); break;
     }
 }
@@ -632,10 +634,13 @@ __global__ __launch_bounds__(BLK) void kb_prep(BDev B, const int* __restrict__ a
 }
 
 // per chart (not final): inner tolerance, step state.  plain: tol * ||b|| on the recursive residual
-// (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target.
-// One workgroup per chart (the chart's tile partials summed in a fixed order).
+// (solver.rs:465-468); mixed: inner_tol * ||r64|| but no tighter than a quarter of the outer target, and, when that
+// reduction would bring the chart within twice its outer target (the round can end the chart), aim times the
+// target (aim > 0): stopping the recursive residual right at the target left the refined one above it on most
+// charts, which then paid a whole further round.  One workgroup per chart (the chart's tile partials summed in a
+// fixed order).
 static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double inner_tol, int mixed, int cont,
-                                                      double gap) {
+                                                      double gap, double aim) {
     __shared__ double sh[WAVES];
     const int c = blockIdx.x;
     if (B.c_final[c]) {
@@ -658,8 +663,13 @@ static __global__ __launch_bounds__(BLK) void kb_init(BDev B, double tol, double
     if (threadIdx.x != 0) return;
     double ta;
     const double tres = (bn <= 1e-300) ? tol : tol * bn;
-    if (mixed) ta = fmax(inner_tol * rn, 0.25 * (B.c_tgt[c] > 0.0 ? B.c_tgt[c] : tres));
-    else ta = tres;
+    if (mixed) {
+        const double tg = B.c_tgt[c] > 0.0 ? B.c_tgt[c] : tres;
+        ta = fmax(inner_tol * rn, 0.25 * tg);
+        if (aim > 0.0 && inner_tol * rn <= 2.0 * tg) ta = aim * tg;
+    } else {
+        ta = tres;
+    }
     B.c_tol[c] = ta;
     B.c_iters[c] = 0;
     B.c_rz[B.nchart + c] = rz_old;
