@@ -128,6 +128,52 @@ __device__ __forceinline__ void cput(Bf16* p, double v) {
 __device__ __forceinline__ void cput(double* p, double v) { *p = v; }
 
 
+// The finest operator's band form (batch_kernels.hpp) as a source of the setup's level-0 rows: null bw for the other
+// levels or a batch without band charts.  Values in FP32 (f32, the mixed batch) or FP64 (f64, cast to FP32 as the
+// setup CSR holds them: the same values).
+struct BandSrc {
+    const int32_t* bw;     // per chart: grid width, 0 when not in band form
+    const int32_t* bmask;
+    const int64_t* bbase;
+    const int64_t* crow;
+    const int32_t* chart;  // row -> chart
+    const float* f32;
+    const double* f64;
+};
+
+// Row i's entries from the band form in ascending column order -- the order of the setup CSR -- as col[q], val[q]
+// for the offsets -(W+1), -W, -(W-1), -1, 0, 1, W-1, W, W+1 (an absent entry reads as 0 and is to be skipped: the
+// CSR holds no zero entries).  Returns false (nothing read) when row i's chart is not in band form.
+__device__ __forceinline__ bool band_row(const BandSrc& S, int64_t i, int32_t (&col)[9], float (&val)[9]) {
+    if (!S.bw) return false;
+    const int c = S.chart[i];
+    const int W = S.bw[c];
+    if (W <= 0) return false;
+    const int m = S.bmask[c];
+    const int64_t lo = S.crow[c];
+    const int n = (int)(S.crow[c + 1] - lo);
+    const int l = (int)(i - lo);
+    // slot of each offset in the chart's band arrays (an absent offset reads the diagonal's slot, zeroed below)
+    int sl[4], k = 1;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) sl[b] = ((m >> b) & 1) ? k++ : 0;
+    const int64_t base = S.bbase[c];
+    const int d[4] = {1, W - 1, W, W + 1};
+    const int off[9] = {-d[3], -d[2], -d[1], -d[0], 0, d[0], d[1], d[2], d[3]};
+    const int slots[9] = {sl[3], sl[2], sl[1], sl[0], 0, sl[0], sl[1], sl[2], sl[3]};
+    const int bits[9] = {3, 2, 1, 0, -1, 0, 1, 2, 3};
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int r = q < 4 ? l + off[q] : l;  // a lower entry is the upper entry of row l - d
+        const bool ok = r >= 0 && (bits[q] < 0 || ((m >> bits[q]) & 1));
+        const int64_t at = base + (int64_t)slots[q] * n + (ok ? r : l);
+        const float v = S.f32 ? S.f32[at] : (float)S.f64[at];
+        val[q] = ok ? v : 0.0f;
+        col[q] = (int32_t)(i + off[q]);
+    }
+    return true;
+}
+
 // Preference of the coupling (i, j): its strength in bins of a quarter octave, then a hash of the unordered
 // pair.  Couplings that vary smoothly along a line (uniform or geometric spacing) fall into one bin, where the
 // hash -- the same seen from either end -- acts as a random priority; ordering by exact strength or by index
@@ -140,7 +186,13 @@ __device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
     h ^= h >> 12;
     h *= 0x297A2D39u;
     h ^= h >> 15;
-    const int bin = (int)floor(4.0 * log2(s)) + 4096;  // s > 0
+    // bin = floor(4 log2 s) for s > 0, from the exponent and the mantissa against 2^(1/4), 2^(1/2), 2^(3/4)
+    // (the rounded constants; no logarithm); subnormal s fall into the lowest bins
+    const uint64_t u = (uint64_t)__double_as_longlong(s);
+    const int ex = (int)((u >> 52) & 0x7FF) - 1023;
+    const double mant = __longlong_as_double((long long)((u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    const int q = (mant >= 1.6817928305074290) + (mant >= 1.4142135623730951) + (mant >= 1.1892071150027210);
+    const int bin = 4 * ex + q + 4096;
     return ((uint64_t)(uint32_t)max(bin, 0) << 32) | h;
 }
 
@@ -156,22 +208,43 @@ __device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, i
 // ranking and the first round's proposals (every row is unmatched then)
 // Also the level's Gershgorin bound on D^-1 A, max over rows of sum_j |a_ij| / a_ii, into *gmax (the bits of a
 // non-negative double order like the double).
-__global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, const int32_t* __restrict__ group, double theta,
-                                                    int32_t* __restrict__ cand, int32_t* __restrict__ ncand,
-                                                    int64_t* __restrict__ prop, unsigned long long* __restrict__ gmax) {
+// The level-0 rows of band charts are read from the band form (the same entries in the same order, coalesced), the
+// others from the setup CSR.
+__global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, BandSrc band, const int32_t* __restrict__ group,
+                                                    double theta, int32_t* __restrict__ cand,
+                                                    int32_t* __restrict__ ncand, int64_t* __restrict__ prop,
+                                                    unsigned long long* __restrict__ gmax) {
     __shared__ double sh[WAVES];
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     const bool in = i < A.n;
-    const int64_t e0 = in ? A.ptr[i] : 0, e1 = in ? A.ptr[i + 1] : 0;
+    int32_t bc[9];
+    float bv[9];
+    const bool fromband = in && band_row(band, i, bc, bv);
+    const int64_t e0 = (in && !fromband) ? A.ptr[i] : 0, e1 = (in && !fromband) ? A.ptr[i + 1] : 0;
+    // f(j, a_ij) over the row's entries in ascending column order
+    const auto each = [&](auto&& f) {
+        if (fromband) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                if (bv[q] != 0.0f) f(bc[q], bv[q]);
+        } else {
+            for (int64_t e = e0; e < e1; ++e) f(A.cols[e], A.vals[e]);
+        }
+    };
     double smax = 0.0, asum = 0.0, d = 0.0;
-    for (int64_t e = e0; e < e1; ++e) {
-        const double v = A.vals[e];
+    each([&](int32_t j, float a) {
+        const double v = a;
         asum += fabs(v);
-        if (A.cols[e] == i) d += v;
+        if (j == i) d += v;
         else if (v < 0.0) smax = fmax(smax, -v);
-    }
+    });
     const double gb = block_max_all((d > 0.0) ? asum / d : 0.0, sh);
-    if (threadIdx.x == 0) atomicMax(gmax, (unsigned long long)__double_as_longlong(gb));
+    // one atomic on one word from every workgroup serialises (about 12 ns each: 1.7 ms for the 142 000 workgroups
+    // of 36 M rows); the bound only grows, so a workgroup whose maximum the word already covers skips it
+    if (threadIdx.x == 0) {
+        const unsigned long long v = (unsigned long long)__double_as_longlong(gb);
+        if (v > __hip_atomic_load(gmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(gmax, v);
+    }
     if (!in) return;
     const double thr = theta * smax;
     const int32_t g = group[i];
@@ -183,11 +256,11 @@ __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, const int32_t* _
         kj[k] = INT32_MAX;
     }
     int cnt = 0;
-    for (int64_t e = e0; e < e1; ++e) {
-        const int32_t j = A.cols[e];
-        if (j == i || !(A.vals[e] < 0.0) || group[j] != g) continue;
-        const double sv = -A.vals[e];
-        if (sv < thr) continue;
+    each([&](int32_t j, float a) {
+        // a band row's neighbours lie in its chart, the level-0 group
+        if (j == i || !(a < 0.0f) || (!fromband && group[j] != g)) return;
+        const double sv = -(double)a;
+        if (sv < thr) return;
         // carry (p, j) down the descending list: it takes the first slot it precedes, or the first free one
         uint64_t cp = pair_pref(sv, i, j);
         int32_t cj = j;
@@ -210,7 +283,7 @@ __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, const int32_t* _
             }
         }
         if (cnt < MCAND) ++cnt;
-    }
+    });
 #pragma unroll
     for (int k = 0; k < MCAND; ++k)
         if (k < cnt) cand[i * MCAND + k] = kj[k];
@@ -452,6 +525,29 @@ __global__ __launch_bounds__(BLK) void k_galerkin_stage(int64_t nrow, const int6
     }
     int n = 0;
     bool full = false;
+    // add a_ij into the slot of column c = agg[j] (a new slot for a new column; full when none is left)
+    const auto merge = [&](int32_t c, double v) {
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < SL; ++k)
+            if (key[k] == c) {
+                val[k] += v;
+                hit = true;
+            }
+        if (!hit) {
+            if (n < SL) {
+#pragma unroll
+                for (int k = 0; k < SL; ++k)
+                    if (k == n) {
+                        key[k] = c;
+                        val[k] += v;
+                    }
+                ++n;
+            } else {
+                full = true;
+            }
+        }
+    };
     const int64_t m1 = mptr[I + 1];
     for (int64_t q = mptr[I]; q < m1 && !full; ++q) {
         const int64_t r = mrows[q];
@@ -472,28 +568,7 @@ __global__ __launch_bounds__(BLK) void k_galerkin_stage(int64_t nrow, const int6
 #pragma unroll
             for (int u = 0; u < GR_BATCH; ++u) {
                 if (e0 + u >= e1) break;
-                const int32_t c = cb[u];
-                const double v = vb[u];
-                bool hit = false;
-#pragma unroll
-                for (int k = 0; k < SL; ++k)
-                    if (key[k] == c) {
-                        val[k] += v;
-                        hit = true;
-                    }
-                if (!hit) {
-                    if (n < SL) {
-#pragma unroll
-                        for (int k = 0; k < SL; ++k)
-                            if (k == n) {
-                                key[k] = c;
-                                val[k] += v;
-                            }
-                        ++n;
-                    } else {
-                        full = true;
-                    }
-                }
+                merge(cb[u], vb[u]);
             }
         }
     }

@@ -143,8 +143,8 @@ struct Matched {
     double gersh = 0.0;  // Gershgorin bound on D^-1 A of the matched operator
 };
 
-int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, double theta, int rounds,
-                      Scratch& S, Matched* M) {
+int match_and_coarsen(hipStream_t s, const SetupCsr& A, const BandSrc& band, const int32_t* group, double theta,
+                      int rounds, Scratch& S, Matched* M) {
     const int64_t n = A.n;
     int32_t *cand = nullptr, *ncand = nullptr;
     int64_t *prop = nullptr, *flag = nullptr, *mate = nullptr, *join = nullptr;
@@ -160,7 +160,7 @@ int match_and_coarsen(hipStream_t s, const SetupCsr& A, const int32_t* group, do
     HIPCHK(S.alloc(&M->cid, n + 1));
     HIPCHK(S.alloc(&M->agg, n));
     HIPCHK(hipMemsetAsync(mate, 0xff, sizeof(int64_t) * n, s));  // -1: unmatched
-    hipLaunchKernelGGL(k_candidates, rows_grid(n), dim3(BLK), 0, s, A, group, theta, cand, ncand, prop, gmax);
+    hipLaunchKernelGGL(k_candidates, rows_grid(n), dim3(BLK), 0, s, A, band, group, theta, cand, ncand, prop, gmax);
     for (int r = 0; r < rounds; ++r) {
         if (r > 0)
             hipLaunchKernelGGL(k_propose, rows_grid(n), dim3(BLK), 0, s, n, (const int32_t*)cand,
@@ -410,7 +410,11 @@ int bamg_build(BatchImpl& Q) {
         // operator, aggregates of about four, doubled the iterations on jittered meshes)
         std::unique_ptr<Scratch> S1(new Scratch(&Q.pool));
         Matched M1;
-        int rc = match_and_coarsen(s, cur, group, theta, rounds, *S1, &M1);
+        // level 0: the band charts' rows ranked from their band form (the same entries, read coalesced)
+        BandSrc band{};
+        if (l == 0 && Q.nband > 0)
+            band = BandSrc{Q.B.c_bw, Q.B.c_bmask, Q.B.c_bbase, Q.B.crow, Q.amg_group, Q.bv32, Q.bv64};
+        int rc = match_and_coarsen(s, cur, band, group, theta, rounds, *S1, &M1);
         if (rc) return rc;
         const Matched& MF = M1;
         N.omega = (M1.gersh > 0.0) ? std::min(A->omega_max, 1.9 / M1.gersh) : A->omega_max;

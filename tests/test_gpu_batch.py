@@ -256,6 +256,39 @@ def test_batch_band_form(mixed, precond, monkeypatch):
         assert np.abs(x - sols["0", "2"]).max() <= 1e-6 * np.abs(sols["0", "2"]).max(), key
 
 
+@pytest.mark.parametrize("mixed", [True, False])
+def test_batch_amg_setup_from_band_form_is_bitwise_the_csr_setup(mixed, monkeypatch):
+    """The finest level's matching reads band charts' rows from the band form (the same entries in the same column
+    order as the setup CSR): every level of the hierarchy -- aggregate maps, charts, operator structure and values --
+    equals, bit for bit, the hierarchy built from the CSR with the band form off."""
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    say(f"AMG setup from the band form against the CSR setup, mixed={mixed}")
+    meshes = _meshes(10, seed=31)
+    lb = B.LscmBatchBuilder(meshes, threads=2)
+    monkeypatch.setenv("RUV_BATCH_SLICES_PER_TILE", "2")
+    levels = {}
+    for band in ("1", "0"):
+        monkeypatch.setenv("RUV_BATCH_BAND", band)
+        bt = lb.device_batch(precond=Precond.AMG, mixed=mixed)
+        assert (bt.band_info()["band_charts"] > 0) == (band == "1")
+        nl = bt.precond_info()["levels"]
+        levels[band] = [bt.amg_level(l) for l in range(nl)]
+        bt.close()
+    assert len(levels["1"]) == len(levels["0"])
+    for l, (a, b) in enumerate(zip(levels["1"], levels["0"])):
+        np.testing.assert_array_equal(a["chart"], b["chart"])
+        assert a["n_next"] == b["n_next"]
+        if a["agg"] is not None:
+            np.testing.assert_array_equal(a["agg"], b["agg"])
+        if l > 0:
+            assert (a["op"] != b["op"]).nnz == 0
+            np.testing.assert_array_equal(a["op"].indptr, b["op"].indptr)
+            np.testing.assert_array_equal(a["op"].indices, b["op"].indices)
+            np.testing.assert_array_equal(a["op"].data, b["op"].data)
+
+
 def test_batch_of_small_spd_systems_matches_golden():
     """One right-hand side per row (rhs_per_row = 1): the golden small SPD systems as one batch."""
     import os
