@@ -780,6 +780,8 @@ __device__ __forceinline__ double rsqrt_or0(double d) { return d > 0.0 ? 1.0 / s
 // product of the two factors formed first so that (i, j) and (j, i) round alike), columns as stored rows (CI 0:
 // 16-bit offsets from the row; CI 1: 32-bit rows); padding: value 0, the row itself.  One thread per stored row p
 // (constructed row ipos[p]), so the stores of a slot are contiguous.
+// CI 2 (FP32 cycle, 16-bit offsets): value and offset packed in one word, the bfloat16 value in the high half and
+// the offset in the low half (ec, uint32), so a slot is one load; ev is unused.
 template <typename T, int CI>
 __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const double* __restrict__ diag,
                                                    const int32_t* __restrict__ rpos,
@@ -797,12 +799,25 @@ __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const 
     const double si = rsqrt_or0(diag[i]);
     for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
         const int64_t c = rpos ? rpos[A.cols[a]] : A.cols[a];
-        cput(ev + e0 + k * ns, (double)A.vals[a] * (si * rsqrt_or0(diag[A.cols[a]])));
+        const double v = (double)A.vals[a] * (si * rsqrt_or0(diag[A.cols[a]]));
+        if constexpr (CI == 2) {
+            Bf16 h;
+            cput(&h, v);
+            static_cast<uint32_t*>(ec)[e0 + k * ns] = ((uint32_t)h.bits << 16) | (uint32_t)(uint16_t)(int16_t)(c - p);
+            continue;
+        } else {
+            cput(ev + e0 + k * ns, v);
+        }
         if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)(c - p);
         else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)c;
     }
     for (; k < w[b]; ++k) {
-        cput(ev + e0 + k * ns, 0.0);
+        if constexpr (CI == 2) {
+            static_cast<uint32_t*>(ec)[e0 + k * ns] = 0u;
+            continue;
+        } else {
+            cput(ev + e0 + k * ns, 0.0);
+        }
         if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = 0;
         else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)p;
     }
@@ -827,7 +842,7 @@ struct CycLevel {
     int64_t n;
     const int64_t* eoff;     // sliced ELL of the level operator (levels between the finest and the coarsest)
     const int32_t* ew;
-    const void* ecols;       // int16 offsets (CI 0) or int32 columns (CI 1)
+    const void* ecols;       // int16 offsets (CI 0), int32 columns (CI 1) or packed bfloat16 value + offset (CI 2)
     const typename CoarseVal<T>::type* evals;
     const int32_t* srow;     // slice row starts (aligned to aggregate starts)
     const int32_t* sagg;     // first next-level aggregate of every slice
@@ -877,14 +892,26 @@ struct EllRow {
 
 // acc[c] += sum over the ELL row of a level operator of a_ij * g(j)[c], the slot loop unrolled for the slice
 // width (a switch, uniform over the workgroup) as in row_slots
+// slot e of a level operator: its value and column (CI 2: one packed word, bfloat16 value high, offset low)
+template <typename T, int CI>
+__device__ __forceinline__ T ell_entry(const CycLevel<T>& L, int64_t e, int64_t i, int64_t* col) {
+    if constexpr (CI == 2) {
+        const uint32_t u = static_cast<const uint32_t*>(L.ecols)[e];
+        *col = i + (int64_t)(int16_t)(u & 0xFFFFu);
+        return (T)__uint_as_float(u & 0xFFFF0000u);
+    } else {
+        *col = col_of<CI>(L.ecols, e, i);
+        return (T)cval(L.evals[e]);
+    }
+}
 template <int W, typename T, int R, int CI, typename G>
 __device__ __forceinline__ void ell_slots(const CycLevel<T>& L, int64_t e0, int64_t ns, int64_t i, G g,
                                           T (&acc)[R]) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const int64_t e = e0 + k * ns;
-        const T a = (T)cval(L.evals[e]);
-        const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
+        int64_t j;
+        const T a = ell_entry<T, CI>(L, e0 + k * ns, i, &j);
+        const Vr<T, R> x = g(j);
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
     }
@@ -900,9 +927,9 @@ __device__ __forceinline__ void ell_apply(const CycLevel<T>& L, const EllRow& r,
 #undef RUV_EL
         default:
             for (int k = 0; k < r.w; ++k) {
-                const int64_t e = r.e0 + k * r.ns;
-                const T a = (T)cval(L.evals[e]);
-                const Vr<T, R> x = g(col_of<CI>(L.ecols, e, i));
+                int64_t j;
+                const T a = ell_entry<T, CI>(L, r.e0 + k * r.ns, i, &j);
+                const Vr<T, R> x = g(j);
 #pragma unroll
                 for (int cc = 0; cc < R; ++cc) acc[cc] = fma(a, x.c[cc], acc[cc]);
             }

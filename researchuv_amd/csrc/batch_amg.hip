@@ -588,18 +588,24 @@ int bamg_build(BatchImpl& Q) {
                 }
         }
         if (hf[0]) return fail(RUV_ERR_PRECOND, "batch amg: an aggregate holds more rows than a slice allows");
-        L.ci = hf[1] ? 1 : 0;
+        // columns: 16-bit offsets from the row (packed with the bfloat16 value under the FP32 cycle: one load a
+        // slot), or 32-bit stored rows when some offset does not fit
+        L.ci = hf[1] ? 1 : (f32 ? 2 : 0);
         char *v = nullptr, *q = nullptr, *rs = nullptr, *ec = nullptr;
-        HIPCHK(A->alloc(&ec, L.nell * (L.ci ? sizeof(int32_t) : sizeof(int16_t))));
+        HIPCHK(A->alloc(&ec, L.nell * (L.ci == 0 ? sizeof(int16_t) : sizeof(int32_t))));
         L.ecols = ec;
-        HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
+        if (L.ci != 2) HIPCHK(A->alloc(&v, L.nell * (f32 ? sizeof(CoarseVal<float>::type) : sizeof(double))));
         HIPCHK(A->alloc(&q, n * sv));
         HIPCHK(A->alloc(&rs, n * sv));
         auto fill = [&](auto tv) {
             using TT = decltype(tv);
             using VT = typename CoarseVal<TT>::type;
-            if (L.ci)
+            if (L.ci == 1)
                 hipLaunchKernelGGL((k_pell_fill<TT, 1>), rows_grid(n), dim3(BLK), 0, s, n, N.A, N.diag,
+                                   (const int32_t*)rpos, (const int32_t*)ipos, nsl, (const int32_t*)L.srow,
+                                   (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
+            else if (L.ci == 2)
+                hipLaunchKernelGGL((k_pell_fill<TT, 2>), rows_grid(n), dim3(BLK), 0, s, n, N.A, N.diag,
                                    (const int32_t*)rpos, (const int32_t*)ipos, nsl, (const int32_t*)L.srow,
                                    (const int64_t*)L.eoff, (const int32_t*)L.ew, (VT*)v, L.ecols);
             else
@@ -751,6 +757,7 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     BamgLevel& L0 = A.lv[0];
     BamgLevel& L1 = A.lv[1];
     using VT = typename CoarseVal<T>::type;
+    constexpr int PACKED = sizeof(T) == 4 ? 2 : 0;  // the packed column mode exists under the FP32 cycle only
     const VT* vals = (const VT*)A.ev0;
     const VT* bvals = (const VT*)A.bv0;
     const dim3 gt((unsigned)(((na + XCDS - 1) / XCDS) * XCDS));
@@ -786,7 +793,7 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
     // tables (row start, aggregate start, chart range, width, offset)
     auto lmat = [&](size_t l, bool stored) {
         const BamgLevel& L = A.lv[l];
-        const double e = cv + (L.ci ? 4.0 : 2.0);
+        const double e = L.ci == 2 ? 4.0 : cv + (L.ci == 1 ? 4.0 : 2.0);  // value and column bytes a slot
         return (stored ? (double)L.nell : (double)L.nnz) * e + (double)L.nsl * 28.0;
     };
     if (bgrid(1) > 0) {
@@ -801,7 +808,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         const T w = (T)A.lv[l].omega;
         const int64_t ns = sgrid(l);
         if (ns == 0) continue;
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_down<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        if (A.lv[l].ci == 1) hipLaunchKernelGGL((kc_down<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
+        else if (A.lv[l].ci == 2) hipLaunchKernelGGL((kc_down<T, R, PACKED>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
         else hipLaunchKernelGGL((kc_down<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T*)A.lv[l + 1].b);
         if (prof) {  // b, dinv read; next level's member offsets read, its b written
             const double v = (double)A.lv[l].n * (vb + db) + (double)A.lv[l + 1].n * (4.0 + vb);
@@ -829,8 +837,10 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         const T* zn = (const T*)A.lv[l + 1].z;
         const double nL = (double)A.lv[l].n, nN = (double)A.lv[l + 1].n;
         if (A.lv[l].n <= A.fuse_rows) {  // small level: the prolongation inside the post-smoothing pass
-            if (A.lv[l].ci)
+            if (A.lv[l].ci == 1)
                 hipLaunchKernelGGL((kc_post<T, R, 1, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
+            else if (A.lv[l].ci == 2)
+                hipLaunchKernelGGL((kc_post<T, R, PACKED, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
             else
                 hipLaunchKernelGGL((kc_post<T, R, 0, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
             if (prof) {  // b, dinv, agg read, z written; the next level's z read
@@ -844,7 +854,8 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
             const double v = nL * (2.0 * vb + db + 4.0) + nN * vb;
             if ((rc = mark(BK_PROLONG, (int)l, v, v))) return rc;
         }
-        if (A.lv[l].ci) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
+        if (A.lv[l].ci == 1) hipLaunchKernelGGL((kc_post<T, R, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
+        else if (A.lv[l].ci == 2) hipLaunchKernelGGL((kc_post<T, R, PACKED>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
         else hipLaunchKernelGGL((kc_post<T, R, 0>), g1(ns), dim3(BLK), 0, s, cyc(l), w, T(0), zn);
         if (prof) {  // b, t, dinv read, z written
             const double v = nL * (3.0 * vb + db);
@@ -960,14 +971,18 @@ int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* c
         HIPCHK(hipMemcpyAsync(eoff.data(), L.eoff, sizeof(int64_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(ew.data(), L.ew, sizeof(int32_t) * nsl, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(srow.data(), L.srow, sizeof(int32_t) * (nsl + 1), hipMemcpyDeviceToHost, s));
+        // columns: 32-bit rows (ci 1) or 32-bit packed words (ci 2: bfloat16 value high, offset low) in c32,
+        // 16-bit offsets (ci 0) in c16; values of ci 0 and 1 in b16 (FP32 cycle) or d64
         std::vector<int32_t> c32(L.ci ? (size_t)std::max<int64_t>(ne, 1) : 0);
         std::vector<int16_t> c16(L.ci ? 0 : (size_t)std::max<int64_t>(ne, 1));
         if (L.ci) HIPCHK(hipMemcpyAsync(c32.data(), L.ecols, sizeof(int32_t) * ne, hipMemcpyDeviceToHost, s));
         else HIPCHK(hipMemcpyAsync(c16.data(), L.ecols, sizeof(int16_t) * ne, hipMemcpyDeviceToHost, s));
-        std::vector<uint16_t> b16(f32 ? (size_t)std::max<int64_t>(ne, 1) : 0);
+        std::vector<uint16_t> b16(f32 && L.ci != 2 ? (size_t)std::max<int64_t>(ne, 1) : 0);
         std::vector<double> d64(f32 ? 0 : (size_t)std::max<int64_t>(ne, 1));
-        if (f32) HIPCHK(hipMemcpyAsync(b16.data(), L.evals, sizeof(uint16_t) * ne, hipMemcpyDeviceToHost, s));
-        else HIPCHK(hipMemcpyAsync(d64.data(), L.evals, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
+        if (f32 && L.ci != 2)
+            HIPCHK(hipMemcpyAsync(b16.data(), L.evals, sizeof(uint16_t) * ne, hipMemcpyDeviceToHost, s));
+        else if (!f32)
+            HIPCHK(hipMemcpyAsync(d64.data(), L.evals, sizeof(double) * ne, hipMemcpyDeviceToHost, s));
         // the stored values are scaled by 1 / sqrt(d_i d_j): undone with the stored sqrt(d)
         std::vector<float> q32(f32 ? (size_t)std::max<int64_t>(n, 1) : 0);
         std::vector<double> q64(f32 ? 0 : (size_t)std::max<int64_t>(n, 1));
@@ -980,10 +995,12 @@ int bamg_level(BatchImpl& Q, int l, int64_t* sizes, int64_t* row_ptr, int32_t* c
             for (int64_t p = r0; p < r0 + ns; ++p) {
                 for (int k = 0; k < ew[b]; ++k) {
                     const int64_t e = eoff[b] + (p - r0) + (int64_t)k * ns;
-                    const int64_t c = L.ci ? (int64_t)c32[e] : p + (int64_t)c16[e];
+                    const uint32_t w = (uint32_t)(L.ci ? c32[e] : 0);
+                    const int64_t c = L.ci == 1 ? (int64_t)c32[e]
+                                                : p + (L.ci == 2 ? (int64_t)(int16_t)(w & 0xFFFFu) : (int64_t)c16[e]);
                     double v;
                     if (f32) {
-                        const uint32_t u = (uint32_t)b16[e] << 16;
+                        const uint32_t u = L.ci == 2 ? (w & 0xFFFF0000u) : (uint32_t)b16[e] << 16;
                         float f;
                         std::memcpy(&f, &u, sizeof(f));
                         v = f;
