@@ -621,6 +621,14 @@ __global__ __launch_bounds__(BLK) void k_to_bf16(int64_t n, const float* __restr
     if (i < n) cput(out + i, (double)in[i]);
 }
 
+// the same over listed ranges of entries (first, count), one workgroup per range: the sliced-ELL entries of the
+// charts not in band form
+__global__ __launch_bounds__(BLK) void k_to_bf16_ranges(const int64_t* __restrict__ rng, const float* __restrict__ in,
+                                                        Bf16* __restrict__ out) {
+    const int64_t a = rng[2 * blockIdx.x], n = rng[2 * blockIdx.x + 1];
+    for (int64_t k = threadIdx.x; k < n; k += BLK) cput(out + a + k, (double)in[a + k]);
+}
+
 // int64 -> int32 (row indices of one level, below 2^31)
 __global__ __launch_bounds__(BLK) void k_narrow(int64_t n, const int64_t* __restrict__ in, int32_t* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;

@@ -237,17 +237,16 @@ int band_setup(BatchImpl& Q, const int64_t* ptr, const int32_t* cols, const doub
                 }
         }
     });
+    // FP64 for the refinement residual (and the plain solve), FP32 for the mixed inner solve
+    HIPCHK(Q.upload(&Q.bv64, bv.data(), bv.size()));
     if (Q.mode == RUV_MODE_MIXED) {
         std::vector<float> f((size_t)Q.nbv);
         parallel_for(Q.nbv, [&](int64_t a, int64_t b) {
             for (int64_t k = a; k < b; ++k) f[k] = (float)bv[k];
         });
         HIPCHK(Q.upload(&Q.bv32, f.data(), f.size()));
-        HIPCHK(hipStreamSynchronize(Q.stream));
-    } else {
-        HIPCHK(Q.upload(&Q.bv64, bv.data(), bv.size()));
-        HIPCHK(hipStreamSynchronize(Q.stream));
     }
+    HIPCHK(hipStreamSynchronize(Q.stream));
     int32_t *d_bw = nullptr, *d_bm = nullptr;
     int64_t* d_bb = nullptr;
     HIPCHK(Q.upload(&d_bw, Q.h_bw.data(), Q.h_bw.size()));
@@ -411,6 +410,20 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     });
     if (bad[0] >= 0) return fail(RUV_ERR_PRECOND, "zero diagonal at row " + std::to_string(bad[0]));
     Q.h_sl_off = sl_off;
+    {  // the sliced-ELL entries of the charts not in band form, in ranges of at most 1 << 16 (for their copies)
+        std::vector<int64_t> rng;
+        for (int c = 0; c < nchart; ++c) {
+            if (Q.h_bw[c] || Q.chart_s0[c] == Q.chart_s0[c + 1]) continue;
+            const int64_t a = sl_off[Q.chart_s0[c]];
+            const int64_t e = Q.chart_s0[c + 1] < nsl ? sl_off[Q.chart_s0[c + 1]] : nell;
+            for (int64_t k = a; k < e; k += (1 << 16)) {
+                rng.push_back(k);
+                rng.push_back(std::min<int64_t>(1 << 16, e - k));
+            }
+        }
+        Q.n_ell_ranges = (int64_t)rng.size() / 2;
+        if (Q.n_ell_ranges) HIPCHK(Q.upload(&Q.ell_ranges, rng.data(), rng.size()));
+    }
     Q.h_sl_w = sl_w;
     Q.h_dtab = dtab;
     Q.h_stencil = stencil;
@@ -672,7 +685,7 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
         hipLaunchKernelGGL((kb_axpy<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
                            (const T*)Q.xv, d_x);
         hipLaunchKernelGGL((kb_residual<R, CI>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na,
-                           (const double*)Q.ev64, d_b, (const double*)d_x, Q.r64);
+                           (const double*)Q.ev64, (const double*)Q.bv64, d_b, (const double*)d_x, Q.r64);
         hipLaunchKernelGGL(kb_outer, dim3((unsigned)Q.nchart), dim3(BLK), 0, s, B, tol, mixed ? Q.uv_tol : 0.0);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(Q.h_final, B.c_final, sizeof(int) * Q.nchart, hipMemcpyDeviceToHost, s));

@@ -632,10 +632,11 @@ int bamg_build(BatchImpl& Q) {
         if (f32) {  // bfloat16 copy of the batch's FP32 values (they may have changed since the last build): the
                     // sliced ELL unless every chart is in band form, and the band form
             A->ev0 = nullptr;
-            if (Q.nband < Q.nchart) {
+            if (Q.n_ell_ranges > 0) {  // the entries of the charts not in band form (the others stay unwritten)
                 Bf16* e16 = nullptr;
                 HIPCHK(A->alloc(&e16, (size_t)Q.nell));
-                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nell), dim3(BLK), 0, s, Q.nell, (const float*)Q.ev32, e16);
+                hipLaunchKernelGGL(k_to_bf16_ranges, dim3((unsigned)Q.n_ell_ranges), dim3(BLK), 0, s,
+                                   (const int64_t*)Q.ell_ranges, (const float*)Q.ev32, e16);
                 A->ev0 = e16;
             }
             A->bv0 = nullptr;

@@ -99,6 +99,8 @@ struct BatchImpl {
     float* bv32 = nullptr;
     double* bv64 = nullptr;
     double fin_slots = 0.0;  // value slots a finest-operator pass streams: ELL slots of the other charts, band slots
+    int64_t* ell_ranges = nullptr;  // (first, count) ranges of the sliced-ELL entries of the charts not in band form
+    int64_t n_ell_ranges = 0;
     float* dinv32 = nullptr;
     double* dinv64 = nullptr;
     // vectors

@@ -738,7 +738,8 @@ __global__ __launch_bounds__(BLK) void kb_axpy(BDev B, const int* __restrict__ a
 // rows other tiles update)
 template <int R, int CI>
 __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict__ act, int na,
-                                                   const double* __restrict__ vals64, const double* __restrict__ b,
+                                                   const double* __restrict__ vals64,
+                                                   const double* __restrict__ bvals64, const double* __restrict__ b,
                                                    const double* __restrict__ x64, double* __restrict__ r64) {
     __shared__ double sh[WAVES];
     const int slot = xcd_slot(na);
@@ -747,6 +748,24 @@ __global__ __launch_bounds__(BLK) void kb_residual(BDev B, const int* __restrict
     const int c = B.tile_chart[t];
     if (B.c_final[c]) return;
     double acc = 0.0;
+    if (B.c_bw && B.c_bw[c] > 0) {  // band chart: its FP64 band form
+        const int t0 = B.chart_t0[c];
+        band_rows<double, R, 0>(B, c, t - t0, B.chart_t0[c + 1] - t0, bvals64,
+                                [&](int64_t j) { return ld<double, R>(x64, j); },
+                                [&](int64_t i, const double (&sum)[R]) {
+                                    const Vr<double, R> bi = ld<double, R>(b, i);
+                                    Vr<double, R> ri;
+#pragma unroll
+                                    for (int cc = 0; cc < R; ++cc) {
+                                        ri.c[cc] = bi.c[cc] - sum[cc];
+                                        acc += ri.c[cc] * ri.c[cc];
+                                    }
+                                    st<double, R>(r64, i, ri);
+                                });
+        const double ss = block_sum_all(acc, sh);
+        if (threadIdx.x == 0) B.part_o[t] = ss;
+        return;
+    }
     const int64_t lo = B.crow[c], hi = B.crow[c + 1] - 1;
     const int s1 = B.tile_s0[t + 1];
     for (int s = B.tile_s0[t]; s < s1; ++s) {
