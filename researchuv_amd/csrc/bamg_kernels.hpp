@@ -68,32 +68,38 @@ __global__ __launch_bounds__(BLK) void k_scan_blocks(int64_t n, const int64_t* _
     if (threadIdx.x == BLK - 1) totals[blockIdx.x] = part[BLK - 1];
 }
 
-// one workgroup: exclusive scan of the block totals in place, grand total in totals[nb]
-__global__ __launch_bounds__(BLK) void k_scan_totals(int64_t nb, int64_t* __restrict__ totals) {
-    __shared__ int64_t part[BLK];
-    int64_t carry = 0;
-    for (int64_t b0 = 0; b0 < nb; b0 += BLK) {
-        const int64_t k = b0 + threadIdx.x;
-        const int64_t v = (k < nb) ? totals[k] : 0;
-        part[threadIdx.x] = v;
+// one workgroup of TOT_BLK threads: exclusive scan of the block totals in place, grand total in totals[nb]; each
+// thread sums a contiguous run of the totals, the runs' sums are scanned in LDS, then each run is written out
+constexpr int TOT_BLK = 1024;
+__global__ __launch_bounds__(TOT_BLK) void k_scan_totals(int64_t nb, int64_t* __restrict__ totals) {
+    __shared__ int64_t part[TOT_BLK];
+    const int64_t per = (nb + TOT_BLK - 1) / TOT_BLK;
+    const int64_t a = (int64_t)threadIdx.x * per, b = min(nb, a + per);
+    int64_t sum = 0;
+    for (int64_t k = a; k < b; ++k) sum += totals[k];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < TOT_BLK; off <<= 1) {
+        const int64_t u = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
         __syncthreads();
-        for (int off = 1; off < BLK; off <<= 1) {
-            const int64_t u = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
-            __syncthreads();
-            part[threadIdx.x] += u;
-            __syncthreads();
-        }
-        if (k < nb) totals[k] = carry + part[threadIdx.x] - v;
-        carry += part[BLK - 1];
+        part[threadIdx.x] += u;
         __syncthreads();
     }
-    if (threadIdx.x == 0) totals[nb] = carry;
+    int64_t run = (threadIdx.x == 0) ? 0 : part[threadIdx.x - 1];
+    for (int64_t k = a; k < b; ++k) {
+        const int64_t v = totals[k];
+        totals[k] = run;
+        run += v;
+    }
+    if (threadIdx.x == TOT_BLK - 1) totals[nb] = part[TOT_BLK - 1];
 }
 
-__global__ __launch_bounds__(BLK) void k_scan_add(int64_t n, int64_t* __restrict__ out,
+// out[k] += the exclusive total of k's block; out[n] = the grand total
+__global__ __launch_bounds__(BLK) void k_scan_add(int64_t n, int64_t nb, int64_t* __restrict__ out,
                                                   const int64_t* __restrict__ totals) {
     const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (k < n) out[k] += totals[k / SCAN_BLK];
+    else if (k == n) out[n] = totals[nb];
 }
 
 // ---------------------------------------------------------------- matching
@@ -615,10 +621,27 @@ __global__ __launch_bounds__(BLK) void k_block_charts(int64_t n, const int32_t* 
     blk[b] = make_int2(chart[b * BLK], chart[min(n, (b + 1) * BLK) - 1]);
 }
 
-// the finest operator's values for the cycle's smoothing legs
+// the finest operator's values for the cycle's smoothing legs, 8 a thread (two 16-byte loads, one 16-byte
+// store; both buffers 16-byte aligned)
+constexpr int BF16_VEC = 8;
 __global__ __launch_bounds__(BLK) void k_to_bf16(int64_t n, const float* __restrict__ in, Bf16* __restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i < n) cput(out + i, (double)in[i]);
+    const int64_t i = ((int64_t)blockIdx.x * BLK + threadIdx.x) * BF16_VEC;
+    if (i + BF16_VEC <= n) {
+        const float4 a = *reinterpret_cast<const float4*>(in + i);
+        const float4 b = *reinterpret_cast<const float4*>(in + i + 4);
+        const float f[BF16_VEC] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        Bf16 h[BF16_VEC];
+#pragma unroll
+        for (int k = 0; k < BF16_VEC; ++k) cput(h + k, (double)f[k]);
+        uint4 o;
+        o.x = (uint32_t)h[0].bits | ((uint32_t)h[1].bits << 16);
+        o.y = (uint32_t)h[2].bits | ((uint32_t)h[3].bits << 16);
+        o.z = (uint32_t)h[4].bits | ((uint32_t)h[5].bits << 16);
+        o.w = (uint32_t)h[6].bits | ((uint32_t)h[7].bits << 16);
+        *reinterpret_cast<uint4*>(out + i) = o;
+    } else {
+        for (int64_t k = i; k < n; ++k) cput(out + k, (double)in[k]);
+    }
 }
 
 // the same over listed ranges of entries (first, count), one workgroup per range: the sliced-ELL entries of the
@@ -668,8 +691,12 @@ __global__ __launch_bounds__(BLK) void k_cast(int64_t n, const double* __restric
 // level is a sum over runs of consecutive rows, done in the pass that forms the residual, and the prolongation
 // reads the coarse correction in ascending order.  A level's rows are cut into slices that start where an
 // aggregate starts: slice b holds rows srow[b] .. srow[b + 1], srow[b] = the first aggregate start at or after
-// b * SLICE, and sagg[b] is that aggregate.  Permutations map a row of the constructed order to its stored row;
+// b * CSLICE_STEP, and sagg[b] is that aggregate.  Permutations map a row of the constructed order to its stored row;
 // a null permutation is the identity.  Charts keep their row ranges (aggregates never leave their chart).
+// The slice grid steps by CSLICE_STEP = SLICE - 4 rows, so that a slice (its grid step plus the rest of the last
+// aggregate it starts, at most 3 rows for aggregates of up to 4 rows) fits one row a thread of a workgroup; a
+// slice of more rows has its first threads take a second row.
+constexpr int CSLICE_STEP = SLICE - 4;
 constexpr int MAX_SLICE_ROWS = 4 * SLICE;  // rows of one slice, bounded by the LDS of the restriction
 
 // sz[J] = member count of aggregate I, J = its stored index
@@ -711,7 +738,7 @@ __global__ __launch_bounds__(BLK) void k_perm_inverse(int64_t n, const int32_t* 
     if (i < n) ipos[rpos[i]] = (int32_t)i;
 }
 
-// first aggregate start at or after min(b * SLICE, n) (start ascends over nI + 1 entries, start[nI] = n)
+// first aggregate start at or after target (start ascends over nI + 1 entries, start[nI] = n)
 __device__ __forceinline__ int64_t first_start_at(int64_t target, int64_t nI, const int64_t* __restrict__ start) {
     int64_t lo = 0, hi = nI;  // smallest J with start[J] >= target; start[nI] = n >= target
     while (lo < hi) {
@@ -727,20 +754,20 @@ __global__ __launch_bounds__(BLK) void k_slice_rows(int64_t nsl, int64_t n, int6
                                                     int32_t* __restrict__ sagg, int32_t* __restrict__ tall) {
     const int64_t b = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (b > nsl) return;
-    const int64_t J = first_start_at(min(b * SLICE, n), nI, start);
+    const int64_t J = first_start_at(min(b * CSLICE_STEP, n), nI, start);
     srow[b] = (int32_t)start[J];
     sagg[b] = (int32_t)J;
     if (b < nsl) {
-        const int64_t J1 = first_start_at(min((b + 1) * SLICE, n), nI, start);
+        const int64_t J1 = first_start_at(min((b + 1) * CSLICE_STEP, n), nI, start);
         if (start[J1] - start[J] > MAX_SLICE_ROWS) atomicOr(tall, 1);
     }
 }
 
-// slice of stored row p: the largest b with srow[b] <= p (empty slices are passed over).  srow[b] >= b * SLICE,
-// so it is p / SLICE or, where p lies before that slice's first aggregate start, one of the slices before it
-// (srow[0] = 0 ends the walk).
+// slice of stored row p: the largest b with srow[b] <= p (empty slices are passed over).  srow[b] >=
+// b * CSLICE_STEP, so it is p / CSLICE_STEP or, where p lies before that slice's first aggregate start, one of the
+// slices before it (srow[0] = 0 ends the walk).
 __device__ __forceinline__ int64_t slice_of(int64_t p, int64_t nsl, const int32_t* __restrict__ srow) {
-    int64_t b = min(p / SLICE, nsl - 1);
+    int64_t b = min(p / CSLICE_STEP, nsl - 1);
     while (srow[b] > p) --b;
     return b;
 }
@@ -1010,6 +1037,7 @@ __global__ __launch_bounds__(BLK) void kc_down(CycLevel<T> L, T w, T* __restrict
     }
 }
 
+constexpr int RESTRICT_U = 4;
 // per aggregate I of the next level (chart cnext[I]): b_next[I] = sum over its member rows of t (times
 // 1 / sqrt(d_next,I) when the next level is scaled)
 template <typename T, int R>
@@ -1024,14 +1052,29 @@ __global__ __launch_bounds__(BLK) void kc_restrict(CycLevel<T> L, int64_t nc, co
     T acc[R];
 #pragma unroll
     for (int c = 0; c < R; ++c) acc[c] = T(0);
-    const int32_t q1 = L.mptr[I + 1];
-    for (int32_t q = L.mptr[I]; q < q1; ++q) {
-        const Vr<T, R> tm = ld<T, R>(L.t, L.mrows[q]);
+    const int32_t q0 = L.mptr[I], q1 = L.mptr[I + 1];
+    const T sI = L.rsn ? L.rsn[I] : T(1);
+    // the first RESTRICT_U members' row indices, then their rows, each group of loads issued together without
+    // branches (past the last member the index repeats the last member's; an aggregate has at least one); the sum
+    // keeps member order
+    int32_t m[RESTRICT_U];
 #pragma unroll
-        for (int c = 0; c < R; ++c) acc[c] += tm.c[c];
+    for (int u = 0; u < RESTRICT_U; ++u) m[u] = L.mrows[min(q0 + u, q1 - 1)];
+    Vr<T, R> tm[RESTRICT_U];
+#pragma unroll
+    for (int u = 0; u < RESTRICT_U; ++u) tm[u] = ld<T, R>(L.t, m[u]);
+#pragma unroll
+    for (int u = 0; u < RESTRICT_U; ++u) {
+        const T f = (u == 0 || q0 + u < q1) ? T(1) : T(0);  // 1 * t + acc rounds as t + acc
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[c] = fma(f, tm[u].c[c], acc[c]);
+    }
+    for (int32_t q = q0 + RESTRICT_U; q < q1; ++q) {
+        const Vr<T, R> t1 = ld<T, R>(L.t, L.mrows[q]);
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[c] += t1.c[c];
     }
     Vr<T, R> o;
-    const T sI = L.rsn ? L.rsn[I] : T(1);
 #pragma unroll
     for (int c = 0; c < R; ++c) o.c[c] = L.rsn ? acc[c] * sI : acc[c];
     st<T, R>(bnext, I, o);

@@ -123,9 +123,8 @@ int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) 
     int64_t* tot = nullptr;
     HIPCHK(S.alloc(&tot, (size_t)nb + 1));
     hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(BLK), 0, s, n, in, out, tot);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(BLK), 0, s, nb, tot);
-    hipLaunchKernelGGL(k_scan_add, rows_grid(n), dim3(BLK), 0, s, n, out, (const int64_t*)tot);
-    HIPCHK(hipMemcpyAsync(out + n, tot + nb, sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(TOT_BLK), 0, s, nb, tot);
+    hipLaunchKernelGGL(k_scan_add, rows_grid(n + 1), dim3(BLK), 0, s, n, nb, out, (const int64_t*)tot);
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }
@@ -546,7 +545,7 @@ int bamg_build(BatchImpl& Q) {
         HIPCHK(A->alloc(&L.mptr, nI + 1));
         hipLaunchKernelGGL(k_narrow, rows_grid(nI + 1), dim3(BLK), 0, s, nI + 1, (const int64_t*)start, L.mptr);
         // slices aligned to aggregate starts, their ELL blocks
-        L.nsl = (n + SLICE - 1) / SLICE;
+        L.nsl = (n + CSLICE_STEP - 1) / CSLICE_STEP;
         const int64_t nsl = L.nsl;
         int64_t* sz = nullptr;
         HIPCHK(A->alloc(&L.srow, nsl + 1));
@@ -575,7 +574,7 @@ int bamg_build(BatchImpl& Q) {
         {  // each chart's slices: those holding its first and last rows, and all between
             const std::vector<int64_t>& cr = nat[l].crow;
             auto slice_of = [&](int64_t p) {
-                int64_t b = std::min<int64_t>(p / SLICE, nsl - 1);
+                int64_t b = std::min<int64_t>(p / CSLICE_STEP, nsl - 1);
                 while (hsrow[b] > p) --b;
                 return (int32_t)b;
             };
@@ -643,7 +642,8 @@ int bamg_build(BatchImpl& Q) {
             if (Q.nbv) {
                 Bf16* b16 = nullptr;
                 HIPCHK(A->alloc(&b16, (size_t)Q.nbv));
-                hipLaunchKernelGGL(k_to_bf16, rows_grid(Q.nbv), dim3(BLK), 0, s, Q.nbv, (const float*)Q.bv32, b16);
+                hipLaunchKernelGGL(k_to_bf16, rows_grid((Q.nbv + BF16_VEC - 1) / BF16_VEC), dim3(BLK), 0, s, Q.nbv,
+                                   (const float*)Q.bv32, b16);
                 A->bv0 = b16;
             }
             HIPCHK(hipGetLastError());
