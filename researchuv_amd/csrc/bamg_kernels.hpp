@@ -1325,16 +1325,27 @@ __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict
     if (slot < 0) return;
     const int t = act[slot];
     if (B.c_done[B.tile_chart[t]] != B_RUNNING) return;
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
-        const T di = w * dinv[i];
-        const Vr<T, R> ri = ld<T, R>(r, i);
-        const Vr<T, R> zc = ld<T, R>(z1, agg[i]);
-        Vr<T, R> o;
+    tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i0, int64_t i2, bool two) {
+        const int64_t ix[2] = {i0, i2};
+        int32_t ag[2];
+        T di[2];
+        Vr<T, R> ri[2], zc[2];
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) o.c[cc] = fma(di, ri.c[cc], alpha * zc.c[cc]);
-        st<T, R>(e, i, o);
-    }
+        for (int k = 0; k < 2; ++k) {
+            ag[k] = agg[ix[k]];
+            di[k] = w * dinv[ix[k]];
+            ri[k] = ld<T, R>(r, ix[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) zc[k] = ld<T, R>(z1, ag[k]);
+        Vr<T, R> o[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) o[k].c[cc] = fma(di[k], ri[k].c[cc], alpha * zc[k].c[cc]);
+        st<T, R>(e, i0, o[0]);
+        st<T, R>(e, i2, two ? o[1] : o[0]);  // two false: row i0 once more
+    });
 }
 
 template <typename T, int R, int CI>

@@ -387,6 +387,19 @@ __device__ __forceinline__ void band_rows(const BDev& B, int c, int k, int ntl, 
     }
 }
 
+// f(i, i2, two) over the rows r0 + threadIdx.x + k BLK < r1 of a tile, two at a time (i and i2 = i + BLK), so a
+// body can issue both rows' loads before it stores either; past the tile's last row i2 repeats i and two is false.
+// A body uses the second row's loads unconditionally (where two is false it stores row i's results to row i once
+// more, and adds nothing to its sums through a select): loads used only under a branch are moved into it by the
+// compiler, and then wait for the first row's stores.
+template <typename F>
+__device__ __forceinline__ void tile_rows2(int64_t r0, int64_t r1, F f) {
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 2 * BLK) {
+        const bool two = i + BLK < r1;
+        f(i, two ? i + BLK : i, two);
+    }
+}
+
 // Opens step `iter` (parity par) of every live chart -- convergence on ||r||, max_iter, breakdown, beta from the
 // r.z partials -- then p' = z + beta p over the tile's rows.
 template <typename T, int R>
@@ -429,14 +442,18 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
         B.c_iters[c] = iter + 1;
     }
     const T beta = (T)(rz / rzold);
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
+    tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i, int64_t i2, bool two) {
         const Vr<T, R> zi = ld<T, R>(z, i), pi = ld<T, R>(pold, i);
-        Vr<T, R> pn;
+        const Vr<T, R> z2 = ld<T, R>(z, i2), p2 = ld<T, R>(pold, i2);
+        Vr<T, R> pn, pn2;
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) pn.c[cc] = fma(beta, pi.c[cc], zi.c[cc]);
+        for (int cc = 0; cc < R; ++cc) {
+            pn.c[cc] = fma(beta, pi.c[cc], zi.c[cc]);
+            pn2.c[cc] = fma(beta, p2.c[cc], z2.c[cc]);
+        }
         st<T, R>(pnew, i, pn);
-    }
+        st<T, R>(pnew, i2, two ? pn2 : pn);
+    });
 }
 
 // Rows threadIdx.x of slices s .. s + P - 1 through stencil_group when group_width allows, each finished by fin;
@@ -543,23 +560,40 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
     }
     const T a = (T)(B.c_rz[(int64_t)par * B.nchart + c] / pq);
     double arr = 0.0, arz = 0.0;
-    const int64_t r1 = B.tile_r1[t];
-    for (int64_t i = B.tile_r0[t] + threadIdx.x; i < r1; i += BLK) {
-        const T d = (PC == 1) ? dinv[i] : T(1);
-        const Vr<T, R> pi = ld<T, R>(p, i), qi = ld<T, R>(q, i);
-        Vr<T, R> xi = ld<T, R>(x, i), ri = ld<T, R>(r, i), zi;
+    tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i0, int64_t i2, bool two) {
+        const int64_t ix[2] = {i0, i2};
+        T d[2];
+        Vr<T, R> pi[2], qi[2], xi[2], ri[2], zi[2];
 #pragma unroll
-        for (int cc = 0; cc < R; ++cc) {
-            xi.c[cc] = fma(a, pi.c[cc], xi.c[cc]);
-            ri.c[cc] = fma(-a, qi.c[cc], ri.c[cc]);
-            zi.c[cc] = d * ri.c[cc];
-            arr += (double)ri.c[cc] * (double)ri.c[cc];
-            arz += (double)ri.c[cc] * (double)zi.c[cc];
+        for (int k = 0; k < 2; ++k) {
+            d[k] = (PC == 1) ? dinv[ix[k]] : T(1);
+            pi[k] = ld<T, R>(p, ix[k]);
+            qi[k] = ld<T, R>(q, ix[k]);
+            xi[k] = ld<T, R>(x, ix[k]);
+            ri[k] = ld<T, R>(r, ix[k]);
         }
-        st<T, R>(x, i, xi);
-        st<T, R>(r, i, ri);
-        if (PC == 1) st<T, R>(z, i, zi);
-    }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool on = k == 0 || two;
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) {
+                xi[k].c[cc] = fma(a, pi[k].c[cc], xi[k].c[cc]);
+                ri[k].c[cc] = fma(-a, qi[k].c[cc], ri[k].c[cc]);
+                zi[k].c[cc] = d[k] * ri[k].c[cc];
+                const double vrr = (double)ri[k].c[cc] * (double)ri[k].c[cc];
+                const double vrz = (double)ri[k].c[cc] * (double)zi[k].c[cc];
+                arr += on ? vrr : 0.0;
+                arz += on ? vrz : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int q = (k == 0 || two) ? k : 0;  // where two is false row i's results go to row i again
+            st<T, R>(x, ix[k], xi[q]);
+            st<T, R>(r, ix[k], ri[q]);
+            if (PC == 1) st<T, R>(z, ix[k], zi[q]);
+        }
+    });
     // PC 2 (AMG): the cycle that follows writes z and the r.z partials
     double srr, srz;
     {
