@@ -482,6 +482,8 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
     B.ecol = d_col;
     B.nchart = nchart;
     HIPCHK(Q.alloc(&B.c_rz, 2 * (size_t)nchart));
+    HIPCHK(Q.alloc(&B.c_alpha, nchart));
+    HIPCHK(hipMemsetAsync(B.c_alpha, 0, sizeof(double) * (size_t)std::max(nchart, 1), Q.stream));
     HIPCHK(Q.alloc(&B.c_tol, nchart));
     HIPCHK(Q.alloc(&B.c_bnorm, nchart));
     HIPCHK(Q.alloc(&B.c_true, nchart));
@@ -588,7 +590,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     auto run_chunk = [&](int steps) -> int {
         for (int it = 0; it < steps; ++it) {
             hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
-                               (const T*)pb[par ^ 1], pb[par]);
+                               (const T*)pb[par ^ 1], pb[par], x);
             if (Q.ktime) {
                 if (Q.kev_used + 2 > Q.kev.size()) {
                     hipEvent_t e0, e1;
@@ -607,7 +609,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
             }
             chart_sums(B.part_a, nullptr, B.c_sa, nullptr);
             hipLaunchKernelGGL((kb_update<T, R, PC>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par,
-                               dinv, (const T*)pb[par], (const T*)q, x, r, z);
+                               dinv, (const T*)q, r, z);
             if (PC == 2) {
                 int rc2 = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z);
                 if (rc2) return rc2;
@@ -655,7 +657,7 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     for (int c = 0; c < Q.nchart; ++c) {
         if (Q.h_final[c]) continue;
         Q.iters_total[c] += Q.h_iters[c];
-        if (Q.h_done[c] == B_BREAKDOWN)
+        if (Q.h_done[c] == B_BREAKDOWN || Q.h_done[c] == B_BREAKDOWN_PQ)
             return fail(RUV_ERR_BREAKDOWN, "batch: chart " + std::to_string(c) + " broke down");
         if (Q.h_done[c] == B_MAXITER)
             return fail(RUV_ERR_NOT_CONVERGED, "batch: chart " + std::to_string(c) + " hit max_iter");
@@ -798,15 +800,15 @@ int step_profile_kernel(BatchImpl& Q, int reps) {
     int par = 0;
     for (int it = 0; it < reps; ++it) {
         if ((rc = reset())) return rc;
-        hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, 0u, z,
-                           (const T*)pb[par ^ 1], pb[par]);
-        HIPCHK(Q.mark(BK_DIR, 0, 3.0 * n * vb, 3.0 * n * vb));  // z, old p read; p written
+        hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, 1u, z,
+                           (const T*)pb[par ^ 1], pb[par], x);
+        HIPCHK(Q.mark(BK_DIR, 0, 5.0 * n * vb, 5.0 * n * vb));  // z, old p, x read; p, x written
         launch_spmv<T, R, CI>(Q, na, pb[par], q);
         HIPCHK(Q.mark(BK_SPMV, 0, spmv, spmv_st));
         if ((rc = reset())) return rc;
         hipLaunchKernelGGL((kb_update<T, R, 2>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, dinv,
-                           (const T*)pb[par], (const T*)q, x, r, z);
-        HIPCHK(Q.mark(BK_UPDATE, 0, 6.0 * n * vb, 6.0 * n * vb));  // p, q, x, r read; x, r written
+                           (const T*)q, r, z);
+        HIPCHK(Q.mark(BK_UPDATE, 0, 3.0 * n * vb, 3.0 * n * vb));  // q, r read; r written
         if ((rc = reset())) return rc;
         if ((rc = bamg_apply<T, R, CI>(Q, na, par ^ 1, (const T*)r, z))) return rc;
         par ^= 1;

@@ -53,7 +53,9 @@ constexpr int SLOTS = 8;    // column offsets of a stencil slice
 constexpr int BAND_H = 4;   // band charts: grid rows per thread
 constexpr int XCDS = 8;     // MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
 
-enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4 };
+// chart states; B_BREAKDOWN_PQ: p.q broke down in the update pass (its step's x update is not made), told apart from
+// the direction pass's own stops
+enum : int32_t { B_RUNNING = 0, B_CONVERGED = 1, B_BREAKDOWN = 2, B_MAXITER = 3, B_SKIP = 4, B_BREAKDOWN_PQ = 5 };
 
 struct BDev {
     // sliced ELL (per slice)
@@ -73,6 +75,7 @@ struct BDev {
     const int32_t* chart_t0;  // nchart + 1
     // per chart
     double* c_rz;  // [2][nchart] by step parity
+    double* c_alpha;  // alpha of the chart's last update pass (its x update is made by the next direction pass)
     double* c_tol;
     double* c_bnorm;
     double* c_true;
@@ -401,17 +404,23 @@ __device__ __forceinline__ void tile_rows2(int64_t r0, int64_t r1, F f) {
 }
 
 // Opens step `iter` (parity par) of every live chart -- convergence on ||r||, max_iter, breakdown, beta from the
-// r.z partials -- then p' = z + beta p over the tile's rows.
+// r.z partials -- then p' = z + beta p over the tile's rows, and the previous step's x update x += alpha p (alpha
+// from the update pass, p the old direction this pass reads anyway): also when the chart stops here, so an inner
+// solve ends with x complete.  The pass's workgroups of one chart each make the same decision from the same sums;
+// the first writes the chart's state, so another one of the same launch may already see the new state: a
+// workgroup leaves at once only for a chart stopped before this step (state not running and c_iters below iter --
+// a stop here leaves c_iters at iter) or broken down in the last update pass.
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ act, int na, int par, uint32_t iter,
                                               const T* __restrict__ z, const T* __restrict__ pold,
-                                              T* __restrict__ pnew) {
+                                              T* __restrict__ pnew, T* __restrict__ x) {
     __shared__ double sh[2 * WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
     const int t = act[slot];
     const int c = B.tile_chart[t];
-    if (B.c_done[c] != B_RUNNING) return;
+    const int32_t state = B.c_done[c];
+    if (state != B_RUNNING && (state == B_BREAKDOWN_PQ || B.c_iters[c] < iter)) return;
     const int t0 = B.chart_t0[c], t1 = B.chart_t0[c + 1];
     double rr, rz;
     if (B.big) {  // summed per chart by kb_chart_sums
@@ -421,20 +430,40 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
         sum_range2(B.part_b + (int64_t)par * B.ntile, B.part_c + (int64_t)par * B.ntile, t0, t1, sh, &rr, &rz);
     }
     const bool lead = (t == t0) && threadIdx.x == 0;
+    const T a = iter > 0 ? (T)B.c_alpha[c] : T(0);
+    // the previous step's x update alone, for a chart that stops here
+    const auto x_only = [&]() {
+        if (iter == 0) return;
+        tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i, int64_t i2, bool two) {
+            const Vr<T, R> pi = ld<T, R>(pold, i), xi = ld<T, R>(x, i);
+            const Vr<T, R> p2 = ld<T, R>(pold, i2), x2 = ld<T, R>(x, i2);
+            Vr<T, R> xn, xn2;
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) {
+                xn.c[cc] = fma(a, pi.c[cc], xi.c[cc]);
+                xn2.c[cc] = fma(a, p2.c[cc], x2.c[cc]);
+            }
+            st<T, R>(x, i, xn);
+            st<T, R>(x, i2, two ? xn2 : xn);
+        });
+    };
     if (sqrt(rr) <= B.c_tol[c]) {
         if (lead) {
             B.c_done[c] = B_CONVERGED;
             B.c_rrec[c] = sqrt(rr);
         }
+        x_only();
         return;
     }
     if (iter >= B.max_iter) {
         if (lead) B.c_done[c] = B_MAXITER;
+        x_only();
         return;
     }
     const double rzold = B.c_rz[(int64_t)(par ^ 1) * B.nchart + c];
     if (!isfinite(rz) || !(fabs(rzold) > 0.0)) {
         if (lead) B.c_done[c] = B_BREAKDOWN;
+        x_only();
         return;
     }
     if (lead) {
@@ -442,17 +471,36 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
         B.c_iters[c] = iter + 1;
     }
     const T beta = (T)(rz / rzold);
+    if (iter == 0) {  // no update before the first step
+        tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i, int64_t i2, bool two) {
+            const Vr<T, R> zi = ld<T, R>(z, i), pi = ld<T, R>(pold, i);
+            const Vr<T, R> z2 = ld<T, R>(z, i2), p2 = ld<T, R>(pold, i2);
+            Vr<T, R> pn, pn2;
+#pragma unroll
+            for (int cc = 0; cc < R; ++cc) {
+                pn.c[cc] = fma(beta, pi.c[cc], zi.c[cc]);
+                pn2.c[cc] = fma(beta, p2.c[cc], z2.c[cc]);
+            }
+            st<T, R>(pnew, i, pn);
+            st<T, R>(pnew, i2, two ? pn2 : pn);
+        });
+        return;
+    }
     tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i, int64_t i2, bool two) {
-        const Vr<T, R> zi = ld<T, R>(z, i), pi = ld<T, R>(pold, i);
-        const Vr<T, R> z2 = ld<T, R>(z, i2), p2 = ld<T, R>(pold, i2);
-        Vr<T, R> pn, pn2;
+        const Vr<T, R> zi = ld<T, R>(z, i), pi = ld<T, R>(pold, i), xi = ld<T, R>(x, i);
+        const Vr<T, R> z2 = ld<T, R>(z, i2), p2 = ld<T, R>(pold, i2), x2 = ld<T, R>(x, i2);
+        Vr<T, R> pn, pn2, xn, xn2;
 #pragma unroll
         for (int cc = 0; cc < R; ++cc) {
             pn.c[cc] = fma(beta, pi.c[cc], zi.c[cc]);
             pn2.c[cc] = fma(beta, p2.c[cc], z2.c[cc]);
+            xn.c[cc] = fma(a, pi.c[cc], xi.c[cc]);
+            xn2.c[cc] = fma(a, p2.c[cc], x2.c[cc]);
         }
         st<T, R>(pnew, i, pn);
         st<T, R>(pnew, i2, two ? pn2 : pn);
+        st<T, R>(x, i, xn);
+        st<T, R>(x, i2, two ? xn2 : xn);
     });
 }
 
@@ -539,13 +587,13 @@ __global__ __launch_bounds__(BLK) void kb_spmv(BDev B, const int* __restrict__ a
     if (threadIdx.x == 0) B.part_a[t] = sum;
 }
 
-// alpha = rz / p.q;  x += alpha p;  r -= alpha q;  z = D^-1 r (PC 1; z aliases r for PC 0; PC 2 leaves z and
-// the r.z partials to the AMG cycle); partials r.r and r.z for step parity par ^ 1.
+// alpha = rz / p.q (kept per chart: the next direction pass makes x += alpha p);  r -= alpha q;  z = D^-1 r
+// (PC 1; z aliases r for PC 0; PC 2 leaves z and the r.z partials to the AMG cycle); partials r.r and r.z for step
+// parity par ^ 1.
 template <typename T, int R, int PC>
 __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__ act, int na, int par,
-                                                 const T* __restrict__ dinv, const T* __restrict__ p,
-                                                 const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                 T* __restrict__ z) {
+                                                 const T* __restrict__ dinv, const T* __restrict__ q,
+                                                 T* __restrict__ r, T* __restrict__ z) {
     __shared__ double sh[2 * WAVES];
     const int slot = xcd_slot(na);
     if (slot < 0) return;
@@ -555,21 +603,20 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
     const int t0 = B.chart_t0[c];
     const double pq = B.big ? B.c_sa[c] : sum_range(B.part_a, t0, B.chart_t0[c + 1], sh);
     if (!(fabs(pq) > 1e-300) || !isfinite(pq)) {
-        if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_BREAKDOWN;
+        if (t == t0 && threadIdx.x == 0) B.c_done[c] = B_BREAKDOWN_PQ;
         return;
     }
     const T a = (T)(B.c_rz[(int64_t)par * B.nchart + c] / pq);
+    if (t == t0 && threadIdx.x == 0) B.c_alpha[c] = (double)a;
     double arr = 0.0, arz = 0.0;
     tile_rows2(B.tile_r0[t], B.tile_r1[t], [&](int64_t i0, int64_t i2, bool two) {
         const int64_t ix[2] = {i0, i2};
         T d[2];
-        Vr<T, R> pi[2], qi[2], xi[2], ri[2], zi[2];
+        Vr<T, R> qi[2], ri[2], zi[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             d[k] = (PC == 1) ? dinv[ix[k]] : T(1);
-            pi[k] = ld<T, R>(p, ix[k]);
             qi[k] = ld<T, R>(q, ix[k]);
-            xi[k] = ld<T, R>(x, ix[k]);
             ri[k] = ld<T, R>(r, ix[k]);
         }
 #pragma unroll
@@ -577,7 +624,6 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
             const bool on = k == 0 || two;
 #pragma unroll
             for (int cc = 0; cc < R; ++cc) {
-                xi[k].c[cc] = fma(a, pi[k].c[cc], xi[k].c[cc]);
                 ri[k].c[cc] = fma(-a, qi[k].c[cc], ri[k].c[cc]);
                 zi[k].c[cc] = d[k] * ri[k].c[cc];
                 const double vrr = (double)ri[k].c[cc] * (double)ri[k].c[cc];
@@ -589,7 +635,6 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int q = (k == 0 || two) ? k : 0;  // where two is false row i's results go to row i again
-            st<T, R>(x, ix[k], xi[q]);
             st<T, R>(r, ix[k], ri[q]);
             if (PC == 1) st<T, R>(z, ix[k], zi[q]);
         }

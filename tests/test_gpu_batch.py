@@ -569,7 +569,9 @@ def test_batch_step_profile(mixed):
     vb = 2 * (4 if mixed else 8)
     n = rows[0]
     upd = [e for e in prof if e["kernel"] == "kb_update"][0]
-    assert upd["bytes"] == 6 * n * vb  # p, q, x, r read; x, r written
+    assert upd["bytes"] == 3 * n * vb  # q, r read; r written
+    dr = [e for e in prof if e["kernel"] == "kb_dir"][0]
+    assert dr["bytes"] == 5 * n * vb  # z, old p, x read; p, x written (x += alpha p of the previous step)
     coarsest = [e for e in prof if e["kernel"] == "kc_coarsest"][0]
     assert coarsest["level"] == nl - 1
     x2, _, _, _ = bt.solve(a["rhs"], max_iter=20000, tol=1e-10)
