@@ -58,7 +58,8 @@ struct Bamg {
     // with aggregates of about two rows the plain correction is too short, and alpha = 1.2 cut the PCG
     // iterations by about a third on height fields (49 -> 34 on a 256^2 patch in a CPU model of the cycle)
     double alpha = 1.2;
-    // levels of at most this many rows fold the prolongation into the post-smoothing pass (kc_post<FUSED>)
+    // levels of at most this many rows (live rows, late in a solve) fold the prolongation into the post-smoothing
+    // pass (kc_post<FUSED>)
     int64_t fuse_rows = 65536;
     double setup_seconds = 0.0;
     // Lists of the slices and BLK-row blocks of the running charts per level, for launches late in a solve
@@ -837,7 +838,9 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         if (ns == 0 || nb == 0) continue;
         const T* zn = (const T*)A.lv[l + 1].z;
         const double nL = (double)A.lv[l].n, nN = (double)A.lv[l + 1].n;
-        if (A.lv[l].n <= A.fuse_rows) {  // small level: the prolongation inside the post-smoothing pass
+        // small level, or few rows of it live late in a solve: the prolongation inside the post-smoothing pass
+        const int64_t live = A.slist[l] ? A.nslist[l] * (int64_t)SLICE : A.lv[l].n;
+        if (live <= A.fuse_rows) {
             if (A.lv[l].ci == 1)
                 hipLaunchKernelGGL((kc_post<T, R, 1, 1>), g1(ns), dim3(BLK), 0, s, cyc(l), w, (T)A.alpha, zn);
             else if (A.lv[l].ci == 2)
