@@ -817,6 +817,7 @@ __device__ __forceinline__ double rsqrt_or0(double d) { return d > 0.0 ? 1.0 / s
 // (constructed row ipos[p]), so the stores of a slot are contiguous.
 // CI 2 (FP32 cycle, 16-bit offsets): value and offset packed in one word, the bfloat16 value in the high half and
 // the offset in the low half (ec, uint32), so a slot is one load; ev is unused.
+constexpr int PF_U = 8;
 template <typename T, int CI>
 __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const double* __restrict__ diag,
                                                    const int32_t* __restrict__ rpos,
@@ -830,23 +831,50 @@ __global__ __launch_bounds__(BLK) void k_pell_fill(int64_t n, SetupCsr A, const 
     const int64_t b = slice_of(p, nsl, srow);
     const int64_t ns = srow[b + 1] - srow[b];
     const int64_t e0 = off[b] + (p - srow[b]);
-    int k = 0;
     const double si = rsqrt_or0(diag[i]);
-    for (int64_t a = A.ptr[i]; a < A.ptr[i + 1]; ++a, ++k) {
-        const int64_t c = rpos ? rpos[A.cols[a]] : A.cols[a];
-        const double v = (double)A.vals[a] * (si * rsqrt_or0(diag[A.cols[a]]));
-        if constexpr (CI == 2) {
-            Bf16 h;
-            cput(&h, v);
-            static_cast<uint32_t*>(ec)[e0 + k * ns] = ((uint32_t)h.bits << 16) | (uint32_t)(uint16_t)(int16_t)(c - p);
-            continue;
-        } else {
-            cput(ev + e0 + k * ns, v);
+    const int64_t a0 = A.ptr[i], a1 = A.ptr[i + 1];
+    // PF_U entries at a time: their columns and values, then the columns' stored rows and diagonals, each group
+    // of loads issued together; past the row's end the index repeats its last entry, whose slot is then written
+    // again with the same word (so the stores need no branch)
+    const auto batch = [&](int64_t ab, auto perm) {
+        constexpr bool P = decltype(perm)::value;
+        int32_t cj[PF_U];
+        float vj[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+            const int64_t a = min(ab + u, a1 - 1);
+            cj[u] = A.cols[a];
+            vj[u] = A.vals[a];
         }
-        if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)(c - p);
-        else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)c;
-    }
-    for (; k < w[b]; ++k) {
+        int64_t cs[PF_U];
+        double dj[PF_U];
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+            cs[u] = P ? rpos[cj[u]] : cj[u];
+            dj[u] = diag[cj[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < PF_U; ++u) {
+            const int64_t k = min(ab + u, a1 - 1) - a0;
+            const int64_t c = cs[u];
+            const double v = (double)vj[u] * (si * rsqrt_or0(dj[u]));
+            if constexpr (CI == 2) {
+                Bf16 h;
+                cput(&h, v);
+                static_cast<uint32_t*>(ec)[e0 + k * ns] =
+                    ((uint32_t)h.bits << 16) | (uint32_t)(uint16_t)(int16_t)(c - p);
+            } else {
+                cput(ev + e0 + k * ns, v);
+                if constexpr (CI == 0) static_cast<int16_t*>(ec)[e0 + k * ns] = (int16_t)(c - p);
+                else static_cast<int32_t*>(ec)[e0 + k * ns] = (int32_t)c;
+            }
+        }
+    };
+    if (rpos)
+        for (int64_t ab = a0; ab < a1; ab += PF_U) batch(ab, std::true_type{});
+    else
+        for (int64_t ab = a0; ab < a1; ab += PF_U) batch(ab, std::false_type{});
+    for (int64_t k = a1 - a0; k < w[b]; ++k) {
         if constexpr (CI == 2) {
             static_cast<uint32_t*>(ec)[e0 + k * ns] = 0u;
             continue;
