@@ -558,24 +558,23 @@ __global__ __launch_bounds__(BLK) void k_galerkin_stage(int64_t nrow, const int6
     for (int64_t q = mptr[I]; q < m1 && !full; ++q) {
         const int64_t r = mrows[q];
         const int64_t e1 = A.ptr[r + 1];
-        // entries in batches of GR_BATCH: the column loads, then the aggregate lookups, are issued together
+        // entries in batches of GR_BATCH: the column loads, then the aggregate lookups, are issued together with
+        // no branch (past the row's end the index repeats its last entry, merged again as + 0.0: its slot exists,
+        // and a slot's sum of nonzero entries from +0.0 is never -0.0, so adding +0.0 changes nothing; loads used
+        // only under a branch would be moved into it and serialised)
         for (int64_t e0 = A.ptr[r]; e0 < e1; e0 += GR_BATCH) {
             int32_t cb[GR_BATCH];
-            double vb[GR_BATCH];
-#pragma unroll
-            for (int u = 0; u < GR_BATCH; ++u)
-                if (e0 + u < e1) {
-                    cb[u] = A.cols[e0 + u];
-                    vb[u] = A.vals[e0 + u];
-                }
-#pragma unroll
-            for (int u = 0; u < GR_BATCH; ++u)
-                if (e0 + u < e1) cb[u] = agg[cb[u]];
+            float vb[GR_BATCH];
 #pragma unroll
             for (int u = 0; u < GR_BATCH; ++u) {
-                if (e0 + u >= e1) break;
-                merge(cb[u], vb[u]);
+                const int64_t e = min(e0 + u, e1 - 1);
+                cb[u] = A.cols[e];
+                vb[u] = A.vals[e];
             }
+#pragma unroll
+            for (int u = 0; u < GR_BATCH; ++u) cb[u] = agg[cb[u]];
+#pragma unroll
+            for (int u = 0; u < GR_BATCH; ++u) merge(cb[u], e0 + u < e1 ? (double)vb[u] : 0.0);
         }
     }
     if (full) {
