@@ -168,15 +168,25 @@ __device__ __forceinline__ bool band_row(const BandSrc& S, int64_t i, int32_t (&
     const int off[9] = {-d[3], -d[2], -d[1], -d[0], 0, d[0], d[1], d[2], d[3]};
     const int slots[9] = {sl[3], sl[2], sl[1], sl[0], 0, sl[0], sl[1], sl[2], sl[3]};
     const int bits[9] = {3, 2, 1, 0, -1, 0, 1, 2, 3};
+    int64_t at[9];
+    bool ok[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const int r = q < 4 ? l + off[q] : l;  // a lower entry is the upper entry of row l - d
-        const bool ok = r >= 0 && (bits[q] < 0 || ((m >> bits[q]) & 1));
-        const int64_t at = base + (int64_t)slots[q] * n + (ok ? r : l);
-        const float v = S.f32 ? S.f32[at] : (float)S.f64[at];
-        val[q] = ok ? v : 0.0f;
+        ok[q] = r >= 0 && (bits[q] < 0 || ((m >> bits[q]) & 1));
+        at[q] = base + (int64_t)slots[q] * n + (ok[q] ? r : l);
         col[q] = (int32_t)(i + off[q]);
     }
+    // the nine loads issue together (one branch on the precision, outside the loop)
+    if (S.f32) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) val[q] = S.f32[at[q]];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) val[q] = (float)S.f64[at[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) val[q] = ok[q] ? val[q] : 0.0f;
     return true;
 }
 
@@ -206,6 +216,7 @@ __device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
 // -a_ij >= theta * the row's strongest -- by pair_pref (ties to the lower row), keeping the best MCAND; every round
 // an unmatched row proposes the first of them still unmatched, and mutual proposals become pairs.
 constexpr int MCAND = 4;
+constexpr int CAND_B = 8;  // CSR entries whose loads issue together
 
 __device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, int32_t k) {
     return p > q || (p == q && j < k);
@@ -227,23 +238,42 @@ __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, BandSrc band, co
     float bv[9];
     const bool fromband = in && band_row(band, i, bc, bv);
     const int64_t e0 = (in && !fromband) ? A.ptr[i] : 0, e1 = (in && !fromband) ? A.ptr[i + 1] : 0;
-    // f(j, a_ij) over the row's entries in ascending column order
-    const auto each = [&](auto&& f) {
+    // f(j, a_ij, group of j) over the row's entries in ascending column order (the group read when G).  CSR rows
+    // in batches of CAND_B entries whose loads issue together with no branch: past the row's end the index repeats
+    // its last entry, passed as a_ij = 0, which both passes skip (it adds 0 to the sums: the diagonal and |a| sums
+    // start at +0.0 and add nonzero entries, so adding +0.0 changes nothing)
+    const auto each = [&](auto&& f, auto gflag) {
+        constexpr bool G = decltype(gflag)::value;
         if (fromband) {
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                if (bv[q] != 0.0f) f(bc[q], bv[q]);
+                if (bv[q] != 0.0f) f(bc[q], bv[q], 0);
         } else {
-            for (int64_t e = e0; e < e1; ++e) f(A.cols[e], A.vals[e]);
+            for (int64_t e = e0; e < e1; e += CAND_B) {
+                int32_t cj[CAND_B], gj[CAND_B];
+                float vj[CAND_B];
+#pragma unroll
+                for (int u = 0; u < CAND_B; ++u) {
+                    const int64_t a = min(e + u, e1 - 1);
+                    cj[u] = A.cols[a];
+                    vj[u] = A.vals[a];
+                }
+#pragma unroll
+                for (int u = 0; u < CAND_B; ++u) gj[u] = G ? group[cj[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < CAND_B; ++u) f(cj[u], e + u < e1 ? vj[u] : 0.0f, gj[u]);
+            }
         }
     };
     double smax = 0.0, asum = 0.0, d = 0.0;
-    each([&](int32_t j, float a) {
-        const double v = a;
-        asum += fabs(v);
-        if (j == i) d += v;
-        else if (v < 0.0) smax = fmax(smax, -v);
-    });
+    each(
+        [&](int32_t j, float a, int32_t) {
+            const double v = a;
+            asum += fabs(v);
+            if (j == i) d += v;
+            else if (v < 0.0) smax = fmax(smax, -v);
+        },
+        std::false_type{});
     const double gb = block_max_all((d > 0.0) ? asum / d : 0.0, sh);
     // one atomic on one word from every workgroup serialises (about 12 ns each: 1.7 ms for the 142 000 workgroups
     // of 36 M rows); the bound only grows, so a workgroup whose maximum the word already covers skips it
@@ -262,9 +292,9 @@ __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, BandSrc band, co
         kj[k] = INT32_MAX;
     }
     int cnt = 0;
-    each([&](int32_t j, float a) {
+    each([&](int32_t j, float a, int32_t gj) {
         // a band row's neighbours lie in its chart, the level-0 group
-        if (j == i || !(a < 0.0f) || (!fromband && group[j] != g)) return;
+        if (j == i || !(a < 0.0f) || (!fromband && gj != g)) return;
         const double sv = -(double)a;
         if (sv < thr) return;
         // carry (p, j) down the descending list: it takes the first slot it precedes, or the first free one
@@ -289,7 +319,7 @@ __global__ __launch_bounds__(BLK) void k_candidates(SetupCsr A, BandSrc band, co
             }
         }
         if (cnt < MCAND) ++cnt;
-    });
+    }, std::true_type{});
 #pragma unroll
     for (int k = 0; k < MCAND; ++k)
         if (k < cnt) cand[i * MCAND + k] = kj[k];
