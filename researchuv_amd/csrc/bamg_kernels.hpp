@@ -216,7 +216,7 @@ __device__ __forceinline__ uint64_t pair_pref(double s, int64_t i, int64_t j) {
 // -a_ij >= theta * the row's strongest -- by pair_pref (ties to the lower row), keeping the best MCAND; every round
 // an unmatched row proposes the first of them still unmatched, and mutual proposals become pairs.
 constexpr int MCAND = 4;
-constexpr int CAND_B = 8;  // CSR entries whose loads issue together
+constexpr int CAND_B = 4;  // CSR entries whose loads issue together (8 costs occupancy)
 
 __device__ __forceinline__ bool pref_before(uint64_t p, int32_t j, uint64_t q, int32_t k) {
     return p > q || (p == q && j < k);
