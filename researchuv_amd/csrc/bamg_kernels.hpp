@@ -1323,7 +1323,8 @@ __global__ __launch_bounds__(BLK) void k_coarsest_chol(SetupCsr A, const int64_t
 // Finest level (the batch's sliced-ELL operator, or the band form of a band chart), per live tile.  b is the PCG
 // residual r.
 //   down: t = r - A (w D^-1 r) for the tile's rows; kc_restrict then sums t over each aggregate.
-template <typename T, int R, int CI>
+// FORM: 0 any tile, 1 tiles of band charts only, 2 the others only (each launch then carries one path's registers)
+template <typename T, int R, int CI, int FORM = 0>
 __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__ act, int na,
                                                  const typename CoarseVal<T>::type* __restrict__ vals,
                                                  const typename CoarseVal<T>::type* __restrict__ bvals,
@@ -1349,7 +1350,7 @@ __global__ __launch_bounds__(BLK) void kc_resid0(BDev B, const int* __restrict__
         for (int cc = 0; cc < R; ++cc) o.c[cc] = ri.c[cc] - acc[cc];
         st<T, R>(t_out, i, o);
     };
-    if (B.c_bw && B.c_bw[c] > 0) {
+    if (FORM == 1 || (FORM == 0 && B.c_bw && B.c_bw[c] > 0)) {
         const int t0 = B.chart_t0[c];
         band_rows<T, R, 0>(B, c, t - t0, B.chart_t0[c + 1] - t0, bvals, g, fin);
         return;

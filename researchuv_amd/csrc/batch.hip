@@ -512,14 +512,20 @@ int batch_setup(BatchImpl& Q, int64_t nrows, const int64_t* ptr, const int32_t* 
 // between two halves, so a list can be written while the copy of the previous one is still queued (the copy of the
 // one before that has completed: every caller waits on the stream, or on a look recorded after that copy, at
 // least once between two lists).
+// The tiles of band charts come first (Q.na_band of them), so a pass whose two paths differ in register use can
+// launch one kernel per storage form.
 template <typename Pred>
 int set_active(BatchImpl& Q, Pred pred, int* count) {
     Q.act_half ^= 1;
     int* h = Q.h_act + (size_t)Q.act_half * (size_t)Q.ntile;
     int n = 0;
-    for (int c = 0; c < Q.nchart; ++c)
-        if (pred(c))
-            for (int t = Q.chart_t0[c]; t < Q.chart_t0[c + 1]; ++t) h[n++] = t;
+    const bool hb = !Q.h_bw.empty();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int c = 0; c < Q.nchart; ++c)
+            if ((hb && Q.h_bw[c] > 0) == (pass == 0) && pred(c))
+                for (int t = Q.chart_t0[c]; t < Q.chart_t0[c + 1]; ++t) h[n++] = t;
+        if (pass == 0) Q.na_band = n;
+    }
     *count = n;
     if (n) HIPCHK(hipMemcpyAsync(Q.act, h, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, Q.stream));
     return RUV_OK;

@@ -774,8 +774,16 @@ int bamg_apply(BatchImpl& Q, int na, int par, const T* r, T* z) {
         return RUV_OK;
     };
     int rc = RUV_OK;
-    hipLaunchKernelGGL((kc_resid0<T, R, CI>), gt, dim3(BLK), 0, s, Q.B, (const int*)Q.act, na, vals, bvals,
-                       (const T*)L0.dinv, w0, r, (T*)L0.t);
+    {  // the tiles of band charts (listed first) and the others, one launch each
+        const int nb = std::min(std::max(Q.na_band, 0), na);
+        const auto g = [](int n) { return dim3((unsigned)(((n + XCDS - 1) / XCDS) * XCDS)); };
+        if (nb > 0)
+            hipLaunchKernelGGL((kc_resid0<T, R, CI, 1>), g(nb), dim3(BLK), 0, s, Q.B, (const int*)Q.act, nb, vals,
+                               bvals, (const T*)L0.dinv, w0, r, (T*)L0.t);
+        if (na > nb)
+            hipLaunchKernelGGL((kc_resid0<T, R, CI, 2>), g(na - nb), dim3(BLK), 0, s, Q.B, (const int*)Q.act + nb,
+                               na - nb, vals, bvals, (const T*)L0.dinv, w0, r, (T*)L0.t);
+    }
     if (prof) {  // r, dinv read, t written
         const double v = n0 * (2.0 * vb + db);
         if ((rc = mark(BK_RESID0, 0, fin + v, fin_st + v))) return rc;

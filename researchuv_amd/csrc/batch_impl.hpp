@@ -110,6 +110,7 @@ struct BatchImpl {
     int* h_done_b = nullptr;           // second pinned flag buffer (the look one chunk late)
     hipEvent_t fev[2] = {nullptr, nullptr};
     int act_half = 0;                  // half of h_act the last list was staged in
+    int na_band = 0;                   // leading tiles of the last list that belong to band charts
     // 1: look at which charts run one chunk late while most tiles run, so the device never waits for a look
     // (measured on the 4096-patch batch: 154.5 ms against 152.9 ms for looking right after each chunk)
     int look_lag = 0;
