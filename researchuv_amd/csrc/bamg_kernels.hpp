@@ -1379,7 +1379,7 @@ __global__ __launch_bounds__(BLK) void kc_prolong0(BDev B, const int* __restrict
                                                    const T* __restrict__ dinv, T w, T alpha,
                                                    const T* __restrict__ r, const int32_t* __restrict__ agg,
                                                    const T* __restrict__ z1, T* __restrict__ e) {
-    const int slot = xcd_slot(na);
+    const int slot = xcd_slot_rev(na);
     if (slot < 0) return;
     const int t = act[slot];
     if (B.c_done[B.tile_chart[t]] != B_RUNNING) return;

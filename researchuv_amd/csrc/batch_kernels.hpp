@@ -139,6 +139,14 @@ __device__ __forceinline__ int xcd_slot(int na) {
     const int slot = (b % XCDS) * per + b / XCDS;
     return slot < na ? slot : -1;
 }
+// the same slots, each XCD walking its range from the end: a pass that reads what the pass before it wrote in the
+// other direction meets the most recently written rows first, while they are still in the memory-side cache
+__device__ __forceinline__ int xcd_slot_rev(int na) {
+    const int per = (na + XCDS - 1) / XCDS;
+    const int b = (int)blockIdx.x;
+    const int slot = (b % XCDS) * per + (per - 1 - b / XCDS);
+    return slot < na ? slot : -1;
+}
 
 // sums of two partial arrays over [a, b), fixed order, identical in every workgroup
 __device__ __forceinline__ void sum_range2(const double* __restrict__ pa, const double* __restrict__ pb, int64_t a,
@@ -415,7 +423,7 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
                                               const T* __restrict__ z, const T* __restrict__ pold,
                                               T* __restrict__ pnew, T* __restrict__ x) {
     __shared__ double sh[2 * WAVES];
-    const int slot = xcd_slot(na);
+    const int slot = xcd_slot_rev(na);
     if (slot < 0) return;
     const int t = act[slot];
     const int c = B.tile_chart[t];
@@ -595,7 +603,7 @@ __global__ __launch_bounds__(BLK) void kb_update(BDev B, const int* __restrict__
                                                  const T* __restrict__ dinv, const T* __restrict__ q,
                                                  T* __restrict__ r, T* __restrict__ z) {
     __shared__ double sh[2 * WAVES];
-    const int slot = xcd_slot(na);
+    const int slot = xcd_slot_rev(na);
     if (slot < 0) return;
     const int t = act[slot];
     const int c = B.tile_chart[t];
