@@ -334,15 +334,20 @@ __global__ __launch_bounds__(BLK) void k_propose(int64_t n, const int32_t* __res
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     int64_t best = -1;
-    if (mate[i] < 0) {
-        const int c = ncand[i];
-        for (int k = 0; k < c; ++k) {
-            const int32_t j = cand[i * MCAND + k];
-            if (mate[j] < 0) {
-                best = j;
-                break;
-            }
-        }
+    // the row's state, candidate count and candidates, then the candidates' states, each group of loads issued
+    // together (a candidate past the count repeats the first, whose state is loaded anyway)
+    const int64_t mi = mate[i];
+    const int c = ncand[i];
+    int32_t cj[MCAND];
+#pragma unroll
+    for (int k = 0; k < MCAND; ++k) cj[k] = cand[i * MCAND + (k < c ? k : 0)];
+    if (mi < 0 && c > 0) {
+        int64_t mj[MCAND];
+#pragma unroll
+        for (int k = 0; k < MCAND; ++k) mj[k] = mate[cj[k]];
+#pragma unroll
+        for (int k = MCAND - 1; k >= 0; --k)
+            if (k < c && mj[k] < 0) best = cj[k];
     }
     prop[i] = best;
 }
@@ -350,8 +355,9 @@ __global__ __launch_bounds__(BLK) void k_propose(int64_t n, const int32_t* __res
 __global__ __launch_bounds__(BLK) void k_accept(int64_t n, const int64_t* __restrict__ prop,
                                                 int64_t* __restrict__ mate) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (i >= n || mate[i] >= 0) return;
-    const int64_t j = prop[i];
+    if (i >= n) return;
+    const int64_t mi = mate[i], j = prop[i];  // loaded together
+    if (mi >= 0) return;
     if (j >= 0 && prop[j] == i) mate[i] = j;
 }
 
@@ -363,15 +369,19 @@ __global__ __launch_bounds__(BLK) void k_join(int64_t n, const int32_t* __restri
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     int64_t best = -1;
-    if (mate[i] < 0) {
-        const int c = ncand[i];
-        for (int k = 0; k < c; ++k) {
-            const int64_t j = cand[i * MCAND + k];
-            if (mate[j] >= 0) {
-                best = min(j, mate[j]);
-                break;
-            }
-        }
+    // loads grouped as in k_propose
+    const int64_t mi = mate[i];
+    const int c = ncand[i];
+    int32_t cj[MCAND];
+#pragma unroll
+    for (int k = 0; k < MCAND; ++k) cj[k] = cand[i * MCAND + (k < c ? k : 0)];
+    if (mi < 0 && c > 0) {
+        int64_t mj[MCAND];
+#pragma unroll
+        for (int k = 0; k < MCAND; ++k) mj[k] = mate[cj[k]];
+#pragma unroll
+        for (int k = MCAND - 1; k >= 0; --k)
+            if (k < c && mj[k] >= 0) best = min((int64_t)cj[k], mj[k]);
     }
     join[i] = best;
 }
