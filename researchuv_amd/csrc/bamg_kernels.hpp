@@ -434,6 +434,26 @@ __global__ __launch_bounds__(BLK) void k_member_sort(int64_t nc, const int64_t* 
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (I >= nc) return;
     const int64_t a = mptr[I], b = mptr[I + 1];
+    if (b - a <= 4) {  // the usual sizes (pairs, and rows that joined them): sorted in registers, loads together
+        int64_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (a + k < b) ? mrows[min(a + k, b - 1)] : INT64_MAX;
+        const auto cs = [&](int x, int y) {
+            const int64_t lo = min(v[x], v[y]), hi = max(v[x], v[y]);
+            v[x] = lo;
+            v[y] = hi;
+        };
+        cs(0, 1);
+        cs(2, 3);
+        cs(0, 2);
+        cs(1, 3);
+        cs(1, 2);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (a + k < b) mrows[a + k] = v[k];
+        if (group_c && b > a) group_c[I] = group[v[0]];
+        return;
+    }
     for (int64_t p = a + 1; p < b; ++p) {
         const int64_t v = mrows[p];
         int64_t q = p - 1;
@@ -745,6 +765,7 @@ __global__ __launch_bounds__(BLK) void k_perm_sizes(int64_t nI, const int32_t* _
     if (I < nI) sz[pos ? pos[I] : I] = mptr[I + 1] - mptr[I];
 }
 
+constexpr int PE_U = 4;
 // the k-th member m of aggregate I (stored at J) goes to stored row start[J] + k: rpos[m] = start[J] + k, and
 // mout[start[J] + k] = m (each skipped when null)
 __global__ __launch_bounds__(BLK) void k_perm_expand(int64_t nI, const int32_t* __restrict__ pos,
@@ -754,10 +775,18 @@ __global__ __launch_bounds__(BLK) void k_perm_expand(int64_t nI, const int32_t* 
     const int64_t I = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (I >= nI) return;
     const int64_t J = pos ? pos[I] : I, s0 = start[J], m0 = mptr[I], m1 = mptr[I + 1];
-    for (int64_t k = 0; k < m1 - m0; ++k) {
-        const int64_t m = mrows[m0 + k];
-        if (rpos) rpos[m] = (int32_t)(s0 + k);
-        if (mout) mout[s0 + k] = (int32_t)m;
+    // members PE_U at a time, their loads issued together; past the last member the index repeats it (the same
+    // values stored again)
+    for (int64_t k0 = 0; k0 < m1 - m0; k0 += PE_U) {
+        int64_t m[PE_U];
+#pragma unroll
+        for (int u = 0; u < PE_U; ++u) m[u] = mrows[m0 + min(k0 + u, m1 - m0 - 1)];
+#pragma unroll
+        for (int u = 0; u < PE_U; ++u) {
+            const int64_t k = min(k0 + u, m1 - m0 - 1);
+            if (rpos) rpos[m[u]] = (int32_t)(s0 + k);
+            if (mout) mout[s0 + k] = (int32_t)m[u];
+        }
     }
 }
 
@@ -838,13 +867,21 @@ __global__ __launch_bounds__(BLK) void k_poffsets_wide(int64_t n, const int64_t*
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
     const int64_t p = rpos ? rpos[i] : i;
-    for (int64_t a = ptr[i]; a < ptr[i + 1]; ++a) {
-        const int64_t d = (int64_t)(rpos ? rpos[cols[a]] : cols[a]) - p;
-        if (d > 32767 || d < -32767) {
-            atomicOr(wide, 1);
-            return;
+    const int64_t a0 = ptr[i], a1 = ptr[i + 1];
+    // entries 8 at a time: columns, then their stored rows, each group of loads issued together (past the row's
+    // end the index repeats its last entry)
+    bool far = false;
+    for (int64_t ab = a0; ab < a1; ab += 8) {
+        int32_t c[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = cols[min(ab + u, a1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t d = (int64_t)(rpos ? rpos[c[u]] : c[u]) - p;
+            far = far || d > 32767 || d < -32767;
         }
     }
+    if (far) atomicOr(wide, 1);
 }
 
 // 1 / sqrt(d) of a diagonal entry, 0 for a non-positive one (a row without couplings keeps zeros)
