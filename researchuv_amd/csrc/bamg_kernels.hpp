@@ -35,37 +35,67 @@ namespace ruv {
 constexpr int SCAN_BLK = 1024;  // elements per workgroup of the scan
 
 // ---------------------------------------------------------------- exclusive scan of int64 counts
-__global__ __launch_bounds__(BLK) void k_scan_blocks(int64_t n, const int64_t* __restrict__ in,
-                                                     int64_t* __restrict__ out, int64_t* __restrict__ totals) {
-    __shared__ int64_t sh[SCAN_BLK];
+// Reduce, then scan: k_scan_reduce sums each block of SCAN_BLK entries, k_scan_totals scans the block sums,
+// k_scan_down scans each block again with its offset and writes the result (two reads and one write of the
+// entries).  A thread owns SCAN_BLK / BLK consecutive entries; the waves scan their threads' sums by shuffles.
+constexpr int SCAN_PER = SCAN_BLK / BLK;
+
+// the thread's entries (zeros past n) and their sum
+__device__ __forceinline__ int64_t scan_load(int64_t n, const int64_t* __restrict__ in, int64_t base,
+                                             int64_t (&v)[SCAN_PER]) {
+    int64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; ++q) {
+        const int64_t k = base + threadIdx.x * SCAN_PER + q;
+        v[q] = k < n ? in[k] : 0;
+        sum += v[q];
+    }
+    return sum;
+}
+
+__global__ __launch_bounds__(BLK) void k_scan_reduce(int64_t n, const int64_t* __restrict__ in,
+                                                     int64_t* __restrict__ totals) {
+    __shared__ int64_t sh[WAVES];
+    int64_t v[SCAN_PER];
+    int64_t sum = scan_load(n, in, (int64_t)blockIdx.x * SCAN_BLK, v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) t += sh[w];
+        totals[blockIdx.x] = t;
+    }
+}
+
+// out[k] = the exclusive scan within the block plus the block's offset totals[b] (scanned); the last block also
+// writes out[n] = totals[nb]
+__global__ __launch_bounds__(BLK) void k_scan_down(int64_t n, int64_t nb, const int64_t* __restrict__ in,
+                                                   const int64_t* __restrict__ totals, int64_t* __restrict__ out) {
+    __shared__ int64_t sh[WAVES];
     const int64_t base = (int64_t)blockIdx.x * SCAN_BLK;
-    for (int k = threadIdx.x; k < SCAN_BLK; k += BLK) sh[k] = (base + k < n) ? in[base + k] : 0;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over SCAN_BLK entries with BLK threads (each owns 4 consecutive)
-    int64_t loc[SCAN_BLK / BLK];
-    int64_t run = 0;
+    int64_t v[SCAN_PER];
+    const int64_t sum = scan_load(n, in, base, v);
+    const int lane = threadIdx.x & 63;
+    int64_t incl = sum;  // inclusive scan of the threads' sums within the wave
 #pragma unroll
-    for (int q = 0; q < SCAN_BLK / BLK; ++q) {
-        run += sh[threadIdx.x * (SCAN_BLK / BLK) + q];
-        loc[q] = run;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
     }
-    __shared__ int64_t part[BLK];
-    part[threadIdx.x] = run;
+    if (lane == 63) sh[threadIdx.x >> 6] = incl;
     __syncthreads();
-    for (int off = 1; off < BLK; off <<= 1) {
-        const int64_t v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const int64_t before = (threadIdx.x == 0) ? 0 : part[threadIdx.x - 1];
+    int64_t run = totals[blockIdx.x] + incl - sum;  // the block's offset, earlier waves, earlier threads
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += sh[w];
 #pragma unroll
-    for (int q = 0; q < SCAN_BLK / BLK; ++q) {
-        const int64_t k = base + threadIdx.x * (SCAN_BLK / BLK) + q;
-        const int64_t incl = before + loc[q];
-        if (k < n) out[k] = incl - ((k < n) ? in[k] : 0);  // exclusive
+    for (int q = 0; q < SCAN_PER; ++q) {
+        const int64_t k = base + threadIdx.x * SCAN_PER + q;
+        if (k < n) out[k] = run;
+        run += v[q];
     }
-    if (threadIdx.x == BLK - 1) totals[blockIdx.x] = part[BLK - 1];
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = totals[nb];
 }
 
 // one workgroup of TOT_BLK threads: exclusive scan of the block totals in place, grand total in totals[nb]; each
@@ -94,13 +124,6 @@ __global__ __launch_bounds__(TOT_BLK) void k_scan_totals(int64_t nb, int64_t* __
     if (threadIdx.x == TOT_BLK - 1) totals[nb] = part[TOT_BLK - 1];
 }
 
-// out[k] += the exclusive total of k's block; out[n] = the grand total
-__global__ __launch_bounds__(BLK) void k_scan_add(int64_t n, int64_t nb, int64_t* __restrict__ out,
-                                                  const int64_t* __restrict__ totals) {
-    const int64_t k = (int64_t)blockIdx.x * BLK + threadIdx.x;
-    if (k < n) out[k] += totals[k / SCAN_BLK];
-    else if (k == n) out[n] = totals[nb];
-}
 
 // ---------------------------------------------------------------- matching
 // A level's operator during the setup (CSR).  Values in FP32: the cycle stores them in bfloat16 (FP64 under the

@@ -123,9 +123,9 @@ int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) 
     const int64_t nb = std::max<int64_t>(1, (n + SCAN_BLK - 1) / SCAN_BLK);
     int64_t* tot = nullptr;
     HIPCHK(S.alloc(&tot, (size_t)nb + 1));
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(BLK), 0, s, n, in, out, tot);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLK), 0, s, n, in, tot);
     hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(TOT_BLK), 0, s, nb, tot);
-    hipLaunchKernelGGL(k_scan_add, rows_grid(n + 1), dim3(BLK), 0, s, n, nb, out, (const int64_t*)tot);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(BLK), 0, s, n, nb, in, (const int64_t*)tot, out);
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }
