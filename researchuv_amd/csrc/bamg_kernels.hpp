@@ -70,14 +70,31 @@ __global__ __launch_bounds__(BLK) void k_scan_reduce(int64_t n, const int64_t* _
     }
 }
 
-// out[k] = the exclusive scan within the block plus the block's offset totals[b] (scanned); the last block also
-// writes out[n] = totals[nb]
+// out[k] = the exclusive scan within the block plus the block's offset; the last block also writes out[n] = the
+// grand total.  scanned: totals holds the scanned block sums and the grand total at nb (k_scan_totals); else it
+// holds the block sums, and each block adds up those before it (few blocks: no launch for the totals).
 __global__ __launch_bounds__(BLK) void k_scan_down(int64_t n, int64_t nb, const int64_t* __restrict__ in,
-                                                   const int64_t* __restrict__ totals, int64_t* __restrict__ out) {
+                                                   const int64_t* __restrict__ totals, int64_t* __restrict__ out,
+                                                   int scanned) {
     __shared__ int64_t sh[WAVES];
+    __shared__ int64_t sb[WAVES];
     const int64_t base = (int64_t)blockIdx.x * SCAN_BLK;
     int64_t v[SCAN_PER];
     const int64_t sum = scan_load(n, in, base, v);
+    int64_t offset;
+    if (scanned) {
+        offset = totals[blockIdx.x];
+    } else {
+        int64_t a = 0;
+        for (int64_t k = threadIdx.x; k < (int64_t)blockIdx.x; k += BLK) a += totals[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+        if ((threadIdx.x & 63) == 0) sb[threadIdx.x >> 6] = a;
+        __syncthreads();
+        offset = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) offset += sb[w];
+    }
     const int lane = threadIdx.x & 63;
     int64_t incl = sum;  // inclusive scan of the threads' sums within the wave
 #pragma unroll
@@ -87,7 +104,7 @@ __global__ __launch_bounds__(BLK) void k_scan_down(int64_t n, int64_t nb, const 
     }
     if (lane == 63) sh[threadIdx.x >> 6] = incl;
     __syncthreads();
-    int64_t run = totals[blockIdx.x] + incl - sum;  // the block's offset, earlier waves, earlier threads
+    int64_t run = offset + incl - sum;  // the block's offset, earlier waves, earlier threads
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += sh[w];
 #pragma unroll
     for (int q = 0; q < SCAN_PER; ++q) {
@@ -95,7 +112,7 @@ __global__ __launch_bounds__(BLK) void k_scan_down(int64_t n, int64_t nb, const 
         if (k < n) out[k] = run;
         run += v[q];
     }
-    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = totals[nb];
+    if (blockIdx.x == nb - 1 && threadIdx.x == BLK - 1) out[n] = run;  // the last thread's run: the grand total
 }
 
 // one workgroup of TOT_BLK threads: exclusive scan of the block totals in place, grand total in totals[nb]; each
@@ -425,7 +442,8 @@ __global__ __launch_bounds__(BLK) void k_root_agg(int64_t n, const int64_t* __re
                                                   int32_t* __restrict__ agg, unsigned long long* __restrict__ cnt) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const int64_t m = mate[i], jn = (m >= 0) ? -1 : join[i];
+    const int64_t m = mate[i], jr = join[i];  // loaded together (join is written for every row)
+    const int64_t jn = (m >= 0) ? -1 : jr;
     const int64_t root = (m >= 0) ? min(i, m) : (jn >= 0 ? jn : i);
     const int64_t a = cid[root];
     agg[i] = (int32_t)a;
@@ -442,12 +460,13 @@ __global__ __launch_bounds__(BLK) void k_member_fill(int64_t n, const int64_t* _
                                                      int64_t* __restrict__ mrows) {
     const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
     if (i >= n) return;
-    const int64_t a = agg[i], m = mate[i];
+    const int64_t a = agg[i], m = mate[i], jr = join[i];  // loaded together
+    const int64_t base = mptr[a];
     int64_t slot;
     if (m >= 0) slot = (i < m) ? 0 : 1;
-    else if (join[i] >= 0) slot = 2 + (int64_t)atomicAdd(&cursor[a], 1ull);
+    else if (jr >= 0) slot = 2 + (int64_t)atomicAdd(&cursor[a], 1ull);
     else slot = 0;
-    mrows[mptr[a] + slot] = i;
+    mrows[base + slot] = i;
 }
 
 // each aggregate's member list sorted ascending; with group, the aggregate's group (that of its rows)

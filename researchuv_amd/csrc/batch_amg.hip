@@ -118,14 +118,18 @@ void hand_over(Scratch& from, void* p, std::vector<void*>& to) {
     to.push_back(p);
 }
 
+constexpr int64_t SCAN_FEW = 4096;
 // out[0..n) = exclusive scan of in, out[n] = total
 int scan(hipStream_t s, int64_t n, const int64_t* in, int64_t* out, Scratch& S) {
     const int64_t nb = std::max<int64_t>(1, (n + SCAN_BLK - 1) / SCAN_BLK);
     int64_t* tot = nullptr;
     HIPCHK(S.alloc(&tot, (size_t)nb + 1));
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(BLK), 0, s, n, in, tot);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(TOT_BLK), 0, s, nb, tot);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(BLK), 0, s, n, nb, in, (const int64_t*)tot, out);
+    // up to SCAN_FEW blocks each add up the block sums before their own; more get them scanned in a launch
+    const int scanned = nb > SCAN_FEW ? 1 : 0;
+    if (scanned) hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(TOT_BLK), 0, s, nb, tot);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(BLK), 0, s, n, nb, in, (const int64_t*)tot, out,
+                       scanned);
     HIPCHK(hipGetLastError());
     return RUV_OK;
 }
