@@ -672,3 +672,37 @@ def test_batch_refinement_rounds_and_live_lists(orc, monkeypatch):
             rc = ref[c].reshape(-1, 2)
             worst = max(worst, np.linalg.norm(xc - rc, axis=1).max() / max(np.linalg.norm(rc, axis=1).max(), 1e-30))
         assert worst < 1e-6
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_batch_reports_a_chart_whose_step_breaks_down(mixed):
+    """A chart whose first step meets p.Ap = 0 (the indefinite block diag(1, -1), no preconditioner) stops as broken
+    down in the update pass while the SPD chart beside it runs on; the batch call reports the breakdown
+    (solver.rs:520-526 stops on a zero p.Ap)."""
+    import scipy.sparse as sp
+
+    from researchuv_amd import _lib
+    from researchuv_amd import batch as B
+    from researchuv_amd.solver import Precond
+
+    say(f"batch breakdown in the update pass mixed={mixed}")
+    spd = sp.diags([-np.ones(63), 2.0 * np.ones(64), -np.ones(63)], [-1, 0, 1], format="csr")
+    bad = sp.csr_matrix(np.diag([1.0, -1.0]))
+    A = sp.block_diag([spd, bad], format="csr")
+    A.sort_indices()
+    off = [0, 64, 66]
+    bt = B.Batch.from_csr(A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, off, precond=Precond.NONE,
+                          mixed=mixed, R=1)
+    rhs = np.ones(66)
+    with pytest.raises(_lib.RuvError) as e:
+        bt.solve(rhs, max_iter=1000, tol=1e-10)
+    assert "BREAKDOWN" in str(e.value) or "broke down" in str(e.value)
+    bt.close()
+    # the SPD chart alone solves
+    A1 = spd.tocsr()
+    A1.sort_indices()
+    b1 = B.Batch.from_csr(A1.indptr.astype(np.int64), A1.indices.astype(np.int32), A1.data, [0, 64],
+                          precond=Precond.NONE, mixed=mixed, R=1)
+    x, info, it, res = b1.solve(np.ones(64), max_iter=1000, tol=1e-10)
+    assert np.abs(spd @ x - 1.0).max() < 1e-8
+    b1.close()
