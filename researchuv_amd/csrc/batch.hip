@@ -548,7 +548,7 @@ void launch_spmv(BatchImpl& Q, int na, const T* p, T* q) {
 // One inner PCG over the charts not yet final.  cont: go on from the previous inner solve's last direction (the
 // residual replaced by the refined FP64 one) instead of starting afresh.
 template <typename T, int R, int PC, int CI>
-int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
+int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont, double cut = 0.0) {
     hipStream_t s = Q.stream;
     BDev& B = Q.B;
     B.max_iter = (unsigned)std::min<uint64_t>(max_iter, 0xffffffffu);
@@ -631,11 +631,18 @@ int inner_pcg(BatchImpl& Q, double tol, uint64_t max_iter, int cont) {
     // flags[j % 2] receives the flags after chunk j (chunk 0: the state kb_init left, in flags[0]); `known` is the
     // buffer of the latest flags waited for
     int known = 0;
+    const int na0 = na;
     for (int64_t j = 1;; ++j) {
         const int* kf = flags[known];
         rc = set_active(Q, [&](int c) { return kf[c] == B_RUNNING; }, &na);
         if (rc) return rc;
         if (na == 0) break;
+        if (cut > 0.0 && (double)na < cut * (double)na0) {  // the few charts still running stop here
+            hipLaunchKernelGGL((kb_dir<T, R>), grid_of(na), dim3(BLK), 0, s, B, (const int*)Q.act, na, par, iter, z,
+                               (const T*)pb[par ^ 1], pb[par], x, 1);
+            HIPCHK(hipGetLastError());
+            break;
+        }
         // once most charts have stopped, the coarse levels launch only the running charts' slices and blocks
         if (PC == 2) rc = bamg_set_live(Q, (double)na < Q.list_below * (double)Q.ntile ? kf : nullptr);
         if (rc) return rc;
@@ -684,7 +691,8 @@ int batch_run(BatchImpl& Q, const double* d_b, double* d_x, uint64_t max_iter, d
     int na = 0;
     for (int round = 0;; ++round) {
         if (round >= Q.max_rounds) return fail(RUV_ERR_NOT_CONVERGED, "batch: refinement did not converge");
-        int rc = inner_pcg<T, R, PC, CI>(Q, tol, max_iter, (round > 0 && Q.continue_rounds) ? 1 : 0);
+        int rc = inner_pcg<T, R, PC, CI>(Q, tol, max_iter, (round > 0 && Q.continue_rounds) ? 1 : 0,
+                                         (mixed && round == 0) ? Q.first_cut : 0.0);
         if (rc) return rc;
         rc = set_active(Q, [&](int c) { return Q.h_final[c] == 0; }, &na);
         if (rc) return rc;
@@ -881,6 +889,7 @@ int batch_create(int64_t nrows, const int64_t* ptr, const int32_t* cols, const d
     if (const char* e = std::getenv("RUV_BATCH_LIST_BELOW")) Q->list_below = std::atof(e);
     if (const char* e = std::getenv("RUV_UV_TOL")) Q->uv_tol = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_FINAL_AIM")) Q->final_aim = std::atof(e);
+    if (const char* e = std::getenv("RUV_BATCH_FIRST_CUT")) Q->first_cut = std::atof(e);
     if (const char* e = std::getenv("RUV_BATCH_LOOK_LAG")) Q->look_lag = std::atoi(e) ? 1 : 0;
     int rc = batch_setup(*Q, nrows, ptr, cols, vals, nchart, crow, R, precond, mode);
     if (rc) return rc;

@@ -128,6 +128,10 @@ struct BatchImpl {
     // mixed refinement: a round that can end a chart aims its inner solve at final_aim times the chart's outer
     // target (0: at the target itself, like the other rounds' floor); 0.35 measured best of 0.15 .. 0.6
     double final_aim = 0.35;
+    // mixed: the first inner solve stops for the charts still running once fewer than this fraction of its tiles
+    // run; they go on in the next round (from their last direction) beside the others, instead of a tail of few
+    // charts (0.02, 0.05, 0.1 measured alike; 0: off)
+    double first_cut = 0.05;
     int max_rounds = 40;
     // rounds after the first go on from the previous round's last direction (1) or start afresh (0)
     int continue_rounds = 1;

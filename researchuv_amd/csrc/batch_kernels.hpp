@@ -421,7 +421,7 @@ __device__ __forceinline__ void tile_rows2(int64_t r0, int64_t r1, F f) {
 template <typename T, int R>
 __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ act, int na, int par, uint32_t iter,
                                               const T* __restrict__ z, const T* __restrict__ pold,
-                                              T* __restrict__ pnew, T* __restrict__ x) {
+                                              T* __restrict__ pnew, T* __restrict__ x, int stop = 0) {
     __shared__ double sh[2 * WAVES];
     const int slot = xcd_slot_rev(na);
     if (slot < 0) return;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(BLK) void kb_dir(BDev B, const int* __restrict__ ac
             st<T, R>(x, i2, two ? xn2 : xn);
         });
     };
-    if (sqrt(rr) <= B.c_tol[c]) {
+    if (stop || sqrt(rr) <= B.c_tol[c]) {  // stop: the inner solve ends here for every chart still running
         if (lead) {
             B.c_done[c] = B_CONVERGED;
             B.c_rrec[c] = sqrt(rr);
